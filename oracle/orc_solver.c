@@ -1,0 +1,293 @@
+/*
+ * orc_solver.c -- oracle (test infrastructure only): preconditioners and
+ * Krylov solvers for A(n) phi = b, standing in for the PETSc KSP that the
+ * reference drives at potential.c:6-16 / hybrid.c:73-78.  FP64; see orc.h.
+ */
+#include "orc.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static int is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* In-place complex DFT of a strided line (radix-2 when n is a power of two,
+ * direct summation otherwise); sign -1 forward, +1 inverse (unscaled). */
+static void dft_line(double *re, double *im, int64_t n, double sign, double *tr, double *ti)
+{
+  if (is_pow2(n)) {
+    for (int64_t i = 1, j = 0; i < n; i++) {
+      int64_t bit = n >> 1;
+      for (; j & bit; bit >>= 1) j ^= bit;
+      j ^= bit;
+      if (i < j) {
+        double t = re[i]; re[i] = re[j]; re[j] = t;
+        t = im[i]; im[i] = im[j]; im[j] = t;
+      }
+    }
+    for (int64_t len = 2; len <= n; len <<= 1) {
+      const double ang = sign * 2.0 * M_PI / (double)len;
+      for (int64_t i = 0; i < n; i += len) {
+        for (int64_t k = 0; k < len / 2; k++) {
+          const double wr = cos(ang * (double)k), wi = sin(ang * (double)k);
+          const int64_t a = i + k, b = i + k + len / 2;
+          const double xr = re[b] * wr - im[b] * wi, xi = re[b] * wi + im[b] * wr;
+          re[b] = re[a] - xr; im[b] = im[a] - xi;
+          re[a] += xr; im[a] += xi;
+        }
+      }
+    }
+    return;
+  }
+  for (int64_t k = 0; k < n; k++) {
+    double sr = 0.0, si = 0.0;
+    for (int64_t j = 0; j < n; j++) {
+      const double a = sign * 2.0 * M_PI * (double)((k * j) % n) / (double)n;
+      sr += re[j] * cos(a) - im[j] * sin(a);
+      si += re[j] * sin(a) + im[j] * cos(a);
+    }
+    tr[k] = sr; ti[k] = si;
+  }
+  memcpy(re, tr, sizeof(double) * n);
+  memcpy(im, ti, sizeof(double) * n);
+}
+
+void orc_fft3d(const int64_t N[3], double *data, int inverse)
+{
+  const double sign = inverse ? 1.0 : -1.0;
+  int64_t nmax = N[0];
+  if (N[1] > nmax) nmax = N[1];
+  if (N[2] > nmax) nmax = N[2];
+  double *re = malloc(sizeof(double) * nmax), *im = malloc(sizeof(double) * nmax);
+  double *tr = malloc(sizeof(double) * nmax), *ti = malloc(sizeof(double) * nmax);
+  const int64_t str[3] = {1, N[0], N[0] * N[1]};
+  const int64_t nc = N[0] * N[1] * N[2];
+  for (int ax = 0; ax < 3; ax++) {
+    const int64_t n = N[ax];
+    for (int64_t base = 0; base < nc; base++) {
+      /* lines start where the index along ax is 0 */
+      if ((base / str[ax]) % n != 0) continue;
+      for (int64_t t = 0; t < n; t++) {
+        re[t] = data[2 * (base + t * str[ax])];
+        im[t] = data[2 * (base + t * str[ax]) + 1];
+      }
+      dft_line(re, im, n, sign, tr, ti);
+      for (int64_t t = 0; t < n; t++) {
+        data[2 * (base + t * str[ax])] = re[t];
+        data[2 * (base + t * str[ax]) + 1] = im[t];
+      }
+    }
+  }
+  free(re); free(im); free(tr); free(ti);
+}
+
+/* Row of the constant-coefficient operator A(nbar) (uniform density). */
+static void uniform_row(int lhs_type, const int64_t N[3], const double d[3], const double K[3],
+                        double nbar, double v[19])
+{
+  (void)N;
+  if (lhs_type == ORC_LHS_FULL) {
+    /* evaluate the full row on a uniform 3x3x3 field (centre node 13) */
+    const int64_t M[3] = {3, 3, 3};
+    double n27[27];
+    for (int q = 0; q < 27; q++) n27[q] = nbar;
+    orc_lhs_full_row(M, d, K, n27, 13, v);
+  } else if (lhs_type == ORC_LHS_LAPLACIAN) {
+    const int64_t M[3] = {3, 3, 3};
+    double n27[27] = {0};
+    double A[27 * 27];
+    orc_lhs_dense(ORC_LHS_LAPLACIAN, M, d, K, n27, A);
+    int off[19][3];
+    orc_stencil_offsets(off);
+    for (int q = 0; q < 19; q++) {
+      const int64_t col = (1 + off[q][0]) + 3 * ((1 + off[q][1]) + 3 * (1 + off[q][2]));
+      v[q] = A[13 * 27 + col];
+    }
+  } else {
+    for (int q = 0; q < 19; q++) v[q] = 0.0;
+    v[18] = 1.0;
+  }
+}
+
+void orc_pc_apply(int pc, int lhs_type, const int64_t N[3], const double d[3], const double K[3],
+                  const double *n, const double *r, double *z)
+{
+  const int64_t nc = N[0] * N[1] * N[2];
+  if (pc == ORC_PC_JACOBI) {
+    for (int64_t c = 0; c < nc; c++) {
+      double v[19];
+      if (lhs_type == ORC_LHS_FULL) orc_lhs_full_row(N, d, K, n, c, v);
+      else uniform_row(lhs_type, N, d, K, 0.0, v);
+      z[c] = r[c] / v[18];
+    }
+    orc_remove_mean(nc, z);
+    return;
+  }
+  if (pc == ORC_PC_FFT) {
+    double s = 0.0;
+    for (int64_t c = 0; c < nc; c++) s += n[c];
+    const double nbar = s / (double)nc;
+    double v[19];
+    int off[19][3];
+    uniform_row(lhs_type, N, d, K, nbar, v);
+    orc_stencil_offsets(off);
+    double *buf = malloc(sizeof(double) * 2 * nc);
+    for (int64_t c = 0; c < nc; c++) { buf[2 * c] = r[c]; buf[2 * c + 1] = 0.0; }
+    orc_fft3d(N, buf, 0);
+    for (int64_t c = 0; c < nc; c++) {
+      const int64_t kx = c % N[0], ky = (c / N[0]) % N[1], kz = c / (N[0] * N[1]);
+      double lam = 0.0;
+      for (int q = 0; q < 19; q++) {
+        const double th = 2.0 * M_PI * ((double)(kx * off[q][0]) / (double)N[0] +
+                                        (double)(ky * off[q][1]) / (double)N[1] +
+                                        (double)(kz * off[q][2]) / (double)N[2]);
+        lam += v[q] * cos(th);
+      }
+      if (c == 0 || lam == 0.0) { buf[2 * c] = buf[2 * c + 1] = 0.0; }
+      else { buf[2 * c] /= lam; buf[2 * c + 1] /= lam; }
+    }
+    orc_fft3d(N, buf, 1);
+    for (int64_t c = 0; c < nc; c++) z[c] = buf[2 * c] / (double)nc;
+    free(buf);
+    orc_remove_mean(nc, z);
+    return;
+  }
+  memcpy(z, r, sizeof(double) * nc);
+  orc_remove_mean(nc, z);
+}
+
+static double dot(int64_t n, const double *a, const double *b)
+{
+  double s = 0.0;
+  for (int64_t i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+static void residual(int lhs_type, const int64_t N[3], const double d[3], const double K[3],
+                     const double *n, const double *b, const double *phi, double *r)
+{
+  const int64_t nc = N[0] * N[1] * N[2];
+  orc_lhs_apply(lhs_type, N, d, K, n, phi, r);
+  for (int64_t c = 0; c < nc; c++) r[c] = b[c] - r[c];
+}
+
+static void gmres(int pc, int lhs_type, const int64_t N[3], const double d[3], const double K[3],
+                  const double *n, const double *b, double *phi, double tol,
+                  int64_t max_it, int64_t m, int64_t info[2])
+{
+  const int64_t nc = N[0] * N[1] * N[2];
+  double *V = malloc(sizeof(double) * nc * (m + 1));
+  double *Z = malloc(sizeof(double) * nc * m);
+  double *w = malloc(sizeof(double) * nc);
+  double *H = calloc((size_t)(m + 1) * m, sizeof(double));
+  double *cs = malloc(sizeof(double) * m), *sn = malloc(sizeof(double) * m);
+  double *g = malloc(sizeof(double) * (m + 1)), *y = malloc(sizeof(double) * m);
+  double *h = malloc(sizeof(double) * (m + 1)), *h2 = malloc(sizeof(double) * (m + 1));
+  int64_t it = 0;
+  info[1] = 0;
+  for (;;) {
+    residual(lhs_type, N, d, K, n, b, phi, V);
+    double beta = sqrt(dot(nc, V, V));
+    if (beta <= tol) { info[1] = 1; break; }
+    if (it >= max_it) break;
+    for (int64_t c = 0; c < nc; c++) V[c] /= beta;
+    for (int64_t q = 0; q <= m; q++) g[q] = 0.0;
+    g[0] = beta;
+    int64_t j = 0;
+    for (; j < m && it < max_it; j++, it++) {
+      double *zj = Z + nc * j;
+      orc_pc_apply(pc, lhs_type, N, d, K, n, V + nc * j, zj);
+      orc_lhs_apply(lhs_type, N, d, K, n, zj, w);
+      /* two-pass classical Gram-Schmidt */
+      for (int64_t q = 0; q <= j; q++) h[q] = dot(nc, V + nc * q, w);
+      for (int64_t q = 0; q <= j; q++) for (int64_t c = 0; c < nc; c++) w[c] -= h[q] * V[nc * q + c];
+      for (int64_t q = 0; q <= j; q++) h2[q] = dot(nc, V + nc * q, w);
+      for (int64_t q = 0; q <= j; q++) for (int64_t c = 0; c < nc; c++) w[c] -= h2[q] * V[nc * q + c];
+      for (int64_t q = 0; q <= j; q++) h[q] += h2[q];
+      h[j + 1] = sqrt(dot(nc, w, w));
+      for (int64_t c = 0; c < nc; c++) V[nc * (j + 1) + c] = (h[j + 1] > 0.0) ? w[c] / h[j + 1] : 0.0;
+      for (int64_t q = 0; q < j; q++) {
+        const double t = cs[q] * h[q] + sn[q] * h[q + 1];
+        h[q + 1] = -sn[q] * h[q] + cs[q] * h[q + 1];
+        h[q] = t;
+      }
+      const double rr = sqrt(h[j] * h[j] + h[j + 1] * h[j + 1]);
+      cs[j] = (rr > 0.0) ? h[j] / rr : 1.0;
+      sn[j] = (rr > 0.0) ? h[j + 1] / rr : 0.0;
+      h[j] = rr;
+      h[j + 1] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      for (int64_t q = 0; q <= j; q++) H[q * m + j] = h[q];
+      if (fabs(g[j + 1]) <= tol) { j++; it++; break; }
+    }
+    for (int64_t q = j - 1; q >= 0; q--) {
+      double s = g[q];
+      for (int64_t r = q + 1; r < j; r++) s -= H[q * m + r] * y[r];
+      y[q] = s / H[q * m + q];
+    }
+    for (int64_t q = 0; q < j; q++) for (int64_t c = 0; c < nc; c++) phi[c] += y[q] * Z[nc * q + c];
+  }
+  info[0] = it;
+  free(V); free(Z); free(w); free(H); free(cs); free(sn); free(g); free(y); free(h); free(h2);
+}
+
+static void bicgstab(int pc, int lhs_type, const int64_t N[3], const double d[3], const double K[3],
+                     const double *n, const double *b, double *phi, double tol,
+                     int64_t max_it, int64_t info[2])
+{
+  const int64_t nc = N[0] * N[1] * N[2];
+  double *r = malloc(sizeof(double) * nc), *rh = malloc(sizeof(double) * nc);
+  double *p = calloc(nc, sizeof(double)), *v = calloc(nc, sizeof(double));
+  double *yv = malloc(sizeof(double) * nc), *s = malloc(sizeof(double) * nc);
+  double *zv = malloc(sizeof(double) * nc), *t = malloc(sizeof(double) * nc);
+  residual(lhs_type, N, d, K, n, b, phi, r);
+  memcpy(rh, r, sizeof(double) * nc);
+  double rho = 1.0, alpha = 1.0, omega = 1.0;
+  int64_t it = 0;
+  info[1] = 0;
+  while (it < max_it) {
+    if (sqrt(dot(nc, r, r)) <= tol) { info[1] = 1; break; }
+    const double rho1 = dot(nc, rh, r);
+    const double beta = (rho1 / rho) * (alpha / omega);
+    for (int64_t c = 0; c < nc; c++) p[c] = r[c] + beta * (p[c] - omega * v[c]);
+    orc_pc_apply(pc, lhs_type, N, d, K, n, p, yv);
+    orc_lhs_apply(lhs_type, N, d, K, n, yv, v);
+    alpha = rho1 / dot(nc, rh, v);
+    for (int64_t c = 0; c < nc; c++) s[c] = r[c] - alpha * v[c];
+    it++;
+    if (sqrt(dot(nc, s, s)) <= tol) {
+      for (int64_t c = 0; c < nc; c++) phi[c] += alpha * yv[c];
+      info[1] = 1;
+      break;
+    }
+    orc_pc_apply(pc, lhs_type, N, d, K, n, s, zv);
+    orc_lhs_apply(lhs_type, N, d, K, n, zv, t);
+    omega = dot(nc, t, s) / dot(nc, t, t);
+    for (int64_t c = 0; c < nc; c++) {
+      phi[c] += alpha * yv[c] + omega * zv[c];
+      r[c] = s[c] - omega * t[c];
+    }
+    rho = rho1;
+  }
+  info[0] = it;
+  free(r); free(rh); free(p); free(v); free(yv); free(s); free(zv); free(t);
+}
+
+void orc_solve(int ksp, int pc, int lhs_type, const int64_t N[3], const double d[3],
+               const double K[3], const double *n, const double *b, double *phi,
+               double rtol, double atol, int64_t max_it, int64_t restart,
+               int64_t info[2], double resid[1])
+{
+  const int64_t nc = N[0] * N[1] * N[2];
+  const double bn = sqrt(dot(nc, b, b));
+  double tol = rtol * bn;
+  if (atol > tol) tol = atol;
+  if (ksp == ORC_KSP_BCGS) bicgstab(pc, lhs_type, N, d, K, n, b, phi, tol, max_it, info);
+  else gmres(pc, lhs_type, N, d, K, n, b, phi, tol, max_it, restart, info);
+  orc_remove_mean(nc, phi);
+  double *r = malloc(sizeof(double) * nc);
+  residual(lhs_type, N, d, K, n, b, phi, r);
+  resid[0] = (bn > 0.0) ? sqrt(dot(nc, r, r)) / bn : sqrt(dot(nc, r, r));
+  free(r);
+}

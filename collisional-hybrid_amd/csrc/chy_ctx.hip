@@ -1,0 +1,734 @@
+// chy_ctx.hip -- context, C ABI entry points and the potential solver.
+//
+// The solver replaces the PETSc KSP driven at reference hybrid.c:73-78 and
+// potential.c:6-55.  It is an outer FP64 defect correction
+//     r = b - A(n) phi (FP64),  A e = r / |r| (inner Krylov),  phi += |r| e
+// around an inner right-preconditioned GMRES(m) (two-pass classical
+// Gram-Schmidt with fused multi-dot kernels) or BiCGStab, in FP64 (plain
+// mode, one outer sweep) or FP32 (mixed mode).  Inner products and norms
+// always accumulate in FP64.  The constant null space is handled by the
+// mean-free forcing, the mean-free FFT preconditioner and a final mean
+// projection of phi.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "chy_internal.h"
+
+using namespace chy;
+
+namespace chy {
+
+void timer_start(chy_ctx *ctx, const char *name)
+{
+  if (!ctx->timing) return;
+  Timer &t = ctx->timers[name];
+  if (!t.a) { (void)hipEventCreate(&t.a); (void)hipEventCreate(&t.b); }
+  (void)hipEventRecord(t.a, ctx->stream);
+}
+
+void timer_stop(chy_ctx *ctx, const char *name)
+{
+  if (!ctx->timing) return;
+  Timer &t = ctx->timers[name];
+  (void)hipEventRecord(t.b, ctx->stream);
+  (void)hipEventSynchronize(t.b);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, t.a, t.b);
+  t.total_ms += ms;
+  t.count++;
+}
+
+}  // namespace chy
+
+namespace {
+
+template <typename T> T *mom(chy_ctx *c, int a) { return (T *)c->mom[a]; }
+
+template <typename T>
+int dalloc(chy_ctx *ctx, T **p, int64_t n)
+{
+  if (n <= 0) { *p = nullptr; return CHY_OK; }
+  const hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess)
+    return set_err(ctx, CHY_ERR_MEM, "hipMalloc of " + std::to_string(sizeof(T) * (size_t)n) + " bytes: " + hipGetErrorString(e));
+  return CHY_OK;
+}
+
+#define TRY(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
+
+template <typename T>
+int alloc_state(chy_ctx *ctx, PState<T> &s, int64_t cap, bool w)
+{
+  TRY(dalloc(ctx, &s.x, cap)); TRY(dalloc(ctx, &s.y, cap)); TRY(dalloc(ctx, &s.z, cap));
+  TRY(dalloc(ctx, &s.vx, cap)); TRY(dalloc(ctx, &s.vy, cap)); TRY(dalloc(ctx, &s.vz, cap));
+  if (w) TRY(dalloc(ctx, &s.w, cap));
+  TRY(dalloc(ctx, &s.pid, cap));
+  TRY(dalloc(ctx, &s.cell, cap));
+  return CHY_OK;
+}
+
+template <typename T>
+void free_state(PState<T> &s)
+{
+  void *ps[] = {s.x, s.y, s.z, s.vx, s.vy, s.vz, s.w, s.pid, s.cell};
+  for (void *p : ps) if (p) (void)hipFree(p);
+  s = PState<T>();
+}
+
+int sync(chy_ctx *ctx)
+{
+  CHY_HIP(hipStreamSynchronize(ctx->stream));
+  return CHY_OK;
+}
+
+// Basis slots: GMRES uses V_0..V_m, BiCGStab slots 0..7; the last slot holds
+// the inner correction e.
+int64_t nvec_V(const chy_ctx *ctx) { return std::max<int64_t>(ctx->restart + 2, 10); }
+
+// ---- preconditioner ------------------------------------------------------------------
+template <typename T>
+int pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
+{
+  const int64_t nc = ctx->g.nc;
+  timer_start(ctx, "pc");
+  int rc = CHY_OK;
+  const int pc = ctx->prm.pc_type;
+  if (pc == CHY_PC_FFT && ctx->fft_ok && ctx->prm.lhs_type != CHY_LHS_IDENTITY) {
+    rc = fft_pc_apply<T>(ctx, v19, r, z);
+  } else if (pc == CHY_PC_JACOBI || (pc == CHY_PC_FFT && !ctx->fft_ok)) {
+    launch_jacobi<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), r, z);
+  } else {
+    CHY_HIP(hipMemcpyAsync(z, r, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  timer_stop(ctx, "pc");
+  return rc;
+}
+
+// Read nv doubles from the device reduction buffer (one sync).
+int fetch(chy_ctx *ctx, const double *dev, int nv, double *host)
+{
+  CHY_HIP(hipMemcpyAsync(host, dev, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream));
+  CHY_HIP(hipStreamSynchronize(ctx->stream));
+  return CHY_OK;
+}
+
+// ---- inner GMRES on A e = rhs (e starts at zero), precision T -------------------
+template <typename T>
+int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double rtol, int64_t max_it,
+                int64_t *its, double *relres)
+{
+  const int64_t nc = ctx->g.nc, m = ctx->restart;
+  T *V = (T *)ctx->V, *Z = (T *)ctx->Z, *w = (T *)ctx->wv;
+  const T *n = mom<T>(ctx, 0);
+  const int lt = ctx->prm.lhs_type;
+  double *hdev = ctx->red_out, *h2dev = ctx->red_out + (m + 2), *ndev = ctx->red_out + 2 * (m + 2);
+  std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m), h(m + 2), h2(m + 2);
+  launch_fill<T>(ctx, nc, e, (T)0);
+  double beta0 = -1.0;
+  *its = 0;
+  *relres = 1.0;
+  for (;;) {
+    // V0 = rhs - A e
+    if (*its == 0) {
+      CHY_HIP(hipMemcpyAsync(V, rhs, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+      launch_lhs_apply<T, T, T>(ctx, lt, false, n, e, w);
+      CHY_HIP(hipMemcpyAsync(V, rhs, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+      launch_axpby<T>(ctx, nc, -1.0, w, 1.0, V);
+    }
+    launch_mdot<T>(ctx, 1, V, nc, V, nc, ndev);
+    double nn;
+    TRY(fetch(ctx, ndev, 1, &nn));
+    const double beta = sqrt(nn);
+    if (beta0 < 0) beta0 = beta;
+    if (!(beta0 > 0.0)) { *relres = 0.0; return CHY_OK; }
+    *relres = beta / beta0;
+    if (!std::isfinite(beta)) return set_err(ctx, CHY_ERR_FP, "non-finite residual in the Krylov solve");
+    if (beta <= rtol * beta0 || *its >= max_it) return CHY_OK;
+    launch_axpby<T>(ctx, nc, 1.0 / beta, V, 0.0, V);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int64_t j = 0;
+    bool done = false;
+    for (; j < m && *its < max_it; j++) {
+      T *Vj = V + j * nc, *Zj = Z + j * nc, *Vn = V + (j + 1) * nc;
+      TRY(pc_apply<T>(ctx, v19, Vj, Zj));
+      timer_start(ctx, "lhs_apply");
+      launch_lhs_apply<T, T, T>(ctx, lt, false, n, Zj, w);
+      timer_stop(ctx, "lhs_apply");
+      timer_start(ctx, "krylov");
+      launch_mdot<T>(ctx, (int)(j + 1), V, nc, w, nc, hdev);
+      launch_mgs_update<T>(ctx, (int)(j + 1), V, nc, hdev, w, nc);
+      launch_mdot<T>(ctx, (int)(j + 1), V, nc, w, nc, h2dev);
+      launch_mgs_update<T>(ctx, (int)(j + 1), V, nc, h2dev, w, nc);
+      launch_mdot<T>(ctx, 1, w, nc, w, nc, ndev);
+      TRY(fetch(ctx, ctx->red_out, (int)(2 * (m + 2) + 1), ctx->h_red));
+      timer_stop(ctx, "krylov");
+      for (int64_t q = 0; q <= j; q++) h[q] = ctx->h_red[q] + ctx->h_red[(m + 2) + q];
+      h[j + 1] = sqrt(ctx->h_red[2 * (m + 2)]);
+      if (!std::isfinite(h[j + 1])) return set_err(ctx, CHY_ERR_FP, "non-finite Arnoldi vector");
+      launch_axpby<T>(ctx, nc, h[j + 1] > 0.0 ? 1.0 / h[j + 1] : 0.0, w, 0.0, Vn);
+      for (int64_t q = 0; q < j; q++) {
+        const double t = cs[q] * h[q] + sn[q] * h[q + 1];
+        h[q + 1] = -sn[q] * h[q] + cs[q] * h[q + 1];
+        h[q] = t;
+      }
+      const double rr = sqrt(h[j] * h[j] + h[j + 1] * h[j + 1]);
+      cs[j] = rr > 0.0 ? h[j] / rr : 1.0;
+      sn[j] = rr > 0.0 ? h[j + 1] / rr : 0.0;
+      h[j] = rr;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      for (int64_t q = 0; q <= j; q++) H[q * m + j] = h[q];
+      (*its)++;
+      if (ctx->prm.ksp_monitor) printf("  %lld KSP Residual norm %.12e\n", (long long)*its, fabs(g[j + 1]));
+      if (fabs(g[j + 1]) <= rtol * beta0) { j++; done = true; break; }
+    }
+    // e += Z y
+    for (int64_t q = j - 1; q >= 0; q--) {
+      double s = g[q];
+      for (int64_t r = q + 1; r < j; r++) s -= H[q * m + r] * y[r];
+      y[q] = s / H[q * m + q];
+    }
+    std::vector<double> negy(j);
+    for (int64_t q = 0; q < j; q++) negy[q] = -y[q];
+    CHY_HIP(hipMemcpyAsync(hdev, negy.data(), sizeof(double) * j, hipMemcpyHostToDevice, ctx->stream));
+    launch_mgs_update<T>(ctx, (int)j, Z, nc, hdev, e, nc);
+    CHY_HIP(hipStreamSynchronize(ctx->stream));
+    (void)done;
+  }
+}
+
+// ---- inner BiCGStab (right preconditioned) --------------------------------------------
+template <typename T>
+int bicgstab_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *x, double rtol, int64_t max_it,
+                   int64_t *its, double *relres)
+{
+  const int64_t nc = ctx->g.nc;
+  T *V = (T *)ctx->V;
+  T *r = V, *rh = V + nc, *p = V + 2 * nc, *v = V + 3 * nc, *yv = V + 4 * nc, *s = V + 5 * nc, *zv = V + 6 * nc,
+    *t = V + 7 * nc;
+  const T *n = mom<T>(ctx, 0);
+  const int lt = ctx->prm.lhs_type;
+  double *d = ctx->red_out;
+  launch_fill<T>(ctx, nc, x, (T)0);
+  launch_fill<T>(ctx, nc, p, (T)0);
+  launch_fill<T>(ctx, nc, v, (T)0);
+  CHY_HIP(hipMemcpyAsync(r, rhs, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+  CHY_HIP(hipMemcpyAsync(rh, rhs, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+  launch_mdot<T>(ctx, 1, r, nc, r, nc, d);
+  double nn;
+  TRY(fetch(ctx, d, 1, &nn));
+  const double b0 = sqrt(nn);
+  double rho = 1.0, alpha = 1.0, omega = 1.0;
+  *its = 0;
+  *relres = 1.0;
+  if (!(b0 > 0.0)) { *relres = 0.0; return CHY_OK; }
+  while (*its < max_it) {
+    launch_mdot<T>(ctx, 1, rh, nc, r, nc, d);
+    double rho1;
+    TRY(fetch(ctx, d, 1, &rho1));
+    const double beta = (rho1 / rho) * (alpha / omega);
+    // p = r + beta (p - omega v)
+    launch_axpby<T>(ctx, nc, -omega, v, 1.0, p);
+    launch_axpby<T>(ctx, nc, 1.0, r, beta, p);
+    TRY(pc_apply<T>(ctx, v19, p, yv));
+    launch_lhs_apply<T, T, T>(ctx, lt, false, n, yv, v);
+    launch_mdot<T>(ctx, 1, rh, nc, v, nc, d);
+    double rhv;
+    TRY(fetch(ctx, d, 1, &rhv));
+    alpha = rho1 / rhv;
+    CHY_HIP(hipMemcpyAsync(s, r, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+    launch_axpby<T>(ctx, nc, -alpha, v, 1.0, s);
+    launch_axpby<T>(ctx, nc, alpha, yv, 1.0, x);
+    (*its)++;
+    launch_mdot<T>(ctx, 1, s, nc, s, nc, d);
+    double ss;
+    TRY(fetch(ctx, d, 1, &ss));
+    *relres = sqrt(ss) / b0;
+    if (!std::isfinite(*relres)) return set_err(ctx, CHY_ERR_FP, "non-finite residual in BiCGStab");
+    if (*relres <= rtol) return CHY_OK;
+    TRY(pc_apply<T>(ctx, v19, s, zv));
+    launch_lhs_apply<T, T, T>(ctx, lt, false, n, zv, t);
+    launch_mdot<T>(ctx, 1, t, nc, s, nc, d);
+    launch_mdot<T>(ctx, 1, t, nc, t, nc, d + 1);
+    double ts[2];
+    TRY(fetch(ctx, d, 2, ts));
+    omega = ts[1] > 0.0 ? ts[0] / ts[1] : 0.0;
+    launch_axpby<T>(ctx, nc, omega, zv, 1.0, x);
+    CHY_HIP(hipMemcpyAsync(r, s, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+    launch_axpby<T>(ctx, nc, -omega, t, 1.0, r);
+    launch_mdot<T>(ctx, 1, r, nc, r, nc, d);
+    double rr;
+    TRY(fetch(ctx, d, 1, &rr));
+    *relres = sqrt(rr) / b0;
+    if (*relres <= rtol) return CHY_OK;
+    rho = rho1;
+  }
+  return CHY_OK;
+}
+
+// Outer FP64 defect correction; phi (device, FP64) in/out, b (device) mean-free.
+template <typename T>
+int solve_impl(chy_ctx *ctx, chy_solve_stats *st)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t nc = ctx->g.nc;
+  const chy_params &p = ctx->prm;
+  const T *n = mom<T>(ctx, 0);
+  // preconditioner symbol from the mean density
+  double v19[19];
+  {
+    launch_sum<T>(ctx, n, nc, ctx->red_out);
+    double s;
+    TRY(fetch(ctx, ctx->red_out, 1, &s));
+    uniform_row(p.lhs_type, p.d, p.kappa_e, s / (double)nc, v19);
+  }
+  launch_mdot<double>(ctx, 1, ctx->b, nc, ctx->b, nc, ctx->red_out);
+  double bb;
+  TRY(fetch(ctx, ctx->red_out, 1, &bb));
+  const double bn = sqrt(bb);
+  const double tol = std::max(p.ksp_rtol * bn, p.ksp_atol);
+  int64_t its = 0, outer = 0;
+  double rn = 0.0;
+  int converged = 0;
+  T *rhsT = (T *)ctx->t64;  // FP64-sized buffer holds the inner right-hand side
+  T *ev = (T *)ctx->V + (nvec_V(ctx) - 1) * nc;
+  for (;;) {
+    launch_residual<T, double>(ctx, p.lhs_type, n, ctx->b, ctx->phi, ctx->r64);
+    launch_mdot<double>(ctx, 1, ctx->r64, nc, ctx->r64, nc, ctx->red_out);
+    double rr;
+    TRY(fetch(ctx, ctx->red_out, 1, &rr));
+    rn = sqrt(rr);
+    if (!std::isfinite(rn)) return set_err(ctx, CHY_ERR_FP, "non-finite residual in the potential solve");
+    if (rn <= tol) { converged = 1; break; }
+    if (its >= p.ksp_max_it || outer >= 64) break;
+    const double inner_rtol = ctx->mixed ? std::max(1e-5, 0.5 * tol / rn) : std::min(1.0, 0.5 * tol / rn);
+    launch_convert<double, T>(ctx, nc, ctx->r64, 1.0 / rn, rhsT);
+    int64_t it_in = 0;
+    double rel = 1.0;
+    if (p.ksp_type == CHY_KSP_BCGS)
+      TRY(bicgstab_inner<T>(ctx, v19, rhsT, ev, inner_rtol, p.ksp_max_it - its, &it_in, &rel));
+    else
+      TRY(gmres_inner<T>(ctx, v19, rhsT, ev, inner_rtol, p.ksp_max_it - its, &it_in, &rel));
+    its += it_in;
+    outer++;
+    // phi += rn * e
+    launch_convert<T, double>(ctx, nc, ev, rn, ctx->r64);
+    launch_axpby<double>(ctx, nc, 1.0, ctx->r64, 1.0, ctx->phi);
+    if (it_in == 0) break;
+  }
+  // mean projection of phi (null space)
+  launch_sum<double>(ctx, ctx->phi, nc, ctx->red_out);
+  launch_shift<double>(ctx, nc, ctx->phi, ctx->red_out, -1.0 / (double)nc);
+  TRY(sync(ctx));
+  const auto t1 = std::chrono::steady_clock::now();
+  if (st) {
+    st->iterations = its;
+    st->outer = outer;
+    st->residual = bn > 0.0 ? rn / bn : rn;
+    st->converged = converged;
+    st->reason = converged ? (rn <= p.ksp_rtol * bn ? 2 : 3) : -3;
+    st->ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  }
+  if (p.ksp_converged_reason)
+    printf("Linear solve %s due to %s iterations %lld\n", converged ? "converged" : "did not converge",
+           converged ? "CONVERGED_RTOL" : "DIVERGED_ITS", (long long)its);
+  if (!converged && p.ksp_error_if_not_converged)
+    return set_err(ctx, CHY_ERR_NOT_CONVERGED, "KSPSolve has not converged");
+  ctx->phi_valid = true;
+  return CHY_OK;
+}
+
+int do_rhs(chy_ctx *ctx)
+{
+  if (!ctx->moments_valid) return set_err(ctx, CHY_ERR_ARG_WRONG, "no moments: collect or set them first");
+  timer_start(ctx, "rhs");
+  if (ctx->mixed)
+    launch_rhs<float>(ctx, ctx->prm.rhs_type, mom<float>(ctx, 0), mom<float>(ctx, 1), mom<float>(ctx, 2),
+                      mom<float>(ctx, 3), ctx->b);
+  else
+    launch_rhs<double>(ctx, ctx->prm.rhs_type, mom<double>(ctx, 0), mom<double>(ctx, 1), mom<double>(ctx, 2),
+                       mom<double>(ctx, 3), ctx->b);
+  timer_stop(ctx, "rhs");
+  return CHY_OK;
+}
+
+int do_solve(chy_ctx *ctx, chy_solve_stats *st)
+{
+  if (!ctx->phi_valid) launch_fill<double>(ctx, ctx->g.nc, ctx->phi, 0.0);  // potential.c:20-29
+  return ctx->mixed ? solve_impl<float>(ctx, st) : solve_impl<double>(ctx, st);
+}
+
+int do_efield(chy_ctx *ctx)
+{
+  if (!ctx->phi_valid) launch_fill<double>(ctx, ctx->g.nc, ctx->phi, 0.0);
+  if (ctx->mixed)
+    launch_efield<float>(ctx, ctx->phi, (float *)ctx->efield[0], (float *)ctx->efield[1], (float *)ctx->efield[2]);
+  else
+    launch_efield<double>(ctx, ctx->phi, (double *)ctx->efield[0], (double *)ctx->efield[1], (double *)ctx->efield[2]);
+  return CHY_OK;
+}
+
+bool check(const chy_ctx *c) { return c != nullptr; }
+
+}  // namespace
+
+// =============================== C ABI ===================================
+extern "C" {
+
+int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
+{
+  if (!prm_in || !out) return CHY_ERR_ARG_WRONG;
+  *out = nullptr;
+  chy_ctx *ctx = new chy_ctx();
+  ctx->prm = *prm_in;
+  char err[2048] = "";
+  int rc = chy_params_derive(&ctx->prm, err, sizeof err);
+  if (rc) { int r = rc; delete ctx; fprintf(stderr, "chy_create: %s", err); return r; }
+  const chy_params &p = ctx->prm;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    delete ctx;
+    return CHY_ERR_LIB;
+  }
+  if (hipSetDevice(0) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) { delete ctx; return CHY_ERR_LIB; }
+  ctx->mixed = p.precision == CHY_PREC_MIXED;
+  Grid &g = ctx->g;
+  g.nx = (int)p.N[0]; g.ny = (int)p.N[1]; g.nz = (int)p.N[2];
+  g.nc = p.N[0] * p.N[1] * p.N[2];
+  for (int a = 0; a < 3; a++) { g.p0[a] = p.p0[a]; g.p1[a] = p.p1[a]; g.d[a] = p.d[a]; g.L[a] = p.L[a]; }
+  if (g.nc >= (int64_t)1 << 31) { delete ctx; return CHY_ERR_ARG_RANGE; }
+  ctx->cap = np_capacity > 0 ? np_capacity : std::max<int64_t>(p.Np, 1);
+  ctx->weighted = p.perturbation != CHY_PERT_NONE || p.weighted;
+  ctx->restart = std::max<int64_t>(1, std::min<int64_t>(p.ksp_restart, 64));
+  const int64_t nc = g.nc, cap = ctx->cap;
+  const size_t ts = ctx->mixed ? 4 : 8;
+#define A(x) do { int r_ = (x); if (r_) { chy_destroy(ctx); return r_; } } while (0)
+  for (int s = 0; s < 2; s++) {
+    if (ctx->mixed) A(alloc_state<float>(ctx, ctx->pf[s], cap, ctx->weighted));
+    else A(alloc_state<double>(ctx, ctx->pd[s], cap, ctx->weighted));
+  }
+  A(dalloc(ctx, &ctx->perm, cap));
+  A(dalloc(ctx, &ctx->slotkey, cap));
+  A(dalloc(ctx, &ctx->counts, nc + 1));
+  A(dalloc(ctx, &ctx->offsets, nc + 1));
+  A(dalloc(ctx, &ctx->cursor, nc + 1));
+  A(dalloc(ctx, &ctx->scan_tmp, nc / 4096 + 2));
+  A(dalloc(ctx, &ctx->maxcount, 1));
+  {
+    unsigned char *pp = nullptr;
+    A(dalloc(ctx, &pp, (int64_t)(32 * ts) * nc));
+    ctx->partials = pp;
+  }
+  A(dalloc(ctx, &ctx->sred, 4096));
+  A(dalloc(ctx, &ctx->S_dev, 1));
+  A(dalloc(ctx, &ctx->cstats, 4));
+  for (int a = 0; a < 4; a++) { unsigned char *q = nullptr; A(dalloc(ctx, &q, (int64_t)ts * nc)); ctx->mom[a] = q; }
+  for (int a = 0; a < 3; a++) { unsigned char *q = nullptr; A(dalloc(ctx, &q, (int64_t)ts * nc)); ctx->efield[a] = q; }
+  A(dalloc(ctx, &ctx->phi, nc));
+  A(dalloc(ctx, &ctx->b, nc));
+  A(dalloc(ctx, &ctx->r64, nc));
+  A(dalloc(ctx, &ctx->t64, nc));
+  {
+    unsigned char *q = nullptr;
+    A(dalloc(ctx, &q, (int64_t)ts * nc * nvec_V(ctx))); ctx->V = q;
+    A(dalloc(ctx, &q, (int64_t)ts * nc * ctx->restart)); ctx->Z = q;
+    A(dalloc(ctx, &q, (int64_t)ts * nc)); ctx->wv = q;
+    A(dalloc(ctx, &q, (int64_t)ts * 2 * nc)); ctx->fft_buf = q;
+  }
+  A(dalloc(ctx, &ctx->red, (int64_t)kRedBlocks * 32));
+  A(dalloc(ctx, &ctx->red_out, 3 * (ctx->restart + 2) + 16));
+  if (hipHostMalloc((void **)&ctx->h_red, sizeof(double) * (3 * (ctx->restart + 2) + 16)) != hipSuccess) {
+    chy_destroy(ctx);
+    return CHY_ERR_MEM;
+  }
+  A(fft_setup(ctx));
+  launch_fill<double>(ctx, nc, ctx->phi, 0.0);
+  launch_fill<double>(ctx, 1, ctx->S_dev, 0.0);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) { chy_destroy(ctx); return CHY_ERR_LIB; }
+#undef A
+  *out = ctx;
+  return CHY_OK;
+}
+
+int chy_destroy(chy_ctx *ctx)
+{
+  if (!ctx) return CHY_OK;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (int s = 0; s < 2; s++) { free_state(ctx->pd[s]); free_state(ctx->pf[s]); }
+  void *ps[] = {ctx->perm, ctx->slotkey, ctx->counts, ctx->offsets, ctx->cursor, ctx->scan_tmp, ctx->maxcount,
+                ctx->partials,
+                ctx->sred, ctx->S_dev, ctx->cstats, ctx->mom[0], ctx->mom[1], ctx->mom[2], ctx->mom[3],
+                ctx->efield[0], ctx->efield[1], ctx->efield[2], ctx->phi, ctx->b, ctx->r64, ctx->t64, ctx->V,
+                ctx->Z, ctx->wv, ctx->fft_buf, ctx->red, ctx->red_out, ctx->tw[0], ctx->tw[1], ctx->tw[2],
+                ctx->trig[0], ctx->trig[1], ctx->trig[2]};
+  for (void *p : ps) if (p) (void)hipFree(p);
+  if (ctx->h_red) (void)hipHostFree(ctx->h_red);
+  for (auto &kv : ctx->timers) {
+    if (kv.second.a) (void)hipEventDestroy(kv.second.a);
+    if (kv.second.b) (void)hipEventDestroy(kv.second.b);
+  }
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CHY_OK;
+}
+
+const char *chy_last_error(const chy_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int chy_sync(chy_ctx *ctx) { return check(ctx) ? sync(ctx) : CHY_ERR_ARG_WRONG; }
+
+void *chy_stream(chy_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int chy_get_params(const chy_ctx *ctx, chy_params *prm)
+{
+  if (!ctx || !prm) return CHY_ERR_ARG_WRONG;
+  *prm = ctx->prm;
+  return CHY_OK;
+}
+
+int64_t chy_num_particles(const chy_ctx *ctx) { return ctx ? ctx->np : -1; }
+
+int chy_set_step(chy_ctx *ctx, int64_t it)
+{
+  if (!ctx || it < 0) return CHY_ERR_ARG_WRONG;
+  ctx->step = it;
+  return CHY_OK;
+}
+
+int chy_set_particles(chy_ctx *ctx, int64_t np, const double *pos, const double *vel, const double *w,
+                      const uint32_t *pid)
+{
+  if (!ctx || (np > 0 && (!pos || !vel))) return CHY_ERR_ARG_WRONG;
+  int rc = particles_upload(ctx, np, pos, vel, w, pid);
+  if (rc) return rc;
+  return sync(ctx);
+}
+
+int chy_get_particles(chy_ctx *ctx, double *pos, double *vel, double *w, uint32_t *pid)
+{
+  if (!ctx) return CHY_ERR_ARG_WRONG;
+  return particles_download(ctx, pos, vel, w, pid);
+}
+
+int chy_init_particles(chy_ctx *ctx)
+{
+  if (!ctx) return CHY_ERR_ARG_WRONG;
+  int rc = particles_init(ctx);
+  if (rc) return rc;
+  return sync(ctx);
+}
+
+int chy_update_velocities(chy_ctx *ctx, chy_collision_stats *stats)
+{
+  if (!ctx) return CHY_ERR_ARG_WRONG;
+  do_efield(ctx);
+  const int rc = particles_push(ctx, false, ctx->prm.mn > 0.0, stats);
+  ctx->step++;
+  return rc;
+}
+
+int chy_update_positions(chy_ctx *ctx)
+{
+  if (!ctx) return CHY_ERR_ARG_WRONG;
+  int rc = particles_move_only(ctx);
+  if (rc) return rc;
+  rc = particles_sort(ctx);
+  if (rc) return rc;
+  ctx->moments_valid = false;
+  return sync(ctx);
+}
+
+int chy_collect_moments(chy_ctx *ctx, double *vlasov)
+{
+  if (!ctx) return CHY_ERR_ARG_WRONG;
+  timer_start(ctx, "deposit");
+  int rc = particles_deposit(ctx);
+  timer_stop(ctx, "deposit");
+  if (rc) return rc;
+  if (vlasov) {
+    rc = sync(ctx);
+    if (rc) return rc;
+    return chy_get_moments(ctx, vlasov);
+  }
+  return sync(ctx);
+}
+
+int chy_set_moments(chy_ctx *ctx, const double *vlasov)
+{
+  if (!ctx || !vlasov) return CHY_ERR_ARG_WRONG;
+  const int64_t nc = ctx->g.nc;
+  for (int a = 0; a < 4; a++) {
+    if (ctx->mixed) {
+      std::vector<float> f(nc);
+      for (int64_t c = 0; c < nc; c++) f[c] = (float)vlasov[4 * c + a];
+      CHY_HIP(hipMemcpy(ctx->mom[a], f.data(), sizeof(float) * nc, hipMemcpyHostToDevice));
+    } else {
+      std::vector<double> f(nc);
+      for (int64_t c = 0; c < nc; c++) f[c] = vlasov[4 * c + a];
+      CHY_HIP(hipMemcpy(ctx->mom[a], f.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+    }
+  }
+  ctx->moments_valid = true;
+  return CHY_OK;
+}
+
+int chy_get_moments(chy_ctx *ctx, double *vlasov)
+{
+  if (!ctx || !vlasov) return CHY_ERR_ARG_WRONG;
+  const int64_t nc = ctx->g.nc;
+  if (!ctx->moments_valid) { for (int64_t c = 0; c < 4 * nc; c++) vlasov[c] = 0.0; return CHY_OK; }
+  for (int a = 0; a < 4; a++) {
+    std::vector<double> m(nc);
+    if (ctx->mixed) {
+      std::vector<float> f(nc);
+      CHY_HIP(hipMemcpy(f.data(), ctx->mom[a], sizeof(float) * nc, hipMemcpyDeviceToHost));
+      for (int64_t c = 0; c < nc; c++) m[c] = f[c];
+    } else {
+      CHY_HIP(hipMemcpy(m.data(), ctx->mom[a], sizeof(double) * nc, hipMemcpyDeviceToHost));
+    }
+    for (int64_t c = 0; c < nc; c++) vlasov[4 * c + a] = m[c];
+  }
+  return CHY_OK;
+}
+
+int chy_compute_rhs(chy_ctx *ctx, double *b)
+{
+  if (!ctx) return CHY_ERR_ARG_WRONG;
+  int rc = do_rhs(ctx);
+  if (rc) return rc;
+  if (b) CHY_HIP(hipMemcpy(b, ctx->b, sizeof(double) * ctx->g.nc, hipMemcpyDeviceToHost));
+  return sync(ctx);
+}
+
+static int lhs_apply_host(chy_ctx *ctx, const double *x, double *y, bool transpose)
+{
+  if (!ctx || !x || !y) return CHY_ERR_ARG_WRONG;
+  if (!ctx->moments_valid && ctx->prm.lhs_type == CHY_LHS_FULL)
+    return set_err(ctx, CHY_ERR_ARG_WRONG, "no density: collect or set moments first");
+  const int64_t nc = ctx->g.nc;
+  CHY_HIP(hipMemcpy(ctx->r64, x, sizeof(double) * nc, hipMemcpyHostToDevice));
+  timer_start(ctx, "lhs_apply");
+  if (ctx->mixed)
+    launch_lhs_apply<float, double, double>(ctx, ctx->prm.lhs_type, transpose, mom<float>(ctx, 0), ctx->r64, ctx->t64);
+  else
+    launch_lhs_apply<double, double, double>(ctx, ctx->prm.lhs_type, transpose, mom<double>(ctx, 0), ctx->r64,
+                                             ctx->t64);
+  timer_stop(ctx, "lhs_apply");
+  CHY_HIP(hipMemcpyAsync(y, ctx->t64, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream));
+  return sync(ctx);
+}
+
+int chy_lhs_apply(chy_ctx *ctx, const double *x, double *y) { return lhs_apply_host(ctx, x, y, false); }
+int chy_lhs_apply_transpose(chy_ctx *ctx, const double *x, double *y) { return lhs_apply_host(ctx, x, y, true); }
+
+int chy_solve_potential(chy_ctx *ctx, double *phi, chy_solve_stats *st)
+{
+  if (!ctx) return CHY_ERR_ARG_WRONG;
+  const auto t0 = std::chrono::steady_clock::now();
+  int rc = do_rhs(ctx);
+  if (rc) return rc;
+  rc = do_solve(ctx, st);
+  if (st) st->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (rc) return rc;
+  if (phi) CHY_HIP(hipMemcpy(phi, ctx->phi, sizeof(double) * ctx->g.nc, hipMemcpyDeviceToHost));
+  return CHY_OK;
+}
+
+int chy_solve(chy_ctx *ctx, const double *b, double *phi, chy_solve_stats *st)
+{
+  if (!ctx || !b || !phi) return CHY_ERR_ARG_WRONG;
+  if (!ctx->moments_valid && ctx->prm.lhs_type == CHY_LHS_FULL)
+    return set_err(ctx, CHY_ERR_ARG_WRONG, "no density: collect or set moments first");
+  const int64_t nc = ctx->g.nc;
+  CHY_HIP(hipMemcpy(ctx->b, b, sizeof(double) * nc, hipMemcpyHostToDevice));
+  CHY_HIP(hipMemcpy(ctx->phi, phi, sizeof(double) * nc, hipMemcpyHostToDevice));
+  ctx->phi_valid = true;
+  int rc = do_solve(ctx, st);
+  if (rc) return rc;
+  CHY_HIP(hipMemcpy(phi, ctx->phi, sizeof(double) * nc, hipMemcpyDeviceToHost));
+  return CHY_OK;
+}
+
+int chy_get_potential(chy_ctx *ctx, double *phi)
+{
+  if (!ctx || !phi) return CHY_ERR_ARG_WRONG;
+  if (!ctx->phi_valid) { for (int64_t c = 0; c < ctx->g.nc; c++) phi[c] = 0.0; return CHY_OK; }
+  CHY_HIP(hipMemcpy(phi, ctx->phi, sizeof(double) * ctx->g.nc, hipMemcpyDeviceToHost));
+  return CHY_OK;
+}
+
+int chy_set_potential(chy_ctx *ctx, const double *phi)
+{
+  if (!ctx || !phi) return CHY_ERR_ARG_WRONG;
+  CHY_HIP(hipMemcpy(ctx->phi, phi, sizeof(double) * ctx->g.nc, hipMemcpyHostToDevice));
+  ctx->phi_valid = true;
+  return CHY_OK;
+}
+
+int chy_get_rhs(chy_ctx *ctx, double *b)
+{
+  if (!ctx || !b) return CHY_ERR_ARG_WRONG;
+  CHY_HIP(hipMemcpy(b, ctx->b, sizeof(double) * ctx->g.nc, hipMemcpyDeviceToHost));
+  return CHY_OK;
+}
+
+int chy_step(chy_ctx *ctx, chy_step_stats *st)
+{
+  if (!ctx) return CHY_ERR_ARG_WRONG;
+  const auto t0 = std::chrono::steady_clock::now();
+  chy_step_stats local;
+  memset(&local, 0, sizeof local);
+  do_efield(ctx);
+  int rc = particles_push(ctx, true, ctx->prm.mn > 0.0, &local.collisions);
+  ctx->step++;
+  if (rc) return rc;
+  timer_start(ctx, "sort");
+  rc = particles_sort(ctx);
+  timer_stop(ctx, "sort");
+  if (rc) return rc;
+  timer_start(ctx, "deposit");
+  rc = particles_deposit(ctx);
+  timer_stop(ctx, "deposit");
+  if (rc) return rc;
+  rc = sync(ctx);
+  if (rc) return rc;
+  const auto t1 = std::chrono::steady_clock::now();
+  rc = do_rhs(ctx);
+  if (rc) return rc;
+  rc = do_solve(ctx, &local.solve);
+  if (rc) return rc;
+  const auto t2 = std::chrono::steady_clock::now();
+  local.solve.ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  local.ms_particles = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  local.ms_total = std::chrono::duration<double, std::milli>(t2 - t0).count();
+  if (st) *st = local;
+  return CHY_OK;
+}
+
+int chy_timing_enable(chy_ctx *ctx, int on)
+{
+  if (!ctx) return CHY_ERR_ARG_WRONG;
+  ctx->timing = on != 0;
+  return CHY_OK;
+}
+
+int chy_timing_get(chy_ctx *ctx, const char *name, double *total_ms, int64_t *count)
+{
+  if (!ctx || !name) return CHY_ERR_ARG_WRONG;
+  auto it = ctx->timers.find(name);
+  if (total_ms) *total_ms = it == ctx->timers.end() ? 0.0 : it->second.total_ms;
+  if (count) *count = it == ctx->timers.end() ? 0 : it->second.count;
+  return CHY_OK;
+}
+
+int chy_timing_reset(chy_ctx *ctx)
+{
+  if (!ctx) return CHY_ERR_ARG_WRONG;
+  for (auto &kv : ctx->timers) { kv.second.total_ms = 0.0; kv.second.count = 0; }
+  return CHY_OK;
+}
+
+}  // extern "C"

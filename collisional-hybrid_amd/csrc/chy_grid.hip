@@ -1,0 +1,555 @@
+// chy_grid.hip -- grid kernels for gfx950: the matrix-free 19-point electron
+// operator (reference lhs.c:350-543), its residual and transpose, the node
+// electric field, the forcing (rhs.c:8-303 with calculus.c stencils), and
+// deterministic FP64 reductions for the Krylov solver.
+//
+// Every kernel is HBM-bound (no dense contraction), so the launches are
+// shaped for coalesced x-fastest access; FP64 kernels use the oracle's
+// operation order (the library is built with -ffp-contract=off).
+#include <cmath>
+
+#include "chy_internal.h"
+
+namespace chy {
+
+SCoef full_scoef(const double d[3], const double K[3])
+{
+  const double Kx = K[0], Ky = K[1], Kz = K[2];
+  const double detA = 1 + Kx * Kx + Ky * Ky + Kz * Kz;
+  const double rxx = 1 + Kx * Kx, rxy = Ky * Kx - Kz, rxz = Kz * Kx + Ky;
+  const double ryx = Kx * Ky + Kz, ryy = 1 + Ky * Ky, ryz = Kz * Ky - Kx;
+  const double rzx = Kx * Kz - Ky, rzy = Ky * Kz + Kx, rzz = 1 + Kz * Kz;
+  const double dx = d[0], dy = d[1], dz = d[2];
+  const double hxx = 1.0 / (2.0 * dx * dx), hyy = 1.0 / (2.0 * dy * dy), hzz = 1.0 / (2.0 * dz * dz);
+  const double hxy = 1.0 / (8.0 * dy * dx), hxz = 1.0 / (8.0 * dz * dx), hyz = 1.0 / (8.0 * dy * dz);
+  const double scale = 1.0 / detA;
+  SCoef s;
+  s.sxx = scale * rxx * hxx; s.syy = scale * ryy * hyy; s.szz = scale * rzz * hzz;
+  s.sxy = scale * rxy * hxy; s.sxz = scale * rxz * hxz; s.syz = scale * ryz * hyz;
+  s.syx = scale * ryx * hxy; s.szx = scale * rzx * hxz; s.szy = scale * rzy * hyz;
+  return s;
+}
+
+// Column offsets of one row, reference lhs.c:11-104.
+__constant__ int kOff[19][3] = {
+  {+1, 0, 0}, {-1, 0, 0}, {0, +1, 0}, {0, -1, 0}, {0, 0, +1}, {0, 0, -1},
+  {+1, +1, 0}, {+1, -1, 0}, {-1, +1, 0}, {-1, -1, 0},
+  {+1, 0, +1}, {+1, 0, -1}, {-1, 0, +1}, {-1, 0, -1},
+  {0, +1, +1}, {0, +1, -1}, {0, -1, +1}, {0, -1, -1},
+  {0, 0, 0}};
+static const int hOff[19][3] = {
+  {+1, 0, 0}, {-1, 0, 0}, {0, +1, 0}, {0, -1, 0}, {0, 0, +1}, {0, 0, -1},
+  {+1, +1, 0}, {+1, -1, 0}, {-1, +1, 0}, {-1, -1, 0},
+  {+1, 0, +1}, {+1, 0, -1}, {-1, 0, +1}, {-1, 0, -1},
+  {0, +1, +1}, {0, +1, -1}, {0, -1, +1}, {0, -1, -1},
+  {0, 0, 0}};
+
+// Row coefficients from the 7-point density star (lhs.c:479-511).
+template <typename C>
+__host__ __device__ __forceinline__ void row_from_star(const SCoef &s, C nc, C nxp, C nxm, C nyp, C nym,
+                                                       C nzp, C nzm, C v[19])
+{
+  const C sxx = (C)s.sxx, syy = (C)s.syy, szz = (C)s.szz, sxy = (C)s.sxy, sxz = (C)s.sxz;
+  const C syz = (C)s.syz, syx = (C)s.syx, szx = (C)s.szx, szy = (C)s.szy;
+  v[6] = sxy * (nxp + nc) + syx * (nyp + nc);
+  v[7] = -sxy * (nxp + nc) - syx * (nc + nym);
+  v[8] = -sxy * (nc + nxm) - syx * (nyp + nc);
+  v[9] = sxy * (nc + nxm) + syx * (nc + nym);
+  v[10] = sxz * (nxp + nc) + szx * (nzp + nc);
+  v[11] = -sxz * (nxp + nc) - szx * (nc + nzm);
+  v[12] = -sxz * (nc + nxm) - szx * (nzp + nc);
+  v[13] = sxz * (nc + nxm) + szx * (nc + nzm);
+  v[14] = syz * (nyp + nc) + szy * (nzp + nc);
+  v[15] = -syz * (nyp + nc) - szy * (nc + nzm);
+  v[16] = -syz * (nc + nym) - szy * (nzp + nc);
+  v[17] = syz * (nc + nym) + szy * (nc + nzm);
+  v[0] = sxx * (nxp + nc) + syx * (nyp - nym) + szx * (nzp - nzm);
+  v[1] = sxx * (nc + nxm) - syx * (nyp - nym) - szx * (nzp - nzm);
+  v[2] = syy * (nyp + nc) + sxy * (nxp - nxm) + szy * (nzp - nzm);
+  v[3] = syy * (nc + nym) - sxy * (nxp - nxm) - szy * (nzp - nzm);
+  v[4] = szz * (nzp + nc) + sxz * (nxp - nxm) + syz * (nyp - nym);
+  v[5] = szz * (nc + nzm) - sxz * (nxp - nxm) - syz * (nyp - nym);
+  v[18] = -(sxx * (nxp + 2 * nc + nxm) + syy * (nyp + 2 * nc + nym) + szz * (nzp + 2 * nc + nzm));
+}
+
+void uniform_row(int lhs_type, const double d[3], const double K[3], double nbar, double v[19])
+{
+  for (int q = 0; q < 19; q++) v[q] = 0.0;
+  if (lhs_type == CHY_LHS_FULL) {
+    const SCoef s = full_scoef(d, K);
+    row_from_star<double>(s, nbar, nbar, nbar, nbar, nbar, nbar, nbar, v);
+  } else if (lhs_type == CHY_LHS_LAPLACIAN) {
+    const double sxx = d[1] * d[2] / d[0], syy = d[0] * d[2] / d[1], szz = d[0] * d[1] / d[2];
+    v[0] = sxx; v[1] = sxx; v[2] = syy; v[3] = syy; v[4] = szz; v[5] = szz;
+    v[18] = -(v[0] + v[2] + v[4] + v[1] + v[3] + v[5]);
+  } else {
+    v[18] = 1.0;
+  }
+}
+
+// Coefficients of row c for any operator type (laplacian / identity rows are
+// passed in lapv).
+template <typename C, typename TN>
+__device__ __forceinline__ void row_coeffs(const Grid &g, int lhs_type, const SCoef &s, const double *lapv,
+                                           const TN *n, int i, int j, int k, C v[19])
+{
+  if (lhs_type == CHY_LHS_FULL) {
+    const C nc = (C)n[lin(g, i, j, k)];
+    row_from_star<C>(s, nc, (C)n[lin(g, i + 1, j, k)], (C)n[lin(g, i - 1, j, k)], (C)n[lin(g, i, j + 1, k)],
+                     (C)n[lin(g, i, j - 1, k)], (C)n[lin(g, i, j, k + 1)], (C)n[lin(g, i, j, k - 1)], v);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 19; q++) v[q] = (C)lapv[q];
+  }
+}
+
+struct LapRow { double v[19]; };
+
+// y = A x (FP64: oracle order sum_q v_q x_q; FP32: difference form
+// sum_q v_q (x_q - x_c), exact in exact arithmetic because rows sum to zero).
+template <typename TN, typename TX, typename TY, typename C, bool DIFF>
+__global__ __launch_bounds__(256) void k_lhs_apply(Grid g, int lhs_type, SCoef s, LapRow lap, const TN *__restrict__ n,
+                                                   const TX *__restrict__ x, TY *__restrict__ y)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  C v[19];
+  row_coeffs<C, TN>(g, lhs_type, s, lap.v, n, i, j, k, v);
+  C acc = (C)0;
+  if (DIFF) {
+    const C xc = (C)x[c];
+#pragma unroll
+    for (int q = 0; q < 18; q++) acc += v[q] * ((C)x[lin(g, i + kOff[q][0], j + kOff[q][1], k + kOff[q][2])] - xc);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 19; q++) acc += v[q] * (C)x[lin(g, i + kOff[q][0], j + kOff[q][1], k + kOff[q][2])];
+  }
+  y[c] = (TY)acc;
+}
+
+// y = A^T x: (A^T x)_c = sum_q A[c - o_q, c] x[c - o_q]; the coefficient of
+// column c in row r = c - o_q is that row's entry q.
+template <typename TN, typename TX, typename TY, typename C>
+__global__ __launch_bounds__(256) void k_lhs_apply_t(Grid g, int lhs_type, SCoef s, LapRow lap, const TN *__restrict__ n,
+                                                     const TX *__restrict__ x, TY *__restrict__ y)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  C acc = (C)0;
+#pragma unroll 1
+  for (int q = 0; q < 19; q++) {
+    const int ri = i - kOff[q][0], rj = j - kOff[q][1], rk = k - kOff[q][2];
+    C v[19];
+    row_coeffs<C, TN>(g, lhs_type, s, lap.v, n, ri, rj, rk, v);
+    acc += v[q] * (C)x[lin(g, ri, rj, rk)];
+  }
+  y[c] = (TY)acc;
+}
+
+// r = b - A x (FP64 outer residual).
+template <typename TN, typename TX>
+__global__ __launch_bounds__(256) void k_residual(Grid g, int lhs_type, SCoef s, LapRow lap, const TN *__restrict__ n,
+                                                  const double *__restrict__ b, const TX *__restrict__ x,
+                                                  double *__restrict__ r)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  double v[19];
+  row_coeffs<double, TN>(g, lhs_type, s, lap.v, n, i, j, k, v);
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < 19; q++) acc += v[q] * (double)x[lin(g, i + kOff[q][0], j + kOff[q][1], k + kOff[q][2])];
+  r[c] = b[c] - acc;
+}
+
+static LapRow lap_row(chy_ctx *ctx, int lhs_type)
+{
+  LapRow L;
+  uniform_row(lhs_type == CHY_LHS_FULL ? CHY_LHS_IDENTITY : lhs_type, ctx->prm.d, ctx->prm.kappa_e, 0.0, L.v);
+  return L;
+}
+
+static inline unsigned nblocks(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+
+template <typename TN, typename TX, typename TY>
+void launch_lhs_apply(chy_ctx *ctx, int lhs_type, bool transpose, const TN *n, const TX *x, TY *y)
+{
+  const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
+  const LapRow L = lap_row(ctx, lhs_type);
+  constexpr bool fp32 = sizeof(TX) == 4 && sizeof(TY) == 4;
+  using C = typename std::conditional<fp32, float, double>::type;
+  if (transpose)
+    hipLaunchKernelGGL((k_lhs_apply_t<TN, TX, TY, C>), nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g,
+                       lhs_type, s, L, n, x, y);
+  else
+    hipLaunchKernelGGL((k_lhs_apply<TN, TX, TY, C, fp32>), nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g,
+                       lhs_type, s, L, n, x, y);
+}
+
+template <typename TN, typename TX>
+void launch_residual(chy_ctx *ctx, int lhs_type, const TN *n, const double *b, const TX *x, double *r)
+{
+  const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
+  const LapRow L = lap_row(ctx, lhs_type);
+  hipLaunchKernelGGL((k_residual<TN, TX>), nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g, lhs_type, s, L, n, b,
+                     x, r);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_jacobi(Grid g, int lhs_type, SCoef s, LapRow lap, const T *__restrict__ n,
+                                                const T *__restrict__ r, T *__restrict__ z)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  double v[19];
+  row_coeffs<double, T>(g, lhs_type, s, lap.v, n, i, j, k, v);
+  z[c] = (T)((double)r[c] / v[18]);
+}
+
+template <typename T>
+void launch_jacobi(chy_ctx *ctx, int lhs_type, const T *n, const T *r, T *z)
+{
+  const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
+  const LapRow L = lap_row(ctx, lhs_type);
+  hipLaunchKernelGGL((k_jacobi<T>), nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g, lhs_type, s, L, n, r, z);
+}
+
+// Node field E = E0 - grad(phi), central differences (periodic).
+template <typename T>
+__global__ __launch_bounds__(256) void k_efield(Grid g, double e0x, double e0y, double e0z, double hx, double hy,
+                                                double hz, const double *__restrict__ phi, T *__restrict__ ex,
+                                                T *__restrict__ ey, T *__restrict__ ez)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  ex[c] = (T)(e0x - (phi[lin(g, i + 1, j, k)] - phi[lin(g, i - 1, j, k)]) * hx);
+  ey[c] = (T)(e0y - (phi[lin(g, i, j + 1, k)] - phi[lin(g, i, j - 1, k)]) * hy);
+  ez[c] = (T)(e0z - (phi[lin(g, i, j, k + 1)] - phi[lin(g, i, j, k - 1)]) * hz);
+}
+
+template <typename T>
+void launch_efield(chy_ctx *ctx, const double *phi, T *ex, T *ey, T *ez)
+{
+  const double *E0 = ctx->prm.E0, *d = ctx->prm.d;
+  hipLaunchKernelGGL((k_efield<T>), nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g, E0[0], E0[1], E0[2],
+                     0.5 / d[0], 0.5 / d[1], 0.5 / d[2], phi, ex, ey, ez);
+}
+
+// Full forcing (rhs.c:244-285) with the intended y-z cross derivative.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rhs_full(Grid g, RhsCoef rc, const T *__restrict__ n,
+                                                  const T *__restrict__ gx, const T *__restrict__ gy,
+                                                  const T *__restrict__ gz, double *__restrict__ b)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  const double dx = rc.dx, dy = rc.dy, dz = rc.dz;
+#define F(a, di, dj, dk) ((double)a[lin(g, i + (di), j + (dj), k + (dk))])
+  const double dndx = (F(n, 1, 0, 0) - F(n, -1, 0, 0)) / (2.0 * dx);
+  const double dndy = (F(n, 0, 1, 0) - F(n, 0, -1, 0)) / (2.0 * dy);
+  const double dndz = (F(n, 0, 0, 1) - F(n, 0, 0, -1)) / (2.0 * dz);
+  const double dGxdx = (F(gx, 1, 0, 0) - F(gx, -1, 0, 0)) / (2.0 * dx);
+  const double dGydy = (F(gy, 0, 1, 0) - F(gy, 0, -1, 0)) / (2.0 * dy);
+  const double dGzdz = (F(gz, 0, 0, 1) - F(gz, 0, 0, -1)) / (2.0 * dz);
+  double Pterm = 0.0;
+  if (rc.cth != 0.0) {
+    const double d2xx = (F(n, 1, 0, 0) - 2.0 * F(n, 0, 0, 0) + F(n, -1, 0, 0)) / (dx * dx);
+    const double d2yy = (F(n, 0, 1, 0) - 2.0 * F(n, 0, 0, 0) + F(n, 0, -1, 0)) / (dy * dy);
+    const double d2zz = (F(n, 0, 0, 1) - 2.0 * F(n, 0, 0, 0) + F(n, 0, 0, -1)) / (dz * dz);
+    const double d2xy = (F(n, 1, 1, 0) - F(n, -1, 1, 0) - F(n, 1, -1, 0) + F(n, -1, -1, 0)) / (4.0 * dx * dy);
+    const double d2xz = (F(n, 1, 0, 1) - F(n, -1, 0, 1) - F(n, 1, 0, -1) + F(n, -1, 0, -1)) / (4.0 * dx * dz);
+    const double d2yz = (F(n, 0, 1, 1) - F(n, 0, -1, 1) - F(n, 0, 1, -1) + F(n, 0, -1, -1)) / (4.0 * dy * dz);
+    Pterm = rc.cth * (rc.r[0][0] * d2xx + rc.r[0][1] * d2xy + rc.r[0][2] * d2xz +
+                      rc.r[1][0] * d2xy + rc.r[1][1] * d2yy + rc.r[1][2] * d2yz +
+                      rc.r[2][0] * d2xz + rc.r[2][1] * d2yz + rc.r[2][2] * d2zz);
+  }
+#undef F
+  const double Eterm = (rc.r[0][0] * rc.E0[0] + rc.r[0][1] * rc.E0[1] + rc.r[0][2] * rc.E0[2]) * dndx +
+                       (rc.r[1][0] * rc.E0[0] + rc.r[1][1] * rc.E0[1] + rc.r[1][2] * rc.E0[2]) * dndy +
+                       (rc.r[2][0] * rc.E0[0] + rc.r[2][1] * rc.E0[1] + rc.r[2][2] * rc.E0[2]) * dndz;
+  const double Gterm = rc.detA * rc.cG * (dGxdx + dGydy + dGzdz);
+  b[c] = rc.scale * (Eterm + Pterm + Gterm);
+}
+
+// Sinusoidal forcing (rhs.c:98-112); n0 is a device scalar (sum of n).
+__global__ __launch_bounds__(256) void k_rhs_sin(Grid g, double dx, double dy, double dz, double fac, double inv_nc,
+                                                 const double *__restrict__ nsum, double *__restrict__ b)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  const double n0 = nsum[0] * inv_nc;
+  const double x = ((double)i + 0.5) * dx, y = ((double)j + 0.5) * dy, z = ((double)k + 0.5) * dz;
+  const double val = cos(2 * M_PI * x) * cos(2 * M_PI * y) * cos(2 * M_PI * z);
+  b[c] = n0 * val * fac;
+}
+
+__global__ __launch_bounds__(256) void k_rhs_const(int64_t nc, double fac, double inv_nc, const double *__restrict__ nsum,
+                                                   double *__restrict__ b)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  b[c] = (nsum[0] * inv_nc) * fac;
+}
+
+template <typename T>
+void launch_rhs(chy_ctx *ctx, int rhs_type, const T *n, const T *gx, const T *gy, const T *gz, double *b)
+{
+  const chy_params &p = ctx->prm;
+  const double Kx = p.kappa_e[0], Ky = p.kappa_e[1], Kz = p.kappa_e[2];
+  const double detA = 1 + Kx * Kx + Ky * Ky + Kz * Kz;
+  const int64_t nc = ctx->g.nc;
+  if (rhs_type == CHY_RHS_FULL) {
+    RhsCoef rc;
+    rc.r[0][0] = 1 + Kx * Kx; rc.r[0][1] = Ky * Kx - Kz; rc.r[0][2] = Kz * Kx + Ky;
+    rc.r[1][0] = Kx * Ky + Kz; rc.r[1][1] = 1 + Ky * Ky; rc.r[1][2] = Kz * Ky - Kx;
+    rc.r[2][0] = Kx * Kz - Ky; rc.r[2][1] = Ky * Kz + Kx; rc.r[2][2] = 1 + Kz * Kz;
+    rc.detA = detA;
+    rc.scale = 1.0 / detA;
+    rc.cth = p.gamma_e * 1.3807e-23 * p.Te / 1.6022e-19;
+    rc.cG = p.me * p.nue / 1.6022e-19;
+    for (int a = 0; a < 3; a++) rc.E0[a] = p.E0[a];
+    rc.dx = p.d[0]; rc.dy = p.d[1]; rc.dz = p.d[2];
+    hipLaunchKernelGGL((k_rhs_full<T>), nblocks(nc), kBlock, 0, ctx->stream, ctx->g, rc, n, gx, gy, gz, b);
+  } else {
+    launch_sum<T>(ctx, n, nc, ctx->red_out);
+    const double fac = 2.0 * p.d[0] * p.d[1] * p.d[2] / detA;
+    if (rhs_type == CHY_RHS_SINUSOIDAL)
+      hipLaunchKernelGGL(k_rhs_sin, nblocks(nc), kBlock, 0, ctx->stream, ctx->g, p.d[0], p.d[1], p.d[2], fac,
+                         1.0 / (double)nc, ctx->red_out, b);
+    else
+      hipLaunchKernelGGL(k_rhs_const, nblocks(nc), kBlock, 0, ctx->stream, nc, fac, 1.0 / (double)nc,
+                         ctx->red_out, b);
+  }
+  if (rhs_type != CHY_RHS_CONSTANT) {
+    // null-space removal (rhs.c:294-296): b -= mean(b)
+    launch_sum<double>(ctx, b, nc, ctx->red_out);
+    launch_shift<double>(ctx, nc, b, ctx->red_out, -1.0 / (double)nc);
+  }
+}
+
+// ---- deterministic reductions ---------------------------------------------------
+constexpr int kMaxDots = 32;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// partial[blk*nv + k] = sum over this block's grid-stride range of A_k . b
+template <typename T, int NV>
+__global__ __launch_bounds__(256) void k_mdot(int nv, const T *__restrict__ A, int64_t stride, const T *__restrict__ b,
+                                              int64_t n, double *__restrict__ partial)
+{
+  __shared__ double sh[4][NV];
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) acc[k] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double bi = (double)b[i];
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+      if (k < nv) acc[k] += (double)A[k * stride + i] * bi;
+  }
+  const int wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    if (k < nv) {
+      const double s = wave_sum(acc[k]);
+      if ((threadIdx.x & 63) == 0) sh[wid][k] = s;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    const int k = threadIdx.x;
+    partial[(int64_t)blockIdx.x * nv + k] = (sh[0][k] + sh[1][k]) + (sh[2][k] + sh[3][k]);
+  }
+}
+
+// out[k] = sum_blk partial[blk*nv + k], fixed tree order.
+__global__ __launch_bounds__(256) void k_combine(const double *__restrict__ partial, int nblk, int nv,
+                                                 double *__restrict__ out)
+{
+  __shared__ double sh[4];
+  for (int k = 0; k < nv; k++) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += partial[(int64_t)b * nv + k];
+    const double s = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[k] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    __syncthreads();
+  }
+}
+
+void launch_combine_sum(chy_ctx *ctx, const double *partials, int nblk, int nv, double *out)
+{
+  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, partials, nblk, nv, out);
+}
+
+template <typename T>
+void launch_mdot(chy_ctx *ctx, int nv, const T *A, int64_t stride, const T *b, int64_t n, double *out_dev)
+{
+  for (int k0 = 0; k0 < nv; k0 += kMaxDots) {
+    const int m = (nv - k0 < kMaxDots) ? nv - k0 : kMaxDots;
+    const unsigned nb = (unsigned)std::min<int64_t>(kRedBlocks, (n + kBlock - 1) / kBlock);
+    if (m <= 1)
+      hipLaunchKernelGGL((k_mdot<T, 1>), nb, kBlock, 0, ctx->stream, m, A + k0 * stride, stride, b, n, ctx->red);
+    else if (m <= 8)
+      hipLaunchKernelGGL((k_mdot<T, 8>), nb, kBlock, 0, ctx->stream, m, A + k0 * stride, stride, b, n, ctx->red);
+    else
+      hipLaunchKernelGGL((k_mdot<T, kMaxDots>), nb, kBlock, 0, ctx->stream, m, A + k0 * stride, stride, b, n,
+                         ctx->red);
+    hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, m, out_dev + k0);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sum(const T *__restrict__ a, int64_t n, double *__restrict__ partial)
+{
+  __shared__ double sh[4];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc += (double)a[i];
+  const double s = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+template <typename T>
+void launch_sum(chy_ctx *ctx, const T *a, int64_t n, double *out_dev)
+{
+  const unsigned nb = (unsigned)std::min<int64_t>(kRedBlocks, (n + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((k_sum<T>), nb, kBlock, 0, ctx->stream, a, n, ctx->red);
+  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, 1, out_dev);
+}
+
+// w -= sum_k h_k V_k  (h on the device; accumulation in FP64)
+template <typename T, int NV>
+__global__ __launch_bounds__(256) void k_mgs(int nv, const T *__restrict__ V, int64_t stride,
+                                             const double *__restrict__ h, T *__restrict__ w, int64_t n)
+{
+  __shared__ double hs[NV];
+  if (threadIdx.x < nv) hs[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = (double)w[i];
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+      if (k < nv) acc -= hs[k] * (double)V[k * stride + i];
+    w[i] = (T)acc;
+  }
+}
+
+template <typename T>
+void launch_mgs_update(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n)
+{
+  for (int k0 = 0; k0 < nv; k0 += kMaxDots) {
+    const int m = (nv - k0 < kMaxDots) ? nv - k0 : kMaxDots;
+    const unsigned nb = (unsigned)std::min<int64_t>(4096, (n + kBlock - 1) / kBlock);
+    if (m <= 8)
+      hipLaunchKernelGGL((k_mgs<T, 8>), nb, kBlock, 0, ctx->stream, m, V + k0 * stride, stride, h + k0, w, n);
+    else
+      hipLaunchKernelGGL((k_mgs<T, kMaxDots>), nb, kBlock, 0, ctx->stream, m, V + k0 * stride, stride, h + k0, w, n);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_axpby(int64_t n, double a, const T *__restrict__ x, double b, T *__restrict__ y)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (T)(a * (double)x[i] + (b == 0.0 ? 0.0 : b * (double)y[i]));
+}
+
+template <typename T>
+void launch_axpby(chy_ctx *ctx, int64_t n, double a, const T *x, double b, T *y)
+{
+  const unsigned nb = (unsigned)std::min<int64_t>(8192, (n + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((k_axpby<T>), nb, kBlock, 0, ctx->stream, n, a, x, b, y);
+}
+
+template <typename TS, typename TD>
+__global__ __launch_bounds__(256) void k_convert(int64_t n, const TS *__restrict__ s, double scale, TD *__restrict__ d)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = (TD)((double)s[i] * scale);
+}
+
+template <typename TS, typename TD>
+void launch_convert(chy_ctx *ctx, int64_t n, const TS *src, double scale, TD *dst)
+{
+  const unsigned nb = (unsigned)std::min<int64_t>(8192, (n + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((k_convert<TS, TD>), nb, kBlock, 0, ctx->stream, n, src, scale, dst);
+}
+
+// v += scale * shift[0]
+template <typename T>
+__global__ __launch_bounds__(256) void k_shift(int64_t n, T *__restrict__ v, const double *__restrict__ shift, double scale)
+{
+  const double s = shift[0] * scale;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = (T)((double)v[i] + s);
+}
+
+template <typename T>
+void launch_shift(chy_ctx *ctx, int64_t n, T *v, const double *shift_dev, double scale)
+{
+  const unsigned nb = (unsigned)std::min<int64_t>(8192, (n + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((k_shift<T>), nb, kBlock, 0, ctx->stream, n, v, shift_dev, scale);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(int64_t n, T *__restrict__ v, T val)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = val;
+}
+
+template <typename T>
+void launch_fill(chy_ctx *ctx, int64_t n, T *v, T val)
+{
+  const unsigned nb = (unsigned)std::min<int64_t>(8192, (n + kBlock - 1) / kBlock);
+  if (nb) hipLaunchKernelGGL((k_fill<T>), nb, kBlock, 0, ctx->stream, n, v, val);
+}
+
+// explicit instantiations
+template void launch_lhs_apply<double, double, double>(chy_ctx *, int, bool, const double *, const double *, double *);
+template void launch_lhs_apply<float, float, float>(chy_ctx *, int, bool, const float *, const float *, float *);
+template void launch_lhs_apply<float, double, double>(chy_ctx *, int, bool, const float *, const double *, double *);
+template void launch_residual<double, double>(chy_ctx *, int, const double *, const double *, const double *, double *);
+template void launch_residual<float, double>(chy_ctx *, int, const float *, const double *, const double *, double *);
+template void launch_efield<double>(chy_ctx *, const double *, double *, double *, double *);
+template void launch_efield<float>(chy_ctx *, const double *, float *, float *, float *);
+template void launch_rhs<double>(chy_ctx *, int, const double *, const double *, const double *, const double *, double *);
+template void launch_rhs<float>(chy_ctx *, int, const float *, const float *, const float *, const float *, double *);
+template void launch_mdot<double>(chy_ctx *, int, const double *, int64_t, const double *, int64_t, double *);
+template void launch_mdot<float>(chy_ctx *, int, const float *, int64_t, const float *, int64_t, double *);
+template void launch_sum<double>(chy_ctx *, const double *, int64_t, double *);
+template void launch_sum<float>(chy_ctx *, const float *, int64_t, double *);
+template void launch_mgs_update<double>(chy_ctx *, int, const double *, int64_t, const double *, double *, int64_t);
+template void launch_mgs_update<float>(chy_ctx *, int, const float *, int64_t, const double *, float *, int64_t);
+template void launch_axpby<double>(chy_ctx *, int64_t, double, const double *, double, double *);
+template void launch_axpby<float>(chy_ctx *, int64_t, double, const float *, double, float *);
+template void launch_convert<double, float>(chy_ctx *, int64_t, const double *, double, float *);
+template void launch_convert<float, double>(chy_ctx *, int64_t, const float *, double, double *);
+template void launch_convert<double, double>(chy_ctx *, int64_t, const double *, double, double *);
+template void launch_convert<float, float>(chy_ctx *, int64_t, const float *, double, float *);
+template void launch_shift<double>(chy_ctx *, int64_t, double *, const double *, double);
+template void launch_shift<float>(chy_ctx *, int64_t, float *, const double *, double);
+template void launch_fill<double>(chy_ctx *, int64_t, double *, double);
+template void launch_fill<float>(chy_ctx *, int64_t, float *, float);
+template void launch_fill<uint32_t>(chy_ctx *, int64_t, uint32_t *, uint32_t);
+template void launch_fill<unsigned long long>(chy_ctx *, int64_t, unsigned long long *, unsigned long long);
+template void launch_jacobi<double>(chy_ctx *, int, const double *, const double *, double *);
+template void launch_jacobi<float>(chy_ctx *, int, const float *, const float *, float *);
+
+}  // namespace chy

@@ -1,0 +1,168 @@
+// chy_internal.h -- context layout and the launch wrappers shared between the
+// HIP translation units.  Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "chy_device.h"
+#include "chybrid.h"
+
+namespace chy {
+
+// Scaled operator coefficients s_ab = r_ab h_ab / detA (reference lhs.c:405-447).
+struct SCoef { double sxx, syy, szz, sxy, sxz, syz, syx, szx, szy; };
+
+// Forcing constants (reference rhs.c:199-223).
+struct RhsCoef {
+  double r[3][3];
+  double detA, scale, cth, cG, E0[3];
+  double dx, dy, dz;
+};
+
+// Device buffers of one particle generation (ping-pong pair in the context).
+template <typename T>
+struct PState {
+  // FP64: x,y,z absolute positions, vx,vy,vz.  Mixed: ox,oy,oz in-cell
+  // offsets in [0,1) with the integer cell in `cell`.
+  T *x = nullptr, *y = nullptr, *z = nullptr;
+  T *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  T *w = nullptr;          // optional weights
+  uint32_t *pid = nullptr;
+  int32_t *cell = nullptr; // cell key (both modes)
+};
+
+struct Timer {
+  hipEvent_t a = nullptr, b = nullptr;
+  double total_ms = 0.0;
+  int64_t count = 0;
+  bool pending = false;
+};
+
+}  // namespace chy
+
+struct chy_ctx {
+  chy_params prm;
+  chy::Grid g;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::string err;
+  bool mixed = false;
+
+  int64_t cap = 0;   // particle capacity
+  int64_t np = 0;    // live particles
+  bool weighted = false;
+  int64_t step = 0;  // collision stream step counter
+
+  // particle state (double or float pairs, one of them used)
+  chy::PState<double> pd[2];
+  chy::PState<float> pf[2];
+  int cur = 0;
+  uint32_t *perm = nullptr;     // slot -> old index
+  int32_t *slotkey = nullptr;   // slot -> cell
+  uint32_t *counts = nullptr;   // nc + 1
+  uint32_t *offsets = nullptr;  // nc + 1
+  uint32_t *cursor = nullptr;   // nc
+  uint32_t *scan_tmp = nullptr; // block sums for the scan
+  uint32_t *maxcount = nullptr; // largest cell occupancy of the last sort
+  void *partials = nullptr;     // per-cell corner partials [32][nc] (T)
+  double *sred = nullptr;       // collision-normaliser block partials
+  double *S_dev = nullptr;      // collision normaliser (device scalar)
+  unsigned long long *cstats = nullptr; // accepted, refused, capped, nonfinite
+
+  // grid fields
+  void *mom[4] = {nullptr, nullptr, nullptr, nullptr}; // n, Gx, Gy, Gz (T)
+  void *efield[3] = {nullptr, nullptr, nullptr};        // node E (T)
+  double *phi = nullptr, *b = nullptr, *r64 = nullptr, *t64 = nullptr;
+  bool phi_valid = false;
+  bool moments_valid = false;
+
+  // Krylov workspace (T = double in FP64 mode, float in mixed mode)
+  int64_t restart = 30;
+  void *V = nullptr, *Z = nullptr, *wv = nullptr;
+  double *red = nullptr;        // reduction partials [nblk][restart+2]
+  double *red_out = nullptr;    // reduced values [restart+2]
+  double *h_red = nullptr;      // pinned host mirror of red_out
+  // FFT preconditioner
+  void *fft_buf = nullptr;      // complex work array
+  double *tw[3] = {nullptr, nullptr, nullptr};   // twiddles per axis (complex)
+  double *trig[3] = {nullptr, nullptr, nullptr}; // cos/sin of 2 pi k / N per axis
+  bool fft_ok = false;
+
+  bool timing = false;
+  std::map<std::string, chy::Timer> timers;
+};
+
+namespace chy {
+
+constexpr int kRedBlocks = 1024;  // fixed reduction grid: deterministic sums
+constexpr int kBlock = 256;
+
+inline int set_err(chy_ctx *c, int code, const std::string &msg)
+{
+  if (c) c->err = msg;
+  return code;
+}
+
+#define CHY_HIP(call)                                                                         \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return chy::set_err(ctx, CHY_ERR_LIB, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// timers (no-ops unless ctx->timing)
+void timer_start(chy_ctx *ctx, const char *name);
+void timer_stop(chy_ctx *ctx, const char *name);
+
+// ---- grid kernels (chy_grid.hip) ----
+SCoef full_scoef(const double d[3], const double K[3]);
+void uniform_row(int lhs_type, const double d[3], const double K[3], double nbar, double v[19]);
+template <typename TN, typename TX, typename TY>
+void launch_lhs_apply(chy_ctx *ctx, int lhs_type, bool transpose, const TN *n, const TX *x, TY *y);
+template <typename TN, typename TX>
+void launch_residual(chy_ctx *ctx, int lhs_type, const TN *n, const double *b, const TX *x, double *r);
+template <typename T>
+void launch_efield(chy_ctx *ctx, const double *phi, T *ex, T *ey, T *ez);
+template <typename T>
+void launch_rhs(chy_ctx *ctx, int rhs_type, const T *n, const T *gx, const T *gy, const T *gz, double *b);
+// reductions: out[k] = sum_i a_k[i] * b[i] for k < nv (a_k = A + k*stride)
+template <typename T>
+void launch_mdot(chy_ctx *ctx, int nv, const T *A, int64_t stride, const T *b, int64_t n, double *out_dev);
+template <typename T>
+void launch_sum(chy_ctx *ctx, const T *a, int64_t n, double *out_dev);
+// y -= sum_k h_k V_k (h on device, FP64) ; optional normalisation
+template <typename T>
+void launch_mgs_update(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n);
+template <typename T>
+void launch_axpby(chy_ctx *ctx, int64_t n, double a, const T *x, double b, T *y);
+template <typename TS, typename TD>
+void launch_convert(chy_ctx *ctx, int64_t n, const TS *src, double scale, TD *dst);
+template <typename T>
+void launch_shift(chy_ctx *ctx, int64_t n, T *v, const double *shift_dev, double scale);
+template <typename T>
+void launch_fill(chy_ctx *ctx, int64_t n, T *v, T val);
+template <typename T>
+void launch_jacobi(chy_ctx *ctx, int lhs_type, const T *n, const T *r, T *z);
+void launch_combine_sum(chy_ctx *ctx, const double *partials, int nblk, int nv, double *out);
+
+// ---- particle kernels (chy_particles.hip) ----
+int particles_upload(chy_ctx *ctx, int64_t np, const double *pos, const double *vel, const double *w,
+                     const uint32_t *pid);
+int particles_download(chy_ctx *ctx, double *pos, double *vel, double *w, uint32_t *pid);
+int particles_init(chy_ctx *ctx);
+int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *st);
+int particles_move_only(chy_ctx *ctx);
+int particles_sort(chy_ctx *ctx);
+int particles_deposit(chy_ctx *ctx);
+int particles_normaliser(chy_ctx *ctx);
+
+// ---- FFT preconditioner (chy_fft.hip) ----
+int fft_setup(chy_ctx *ctx);
+template <typename T>
+int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z);
+
+}  // namespace chy

@@ -1,0 +1,923 @@
+// chy_particles.hip -- particle kernels for gfx950.
+//
+//  push     gather of E = E0 - grad(phi) (trilinear, the transpose of the
+//           deposit), Boris rotation, ion-neutral collisions and the periodic
+//           move, in one pass over cell-sorted particles (reference
+//           particles.c:347-718, vectors.c:12-162);
+//  sort     stable counting sort by cell: integer histogram (fused into the
+//           push), exclusive scan, wave-aggregated scatter, then a per-cell
+//           rank by source index that restores the stable order, so the
+//           result is deterministic although the scatter uses atomics
+//           (replaces DMSwarmMigrate, particles.c:714);
+//  deposit  CIC moments without floating-point atomics: per-cell corner
+//           partial sums in sorted order, then per-node sums over the 8
+//           adjacent cells in fixed corner order (particles.c:136-255).
+//
+// FP64 state: absolute x,y,z, v (f64), pid, optional weight, cell key.
+// Mixed state: int32 cell + FP32 in-cell offsets, FP32 v, pid, weight.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "chy_internal.h"
+
+namespace chy {
+
+template <typename T> void launch_fill(chy_ctx *ctx, int64_t n, T *v, T val);
+
+struct PushConst {
+  double tau, t[3], s[3];          // Boris constants (particles.c:374-393)
+  double dt, cdt[3];               // dt and dt/d per axis (mixed move)
+  double mi, mn, viT, vnT, vn0[3];
+  double Nc;
+  uint32_t k0, k1, step;
+};
+
+static PushConst push_const(const chy_ctx *ctx)
+{
+  const chy_params &p = ctx->prm;
+  PushConst c;
+  c.tau = 0.5 * (p.qi / p.mi) * p.dt;
+  for (int a = 0; a < 3; a++) c.t[a] = c.tau * p.B0[a];
+  const double tt = c.t[0] * c.t[0] + c.t[1] * c.t[1] + c.t[2] * c.t[2];
+  for (int a = 0; a < 3; a++) c.s[a] = 2.0 * c.t[a] / (1 + tt);
+  c.dt = p.dt;
+  for (int a = 0; a < 3; a++) c.cdt[a] = p.dt / p.d[a];
+  c.mi = p.mi; c.mn = p.mn; c.viT = p.viT_r; c.vnT = p.vnT;
+  for (int a = 0; a < 3; a++) c.vn0[a] = p.vn0[a];
+  c.Nc = 0.0;
+  c.k0 = (uint32_t)p.seed;
+  c.k1 = (uint32_t)(p.seed >> 32);
+  c.step = (uint32_t)ctx->step;
+  return c;
+}
+
+static int64_t collision_count(const chy_params &p, int64_t Np)
+{
+  // particles.c:510-519
+  const double fc = p.nui * p.dt;
+  const double dvx = p.vi0[0] - p.vn0[0], dvy = p.vi0[1] - p.vn0[1], dvz = p.vi0[2] - p.vn0[2];
+  const double vrm = 4.0 * p.viT_r + sqrt(dvx * dvx + dvy * dvy + dvz * dvz);
+  const double den = (p.vnT + p.vn0_r) * sqrt(p.mn / p.mi);
+  if (!(den > 0.0)) return Np;
+  const double x = ((double)Np * fc * vrm) / den;
+  if (!(x < (double)Np)) return Np;
+  if (!(x > 0.0)) return 0;
+  return (int64_t)x;
+}
+
+// ---- one ion-neutral scattering (reference particles.c:529-611) -----------------
+// Per-ion Bernoulli trial with probability Nc |v_r| / S, keyed random numbers.
+// Returns 1 accepted, 2 refused, 0 no collision; sets capped.
+template <typename T>
+__device__ __forceinline__ int collide_one(const PushConst &pc, double S, uint32_t pid, T &vx, T &vy, T &vz,
+                                           int &capped)
+{
+  const U4 a = philox4x32_10(U4{pid, pc.step, STREAM_COLLIDE_A, 0u}, pc.k0, pc.k1);
+  const T u0 = u24<T>(a.x), u1 = u24<T>(a.y), u2 = u24<T>(a.z), u3 = u24<T>(a.w);
+  const T twopi = (T)(2.0 * M_PI);
+  const T r1 = sqrt((T)-2.0 * log(u0)), a1 = twopi * u1;
+  const T r2 = sqrt((T)-2.0 * log(u2)), a2 = twopi * u3;
+  const T vnT = (T)pc.vnT;
+  const T vnx = (T)pc.vn0[0] + vnT * (r1 * cos(a1));
+  const T vny = (T)pc.vn0[1] + vnT * (r1 * sin(a1));
+  const T vnz = (T)pc.vn0[2] + vnT * (r2 * cos(a2));
+  const T vrx = vx - vnx, vry = vy - vny, vrz = vz - vnz;
+  const T vrr = sqrt(vrx * vrx + vry * vry + vrz * vrz);
+  T p = (S > 0.0) ? ((T)pc.Nc * vrr) / (T)S : (T)0;
+  capped = 0;
+  if (p > (T)1) { p = (T)1; capped = 1; }
+  const U4 b = philox4x32_10(U4{pid, pc.step, STREAM_COLLIDE_B, 0u}, pc.k0, pc.k1);
+  const T ud = u24<T>(b.x);
+  if (!(ud < p)) return 0;
+  const T mi = (T)pc.mi, mn = (T)pc.mn, M = mn + mi;
+  const T vcx = (vx * mi + vnx * mn) / M;
+  const T vcy = (vy * mi + vny * mn) / M;
+  const T vcz = (vz * mi + vnz * mn) / M;
+  const T ex = vx - vcx, ey = vy - vcy, ez = vz - vcz;
+  const T vcr = sqrt(ex * ex + ey * ey + ez * ez);
+  const T uz = (T)2.0 * u24<T>(b.y) - (T)1.0;
+  const T uperp = sqrt((T)1.0 - uz * uz);
+  const T uphi = twopi * u24<T>(b.z);
+  const T ux = uperp * cos(uphi), uy = uperp * sin(uphi);
+  const T costht = vrz / vrr;
+  const T sintht = sqrt((T)1.0 - costht * costht);
+  T cosphi = (T)1, sinphi = (T)0;
+  if (sintht > (T)1e-12) {
+    cosphi = vrx / (vrr * sintht);
+    sinphi = vry / (vrr * sintht);
+  }
+  const T ux1 = uz * sintht + ux * costht, uy1 = uy, uz1 = uz * costht - ux * sintht;
+  const T ux2 = ux1 * cosphi - uy1 * sinphi, uy2 = ux1 * sinphi + uy1 * cosphi, uz2 = uz1;
+  const T vfx = vcx + vcr * ux2, vfy = vcy + vcr * uy2, vfz = vcz + vcr * uz2;
+  const T vfr = sqrt(vfx * vfx + vfy * vfy + vfz * vfz);
+  if (vfr / (T)pc.viT > (T)10.0) return 2;
+  vx = vfx; vy = vfy; vz = vfz;
+  return 1;
+}
+
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return r;
+}
+
+// Wave-aggregated integer atomics on per-cell counters for runs of equal keys.
+__device__ __forceinline__ void count_key(uint32_t *counts, int key)
+{
+  int head, len;
+  key_runs(key, head, len);
+  if (key >= 0 && lane_id() == head) atomicAdd(&counts[key], (uint32_t)len);
+}
+
+__device__ __forceinline__ void count_stats(unsigned long long *cs, int r, int capped, bool bad)
+{
+  const uint64_t acc = __ballot(r == 1), ref = __ballot(r == 2), cap = __ballot(capped != 0), nf = __ballot(bad);
+  if (lane_id() == 0) {
+    if (acc) atomicAdd(&cs[0], (unsigned long long)__popcll(acc));
+    if (ref) atomicAdd(&cs[1], (unsigned long long)__popcll(ref));
+    if (cap) atomicAdd(&cs[2], (unsigned long long)__popcll(cap));
+    if (nf) atomicAdd(&cs[3], (unsigned long long)__popcll(nf));
+  }
+}
+
+// ---- FP64 push ------------------------------------------------------------------
+template <bool COLLIDE, bool MOVE, bool GATHER>
+__global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t np, PState<double> st,
+                                                   const double *__restrict__ ex, const double *__restrict__ ey,
+                                                   const double *__restrict__ ez, const double *__restrict__ Sdev,
+                                                   uint32_t *__restrict__ counts, double *__restrict__ sred,
+                                                   unsigned long long *__restrict__ cstats)
+{
+  __shared__ double sh[4];
+  const double S = COLLIDE ? Sdev[0] : 0.0;
+  double sacc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // every lane iterates the same number of times so wave-level collectives stay convergent
+  const int64_t iters = (np + stride - 1) / stride;
+  for (int64_t it = 0; it < iters; it++) {
+    const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = p < np;
+    int key = -1, r = 0, capped = 0;
+    bool bad = false;
+    if (live) {
+      double x = st.x[p], y = st.y[p], z = st.z[p];
+      double vx = st.vx[p], vy = st.vy[p], vz = st.vz[p];
+      if (GATHER) {
+        int i, j, k;
+        double fx, fy, fz, W[8];
+        cell_coord(x, g.p0[0], g.d[0], g.nx, i, fx);
+        cell_coord(y, g.p0[1], g.d[1], g.ny, j, fy);
+        cell_coord(z, g.p0[2], g.d[2], g.nz, k, fz);
+        corner_weights<double>(fx, fy, fz, W);
+        double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const int64_t c = lin(g, i + (q & 1), j + ((q >> 1) & 1), k + (q >> 2));
+          e0 += W[q] * ex[c];
+          e1 += W[q] * ey[c];
+          e2 += W[q] * ez[c];
+        }
+        const double tau = pc.tau, t0 = pc.t[0], t1 = pc.t[1], t2 = pc.t[2];
+        const double s0 = pc.s[0], s1 = pc.s[1], s2 = pc.s[2];
+        const double vm0 = vx + tau * e0, vm1 = vy + tau * e1, vm2 = vz + tau * e2;
+        const double vp0 = vm0 + (vm1 * t2 - vm2 * t1);
+        const double vp1 = vm1 + (vm2 * t0 - vm0 * t2);
+        const double vp2 = vm2 + (vm0 * t1 - vm1 * t0);
+        const double vq0 = vm0 + (vp1 * s2 - vp2 * s1);
+        const double vq1 = vm1 + (vp2 * s0 - vp0 * s2);
+        const double vq2 = vm2 + (vp0 * s1 - vp1 * s0);
+        vx = vq0 + tau * e0;
+        vy = vq1 + tau * e1;
+        vz = vq2 + tau * e2;
+        if (COLLIDE) r = collide_one<double>(pc, S, st.pid[p], vx, vy, vz, capped);
+        st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
+      }
+      if (MOVE) {
+        x = wrap_pos(x + vx * pc.dt, g.p0[0], g.p1[0], g.L[0]);
+        y = wrap_pos(y + vy * pc.dt, g.p0[1], g.p1[1], g.L[1]);
+        z = wrap_pos(z + vz * pc.dt, g.p0[2], g.p1[2], g.L[2]);
+        st.x[p] = x; st.y[p] = y; st.z[p] = z;
+        int i, j, k;
+        double f;
+        cell_coord(x, g.p0[0], g.d[0], g.nx, i, f);
+        cell_coord(y, g.p0[1], g.d[1], g.ny, j, f);
+        cell_coord(z, g.p0[2], g.d[2], g.nz, k, f);
+        key = (int)(i + (int64_t)g.nx * (j + (int64_t)g.ny * k));
+        st.cell[p] = key;
+      }
+      bad = !isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(x) || !isfinite(y) || !isfinite(z);
+      if (COLLIDE) sacc += mean_rel_speed(vx, vy, vz, pc.vn0[0], pc.vn0[1], pc.vn0[2], pc.vnT);
+    }
+    if (MOVE) count_key(counts, key);
+    count_stats(cstats, r, capped, bad);
+  }
+  if (COLLIDE) {
+    const double s = block_sum(sacc, sh);
+    if (threadIdx.x == 0) sred[blockIdx.x] = s;
+  }
+}
+
+// ---- mixed push (FP32 state: cell + offset, FP32 v) --------------------------------
+template <bool COLLIDE, bool MOVE, bool GATHER>
+__global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConst pc, int64_t np, PState<float> st,
+                                                    const float *__restrict__ ex, const float *__restrict__ ey,
+                                                    const float *__restrict__ ez, const double *__restrict__ Sdev,
+                                                    uint32_t *__restrict__ counts, double *__restrict__ sred,
+                                                    unsigned long long *__restrict__ cstats)
+{
+  __shared__ double sh[4];
+  const double S = COLLIDE ? Sdev[0] : 0.0;
+  double sacc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t iters = (np + stride - 1) / stride;
+  const float tau = (float)pc.tau, t0 = (float)pc.t[0], t1 = (float)pc.t[1], t2 = (float)pc.t[2];
+  const float s0 = (float)pc.s[0], s1 = (float)pc.s[1], s2 = (float)pc.s[2];
+  for (int64_t it = 0; it < iters; it++) {
+    const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = p < np;
+    int key = -1, r = 0, capped = 0;
+    bool bad = false;
+    if (live) {
+      const int c = st.cell[p];
+      int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny);
+      float fx = st.x[p], fy = st.y[p], fz = st.z[p];
+      float vx = st.vx[p], vy = st.vy[p], vz = st.vz[p];
+      if (GATHER) {
+        float W[8];
+        corner_weights<float>(fx, fy, fz, W);
+        float e0 = 0.f, e1 = 0.f, e2 = 0.f;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const int64_t cc = lin(g, i + (q & 1), j + ((q >> 1) & 1), k + (q >> 2));
+          e0 = fmaf(W[q], ex[cc], e0);
+          e1 = fmaf(W[q], ey[cc], e1);
+          e2 = fmaf(W[q], ez[cc], e2);
+        }
+        const float vm0 = fmaf(tau, e0, vx), vm1 = fmaf(tau, e1, vy), vm2 = fmaf(tau, e2, vz);
+        const float vp0 = vm0 + (vm1 * t2 - vm2 * t1);
+        const float vp1 = vm1 + (vm2 * t0 - vm0 * t2);
+        const float vp2 = vm2 + (vm0 * t1 - vm1 * t0);
+        vx = fmaf(tau, e0, vm0 + (vp1 * s2 - vp2 * s1));
+        vy = fmaf(tau, e1, vm1 + (vp2 * s0 - vp0 * s2));
+        vz = fmaf(tau, e2, vm2 + (vp0 * s1 - vp1 * s0));
+        if (COLLIDE) r = collide_one<float>(pc, S, st.pid[p], vx, vy, vz, capped);
+        st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
+      }
+      if (MOVE) {
+        // offset += v dt / d with an exact integer carry into the cell index
+        float o[3] = {fmaf(vx, (float)pc.cdt[0], fx), fmaf(vy, (float)pc.cdt[1], fy), fmaf(vz, (float)pc.cdt[2], fz)};
+        int ci[3] = {i, j, k};
+        const int nn[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          float fl = floorf(o[a]);
+          float f = o[a] - fl;
+          int sh_ = (int)fl;
+          if (f >= 1.0f) { f -= 1.0f; sh_ += 1; }
+          int cc = (ci[a] + sh_) % nn[a];
+          if (cc < 0) cc += nn[a];
+          ci[a] = cc;
+          o[a] = f;
+        }
+        st.x[p] = o[0]; st.y[p] = o[1]; st.z[p] = o[2];
+        key = (int)(ci[0] + (int64_t)g.nx * (ci[1] + (int64_t)g.ny * ci[2]));
+        st.cell[p] = key;
+        fx = o[0]; fy = o[1]; fz = o[2];
+      }
+      bad = !isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(fx) || !isfinite(fy) || !isfinite(fz);
+      if (COLLIDE) sacc += mean_rel_speed(vx, vy, vz, pc.vn0[0], pc.vn0[1], pc.vn0[2], pc.vnT);
+    }
+    if (MOVE) count_key(counts, key);
+    count_stats(cstats, r, capped, bad);
+  }
+  if (COLLIDE) {
+    const double s = block_sum(sacc, sh);
+    if (threadIdx.x == 0) sred[blockIdx.x] = s;
+  }
+}
+
+// Collision normaliser partials over the current velocities.
+template <typename T>
+__global__ __launch_bounds__(256) void k_meanrel(int64_t np, const T *__restrict__ vx, const T *__restrict__ vy,
+                                                 const T *__restrict__ vz, double n0x, double n0y, double n0z,
+                                                 double vnT, double *__restrict__ sred)
+{
+  __shared__ double sh[4];
+  double acc = 0.0;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x)
+    acc += mean_rel_speed((double)vx[p], (double)vy[p], (double)vz[p], n0x, n0y, n0z, vnT);
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) sred[blockIdx.x] = s;
+}
+
+// FP64 keys and histogram from absolute positions.
+__global__ __launch_bounds__(256) void k_keys_fp64(Grid g, int64_t np, PState<double> st, uint32_t *__restrict__ counts)
+{
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t iters = (np + stride - 1) / stride;
+  for (int64_t it = 0; it < iters; it++) {
+    const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int key = -1;
+    if (p < np) {
+      int i, j, k;
+      double f;
+      cell_coord(st.x[p], g.p0[0], g.d[0], g.nx, i, f);
+      cell_coord(st.y[p], g.p0[1], g.d[1], g.ny, j, f);
+      cell_coord(st.z[p], g.p0[2], g.d[2], g.nz, k, f);
+      key = (int)(i + (int64_t)g.nx * (j + (int64_t)g.ny * k));
+      st.cell[p] = key;
+    }
+    count_key(counts, key);
+  }
+}
+
+// Histogram from stored keys (mixed state).
+__global__ __launch_bounds__(256) void k_keys_count(int64_t np, const int32_t *__restrict__ cell, uint32_t *__restrict__ counts)
+{
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t iters = (np + stride - 1) / stride;
+  for (int64_t it = 0; it < iters; it++) {
+    const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    count_key(counts, p < np ? cell[p] : -1);
+  }
+}
+
+// ---- exclusive scan of the cell histogram ------------------------------------------
+constexpr int kScanTile = 4096;  // 256 threads x 16
+constexpr uint32_t kMaxCellOccupancy = 8192;
+
+__global__ __launch_bounds__(256) void k_tile_sum(int64_t n, const uint32_t *__restrict__ in, uint32_t *__restrict__ tsum,
+                                                  uint32_t *__restrict__ maxcount)
+{
+  __shared__ uint32_t sh[4];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  uint32_t acc = 0, mx = 0;
+  for (int q = threadIdx.x; q < kScanTile; q += blockDim.x) {
+    const int64_t i = base + q;
+    if (i < n) { acc += in[i]; mx = in[i] > mx ? in[i] : mx; }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) { const uint32_t o = __shfl_down(mx, off, 64); mx = o > mx ? o : mx; }
+  if ((threadIdx.x & 63) == 0 && mx) atomicMax(maxcount, mx);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) tsum[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// Block-wide exclusive scan of 256 values (one per thread); returns the total.
+__device__ __forceinline__ uint32_t block_exscan(uint32_t v, uint32_t &total, uint32_t *sh)
+{
+  const int l = lane_id(), w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (l >= off) x += y;
+  }
+  if (l == 63) sh[w] = x;
+  __syncthreads();
+  uint32_t wbase = 0;
+  for (int q = 0; q < w; q++) wbase += sh[q];
+  total = sh[0] + sh[1] + sh[2] + sh[3];
+  __syncthreads();
+  return wbase + x - v;
+}
+
+// Single block: exclusive scan of the tile sums in place.
+__global__ __launch_bounds__(256) void k_scan_tiles(int64_t nt, uint32_t *__restrict__ tsum)
+{
+  __shared__ uint32_t sh[4];
+  uint32_t carry = 0;
+  for (int64_t b0 = 0; b0 < nt; b0 += 256) {
+    const int64_t i = b0 + threadIdx.x;
+    const uint32_t v = i < nt ? tsum[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exscan(v, tot, sh);
+    if (i < nt) tsum[i] = carry + ex;
+    carry += tot;
+  }
+}
+
+// Per-tile scan; writes offsets[0..n) and the cursor copy; offsets[n] = total.
+__global__ __launch_bounds__(256) void k_tile_scan(int64_t n, const uint32_t *__restrict__ in,
+                                                   const uint32_t *__restrict__ tsum, uint32_t *__restrict__ offs,
+                                                   uint32_t *__restrict__ cursor, uint32_t total)
+{
+  __shared__ uint32_t sh[4];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  uint32_t carry = tsum[blockIdx.x];
+  for (int q0 = 0; q0 < kScanTile; q0 += 256) {
+    const int64_t i = base + q0 + threadIdx.x;
+    const uint32_t v = i < n ? in[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exscan(v, tot, sh);
+    if (i < n) { offs[i] = carry + ex; cursor[i] = carry + ex; }
+    carry += tot;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) offs[n] = total;
+}
+
+// Scatter: slot = cursor[key]++ (run-aggregated), perm[slot] = p.
+__global__ __launch_bounds__(256) void k_scatter(int64_t np, const int32_t *__restrict__ cell, uint32_t *__restrict__ cursor,
+                                                 uint32_t *__restrict__ perm, int32_t *__restrict__ slotkey)
+{
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t iters = (np + stride - 1) / stride;
+  for (int64_t it = 0; it < iters; it++) {
+    const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int key = p < np ? cell[p] : -1;
+    int head, len;
+    key_runs(key, head, len);
+    uint32_t base = 0;
+    if (key >= 0 && lane_id() == head) base = atomicAdd(&cursor[key], (uint32_t)len);
+    base = __shfl(base, head, 64);
+    if (key >= 0) {
+      const uint32_t slot = base + (uint32_t)(lane_id() - head);
+      perm[slot] = (uint32_t)p;
+      slotkey[slot] = key;
+    }
+  }
+}
+
+// Stable order within each cell: rank of a slot's source index among its
+// cell's sources; moves the particle from generation `a` to `b`.
+template <typename T>
+__global__ __launch_bounds__(256) void k_reorder(int64_t np, const uint32_t *__restrict__ perm,
+                                                 const int32_t *__restrict__ slotkey, const uint32_t *__restrict__ offs,
+                                                 PState<T> a, PState<T> b, bool weighted)
+{
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= np) return;
+  const int c = slotkey[s];
+  const uint32_t lo = offs[c], hi = offs[c + 1];
+  const uint32_t me = perm[s];
+  uint32_t rank = 0;
+  for (uint32_t t = lo; t < hi; t++) rank += perm[t] < me ? 1u : 0u;
+  const uint32_t d = lo + rank;
+  b.x[d] = a.x[me]; b.y[d] = a.y[me]; b.z[d] = a.z[me];
+  b.vx[d] = a.vx[me]; b.vy[d] = a.vy[me]; b.vz[d] = a.vz[me];
+  b.pid[d] = a.pid[me];
+  b.cell[d] = c;
+  if (weighted) b.w[d] = a.w[me];
+}
+
+// ---- deposit ---------------------------------------------------------------------
+// Per-cell corner partials P[(4q+m)*nc + c], m in {w, w vx, w vy, w vz}.
+template <typename T, bool ABS>
+__global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *__restrict__ offs, PState<T> st,
+                                                       bool weighted, T *__restrict__ P)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  T acc[32];
+#pragma unroll
+  for (int q = 0; q < 32; q++) acc[q] = (T)0;
+  const uint32_t lo = offs[c], hi = offs[c + 1];
+  for (uint32_t s = lo; s < hi; s++) {
+    T fx, fy, fz;
+    if (ABS) {
+      int i;
+      double f;
+      cell_coord((double)st.x[s], g.p0[0], g.d[0], g.nx, i, f); fx = (T)f;
+      cell_coord((double)st.y[s], g.p0[1], g.d[1], g.ny, i, f); fy = (T)f;
+      cell_coord((double)st.z[s], g.p0[2], g.d[2], g.nz, i, f); fz = (T)f;
+    } else {
+      fx = st.x[s]; fy = st.y[s]; fz = st.z[s];
+    }
+    T W[8];
+    corner_weights<T>(fx, fy, fz, W);
+    const T vx = st.vx[s], vy = st.vy[s], vz = st.vz[s];
+    const T wt = weighted ? st.w[s] : (T)1;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const T a = weighted ? wt * W[q] : W[q];
+      acc[4 * q + 0] += a;
+      acc[4 * q + 1] += a * vx;
+      acc[4 * q + 2] += a * vy;
+      acc[4 * q + 3] += a * vz;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 32; q++) P[(int64_t)q * g.nc + c] = acc[q];
+}
+
+// Node moments: sum over the 8 cells sharing the node, corner order q = 0..7.
+template <typename T>
+__global__ __launch_bounds__(256) void k_node_sum(Grid g, const T *__restrict__ P, T *__restrict__ n, T *__restrict__ gx,
+                                                  T *__restrict__ gy, T *__restrict__ gz)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  T m[4] = {(T)0, (T)0, (T)0, (T)0};
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int64_t s = lin(g, i - (q & 1), j - ((q >> 1) & 1), k - (q >> 2));
+#pragma unroll
+    for (int a = 0; a < 4; a++) m[a] += P[(int64_t)(4 * q + a) * g.nc + s];
+  }
+  n[c] = m[0]; gx[c] = m[1]; gy[c] = m[2]; gz[c] = m[3];
+}
+
+// ---- device initialisation (seeded; mirrors the oracle's placement) ----------------
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_init(Grid g, int64_t ppc, uint32_t k0, uint32_t k1, double v0x, double v0y,
+                                              double v0z, double vTx, double vTy, double vTz, int pert, double amp,
+                                              PState<double> sd, PState<float> sf)
+{
+  const int64_t np = g.nc * ppc;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int64_t c = p / ppc;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  const U4 a = philox4x32_10(U4{(uint32_t)p, 0u, STREAM_INIT, 0u}, k0, k1);
+  const U4 b = philox4x32_10(U4{(uint32_t)p, 0u, STREAM_INIT, 1u}, k0, k1);
+  const double u0 = u24<double>(a.x), u1 = u24<double>(a.y), u2 = u24<double>(a.z);
+  const double r1 = sqrt(-2.0 * log(u24<double>(b.x))), a1 = 2.0 * M_PI * u24<double>(b.y);
+  const double r2 = sqrt(-2.0 * log(u24<double>(b.z))), a2 = 2.0 * M_PI * u24<double>(b.w);
+  const double vx = v0x + vTx * (r1 * cos(a1));
+  const double vy = v0y + vTy * (r1 * sin(a1));
+  const double vz = v0z + vTz * (r2 * cos(a2));
+  double wgt = 1.0;
+  if (pert == CHY_PERT_UNIFORM || pert == CHY_PERT_GAUSSIAN) {
+    const U4 wv = philox4x32_10(U4{(uint32_t)c, 0u, STREAM_WEIGHT, 0u}, k0, k1);
+    const double w0 = u24<double>(wv.x), w1 = u24<double>(wv.y);
+    wgt = (pert == CHY_PERT_UNIFORM) ? 1.0 + amp * (2.0 * w0 - 1.0)
+                                     : 1.0 + amp * (sqrt(-2.0 * log(w0)) * cos(2.0 * M_PI * w1));
+  }
+  if (!MIXED) {
+    // L = d * N exactly as the oracle's init computes it
+    const double Lx = g.d[0] * (double)g.nx, Ly = g.d[1] * (double)g.ny, Lz = g.d[2] * (double)g.nz;
+    const double x = wrap_pos(g.p0[0] + ((double)i + u0) * g.d[0], g.p0[0], g.p0[0] + Lx, Lx);
+    const double y = wrap_pos(g.p0[1] + ((double)j + u1) * g.d[1], g.p0[1], g.p0[1] + Ly, Ly);
+    const double z = wrap_pos(g.p0[2] + ((double)k + u2) * g.d[2], g.p0[2], g.p0[2] + Lz, Lz);
+    sd.x[p] = x; sd.y[p] = y; sd.z[p] = z;
+    sd.vx[p] = vx; sd.vy[p] = vy; sd.vz[p] = vz;
+    sd.pid[p] = (uint32_t)p;
+    if (pert == CHY_PERT_SINUSOIDAL_X) wgt = 1.0 + amp * sin(2.0 * M_PI * (x - g.p0[0]) / Lx);
+    if (sd.w) sd.w[p] = wgt;
+  } else {
+    sf.x[p] = (float)u0; sf.y[p] = (float)u1; sf.z[p] = (float)u2;  // exact: 24-bit uniforms
+    sf.cell[p] = (int32_t)c;
+    sf.vx[p] = (float)vx; sf.vy[p] = (float)vy; sf.vz[p] = (float)vz;
+    sf.pid[p] = (uint32_t)p;
+    if (pert == CHY_PERT_SINUSOIDAL_X) wgt = 1.0 + amp * sin(2.0 * M_PI * ((double)i + u0) / (double)g.nx);
+    if (sf.w) sf.w[p] = (float)wgt;
+  }
+}
+
+// ---- host orchestration ------------------------------------------------------------
+static unsigned grid_for(int64_t n, unsigned cap = 4096)
+{
+  const int64_t b = (n + kBlock - 1) / kBlock;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, cap));
+}
+
+int particles_sort(chy_ctx *ctx)
+{
+  // counts[] was filled by the key pass (push or keys kernel)
+  const int64_t nc = ctx->g.nc, np = ctx->np;
+  const int64_t nt = (nc + kScanTile - 1) / kScanTile;
+  launch_fill<uint32_t>(ctx, 1, ctx->maxcount, 0u);
+  hipLaunchKernelGGL(k_tile_sum, (unsigned)nt, kBlock, 0, ctx->stream, nc, ctx->counts, ctx->scan_tmp, ctx->maxcount);
+  hipLaunchKernelGGL(k_scan_tiles, 1, kBlock, 0, ctx->stream, nt, ctx->scan_tmp);
+  hipLaunchKernelGGL(k_tile_scan, (unsigned)nt, kBlock, 0, ctx->stream, nc, ctx->counts, ctx->scan_tmp, ctx->offsets,
+                     ctx->cursor, (uint32_t)np);
+  const int32_t *cell = ctx->mixed ? ctx->pf[ctx->cur].cell : ctx->pd[ctx->cur].cell;
+  hipLaunchKernelGGL(k_scatter, grid_for(np), kBlock, 0, ctx->stream, np, cell, ctx->cursor, ctx->perm, ctx->slotkey);
+  // the stable per-cell rank is quadratic in the cell occupancy: bound it
+  uint32_t mx = 0;
+  CHY_HIP(hipMemcpyAsync(&mx, ctx->maxcount, sizeof(mx), hipMemcpyDeviceToHost, ctx->stream));
+  CHY_HIP(hipStreamSynchronize(ctx->stream));
+  if (mx > kMaxCellOccupancy)
+    return set_err(ctx, CHY_ERR_SUP, "a cell holds " + std::to_string(mx) + " particles; at most " +
+                                         std::to_string(kMaxCellOccupancy) + " are supported");
+  const unsigned nb = (unsigned)((np + kBlock - 1) / kBlock);
+  if (nb) {
+    if (ctx->mixed)
+      hipLaunchKernelGGL((k_reorder<float>), nb, kBlock, 0, ctx->stream, np, ctx->perm, ctx->slotkey, ctx->offsets,
+                         ctx->pf[ctx->cur], ctx->pf[1 - ctx->cur], ctx->weighted);
+    else
+      hipLaunchKernelGGL((k_reorder<double>), nb, kBlock, 0, ctx->stream, np, ctx->perm, ctx->slotkey, ctx->offsets,
+                         ctx->pd[ctx->cur], ctx->pd[1 - ctx->cur], ctx->weighted);
+  }
+  ctx->cur = 1 - ctx->cur;
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("sort: ") + hipGetErrorString(e));
+  return CHY_OK;
+}
+
+int particles_deposit(chy_ctx *ctx)
+{
+  const unsigned nb = (unsigned)((ctx->g.nc + kBlock - 1) / kBlock);
+  if (ctx->mixed) {
+    float *P = (float *)ctx->partials;
+    hipLaunchKernelGGL((k_cell_partials<float, false>), nb, kBlock, 0, ctx->stream, ctx->g, ctx->offsets,
+                       ctx->pf[ctx->cur], ctx->weighted, P);
+    hipLaunchKernelGGL((k_node_sum<float>), nb, kBlock, 0, ctx->stream, ctx->g, P, (float *)ctx->mom[0],
+                       (float *)ctx->mom[1], (float *)ctx->mom[2], (float *)ctx->mom[3]);
+  } else {
+    double *P = (double *)ctx->partials;
+    hipLaunchKernelGGL((k_cell_partials<double, true>), nb, kBlock, 0, ctx->stream, ctx->g, ctx->offsets,
+                       ctx->pd[ctx->cur], ctx->weighted, P);
+    hipLaunchKernelGGL((k_node_sum<double>), nb, kBlock, 0, ctx->stream, ctx->g, P, (double *)ctx->mom[0],
+                       (double *)ctx->mom[1], (double *)ctx->mom[2], (double *)ctx->mom[3]);
+  }
+  ctx->moments_valid = true;
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("deposit: ") + hipGetErrorString(e));
+  return CHY_OK;
+}
+
+int particles_normaliser(chy_ctx *ctx)
+{
+  const chy_params &p = ctx->prm;
+  const unsigned nb = grid_for(ctx->np);
+  if (ctx->mixed) {
+    const PState<float> &s = ctx->pf[ctx->cur];
+    hipLaunchKernelGGL((k_meanrel<float>), nb, kBlock, 0, ctx->stream, ctx->np, s.vx, s.vy, s.vz, p.vn0[0], p.vn0[1],
+                       p.vn0[2], p.vnT, ctx->sred);
+  } else {
+    const PState<double> &s = ctx->pd[ctx->cur];
+    hipLaunchKernelGGL((k_meanrel<double>), nb, kBlock, 0, ctx->stream, ctx->np, s.vx, s.vy, s.vz, p.vn0[0],
+                       p.vn0[1], p.vn0[2], p.vnT, ctx->sred);
+  }
+  launch_combine_sum(ctx, ctx->sred, (int)nb, 1, ctx->S_dev);
+  return CHY_OK;
+}
+
+// Keys + histogram + sort + normaliser after new particle data.
+static int settle(chy_ctx *ctx)
+{
+  launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
+  if (ctx->mixed)
+    hipLaunchKernelGGL(k_keys_count, grid_for(ctx->np), kBlock, 0, ctx->stream, ctx->np, ctx->pf[ctx->cur].cell,
+                       ctx->counts);
+  else
+    hipLaunchKernelGGL(k_keys_fp64, grid_for(ctx->np), kBlock, 0, ctx->stream, ctx->g, ctx->np, ctx->pd[ctx->cur],
+                       ctx->counts);
+  int rc = particles_sort(ctx);
+  if (rc) return rc;
+  return particles_normaliser(ctx);
+}
+
+// The device wrap rule on the host (upload of positions from the caller).
+static double host_wrap(double x, double p0, double p1, double L)
+{
+  if (!(x >= p0 - L && x < p1 + L)) {
+    if (!std::isfinite(x)) return x;
+    x = x - std::floor((x - p0) / L) * L;
+  }
+  if (x < p0) { x += L; if (x >= p1) x = p0; }
+  else if (x >= p1) { x -= L; if (x < p0) x = p0; }
+  return x;
+}
+
+int particles_upload(chy_ctx *ctx, int64_t np, const double *pos, const double *vel, const double *w,
+                     const uint32_t *pid)
+{
+  if (np < 0 || np > ctx->cap) return set_err(ctx, CHY_ERR_ARG_RANGE, "particle count exceeds the context capacity");
+  if (w && !ctx->weighted) return set_err(ctx, CHY_ERR_ARG_INCOMP, "weights given but the context has no weight buffers");
+  ctx->np = np;
+  if (np == 0) return settle(ctx);
+  const chy::Grid &g = ctx->g;
+  std::vector<uint32_t> hp(np);
+  for (int64_t i = 0; i < np; i++) hp[i] = pid ? pid[i] : (uint32_t)i;
+  if (!ctx->mixed) {
+    std::vector<double> a(np);
+    PState<double> &s = ctx->pd[ctx->cur];
+    double *dst[6] = {s.x, s.y, s.z, s.vx, s.vy, s.vz};
+    for (int f = 0; f < 6; f++) {
+      const double *src = f < 3 ? pos : vel;
+      for (int64_t i = 0; i < np; i++) a[i] = src[4 * i + (f % 3)];
+      if (f < 3) {
+        // wrap into [p0, p1) as the move does
+        for (int64_t i = 0; i < np; i++) {
+          a[i] = host_wrap(a[i], g.p0[f], g.p1[f], g.L[f]);
+        }
+      }
+      CHY_HIP(hipMemcpyAsync(dst[f], a.data(), sizeof(double) * np, hipMemcpyHostToDevice, ctx->stream));
+      CHY_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    if (w) CHY_HIP(hipMemcpyAsync(s.w, w, sizeof(double) * np, hipMemcpyHostToDevice, ctx->stream));
+    else if (ctx->weighted) launch_fill<double>(ctx, np, s.w, 1.0);
+    CHY_HIP(hipMemcpyAsync(s.pid, hp.data(), sizeof(uint32_t) * np, hipMemcpyHostToDevice, ctx->stream));
+    CHY_HIP(hipStreamSynchronize(ctx->stream));
+  } else {
+    PState<float> &s = ctx->pf[ctx->cur];
+    std::vector<float> o[3], v[3];
+    std::vector<int32_t> cell(np);
+    for (int a = 0; a < 3; a++) { o[a].resize(np); v[a].resize(np); }
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    for (int64_t i = 0; i < np; i++) {
+      int ci[3];
+      for (int a = 0; a < 3; a++) {
+        const double p0 = g.p0[a];
+        const double x = host_wrap(pos[4 * i + a], p0, g.p1[a], g.L[a]);
+        const double xi = (x - p0) / g.d[a];
+        int ii = (int)xi;
+        if (ii > nn[a] - 1) ii = nn[a] - 1;
+        float f = (float)(xi - (double)ii);
+        if (f >= 1.0f) { f = 0.0f; ii = (ii + 1) % nn[a]; }
+        ci[a] = ii;
+        o[a][i] = f;
+        v[a][i] = (float)vel[4 * i + a];
+      }
+      cell[i] = ci[0] + g.nx * (ci[1] + g.ny * ci[2]);
+    }
+    float *dst[6] = {s.x, s.y, s.z, s.vx, s.vy, s.vz};
+    for (int a = 0; a < 3; a++) {
+      CHY_HIP(hipMemcpyAsync(dst[a], o[a].data(), sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+      CHY_HIP(hipMemcpyAsync(dst[3 + a], v[a].data(), sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+    }
+    CHY_HIP(hipMemcpyAsync(s.cell, cell.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, ctx->stream));
+    CHY_HIP(hipMemcpyAsync(s.pid, hp.data(), sizeof(uint32_t) * np, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<float> wf;
+    if (w) {
+      wf.resize(np);
+      for (int64_t i = 0; i < np; i++) wf[i] = (float)w[i];
+      CHY_HIP(hipMemcpyAsync(s.w, wf.data(), sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+    } else if (ctx->weighted) {
+      launch_fill<float>(ctx, np, s.w, 1.0f);
+    }
+    CHY_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return settle(ctx);
+}
+
+int particles_download(chy_ctx *ctx, double *pos, double *vel, double *w, uint32_t *pid)
+{
+  const int64_t np = ctx->np;
+  if (np == 0) return CHY_OK;
+  const chy::Grid &g = ctx->g;
+  if (!ctx->mixed) {
+    const PState<double> &s = ctx->pd[ctx->cur];
+    std::vector<double> a[6];
+    const double *src[6] = {s.x, s.y, s.z, s.vx, s.vy, s.vz};
+    for (int f = 0; f < 6; f++) {
+      a[f].resize(np);
+      CHY_HIP(hipMemcpyAsync(a[f].data(), src[f], sizeof(double) * np, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    CHY_HIP(hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < np; i++) {
+      if (pos) {
+        for (int q = 0; q < 3; q++) pos[4 * i + q] = a[q][i];
+        pos[4 * i + 3] = sqrt(a[0][i] * a[0][i] + a[1][i] * a[1][i] + a[2][i] * a[2][i]);
+      }
+      if (vel) {
+        for (int q = 0; q < 3; q++) vel[4 * i + q] = a[3 + q][i];
+        vel[4 * i + 3] = sqrt(a[3][i] * a[3][i] + a[4][i] * a[4][i] + a[5][i] * a[5][i]);
+      }
+    }
+    if (w) {
+      if (ctx->weighted) CHY_HIP(hipMemcpy(w, s.w, sizeof(double) * np, hipMemcpyDeviceToHost));
+      else for (int64_t i = 0; i < np; i++) w[i] = 1.0;
+    }
+    if (pid) CHY_HIP(hipMemcpy(pid, s.pid, sizeof(uint32_t) * np, hipMemcpyDeviceToHost));
+  } else {
+    const PState<float> &s = ctx->pf[ctx->cur];
+    std::vector<float> a[6];
+    std::vector<int32_t> cell(np);
+    const float *src[6] = {s.x, s.y, s.z, s.vx, s.vy, s.vz};
+    for (int f = 0; f < 6; f++) {
+      a[f].resize(np);
+      CHY_HIP(hipMemcpyAsync(a[f].data(), src[f], sizeof(float) * np, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    CHY_HIP(hipMemcpyAsync(cell.data(), s.cell, sizeof(int32_t) * np, hipMemcpyDeviceToHost, ctx->stream));
+    CHY_HIP(hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < np; i++) {
+      const int c = cell[i];
+      const int ci[3] = {c % g.nx, (c / g.nx) % g.ny, c / (g.nx * g.ny)};
+      double r2 = 0.0, v2 = 0.0;
+      for (int q = 0; q < 3; q++) {
+        const double x = g.p0[q] + ((double)ci[q] + (double)a[q][i]) * g.d[q];
+        if (pos) pos[4 * i + q] = x;
+        if (vel) vel[4 * i + q] = (double)a[3 + q][i];
+        r2 += x * x;
+        v2 += (double)a[3 + q][i] * (double)a[3 + q][i];
+      }
+      if (pos) pos[4 * i + 3] = sqrt(r2);
+      if (vel) vel[4 * i + 3] = sqrt(v2);
+    }
+    if (w) {
+      if (ctx->weighted) {
+        std::vector<float> wf(np);
+        CHY_HIP(hipMemcpy(wf.data(), s.w, sizeof(float) * np, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < np; i++) w[i] = wf[i];
+      } else {
+        for (int64_t i = 0; i < np; i++) w[i] = 1.0;
+      }
+    }
+    if (pid) CHY_HIP(hipMemcpy(pid, s.pid, sizeof(uint32_t) * np, hipMemcpyDeviceToHost));
+  }
+  return CHY_OK;
+}
+
+int particles_init(chy_ctx *ctx)
+{
+  const chy_params &p = ctx->prm;
+  const int64_t nc = ctx->g.nc;
+  int64_t ppc = p.ppc > 0 ? p.ppc : (p.Np % nc == 0 ? p.Np / nc : 0);
+  if (ppc <= 0) return set_err(ctx, CHY_ERR_ARG_INCOMP, "device initialisation needs Np to be a multiple of the cell count (-ppc)");
+  const int64_t np = ppc * nc;
+  if (np > ctx->cap) return set_err(ctx, CHY_ERR_ARG_RANGE, "ppc x cells exceeds the context capacity");
+  if (p.perturbation != CHY_PERT_NONE && !ctx->weighted)
+    return set_err(ctx, CHY_ERR_ARG_INCOMP, "perturbation needs weight buffers");
+  ctx->np = np;
+  const unsigned nb = (unsigned)((np + kBlock - 1) / kBlock);
+  PState<double> sd = ctx->pd[ctx->cur];
+  PState<float> sf = ctx->pf[ctx->cur];
+  if (ctx->mixed)
+    hipLaunchKernelGGL((k_init<true>), nb, kBlock, 0, ctx->stream, ctx->g, ppc, (uint32_t)p.seed,
+                       (uint32_t)(p.seed >> 32), p.vi0[0], p.vi0[1], p.vi0[2], p.viT[0], p.viT[1], p.viT[2],
+                       p.perturbation, p.perturbation_amplitude, sd, sf);
+  else
+    hipLaunchKernelGGL((k_init<false>), nb, kBlock, 0, ctx->stream, ctx->g, ppc, (uint32_t)p.seed,
+                       (uint32_t)(p.seed >> 32), p.vi0[0], p.vi0[1], p.vi0[2], p.viT[0], p.viT[1], p.viT[2],
+                       p.perturbation, p.perturbation_amplitude, sd, sf);
+  if (ctx->weighted && p.perturbation == CHY_PERT_NONE) {
+    if (ctx->mixed) launch_fill<float>(ctx, np, sf.w, 1.0f);
+    else launch_fill<double>(ctx, np, sd.w, 1.0);
+  }
+  return settle(ctx);
+}
+
+int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *stats)
+{
+  PushConst pc = push_const(ctx);
+  const int64_t Nc = collide ? collision_count(ctx->prm, ctx->np) : 0;
+  pc.Nc = (double)Nc;
+  if (collide && Nc == 0) collide = false;
+  launch_fill<unsigned long long>(ctx, 4, ctx->cstats, 0ull);
+  if (move) launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
+  const unsigned nb = grid_for(ctx->np);
+  timer_start(ctx, "push");
+#define LAUNCH(KER, T, C, M, G)                                                                             \
+  hipLaunchKernelGGL((KER<C, M, G>), nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, st, (const T *)ctx->efield[0], \
+                     (const T *)ctx->efield[1], (const T *)ctx->efield[2], ctx->S_dev, ctx->counts, ctx->sred,      \
+                     ctx->cstats)
+  if (ctx->mixed) {
+    PState<float> st = ctx->pf[ctx->cur];
+    if (collide && move) LAUNCH(k_push_mixed, float, true, true, true);
+    else if (collide) LAUNCH(k_push_mixed, float, true, false, true);
+    else if (move) LAUNCH(k_push_mixed, float, false, true, true);
+    else LAUNCH(k_push_mixed, float, false, false, true);
+  } else {
+    PState<double> st = ctx->pd[ctx->cur];
+    if (collide && move) LAUNCH(k_push_fp64, double, true, true, true);
+    else if (collide) LAUNCH(k_push_fp64, double, true, false, true);
+    else if (move) LAUNCH(k_push_fp64, double, false, true, true);
+    else LAUNCH(k_push_fp64, double, false, false, true);
+  }
+  timer_stop(ctx, "push");
+  if (collide) launch_combine_sum(ctx, ctx->sred, (int)nb, 1, ctx->S_dev);
+  unsigned long long cs[4];
+  CHY_HIP(hipMemcpyAsync(cs, ctx->cstats, sizeof(cs), hipMemcpyDeviceToHost, ctx->stream));
+  CHY_HIP(hipStreamSynchronize(ctx->stream));
+  if (stats) {
+    stats->accepted = (int64_t)cs[0];
+    stats->refused = (int64_t)cs[1];
+    stats->capped = (int64_t)cs[2];
+    stats->Nc = Nc;
+  }
+  if (cs[3]) return set_err(ctx, CHY_ERR_FP, "non-finite particle position or velocity after the push");
+  if (cs[1] >= 10)
+    return set_err(ctx, CHY_ERR_SIG, "Failed to collide " + std::to_string(cs[1]) + " ion-neutral pairs. Aborting.");
+  return CHY_OK;
+#undef LAUNCH
+}
+
+int particles_move_only(chy_ctx *ctx)
+{
+  PushConst pc = push_const(ctx);
+  launch_fill<unsigned long long>(ctx, 4, ctx->cstats, 0ull);
+  launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
+  const unsigned nb = grid_for(ctx->np);
+  if (ctx->mixed) {
+    PState<float> st = ctx->pf[ctx->cur];
+    hipLaunchKernelGGL((k_push_mixed<false, true, false>), nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, st,
+                       (const float *)ctx->efield[0], (const float *)ctx->efield[1], (const float *)ctx->efield[2],
+                       ctx->S_dev, ctx->counts, ctx->sred, ctx->cstats);
+  } else {
+    PState<double> st = ctx->pd[ctx->cur];
+    hipLaunchKernelGGL((k_push_fp64<false, true, false>), nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, st,
+                       (const double *)ctx->efield[0], (const double *)ctx->efield[1], (const double *)ctx->efield[2],
+                       ctx->S_dev, ctx->counts, ctx->sred, ctx->cstats);
+  }
+  unsigned long long cs[4];
+  CHY_HIP(hipMemcpyAsync(cs, ctx->cstats, sizeof(cs), hipMemcpyDeviceToHost, ctx->stream));
+  CHY_HIP(hipStreamSynchronize(ctx->stream));
+  if (cs[3]) return set_err(ctx, CHY_ERR_FP, "non-finite particle position after the move");
+  return CHY_OK;
+}
+
+}  // namespace chy

@@ -1,0 +1,96 @@
+/*
+ * chy_output.c -- per-step field output, the counterpart of OutputHDF5
+ * (reference fileio.c:73-126): six f64 datasets of shape (Nz, Ny, Nx) named
+ * as the DMDA fields of setup.c:53-56 plus "rhs" and "potential".
+ *
+ * HDF5's C library is opened with dlopen so libchybrid.so has no link-time
+ * dependency on it; the handful of entry points used are resolved by name.
+ */
+#include "chybrid.h"
+
+#include <dlfcn.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef int64_t hid_t;          /* HDF5 >= 1.10 */
+typedef int herr_t;
+typedef unsigned long long hsize_t;
+
+static struct {
+  int tried, ok;
+  herr_t (*open)(void);
+  hid_t (*fcreate)(const char *, unsigned, hid_t, hid_t);
+  hid_t (*screate)(int, const hsize_t *, const hsize_t *);
+  hid_t (*dcreate)(hid_t, const char *, hid_t, hid_t, hid_t, hid_t, hid_t);
+  herr_t (*dwrite)(hid_t, hid_t, hid_t, hid_t, hid_t, const void *);
+  herr_t (*dclose)(hid_t), (*sclose)(hid_t), (*fclose)(hid_t);
+  hid_t *native_double;
+} H5;
+
+static void h5_load(void)
+{
+  if (H5.tried) return;
+  H5.tried = 1;
+  const char *names[] = {"libhdf5.so", "libhdf5.so.103", "libhdf5_serial.so", "/opt/conda/lib/libhdf5.so", NULL};
+  void *h = NULL;
+  for (int i = 0; names[i] && !h; i++) h = dlopen(names[i], RTLD_NOW | RTLD_LOCAL);
+  if (!h) return;
+  *(void **)&H5.open = dlsym(h, "H5open");
+  *(void **)&H5.fcreate = dlsym(h, "H5Fcreate");
+  *(void **)&H5.screate = dlsym(h, "H5Screate_simple");
+  *(void **)&H5.dcreate = dlsym(h, "H5Dcreate2");
+  *(void **)&H5.dwrite = dlsym(h, "H5Dwrite");
+  *(void **)&H5.dclose = dlsym(h, "H5Dclose");
+  *(void **)&H5.sclose = dlsym(h, "H5Sclose");
+  *(void **)&H5.fclose = dlsym(h, "H5Fclose");
+  H5.native_double = (hid_t *)dlsym(h, "H5T_NATIVE_DOUBLE_g");
+  H5.ok = H5.open && H5.fcreate && H5.screate && H5.dcreate && H5.dwrite && H5.dclose && H5.sclose &&
+          H5.fclose && H5.native_double && H5.open() >= 0;
+}
+
+int chy_output_write(chy_ctx *ctx, const char *stem, const char *insert)
+{
+  chy_params p;
+  int rc = chy_get_params(ctx, &p);
+  if (rc) return rc;
+  const int64_t nc = p.N[0] * p.N[1] * p.N[2];
+  double *vl = malloc(sizeof(double) * 4 * nc), *f[6];
+  for (int a = 0; a < 6; a++) f[a] = malloc(sizeof(double) * nc);
+  rc = chy_get_moments(ctx, vl);
+  if (!rc) rc = chy_get_rhs(ctx, f[4]);
+  if (!rc) rc = chy_get_potential(ctx, f[5]);
+  for (int64_t c = 0; c < nc && !rc; c++)
+    for (int a = 0; a < 4; a++) f[a][c] = vl[4 * c + a];
+  const char *names[6] = {"density", "x flux", "y flux", "z flux", "rhs", "potential"};
+  char path[8300];
+  if (!rc) {
+    h5_load();
+    if (H5.ok) {
+      snprintf(path, sizeof path, "%s%s.hdf", stem, insert);
+      const hid_t file = H5.fcreate(path, 0x0002u /* H5F_ACC_TRUNC */, 0, 0);
+      if (file < 0) rc = CHY_ERR_LIB;
+      const hsize_t dims[3] = {(hsize_t)p.N[2], (hsize_t)p.N[1], (hsize_t)p.N[0]};
+      for (int a = 0; a < 6 && !rc; a++) {
+        const hid_t sp = H5.screate(3, dims, NULL);
+        const hid_t ds = H5.dcreate(file, names[a], *H5.native_double, sp, 0, 0, 0);
+        if (sp < 0 || ds < 0 || H5.dwrite(ds, *H5.native_double, 0, 0, 0, f[a]) < 0) rc = CHY_ERR_LIB;
+        if (ds >= 0) H5.dclose(ds);
+        if (sp >= 0) H5.sclose(sp);
+      }
+      if (file >= 0) H5.fclose(file);
+    } else {
+      for (int a = 0; a < 6 && !rc; a++) {
+        snprintf(path, sizeof path, "%s%s.%s.f64", stem, insert, names[a]);
+        for (char *q = path + strlen(stem); *q; q++) if (*q == ' ') *q = '_';
+        FILE *fp = fopen(path, "wb");
+        if (!fp || fwrite(f[a], sizeof(double), (size_t)nc, fp) != (size_t)nc) rc = CHY_ERR_LIB;
+        if (fp) fclose(fp);
+      }
+      if (!rc) rc = CHY_ERR_SUP;
+    }
+  }
+  free(vl);
+  for (int a = 0; a < 6; a++) free(f[a]);
+  return rc;
+}

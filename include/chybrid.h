@@ -1,0 +1,216 @@
+/*
+ * chybrid.h -- C ABI of the MI355X (gfx950) implementation of the
+ * collisional-hybrid per-timestep hot path.
+ *
+ * One opaque context (chy_ctx) owns every device allocation of one run.
+ * Host pointers passed in are borrowed for the duration of the call; outputs
+ * are caller-allocated.  Every entry point returns 0 on success or a negative
+ * error code whose magnitude is the PETSc error class the reference raises in
+ * the same situation (reference hybrid.h / PetscErrorCode convention);
+ * chy_last_error() returns the message.  No exception or abort crosses the
+ * ABI.  A context is used by one host thread at a time; calls are ordered on
+ * the context's HIP stream and are complete when they return.
+ *
+ * Layout contract (reference setup.c:6-146, hybrid.h:40-45,140-143):
+ *   grid fields   [Nz][Ny][Nx] f64, DMDA natural order (x fastest);
+ *   vlasov        [Nz][Ny][Nx][4] f64 interleaved {n, Gx, Gy, Gz} (GridNode);
+ *   particles     RealVector AoS [np][4] f64 {x, y, z, r} for position and
+ *                 velocity (DMSwarm fields "position" and "velocity"; r is
+ *                 written as the magnitude and ignored on input).
+ */
+#ifndef CHYBRID_H
+#define CHYBRID_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes (negated PETSc classes) -------------------------------- */
+#define CHY_OK                 0
+#define CHY_ERR_MEM          (-55)  /* PETSC_ERR_MEM */
+#define CHY_ERR_SUP          (-56)  /* PETSC_ERR_SUP */
+#define CHY_ERR_SIG          (-59)  /* PETSC_ERR_SIG: collision refusals, particles.c:603-605 */
+#define CHY_ERR_ARG_WRONG    (-62)  /* PETSC_ERR_ARG_WRONG: parameters.c:454,468,505 */
+#define CHY_ERR_ARG_RANGE    (-63)  /* PETSC_ERR_ARG_OUTOFRANGE */
+#define CHY_ERR_FP           (-72)  /* PETSC_ERR_FP: NaN/Inf detected on the GPU */
+#define CHY_ERR_ARG_INCOMP   (-75)  /* PETSC_ERR_ARG_INCOMP: parameters.c:486,493,500 */
+#define CHY_ERR_LIB          (-76)  /* PETSC_ERR_LIB: HIP runtime failure */
+#define CHY_ERR_NOT_CONVERGED (-91) /* PETSC_ERR_NOT_CONVERGED: -ksp_error_if_not_converged */
+
+/* ---- enumerations (reference lhs.h:8-12, rhs.h:8-12, hybrid.h:84-102) ---- */
+enum { CHY_LHS_IDENTITY = 0, CHY_LHS_LAPLACIAN = 1, CHY_LHS_FULL = 2 };
+enum { CHY_RHS_CONSTANT = 0, CHY_RHS_SINUSOIDAL = 1, CHY_RHS_FULL = 2 };
+enum { CHY_DENSITY_FLAT_SOBOL = 0, CHY_DENSITY_FLAT_REVERSE, CHY_DENSITY_FLAT_NORMAL,
+       CHY_DENSITY_UNIFORM, CHY_DENSITY_SINUSOIDAL, CHY_DENSITY_GAUSSIAN };
+enum { CHY_BC_PERIODIC = 0, CHY_BC_INJECTION, CHY_BC_REFLECTION, CHY_BC_DIRICHLET, CHY_BC_NEUMANN };
+/* build additions */
+enum { CHY_PREC_FP64 = 0, CHY_PREC_MIXED = 1 };
+enum { CHY_KSP_GMRES = 0, CHY_KSP_BCGS = 1 };
+enum { CHY_PC_NONE = 0, CHY_PC_JACOBI = 1, CHY_PC_FFT = 2 };
+enum { CHY_PERT_NONE = 0, CHY_PERT_SINUSOIDAL_X = 1, CHY_PERT_UNIFORM = 2, CHY_PERT_GAUSSIAN = 3 };
+
+/* ---- run parameters: the scalars of the reference Context (hybrid.h:106-138)
+ * plus their derived values (parameters.c:507-556, setup.c:43-49). ---------- */
+typedef struct {
+  /* Grid (hybrid.h:47-53) */
+  int64_t N[3];          /* nodes per axis */
+  double  p0[3], p1[3];  /* lower / upper bounds [m] */
+  double  L[3], d[3];    /* derived: p1 - p0, L / N */
+  int32_t bc[3][2];      /* CHY_BC_*; only all-periodic is supported */
+  /* electrons (Charged) */
+  double  qe, me, nue, Te, gamma_e;
+  double  ve0[3], veT[3], veT_r;
+  double  Omega_e[3], kappa_e[3];
+  /* ions (Charged) */
+  double  qi, mi, nui, Ti;
+  double  vi0[3], viT[3], viT_r;
+  double  Omega_i[3], kappa_i[3];
+  /* neutrals (Neutral, hybrid.h:67-72) */
+  double  mn, Tn, vnT;
+  double  vn0[3], vn0_r;
+  /* plasma (Plasma, hybrid.h:74-78) */
+  double  B0[3], E0[3];
+  int64_t Np;
+  /* time loop (hybrid.h:130-133) */
+  int64_t Nt;
+  double  dt;
+  /* operator / forcing / initial density selection */
+  int32_t lhs_type, rhs_type, density_type;
+  int32_t view_lhs;
+  /* solver (PETSc KSP/PC subset mapped onto the GPU solver) */
+  int32_t ksp_type, pc_type;
+  double  ksp_rtol, ksp_atol;
+  int64_t ksp_max_it, ksp_restart;
+  int32_t ksp_monitor, ksp_converged_reason, ksp_error_if_not_converged;
+  /* build additions: seed, particles per cell, perturbation, precision, output cadence */
+  uint64_t seed;
+  int64_t ppc;
+  int32_t perturbation;
+  double  perturbation_amplitude;
+  int32_t weighted;      /* allocate per-particle weights (implied by a perturbation) */
+  int32_t precision;
+  int64_t output_interval;
+  /* solver binary (parameters.c:419-440) */
+  double  flux_scale[3];
+  char    input[4096];
+  char    output[4096];
+} chy_params;
+
+typedef struct chy_ctx chy_ctx;
+
+typedef struct {
+  int64_t accepted;      /* collisions applied this step */
+  int64_t refused;       /* refused (final speed > 10 vT), particles.c:597-605 */
+  int64_t capped;        /* Bernoulli probabilities clipped at 1 */
+  int64_t Nc;            /* target count, particles.c:514 */
+} chy_collision_stats;
+
+typedef struct {
+  int64_t iterations;    /* total inner Krylov iterations */
+  int64_t outer;         /* FP64 defect-correction sweeps (1 in FP64 mode) */
+  double  residual;      /* true ||b - A phi|| / ||b|| in FP64 */
+  int32_t converged;     /* 1 converged, 0 not (max_it reached) */
+  int32_t reason;        /* PETSc-style KSPConvergedReason (>0 ok, <0 diverged) */
+  double  ms;            /* wall time of the solve (RHS + Krylov + projections) */
+} chy_solve_stats;
+
+typedef struct {
+  chy_collision_stats collisions;
+  chy_solve_stats     solve;
+  double              ms_particles;  /* push + sort + deposit */
+  double              ms_total;
+} chy_step_stats;
+
+/* ---- parameters ----------------------------------------------------------
+ * Defaults of ProcessOptions (reference parameters.c:28-441). */
+int chy_params_default(chy_params *prm);
+/* Parse an options-database style argument vector (reference flag names of
+ * parameters.c plus -ksp_* / -pc_* and the build additions), then derive
+ * (parameters.c:442-556, setup.c:9-49).  Unknown options are ignored like
+ * PETSc's, except that err gets a warning list.  Replaces ProcessOptions. */
+int chy_params_from_options(int argc, const char *const *argv, chy_params *prm,
+                            char *err, size_t errlen);
+/* Recompute derived values after editing a chy_params by hand. */
+int chy_params_derive(chy_params *prm, char *err, size_t errlen);
+/* Write the options.txt report (reference parameters.c:566-654). */
+int chy_write_options_txt(const chy_params *prm, int argc, const char *const *argv, const char *path);
+
+/* ---- context --------------------------------------------------------------
+ * Replaces SetUpVlasovDM / SetUpIonsDM / SetUpPotentialDM / KSPCreate
+ * (reference hybrid.c:52-78, setup.c:6-146).  np_capacity = 0 uses prm->Np. */
+int chy_create(const chy_params *prm, int64_t np_capacity, chy_ctx **ctx);
+int chy_destroy(chy_ctx *ctx);
+const char *chy_last_error(const chy_ctx *ctx);
+int chy_sync(chy_ctx *ctx);
+/* The context's HIP stream (a hipStream_t), for callers that enqueue work. */
+void *chy_stream(chy_ctx *ctx);
+int chy_get_params(const chy_ctx *ctx, chy_params *prm);
+int64_t chy_num_particles(const chy_ctx *ctx);
+/* Step counter used to key the collision random streams. */
+int chy_set_step(chy_ctx *ctx, int64_t it);
+
+/* ---- particles (DMSwarm fields "position", "velocity"; setup.c:130-132) --- */
+/* pos, vel: [np][4] f64; w may be NULL (unit weights); pid may be NULL
+ * (pid = array index).  Particles are re-sorted by cell on upload. */
+int chy_set_particles(chy_ctx *ctx, int64_t np, const double *pos, const double *vel,
+                      const double *w, const uint32_t *pid);
+/* Any output may be NULL.  Order is the current cell-sorted order. */
+int chy_get_particles(chy_ctx *ctx, double *pos, double *vel, double *w, uint32_t *pid);
+/* Seeded placement on the device: ppc per cell, offsets U[0,1)^3, velocities
+ * vi0 + viT N(0,1); weights from prm perturbation (replaces the once-per-run
+ * InitializePositions / InitializeVelocities, particles.c:9-132). */
+int chy_init_particles(chy_ctx *ctx);
+
+/* ---- the hot path --------------------------------------------------------
+ * UpdateVelocities (particles.c:628-643): gather E = E0 - grad(phi) at each
+ * ion, Boris rotation, then collisions when mn > 0. */
+int chy_update_velocities(chy_ctx *ctx, chy_collision_stats *stats);
+/* UpdatePositions (particles.c:647-718): x += v dt, periodic wrap, then the
+ * stable counting sort by cell that replaces DMSwarmMigrate. */
+int chy_update_positions(chy_ctx *ctx);
+/* CollectVlasovQuantities (particles.c:136-255): deterministic CIC deposit of
+ * density and flux.  vlasov may be NULL; otherwise [Nz][Ny][Nx][4] f64. */
+int chy_collect_moments(chy_ctx *ctx, double *vlasov);
+/* Upload moments (solver input, fileio.c:6-70) / download the current ones. */
+int chy_set_moments(chy_ctx *ctx, const double *vlasov);
+int chy_get_moments(chy_ctx *ctx, double *vlasov);
+/* ComputeRHS -> rhsFunc (potential.c:33-42, rhs.c): b [Nz][Ny][Nx] may be NULL. */
+int chy_compute_rhs(chy_ctx *ctx, double *b);
+/* ComputeLHS -> lhsFunc (potential.c:46-55, lhs.c) as a matrix-free product
+ * with the operator of the current density: y = A(n) x, host arrays. */
+int chy_lhs_apply(chy_ctx *ctx, const double *x, double *y);
+int chy_lhs_apply_transpose(chy_ctx *ctx, const double *x, double *y);
+/* ComputePotential (potential.c:6-16): RHS from the current moments, then the
+ * Krylov solve warm-started from the previous phi (zero on the first call,
+ * potential.c:20-29).  phi may be NULL. */
+int chy_solve_potential(chy_ctx *ctx, double *phi, chy_solve_stats *stats);
+/* Solve A(n) phi = b for a given b (mean removed); phi in/out. */
+int chy_solve(chy_ctx *ctx, const double *b, double *phi, chy_solve_stats *stats);
+int chy_get_potential(chy_ctx *ctx, double *phi);
+int chy_set_potential(chy_ctx *ctx, const double *phi);
+int chy_get_rhs(chy_ctx *ctx, double *b);
+/* One time step of the loop body hybrid.c:108-117 (velocities, positions,
+ * moments, potential), fused on the device. */
+int chy_step(chy_ctx *ctx, chy_step_stats *stats);
+
+/* ---- output (OutputHDF5, reference fileio.c:73-126) -------------------------
+ * Writes <stem><insert>.hdf with datasets "density", "x flux", "y flux",
+ * "z flux", "rhs" and "potential", each (Nz, Ny, Nx) f64.  HDF5 is loaded at
+ * run time; without it the six arrays go to <stem><insert>.<name>.f64 raw
+ * files and the call returns CHY_ERR_SUP after writing them. */
+int chy_output_write(chy_ctx *ctx, const char *stem, const char *insert);
+
+/* ---- instrumentation -------------------------------------------------------
+ * Per-kernel HIP-event timing on the context stream (off by default).
+ * Names: "push", "sort", "deposit", "lhs_apply", "rhs", "pc", "krylov". */
+int chy_timing_enable(chy_ctx *ctx, int on);
+int chy_timing_get(chy_ctx *ctx, const char *name, double *total_ms, int64_t *count);
+int chy_timing_reset(chy_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -48,11 +48,16 @@ enum { STREAM_INIT = 0, STREAM_COLLIDE_A = 1, STREAM_COLLIDE_B = 2, STREAM_WEIGH
 
 double orc_wrap(double x, double p0, double p1, double L)
 {
+  /* far outside (more than one period): reduce with floor first */
+  if (!(x >= p0 - L && x < p1 + L)) {
+    if (!isfinite(x)) return x;
+    x = x - floor((x - p0) / L) * L;
+  }
   if (x < p0) {
-    do { x += L; } while (x < p0);
+    x += L;
     if (x >= p1) x = p0;
   } else if (x >= p1) {
-    do { x -= L; } while (x >= p1);
+    x -= L;
     if (x < p0) x = p0;
   }
   return x;

@@ -1,0 +1,124 @@
+"""GPU parity of the grid kernels through the C ABI: matrix-free operator and
+its transpose (reference lhs.c), forcing (rhs.c), and the potential solve
+(potential.c + KSP), against the oracle on identical inputs."""
+import numpy as np
+import pytest
+
+import chybrid
+import orc
+from gpu_cases import context, grid_of, opts, relmax, vlasov_from
+
+pytestmark = pytest.mark.gpu
+
+
+def _moments(g, rng, amp=0.05):
+    n = 16 * (1 + amp * rng.uniform(-1, 1, g.nc))
+    G = [rng.standard_normal(g.nc) * 200 * n for _ in range(3)]
+    return [n] + G
+
+
+@pytest.mark.parametrize("N,extra", [
+    ((8, 8, 8), ""),
+    ((6, 5, 7), "-B0x 2e-6 -B0y -3e-6"),
+    ((32, 32, 32), ""),
+    ((16, 8, 12), "--lhs-type laplacian"),
+    ((5, 4, 3), "--lhs-type identity"),
+])
+def test_lhs_apply_fp64_matches_oracle(N, extra):
+    rng = np.random.default_rng(sum(N))
+    with context(opts(N, extra=extra)) as ctx:
+        g = grid_of(ctx.prm)
+        mom = _moments(g, rng)
+        ctx.set_moments(vlasov_from(mom))
+        x = rng.standard_normal(g.nc)
+        y = ctx.lhs_apply(x)
+        yt = ctx.lhs_apply(x, transpose=True)
+    K = list(ctx.prm.kappa_e)
+    ref = orc.lhs_apply(g, K, mom[0], x, lhs_type=ctx.prm.lhs_type)
+    reft = orc.lhs_apply(g, K, mom[0], x, lhs_type=ctx.prm.lhs_type, transpose=True)
+    # same operation order, no contraction: FP64 results are bitwise equal
+    assert np.array_equal(y, ref), relmax(y, ref)
+    assert relmax(yt, reft) <= 1e-14
+
+
+def test_lhs_apply_mixed_uses_fp32_density_fp64_arithmetic():
+    rng = np.random.default_rng(3)
+    with context(opts((8, 6, 10), extra="-precision mixed")) as ctx:
+        g = grid_of(ctx.prm)
+        mom = _moments(g, rng)
+        ctx.set_moments(vlasov_from(mom))
+        x = rng.standard_normal(g.nc)
+        y = ctx.lhs_apply(x)
+    n32 = mom[0].astype(np.float32).astype(np.float64)
+    ref = orc.lhs_apply(g, list(ctx.prm.kappa_e), n32, x)
+    assert relmax(y, ref) <= 1e-14
+
+
+@pytest.mark.parametrize("N,extra", [((8, 8, 8), ""), ((6, 10, 4), "-Te 1500 -B0x 1e-5 -E0y 0.02 -E0z 0.01"),
+                                     ((8, 8, 8), "--rhs-type sinusoidal"), ((8, 8, 8), "--rhs-type constant")])
+def test_rhs_matches_oracle(N, extra):
+    rng = np.random.default_rng(5)
+    with context(opts(N, extra=extra)) as ctx:
+        g = grid_of(ctx.prm)
+        mom = _moments(g, rng)
+        ctx.set_moments(vlasov_from(mom))
+        b = ctx.compute_rhs()
+    p = ctx.prm
+    cth = p.gamma_e * orc.KB * p.Te / orc.Q
+    cG = p.me * p.nue / orc.Q
+    ref = orc.rhs(g, list(p.kappa_e), list(p.E0), cth, cG, *mom, rhs_type=p.rhs_type)
+    assert relmax(b, ref) <= 1e-13
+    if p.rhs_type != chybrid.RHS_CONSTANT:
+        assert abs(b.mean()) <= 1e-14 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("extra", ["", "-ksp_type bcgs", "-pc_type jacobi -ksp_max_it 20000",
+                                   "-B0x 1e-5 -B0y 2e-5", "--lhs-type laplacian"])
+def test_solve_matches_oracle(extra):
+    rng = np.random.default_rng(7)
+    with context(opts((16, 16, 16), extra="-ksp_rtol 1e-12 " + extra)) as ctx:
+        g = grid_of(ctx.prm)
+        mom = _moments(g, rng, amp=0.1)
+        ctx.set_moments(vlasov_from(mom))
+        phi, st = ctx.solve_potential()
+        b = ctx.get_rhs()
+    assert st.converged == 1 and st.residual <= 1e-12
+    ref, its, conv, res = orc.solve(g, list(ctx.prm.kappa_e), mom[0], b, lhs_type=ctx.prm.lhs_type,
+                                    rtol=1e-13, max_it=20000)
+    assert conv
+    assert relmax(phi, ref) <= 1e-9
+    assert abs(phi.mean()) <= 1e-13 * np.abs(phi).max()
+
+
+def test_mixed_solve_reaches_fp64_accuracy():
+    """FP32 inner Krylov + FP64 defect correction: phi within 1e-10 of the FP64
+    solver on the same density and forcing (the mixed-precision target)."""
+    rng = np.random.default_rng(8)
+    g_args = opts((32, 32, 32), extra="-ksp_rtol 1e-10")
+    mom = None
+    out = {}
+    for prec in ("fp64", "mixed"):
+        with context(g_args + ["-precision", prec]) as ctx:
+            g = grid_of(ctx.prm)
+            if mom is None:
+                mom = _moments(g, rng, amp=0.1)
+                # density representable in FP32 so both solve the same system
+                mom = [m.astype(np.float32).astype(np.float64) for m in mom]
+            ctx.set_moments(vlasov_from(mom))
+            phi, st = ctx.solve_potential()
+            assert st.converged == 1 and st.residual <= 1e-10
+            out[prec] = (phi, st)
+    assert out["mixed"][1].outer >= 2
+    assert relmax(out["mixed"][0], out["fp64"][0]) <= 1e-9
+
+
+def test_solve_warm_start_and_fft_iteration_counts():
+    rng = np.random.default_rng(9)
+    with context(opts((32, 32, 32), extra="-ksp_rtol 1e-10")) as ctx:
+        g = grid_of(ctx.prm)
+        mom = _moments(g, rng, amp=0.01)
+        ctx.set_moments(vlasov_from(mom))
+        phi1, st1 = ctx.solve_potential()
+        assert st1.converged and st1.iterations <= 15
+        phi2, st2 = ctx.solve_potential()   # same system, warm start: converged at once
+        assert st2.iterations == 0 and relmax(phi2, phi1) <= 1e-12

@@ -1,0 +1,191 @@
+"""GPU parity of the particle path through the C ABI: stable cell sort,
+deterministic CIC deposit, gather + Boris, collisions, the move, device
+initialisation and whole steps, against the oracle on identical inputs."""
+import numpy as np
+import pytest
+
+import chybrid
+import orc
+from gpu_cases import (as_particles, context, grid_of, opts, oracle_sorted, push_params_of, random_particles,
+                       relmax, upload, vlasov_from)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("N,n,weights", [((8, 8, 8), 5000, False), ((6, 5, 7), 3000, True),
+                                         ((32, 32, 32), 200000, True)])
+def test_sort_and_deposit_fp64_bitwise(N, n, weights):
+    rng = np.random.default_rng(n)
+    with context(opts(N), capacity=n, weighted=weights) as ctx:
+        g = grid_of(ctx.prm)
+        p = random_particles(g, n, rng, weights=weights)
+        upload(ctx, p)
+        got = as_particles(ctx.get_particles())
+        v = ctx.collect_moments()
+    ps, offs = oracle_sorted(g, p)
+    # stable counting sort: identical order (pid sequence) and payload
+    assert np.array_equal(got.pid, ps.pid)
+    for a in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(getattr(got, a), getattr(ps, a)), a
+    ref = orc.deposit_sorted(g, ps, offs)
+    for a in range(4):
+        assert np.array_equal(v[:, a], ref[a]), (a, relmax(v[:, a], ref[a]))
+
+
+def test_deposit_edge_cases():
+    """Empty set, one crowded cell, positions on the upper bound and outside."""
+    with context(opts((4, 4, 4)), capacity=4000) as ctx:
+        g = grid_of(ctx.prm)
+        ctx.set_particles([], [], [], [], [], [])
+        v = ctx.collect_moments()
+        assert not v.any()
+        rng = np.random.default_rng(1)
+        n = 3000
+        x = np.full(n, g.p0[0] + 0.3 * g.d[0]) + rng.random(n) * 1e-3 * g.d[0]
+        y = np.full(n, g.p1[1])            # == upper bound: wraps to p0
+        z = g.p0[2] - 0.25 * g.d[2] + np.zeros(n)  # below p0: wraps into the last cell
+        vv = rng.standard_normal((3, n))
+        p = orc.Particles(x, y, z, *vv)
+        upload(ctx, p)
+        got = as_particles(ctx.get_particles())
+        v = ctx.collect_moments()
+    assert np.all(got.y == g.p0[1]) and np.all((got.z >= g.p0[2]) & (got.z < g.p1[2]))
+    q = orc.Particles(got.x, got.y, got.z, got.vx, got.vy, got.vz, pid=got.pid)
+    ps, offs = oracle_sorted(g, q)
+    assert np.array_equal(ps.pid, got.pid)
+    ref = orc.deposit_sorted(g, ps, offs)
+    assert np.array_equal(v[:, 0], ref[0])
+    assert v[:, 0].sum() == pytest.approx(n, rel=1e-13)
+
+
+def _phi(g, rng):
+    k, j, i = np.meshgrid(*[np.arange(int(m)) for m in g.shape()], indexing="ij")
+    return (0.02 * np.sin(2 * np.pi * i / g.N[0]) * np.cos(2 * np.pi * j / g.N[1])
+            + 0.01 * np.cos(2 * np.pi * k / g.N[2]) + 1e-3 * rng.standard_normal(g.shape())).ravel()
+
+
+def test_push_and_move_fp64_match_oracle_bitwise():
+    rng = np.random.default_rng(11)
+    n = 20000
+    with context(opts((8, 8, 8), extra="-mn 0"), capacity=n) as ctx:
+        g, P = push_params_of(ctx.prm, step=0)
+        p = random_particles(g, n, rng)
+        upload(ctx, p)
+        start = as_particles(ctx.get_particles())
+        phi = _phi(g, rng)
+        ctx.set_potential(phi)
+        st = ctx.update_velocities()
+        mid = as_particles(ctx.get_particles())
+        ctx.update_positions()
+        end = as_particles(ctx.get_particles())
+    assert st.accepted == 0 and st.Nc == 0
+    ex, ey, ez = orc.efield(g, list(ctx.prm.E0), phi)
+    ref = start.copy()
+    key, _, _ = orc.push(P, ex, ey, ez, ref, 1.0)
+    for a in ("vx", "vy", "vz"):
+        assert np.array_equal(getattr(mid, a), getattr(ref, a)), (a, relmax(getattr(mid, a), getattr(ref, a)))
+    perm, offs = orc.sort_perm(g.nc, key)
+    refs = ref.permute(perm)
+    assert np.array_equal(end.pid, refs.pid)
+    for a in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(getattr(end, a), getattr(refs, a)), a
+
+
+def test_collisions_match_oracle_per_particle():
+    rng = np.random.default_rng(12)
+    n = 50000
+    with context(opts((8, 8, 8), extra="-nui 3e3 -seed 5"), capacity=n) as ctx:
+        prm = ctx.prm
+        g, P = push_params_of(prm, step=0)
+        Pn = orc.push_params(g, prm.qi / prm.mi, prm.dt, list(prm.B0), list(prm.E0), collide=True, mi=prm.mi,
+                             mn=prm.mn, viT=prm.viT_r, vnT=prm.vnT, vn0=list(prm.vn0),
+                             Nc=orc.collision_count(n, prm.nui, prm.dt, prm.viT_r, list(prm.vi0), list(prm.vn0),
+                                                    prm.vnT, prm.mi, prm.mn), seed=prm.seed, step=0)
+        p = random_particles(g, n, rng)
+        upload(ctx, p)
+        start = as_particles(ctx.get_particles())
+        st = ctx.update_velocities()
+        mid = as_particles(ctx.get_particles())
+    S = sum(orc.mean_rel_speed((start.vx[s], start.vy[s], start.vz[s]), list(prm.vn0), prm.vnT)
+            for s in range(n))
+    ex, ey, ez = orc.efield(g, list(prm.E0), np.zeros(g.nc))
+    ref = start.copy()
+    _, _, stats = orc.push(Pn, ex, ey, ez, ref, S)
+    assert st.Nc == Pn.Nc and st.accepted == stats[0] and st.refused == stats[1]
+    sig = np.sqrt(Pn.Nc)
+    assert abs(st.accepted + st.refused - Pn.Nc) < 4 * sig
+    for a in ("vx", "vy", "vz"):
+        assert relmax(getattr(mid, a), getattr(ref, a)) <= 1e-12, a
+
+
+def test_device_init_matches_oracle():
+    for pert, name in ((1, "sinusoidal-x"), (2, "uniform"), (3, "gaussian")):
+        args = opts((8, 4, 6), extra=f"-seed 3 -ppc 5 --perturbation {name} --perturbation-amplitude 0.01")
+        with context(args) as ctx:
+            ctx.init_particles()
+            got = as_particles(ctx.get_particles())
+            g = grid_of(ctx.prm)
+            ref = orc.init_particles(g, 5, 3, list(ctx.prm.vi0), list(ctx.prm.viT), pert=pert, amp=0.01)
+        ps, _ = oracle_sorted(g, ref)
+        assert np.array_equal(got.pid, ps.pid)
+        for a in ("x", "y", "z"):
+            assert np.array_equal(getattr(got, a), getattr(ps, a)), a
+        for a in ("vx", "vy", "vz", "w"):
+            assert relmax(getattr(got, a), getattr(ps, a)) <= 1e-14, a
+
+
+@pytest.mark.parametrize("collide", [False, True])
+def test_steps_match_oracle(collide):
+    """Three whole steps (push, sort, deposit, forcing, solve) from a seeded
+    state; trajectories, moments and phi against the oracle step by step."""
+    extra = "-seed 2 -ppc 16 -ksp_rtol 1e-12 --perturbation uniform --perturbation-amplitude 0.01"
+    if not collide:
+        extra += " -mn 0"
+    with context(opts((8, 8, 8), extra=extra)) as ctx:
+        ctx.init_particles()
+        ctx.collect_moments(download=False)
+        phi, _ = ctx.solve_potential()
+        cur = as_particles(ctx.get_particles())
+        g = grid_of(ctx.prm)
+        prm = ctx.prm
+        S = sum(orc.mean_rel_speed((cur.vx[s], cur.vy[s], cur.vz[s]), list(prm.vn0), prm.vnT)
+                for s in range(cur.np))
+        for it in range(3):
+            st = ctx.step()
+            got = as_particles(ctx.get_particles())
+            mom_gpu = ctx.collect_moments()
+            phi_gpu = ctx.get_potential()
+            _, P = push_params_of(prm, step=it, collide=collide)
+            nxt, keys, mom, S, stats = orc.step(g, P, cur, phi, S, list(prm.E0))
+            assert np.array_equal(got.pid, nxt.pid)
+            for a in ("x", "y", "z", "vx", "vy", "vz"):
+                assert relmax(getattr(got, a), getattr(nxt, a)) <= 1e-12, (it, a)
+            if collide:
+                assert st.collisions.accepted == stats[0]
+            for a in range(4):
+                assert relmax(mom_gpu[:, a], mom[a]) <= 1e-12, (it, a)
+            b = orc.rhs(g, list(prm.kappa_e), list(prm.E0), 0.0, prm.me * prm.nue / orc.Q, *mom)
+            phi_ref, _, conv, _ = orc.solve(g, list(prm.kappa_e), mom[0], b, phi0=phi, rtol=1e-13)
+            assert conv and relmax(phi_gpu, phi_ref) <= 1e-9, it
+            assert st.solve.converged == 1
+            cur, phi = nxt, phi_gpu
+
+
+def test_mixed_step_tracks_fp64_step():
+    """Same seeded initial state in both precisions: after three steps the
+    mixed (int cell + FP32 offset) run agrees with FP64 to FP32 accuracy."""
+    extra = "-seed 4 -ppc 16 -ksp_rtol 1e-10 --perturbation uniform --perturbation-amplitude 0.01 -mn 0"
+    res = {}
+    for prec in ("fp64", "mixed"):
+        with context(opts((16, 16, 16), extra=extra + f" -precision {prec}")) as ctx:
+            ctx.init_particles()
+            ctx.collect_moments(download=False)
+            ctx.solve_potential()
+            for _ in range(3):
+                st = ctx.step()
+                assert st.solve.converged == 1
+            res[prec] = (as_particles(ctx.get_particles()), ctx.collect_moments(), ctx.get_potential())
+    a, b = res["fp64"], res["mixed"]
+    assert np.array_equal(a[0].pid, b[0].pid) or np.mean(a[0].pid == b[0].pid) > 0.999
+    assert relmax(b[1][:, 0], a[1][:, 0]) <= 1e-5
+    assert relmax(b[2], a[2]) <= 1e-3

@@ -1,0 +1,114 @@
+"""CPU tests of the C ABI library: it loads, exports every entry point that
+include/chybrid.h declares, and its host-side options database reproduces the
+reference's flag semantics (parameters.c:28-556) and derived quantities."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import chybrid
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CFG0 = ("-Nx 32 -Ny 32 -Nz 32 -x1 1.6 -y1 1.6 -z1 1.6 -Np 262144 -mi 4.9816e-26 -mn 4.8156e-26 "
+        "-Ti 300 -Tn 300 -B0z 5e-5 -E0x 0.05 -nui 1e3 -nue 1e5 -dt 1e-5 -Nt 100 -seed 0 -ppc 8 "
+        "--perturbation sinusoidal-x --perturbation-amplitude 0.01 -precision fp64").split()
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "chybrid.h")).read()
+    return sorted(set(re.findall(r"\b(chy_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = chybrid.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 30
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) == set(chybrid.SIGNATURES), set(syms) ^ set(chybrid.SIGNATURES)
+
+
+def test_params_struct_layout_defaults():
+    p = chybrid.Params()
+    assert chybrid.lib().chy_params_default(chybrid.C.byref(p)) == 0
+    # fields at both ends of the struct: a layout mismatch garbles the tail
+    assert list(p.N) == [-1, -1, -1]
+    assert p.output == b"results"
+    assert list(p.flux_scale) == [1.0, 1.0, 1.0]
+    assert p.ksp_rtol == 1e-5 and p.ksp_restart == 30 and p.output_interval == 1
+    assert p.qi == 1.6022e-19 and p.mi == 1.6726e-27 and p.qe == -1.6022e-19
+
+
+def test_config0_derived_quantities():
+    p, msg = chybrid.params_from_options(CFG0)
+    assert list(p.N) == [32, 32, 32] and p.Np == 262144 and p.ppc == 8
+    assert np.allclose(list(p.d), 0.05, rtol=1e-15)
+    assert p.viT[0] == pytest.approx(135.93, abs=0.01)          # sqrt(2 kB T / m) / 3
+    assert p.viT_r == pytest.approx(np.sqrt(3) * 135.93, abs=0.02)
+    assert p.vnT == pytest.approx(414.76, abs=0.01)
+    assert p.Omega_i[2] == pytest.approx(160.8, abs=0.1)
+    assert p.Omega_e[2] == pytest.approx(8.794e6, rel=1e-4)
+    assert p.kappa_e[2] == pytest.approx(87.94, abs=0.01)
+    assert 1 + p.kappa_e[2] ** 2 == pytest.approx(7734.8, abs=0.5)
+    assert p.perturbation == chybrid.PERT_SINUSOIDAL_X and p.perturbation_amplitude == 0.01
+    assert p.Tn == 300 and p.mn == 4.8156e-26 and p.dt == 1e-5 and p.Nt == 100
+
+
+def test_reference_defaults_and_fallbacks():
+    p, _ = chybrid.params_from_options([])
+    assert list(p.N) == [7, 7, 7] and p.Np == 343          # setup.c:9-11,43-45
+    assert p.dt == 1.0 and p.mn == p.mi and p.Tn == p.Ti == 0.0
+    assert p.lhs_type == chybrid.LHS_FULL and p.rhs_type == chybrid.RHS_FULL
+    p, _ = chybrid.params_from_options("-nui 250".split())
+    assert p.dt == 1.0 / 250                                  # parameters.c:412-418
+
+
+def test_vector_flags_common_then_component():
+    p, _ = chybrid.params_from_options("-vi0 3 -vi0y 7 -viT 2 -viTz 5".split())
+    assert list(p.vi0) == [3, 7, 3]
+    assert list(p.viT) == [2, 2, 5]
+    assert p.Ti == pytest.approx(0.5 * p.mi / 1.3807e-23 * (4 + 4 + 25))
+
+
+def test_options_case_insensitive_last_wins_env_and_file(tmp_path, monkeypatch):
+    f = tmp_path / "opts"
+    f.write_text("# comment\n-Ny 12\n-E0y 0.5 # trailing\n")
+    monkeypatch.setenv("PETSC_OPTIONS", "-nx 5 -Nz 9")
+    p, _ = chybrid.params_from_options(["-NX", "6", "-Nx", "8", "-options_file", str(f)])
+    assert list(p.N) == [8, 12, 9] and p.E0[1] == 0.5
+
+
+def test_option_errors_mirror_reference_classes():
+    with pytest.raises(chybrid.ChyError) as e:
+        chybrid.params_from_options("--x0-bc periodic --x1-bc dirichlet".split())
+    assert e.value.code == chybrid.ERR_ARG_INCOMP                # parameters.c:486
+    with pytest.raises(chybrid.ChyError) as e:
+        chybrid.params_from_options("--y0-bc neumann --y1-bc neumann".split())
+    assert e.value.code == chybrid.ERR_ARG_WRONG                 # parameters.c:505
+    with pytest.raises(chybrid.ChyError) as e:
+        chybrid.params_from_options("--lhs-type bogus".split())
+    assert e.value.code == chybrid.ERR_ARG_WRONG
+    with pytest.raises(chybrid.ChyError):
+        chybrid.params_from_options("-Nx 4 -Ny 4 -Nz 4 -Np 100 -ppc 2".split())
+    with pytest.raises(chybrid.ChyError):
+        chybrid.params_from_options("-Nx 2".split())
+
+
+def test_solver_option_mapping():
+    p, msg = chybrid.params_from_options("-ksp_type bcgs -pc_type hypre -ksp_rtol 1e-9 -ksp_max_it 77 "
+                                         "-ksp_gmres_restart 12 -ksp_monitor_short -precision mixed".split())
+    assert p.ksp_type == chybrid.KSP_BCGS and p.pc_type == chybrid.PC_FFT and "hypre" in msg
+    assert p.ksp_rtol == 1e-9 and p.ksp_max_it == 77 and p.ksp_restart == 12 and p.ksp_monitor == 1
+    assert p.precision == chybrid.PREC_MIXED
+
+
+def test_options_txt_is_parseable(tmp_path):
+    path = str(tmp_path / "options.txt")
+    p, _ = chybrid.params_from_options(CFG0)
+    chybrid.write_options_txt(p, CFG0, path)
+    text = open(path).read()
+    assert "Parameter Values" in text and "kappa_ez = 87.9" in text
+    # scripts/tools.py reads "-key value" lines of the option table
+    opts = dict(line.split(None, 1) for line in text.splitlines() if line.startswith("-") and " " in line)
+    assert opts["-Nx"] == "32" and opts["-B0z"] == "5e-5"

@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""Benchmark of the collisional-hybrid hot path on one MI355X.
+
+Workload (the reference's metric config at 256^3): 256^3 periodic grid,
+dx = 0.05 m, 16 ions per cell (268,435,456 ions), NO+ ions (30 amu) and a
+29 amu Maxwellian neutral background at 300 K, nu_in = 1e3 s^-1, electron
+nu_e = 1e5 s^-1, B = 5e-5 T along z, E0 = 0.05 V/m along x, dt = 1e-5 s, 1 %
+uniform per-cell density perturbation, Krylov relative tolerance 1e-10,
+collisions on.  A step is one pass of the loop body of the reference driver:
+gather + Boris + collisions + move, stable cell sort, CIC deposit, forcing
+and potential solve.  Inputs are generated and kept in HBM; the timed region
+covers exactly `--steps` steps.
+
+The headline `value` is the mixed-precision form (FP32 particles and inner
+Krylov, FP64 defect correction); the FP64 plain path is timed on the same
+workload and reported beside it.  Per-kernel durations come from HIP events
+recorded on the context stream during the timed steps.  The CPU baseline
+times the oracle (single-threaded C restatement of the reference path) on a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "collisional-hybrid_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+PHYS = ("-mi 4.9816e-26 -mn 4.8156e-26 -Ti 300 -Tn 300 -B0z 5e-5 -E0x 0.05 -nui 1e3 -nue 1e5 -dt 1e-5 "
+        "-seed 0 --perturbation uniform --perturbation-amplitude 0.01 -ksp_rtol 1e-10")
+
+
+def workload_args(N, ppc, prec):
+    L = 0.05 * N
+    return (f"-Nx {N} -Ny {N} -Nz {N} -x1 {L} -y1 {L} -z1 {L} -ppc {ppc} -Nt 200 -precision {prec} "
+            + PHYS).split()
+
+
+# Algorithmic bytes per launch of each kernel: every distinct array argument
+# read once and every output written once at its stored precision.
+# (np = ions, nc = cells = nodes); weights are present in this workload.
+def algorithmic_bytes(kernel, prec, np_, nc):
+    s = 4 if prec == "mixed" else 8
+    if kernel == "push":
+        # state in (position 3s [+ cell 4 in mixed], v 3s, pid 4), state out
+        # (position 3s + cell 4, v 3s), node E 3s per node
+        pos_in = 3 * s + (4 if prec == "mixed" else 0)
+        return np_ * (pos_in + 3 * s + 4 + 3 * s + 4 + 3 * s) + nc * 3 * s
+    if kernel == "sort_scatter":
+        return np_ * (4 + 4 + 4) + nc * 4            # key in, perm + slot key out, cursor
+    if kernel == "sort_reorder":
+        payload = 3 * s + 3 * s + s + 4              # position, v, weight, pid
+        return np_ * (4 + 4 + payload + payload + 4) + nc * 4
+    if kernel == "sort_scan":
+        return nc * (4 + 4 + 4)
+    if kernel == "deposit_cells":
+        return np_ * (3 * s + 3 * s + s) + nc * (4 + 32 * s)
+    if kernel == "deposit_nodes":
+        return nc * (32 * s + 4 * s)
+    if kernel == "lhs_apply":
+        return nc * 3 * s
+    if kernel == "pc":
+        return nc * (s + 2 * s + 4 * (2 * 2 * s) + 2 * s + s)   # 5 FFT passes, complex work array
+    if kernel == "efield":
+        return nc * (8 + 3 * s)
+    if kernel == "rhs":
+        return nc * (4 * s + 8)
+    return None
+
+
+def run_gpu(N, ppc, prec, steps, warmup):
+    import chybrid
+
+    prm, _ = chybrid.params_from_options(workload_args(N, ppc, prec))
+    ctx = chybrid.Context(prm)
+    try:
+        ctx.init_particles()
+        ctx.collect_moments(download=False)
+        ctx.solve_potential()
+        for _ in range(warmup):
+            ctx.step()
+        ctx.timing(True)
+        ctx.timing_reset()
+        ctx.sync()
+        t0 = time.perf_counter()
+        stats = [ctx.step() for _ in range(steps)]
+        ctx.sync()
+        t1 = time.perf_counter()
+        names = ("efield", "push", "sort_scan", "sort_scatter", "sort_reorder", "deposit_cells",
+                 "deposit_nodes", "rhs", "pc", "lhs_apply", "krylov")
+        kern = {}
+        for nme in names:
+            tot, cnt = ctx.timing_get(nme)
+            if cnt:
+                kern[nme] = {"avg_ms": tot / cnt, "launches": cnt, "total_ms": tot}
+        np_ = ctx.num_particles()
+    finally:
+        ctx.close()
+    nc = N ** 3
+    for nme, k in kern.items():
+        b = algorithmic_bytes(nme, prec, np_, nc)
+        if b is not None:
+            k["bytes"] = b
+            k["GBs"] = b / (k["avg_ms"] * 1e-3) / 1e9
+            k["hbm_frac"] = k["GBs"] / HBM_PEAK_GBS
+    elapsed = t1 - t0
+    return {
+        "np": np_,
+        "elapsed_s": elapsed,
+        "ms_per_step": elapsed / steps * 1e3,
+        "particle_steps_per_s": np_ * steps / elapsed,
+        "phi_solve_ms_per_step": sum(s.solve.ms for s in stats) / steps,
+        "particles_ms_per_step": sum(s.ms_particles for s in stats) / steps,
+        "solve_iterations_per_step": sum(s.solve.iterations for s in stats) / steps,
+        "solve_outer_per_step": sum(s.solve.outer for s in stats) / steps,
+        "max_residual": max(s.solve.residual for s in stats),
+        "collisions_per_step": sum(s.collisions.accepted for s in stats) / steps,
+        "collisions_target": stats[-1].collisions.Nc,
+        "kernels": kern,
+    }
+
+
+def cpu_baseline(Ns=64, ppc=16, steps=2):
+    """Oracle (1 thread) on a bounded sample: Ns^3 grid, same physics."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+
+    Q, ME, KB = orc.Q, orc.ME, orc.KB
+    mi, mn = 4.9816e-26, 4.8156e-26
+    vT = np.sqrt(2 * KB * 300 / mi) / 3
+    vnT = np.sqrt(2 * KB * 300 / mn)
+    g = orc.Grid((Ns,) * 3, p1=(0.05 * Ns,) * 3)
+    K = (0.0, 0.0, Q * 5e-5 / ME / 1e5)
+    E0 = (0.05, 0.0, 0.0)
+    p = orc.init_particles(g, ppc, 0, (0, 0, 0), (vT, vT, vT), pert=2, amp=0.01)
+    perm, offs = orc.sort_perm(g.nc, orc.cell_keys(g, p))
+    p = p.permute(perm)
+    mom = orc.deposit_sorted(g, p, offs)
+    b = orc.rhs(g, K, E0, 0.0, ME * 1e5 / Q, *mom)
+    phi, _, _, _ = orc.solve(g, K, mom[0], b, rtol=1e-10)
+    Nc = orc.collision_count(p.np, 1e3, 1e-5, np.sqrt(3) * vT, (0, 0, 0), (0, 0, 0), vnT, mi, mn)
+    S = sum(orc.mean_rel_speed((p.vx[i], p.vy[i], p.vz[i]), (0, 0, 0), vnT) for i in range(0, p.np, 97)) * 97
+    tp = ts = 0.0
+    for it in range(steps):
+        P = orc.push_params(g, Q / mi, 1e-5, (0, 0, 5e-5), E0, collide=True, mi=mi, mn=mn,
+                            viT=np.sqrt(3) * vT, vnT=vnT, Nc=Nc, seed=0, step=it)
+        t0 = time.perf_counter()
+        p, _, mom, S, _ = orc.step(g, P, p, phi, S, E0)
+        t1 = time.perf_counter()
+        b = orc.rhs(g, K, E0, 0.0, ME * 1e5 / Q, *mom)
+        phi, _, _, _ = orc.solve(g, K, mom[0], b, phi0=phi, rtol=1e-10)
+        t2 = time.perf_counter()
+        tp += t1 - t0
+        ts += t2 - t1
+    return {
+        "value": p.np * steps / (tp + ts),
+        "unit": "particle-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle full step (push+collisions+move, stable sort, deposit, forcing, FFT-GMRES solve to 1e-10) "
+                  f"on {Ns}^3 x {ppc} ions/cell ({p.np} ions), {steps} steps, 1 thread",
+        "particles_s_per_step": tp / steps,
+        "phi_solve_ms_per_step": ts / steps * 1e3,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--ppc", type=int, default=16)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain-path comparison")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs the hot path on one GPU; --gpus must be 1 "
+              "(multi-GPU sharding is designed, not built)", file=sys.stderr)
+        sys.exit(2)
+    mixed = run_gpu(a.grid, a.ppc, "mixed", a.steps, a.warmup)
+    fp64 = None if a.no_fp64 else run_gpu(a.grid, a.ppc, "fp64", a.steps, a.warmup)
+    kern = mixed["kernels"]
+    dom = max((k for k in kern if "bytes" in kern[k]), key=lambda k: kern[k]["total_ms"])
+    roof = {
+        "kernel": dom,
+        "bound": "hbm",
+        "achieved": kern[dom]["GBs"],
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": kern[dom]["GBs"] / HBM_PEAK_GBS,
+        "traffic": None,
+    }
+    cpu = None if a.no_cpu else cpu_baseline()
+    out = {
+        "metric": "particle-steps/s (256^3, 16 ions/cell, mixed precision) + phi-solve ms/step",
+        "value": mixed["particle_steps_per_s"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 defect correction, f64 reductions)",
+        "data": "synthetic (seeded device initialisation)",
+        "config": {
+            "workload": f"{a.grid}^3 periodic grid, {a.ppc} ions/cell ({mixed['np']} ions), mixed precision, "
+                        "collisions on, B0z 5e-5 T, E0x 0.05 V/m, dt 1e-5 s, ksp_rtol 1e-10",
+            "grid": a.grid, "ions_per_cell": a.ppc, "ions": mixed["np"], "precision": "mixed",
+        },
+        "phi_solve_ms_per_step": mixed["phi_solve_ms_per_step"],
+        "particles_ms_per_step": mixed["particles_ms_per_step"],
+        "solve_iterations_per_step": mixed["solve_iterations_per_step"],
+        "solve_outer_per_step": mixed["solve_outer_per_step"],
+        "max_solve_residual": mixed["max_residual"],
+        "collisions_per_step": mixed["collisions_per_step"],
+        "collisions_target_per_step": mixed["collisions_target"],
+        "kernels": {k: {q: round(v, 4) if isinstance(v, float) else v for q, v in d.items()}
+                    for k, d in kern.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if fp64 is not None:
+        out["fp64_plain"] = {
+            "value": fp64["particle_steps_per_s"], "ms_per_step": fp64["ms_per_step"],
+            "phi_solve_ms_per_step": fp64["phi_solve_ms_per_step"],
+            "particles_ms_per_step": fp64["particles_ms_per_step"],
+            "solve_iterations_per_step": fp64["solve_iterations_per_step"],
+            "kernels": {k: {q: round(v, 4) if isinstance(v, float) else v for q, v in d.items()}
+                        for k, d in fp64["kernels"].items()},
+        }
+        out["mixed_speedup_vs_fp64"] = fp64["ms_per_step"] / mixed["ms_per_step"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

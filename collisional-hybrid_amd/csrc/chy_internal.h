@@ -35,11 +35,14 @@ struct PState {
   int32_t *cell = nullptr; // cell key (both modes)
 };
 
+// Per-name kernel timer: event pairs recorded on the context stream and
+// resolved lazily (no synchronisation inside the timed work).
 struct Timer {
-  hipEvent_t a = nullptr, b = nullptr;
+  std::vector<hipEvent_t> a, b;  // event pool
+  size_t used = 0;               // pairs recorded since the last resolve
+  bool open = false;
   double total_ms = 0.0;
   int64_t count = 0;
-  bool pending = false;
 };
 
 }  // namespace chy

@@ -588,12 +588,16 @@ int particles_sort(chy_ctx *ctx)
   const int64_t nc = ctx->g.nc, np = ctx->np;
   const int64_t nt = (nc + kScanTile - 1) / kScanTile;
   launch_fill<uint32_t>(ctx, 1, ctx->maxcount, 0u);
+  timer_start(ctx, "sort_scan");
   hipLaunchKernelGGL(k_tile_sum, (unsigned)nt, kBlock, 0, ctx->stream, nc, ctx->counts, ctx->scan_tmp, ctx->maxcount);
   hipLaunchKernelGGL(k_scan_tiles, 1, kBlock, 0, ctx->stream, nt, ctx->scan_tmp);
   hipLaunchKernelGGL(k_tile_scan, (unsigned)nt, kBlock, 0, ctx->stream, nc, ctx->counts, ctx->scan_tmp, ctx->offsets,
                      ctx->cursor, (uint32_t)np);
+  timer_stop(ctx, "sort_scan");
   const int32_t *cell = ctx->mixed ? ctx->pf[ctx->cur].cell : ctx->pd[ctx->cur].cell;
+  timer_start(ctx, "sort_scatter");
   hipLaunchKernelGGL(k_scatter, grid_for(np), kBlock, 0, ctx->stream, np, cell, ctx->cursor, ctx->perm, ctx->slotkey);
+  timer_stop(ctx, "sort_scatter");
   // the stable per-cell rank is quadratic in the cell occupancy: bound it
   uint32_t mx = 0;
   CHY_HIP(hipMemcpyAsync(&mx, ctx->maxcount, sizeof(mx), hipMemcpyDeviceToHost, ctx->stream));
@@ -602,6 +606,7 @@ int particles_sort(chy_ctx *ctx)
     return set_err(ctx, CHY_ERR_SUP, "a cell holds " + std::to_string(mx) + " particles; at most " +
                                          std::to_string(kMaxCellOccupancy) + " are supported");
   const unsigned nb = (unsigned)((np + kBlock - 1) / kBlock);
+  timer_start(ctx, "sort_reorder");
   if (nb) {
     if (ctx->mixed)
       hipLaunchKernelGGL((k_reorder<float>), nb, kBlock, 0, ctx->stream, np, ctx->perm, ctx->slotkey, ctx->offsets,
@@ -610,6 +615,7 @@ int particles_sort(chy_ctx *ctx)
       hipLaunchKernelGGL((k_reorder<double>), nb, kBlock, 0, ctx->stream, np, ctx->perm, ctx->slotkey, ctx->offsets,
                          ctx->pd[ctx->cur], ctx->pd[1 - ctx->cur], ctx->weighted);
   }
+  timer_stop(ctx, "sort_reorder");
   ctx->cur = 1 - ctx->cur;
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("sort: ") + hipGetErrorString(e));
@@ -621,16 +627,24 @@ int particles_deposit(chy_ctx *ctx)
   const unsigned nb = (unsigned)((ctx->g.nc + kBlock - 1) / kBlock);
   if (ctx->mixed) {
     float *P = (float *)ctx->partials;
+    timer_start(ctx, "deposit_cells");
     hipLaunchKernelGGL((k_cell_partials<float, false>), nb, kBlock, 0, ctx->stream, ctx->g, ctx->offsets,
                        ctx->pf[ctx->cur], ctx->weighted, P);
+    timer_stop(ctx, "deposit_cells");
+    timer_start(ctx, "deposit_nodes");
     hipLaunchKernelGGL((k_node_sum<float>), nb, kBlock, 0, ctx->stream, ctx->g, P, (float *)ctx->mom[0],
                        (float *)ctx->mom[1], (float *)ctx->mom[2], (float *)ctx->mom[3]);
+    timer_stop(ctx, "deposit_nodes");
   } else {
     double *P = (double *)ctx->partials;
+    timer_start(ctx, "deposit_cells");
     hipLaunchKernelGGL((k_cell_partials<double, true>), nb, kBlock, 0, ctx->stream, ctx->g, ctx->offsets,
                        ctx->pd[ctx->cur], ctx->weighted, P);
+    timer_stop(ctx, "deposit_cells");
+    timer_start(ctx, "deposit_nodes");
     hipLaunchKernelGGL((k_node_sum<double>), nb, kBlock, 0, ctx->stream, ctx->g, P, (double *)ctx->mom[0],
                        (double *)ctx->mom[1], (double *)ctx->mom[2], (double *)ctx->mom[3]);
+    timer_stop(ctx, "deposit_nodes");
   }
   ctx->moments_valid = true;
   const hipError_t e = hipGetLastError();

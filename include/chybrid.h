@@ -205,7 +205,9 @@ int chy_output_write(chy_ctx *ctx, const char *stem, const char *insert);
 
 /* ---- instrumentation -------------------------------------------------------
  * Per-kernel HIP-event timing on the context stream (off by default).
- * Names: "push", "sort", "deposit", "lhs_apply", "rhs", "pc", "krylov". */
+ * Names: "efield", "push", "sort_scan", "sort_scatter", "sort_reorder",
+ * "deposit_cells", "deposit_nodes", "rhs", "pc", "lhs_apply", "krylov".
+ * Events are resolved when read, so timing adds no synchronisation. */
 int chy_timing_enable(chy_ctx *ctx, int on);
 int chy_timing_get(chy_ctx *ctx, const char *name, double *total_ms, int64_t *count);
 int chy_timing_reset(chy_ctx *ctx);

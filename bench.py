@@ -89,7 +89,7 @@ def run_gpu(N, ppc, prec, steps, warmup):
         stats = [ctx.step() for _ in range(steps)]
         ctx.sync()
         t1 = time.perf_counter()
-        names = ("efield", "push", "sort_scan", "sort_scatter", "sort_reorder", "deposit_cells",
+        names = ("efield", "push", "collide", "sort_scan", "sort_scatter", "sort_reorder", "deposit_cells",
                  "deposit_nodes", "rhs", "pc", "lhs_apply", "krylov")
         kern = {}
         for nme in names:

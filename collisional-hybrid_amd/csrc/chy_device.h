@@ -103,6 +103,18 @@ __device__ __forceinline__ double mean_rel_speed(double vx, double vy, double vz
   return vnT * (c * exp(-0.5 * lam * lam) + (lam + 1.0 / lam) * erf(lam * 0.70710678118654752440));
 }
 
+__device__ __forceinline__ float mean_rel_speed_f(float vx, float vy, float vz, float n0x, float n0y, float n0z,
+                                                  float vnT)
+{
+  const float ux = vx - n0x, uy = vy - n0y, uz = vz - n0z;
+  const float mu = sqrtf(ux * ux + uy * uy + uz * uz);
+  if (!(vnT > 0.f)) return mu;
+  const float lam = mu / vnT;
+  const float c = 0.79788456080286535588f;
+  if (lam < 1e-2f) return vnT * c * (2.0f + lam * lam / 3.0f);
+  return vnT * (c * expf(-0.5f * lam * lam) + (lam + 1.0f / lam) * erff(lam * 0.70710678118654752440f));
+}
+
 // ---- wave-level helpers (64-wide wavefronts) -----------------------------------
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 

@@ -218,26 +218,27 @@ void launch_jacobi(chy_ctx *ctx, int lhs_type, const T *n, const T *r, T *z)
   hipLaunchKernelGGL((k_jacobi<T>), nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g, lhs_type, s, L, n, r, z);
 }
 
-// Node field E = E0 - grad(phi), central differences (periodic).
+// Node field E = E0 - grad(phi), central differences (periodic), written
+// interleaved {Ex, Ey, Ez, 0} so the gather reads one vector per node.
 template <typename T>
 __global__ __launch_bounds__(256) void k_efield(Grid g, double e0x, double e0y, double e0z, double hx, double hy,
-                                                double hz, const double *__restrict__ phi, T *__restrict__ ex,
-                                                T *__restrict__ ey, T *__restrict__ ez)
+                                                double hz, const double *__restrict__ phi, T *__restrict__ E4)
 {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
-  ex[c] = (T)(e0x - (phi[lin(g, i + 1, j, k)] - phi[lin(g, i - 1, j, k)]) * hx);
-  ey[c] = (T)(e0y - (phi[lin(g, i, j + 1, k)] - phi[lin(g, i, j - 1, k)]) * hy);
-  ez[c] = (T)(e0z - (phi[lin(g, i, j, k + 1)] - phi[lin(g, i, j, k - 1)]) * hz);
+  E4[4 * c + 0] = (T)(e0x - (phi[lin(g, i + 1, j, k)] - phi[lin(g, i - 1, j, k)]) * hx);
+  E4[4 * c + 1] = (T)(e0y - (phi[lin(g, i, j + 1, k)] - phi[lin(g, i, j - 1, k)]) * hy);
+  E4[4 * c + 2] = (T)(e0z - (phi[lin(g, i, j, k + 1)] - phi[lin(g, i, j, k - 1)]) * hz);
+  E4[4 * c + 3] = (T)0;
 }
 
 template <typename T>
-void launch_efield(chy_ctx *ctx, const double *phi, T *ex, T *ey, T *ez)
+void launch_efield(chy_ctx *ctx, const double *phi, T *E4)
 {
   const double *E0 = ctx->prm.E0, *d = ctx->prm.d;
   hipLaunchKernelGGL((k_efield<T>), nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g, E0[0], E0[1], E0[2],
-                     0.5 / d[0], 0.5 / d[1], 0.5 / d[2], phi, ex, ey, ez);
+                     0.5 / d[0], 0.5 / d[1], 0.5 / d[2], phi, E4);
 }
 
 // Full forcing (rhs.c:244-285) with the intended y-z cross derivative.
@@ -527,8 +528,8 @@ template void launch_lhs_apply<float, float, float>(chy_ctx *, int, bool, const 
 template void launch_lhs_apply<float, double, double>(chy_ctx *, int, bool, const float *, const double *, double *);
 template void launch_residual<double, double>(chy_ctx *, int, const double *, const double *, const double *, double *);
 template void launch_residual<float, double>(chy_ctx *, int, const float *, const double *, const double *, double *);
-template void launch_efield<double>(chy_ctx *, const double *, double *, double *, double *);
-template void launch_efield<float>(chy_ctx *, const double *, float *, float *, float *);
+template void launch_efield<double>(chy_ctx *, const double *, double *);
+template void launch_efield<float>(chy_ctx *, const double *, float *);
 template void launch_rhs<double>(chy_ctx *, int, const double *, const double *, const double *, const double *, double *);
 template void launch_rhs<float>(chy_ctx *, int, const float *, const float *, const float *, const float *, double *);
 template void launch_mdot<double>(chy_ctx *, int, const double *, int64_t, const double *, int64_t, double *);

@@ -71,6 +71,7 @@ struct chy_ctx {
   uint32_t *cursor = nullptr;   // nc
   uint32_t *scan_tmp = nullptr; // block sums for the scan
   uint32_t *maxcount = nullptr; // largest cell occupancy of the last sort
+  uint32_t *qlen = nullptr;     // length of the collision queue
   void *partials = nullptr;     // per-cell corner partials [32][nc] (T)
   double *sred = nullptr;       // collision-normaliser block partials
   double *S_dev = nullptr;      // collision normaliser (device scalar)
@@ -78,7 +79,7 @@ struct chy_ctx {
 
   // grid fields
   void *mom[4] = {nullptr, nullptr, nullptr, nullptr}; // n, Gx, Gy, Gz (T)
-  void *efield[3] = {nullptr, nullptr, nullptr};        // node E (T)
+  void *efield[1] = {nullptr};  // node E, interleaved {Ex, Ey, Ez, 0} per node (T)
   double *phi = nullptr, *b = nullptr, *r64 = nullptr, *t64 = nullptr;
   bool phi_valid = false;
   bool moments_valid = false;
@@ -129,7 +130,7 @@ void launch_lhs_apply(chy_ctx *ctx, int lhs_type, bool transpose, const TN *n, c
 template <typename TN, typename TX>
 void launch_residual(chy_ctx *ctx, int lhs_type, const TN *n, const double *b, const TX *x, double *r);
 template <typename T>
-void launch_efield(chy_ctx *ctx, const double *phi, T *ex, T *ey, T *ez);
+void launch_efield(chy_ctx *ctx, const double *phi, T *E4);
 template <typename T>
 void launch_rhs(chy_ctx *ctx, int rhs_type, const T *n, const T *gx, const T *gy, const T *gz, double *b);
 // reductions: out[k] = sum_i a_k[i] * b[i] for k < nv (a_k = A + k*stride)

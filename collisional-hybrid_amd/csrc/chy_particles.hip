@@ -66,39 +66,68 @@ static int64_t collision_count(const chy_params &p, int64_t Np)
   return (int64_t)x;
 }
 
-// ---- one ion-neutral scattering (reference particles.c:529-611) -----------------
-// Per-ion Bernoulli trial with probability Nc |v_r| / S, keyed random numbers.
-// Returns 1 accepted, 2 refused, 0 no collision; sets capped.
+// ---- ion-neutral collisions (reference particles.c:529-611) ---------------------
+// Neutral draw and Bernoulli decision from one Philox block: Box-Muller on the
+// top 24 bits of the four words, the decision uniform from their low bytes.
+// The decision is "collide iff u < Nc |v_r| / S" (probability clipped at 1).
 template <typename T>
-__device__ __forceinline__ int collide_one(const PushConst &pc, double S, uint32_t pid, T &vx, T &vy, T &vz,
-                                           int &capped)
+__device__ __forceinline__ void neutral_draw(const PushConst &pc, uint32_t pid, T &vnx, T &vny, T &vnz, T &udec)
 {
   const U4 a = philox4x32_10(U4{pid, pc.step, STREAM_COLLIDE_A, 0u}, pc.k0, pc.k1);
   const T u0 = u24<T>(a.x), u1 = u24<T>(a.y), u2 = u24<T>(a.z), u3 = u24<T>(a.w);
-  const T twopi = (T)(2.0 * M_PI);
-  const T r1 = sqrt((T)-2.0 * log(u0)), a1 = twopi * u1;
-  const T r2 = sqrt((T)-2.0 * log(u2)), a2 = twopi * u3;
+  udec = u24<T>(((a.x & 0xffu) | ((a.y & 0xffu) << 8) | ((a.z & 0xffu) << 16)) << 8);
+  T r1, r2, c1, s1, c2;
+  if (sizeof(T) == 4) {
+    // hardware transcendentals: ample accuracy for a Monte Carlo neutral draw
+    r1 = __fsqrt_rn(-2.0f * __logf((float)u0));
+    r2 = __fsqrt_rn(-2.0f * __logf((float)u2));
+    const float a1 = 6.28318530717958647692f * (float)u1, a2 = 6.28318530717958647692f * (float)u3;
+    c1 = __cosf(a1); s1 = __sinf(a1); c2 = __cosf(a2);
+  } else {
+    r1 = sqrt(-2.0 * log((double)u0));
+    r2 = sqrt(-2.0 * log((double)u2));
+    const double a1 = 2.0 * M_PI * (double)u1, a2 = 2.0 * M_PI * (double)u3;
+    c1 = cos(a1); s1 = sin(a1); c2 = cos(a2);
+  }
   const T vnT = (T)pc.vnT;
-  const T vnx = (T)pc.vn0[0] + vnT * (r1 * cos(a1));
-  const T vny = (T)pc.vn0[1] + vnT * (r1 * sin(a1));
-  const T vnz = (T)pc.vn0[2] + vnT * (r2 * cos(a2));
+  vnx = (T)pc.vn0[0] + vnT * (r1 * c1);
+  vny = (T)pc.vn0[1] + vnT * (r1 * s1);
+  vnz = (T)pc.vn0[2] + vnT * (r2 * c2);
+}
+
+template <typename T>
+__device__ __forceinline__ bool collide_decide(const PushConst &pc, T S, uint32_t pid, T vx, T vy, T vz, int &capped)
+{
+  T vnx, vny, vnz, ud;
+  neutral_draw<T>(pc, pid, vnx, vny, vnz, ud);
   const T vrx = vx - vnx, vry = vy - vny, vrz = vz - vnz;
   const T vrr = sqrt(vrx * vrx + vry * vry + vrz * vrz);
-  T p = (S > 0.0) ? ((T)pc.Nc * vrr) / (T)S : (T)0;
+  T p = (S > (T)0) ? ((T)pc.Nc * vrr) / S : (T)0;
   capped = 0;
   if (p > (T)1) { p = (T)1; capped = 1; }
+  return ud < p;
+}
+
+// Elastic scattering in the centre-of-mass frame, rotated into the frame of
+// the relative velocity (particles.c:552-611).  Returns 1 accepted, 2 refused.
+template <typename T>
+__device__ __forceinline__ int scatter(const PushConst &pc, uint32_t pid, T &vx, T &vy, T &vz)
+{
+  T vnx, vny, vnz, ud;
+  neutral_draw<T>(pc, pid, vnx, vny, vnz, ud);
+  const T vrx = vx - vnx, vry = vy - vny, vrz = vz - vnz;
+  const T vrr = sqrt(vrx * vrx + vry * vry + vrz * vrz);
   const U4 b = philox4x32_10(U4{pid, pc.step, STREAM_COLLIDE_B, 0u}, pc.k0, pc.k1);
-  const T ud = u24<T>(b.x);
-  if (!(ud < p)) return 0;
+  const T twopi = (T)(2.0 * M_PI);
   const T mi = (T)pc.mi, mn = (T)pc.mn, M = mn + mi;
   const T vcx = (vx * mi + vnx * mn) / M;
   const T vcy = (vy * mi + vny * mn) / M;
   const T vcz = (vz * mi + vnz * mn) / M;
   const T ex = vx - vcx, ey = vy - vcy, ez = vz - vcz;
   const T vcr = sqrt(ex * ex + ey * ey + ez * ez);
-  const T uz = (T)2.0 * u24<T>(b.y) - (T)1.0;
+  const T uz = (T)2.0 * u24<T>(b.x) - (T)1.0;
   const T uperp = sqrt((T)1.0 - uz * uz);
-  const T uphi = twopi * u24<T>(b.z);
+  const T uphi = twopi * u24<T>(b.y);
   const T ux = uperp * cos(uphi), uy = uperp * sin(uphi);
   const T costht = vrz / vrr;
   const T sintht = sqrt((T)1.0 - costht * costht);
@@ -136,170 +165,317 @@ __device__ __forceinline__ void count_key(uint32_t *counts, int key)
   if (key >= 0 && lane_id() == head) atomicAdd(&counts[key], (uint32_t)len);
 }
 
-__device__ __forceinline__ void count_stats(unsigned long long *cs, int r, int capped, bool bad)
+__device__ __forceinline__ void count_flags(unsigned long long *cs, int slot, bool f)
 {
-  const uint64_t acc = __ballot(r == 1), ref = __ballot(r == 2), cap = __ballot(capped != 0), nf = __ballot(bad);
-  if (lane_id() == 0) {
-    if (acc) atomicAdd(&cs[0], (unsigned long long)__popcll(acc));
-    if (ref) atomicAdd(&cs[1], (unsigned long long)__popcll(ref));
-    if (cap) atomicAdd(&cs[2], (unsigned long long)__popcll(cap));
-    if (nf) atomicAdd(&cs[3], (unsigned long long)__popcll(nf));
+  const uint64_t m = __ballot(f);
+  if (m && lane_id() == 0) atomicAdd(&cs[slot], (unsigned long long)__popcll(m));
+}
+
+// Collision queue: lanes with `flag` append their particle index to a block
+// buffer in LDS (LDS atomics only); the buffer is flushed to the global queue
+// with one global atomic per flush, so the single global counter sees a few
+// thousand atomics per launch instead of one per wave.
+constexpr uint32_t kQBuf = 4096;
+struct QueueLds { uint32_t buf[kQBuf]; uint32_t n, base; };
+
+__device__ __forceinline__ void queue_flush(QueueLds &q, uint32_t *queue, uint32_t *qlen)
+{
+  // block-uniform: every thread calls this after a barrier
+  if (threadIdx.x == 0) q.base = q.n ? atomicAdd(qlen, q.n) : 0u;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < q.n; i += blockDim.x) queue[q.base + i] = q.buf[i];
+  __syncthreads();
+  if (threadIdx.x == 0) q.n = 0;
+  __syncthreads();
+}
+
+__device__ __forceinline__ void queue_push(QueueLds &q, uint32_t *queue, uint32_t *qlen, bool flag, uint32_t p)
+{
+  const uint64_t m = __ballot(flag);
+  if (m) {
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(&q.n, (uint32_t)__popcll(m));
+    base = __shfl(base, 0, 64);
+    if (flag) q.buf[base + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull))] = p;
+  }
+  __syncthreads();
+  if (q.n > kQBuf - 256) queue_flush(q, queue, qlen);  // room for one more block of appends
+}
+
+template <typename T> struct alignas(4 * sizeof(T)) Vec4 { T x, y, z, w; };
+
+// Boris rotation (particles.c:374-447 with the intended cross products and
+// per-component v-minus); operation order shared with the oracle.
+template <typename T>
+__device__ __forceinline__ void boris(const PushConst &pc, T e0, T e1, T e2, T &vx, T &vy, T &vz)
+{
+  const T tau = (T)pc.tau, t0 = (T)pc.t[0], t1 = (T)pc.t[1], t2 = (T)pc.t[2];
+  const T s0 = (T)pc.s[0], s1 = (T)pc.s[1], s2 = (T)pc.s[2];
+  const T vm0 = vx + tau * e0, vm1 = vy + tau * e1, vm2 = vz + tau * e2;
+  const T vp0 = vm0 + (vm1 * t2 - vm2 * t1);
+  const T vp1 = vm1 + (vm2 * t0 - vm0 * t2);
+  const T vp2 = vm2 + (vm0 * t1 - vm1 * t0);
+  const T vq0 = vm0 + (vp1 * s2 - vp2 * s1);
+  const T vq1 = vm1 + (vp2 * s0 - vp0 * s2);
+  const T vq2 = vm2 + (vp0 * s1 - vp1 * s0);
+  vx = vq0 + tau * e0;
+  vy = vq1 + tau * e1;
+  vz = vq2 + tau * e2;
+}
+
+// Trilinear gather from the interleaved node field, corner order q = 0..7.
+template <typename T>
+__device__ __forceinline__ void gather(const Grid &g, const Vec4<T> *__restrict__ E4, int i, int j, int k, T fx, T fy,
+                                       T fz, T &e0, T &e1, T &e2)
+{
+  T W[8];
+  corner_weights<T>(fx, fy, fz, W);
+  e0 = (T)0; e1 = (T)0; e2 = (T)0;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const Vec4<T> e = E4[lin(g, i + (q & 1), j + ((q >> 1) & 1), k + (q >> 2))];
+    e0 += W[q] * e.x;
+    e1 += W[q] * e.y;
+    e2 += W[q] * e.z;
   }
 }
 
-// ---- FP64 push ------------------------------------------------------------------
-template <bool COLLIDE, bool MOVE, bool GATHER>
-__global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t np, PState<double> st,
-                                                   const double *__restrict__ ex, const double *__restrict__ ey,
-                                                   const double *__restrict__ ez, const double *__restrict__ Sdev,
-                                                   uint32_t *__restrict__ counts, double *__restrict__ sred,
-                                                   unsigned long long *__restrict__ cstats)
+// FP64 absolute move and cell key.
+__device__ __forceinline__ int move_fp64(const Grid &g, double dt, double &x, double &y, double &z, double vx,
+                                         double vy, double vz)
 {
-  __shared__ double sh[4];
+  x = wrap_pos(x + vx * dt, g.p0[0], g.p1[0], g.L[0]);
+  y = wrap_pos(y + vy * dt, g.p0[1], g.p1[1], g.L[1]);
+  z = wrap_pos(z + vz * dt, g.p0[2], g.p1[2], g.L[2]);
+  int i, j, k;
+  double f;
+  cell_coord(x, g.p0[0], g.d[0], g.nx, i, f);
+  cell_coord(y, g.p0[1], g.d[1], g.ny, j, f);
+  cell_coord(z, g.p0[2], g.d[2], g.nz, k, f);
+  return (int)(i + (int64_t)g.nx * (j + (int64_t)g.ny * k));
+}
+
+// Mixed move: in-cell offset += v dt / d with an exact integer carry.
+__device__ __forceinline__ int move_mixed(const Grid &g, const PushConst &pc, int c, float &fx, float &fy, float &fz,
+                                          float vx, float vy, float vz)
+{
+  float o[3] = {fmaf(vx, (float)pc.cdt[0], fx), fmaf(vy, (float)pc.cdt[1], fy), fmaf(vz, (float)pc.cdt[2], fz)};
+  int ci[3] = {c % g.nx, (c / g.nx) % g.ny, c / (g.nx * g.ny)};
+  const int nn[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float fl = floorf(o[a]);
+    float f = o[a] - fl;
+    int sh = (int)fl;
+    if (f >= 1.0f) { f -= 1.0f; sh += 1; }
+    int cc = (ci[a] + sh) % nn[a];
+    if (cc < 0) cc += nn[a];
+    ci[a] = cc;
+    o[a] = f;
+  }
+  fx = o[0]; fy = o[1]; fz = o[2];
+  return (int)(ci[0] + (int64_t)g.nx * (ci[1] + (int64_t)g.ny * ci[2]));
+}
+
+// ---- main push: gather + Boris + collision decision + move ----------------------
+// Ions that collide are queued; their scattering and move run in k_collide.
+template <bool COLLIDE, bool MOVE>
+__global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t np, PState<double> st,
+                                                   const Vec4<double> *__restrict__ E4, const double *__restrict__ Sdev,
+                                                   uint32_t *__restrict__ counts, uint32_t *__restrict__ queue,
+                                                   uint32_t *__restrict__ qlen, unsigned long long *__restrict__ cstats)
+{
   const double S = COLLIDE ? Sdev[0] : 0.0;
-  double sacc = 0.0;
+  __shared__ QueueLds qs;
+  if (COLLIDE) {
+    if (threadIdx.x == 0) qs.n = 0;
+    __syncthreads();
+  }
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   // every lane iterates the same number of times so wave-level collectives stay convergent
   const int64_t iters = (np + stride - 1) / stride;
   for (int64_t it = 0; it < iters; it++) {
     const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = p < np;
-    int key = -1, r = 0, capped = 0;
-    bool bad = false;
+    int key = -1, capped = 0;
+    bool coll = false, bad = false;
     if (live) {
       double x = st.x[p], y = st.y[p], z = st.z[p];
       double vx = st.vx[p], vy = st.vy[p], vz = st.vz[p];
-      if (GATHER) {
-        int i, j, k;
-        double fx, fy, fz, W[8];
-        cell_coord(x, g.p0[0], g.d[0], g.nx, i, fx);
-        cell_coord(y, g.p0[1], g.d[1], g.ny, j, fy);
-        cell_coord(z, g.p0[2], g.d[2], g.nz, k, fz);
-        corner_weights<double>(fx, fy, fz, W);
-        double e0 = 0.0, e1 = 0.0, e2 = 0.0;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-          const int64_t c = lin(g, i + (q & 1), j + ((q >> 1) & 1), k + (q >> 2));
-          e0 += W[q] * ex[c];
-          e1 += W[q] * ey[c];
-          e2 += W[q] * ez[c];
-        }
-        const double tau = pc.tau, t0 = pc.t[0], t1 = pc.t[1], t2 = pc.t[2];
-        const double s0 = pc.s[0], s1 = pc.s[1], s2 = pc.s[2];
-        const double vm0 = vx + tau * e0, vm1 = vy + tau * e1, vm2 = vz + tau * e2;
-        const double vp0 = vm0 + (vm1 * t2 - vm2 * t1);
-        const double vp1 = vm1 + (vm2 * t0 - vm0 * t2);
-        const double vp2 = vm2 + (vm0 * t1 - vm1 * t0);
-        const double vq0 = vm0 + (vp1 * s2 - vp2 * s1);
-        const double vq1 = vm1 + (vp2 * s0 - vp0 * s2);
-        const double vq2 = vm2 + (vp0 * s1 - vp1 * s0);
-        vx = vq0 + tau * e0;
-        vy = vq1 + tau * e1;
-        vz = vq2 + tau * e2;
-        if (COLLIDE) r = collide_one<double>(pc, S, st.pid[p], vx, vy, vz, capped);
-        st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
-      }
-      if (MOVE) {
-        x = wrap_pos(x + vx * pc.dt, g.p0[0], g.p1[0], g.L[0]);
-        y = wrap_pos(y + vy * pc.dt, g.p0[1], g.p1[1], g.L[1]);
-        z = wrap_pos(z + vz * pc.dt, g.p0[2], g.p1[2], g.L[2]);
+      int i, j, k;
+      double fx, fy, fz, e0, e1, e2;
+      cell_coord(x, g.p0[0], g.d[0], g.nx, i, fx);
+      cell_coord(y, g.p0[1], g.d[1], g.ny, j, fy);
+      cell_coord(z, g.p0[2], g.d[2], g.nz, k, fz);
+      gather<double>(g, E4, i, j, k, fx, fy, fz, e0, e1, e2);
+      boris<double>(pc, e0, e1, e2, vx, vy, vz);
+      if (COLLIDE) coll = collide_decide<double>(pc, S, st.pid[p], vx, vy, vz, capped);
+      st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
+      if (MOVE && !coll) {
+        key = move_fp64(g, pc.dt, x, y, z, vx, vy, vz);
         st.x[p] = x; st.y[p] = y; st.z[p] = z;
-        int i, j, k;
-        double f;
-        cell_coord(x, g.p0[0], g.d[0], g.nx, i, f);
-        cell_coord(y, g.p0[1], g.d[1], g.ny, j, f);
-        cell_coord(z, g.p0[2], g.d[2], g.nz, k, f);
-        key = (int)(i + (int64_t)g.nx * (j + (int64_t)g.ny * k));
         st.cell[p] = key;
       }
       bad = !isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(x) || !isfinite(y) || !isfinite(z);
-      if (COLLIDE) sacc += mean_rel_speed(vx, vy, vz, pc.vn0[0], pc.vn0[1], pc.vn0[2], pc.vnT);
     }
+    if (COLLIDE) queue_push(qs, queue, qlen, coll, (uint32_t)p);
     if (MOVE) count_key(counts, key);
-    count_stats(cstats, r, capped, bad);
+    if (COLLIDE) count_flags(cstats, 2, capped != 0);
+    count_flags(cstats, 3, bad);
   }
-  if (COLLIDE) {
-    const double s = block_sum(sacc, sh);
-    if (threadIdx.x == 0) sred[blockIdx.x] = s;
-  }
+  if (COLLIDE) queue_flush(qs, queue, qlen);
 }
 
-// ---- mixed push (FP32 state: cell + offset, FP32 v) --------------------------------
-template <bool COLLIDE, bool MOVE, bool GATHER>
+template <bool COLLIDE, bool MOVE>
 __global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConst pc, int64_t np, PState<float> st,
-                                                    const float *__restrict__ ex, const float *__restrict__ ey,
-                                                    const float *__restrict__ ez, const double *__restrict__ Sdev,
-                                                    uint32_t *__restrict__ counts, double *__restrict__ sred,
-                                                    unsigned long long *__restrict__ cstats)
+                                                    const Vec4<float> *__restrict__ E4, const double *__restrict__ Sdev,
+                                                    uint32_t *__restrict__ counts, uint32_t *__restrict__ queue,
+                                                    uint32_t *__restrict__ qlen, unsigned long long *__restrict__ cstats)
 {
-  __shared__ double sh[4];
-  const double S = COLLIDE ? Sdev[0] : 0.0;
-  double sacc = 0.0;
+  const float S = COLLIDE ? (float)Sdev[0] : 0.f;
+  __shared__ QueueLds qs;
+  if (COLLIDE) {
+    if (threadIdx.x == 0) qs.n = 0;
+    __syncthreads();
+  }
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t iters = (np + stride - 1) / stride;
-  const float tau = (float)pc.tau, t0 = (float)pc.t[0], t1 = (float)pc.t[1], t2 = (float)pc.t[2];
-  const float s0 = (float)pc.s[0], s1 = (float)pc.s[1], s2 = (float)pc.s[2];
   for (int64_t it = 0; it < iters; it++) {
     const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = p < np;
-    int key = -1, r = 0, capped = 0;
-    bool bad = false;
+    int key = -1, capped = 0;
+    bool coll = false, bad = false;
     if (live) {
       const int c = st.cell[p];
-      int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny);
       float fx = st.x[p], fy = st.y[p], fz = st.z[p];
       float vx = st.vx[p], vy = st.vy[p], vz = st.vz[p];
-      if (GATHER) {
-        float W[8];
-        corner_weights<float>(fx, fy, fz, W);
-        float e0 = 0.f, e1 = 0.f, e2 = 0.f;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-          const int64_t cc = lin(g, i + (q & 1), j + ((q >> 1) & 1), k + (q >> 2));
-          e0 = fmaf(W[q], ex[cc], e0);
-          e1 = fmaf(W[q], ey[cc], e1);
-          e2 = fmaf(W[q], ez[cc], e2);
-        }
-        const float vm0 = fmaf(tau, e0, vx), vm1 = fmaf(tau, e1, vy), vm2 = fmaf(tau, e2, vz);
-        const float vp0 = vm0 + (vm1 * t2 - vm2 * t1);
-        const float vp1 = vm1 + (vm2 * t0 - vm0 * t2);
-        const float vp2 = vm2 + (vm0 * t1 - vm1 * t0);
-        vx = fmaf(tau, e0, vm0 + (vp1 * s2 - vp2 * s1));
-        vy = fmaf(tau, e1, vm1 + (vp2 * s0 - vp0 * s2));
-        vz = fmaf(tau, e2, vm2 + (vp0 * s1 - vp1 * s0));
-        if (COLLIDE) r = collide_one<float>(pc, S, st.pid[p], vx, vy, vz, capped);
-        st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
-      }
-      if (MOVE) {
-        // offset += v dt / d with an exact integer carry into the cell index
-        float o[3] = {fmaf(vx, (float)pc.cdt[0], fx), fmaf(vy, (float)pc.cdt[1], fy), fmaf(vz, (float)pc.cdt[2], fz)};
-        int ci[3] = {i, j, k};
-        const int nn[3] = {g.nx, g.ny, g.nz};
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-          float fl = floorf(o[a]);
-          float f = o[a] - fl;
-          int sh_ = (int)fl;
-          if (f >= 1.0f) { f -= 1.0f; sh_ += 1; }
-          int cc = (ci[a] + sh_) % nn[a];
-          if (cc < 0) cc += nn[a];
-          ci[a] = cc;
-          o[a] = f;
-        }
-        st.x[p] = o[0]; st.y[p] = o[1]; st.z[p] = o[2];
-        key = (int)(ci[0] + (int64_t)g.nx * (ci[1] + (int64_t)g.ny * ci[2]));
+      float e0, e1, e2;
+      gather<float>(g, E4, c % g.nx, (c / g.nx) % g.ny, c / (g.nx * g.ny), fx, fy, fz, e0, e1, e2);
+      boris<float>(pc, e0, e1, e2, vx, vy, vz);
+      if (COLLIDE) coll = collide_decide<float>(pc, S, st.pid[p], vx, vy, vz, capped);
+      st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
+      if (MOVE && !coll) {
+        key = move_mixed(g, pc, c, fx, fy, fz, vx, vy, vz);
+        st.x[p] = fx; st.y[p] = fy; st.z[p] = fz;
         st.cell[p] = key;
-        fx = o[0]; fy = o[1]; fz = o[2];
       }
       bad = !isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(fx) || !isfinite(fy) || !isfinite(fz);
-      if (COLLIDE) sacc += mean_rel_speed(vx, vy, vz, pc.vn0[0], pc.vn0[1], pc.vn0[2], pc.vnT);
     }
+    if (COLLIDE) queue_push(qs, queue, qlen, coll, (uint32_t)p);
     if (MOVE) count_key(counts, key);
-    count_stats(cstats, r, capped, bad);
+    if (COLLIDE) count_flags(cstats, 2, capped != 0);
+    count_flags(cstats, 3, bad);
   }
-  if (COLLIDE) {
-    const double s = block_sum(sacc, sh);
-    if (threadIdx.x == 0) sred[blockIdx.x] = s;
+  if (COLLIDE) queue_flush(qs, queue, qlen);
+}
+
+// Scattering, move and key of the queued ions.
+template <bool MOVE>
+__global__ __launch_bounds__(256) void k_collide_fp64(Grid g, PushConst pc, PState<double> st,
+                                                      const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qlen,
+                                                      uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats)
+{
+  __shared__ unsigned int sacc[4];
+  if (threadIdx.x < 4) sacc[threadIdx.x] = 0u;
+  __syncthreads();
+  unsigned int nacc = 0, nref = 0, nbad = 0;
+  const uint32_t n = *qlen;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const uint32_t p = queue[q];
+    double vx = st.vx[p], vy = st.vy[p], vz = st.vz[p];
+    const int r = scatter<double>(pc, st.pid[p], vx, vy, vz);
+    st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
+    if (r == 1) nacc++; else nref++;
+    if (MOVE) {
+      double x = st.x[p], y = st.y[p], z = st.z[p];
+      const int key = move_fp64(g, pc.dt, x, y, z, vx, vy, vz);
+      st.x[p] = x; st.y[p] = y; st.z[p] = z;
+      st.cell[p] = key;
+      atomicAdd(&counts[key], 1u);
+      if (!isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(x) || !isfinite(y) || !isfinite(z))
+        nbad++;
+    }
+  }
+  if (nacc) atomicAdd(&sacc[0], nacc);
+  if (nref) atomicAdd(&sacc[1], nref);
+  if (nbad) atomicAdd(&sacc[3], nbad);
+  __syncthreads();
+  if (threadIdx.x < 4 && sacc[threadIdx.x]) atomicAdd(&cstats[threadIdx.x], (unsigned long long)sacc[threadIdx.x]);
+}
+
+template <bool MOVE>
+__global__ __launch_bounds__(256) void k_collide_mixed(Grid g, PushConst pc, PState<float> st,
+                                                       const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qlen,
+                                                       uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats)
+{
+  __shared__ unsigned int sacc[4];
+  if (threadIdx.x < 4) sacc[threadIdx.x] = 0u;
+  __syncthreads();
+  unsigned int nacc = 0, nref = 0, nbad = 0;
+  const uint32_t n = *qlen;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const uint32_t p = queue[q];
+    float vx = st.vx[p], vy = st.vy[p], vz = st.vz[p];
+    const int r = scatter<float>(pc, st.pid[p], vx, vy, vz);
+    st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
+    if (r == 1) nacc++; else nref++;
+    if (MOVE) {
+      float fx = st.x[p], fy = st.y[p], fz = st.z[p];
+      const int key = move_mixed(g, pc, st.cell[p], fx, fy, fz, vx, vy, vz);
+      st.x[p] = fx; st.y[p] = fy; st.z[p] = fz;
+      st.cell[p] = key;
+      atomicAdd(&counts[key], 1u);
+      if (!isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(fx) || !isfinite(fy) || !isfinite(fz))
+        nbad++;
+    }
+  }
+  if (nacc) atomicAdd(&sacc[0], nacc);
+  if (nref) atomicAdd(&sacc[1], nref);
+  if (nbad) atomicAdd(&sacc[3], nbad);
+  __syncthreads();
+  if (threadIdx.x < 4 && sacc[threadIdx.x]) atomicAdd(&cstats[threadIdx.x], (unsigned long long)sacc[threadIdx.x]);
+}
+
+// Move only (positions from the current velocities), with keys and counts.
+__global__ __launch_bounds__(256) void k_move_fp64(Grid g, PushConst pc, int64_t np, PState<double> st,
+                                                   uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats)
+{
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t iters = (np + stride - 1) / stride;
+  for (int64_t it = 0; it < iters; it++) {
+    const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int key = -1;
+    bool bad = false;
+    if (p < np) {
+      double x = st.x[p], y = st.y[p], z = st.z[p];
+      key = move_fp64(g, pc.dt, x, y, z, st.vx[p], st.vy[p], st.vz[p]);
+      st.x[p] = x; st.y[p] = y; st.z[p] = z;
+      st.cell[p] = key;
+      bad = !isfinite(x) || !isfinite(y) || !isfinite(z);
+    }
+    count_key(counts, key);
+    count_flags(cstats, 3, bad);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_move_mixed(Grid g, PushConst pc, int64_t np, PState<float> st,
+                                                    uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats)
+{
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t iters = (np + stride - 1) / stride;
+  for (int64_t it = 0; it < iters; it++) {
+    const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int key = -1;
+    bool bad = false;
+    if (p < np) {
+      float fx = st.x[p], fy = st.y[p], fz = st.z[p];
+      key = move_mixed(g, pc, st.cell[p], fx, fy, fz, st.vx[p], st.vy[p], st.vz[p]);
+      st.x[p] = fx; st.y[p] = fy; st.z[p] = fz;
+      st.cell[p] = key;
+      bad = !isfinite(fx) || !isfinite(fy) || !isfinite(fz);
+    }
+    count_key(counts, key);
+    count_flags(cstats, 3, bad);
   }
 }
 
@@ -449,65 +625,114 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t np, const int32_t *__re
 }
 
 // Stable order within each cell: rank of a slot's source index among its
-// cell's sources; moves the particle from generation `a` to `b`.
-template <typename T>
+// cell's sources; moves the particle from generation `a` to `b`.  With SUMS
+// it also accumulates the collision normaliser over the final velocities in
+// this deterministic (sorted) order, one partial per block.
+template <typename T, bool SUMS>
 __global__ __launch_bounds__(256) void k_reorder(int64_t np, const uint32_t *__restrict__ perm,
                                                  const int32_t *__restrict__ slotkey, const uint32_t *__restrict__ offs,
-                                                 PState<T> a, PState<T> b, bool weighted)
+                                                 PState<T> a, PState<T> b, bool weighted, double n0x, double n0y,
+                                                 double n0z, double vnT, double *__restrict__ sred)
 {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= np) return;
-  const int c = slotkey[s];
-  const uint32_t lo = offs[c], hi = offs[c + 1];
-  const uint32_t me = perm[s];
-  uint32_t rank = 0;
-  for (uint32_t t = lo; t < hi; t++) rank += perm[t] < me ? 1u : 0u;
-  const uint32_t d = lo + rank;
-  b.x[d] = a.x[me]; b.y[d] = a.y[me]; b.z[d] = a.z[me];
-  b.vx[d] = a.vx[me]; b.vy[d] = a.vy[me]; b.vz[d] = a.vz[me];
-  b.pid[d] = a.pid[me];
-  b.cell[d] = c;
-  if (weighted) b.w[d] = a.w[me];
+  __shared__ double sh[4];
+  double acc = 0.0;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < np; s += (int64_t)gridDim.x * blockDim.x) {
+    const int c = slotkey[s];
+    const uint32_t lo = offs[c], hi = offs[c + 1];
+    const uint32_t me = perm[s];
+    uint32_t rank = 0;
+    for (uint32_t t = lo; t < hi; t++) rank += perm[t] < me ? 1u : 0u;
+    const uint32_t d = lo + rank;
+    const T vx = a.vx[me], vy = a.vy[me], vz = a.vz[me];
+    b.x[d] = a.x[me]; b.y[d] = a.y[me]; b.z[d] = a.z[me];
+    b.vx[d] = vx; b.vy[d] = vy; b.vz[d] = vz;
+    b.pid[d] = a.pid[me];
+    b.cell[d] = c;
+    if (weighted) b.w[d] = a.w[me];
+    if (SUMS) {
+      if (sizeof(T) == 4) acc += (double)mean_rel_speed_f((float)vx, (float)vy, (float)vz, (float)n0x, (float)n0y,
+                                                         (float)n0z, (float)vnT);
+      else acc += mean_rel_speed((double)vx, (double)vy, (double)vz, n0x, n0y, n0z, vnT);
+    }
+  }
+  if (SUMS) {
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) sred[blockIdx.x] = t;
+  }
 }
 
 // ---- deposit ---------------------------------------------------------------------
 // Per-cell corner partials P[(4q+m)*nc + c], m in {w, w vx, w vy, w vz}.
+// A block owns kDepCells consecutive cells, whose particles are contiguous
+// after the sort; they are staged through LDS in coalesced chunks.  Four
+// threads share a cell, each summing two corners over the cell's particles in
+// sorted order (the oracle's order), so the result is deterministic.
+constexpr int kDepCells = 64;
+constexpr int kDepChunk = 1024;
+__device__ __forceinline__ int skew(int e) { return e + (e >> 4); }  // LDS bank spread
+
 template <typename T, bool ABS>
 __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *__restrict__ offs, PState<T> st,
                                                        bool weighted, T *__restrict__ P)
 {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.nc) return;
-  T acc[32];
+  constexpr int L = kDepChunk + kDepChunk / 16;
+  __shared__ T sf[7][L];
+  __shared__ uint32_t so[kDepCells + 1];
+  const int64_t c0 = (int64_t)blockIdx.x * kDepCells;
+  const int ncell = (int)((g.nc - c0) < kDepCells ? (g.nc - c0) : kDepCells);
+  if (threadIdx.x <= ncell) so[threadIdx.x] = offs[c0 + threadIdx.x];
+  __syncthreads();
+  const uint32_t s0 = so[0], s1 = so[ncell];
+  const int cl = threadIdx.x >> 2, part = threadIdx.x & 3;
+  const bool own = cl < ncell;
+  const uint32_t lo = own ? so[cl] : 0, hi = own ? so[cl + 1] : 0;
+  T acc[8];
 #pragma unroll
-  for (int q = 0; q < 32; q++) acc[q] = (T)0;
-  const uint32_t lo = offs[c], hi = offs[c + 1];
-  for (uint32_t s = lo; s < hi; s++) {
-    T fx, fy, fz;
-    if (ABS) {
-      int i;
-      double f;
-      cell_coord((double)st.x[s], g.p0[0], g.d[0], g.nx, i, f); fx = (T)f;
-      cell_coord((double)st.y[s], g.p0[1], g.d[1], g.ny, i, f); fy = (T)f;
-      cell_coord((double)st.z[s], g.p0[2], g.d[2], g.nz, i, f); fz = (T)f;
-    } else {
-      fx = st.x[s]; fy = st.y[s]; fz = st.z[s];
+  for (int q = 0; q < 8; q++) acc[q] = (T)0;
+  for (uint32_t base = s0; base < s1; base += kDepChunk) {
+    const int m = (int)((s1 - base) < (uint32_t)kDepChunk ? (s1 - base) : (uint32_t)kDepChunk);
+    for (int e = threadIdx.x; e < m; e += blockDim.x) {
+      const uint32_t s = base + e;
+      const int k = skew(e);
+      if (ABS) {
+        int i;
+        double f;
+        cell_coord((double)st.x[s], g.p0[0], g.d[0], g.nx, i, f); sf[0][k] = (T)f;
+        cell_coord((double)st.y[s], g.p0[1], g.d[1], g.ny, i, f); sf[1][k] = (T)f;
+        cell_coord((double)st.z[s], g.p0[2], g.d[2], g.nz, i, f); sf[2][k] = (T)f;
+      } else {
+        sf[0][k] = st.x[s]; sf[1][k] = st.y[s]; sf[2][k] = st.z[s];
+      }
+      sf[3][k] = st.vx[s]; sf[4][k] = st.vy[s]; sf[5][k] = st.vz[s];
+      sf[6][k] = weighted ? st.w[s] : (T)1;
     }
-    T W[8];
-    corner_weights<T>(fx, fy, fz, W);
-    const T vx = st.vx[s], vy = st.vy[s], vz = st.vz[s];
-    const T wt = weighted ? st.w[s] : (T)1;
+    __syncthreads();
+    const uint32_t a0 = lo > base ? lo : base;
+    const uint32_t a1 = hi < base + m ? hi : base + m;
+    for (uint32_t s = a0; s < a1; s++) {
+      const int k = skew((int)(s - base));
+      const T fx = sf[0][k], fy = sf[1][k], fz = sf[2][k];
+      const T vx = sf[3][k], vy = sf[4][k], vz = sf[5][k], wt = sf[6][k];
+      const T wx[2] = {(T)1 - fx, fx}, wy[2] = {(T)1 - fy, fy}, wz[2] = {(T)1 - fz, fz};
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const T a = weighted ? wt * W[q] : W[q];
-      acc[4 * q + 0] += a;
-      acc[4 * q + 1] += a * vx;
-      acc[4 * q + 2] += a * vy;
-      acc[4 * q + 3] += a * vz;
+      for (int h = 0; h < 2; h++) {
+        const int q = 2 * part + h;
+        const T W = (wz[q >> 2] * wy[(q >> 1) & 1]) * wx[q & 1];
+        const T a = weighted ? wt * W : W;
+        acc[4 * h + 0] += a;
+        acc[4 * h + 1] += a * vx;
+        acc[4 * h + 2] += a * vy;
+        acc[4 * h + 3] += a * vz;
+      }
     }
+    __syncthreads();
   }
+  if (own) {
 #pragma unroll
-  for (int q = 0; q < 32; q++) P[(int64_t)q * g.nc + c] = acc[q];
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int mm = 0; mm < 4; mm++) P[(int64_t)(4 * (2 * part + h) + mm) * g.nc + c0 + cl] = acc[4 * h + mm];
+  }
 }
 
 // Node moments: sum over the 8 cells sharing the node, corner order q = 0..7.
@@ -605,17 +830,18 @@ int particles_sort(chy_ctx *ctx)
   if (mx > kMaxCellOccupancy)
     return set_err(ctx, CHY_ERR_SUP, "a cell holds " + std::to_string(mx) + " particles; at most " +
                                          std::to_string(kMaxCellOccupancy) + " are supported");
-  const unsigned nb = (unsigned)((np + kBlock - 1) / kBlock);
+  const unsigned nb = grid_for(np);
+  const chy_params &p = ctx->prm;
+  const bool sums = p.mn > 0.0;  // the collision normaliser is needed
   timer_start(ctx, "sort_reorder");
-  if (nb) {
-    if (ctx->mixed)
-      hipLaunchKernelGGL((k_reorder<float>), nb, kBlock, 0, ctx->stream, np, ctx->perm, ctx->slotkey, ctx->offsets,
-                         ctx->pf[ctx->cur], ctx->pf[1 - ctx->cur], ctx->weighted);
-    else
-      hipLaunchKernelGGL((k_reorder<double>), nb, kBlock, 0, ctx->stream, np, ctx->perm, ctx->slotkey, ctx->offsets,
-                         ctx->pd[ctx->cur], ctx->pd[1 - ctx->cur], ctx->weighted);
-  }
+#define REORDER(T, S, st)                                                                                  \
+  hipLaunchKernelGGL((k_reorder<T, S>), nb, kBlock, 0, ctx->stream, np, ctx->perm, ctx->slotkey, ctx->offsets, \
+                     st[ctx->cur], st[1 - ctx->cur], ctx->weighted, p.vn0[0], p.vn0[1], p.vn0[2], p.vnT, ctx->sred)
+  if (ctx->mixed) { if (sums) REORDER(float, true, ctx->pf); else REORDER(float, false, ctx->pf); }
+  else { if (sums) REORDER(double, true, ctx->pd); else REORDER(double, false, ctx->pd); }
+#undef REORDER
   timer_stop(ctx, "sort_reorder");
+  if (sums) launch_combine_sum(ctx, ctx->sred, (int)nb, 1, ctx->S_dev);
   ctx->cur = 1 - ctx->cur;
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("sort: ") + hipGetErrorString(e));
@@ -625,10 +851,11 @@ int particles_sort(chy_ctx *ctx)
 int particles_deposit(chy_ctx *ctx)
 {
   const unsigned nb = (unsigned)((ctx->g.nc + kBlock - 1) / kBlock);
+  const unsigned nbc = (unsigned)((ctx->g.nc + kDepCells - 1) / kDepCells);
   if (ctx->mixed) {
     float *P = (float *)ctx->partials;
     timer_start(ctx, "deposit_cells");
-    hipLaunchKernelGGL((k_cell_partials<float, false>), nb, kBlock, 0, ctx->stream, ctx->g, ctx->offsets,
+    hipLaunchKernelGGL((k_cell_partials<float, false>), nbc, kBlock, 0, ctx->stream, ctx->g, ctx->offsets,
                        ctx->pf[ctx->cur], ctx->weighted, P);
     timer_stop(ctx, "deposit_cells");
     timer_start(ctx, "deposit_nodes");
@@ -638,7 +865,7 @@ int particles_deposit(chy_ctx *ctx)
   } else {
     double *P = (double *)ctx->partials;
     timer_start(ctx, "deposit_cells");
-    hipLaunchKernelGGL((k_cell_partials<double, true>), nb, kBlock, 0, ctx->stream, ctx->g, ctx->offsets,
+    hipLaunchKernelGGL((k_cell_partials<double, true>), nbc, kBlock, 0, ctx->stream, ctx->g, ctx->offsets,
                        ctx->pd[ctx->cur], ctx->weighted, P);
     timer_stop(ctx, "deposit_cells");
     timer_start(ctx, "deposit_nodes");
@@ -872,28 +1099,43 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
   pc.Nc = (double)Nc;
   if (collide && Nc == 0) collide = false;
   launch_fill<unsigned long long>(ctx, 4, ctx->cstats, 0ull);
+  launch_fill<uint32_t>(ctx, 1, ctx->qlen, 0u);
   if (move) launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
   const unsigned nb = grid_for(ctx->np);
+  // the queue of colliding ions lives in the sort's permutation buffer, which
+  // is free until the sort that follows
+  uint32_t *queue = ctx->perm;
   timer_start(ctx, "push");
-#define LAUNCH(KER, T, C, M, G)                                                                             \
-  hipLaunchKernelGGL((KER<C, M, G>), nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, st, (const T *)ctx->efield[0], \
-                     (const T *)ctx->efield[1], (const T *)ctx->efield[2], ctx->S_dev, ctx->counts, ctx->sred,      \
-                     ctx->cstats)
+#define LAUNCH(KER, T, C, M)                                                                                  \
+  hipLaunchKernelGGL((KER<C, M>), nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, st,                       \
+                     (const Vec4<T> *)ctx->efield[0], ctx->S_dev, ctx->counts, queue, ctx->qlen, ctx->cstats)
   if (ctx->mixed) {
     PState<float> st = ctx->pf[ctx->cur];
-    if (collide && move) LAUNCH(k_push_mixed, float, true, true, true);
-    else if (collide) LAUNCH(k_push_mixed, float, true, false, true);
-    else if (move) LAUNCH(k_push_mixed, float, false, true, true);
-    else LAUNCH(k_push_mixed, float, false, false, true);
+    if (collide && move) LAUNCH(k_push_mixed, float, true, true);
+    else if (collide) LAUNCH(k_push_mixed, float, true, false);
+    else if (move) LAUNCH(k_push_mixed, float, false, true);
+    else LAUNCH(k_push_mixed, float, false, false);
   } else {
     PState<double> st = ctx->pd[ctx->cur];
-    if (collide && move) LAUNCH(k_push_fp64, double, true, true, true);
-    else if (collide) LAUNCH(k_push_fp64, double, true, false, true);
-    else if (move) LAUNCH(k_push_fp64, double, false, true, true);
-    else LAUNCH(k_push_fp64, double, false, false, true);
+    if (collide && move) LAUNCH(k_push_fp64, double, true, true);
+    else if (collide) LAUNCH(k_push_fp64, double, true, false);
+    else if (move) LAUNCH(k_push_fp64, double, false, true);
+    else LAUNCH(k_push_fp64, double, false, false);
   }
+#undef LAUNCH
   timer_stop(ctx, "push");
-  if (collide) launch_combine_sum(ctx, ctx->sred, (int)nb, 1, ctx->S_dev);
+  if (collide) {
+    timer_start(ctx, "collide");
+    const unsigned nq = 1024;
+    if (ctx->mixed) {
+      if (move) hipLaunchKernelGGL((k_collide_mixed<true>), nq, kBlock, 0, ctx->stream, ctx->g, pc, ctx->pf[ctx->cur], queue, ctx->qlen, ctx->counts, ctx->cstats);
+      else hipLaunchKernelGGL((k_collide_mixed<false>), nq, kBlock, 0, ctx->stream, ctx->g, pc, ctx->pf[ctx->cur], queue, ctx->qlen, ctx->counts, ctx->cstats);
+    } else {
+      if (move) hipLaunchKernelGGL((k_collide_fp64<true>), nq, kBlock, 0, ctx->stream, ctx->g, pc, ctx->pd[ctx->cur], queue, ctx->qlen, ctx->counts, ctx->cstats);
+      else hipLaunchKernelGGL((k_collide_fp64<false>), nq, kBlock, 0, ctx->stream, ctx->g, pc, ctx->pd[ctx->cur], queue, ctx->qlen, ctx->counts, ctx->cstats);
+    }
+    timer_stop(ctx, "collide");
+  }
   unsigned long long cs[4];
   CHY_HIP(hipMemcpyAsync(cs, ctx->cstats, sizeof(cs), hipMemcpyDeviceToHost, ctx->stream));
   CHY_HIP(hipStreamSynchronize(ctx->stream));
@@ -907,7 +1149,6 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
   if (cs[1] >= 10)
     return set_err(ctx, CHY_ERR_SIG, "Failed to collide " + std::to_string(cs[1]) + " ion-neutral pairs. Aborting.");
   return CHY_OK;
-#undef LAUNCH
 }
 
 int particles_move_only(chy_ctx *ctx)
@@ -916,17 +1157,12 @@ int particles_move_only(chy_ctx *ctx)
   launch_fill<unsigned long long>(ctx, 4, ctx->cstats, 0ull);
   launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
   const unsigned nb = grid_for(ctx->np);
-  if (ctx->mixed) {
-    PState<float> st = ctx->pf[ctx->cur];
-    hipLaunchKernelGGL((k_push_mixed<false, true, false>), nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, st,
-                       (const float *)ctx->efield[0], (const float *)ctx->efield[1], (const float *)ctx->efield[2],
-                       ctx->S_dev, ctx->counts, ctx->sred, ctx->cstats);
-  } else {
-    PState<double> st = ctx->pd[ctx->cur];
-    hipLaunchKernelGGL((k_push_fp64<false, true, false>), nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, st,
-                       (const double *)ctx->efield[0], (const double *)ctx->efield[1], (const double *)ctx->efield[2],
-                       ctx->S_dev, ctx->counts, ctx->sred, ctx->cstats);
-  }
+  if (ctx->mixed)
+    hipLaunchKernelGGL(k_move_mixed, nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, ctx->pf[ctx->cur], ctx->counts,
+                       ctx->cstats);
+  else
+    hipLaunchKernelGGL(k_move_fp64, nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, ctx->pd[ctx->cur], ctx->counts,
+                       ctx->cstats);
   unsigned long long cs[4];
   CHY_HIP(hipMemcpyAsync(cs, ctx->cstats, sizeof(cs), hipMemcpyDeviceToHost, ctx->stream));
   CHY_HIP(hipStreamSynchronize(ctx->stream));

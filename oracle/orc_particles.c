@@ -97,8 +97,15 @@ int64_t orc_collision_count(int64_t Np, double nui, double dt, double viT,
  * when the probability had to be clipped at 1. */
 static int collide_one(const orc_push_params *P, uint32_t pid, double S, double v[3], int *capped)
 {
+  /* One Philox block gives the neutral draw (Box-Muller on the top 24 bits
+   * of the four words) and the decision uniform (from their low bytes). */
+  const uint32_t ctr[4] = {pid, P->step, STREAM_COLLIDE_A, 0};
+  const uint32_t key[2] = {(uint32_t)P->seed, (uint32_t)(P->seed >> 32)};
+  uint32_t a[4];
+  orc_philox4x32_10(ctr, key, a);
   double ua[4], ub[4];
-  philox_u(pid, P->step, STREAM_COLLIDE_A, 0, P->seed, ua);
+  for (int q = 0; q < 4; q++) ua[q] = orc_u24(a[q]);
+  const double udec = orc_u24(((a[0] & 0xffu) | ((a[1] & 0xffu) << 8) | ((a[2] & 0xffu) << 16)) << 8);
   const double r1 = sqrt(-2.0 * log(ua[0])), a1 = 2.0 * M_PI * ua[1];
   const double r2 = sqrt(-2.0 * log(ua[2])), a2 = 2.0 * M_PI * ua[3];
   const double vnx = P->vn0[0] + P->vnT * (r1 * cos(a1));
@@ -109,17 +116,17 @@ static int collide_one(const orc_push_params *P, uint32_t pid, double S, double 
   double p = (S > 0.0) ? ((double)P->Nc * vrr) / S : 0.0;
   *capped = 0;
   if (p > 1.0) { p = 1.0; *capped = 1; }
+  if (!(udec < p)) return 0;
   philox_u(pid, P->step, STREAM_COLLIDE_B, 0, P->seed, ub);
-  if (!(ub[0] < p)) return 0;
   const double mi = P->mi, mn = P->mn, M = mn + mi;
   const double vcx = (v[0] * mi + vnx * mn) / M;
   const double vcy = (v[1] * mi + vny * mn) / M;
   const double vcz = (v[2] * mi + vnz * mn) / M;
   const double ex = v[0] - vcx, ey = v[1] - vcy, ez = v[2] - vcz;
   const double vcr = sqrt(ex * ex + ey * ey + ez * ez);
-  const double uz = 2.0 * ub[1] - 1.0;
+  const double uz = 2.0 * ub[0] - 1.0;
   const double uperp = sqrt(1.0 - uz * uz);
-  const double uphi = 2.0 * M_PI * ub[2];
+  const double uphi = 2.0 * M_PI * ub[1];
   const double ux = uperp * cos(uphi), uy = uperp * sin(uphi);
   const double costht = vrz / vrr;
   const double sintht = sqrt(1.0 - costht * costht);

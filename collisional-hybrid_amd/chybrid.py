@@ -106,6 +106,7 @@ SIGNATURES = {
     "chy_timing_enable": (C.c_int, [P, C.c_int]),
     "chy_timing_get": (C.c_int, [P, C.c_char_p, P, P]),
     "chy_timing_reset": (C.c_int, [P]),
+    "chy_copy_kernel": (C.c_int, [P, C.c_int64, C.c_int]),
 }
 
 _LIB = None
@@ -320,3 +321,6 @@ class Context:
 
     def timing_reset(self):
         self._chk(lib().chy_timing_reset(self._p))
+
+    def copy_kernel(self, nbytes, repeat=1):
+        self._chk(lib().chy_copy_kernel(self._p, int(nbytes), int(repeat)))

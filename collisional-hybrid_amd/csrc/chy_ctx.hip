@@ -763,4 +763,24 @@ int chy_timing_reset(chy_ctx *ctx)
   return CHY_OK;
 }
 
+int chy_copy_kernel(chy_ctx *ctx, int64_t nbytes, int repeat)
+{
+  if (!ctx || nbytes <= 0 || repeat < 1) return CHY_ERR_ARG_WRONG;
+  const int64_t n16 = (nbytes + 15) / 16;
+  void *a = nullptr, *b = nullptr;
+  CHY_HIP(hipMalloc(&a, n16 * 16));
+  if (hipMalloc(&b, n16 * 16) != hipSuccess) { (void)hipFree(a); return set_err(ctx, CHY_ERR_MEM, "copy buffers"); }
+  (void)hipMemsetAsync(a, 1, n16 * 16, ctx->stream);
+  for (int r = 0; r < repeat; r++) {
+    timer_start(ctx, "copy");
+    launch_copy(ctx, n16, a, b);
+    timer_stop(ctx, "copy");
+  }
+  const hipError_t e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, hipGetErrorString(e));
+  return CHY_OK;
+}
+
 }  // extern "C"

@@ -30,135 +30,194 @@ constexpr int kFftMaxN = 2048;  // longest axis handled by one LDS-resident line
 template <typename T>
 static int fft_batch(int n, int64_t avail)
 {
-  const int64_t E = 65536 / (int64_t)(2 * sizeof(T));
+  const int64_t E = 65536 / (int64_t)(2 * sizeof(T)) - n / 2;  // twiddles share the LDS
   int B = 1;
   while ((int64_t)(2 * B) * (n + 1) <= E && 2 * B <= avail) B *= 2;
   return B;
 }
 
-// One batch of lines in LDS: s[b*(n+1) + t].
 template <typename T>
-__device__ __forceinline__ void dif_fwd(cpx<T> *s, int B, int n, int logn, const double *tw)
+static size_t fft_lds(int n, int B) { return ((size_t)B * (n + 1) + (size_t)n / 2) * 2 * sizeof(T); }
+
+// One batch of B lines in LDS: s[b*(n+1) + t] (padded rows).  Twiddles
+// w[j] = exp(-2 pi i j / n), j < n/2, are staged in LDS (tw).  Butterfly
+// indices use shifts and masks (n is a power of two).
+
+// Forward, decimation in frequency (natural order in, bit-reversed out):
+// radix-4 stages (two radix-2 stages fused), then a final radix-2 stage when
+// log2 n is odd.
+template <typename T>
+__device__ __forceinline__ void dif_fwd(cpx<T> *s, int B, int logn, const cpx<T> *tw)
 {
-  // tw[2j], tw[2j+1] = cos, sin of -2 pi j / n  (j < n/2)
-  for (int len = n; len >= 2; len >>= 1) {
-    const int half = len >> 1, step = n / len;
-    for (int m = threadIdx.x; m < B * (n >> 1); m += blockDim.x) {
-      const int b = m / (n >> 1), r = m % (n >> 1);
-      const int grp = r / half, k = r % half;
-      const int i0 = b * (n + 1) + grp * len + k, i1 = i0 + half;
-      const cpx<T> u = s[i0], v = s[i1];
-      const cpx<T> w = {(T)tw[2 * (k * step)], (T)tw[2 * (k * step) + 1]};
+  const int n = 1 << logn;
+  int lg = logn;
+  for (; lg >= 2; lg -= 2) {
+    const int len = 1 << lg, lq = lg - 2, q = 1 << lq, step = n >> lg;
+    const int nb4 = B << (logn - 2);
+    for (int m = threadIdx.x; m < nb4; m += blockDim.x) {
+      const int b = m >> (logn - 2), r = m & ((n >> 2) - 1);
+      const int grp = r >> lq, k = r & (q - 1);
+      const int i0 = b * (n + 1) + grp * len + k;
+      const cpx<T> x0 = s[i0], x1 = s[i0 + q], x2 = s[i0 + 2 * q], x3 = s[i0 + 3 * q];
+      const cpx<T> w1 = tw[k * step], w2 = tw[2 * k * step];
+      const cpx<T> a0 = {x0.x + x2.x, x0.y + x2.y};
+      const cpx<T> a1 = {x1.x + x3.x, x1.y + x3.y};
+      const cpx<T> d02 = {x0.x - x2.x, x0.y - x2.y};
+      const cpx<T> d13 = {x1.x - x3.x, x1.y - x3.y};
+      const cpx<T> a2 = cmul<T>(d02, w1);
+      const cpx<T> a3 = cmul<T>({d13.y, -d13.x}, w1);  // (x1 - x3) W^k (-i)
+      s[i0] = {a0.x + a1.x, a0.y + a1.y};
+      s[i0 + q] = cmul<T>({a0.x - a1.x, a0.y - a1.y}, w2);
+      s[i0 + 2 * q] = {a2.x + a3.x, a2.y + a3.y};
+      s[i0 + 3 * q] = cmul<T>({a2.x - a3.x, a2.y - a3.y}, w2);
+    }
+    __syncthreads();
+  }
+  if (lg == 1) {
+    const int nb2 = B << (logn - 1);
+    for (int m = threadIdx.x; m < nb2; m += blockDim.x) {
+      const int b = m >> (logn - 1), r = m & ((n >> 1) - 1);
+      const int i0 = b * (n + 1) + 2 * r;
+      const cpx<T> u = s[i0], v = s[i0 + 1];
       s[i0] = {u.x + v.x, u.y + v.y};
-      s[i1] = cmul<T>({u.x - v.x, u.y - v.y}, w);
+      s[i0 + 1] = {u.x - v.x, u.y - v.y};
+    }
+    __syncthreads();
+  }
+}
+
+// Inverse, decimation in time (bit-reversed in, natural order out, unscaled):
+// a radix-2 first stage when log2 n is odd, then radix-4 stages.
+template <typename T>
+__device__ __forceinline__ void dit_inv(cpx<T> *s, int B, int logn, const cpx<T> *tw)
+{
+  const int n = 1 << logn;
+  int lg = 0;
+  if (logn & 1) {
+    const int nb2 = B << (logn - 1);
+    for (int m = threadIdx.x; m < nb2; m += blockDim.x) {
+      const int b = m >> (logn - 1), r = m & ((n >> 1) - 1);
+      const int i0 = b * (n + 1) + 2 * r;
+      const cpx<T> u = s[i0], v = s[i0 + 1];
+      s[i0] = {u.x + v.x, u.y + v.y};
+      s[i0 + 1] = {u.x - v.x, u.y - v.y};
+    }
+    __syncthreads();
+    lg = 1;
+  }
+  for (; lg + 2 <= logn; lg += 2) {
+    const int len = 1 << (lg + 2), lq = lg, q = 1 << lq, step = n >> (lg + 2);
+    const int nb4 = B << (logn - 2);
+    for (int m = threadIdx.x; m < nb4; m += blockDim.x) {
+      const int b = m >> (logn - 2), r = m & ((n >> 2) - 1);
+      const int grp = r >> lq, k = r & (q - 1);
+      const int i0 = b * (n + 1) + grp * len + k;
+      const cpx<T> x0 = s[i0], x1 = s[i0 + q], x2 = s[i0 + 2 * q], x3 = s[i0 + 3 * q];
+      const cpx<T> t1 = tw[k * step], t2 = tw[2 * k * step];
+      const cpx<T> W = {t1.x, -t1.y}, V = {t2.x, -t2.y};  // conjugates: +2 pi i
+      const cpx<T> v1 = cmul<T>(x1, V), v3 = cmul<T>(x3, V);
+      const cpx<T> a0 = {x0.x + v1.x, x0.y + v1.y}, a1 = {x0.x - v1.x, x0.y - v1.y};
+      const cpx<T> a2 = {x2.x + v3.x, x2.y + v3.y}, a3 = {x2.x - v3.x, x2.y - v3.y};
+      const cpx<T> b2 = cmul<T>(a2, W);
+      const cpx<T> wa3 = cmul<T>(a3, W);
+      const cpx<T> b3 = {-wa3.y, wa3.x};  // a3 W^k (+i)
+      s[i0] = {a0.x + b2.x, a0.y + b2.y};
+      s[i0 + 2 * q] = {a0.x - b2.x, a0.y - b2.y};
+      s[i0 + q] = {a1.x + b3.x, a1.y + b3.y};
+      s[i0 + 3 * q] = {a1.x - b3.x, a1.y - b3.y};
     }
     __syncthreads();
   }
 }
 
 template <typename T>
-__device__ __forceinline__ void dit_inv(cpx<T> *s, int B, int n, int logn, const double *tw)
+__device__ __forceinline__ void load_twiddles(cpx<T> *tws, const double *tw, int n)
 {
-  for (int len = 2; len <= n; len <<= 1) {
-    const int half = len >> 1, step = n / len;
-    for (int m = threadIdx.x; m < B * (n >> 1); m += blockDim.x) {
-      const int b = m / (n >> 1), r = m % (n >> 1);
-      const int grp = r / half, k = r % half;
-      const int i0 = b * (n + 1) + grp * len + k, i1 = i0 + half;
-      const cpx<T> w = {(T)tw[2 * (k * step)], -(T)tw[2 * (k * step) + 1]};  // conjugate: +2 pi i
-      const cpx<T> u = s[i0], v = cmul<T>(s[i1], w);
-      s[i0] = {u.x + v.x, u.y + v.y};
-      s[i1] = {u.x - v.x, u.y - v.y};
-    }
-    __syncthreads();
-  }
+  for (int j = threadIdx.x; j < (n >> 1); j += blockDim.x) tws[j] = {(T)tw[2 * j], (T)tw[2 * j + 1]};
 }
 
 // Pass along x (contiguous lines).  MODE 0: real in -> forward -> complex out.
 // MODE 1: complex in -> inverse -> real out (real part times scale).
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_fft_x(int n, int logn, int nlines, int B, const T *__restrict__ rin,
+__global__ __launch_bounds__(256) void k_fft_x(int logn, int nlines, int B, const T *__restrict__ rin,
                                                cpx<T> *__restrict__ buf, T *__restrict__ rout, const double *__restrict__ tw,
                                                double scale)
 {
   extern __shared__ unsigned char smem_raw[];
-  cpx<T> *s = reinterpret_cast<cpx<T> *>(smem_raw);
+  const int n = 1 << logn;
+  cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
+  cpx<T> *s = tws + (n >> 1);
+  load_twiddles<T>(tws, tw, n);
   const int64_t l0 = (int64_t)blockIdx.x * B;
   const int nb = (int)((nlines - l0) < B ? (nlines - l0) : B);
-  for (int e = threadIdx.x; e < nb * n; e += blockDim.x) {
-    const int b = e / n, t = e % n;
+  for (int e = threadIdx.x; e < (nb << logn); e += blockDim.x) {
+    const int b = e >> logn, t = e & (n - 1);
     const int64_t a = (l0 + b) * n + t;
     if (MODE == 0) s[b * (n + 1) + t] = {rin[a], (T)0};
     else s[b * (n + 1) + t] = buf[a];
   }
   __syncthreads();
-  if (MODE == 0) dif_fwd<T>(s, nb, n, logn, tw);
-  else dit_inv<T>(s, nb, n, logn, tw);
-  for (int e = threadIdx.x; e < nb * n; e += blockDim.x) {
-    const int b = e / n, t = e % n;
+  if (MODE == 0) dif_fwd<T>(s, nb, logn, tws);
+  else dit_inv<T>(s, nb, logn, tws);
+  for (int e = threadIdx.x; e < (nb << logn); e += blockDim.x) {
+    const int b = e >> logn, t = e & (n - 1);
     const int64_t a = (l0 + b) * n + t;
     if (MODE == 0) buf[a] = s[b * (n + 1) + t];
     else rout[a] = (T)((double)s[b * (n + 1) + t].x * scale);
   }
 }
 
+// Symbol of the constant-coefficient operator in closed form:
+// lam = A0 + sum_a F_a c_a + sum_{a<b} (P_ab c_a c_b - M_ab s_a s_b),
+// c_a = cos(theta_a), s_a = sin(theta_a) (faces and edges paired by a_o = a_-o).
+struct Symbol {
+  double A0, F[3], P[3], M[3];  // pairs ordered xy, xz, yz
+  int nx, lx, ly, lz;
+  double inv_scale;             // 1 / (nx ny nz)
+};
+
 // Pass along a strided axis: line (a, c) at base a + c*cstride, element t at
 // + t*estride; a block holds B consecutive a.  MODE 0 forward, 1 inverse,
 // 2 forward + symbol division + inverse (z axis).
-struct Symbol {
-  double v[19];
-  int off[19][3];
-  int nx, ny, nz, lx, ly, lz;  // sizes and log2 sizes
-  double inv_scale;            // 1 / (nx ny nz)
-};
-
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_fft_s(int n, int logn, int64_t A, int64_t estride, int64_t cstride, int B,
+__global__ __launch_bounds__(256) void k_fft_s(int logn, int64_t A, int64_t estride, int64_t cstride, int B,
                                                cpx<T> *__restrict__ buf, const double *__restrict__ tw, Symbol sym,
                                                const double *__restrict__ trx, const double *__restrict__ try_,
                                                const double *__restrict__ trz)
 {
   extern __shared__ unsigned char smem_raw[];
-  cpx<T> *s = reinterpret_cast<cpx<T> *>(smem_raw);
+  const int n = 1 << logn;
+  cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
+  cpx<T> *s = tws + (n >> 1);
+  load_twiddles<T>(tws, tw, n);
   const int64_t blocks_per_c = A / B;
   const int64_t c = blockIdx.x / blocks_per_c;
   const int64_t a0 = (blockIdx.x % blocks_per_c) * B;
   const int64_t base = a0 + c * cstride;
   for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-    const int t = e / B, b = e % B;
+    const int t = e / B, b = e - t * B;
     s[b * (n + 1) + t] = buf[base + b + (int64_t)t * estride];
   }
   __syncthreads();
   if (MODE == 1) {
-    dit_inv<T>(s, B, n, logn, tw);
+    dit_inv<T>(s, B, logn, tws);
   } else {
-    dif_fwd<T>(s, B, n, logn, tw);
+    dif_fwd<T>(s, B, logn, tws);
     if (MODE == 2) {
-      // line a = ix + nx*iy (positions hold bit-reversed x and y frequencies),
-      // element t holds bit-reversed z frequency
+      // line a = ix + nx*iy holds bit-reversed x and y frequencies, element t
+      // the bit-reversed z frequency
       for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-        const int t = e / B, b = e % B;
+        const int t = e / B, b = e - t * B;
         const int64_t a = a0 + b;
-        const int kx = bitrev((int)(a % sym.nx), sym.lx);
-        const int ky = bitrev((int)(a / sym.nx), sym.ly);
+        const int kx = bitrev((int)(a & (sym.nx - 1)), sym.lx);
+        const int ky = bitrev((int)(a >> sym.lx), sym.ly);
         const int kz = bitrev(t, sym.lz);
-        const double cs[3] = {trx[2 * kx], try_[2 * ky], trz[2 * kz]};
-        const double sn[3] = {trx[2 * kx + 1], try_[2 * ky + 1], trz[2 * kz + 1]};
-        double lam = 0.0;
-#pragma unroll
-        for (int q = 0; q < 19; q++) {
-          double C = 1.0, S = 0.0;
-#pragma unroll
-          for (int ax = 0; ax < 3; ax++) {
-            const int o = sym.off[q][ax];
-            if (o != 0) {
-              const double ss = o * sn[ax];
-              const double nC = C * cs[ax] - S * ss, nS = S * cs[ax] + C * ss;
-              C = nC; S = nS;
-            }
-          }
-          lam += sym.v[q] * C;
-        }
+        const double cx = trx[2 * kx], sx = trx[2 * kx + 1];
+        const double cy = try_[2 * ky], sy = try_[2 * ky + 1];
+        const double cz = trz[2 * kz], sz = trz[2 * kz + 1];
+        const double lam = sym.A0 + sym.F[0] * cx + sym.F[1] * cy + sym.F[2] * cz +
+                           (sym.P[0] * cx * cy - sym.M[0] * sx * sy) + (sym.P[1] * cx * cz - sym.M[1] * sx * sz) +
+                           (sym.P[2] * cy * cz - sym.M[2] * sy * sz);
         cpx<T> &z = s[b * (n + 1) + t];
         if (lam == 0.0 || (kx == 0 && ky == 0 && kz == 0)) {
           z = {(T)0, (T)0};
@@ -168,11 +227,11 @@ __global__ __launch_bounds__(256) void k_fft_s(int n, int logn, int64_t A, int64
         }
       }
       __syncthreads();
-      dit_inv<T>(s, B, n, logn, tw);
+      dit_inv<T>(s, B, logn, tws);
     }
   }
   for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-    const int t = e / B, b = e % B;
+    const int t = e / B, b = e - t * B;
     buf[base + b + (int64_t)t * estride] = s[b * (n + 1) + t];
   }
 }
@@ -208,63 +267,45 @@ int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
 {
   const int nx = ctx->g.nx, ny = ctx->g.ny, nz = ctx->g.nz;
   cpx<T> *buf = (cpx<T> *)ctx->fft_buf;
+  // symbol coefficients from the uniform-density row (column order of
+  // lhs.c:11-104: faces 0..5, xy edges 6..9, xz 10..13, yz 14..17, centre 18)
   Symbol sym;
-  static const int off[19][3] = {
-    {+1, 0, 0}, {-1, 0, 0}, {0, +1, 0}, {0, -1, 0}, {0, 0, +1}, {0, 0, -1},
-    {+1, +1, 0}, {+1, -1, 0}, {-1, +1, 0}, {-1, -1, 0},
-    {+1, 0, +1}, {+1, 0, -1}, {-1, 0, +1}, {-1, 0, -1},
-    {0, +1, +1}, {0, +1, -1}, {0, -1, +1}, {0, -1, -1},
-    {0, 0, 0}};
-  for (int q = 0; q < 19; q++) {
-    sym.v[q] = v19[q];
-    for (int a = 0; a < 3; a++) sym.off[q][a] = off[q][a];
-  }
-  sym.nx = nx; sym.ny = ny; sym.nz = nz;
+  sym.A0 = v19[18];
+  sym.F[0] = 2.0 * v19[0]; sym.F[1] = 2.0 * v19[2]; sym.F[2] = 2.0 * v19[4];
+  sym.P[0] = 2.0 * (v19[6] + v19[7]);   sym.M[0] = 2.0 * (v19[6] - v19[7]);
+  sym.P[1] = 2.0 * (v19[10] + v19[11]); sym.M[1] = 2.0 * (v19[10] - v19[11]);
+  sym.P[2] = 2.0 * (v19[14] + v19[15]); sym.M[2] = 2.0 * (v19[14] - v19[15]);
+  sym.nx = nx;
   sym.lx = ilog2(nx); sym.ly = ilog2(ny); sym.lz = ilog2(nz);
   sym.inv_scale = 1.0 / ((double)nx * ny * nz);
-  // x forward (real in)
-  {
-    const int n = nx, B = fft_batch<T>(n, (int64_t)ny * nz);
-    const int nlines = ny * nz;
-    const unsigned nb = (unsigned)((nlines + B - 1) / B);
-    const size_t sh = (size_t)B * (n + 1) * sizeof(cpx<T>);
-    hipLaunchKernelGGL((k_fft_x<T, 0>), nb, kBlock, sh, ctx->stream, n, ilog2(n), nlines, B, r, buf, (T *)nullptr,
-                       ctx->tw[0], 1.0);
+  {  // x forward (real in)
+    const int n = nx, B = fft_batch<T>(n, (int64_t)ny * nz), nlines = ny * nz;
+    hipLaunchKernelGGL((k_fft_x<T, 0>), (unsigned)((nlines + B - 1) / B), kBlock, fft_lds<T>(n, B), ctx->stream,
+                       ilog2(n), nlines, B, r, buf, (T *)nullptr, ctx->tw[0], 1.0);
   }
-  // y forward: lines (ix, iz); A = nx contiguous, c = iz with stride nx*ny
-  {
+  {  // y forward: lines (ix, iz); A = nx contiguous, c = iz with stride nx*ny
     const int n = ny, B = fft_batch<T>(n, nx);
-    const unsigned nb = (unsigned)((nx / B) * nz);
-    const size_t sh = (size_t)B * (n + 1) * sizeof(cpx<T>);
-    hipLaunchKernelGGL((k_fft_s<T, 0>), nb, kBlock, sh, ctx->stream, n, ilog2(n), (int64_t)nx, (int64_t)nx,
-                       (int64_t)nx * ny, B, buf, ctx->tw[1], sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
+    hipLaunchKernelGGL((k_fft_s<T, 0>), (unsigned)((nx / B) * nz), kBlock, fft_lds<T>(n, B), ctx->stream, ilog2(n),
+                       (int64_t)nx, (int64_t)nx, (int64_t)nx * ny, B, buf, ctx->tw[1], sym, ctx->trig[0],
+                       ctx->trig[1], ctx->trig[2]);
   }
-  // z forward + symbol + inverse: lines a = ix + nx*iy, estride nx*ny
-  {
+  {  // z forward + symbol + inverse: lines a = ix + nx*iy, estride nx*ny
     const int n = nz;
     const int64_t A = (int64_t)nx * ny;
     const int B = fft_batch<T>(n, A);
-    const unsigned nb = (unsigned)(A / B);
-    const size_t sh = (size_t)B * (n + 1) * sizeof(cpx<T>);
-    hipLaunchKernelGGL((k_fft_s<T, 2>), nb, kBlock, sh, ctx->stream, n, ilog2(n), A, A, (int64_t)0, B, buf,
-                       ctx->tw[2], sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
+    hipLaunchKernelGGL((k_fft_s<T, 2>), (unsigned)(A / B), kBlock, fft_lds<T>(n, B), ctx->stream, ilog2(n), A, A,
+                       (int64_t)0, B, buf, ctx->tw[2], sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
   }
-  // y inverse
-  {
+  {  // y inverse
     const int n = ny, B = fft_batch<T>(n, nx);
-    const unsigned nb = (unsigned)((nx / B) * nz);
-    const size_t sh = (size_t)B * (n + 1) * sizeof(cpx<T>);
-    hipLaunchKernelGGL((k_fft_s<T, 1>), nb, kBlock, sh, ctx->stream, n, ilog2(n), (int64_t)nx, (int64_t)nx,
-                       (int64_t)nx * ny, B, buf, ctx->tw[1], sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
+    hipLaunchKernelGGL((k_fft_s<T, 1>), (unsigned)((nx / B) * nz), kBlock, fft_lds<T>(n, B), ctx->stream, ilog2(n),
+                       (int64_t)nx, (int64_t)nx, (int64_t)nx * ny, B, buf, ctx->tw[1], sym, ctx->trig[0],
+                       ctx->trig[1], ctx->trig[2]);
   }
-  // x inverse (real out)
-  {
-    const int n = nx, B = fft_batch<T>(n, (int64_t)ny * nz);
-    const int nlines = ny * nz;
-    const unsigned nb = (unsigned)((nlines + B - 1) / B);
-    const size_t sh = (size_t)B * (n + 1) * sizeof(cpx<T>);
-    hipLaunchKernelGGL((k_fft_x<T, 1>), nb, kBlock, sh, ctx->stream, n, ilog2(n), nlines, B, (const T *)nullptr, buf,
-                       z, ctx->tw[0], 1.0);
+  {  // x inverse (real out)
+    const int n = nx, B = fft_batch<T>(n, (int64_t)ny * nz), nlines = ny * nz;
+    hipLaunchKernelGGL((k_fft_x<T, 1>), (unsigned)((nlines + B - 1) / B), kBlock, fft_lds<T>(n, B), ctx->stream,
+                       ilog2(n), nlines, B, (const T *)nullptr, buf, z, ctx->tw[0], 1.0);
   }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("fft: ") + hipGetErrorString(e));

@@ -522,6 +522,19 @@ void launch_fill(chy_ctx *ctx, int64_t n, T *v, T val)
   if (nb) hipLaunchKernelGGL((k_fill<T>), nb, kBlock, 0, ctx->stream, n, v, val);
 }
 
+// Plain copy (counter calibration): 16-byte vector loads and stores.
+__global__ __launch_bounds__(256) void k_copy(int64_t n16, const uint4 *__restrict__ a, uint4 *__restrict__ b)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+void launch_copy(chy_ctx *ctx, int64_t n16, const void *a, void *b)
+{
+  const unsigned nb = (unsigned)std::min<int64_t>(16384, (n16 + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_copy, nb, kBlock, 0, ctx->stream, n16, (const uint4 *)a, (uint4 *)b);
+}
+
 // explicit instantiations
 template void launch_lhs_apply<double, double, double>(chy_ctx *, int, bool, const double *, const double *, double *);
 template void launch_lhs_apply<float, float, float>(chy_ctx *, int, bool, const float *, const float *, float *);

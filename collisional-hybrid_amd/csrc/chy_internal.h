@@ -152,6 +152,7 @@ void launch_fill(chy_ctx *ctx, int64_t n, T *v, T val);
 template <typename T>
 void launch_jacobi(chy_ctx *ctx, int lhs_type, const T *n, const T *r, T *z);
 void launch_combine_sum(chy_ctx *ctx, const double *partials, int nblk, int nv, double *out);
+void launch_copy(chy_ctx *ctx, int64_t n16, const void *a, void *b);
 
 // ---- particle kernels (chy_particles.hip) ----
 int particles_upload(chy_ctx *ctx, int64_t np, const double *pos, const double *vel, const double *w,

@@ -211,6 +211,9 @@ int chy_output_write(chy_ctx *ctx, const char *stem, const char *insert);
 int chy_timing_enable(chy_ctx *ctx, int on);
 int chy_timing_get(chy_ctx *ctx, const char *name, double *total_ms, int64_t *count);
 int chy_timing_reset(chy_ctx *ctx);
+/* Plain device-to-device copy of nbytes through a simple vector kernel on the
+ * context stream ("copy" timer); used to calibrate HBM byte counters. */
+int chy_copy_kernel(chy_ctx *ctx, int64_t nbytes, int repeat);
 
 #ifdef __cplusplus
 }

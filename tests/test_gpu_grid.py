@@ -145,3 +145,23 @@ def test_solve_warm_start_and_fft_iteration_counts():
         assert st1.converged and st1.iterations <= 15
         phi2, st2 = ctx.solve_potential()   # same system, warm start: converged at once
         assert st2.iterations == 0 and relmax(phi2, phi1) <= 1e-12
+
+
+@pytest.mark.parametrize("N,extra", [((8, 16, 32), ""), ((32, 8, 128), "-B0x 1e-5 -B0y -2e-5"),
+                                     ((4, 4, 4), ""), ((64, 32, 16), "--lhs-type laplacian"),
+                                     ((16, 32, 8), "-precision mixed")])
+def test_fft_preconditioner_exact_for_uniform_density(N, extra):
+    """For uniform density the FFT preconditioner is A(n)^+ itself: one GMRES
+    iteration solves the system (odd and even log2 axis lengths)."""
+    rng = np.random.default_rng(21)
+    with context(opts(N, extra="-ksp_rtol 1e-9 " + extra)) as ctx:
+        g = grid_of(ctx.prm)
+        n = np.full(g.nc, 16.0)
+        G = [rng.standard_normal(g.nc) * 3000 for _ in range(3)]
+        ctx.set_moments(vlasov_from([n] + G))
+        phi, st = ctx.solve_potential()
+        b = ctx.get_rhs()
+    assert st.converged == 1
+    assert st.iterations <= (2 if ctx.prm.precision == chybrid.PREC_FP64 else 6), st.iterations
+    ref = orc.pc_apply(g, list(ctx.prm.kappa_e), n, b, lhs_type=ctx.prm.lhs_type)
+    assert relmax(phi, ref) <= (1e-9 if ctx.prm.precision == chybrid.PREC_FP64 else 1e-8)

@@ -182,20 +182,23 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
       T *Vj = V + j * nc, *Zj = Z + j * nc, *Vn = V + (j + 1) * nc;
       TRY(pc_apply<T>(ctx, v19, Vj, Zj));
       timer_start(ctx, "lhs_apply");
-      launch_lhs_apply<T, T, T>(ctx, lt, false, n, Zj, w);
+      launch_lhs_apply<T, T, T>(ctx, lt, false, n, Zj, Vn);  // w built in place in slot j+1
       timer_stop(ctx, "lhs_apply");
       timer_start(ctx, "krylov");
-      launch_mdot<T>(ctx, (int)(j + 1), V, nc, w, nc, hdev);
-      launch_mgs_update<T>(ctx, (int)(j + 1), V, nc, hdev, w, nc);
-      launch_mdot<T>(ctx, (int)(j + 1), V, nc, w, nc, h2dev);
-      launch_mgs_update<T>(ctx, (int)(j + 1), V, nc, h2dev, w, nc);
-      launch_mdot<T>(ctx, 1, w, nc, w, nc, ndev);
+      // two-pass classical Gram-Schmidt (the preconditioned operator is close
+      // to the identity, so one pass cancels heavily); the norm of the result
+      // is fused into the second update and everything is read back at once
+      launch_mdot<T>(ctx, (int)(j + 1), V, nc, Vn, nc, hdev);
+      launch_mgs_update<T>(ctx, (int)(j + 1), V, nc, hdev, Vn, nc);
+      launch_mdot<T>(ctx, (int)(j + 1), V, nc, Vn, nc, h2dev);
+      launch_mgs_norm<T>(ctx, (int)(j + 1), V, nc, h2dev, Vn, nc, ndev);
       TRY(fetch(ctx, ctx->red_out, (int)(2 * (m + 2) + 1), ctx->h_red));
-      timer_stop(ctx, "krylov");
       for (int64_t q = 0; q <= j; q++) h[q] = ctx->h_red[q] + ctx->h_red[(m + 2) + q];
-      h[j + 1] = sqrt(ctx->h_red[2 * (m + 2)]);
+      const double wn = sqrt(ctx->h_red[2 * (m + 2)]);
+      timer_stop(ctx, "krylov");
+      h[j + 1] = wn;
       if (!std::isfinite(h[j + 1])) return set_err(ctx, CHY_ERR_FP, "non-finite Arnoldi vector");
-      launch_axpby<T>(ctx, nc, h[j + 1] > 0.0 ? 1.0 / h[j + 1] : 0.0, w, 0.0, Vn);
+      launch_axpby<T>(ctx, nc, h[j + 1] > 0.0 ? 1.0 / h[j + 1] : 0.0, Vn, 0.0, Vn);
       for (int64_t q = 0; q < j; q++) {
         const double t = cs[q] * h[q] + sn[q] * h[q + 1];
         h[q + 1] = -sn[q] * h[q] + cs[q] * h[q + 1];

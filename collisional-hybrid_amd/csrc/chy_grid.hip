@@ -165,6 +165,96 @@ __global__ __launch_bounds__(256) void k_residual(Grid g, int lhs_type, SCoef s,
   r[c] = b[c] - acc;
 }
 
+// ---- LDS-tiled (2.5-D) operator apply ------------------------------------------
+// A 64 x 4 thread block owns a (64 x 16) column of nodes (4 rows per thread)
+// over kZChunk planes.
+// It keeps three z-planes of x and n, each with a one-node halo in x and y
+// ((64+2) x (4+2) values), in LDS and marches up in z, loading one plane per
+// step with coalesced x-fastest reads.  Output: y = A x, or r = b - A x.
+constexpr int kTX = 64, kTY = 16, kRows = kTY / 4, kZChunk = 16;  // 64 x 4 threads, 4 rows each
+constexpr int kHX = kTX + 2, kHY = kTY + 2;
+
+template <typename TN, typename TX, typename C>
+__device__ __forceinline__ void load_plane(const Grid &g, int x0, int y0, int k, const TN *__restrict__ n,
+                                           const TX *__restrict__ x, C (*xs)[kHX], C (*ns)[kHX], bool need_n)
+{
+  const int kk = wrapi(k, g.nz);  // k in [-1, nz]
+  for (int e = threadIdx.x; e < kHX * kHY; e += blockDim.x) {
+    const int hy = e / kHX, hx = e - hy * kHX;
+    // tiles may overhang small grids by more than one period: full modulo
+    int i = x0 + hx - 1, j = y0 + hy - 1;
+    if (i < 0) i += g.nx;
+    if (i >= g.nx) i -= g.nx;
+    if (i >= g.nx) i %= g.nx;
+    if (j < 0) j += g.ny;
+    if (j >= g.ny) j -= g.ny;
+    if (j >= g.ny) j %= g.ny;
+    const int64_t a = (int64_t)i + (int64_t)g.nx * ((int64_t)j + (int64_t)g.ny * kk);
+    xs[hy][hx] = (C)x[a];
+    if (need_n) ns[hy][hx] = (C)n[a];
+  }
+}
+
+template <typename TN, typename TX, typename TY, typename C, bool DIFF, bool RESID>
+__global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s, LapRow lap,
+                                                   const TN *__restrict__ n, const TX *__restrict__ x,
+                                                   const double *__restrict__ b, TY *__restrict__ y)
+{
+  __shared__ C xs[3][kHY][kHX];
+  __shared__ C ns[3][kHY][kHX];
+  const bool full = lhs_type == CHY_LHS_FULL;
+  const int tx = threadIdx.x & (kTX - 1), ty0 = threadIdx.x >> 6;
+  const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, z0 = blockIdx.z * kZChunk;
+  const int z1 = z0 + kZChunk < g.nz ? z0 + kZChunk : g.nz;
+  const int i = x0 + tx;
+  load_plane<TN, TX, C>(g, x0, y0, z0 - 1, n, x, xs[(z0 + 2) % 3], ns[(z0 + 2) % 3], full);
+  load_plane<TN, TX, C>(g, x0, y0, z0, n, x, xs[z0 % 3], ns[z0 % 3], full);
+  for (int k = z0; k < z1; k++) {
+    load_plane<TN, TX, C>(g, x0, y0, k + 1, n, x, xs[(k + 1) % 3], ns[(k + 1) % 3], full);
+    __syncthreads();
+    const int pm = (k + 2) % 3, pc = k % 3, pp = (k + 1) % 3;
+#pragma unroll
+    for (int rr = 0; rr < kRows; rr++) {
+      const int ty = ty0 + 4 * rr, j = y0 + ty;
+      if (i < g.nx && j < g.ny) {
+        const int hx = tx + 1, hy = ty + 1;
+        C v[19];
+        if (full) {
+          row_from_star<C>(s, ns[pc][hy][hx], ns[pc][hy][hx + 1], ns[pc][hy][hx - 1], ns[pc][hy + 1][hx],
+                           ns[pc][hy - 1][hx], ns[pp][hy][hx], ns[pm][hy][hx], v);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 19; q++) v[q] = (C)lap.v[q];
+        }
+        // x at the 19 columns, in the column order of the row
+        const C xv[19] = {
+          xs[pc][hy][hx + 1], xs[pc][hy][hx - 1], xs[pc][hy + 1][hx], xs[pc][hy - 1][hx], xs[pp][hy][hx],
+          xs[pm][hy][hx], xs[pc][hy + 1][hx + 1], xs[pc][hy - 1][hx + 1], xs[pc][hy + 1][hx - 1],
+          xs[pc][hy - 1][hx - 1], xs[pp][hy][hx + 1], xs[pm][hy][hx + 1], xs[pp][hy][hx - 1], xs[pm][hy][hx - 1],
+          xs[pp][hy + 1][hx], xs[pm][hy + 1][hx], xs[pp][hy - 1][hx], xs[pm][hy - 1][hx], xs[pc][hy][hx]};
+        C acc = (C)0;
+        if (DIFF) {
+#pragma unroll
+          for (int q = 0; q < 18; q++) acc += v[q] * (xv[q] - xv[18]);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 19; q++) acc += v[q] * xv[q];
+        }
+        const int64_t c = (int64_t)i + (int64_t)g.nx * ((int64_t)j + (int64_t)g.ny * k);
+        if (RESID) y[c] = (TY)(b[c] - (double)acc);
+        else y[c] = (TY)acc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+static inline dim3 tiled_grid(const Grid &g)
+{
+  return dim3((unsigned)((g.nx + kTX - 1) / kTX), (unsigned)((g.ny + kTY - 1) / kTY),
+              (unsigned)((g.nz + kZChunk - 1) / kZChunk));
+}
+
 static LapRow lap_row(chy_ctx *ctx, int lhs_type)
 {
   LapRow L;
@@ -185,8 +275,8 @@ void launch_lhs_apply(chy_ctx *ctx, int lhs_type, bool transpose, const TN *n, c
     hipLaunchKernelGGL((k_lhs_apply_t<TN, TX, TY, C>), nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g,
                        lhs_type, s, L, n, x, y);
   else
-    hipLaunchKernelGGL((k_lhs_apply<TN, TX, TY, C, fp32>), nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g,
-                       lhs_type, s, L, n, x, y);
+    hipLaunchKernelGGL((k_lhs_tiled<TN, TX, TY, C, fp32, false>), tiled_grid(ctx->g), kBlock, 0, ctx->stream, ctx->g,
+                       lhs_type, s, L, n, x, (const double *)nullptr, y);
 }
 
 template <typename TN, typename TX>
@@ -194,8 +284,8 @@ void launch_residual(chy_ctx *ctx, int lhs_type, const TN *n, const double *b, c
 {
   const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
   const LapRow L = lap_row(ctx, lhs_type);
-  hipLaunchKernelGGL((k_residual<TN, TX>), nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g, lhs_type, s, L, n, b,
-                     x, r);
+  hipLaunchKernelGGL((k_lhs_tiled<TN, TX, double, double, false, true>), tiled_grid(ctx->g), kBlock, 0, ctx->stream,
+                     ctx->g, lhs_type, s, L, n, x, b, r);
 }
 
 template <typename T>
@@ -464,6 +554,51 @@ void launch_mgs_update(chy_ctx *ctx, int nv, const T *V, int64_t stride, const d
   }
 }
 
+// w -= sum_k h_k V_k as k_mgs, also accumulating ||w||^2 of the result in
+// FP64 block partials (fixed grid: deterministic).
+template <typename T, int NV>
+__global__ __launch_bounds__(256) void k_mgs_norm(int nv, const T *__restrict__ V, int64_t stride,
+                                                  const double *__restrict__ h, T *__restrict__ w, int64_t n,
+                                                  double *__restrict__ partial)
+{
+  __shared__ double hs[NV];
+  __shared__ double sh[4];
+  if (threadIdx.x < nv) hs[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  double nn = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = (double)w[i];
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+      if (k < nv) acc -= hs[k] * (double)V[k * stride + i];
+    const T r = (T)acc;
+    w[i] = r;
+    nn += (double)r * (double)r;
+  }
+  const double s = wave_sum(nn);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+template <typename T>
+void launch_mgs_norm(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n,
+                     double *norm2_dev)
+{
+  if (nv > kMaxDots) {  // rare: split, norm from the last chunk
+    launch_mgs_update<T>(ctx, nv - kMaxDots, V, stride, h, w, n);
+    V += (int64_t)(nv - kMaxDots) * stride;
+    h += nv - kMaxDots;
+    nv = kMaxDots;
+  }
+  const unsigned nb = (unsigned)std::min<int64_t>(kRedBlocks, (n + kBlock - 1) / kBlock);
+  if (nv <= 8)
+    hipLaunchKernelGGL((k_mgs_norm<T, 8>), nb, kBlock, 0, ctx->stream, nv, V, stride, h, w, n, ctx->red);
+  else
+    hipLaunchKernelGGL((k_mgs_norm<T, kMaxDots>), nb, kBlock, 0, ctx->stream, nv, V, stride, h, w, n, ctx->red);
+  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, 1, norm2_dev);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_axpby(int64_t n, double a, const T *__restrict__ x, double b, T *__restrict__ y)
 {
@@ -551,6 +686,8 @@ template void launch_sum<double>(chy_ctx *, const double *, int64_t, double *);
 template void launch_sum<float>(chy_ctx *, const float *, int64_t, double *);
 template void launch_mgs_update<double>(chy_ctx *, int, const double *, int64_t, const double *, double *, int64_t);
 template void launch_mgs_update<float>(chy_ctx *, int, const float *, int64_t, const double *, float *, int64_t);
+template void launch_mgs_norm<double>(chy_ctx *, int, const double *, int64_t, const double *, double *, int64_t, double *);
+template void launch_mgs_norm<float>(chy_ctx *, int, const float *, int64_t, const double *, float *, int64_t, double *);
 template void launch_axpby<double>(chy_ctx *, int64_t, double, const double *, double, double *);
 template void launch_axpby<float>(chy_ctx *, int64_t, double, const float *, double, float *);
 template void launch_convert<double, float>(chy_ctx *, int64_t, const double *, double, float *);

@@ -142,6 +142,9 @@ void launch_sum(chy_ctx *ctx, const T *a, int64_t n, double *out_dev);
 template <typename T>
 void launch_mgs_update(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n);
 template <typename T>
+void launch_mgs_norm(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n,
+                     double *norm2_dev);
+template <typename T>
 void launch_axpby(chy_ctx *ctx, int64_t n, double a, const T *x, double b, T *y);
 template <typename TS, typename TD>
 void launch_convert(chy_ctx *ctx, int64_t n, const TS *src, double scale, TD *dst);

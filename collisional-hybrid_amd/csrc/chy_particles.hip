@@ -634,26 +634,52 @@ __global__ __launch_bounds__(256) void k_reorder(int64_t np, const uint32_t *__r
                                                  PState<T> a, PState<T> b, bool weighted, double n0x, double n0y,
                                                  double n0z, double vnT, double *__restrict__ sred)
 {
+  // a block takes 256-slot chunks; the permutation of the chunk and of
+  // kRankHalo slots either side is staged in LDS for the per-cell ranks
+  constexpr int kRankHalo = 128, kWin = 256 + 2 * kRankHalo;
+  __shared__ uint32_t pw[kWin];
   __shared__ double sh[4];
   double acc = 0.0;
-  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < np; s += (int64_t)gridDim.x * blockDim.x) {
-    const int c = slotkey[s];
-    const uint32_t lo = offs[c], hi = offs[c + 1];
-    const uint32_t me = perm[s];
-    uint32_t rank = 0;
-    for (uint32_t t = lo; t < hi; t++) rank += perm[t] < me ? 1u : 0u;
-    const uint32_t d = lo + rank;
-    const T vx = a.vx[me], vy = a.vy[me], vz = a.vz[me];
-    b.x[d] = a.x[me]; b.y[d] = a.y[me]; b.z[d] = a.z[me];
-    b.vx[d] = vx; b.vy[d] = vy; b.vz[d] = vz;
-    b.pid[d] = a.pid[me];
-    b.cell[d] = c;
-    if (weighted) b.w[d] = a.w[me];
-    if (SUMS) {
-      if (sizeof(T) == 4) acc += (double)mean_rel_speed_f((float)vx, (float)vy, (float)vz, (float)n0x, (float)n0y,
-                                                         (float)n0z, (float)vnT);
-      else acc += mean_rel_speed((double)vx, (double)vy, (double)vz, n0x, n0y, n0z, vnT);
+  for (int64_t s0 = (int64_t)blockIdx.x * 256; s0 < np; s0 += (int64_t)gridDim.x * 256) {
+    const int64_t wlo = s0 - kRankHalo;
+    for (int e = threadIdx.x; e < kWin; e += blockDim.x) {
+      const int64_t t = wlo + e;
+      pw[e] = (t >= 0 && t < np) ? perm[t] : 0u;
     }
+    __syncthreads();
+    const int64_t s = s0 + threadIdx.x;
+    if (s < np) {
+      const int c = slotkey[s];
+      const uint32_t lo = offs[c], hi = offs[c + 1];
+      const uint32_t me = pw[s - wlo];
+      uint32_t rank = 0;
+      if ((int64_t)lo >= wlo && (int64_t)hi <= wlo + kWin) {
+        const uint32_t *q = pw + (lo - wlo);
+        const int len = (int)(hi - lo);
+        uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+        int t = 0;
+        for (; t + 4 <= len; t += 4) {
+          r0 += q[t] < me; r1 += q[t + 1] < me; r2 += q[t + 2] < me; r3 += q[t + 3] < me;
+        }
+        for (; t < len; t++) r0 += q[t] < me;
+        rank = (r0 + r1) + (r2 + r3);
+      } else {
+        for (uint32_t t = lo; t < hi; t++) rank += perm[t] < me ? 1u : 0u;
+      }
+      const uint32_t d = lo + rank;
+      const T vx = a.vx[me], vy = a.vy[me], vz = a.vz[me];
+      b.x[d] = a.x[me]; b.y[d] = a.y[me]; b.z[d] = a.z[me];
+      b.vx[d] = vx; b.vy[d] = vy; b.vz[d] = vz;
+      b.pid[d] = a.pid[me];
+      b.cell[d] = c;
+      if (weighted) b.w[d] = a.w[me];
+      if (SUMS) {
+        if (sizeof(T) == 4) acc += (double)mean_rel_speed_f((float)vx, (float)vy, (float)vz, (float)n0x, (float)n0y,
+                                                           (float)n0z, (float)vnT);
+        else acc += mean_rel_speed((double)vx, (double)vy, (double)vz, n0x, n0y, n0z, vnT);
+      }
+    }
+    __syncthreads();
   }
   if (SUMS) {
     const double t = block_sum(acc, sh);

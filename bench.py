@@ -28,6 +28,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "collisional-hybrid_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "particle-steps/s + \u03c6-solve ms/step at 256\u00b3, mixed vs FP64 GPU; % HBM peak"
+# timers that bracket exactly one kernel launch (the others cover several)
+SINGLE_KERNEL = ("efield", "push", "collide", "sort_scatter", "sort_reorder", "deposit_cells", "deposit_nodes",
+                 "lhs_apply")
+COPY_BYTES = 1 << 30
 
 PHYS = ("-mi 4.9816e-26 -mn 4.8156e-26 -Ti 300 -Tn 300 -B0z 5e-5 -E0x 0.05 -nui 1e3 -nue 1e5 -dt 1e-5 "
         "-seed 0 --perturbation uniform --perturbation-amplitude 0.01 -ksp_rtol 1e-10")
@@ -71,7 +76,22 @@ def algorithmic_bytes(kernel, prec, np_, nc):
     return None
 
 
-def run_gpu(N, ppc, prec, steps, warmup):
+def _field_energy(phi, shape, d):
+    """sum |grad phi|^2 dV with periodic central differences (diagnostic)."""
+    import numpy as np
+
+    f = phi.reshape(shape)
+    e = 0.0
+    for ax, h in zip((2, 1, 0), d):
+        g = (np.roll(f, -1, axis=ax) - np.roll(f, 1, axis=ax)) / (2 * h)
+        e += float(np.sum(g * g))
+    return e * d[0] * d[1] * d[2]
+
+
+def run_gpu(N, ppc, prec, steps, warmup, keep=False, cross=None):
+    """Times `steps` steps.  keep: also return the final moments, rhs and phi.
+    cross: (moments, rhs) of another run, solved here from a zero guess
+    after the timed region (the FP64 answer to the mixed run's system)."""
     import chybrid
 
     prm, _ = chybrid.params_from_options(workload_args(N, ppc, prec))
@@ -97,6 +117,19 @@ def run_gpu(N, ppc, prec, steps, warmup):
             if cnt:
                 kern[nme] = {"avg_ms": tot / cnt, "launches": cnt, "total_ms": tot}
         np_ = ctx.num_particles()
+        kept = None
+        if keep:
+            kept = {"mom": ctx.get_moments(), "rhs": ctx.get_rhs(), "phi": ctx.get_potential()}
+        if cross is not None:
+            ctx.set_moments(cross["mom"])
+            kept = kept or {}
+            kept["cross_phi"], _ = ctx.solve(cross["rhs"])
+        copy = None
+        if prec == "mixed":
+            ctx.timing_reset()
+            ctx.copy_kernel(COPY_BYTES, repeat=5)
+            tot, cnt = ctx.timing_get("copy")
+            copy = {"bytes": 2 * COPY_BYTES, "avg_ms": tot / cnt, "GBs": 2 * COPY_BYTES / (tot / cnt * 1e-3) / 1e9}
     finally:
         ctx.close()
     nc = N ** 3
@@ -120,7 +153,46 @@ def run_gpu(N, ppc, prec, steps, warmup):
         "collisions_per_step": sum(s.collisions.accepted for s in stats) / steps,
         "collisions_target": stats[-1].collisions.Nc,
         "kernels": kern,
+        "copy_kernel": copy,
+        "kept": kept,
     }
+
+
+def precision_errors(m, f, N):
+    """Mixed against FP64.  'same_system': the mixed run's phi against the FP64
+    solver's phi for the mixed run's own density and forcing (solver error).
+    'after_steps': the two runs' states after the same seeded steps (FP32
+    trajectory rounding included)."""
+    import numpy as np
+
+    def rel(x, y):
+        return float(np.linalg.norm(x - y) / np.linalg.norm(y))
+
+    shape, d = (N, N, N), (0.05, 0.05, 0.05)
+    wm, wf = _field_energy(m["phi"], shape, d), _field_energy(f["phi"], shape, d)
+    wc = _field_energy(f["cross_phi"], shape, d)
+    return {
+        "same_system": {"phi": rel(m["phi"], f["cross_phi"]), "field_energy": abs(wm - wc) / wc},
+        "after_steps": {"phi": rel(m["phi"], f["phi"]), "density": rel(m["mom"][:, 0], f["mom"][:, 0]),
+                        "flux": rel(m["mom"][:, 1:], f["mom"][:, 1:]), "field_energy": abs(wm - wf) / wf},
+    }
+
+
+def measured_traffic(kernel, N, ppc):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary for this workload (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    kernel's reading), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{N}cubed_ppc{ppc}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"| {kernel} |"):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            try:
+                return float(cells[-1])
+            except ValueError:
+                return None
+    return None
 
 
 def cpu_baseline(Ns=64, ppc=16, steps=2):
@@ -183,10 +255,12 @@ def main():
         print("bench.py: this build runs the hot path on one GPU; --gpus must be 1 "
               "(multi-GPU sharding is designed, not built)", file=sys.stderr)
         sys.exit(2)
-    mixed = run_gpu(a.grid, a.ppc, "mixed", a.steps, a.warmup)
-    fp64 = None if a.no_fp64 else run_gpu(a.grid, a.ppc, "fp64", a.steps, a.warmup)
+    mixed = run_gpu(a.grid, a.ppc, "mixed", a.steps, a.warmup, keep=not a.no_fp64)
+    fp64 = None
+    if not a.no_fp64:
+        fp64 = run_gpu(a.grid, a.ppc, "fp64", a.steps, a.warmup, keep=True, cross=mixed["kept"])
     kern = mixed["kernels"]
-    dom = max((k for k in kern if "bytes" in kern[k]), key=lambda k: kern[k]["total_ms"])
+    dom = max((k for k in kern if "bytes" in kern[k] and k in SINGLE_KERNEL), key=lambda k: kern[k]["total_ms"])
     roof = {
         "kernel": dom,
         "bound": "hbm",
@@ -194,11 +268,11 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": kern[dom]["GBs"] / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": measured_traffic(dom, a.grid, a.ppc),
     }
     cpu = None if a.no_cpu else cpu_baseline()
     out = {
-        "metric": "particle-steps/s (256^3, 16 ions/cell, mixed precision) + phi-solve ms/step",
+        "metric": METRIC,
         "value": mixed["particle_steps_per_s"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -224,6 +298,7 @@ def main():
         "collisions_target_per_step": mixed["collisions_target"],
         "kernels": {k: {q: round(v, 4) if isinstance(v, float) else v for q, v in d.items()}
                     for k, d in kern.items()},
+        "copy_kernel": mixed["copy_kernel"],
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -237,6 +312,7 @@ def main():
                         for k, d in fp64["kernels"].items()},
         }
         out["mixed_speedup_vs_fp64"] = fp64["ms_per_step"] / mixed["ms_per_step"]
+        out["mixed_vs_fp64_errors"] = precision_errors(mixed["kept"], fp64["kept"], a.grid)
     print(json.dumps(out))
 
 

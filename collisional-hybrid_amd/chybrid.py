@@ -107,6 +107,8 @@ SIGNATURES = {
     "chy_timing_get": (C.c_int, [P, C.c_char_p, P, P]),
     "chy_timing_reset": (C.c_int, [P]),
     "chy_copy_kernel": (C.c_int, [P, C.c_int64, C.c_int]),
+    "chy_adjoint_forward": (C.c_int, [P, C.c_int64, C.c_int64, P]),
+    "chy_adjoint_backward": (C.c_int, [P, P, P]),
 }
 
 _LIB = None
@@ -321,6 +323,20 @@ class Context:
 
     def timing_reset(self):
         self._chk(lib().chy_timing_reset(self._p))
+
+    # adjoint ---------------------------------------------------------------
+    def adjoint_forward(self, nsteps, ckpt):
+        """Runs the step map from the current ions; returns J = sum (n - mean n)^2."""
+        J = C.c_double()
+        self._chk(lib().chy_adjoint_forward(self._p, int(nsteps), int(ckpt), C.byref(J)))
+        return J.value
+
+    def adjoint_backward(self, want_delta=True):
+        """(dJ/dE0[3], dJ/d delta per cell or None) of the last forward run."""
+        dE = np.zeros(3)
+        dd = np.zeros(self.nc) if want_delta else None
+        self._chk(lib().chy_adjoint_backward(self._p, _ptr(dE), _ptr(dd)))
+        return dE, dd
 
     def copy_kernel(self, nbytes, repeat=1):
         self._chk(lib().chy_copy_kernel(self._p, int(nbytes), int(repeat)))

@@ -67,37 +67,6 @@ namespace {
 
 template <typename T> T *mom(chy_ctx *c, int a) { return (T *)c->mom[a]; }
 
-template <typename T>
-int dalloc(chy_ctx *ctx, T **p, int64_t n)
-{
-  if (n <= 0) { *p = nullptr; return CHY_OK; }
-  const hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)n);
-  if (e != hipSuccess)
-    return set_err(ctx, CHY_ERR_MEM, "hipMalloc of " + std::to_string(sizeof(T) * (size_t)n) + " bytes: " + hipGetErrorString(e));
-  return CHY_OK;
-}
-
-#define TRY(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
-
-template <typename T>
-int alloc_state(chy_ctx *ctx, PState<T> &s, int64_t cap, bool w)
-{
-  TRY(dalloc(ctx, &s.x, cap)); TRY(dalloc(ctx, &s.y, cap)); TRY(dalloc(ctx, &s.z, cap));
-  TRY(dalloc(ctx, &s.vx, cap)); TRY(dalloc(ctx, &s.vy, cap)); TRY(dalloc(ctx, &s.vz, cap));
-  if (w) TRY(dalloc(ctx, &s.w, cap));
-  TRY(dalloc(ctx, &s.pid, cap));
-  TRY(dalloc(ctx, &s.cell, cap));
-  return CHY_OK;
-}
-
-template <typename T>
-void free_state(PState<T> &s)
-{
-  void *ps[] = {s.x, s.y, s.z, s.vx, s.vy, s.vz, s.w, s.pid, s.cell};
-  for (void *p : ps) if (p) (void)hipFree(p);
-  s = PState<T>();
-}
-
 int sync(chy_ctx *ctx)
 {
   CHY_HIP(hipStreamSynchronize(ctx->stream));
@@ -155,7 +124,7 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
     if (*its == 0) {
       CHY_HIP(hipMemcpyAsync(V, rhs, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
     } else {
-      launch_lhs_apply<T, T, T>(ctx, lt, false, n, e, w);
+      launch_lhs_apply<T, T, T>(ctx, lt, ctx->solve_transpose, n, e, w);
       CHY_HIP(hipMemcpyAsync(V, rhs, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
       launch_axpby<T>(ctx, nc, -1.0, w, 1.0, V);
     }
@@ -182,7 +151,7 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
       T *Vj = V + j * nc, *Zj = Z + j * nc, *Vn = V + (j + 1) * nc;
       TRY(pc_apply<T>(ctx, v19, Vj, Zj));
       timer_start(ctx, "lhs_apply");
-      launch_lhs_apply<T, T, T>(ctx, lt, false, n, Zj, Vn);  // w built in place in slot j+1
+      launch_lhs_apply<T, T, T>(ctx, lt, ctx->solve_transpose, n, Zj, Vn);  // w built in place in slot j+1
       timer_stop(ctx, "lhs_apply");
       timer_start(ctx, "krylov");
       // two-pass classical Gram-Schmidt (the preconditioned operator is close
@@ -268,7 +237,7 @@ int bicgstab_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *x, doubl
     launch_axpby<T>(ctx, nc, -omega, v, 1.0, p);
     launch_axpby<T>(ctx, nc, 1.0, r, beta, p);
     TRY(pc_apply<T>(ctx, v19, p, yv));
-    launch_lhs_apply<T, T, T>(ctx, lt, false, n, yv, v);
+    launch_lhs_apply<T, T, T>(ctx, lt, ctx->solve_transpose, n, yv, v);
     launch_mdot<T>(ctx, 1, rh, nc, v, nc, d);
     double rhv;
     TRY(fetch(ctx, d, 1, &rhv));
@@ -284,7 +253,7 @@ int bicgstab_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *x, doubl
     if (!std::isfinite(*relres)) return set_err(ctx, CHY_ERR_FP, "non-finite residual in BiCGStab");
     if (*relres <= rtol) return CHY_OK;
     TRY(pc_apply<T>(ctx, v19, s, zv));
-    launch_lhs_apply<T, T, T>(ctx, lt, false, n, zv, t);
+    launch_lhs_apply<T, T, T>(ctx, lt, ctx->solve_transpose, n, zv, t);
     launch_mdot<T>(ctx, 1, t, nc, s, nc, d);
     launch_mdot<T>(ctx, 1, t, nc, t, nc, d + 1);
     double ts[2];
@@ -330,7 +299,12 @@ int solve_impl(chy_ctx *ctx, chy_solve_stats *st)
   T *rhsT = (T *)ctx->t64;  // FP64-sized buffer holds the inner right-hand side
   T *ev = (T *)ctx->V + (nvec_V(ctx) - 1) * nc;
   for (;;) {
-    launch_residual<T, double>(ctx, p.lhs_type, n, ctx->b, ctx->phi, ctx->r64);
+    if (ctx->solve_transpose) {
+      launch_lhs_apply<T, double, double>(ctx, p.lhs_type, true, n, ctx->phi, ctx->r64);
+      launch_axpby<double>(ctx, nc, 1.0, ctx->b, -1.0, ctx->r64);
+    } else {
+      launch_residual<T, double>(ctx, p.lhs_type, n, ctx->b, ctx->phi, ctx->r64);
+    }
     launch_mdot<double>(ctx, 1, ctx->r64, nc, ctx->r64, nc, ctx->red_out);
     double rr;
     TRY(fetch(ctx, ctx->red_out, 1, &rr));
@@ -495,6 +469,7 @@ int chy_destroy(chy_ctx *ctx)
 {
   if (!ctx) return CHY_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  adjoint_release(ctx);
   for (int s = 0; s < 2; s++) { free_state(ctx->pd[s]); free_state(ctx->pf[s]); }
   void *ps[] = {ctx->perm, ctx->slotkey, ctx->counts, ctx->offsets, ctx->cursor, ctx->scan_tmp, ctx->maxcount, ctx->qlen,
                 ctx->partials,
@@ -713,6 +688,16 @@ int chy_get_rhs(chy_ctx *ctx, double *b)
 int chy_step(chy_ctx *ctx, chy_step_stats *st)
 {
   if (!ctx) return CHY_ERR_ARG_WRONG;
+  return chy::ctx_step(ctx, st);
+}
+
+}  // extern "C"
+
+int chy::ctx_rhs(chy_ctx *ctx) { return do_rhs(ctx); }
+int chy::ctx_solve(chy_ctx *ctx, chy_solve_stats *st) { return do_solve(ctx, st); }
+
+int chy::ctx_step(chy_ctx *ctx, chy_step_stats *st)
+{
   const auto t0 = std::chrono::steady_clock::now();
   chy_step_stats local;
   memset(&local, 0, sizeof local);
@@ -740,6 +725,8 @@ int chy_step(chy_ctx *ctx, chy_step_stats *st)
   if (st) *st = local;
   return CHY_OK;
 }
+
+extern "C" {
 
 int chy_timing_enable(chy_ctx *ctx, int on)
 {

@@ -389,6 +389,25 @@ __global__ __launch_bounds__(256) void k_rhs_const(int64_t nc, double fac, doubl
   b[c] = (nsum[0] * inv_nc) * fac;
 }
 
+// Forcing constants: magnetisation tensor R (rhs.c:205-216), cth, cG.
+RhsCoef rhs_coef(const chy_ctx *ctx)
+{
+  const chy_params &p = ctx->prm;
+  const double Kx = p.kappa_e[0], Ky = p.kappa_e[1], Kz = p.kappa_e[2];
+  const double detA = 1 + Kx * Kx + Ky * Ky + Kz * Kz;
+  RhsCoef rc;
+  rc.r[0][0] = 1 + Kx * Kx; rc.r[0][1] = Ky * Kx - Kz; rc.r[0][2] = Kz * Kx + Ky;
+  rc.r[1][0] = Kx * Ky + Kz; rc.r[1][1] = 1 + Ky * Ky; rc.r[1][2] = Kz * Ky - Kx;
+  rc.r[2][0] = Kx * Kz - Ky; rc.r[2][1] = Ky * Kz + Kx; rc.r[2][2] = 1 + Kz * Kz;
+  rc.detA = detA;
+  rc.scale = 1.0 / detA;
+  rc.cth = p.gamma_e * 1.3807e-23 * p.Te / 1.6022e-19;
+  rc.cG = p.me * p.nue / 1.6022e-19;
+  for (int a = 0; a < 3; a++) rc.E0[a] = p.E0[a];
+  rc.dx = p.d[0]; rc.dy = p.d[1]; rc.dz = p.d[2];
+  return rc;
+}
+
 template <typename T>
 void launch_rhs(chy_ctx *ctx, int rhs_type, const T *n, const T *gx, const T *gy, const T *gz, double *b)
 {
@@ -397,16 +416,7 @@ void launch_rhs(chy_ctx *ctx, int rhs_type, const T *n, const T *gx, const T *gy
   const double detA = 1 + Kx * Kx + Ky * Ky + Kz * Kz;
   const int64_t nc = ctx->g.nc;
   if (rhs_type == CHY_RHS_FULL) {
-    RhsCoef rc;
-    rc.r[0][0] = 1 + Kx * Kx; rc.r[0][1] = Ky * Kx - Kz; rc.r[0][2] = Kz * Kx + Ky;
-    rc.r[1][0] = Kx * Ky + Kz; rc.r[1][1] = 1 + Ky * Ky; rc.r[1][2] = Kz * Ky - Kx;
-    rc.r[2][0] = Kx * Kz - Ky; rc.r[2][1] = Ky * Kz + Kx; rc.r[2][2] = 1 + Kz * Kz;
-    rc.detA = detA;
-    rc.scale = 1.0 / detA;
-    rc.cth = p.gamma_e * 1.3807e-23 * p.Te / 1.6022e-19;
-    rc.cG = p.me * p.nue / 1.6022e-19;
-    for (int a = 0; a < 3; a++) rc.E0[a] = p.E0[a];
-    rc.dx = p.d[0]; rc.dy = p.d[1]; rc.dz = p.d[2];
+    const RhsCoef rc = rhs_coef(ctx);
     hipLaunchKernelGGL((k_rhs_full<T>), nblocks(nc), kBlock, 0, ctx->stream, ctx->g, rc, n, gx, gy, gz, b);
   } else {
     launch_sum<T>(ctx, n, nc, ctx->red_out);
@@ -668,6 +678,126 @@ void launch_copy(chy_ctx *ctx, int64_t n16, const void *a, void *b)
 {
   const unsigned nb = (unsigned)std::min<int64_t>(16384, (n16 + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(k_copy, nb, kBlock, 0, ctx->stream, n16, (const uint4 *)a, (uint4 *)b);
+}
+
+// ---- adjoint pieces of the grid operators (FP64) ------------------------------------
+// d/dn of mu^T A(n) phi.  Row r of A(n) is linear in the 7-point density star
+// of r, so its coefficients for unit density at star slot o are constants
+// (R.v[o]); node m appears at slot o of row r = m - star(o):
+//   out[m] += alpha * sum_o mu[r] * sum_q R.v[o][q] phi[r + off(q)].
+struct StarRows { double v[7][19]; };
+__constant__ int kStar[7][3] = {{0, 0, 0}, {+1, 0, 0}, {-1, 0, 0}, {0, +1, 0}, {0, -1, 0}, {0, 0, +1}, {0, 0, -1}};
+
+__global__ __launch_bounds__(256) void k_lhs_dn(Grid g, StarRows R, const double *__restrict__ mu,
+                                                const double *__restrict__ phi, double alpha, double *__restrict__ out)
+{
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= g.nc) return;
+  const int i = (int)(m % g.nx), j = (int)((m / g.nx) % g.ny), k = (int)(m / ((int64_t)g.nx * g.ny));
+  double acc = 0.0;
+  for (int o = 0; o < 7; o++) {
+    const int ri = i - kStar[o][0], rj = j - kStar[o][1], rk = k - kStar[o][2];
+    const double mr = mu[lin(g, ri, rj, rk)];
+    double row = 0.0;
+#pragma unroll
+    for (int q = 0; q < 19; q++) row += R.v[o][q] * phi[lin(g, ri + kOff[q][0], rj + kOff[q][1], rk + kOff[q][2])];
+    acc += mr * row;
+  }
+  out[m] += alpha * acc;
+}
+
+void launch_lhs_dn(chy_ctx *ctx, const double *mu, const double *phi, double alpha, double *out)
+{
+  if (ctx->prm.lhs_type != CHY_LHS_FULL) return;  // laplacian and identity rows do not depend on n
+  const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
+  StarRows R;
+  for (int o = 0; o < 7; o++) {
+    double e[7] = {0, 0, 0, 0, 0, 0, 0};
+    e[o] = 1.0;
+    row_from_star<double>(s, e[0], e[1], e[2], e[3], e[4], e[5], e[6], R.v[o]);
+  }
+  hipLaunchKernelGGL(k_lhs_dn, nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g, R, mu, phi, alpha, out);
+}
+
+// Transpose of the full forcing (without its mean removal, which the caller
+// applies to mu): ln += dB/dn^T mu, lG_a += dB/dG_a^T mu.  Central first
+// differences are antisymmetric (D^T = -D), second differences symmetric.
+__global__ __launch_bounds__(256) void k_rhs_full_adj(Grid g, RhsCoef rc, const double *__restrict__ mu,
+                                                      double *__restrict__ ln, double *__restrict__ lgx,
+                                                      double *__restrict__ lgy, double *__restrict__ lgz)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  const double dx = rc.dx, dy = rc.dy, dz = rc.dz;
+#define F(di, dj, dk) (mu[lin(g, i + (di), j + (dj), k + (dk))])
+  const double Dx = (F(1, 0, 0) - F(-1, 0, 0)) / (2.0 * dx);
+  const double Dy = (F(0, 1, 0) - F(0, -1, 0)) / (2.0 * dy);
+  const double Dz = (F(0, 0, 1) - F(0, 0, -1)) / (2.0 * dz);
+  double P = 0.0;
+  if (rc.cth != 0.0) {
+    const double d2xx = (F(1, 0, 0) - 2.0 * F(0, 0, 0) + F(-1, 0, 0)) / (dx * dx);
+    const double d2yy = (F(0, 1, 0) - 2.0 * F(0, 0, 0) + F(0, -1, 0)) / (dy * dy);
+    const double d2zz = (F(0, 0, 1) - 2.0 * F(0, 0, 0) + F(0, 0, -1)) / (dz * dz);
+    const double d2xy = (F(1, 1, 0) - F(-1, 1, 0) - F(1, -1, 0) + F(-1, -1, 0)) / (4.0 * dx * dy);
+    const double d2xz = (F(1, 0, 1) - F(-1, 0, 1) - F(1, 0, -1) + F(-1, 0, -1)) / (4.0 * dx * dz);
+    const double d2yz = (F(0, 1, 1) - F(0, -1, 1) - F(0, 1, -1) + F(0, -1, -1)) / (4.0 * dy * dz);
+    P = rc.cth * (rc.r[0][0] * d2xx + (rc.r[0][1] + rc.r[1][0]) * d2xy + (rc.r[0][2] + rc.r[2][0]) * d2xz +
+                  rc.r[1][1] * d2yy + (rc.r[1][2] + rc.r[2][1]) * d2yz + rc.r[2][2] * d2zz);
+  }
+#undef F
+  const double cx = rc.r[0][0] * rc.E0[0] + rc.r[0][1] * rc.E0[1] + rc.r[0][2] * rc.E0[2];
+  const double cy = rc.r[1][0] * rc.E0[0] + rc.r[1][1] * rc.E0[1] + rc.r[1][2] * rc.E0[2];
+  const double cz = rc.r[2][0] * rc.E0[0] + rc.r[2][1] * rc.E0[1] + rc.r[2][2] * rc.E0[2];
+  ln[c] += rc.scale * (P - (cx * Dx + cy * Dy + cz * Dz));
+  const double gG = -rc.scale * rc.detA * rc.cG;
+  lgx[c] += gG * Dx;
+  lgy[c] += gG * Dy;
+  lgz[c] += gG * Dz;
+}
+
+void launch_rhs_full_adj(chy_ctx *ctx, const double *mu, double *ln, double *lgx, double *lgy, double *lgz)
+{
+  hipLaunchKernelGGL(k_rhs_full_adj, nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g, rhs_coef(ctx), mu, ln, lgx,
+                     lgy, lgz);
+}
+
+// Central first differences of n along x, y, z (periodic) into three arrays.
+__global__ __launch_bounds__(256) void k_grad3(Grid g, const double *__restrict__ n, double *__restrict__ gx,
+                                               double *__restrict__ gy, double *__restrict__ gz)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  gx[c] = (n[lin(g, i + 1, j, k)] - n[lin(g, i - 1, j, k)]) / (2.0 * g.d[0]);
+  gy[c] = (n[lin(g, i, j + 1, k)] - n[lin(g, i, j - 1, k)]) / (2.0 * g.d[1]);
+  gz[c] = (n[lin(g, i, j, k + 1)] - n[lin(g, i, j, k - 1)]) / (2.0 * g.d[2]);
+}
+
+void launch_grad3(chy_ctx *ctx, const double *n, double *g3)
+{
+  const int64_t nc = ctx->g.nc;
+  hipLaunchKernelGGL(k_grad3, nblocks(nc), kBlock, 0, ctx->stream, ctx->g, n, g3, g3 + nc, g3 + 2 * nc);
+}
+
+// Transpose of E = E0 - grad(phi) (central differences) for phi:
+//   lphi[c] = sum_a (lE_a[c + e_a] - lE_a[c - e_a]) / (2 d_a).
+__global__ __launch_bounds__(256) void k_efield_adj(Grid g, const double *__restrict__ lex,
+                                                    const double *__restrict__ ley, const double *__restrict__ lez,
+                                                    double *__restrict__ lphi)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  const double ax = (lex[lin(g, i + 1, j, k)] - lex[lin(g, i - 1, j, k)]) * (0.5 / g.d[0]);
+  const double ay = (ley[lin(g, i, j + 1, k)] - ley[lin(g, i, j - 1, k)]) * (0.5 / g.d[1]);
+  const double az = (lez[lin(g, i, j, k + 1)] - lez[lin(g, i, j, k - 1)]) * (0.5 / g.d[2]);
+  lphi[c] = ax + ay + az;
+}
+
+void launch_efield_adj(chy_ctx *ctx, const double *lex, const double *ley, const double *lez, double *lphi)
+{
+  hipLaunchKernelGGL(k_efield_adj, nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g, lex, ley, lez, lphi);
 }
 
 // explicit instantiations

@@ -64,7 +64,8 @@ struct chy_ctx {
   chy::PState<double> pd[2];
   chy::PState<float> pf[2];
   int cur = 0;
-  uint32_t *perm = nullptr;     // slot -> old index
+  uint32_t *perm = nullptr;     // scatter slot -> old index (unstable order)
+  uint32_t *sort_src = nullptr; // when set, the sort records final slot -> old index here
   int32_t *slotkey = nullptr;   // slot -> cell
   uint32_t *counts = nullptr;   // nc + 1
   uint32_t *offsets = nullptr;  // nc + 1
@@ -98,6 +99,9 @@ struct chy_ctx {
 
   bool timing = false;
   std::map<std::string, chy::Timer> timers;
+
+  bool solve_transpose = false; // the Krylov solve runs on A(n)^T (adjoint)
+  void *adj = nullptr;          // adjoint tape (chy_adjoint.hip)
 };
 
 namespace chy {
@@ -117,6 +121,39 @@ inline int set_err(chy_ctx *c, int code, const std::string &msg)
     if (e_ != hipSuccess)                                                                     \
       return chy::set_err(ctx, CHY_ERR_LIB, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+#define TRY(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
+
+// device allocation helpers
+template <typename T>
+int dalloc(chy_ctx *ctx, T **p, int64_t n)
+{
+  if (n <= 0) { *p = nullptr; return CHY_OK; }
+  const hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess)
+    return set_err(ctx, CHY_ERR_MEM, "hipMalloc of " + std::to_string(sizeof(T) * (size_t)n) + " bytes: " + hipGetErrorString(e));
+  return CHY_OK;
+}
+
+
+template <typename T>
+int alloc_state(chy_ctx *ctx, PState<T> &s, int64_t cap, bool w)
+{
+  TRY(dalloc(ctx, &s.x, cap)); TRY(dalloc(ctx, &s.y, cap)); TRY(dalloc(ctx, &s.z, cap));
+  TRY(dalloc(ctx, &s.vx, cap)); TRY(dalloc(ctx, &s.vy, cap)); TRY(dalloc(ctx, &s.vz, cap));
+  if (w) TRY(dalloc(ctx, &s.w, cap));
+  TRY(dalloc(ctx, &s.pid, cap));
+  TRY(dalloc(ctx, &s.cell, cap));
+  return CHY_OK;
+}
+
+template <typename T>
+void free_state(PState<T> &s)
+{
+  void *ps[] = {s.x, s.y, s.z, s.vx, s.vy, s.vz, s.w, s.pid, s.cell};
+  for (void *p : ps) if (p) (void)hipFree(p);
+  s = PState<T>();
+}
 
 // timers (no-ops unless ctx->timing)
 void timer_start(chy_ctx *ctx, const char *name);
@@ -156,6 +193,13 @@ template <typename T>
 void launch_jacobi(chy_ctx *ctx, int lhs_type, const T *n, const T *r, T *z);
 void launch_combine_sum(chy_ctx *ctx, const double *partials, int nblk, int nv, double *out);
 void launch_copy(chy_ctx *ctx, int64_t n16, const void *a, void *b);
+RhsCoef rhs_coef(const chy_ctx *ctx);
+// adjoint pieces (FP64): out += alpha d(mu^T A(n) phi)/dn; transposed forcing;
+// central gradient of n into g3[3][nc]; transposed E-field stencil
+void launch_lhs_dn(chy_ctx *ctx, const double *mu, const double *phi, double alpha, double *out);
+void launch_rhs_full_adj(chy_ctx *ctx, const double *mu, double *ln, double *lgx, double *lgy, double *lgz);
+void launch_grad3(chy_ctx *ctx, const double *n, double *g3);
+void launch_efield_adj(chy_ctx *ctx, const double *lex, const double *ley, const double *lez, double *lphi);
 
 // ---- particle kernels (chy_particles.hip) ----
 int particles_upload(chy_ctx *ctx, int64_t np, const double *pos, const double *vel, const double *w,
@@ -167,6 +211,16 @@ int particles_move_only(chy_ctx *ctx);
 int particles_sort(chy_ctx *ctx);
 int particles_deposit(chy_ctx *ctx);
 int particles_normaliser(chy_ctx *ctx);
+
+// deposit of an arbitrary cell-sorted FP64 state (segments `offs`) into
+// out[4] = {sum w W, sum w W vx, sum w W vy, sum w W vz}
+void deposit_state(chy_ctx *ctx, const PState<double> &st, const uint32_t *offs, bool weighted, double *const out[4]);
+
+// ---- context-level operations (chy_ctx.hip) ----
+int ctx_step(chy_ctx *ctx, chy_step_stats *st);  // one step of the loop body
+int ctx_rhs(chy_ctx *ctx);                       // forcing from the current moments
+int ctx_solve(chy_ctx *ctx, chy_solve_stats *st); // A(n) phi = b (A^T when solve_transpose)
+void adjoint_release(chy_ctx *ctx);              // frees the adjoint tape
 
 // ---- FFT preconditioner (chy_fft.hip) ----
 int fft_setup(chy_ctx *ctx);

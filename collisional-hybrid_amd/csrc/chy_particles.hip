@@ -20,142 +20,11 @@
 #include <vector>
 
 #include "chy_internal.h"
+#include "chy_push.h"
 
 namespace chy {
 
 template <typename T> void launch_fill(chy_ctx *ctx, int64_t n, T *v, T val);
-
-struct PushConst {
-  double tau, t[3], s[3];          // Boris constants (particles.c:374-393)
-  double dt, cdt[3];               // dt and dt/d per axis (mixed move)
-  double mi, mn, viT, vnT, vn0[3];
-  double Nc;
-  uint32_t k0, k1, step;
-};
-
-static PushConst push_const(const chy_ctx *ctx)
-{
-  const chy_params &p = ctx->prm;
-  PushConst c;
-  c.tau = 0.5 * (p.qi / p.mi) * p.dt;
-  for (int a = 0; a < 3; a++) c.t[a] = c.tau * p.B0[a];
-  const double tt = c.t[0] * c.t[0] + c.t[1] * c.t[1] + c.t[2] * c.t[2];
-  for (int a = 0; a < 3; a++) c.s[a] = 2.0 * c.t[a] / (1 + tt);
-  c.dt = p.dt;
-  for (int a = 0; a < 3; a++) c.cdt[a] = p.dt / p.d[a];
-  c.mi = p.mi; c.mn = p.mn; c.viT = p.viT_r; c.vnT = p.vnT;
-  for (int a = 0; a < 3; a++) c.vn0[a] = p.vn0[a];
-  c.Nc = 0.0;
-  c.k0 = (uint32_t)p.seed;
-  c.k1 = (uint32_t)(p.seed >> 32);
-  c.step = (uint32_t)ctx->step;
-  return c;
-}
-
-static int64_t collision_count(const chy_params &p, int64_t Np)
-{
-  // particles.c:510-519
-  const double fc = p.nui * p.dt;
-  const double dvx = p.vi0[0] - p.vn0[0], dvy = p.vi0[1] - p.vn0[1], dvz = p.vi0[2] - p.vn0[2];
-  const double vrm = 4.0 * p.viT_r + sqrt(dvx * dvx + dvy * dvy + dvz * dvz);
-  const double den = (p.vnT + p.vn0_r) * sqrt(p.mn / p.mi);
-  if (!(den > 0.0)) return Np;
-  const double x = ((double)Np * fc * vrm) / den;
-  if (!(x < (double)Np)) return Np;
-  if (!(x > 0.0)) return 0;
-  return (int64_t)x;
-}
-
-// ---- ion-neutral collisions (reference particles.c:529-611) ---------------------
-// Neutral draw and Bernoulli decision from one Philox block: Box-Muller on the
-// top 24 bits of the four words, the decision uniform from their low bytes.
-// The decision is "collide iff u < Nc |v_r| / S" (probability clipped at 1).
-template <typename T>
-__device__ __forceinline__ void neutral_draw(const PushConst &pc, uint32_t pid, T &vnx, T &vny, T &vnz, T &udec)
-{
-  const U4 a = philox4x32_10(U4{pid, pc.step, STREAM_COLLIDE_A, 0u}, pc.k0, pc.k1);
-  const T u0 = u24<T>(a.x), u1 = u24<T>(a.y), u2 = u24<T>(a.z), u3 = u24<T>(a.w);
-  udec = u24<T>(((a.x & 0xffu) | ((a.y & 0xffu) << 8) | ((a.z & 0xffu) << 16)) << 8);
-  T r1, r2, c1, s1, c2;
-  if (sizeof(T) == 4) {
-    // hardware transcendentals: ample accuracy for a Monte Carlo neutral draw
-    r1 = __fsqrt_rn(-2.0f * __logf((float)u0));
-    r2 = __fsqrt_rn(-2.0f * __logf((float)u2));
-    const float a1 = 6.28318530717958647692f * (float)u1, a2 = 6.28318530717958647692f * (float)u3;
-    c1 = __cosf(a1); s1 = __sinf(a1); c2 = __cosf(a2);
-  } else {
-    r1 = sqrt(-2.0 * log((double)u0));
-    r2 = sqrt(-2.0 * log((double)u2));
-    const double a1 = 2.0 * M_PI * (double)u1, a2 = 2.0 * M_PI * (double)u3;
-    c1 = cos(a1); s1 = sin(a1); c2 = cos(a2);
-  }
-  const T vnT = (T)pc.vnT;
-  vnx = (T)pc.vn0[0] + vnT * (r1 * c1);
-  vny = (T)pc.vn0[1] + vnT * (r1 * s1);
-  vnz = (T)pc.vn0[2] + vnT * (r2 * c2);
-}
-
-template <typename T>
-__device__ __forceinline__ bool collide_decide(const PushConst &pc, T S, uint32_t pid, T vx, T vy, T vz, int &capped)
-{
-  T vnx, vny, vnz, ud;
-  neutral_draw<T>(pc, pid, vnx, vny, vnz, ud);
-  const T vrx = vx - vnx, vry = vy - vny, vrz = vz - vnz;
-  const T vrr = sqrt(vrx * vrx + vry * vry + vrz * vrz);
-  T p = (S > (T)0) ? ((T)pc.Nc * vrr) / S : (T)0;
-  capped = 0;
-  if (p > (T)1) { p = (T)1; capped = 1; }
-  return ud < p;
-}
-
-// Elastic scattering in the centre-of-mass frame, rotated into the frame of
-// the relative velocity (particles.c:552-611).  Returns 1 accepted, 2 refused.
-template <typename T>
-__device__ __forceinline__ int scatter(const PushConst &pc, uint32_t pid, T &vx, T &vy, T &vz)
-{
-  T vnx, vny, vnz, ud;
-  neutral_draw<T>(pc, pid, vnx, vny, vnz, ud);
-  const T vrx = vx - vnx, vry = vy - vny, vrz = vz - vnz;
-  const T vrr = sqrt(vrx * vrx + vry * vry + vrz * vrz);
-  const U4 b = philox4x32_10(U4{pid, pc.step, STREAM_COLLIDE_B, 0u}, pc.k0, pc.k1);
-  const T twopi = (T)(2.0 * M_PI);
-  const T mi = (T)pc.mi, mn = (T)pc.mn, M = mn + mi;
-  const T vcx = (vx * mi + vnx * mn) / M;
-  const T vcy = (vy * mi + vny * mn) / M;
-  const T vcz = (vz * mi + vnz * mn) / M;
-  const T ex = vx - vcx, ey = vy - vcy, ez = vz - vcz;
-  const T vcr = sqrt(ex * ex + ey * ey + ez * ez);
-  const T uz = (T)2.0 * u24<T>(b.x) - (T)1.0;
-  const T uperp = sqrt((T)1.0 - uz * uz);
-  const T uphi = twopi * u24<T>(b.y);
-  const T ux = uperp * cos(uphi), uy = uperp * sin(uphi);
-  const T costht = vrz / vrr;
-  const T sintht = sqrt((T)1.0 - costht * costht);
-  T cosphi = (T)1, sinphi = (T)0;
-  if (sintht > (T)1e-12) {
-    cosphi = vrx / (vrr * sintht);
-    sinphi = vry / (vrr * sintht);
-  }
-  const T ux1 = uz * sintht + ux * costht, uy1 = uy, uz1 = uz * costht - ux * sintht;
-  const T ux2 = ux1 * cosphi - uy1 * sinphi, uy2 = ux1 * sinphi + uy1 * cosphi, uz2 = uz1;
-  const T vfx = vcx + vcr * ux2, vfy = vcy + vcr * uy2, vfz = vcz + vcr * uz2;
-  const T vfr = sqrt(vfx * vfx + vfy * vfy + vfz * vfz);
-  if (vfr / (T)pc.viT > (T)10.0) return 2;
-  vx = vfx; vy = vfy; vz = vfz;
-  return 1;
-}
-
-__device__ __forceinline__ double block_sum(double v, double *sh)
-{
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-  __syncthreads();
-  return r;
-}
 
 // Wave-aggregated integer atomics on per-cell counters for runs of equal keys.
 __device__ __forceinline__ void count_key(uint32_t *counts, int key)
@@ -200,59 +69,6 @@ __device__ __forceinline__ void queue_push(QueueLds &q, uint32_t *queue, uint32_
   }
   __syncthreads();
   if (q.n > kQBuf - 256) queue_flush(q, queue, qlen);  // room for one more block of appends
-}
-
-template <typename T> struct alignas(4 * sizeof(T)) Vec4 { T x, y, z, w; };
-
-// Boris rotation (particles.c:374-447 with the intended cross products and
-// per-component v-minus); operation order shared with the oracle.
-template <typename T>
-__device__ __forceinline__ void boris(const PushConst &pc, T e0, T e1, T e2, T &vx, T &vy, T &vz)
-{
-  const T tau = (T)pc.tau, t0 = (T)pc.t[0], t1 = (T)pc.t[1], t2 = (T)pc.t[2];
-  const T s0 = (T)pc.s[0], s1 = (T)pc.s[1], s2 = (T)pc.s[2];
-  const T vm0 = vx + tau * e0, vm1 = vy + tau * e1, vm2 = vz + tau * e2;
-  const T vp0 = vm0 + (vm1 * t2 - vm2 * t1);
-  const T vp1 = vm1 + (vm2 * t0 - vm0 * t2);
-  const T vp2 = vm2 + (vm0 * t1 - vm1 * t0);
-  const T vq0 = vm0 + (vp1 * s2 - vp2 * s1);
-  const T vq1 = vm1 + (vp2 * s0 - vp0 * s2);
-  const T vq2 = vm2 + (vp0 * s1 - vp1 * s0);
-  vx = vq0 + tau * e0;
-  vy = vq1 + tau * e1;
-  vz = vq2 + tau * e2;
-}
-
-// Trilinear gather from the interleaved node field, corner order q = 0..7.
-template <typename T>
-__device__ __forceinline__ void gather(const Grid &g, const Vec4<T> *__restrict__ E4, int i, int j, int k, T fx, T fy,
-                                       T fz, T &e0, T &e1, T &e2)
-{
-  T W[8];
-  corner_weights<T>(fx, fy, fz, W);
-  e0 = (T)0; e1 = (T)0; e2 = (T)0;
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const Vec4<T> e = E4[lin(g, i + (q & 1), j + ((q >> 1) & 1), k + (q >> 2))];
-    e0 += W[q] * e.x;
-    e1 += W[q] * e.y;
-    e2 += W[q] * e.z;
-  }
-}
-
-// FP64 absolute move and cell key.
-__device__ __forceinline__ int move_fp64(const Grid &g, double dt, double &x, double &y, double &z, double vx,
-                                         double vy, double vz)
-{
-  x = wrap_pos(x + vx * dt, g.p0[0], g.p1[0], g.L[0]);
-  y = wrap_pos(y + vy * dt, g.p0[1], g.p1[1], g.L[1]);
-  z = wrap_pos(z + vz * dt, g.p0[2], g.p1[2], g.L[2]);
-  int i, j, k;
-  double f;
-  cell_coord(x, g.p0[0], g.d[0], g.nx, i, f);
-  cell_coord(y, g.p0[1], g.d[1], g.ny, j, f);
-  cell_coord(z, g.p0[2], g.d[2], g.nz, k, f);
-  return (int)(i + (int64_t)g.nx * (j + (int64_t)g.ny * k));
 }
 
 // Mixed move: in-cell offset += v dt / d with an exact integer carry.
@@ -632,7 +448,8 @@ template <typename T, bool SUMS>
 __global__ __launch_bounds__(256) void k_reorder(int64_t np, const uint32_t *__restrict__ perm,
                                                  const int32_t *__restrict__ slotkey, const uint32_t *__restrict__ offs,
                                                  PState<T> a, PState<T> b, bool weighted, double n0x, double n0y,
-                                                 double n0z, double vnT, double *__restrict__ sred)
+                                                 double n0z, double vnT, double *__restrict__ sred,
+                                                 uint32_t *__restrict__ src)
 {
   // a block takes 256-slot chunks; the permutation of the chunk and of
   // kRankHalo slots either side is staged in LDS for the per-cell ranks
@@ -667,6 +484,7 @@ __global__ __launch_bounds__(256) void k_reorder(int64_t np, const uint32_t *__r
         for (uint32_t t = lo; t < hi; t++) rank += perm[t] < me ? 1u : 0u;
       }
       const uint32_t d = lo + rank;
+      if (src) src[d] = me;  // source index of each slot, when asked for
       const T vx = a.vx[me], vy = a.vy[me], vz = a.vz[me];
       b.x[d] = a.x[me]; b.y[d] = a.y[me]; b.z[d] = a.z[me];
       b.vx[d] = vx; b.vy[d] = vy; b.vz[d] = vz;
@@ -862,7 +680,8 @@ int particles_sort(chy_ctx *ctx)
   timer_start(ctx, "sort_reorder");
 #define REORDER(T, S, st)                                                                                  \
   hipLaunchKernelGGL((k_reorder<T, S>), nb, kBlock, 0, ctx->stream, np, ctx->perm, ctx->slotkey, ctx->offsets, \
-                     st[ctx->cur], st[1 - ctx->cur], ctx->weighted, p.vn0[0], p.vn0[1], p.vn0[2], p.vnT, ctx->sred)
+                     st[ctx->cur], st[1 - ctx->cur], ctx->weighted, p.vn0[0], p.vn0[1], p.vn0[2], p.vnT, ctx->sred, \
+                     ctx->sort_src)
   if (ctx->mixed) { if (sums) REORDER(float, true, ctx->pf); else REORDER(float, false, ctx->pf); }
   else { if (sums) REORDER(double, true, ctx->pd); else REORDER(double, false, ctx->pd); }
 #undef REORDER
@@ -903,6 +722,15 @@ int particles_deposit(chy_ctx *ctx)
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("deposit: ") + hipGetErrorString(e));
   return CHY_OK;
+}
+
+void deposit_state(chy_ctx *ctx, const PState<double> &st, const uint32_t *offs, bool weighted, double *const out[4])
+{
+  const unsigned nb = (unsigned)((ctx->g.nc + kBlock - 1) / kBlock);
+  const unsigned nbc = (unsigned)((ctx->g.nc + kDepCells - 1) / kDepCells);
+  double *P = (double *)ctx->partials;
+  hipLaunchKernelGGL((k_cell_partials<double, true>), nbc, kBlock, 0, ctx->stream, ctx->g, offs, st, weighted, P);
+  hipLaunchKernelGGL((k_node_sum<double>), nb, kBlock, 0, ctx->stream, ctx->g, P, out[0], out[1], out[2], out[3]);
 }
 
 int particles_normaliser(chy_ctx *ctx)

@@ -196,6 +196,23 @@ int chy_get_rhs(chy_ctx *ctx, double *b);
  * moments, potential), fused on the device. */
 int chy_step(chy_ctx *ctx, chy_step_stats *stats);
 
+/* ---- adjoint of the deterministic step map --------------------------------
+ * No reference counterpart: the reference has no derivatives.  The forward
+ * map starts from the current cell-sorted ions: CIC moments and the potential
+ * solve from a zero guess (hybrid.c:94-99), then nsteps steps of the loop
+ * body (hybrid.c:101-123).  The objective is the final wave energy
+ * J = sum_c (n_c - mean n)^2 of the deposited density.
+ * chy_adjoint_forward runs the map, keeps a checkpoint every ckpt steps and
+ * returns J.  chy_adjoint_backward returns dJ/dE0 (3 values) and, when
+ * dJ_ddelta is not NULL, dJ/d delta_c for every cell ([Nz][Ny][Nx]), where
+ * delta_c is an additive perturbation of the weight of each ion in cell c at
+ * the start of the forward run.  Collision events and scattering angles are
+ * frozen (common random numbers).  Segments between checkpoints are
+ * recomputed; the context is left at the end state of the forward run.
+ * FP64 contexts with the full operator and forcing only (CHY_ERR_SUP). */
+int chy_adjoint_forward(chy_ctx *ctx, int64_t nsteps, int64_t ckpt, double *J);
+int chy_adjoint_backward(chy_ctx *ctx, double dJ_dE0[3], double *dJ_ddelta);
+
 /* ---- output (OutputHDF5, reference fileio.c:73-126) -------------------------
  * Writes <stem><insert>.hdf with datasets "density", "x flux", "y flux",
  * "z flux", "rhs" and "potential", each (Nz, Ny, Nx) f64.  HDF5 is loaded at

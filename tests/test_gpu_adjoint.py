@@ -1,0 +1,137 @@
+"""Adjoint of the step map through the C ABI (chy_adjoint_forward/backward):
+gradients of the final wave energy J = sum (n - mean n)^2 with respect to E0
+and the per-cell initial weight perturbation, against central finite
+differences of the oracle's forward map run with the same random streams
+(collision events frozen), plus a closed form and checkpoint invariance."""
+import numpy as np
+import pytest
+
+import chybrid
+import orc
+from gpu_cases import as_particles, context, grid_of, opts, push_params_of, relmax
+
+pytestmark = pytest.mark.gpu
+
+BASE = "-seed 3 -ppc 8 --perturbation gaussian --perturbation-amplitude 0.01 -ksp_rtol 1e-12"
+
+
+def _oracle_J(prm, parts, E0, nsteps, collide):
+    """The forward map on the oracle: moments and a zero-guess solve of the
+    given (cell-sorted) ions, then nsteps steps; J of the final density."""
+    g = grid_of(prm)
+    K = list(prm.kappa_e)
+    cth = prm.gamma_e * orc.KB * prm.Te / orc.Q
+    cG = prm.me * prm.nue / orc.Q
+    _, offs = orc.sort_perm(g.nc, orc.cell_keys(g, parts))
+    mom = orc.deposit_sorted(g, parts, offs)
+    b = orc.rhs(g, K, E0, cth, cG, *mom)
+    phi, _, conv, _ = orc.solve(g, K, mom[0], b, rtol=1e-12)
+    assert conv
+    S = sum(orc.mean_rel_speed((parts.vx[s], parts.vy[s], parts.vz[s]), list(prm.vn0), prm.vnT)
+            for s in range(parts.np))
+    cur = parts
+    for it in range(nsteps):
+        _, P = push_params_of(prm, step=it, collide=collide)
+        P.E0[0], P.E0[1], P.E0[2] = E0
+        cur, _, mom, S, _ = orc.step(g, P, cur, phi, S, E0)
+        b = orc.rhs(g, K, E0, cth, cG, *mom)
+        phi, _, conv, _ = orc.solve(g, K, mom[0], b, phi0=phi, rtol=1e-12)
+        assert conv
+    n = mom[0]
+    return float(np.sum((n - n.mean()) ** 2))
+
+
+def _with_weights(p, w):
+    q = p.copy()
+    q.w = w
+    return q
+
+
+@pytest.mark.parametrize("collide", [False, True])
+def test_adjoint_matches_oracle_finite_differences(collide):
+    nsteps, ckpt = 4, 2
+    extra = BASE + ("" if collide else " -mn 0")
+    with context(opts((8, 8, 8), extra=extra)) as ctx:
+        ctx.init_particles()
+        p0 = as_particles(ctx.get_particles())
+        prm = ctx.prm
+        J = ctx.adjoint_forward(nsteps, ckpt)
+        dE, dd = ctx.adjoint_backward()
+    g = grid_of(prm)
+    E0 = list(prm.E0)
+    J_or = _oracle_J(prm, p0, E0, nsteps, collide)
+    assert abs(J - J_or) <= 1e-10 * J_or, (J, J_or)
+    # dJ/dE0: central differences with h = 1e-6 |E0|
+    h = 1e-6 * np.linalg.norm(E0)
+    fd = np.zeros(3)
+    for a in range(3):
+        ep, em = list(E0), list(E0)
+        ep[a] += h
+        em[a] -= h
+        fd[a] = (_oracle_J(prm, p0, ep, nsteps, collide) - _oracle_J(prm, p0, em, nsteps, collide)) / (2 * h)
+    scale = np.abs(fd).max()
+    assert scale > 0
+    assert np.abs(dE - fd).max() <= 1e-5 * scale, (dE, fd)
+    # dJ/d delta_c on random cells: the weights of the cell's ions at t = 0
+    key0 = orc.cell_keys(g, p0)
+    rng = np.random.default_rng(5)
+    cells = rng.choice(g.nc, 6, replace=False)
+    hw = 1e-6
+    for c in cells:
+        sel = key0 == c
+        wp, wm = p0.w.copy(), p0.w.copy()
+        wp[sel] += hw
+        wm[sel] -= hw
+        f = (_oracle_J(prm, _with_weights(p0, wp), E0, nsteps, collide)
+             - _oracle_J(prm, _with_weights(p0, wm), E0, nsteps, collide)) / (2 * hw)
+        assert abs(dd[c] - f) <= 1e-5 * max(abs(f), np.abs(dd).max() * 1e-2), (c, dd[c], f)
+
+
+def test_adjoint_zero_steps_closed_form():
+    """nsteps = 0: J = sum (n - mean n)^2 of the deposit itself, so
+    dJ/d delta_c = sum over the cell's ions of gather(2 (n - mean n)), and
+    dJ/dE0 = 0."""
+    with context(opts((6, 5, 7), extra=BASE)) as ctx:
+        ctx.init_particles()
+        p = as_particles(ctx.get_particles())
+        J = ctx.adjoint_forward(0, 1)
+        dE, dd = ctx.adjoint_backward()
+        g = grid_of(ctx.prm)
+    _, offs = orc.sort_perm(g.nc, orc.cell_keys(g, p))
+    n = orc.deposit_sorted(g, p, offs)[0]
+    assert J == pytest.approx(float(np.sum((n - n.mean()) ** 2)), rel=1e-13)
+    lam = 2 * (n - n.mean())
+    z = np.zeros(g.nc)
+    gl = np.array([orc.gather(g, lam, z, z, p.x[s], p.y[s], p.z[s])[0] for s in range(p.np)])
+    ref = np.bincount(orc.cell_keys(g, p), weights=gl, minlength=g.nc)
+    assert relmax(dd, ref) <= 1e-12
+    assert np.all(dE == 0.0)
+
+
+def test_adjoint_checkpoint_interval_invariance_and_end_state():
+    """Recomputation from checkpoints is exact: every checkpoint interval
+    gives the same gradients, and the context ends at the forward end state."""
+    extra = BASE
+    res = []
+    for ckpt in (1, 2, 5):
+        with context(opts((8, 8, 8), extra=extra)) as ctx:
+            ctx.init_particles()
+            J = ctx.adjoint_forward(5, ckpt)
+            end = as_particles(ctx.get_particles())
+            phi_end = ctx.get_potential()
+            dE, dd = ctx.adjoint_backward()
+            after = as_particles(ctx.get_particles())
+            assert np.array_equal(after.pid, end.pid) and np.array_equal(after.x, end.x)
+            assert np.array_equal(ctx.get_potential(), phi_end)
+            res.append((J, dE, dd))
+    for J, dE, dd in res[1:]:
+        assert J == res[0][0]
+        assert np.array_equal(dE, res[0][1]) and np.array_equal(dd, res[0][2])
+
+
+def test_adjoint_requires_fp64():
+    with context(opts((8, 8, 8), extra=BASE + " -precision mixed")) as ctx:
+        ctx.init_particles()
+        with pytest.raises(chybrid.ChyError) as e:
+            ctx.adjoint_forward(2, 1)
+        assert e.value.code == chybrid.ERR_SUP

@@ -158,6 +158,105 @@ def run_gpu(N, ppc, prec, steps, warmup, keep=False, cross=None):
     }
 
 
+def adjoint_check(N=64, ppc=16, steps=50, ckpt=10, ncoords=64, seed=0):
+    """Adjoint of the step map on the differentiation workload: N^3 x ppc ions
+    (FP64), 1 % Gaussian per-cell weight perturbation, `steps` steps with
+    collisions (frozen under common random numbers), checkpoints every
+    `ckpt`.  Times forward and backward, then compares the gradient with
+    central differences of the device forward map on `ncoords` coordinates
+    (the 3 components of E0 and random cells) at two step sizes each.  The
+    finite differences themselves are limited by rounding for small steps and
+    by cell-face crossings of the CIC weights (kinks of J) and collision
+    decisions flipping inside +-h for large ones."""
+    import math
+
+    import numpy as np
+    import chybrid
+
+    args = workload_args(N, ppc, "fp64") + ["--perturbation", "gaussian", "-ksp_rtol", "1e-12", "-seed", str(seed),
+                                            "-Nt", str(steps)]
+    prm, _ = chybrid.params_from_options(args)
+    E0 = [prm.E0[a] for a in range(3)]
+
+    def J_host(ctx):
+        n = ctx.get_moments()[:, 0]
+        m = math.fsum(n) / n.size
+        return math.fsum(((n - m) ** 2).tolist())
+
+    def forward_J(ctx, p0, w):
+        ctx.set_particles(p0["x"], p0["y"], p0["z"], p0["vx"], p0["vy"], p0["vz"], w=w, pid=p0["pid"])
+        ctx.set_step(0)
+        ctx.adjoint_forward(steps, steps)
+        return J_host(ctx)
+
+    hws = (1e-6, 1e-5)            # weight steps (the unperturbed weight is ~1)
+    hEs = (1e-6, 1e-7)            # E0 steps relative to |E0|
+    with chybrid.Context(prm) as ctx:
+        ctx.init_particles()
+        p0 = ctx.get_particles()
+        ctx.sync()
+        t0 = time.perf_counter()
+        J = ctx.adjoint_forward(steps, ckpt)
+        ctx.sync()
+        t1 = time.perf_counter()
+        dE, dd = ctx.adjoint_backward()
+        t2 = time.perf_counter()
+        J_chk = J_host(ctx)
+        rng = np.random.default_rng(seed + 1)
+        cells = rng.choice(ctx.nc, ncoords - 3, replace=False)
+        g = (np.floor((p0["x"] - prm.p0[0]) / prm.d[0]).astype(np.int64) % prm.N[0]
+             + prm.N[0] * ((np.floor((p0["y"] - prm.p0[1]) / prm.d[1]).astype(np.int64) % prm.N[1])
+                           + prm.N[1] * (np.floor((p0["z"] - prm.p0[2]) / prm.d[2]).astype(np.int64) % prm.N[2])))
+        fd_d = np.zeros((len(hws), len(cells)))
+        for q, c in enumerate(cells):
+            sel = g == c
+            for k, hw in enumerate(hws):
+                wp, wm = p0["w"].copy(), p0["w"].copy()
+                wp[sel] += hw
+                wm[sel] -= hw
+                fd_d[k, q] = (forward_J(ctx, p0, wp) - forward_J(ctx, p0, wm)) / (2 * hw)
+            if q % 16 == 15:
+                print(f"adjoint check: {q + 1}/{len(cells)} cells", file=sys.stderr, flush=True)
+    nE = math.sqrt(sum(e * e for e in E0))
+    fd_E = np.zeros((len(hEs), 3))
+    for k, hr in enumerate(hEs):
+        hE = hr * nE
+        for a in range(3):
+            Jpm = []
+            for sgn in (1, -1):
+                e = list(E0)
+                e[a] += sgn * hE
+                pa, _ = chybrid.params_from_options(args + ["-E0x", repr(e[0]), "-E0y", repr(e[1]), "-E0z", repr(e[2])])
+                with chybrid.Context(pa) as cx:
+                    Jpm.append(forward_J(cx, p0, p0["w"]))
+            fd_E[k, a] = (Jpm[0] - Jpm[1]) / (2 * hE)
+    ad_d = dd[cells]
+    res = {
+        "workload": f"{N}^3 x {ppc} ions/cell, FP64, gaussian 1 % per-cell perturbation, {steps} steps, "
+                    f"checkpoints every {ckpt}, collisions frozen (common random numbers)",
+        "J": J, "J_host_fsum_rel_diff": abs(J - J_chk) / J_chk,
+        "forward_s": t1 - t0, "backward_s": t2 - t1,
+        "coordinates": int(3 + len(cells)),
+        "dJ_dE0": dE.tolist(),
+        "E0": {}, "delta": {},
+    }
+    for k, hr in enumerate(hEs):
+        f = fd_E[k]
+        res["E0"][f"h={hr:g}|E0|"] = {
+            "fd": f.tolist(),
+            "max_err_rel_to_norm": float(np.abs(dE - f).max() / np.linalg.norm(f)),
+            "max_err_rel_each": float((np.abs(dE - f) / np.abs(f)).max()),
+        }
+    for k, hw in enumerate(hws):
+        f = fd_d[k]
+        res["delta"][f"h={hw:g}"] = {
+            "max_err_rel_to_max": float(np.abs(ad_d - f).max() / np.abs(dd).max()),
+            "median_err_rel_each": float(np.median(np.abs(ad_d - f) / np.abs(f))),
+            "max_err_rel_each": float((np.abs(ad_d - f) / np.abs(f)).max()),
+        }
+    return res
+
+
 def precision_errors(m, f, N):
     """Mixed against FP64.  'same_system': the mixed run's phi against the FP64
     solver's phi for the mixed run's own density and forcing (solver error).
@@ -250,6 +349,7 @@ def main():
     ap.add_argument("--ppc", type=int, default=16)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain-path comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradient check")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the hot path on one GPU; --gpus must be 1 "
@@ -271,6 +371,12 @@ def main():
         "traffic": measured_traffic(dom, a.grid, a.ppc),
     }
     cpu = None if a.no_cpu else cpu_baseline()
+    adj = None
+    if not a.no_adjoint:
+        try:
+            adj = adjoint_check()
+        except Exception as e:  # reported, never fatal to the headline line
+            adj = {"error": repr(e)}
     out = {
         "metric": METRIC,
         "value": mixed["particle_steps_per_s"],
@@ -301,6 +407,7 @@ def main():
         "copy_kernel": mixed["copy_kernel"],
         "roofline": roof,
         "cpu_baseline": cpu,
+        "adjoint": adj,
     }
     if fp64 is not None:
         out["fp64_plain"] = {

@@ -491,16 +491,11 @@ int chy_adjoint_forward(chy_ctx *ctx, int64_t nsteps, int64_t ckpt, double *J)
   t->np = np;
   const int64_t nck = std::max<int64_t>(1, (nsteps + t->ckpt - 1) / t->ckpt);
 #define A(x) do { int r_ = (x); if (r_) { adjoint_release(ctx); return r_; } } while (0)
+  // checkpoints and the end state now; the segment and adjoint buffers are
+  // allocated by the first backward sweep
   t->ck.resize(nck);
   for (Stage &s : t->ck) A(alloc_stage(ctx, s, np));
-  t->seg.resize(t->ckpt);
-  for (Stage &s : t->seg) A(alloc_stage(ctx, s, np));
-  t->perm.assign(t->ckpt, nullptr);
-  for (uint32_t *&p : t->perm) A(dalloc(ctx, &p, np));
   A(alloc_stage(ctx, t->fin, np));
-  A(alloc_adp(ctx, t, t->lam[0], np));
-  A(alloc_adp(ctx, t, t->lam[1], np));
-  for (int a = 0; a < 3; a++) A(grid_alloc(ctx, t, &t->lE[a], np));
   A(grid_alloc(ctx, t, &t->ln, nc));
   A(grid_alloc(ctx, t, &t->lg, 3 * nc));
   A(grid_alloc(ctx, t, &t->lphi, nc));
@@ -537,6 +532,16 @@ int chy_adjoint_backward(chy_ctx *ctx, double dJ_dE0[3], double *dJ_ddelta)
   if (!t || !t->done_forward) return set_err(ctx, CHY_ERR_ARG_WRONG, "run chy_adjoint_forward first");
   if (ctx->np != t->np) return set_err(ctx, CHY_ERR_ARG_INCOMP, "the ion count changed since the forward run");
   const int64_t N = t->nsteps, K = t->ckpt, np = t->np, nc = ctx->g.nc;
+  if (t->seg.empty()) {
+    t->seg.resize(K);
+    t->perm.assign(K, nullptr);
+    int rc = CHY_OK;
+    for (Stage &s : t->seg) if (!rc) rc = alloc_stage(ctx, s, np);
+    for (uint32_t *&p : t->perm) if (!rc) rc = dalloc(ctx, &p, np);
+    for (int a = 0; a < 2 && !rc; a++) rc = alloc_adp(ctx, t, t->lam[a], np);
+    for (int a = 0; a < 3 && !rc; a++) rc = grid_alloc(ctx, t, &t->lE[a], np);
+    if (rc) { adjoint_release(ctx); return rc; }
+  }
   double lE0[3] = {0.0, 0.0, 0.0};
   int cur = 0;  // lam[cur]: adjoint of the ions of the latest stage processed
   for (int a = 0; a < 2; a++) {

@@ -48,7 +48,7 @@ class Params(C.Structure):
         ("ksp_rtol", C.c_double), ("ksp_atol", C.c_double),
         ("ksp_max_it", C.c_int64), ("ksp_restart", C.c_int64),
         ("ksp_monitor", C.c_int32), ("ksp_converged_reason", C.c_int32),
-        ("ksp_error_if_not_converged", C.c_int32),
+        ("ksp_error_if_not_converged", C.c_int32), ("pc_cheby_degree", C.c_int32),
         ("seed", C.c_uint64), ("ppc", C.c_int64), ("perturbation", C.c_int32),
         ("perturbation_amplitude", C.c_double), ("weighted", C.c_int32),
         ("precision", C.c_int32), ("output_interval", C.c_int64),
@@ -108,6 +108,7 @@ SIGNATURES = {
     "chy_timing_reset": (C.c_int, [P]),
     "chy_copy_kernel": (C.c_int, [P, C.c_int64, C.c_int]),
     "chy_adjoint_forward": (C.c_int, [P, C.c_int64, C.c_int64, P]),
+    "chy_pc_apply": (C.c_int, [P, P, P]),
     "chy_adjoint_backward": (C.c_int, [P, P, P]),
 }
 
@@ -288,6 +289,12 @@ class Context:
         st = SolveStats()
         self._chk(lib().chy_solve(self._p, _ptr(b), _ptr(phi), C.byref(st)))
         return phi, st
+
+    def pc_apply(self, r):
+        r = _f64(r)
+        z = np.zeros(self.nc)
+        self._chk(lib().chy_pc_apply(self._p, _ptr(r), _ptr(z)))
+        return z
 
     def get_potential(self):
         phi = np.zeros(self.nc)

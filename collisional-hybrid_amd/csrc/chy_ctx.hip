@@ -85,9 +85,16 @@ int pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
   timer_start(ctx, "pc");
   int rc = CHY_OK;
   const int pc = ctx->prm.pc_type;
+  T *sb = (T *)ctx->pc_buf;
   if (pc == CHY_PC_FFT && ctx->fft_ok && ctx->prm.lhs_type != CHY_LHS_IDENTITY) {
     rc = fft_pc_apply<T>(ctx, v19, r, z);
-  } else if (pc == CHY_PC_JACOBI || (pc == CHY_PC_FFT && !ctx->fft_ok)) {
+  } else if (pc == CHY_PC_ZLINE || (pc == CHY_PC_FFT && !ctx->fft_ok)) {
+    // grids the FFT does not take fall back to the z-line blocks
+    launch_zline<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), r, z, sb, sb + nc);
+  } else if (pc == CHY_PC_CHEBY) {
+    launch_cheby<T>(ctx, ctx->prm.lhs_type, ctx->solve_transpose, ctx->prm.pc_cheby_degree, mom<T>(ctx, 0), r, z, sb,
+                    sb + nc, sb + 2 * nc);
+  } else if (pc == CHY_PC_JACOBI) {
     launch_jacobi<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), r, z);
   } else {
     CHY_HIP(hipMemcpyAsync(z, r, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
@@ -272,6 +279,19 @@ int bicgstab_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *x, doubl
   return CHY_OK;
 }
 
+// Row of the constant-coefficient operator at the mean density (the FFT
+// preconditioner's symbol).
+template <typename T>
+int pc_symbol(chy_ctx *ctx, double v19[19])
+{
+  const int64_t nc = ctx->g.nc;
+  launch_sum<T>(ctx, mom<T>(ctx, 0), nc, ctx->red_out);
+  double s;
+  TRY(fetch(ctx, ctx->red_out, 1, &s));
+  uniform_row(ctx->prm.lhs_type, ctx->prm.d, ctx->prm.kappa_e, s / (double)nc, v19);
+  return CHY_OK;
+}
+
 // Outer FP64 defect correction; phi (device, FP64) in/out, b (device) mean-free.
 template <typename T>
 int solve_impl(chy_ctx *ctx, chy_solve_stats *st)
@@ -280,14 +300,8 @@ int solve_impl(chy_ctx *ctx, chy_solve_stats *st)
   const int64_t nc = ctx->g.nc;
   const chy_params &p = ctx->prm;
   const T *n = mom<T>(ctx, 0);
-  // preconditioner symbol from the mean density
   double v19[19];
-  {
-    launch_sum<T>(ctx, n, nc, ctx->red_out);
-    double s;
-    TRY(fetch(ctx, ctx->red_out, 1, &s));
-    uniform_row(p.lhs_type, p.d, p.kappa_e, s / (double)nc, v19);
-  }
+  TRY(pc_symbol<T>(ctx, v19));
   launch_mdot<double>(ctx, 1, ctx->b, nc, ctx->b, nc, ctx->red_out);
   double bb;
   TRY(fetch(ctx, ctx->red_out, 1, &bb));
@@ -449,6 +463,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
     A(dalloc(ctx, &q, (int64_t)ts * nc * ctx->restart)); ctx->Z = q;
     A(dalloc(ctx, &q, (int64_t)ts * nc)); ctx->wv = q;
     A(dalloc(ctx, &q, (int64_t)ts * 2 * nc)); ctx->fft_buf = q;
+    A(dalloc(ctx, &q, (int64_t)ts * 3 * nc)); ctx->pc_buf = q;
   }
   A(dalloc(ctx, &ctx->red, (int64_t)kRedBlocks * 32));
   A(dalloc(ctx, &ctx->red_out, 3 * (ctx->restart + 2) + 16));
@@ -475,7 +490,7 @@ int chy_destroy(chy_ctx *ctx)
                 ctx->partials,
                 ctx->sred, ctx->S_dev, ctx->cstats, ctx->mom[0], ctx->mom[1], ctx->mom[2], ctx->mom[3],
                 ctx->efield[0], ctx->phi, ctx->b, ctx->r64, ctx->t64, ctx->V,
-                ctx->Z, ctx->wv, ctx->fft_buf, ctx->red, ctx->red_out, ctx->tw[0], ctx->tw[1], ctx->tw[2],
+                ctx->Z, ctx->wv, ctx->fft_buf, ctx->pc_buf, ctx->red, ctx->red_out, ctx->tw[0], ctx->tw[1], ctx->tw[2],
                 ctx->trig[0], ctx->trig[1], ctx->trig[2]};
   for (void *p : ps) if (p) (void)hipFree(p);
   if (ctx->h_red) (void)hipHostFree(ctx->h_red);
@@ -660,6 +675,28 @@ int chy_solve(chy_ctx *ctx, const double *b, double *phi, chy_solve_stats *st)
   if (rc) return rc;
   CHY_HIP(hipMemcpy(phi, ctx->phi, sizeof(double) * nc, hipMemcpyDeviceToHost));
   return CHY_OK;
+}
+
+int chy_pc_apply(chy_ctx *ctx, const double *r, double *z)
+{
+  if (!ctx || !r || !z) return CHY_ERR_ARG_WRONG;
+  if (!ctx->moments_valid && ctx->prm.lhs_type == CHY_LHS_FULL)
+    return set_err(ctx, CHY_ERR_ARG_WRONG, "no density: collect or set moments first");
+  const int64_t nc = ctx->g.nc;
+  CHY_HIP(hipMemcpy(ctx->r64, r, sizeof(double) * nc, hipMemcpyHostToDevice));
+  double v19[19];
+  if (ctx->mixed) {
+    TRY(pc_symbol<float>(ctx, v19));
+    float *rT = (float *)ctx->t64, *zT = (float *)ctx->wv;
+    launch_convert<double, float>(ctx, nc, ctx->r64, 1.0, rT);
+    TRY(pc_apply<float>(ctx, v19, rT, zT));
+    launch_convert<float, double>(ctx, nc, zT, 1.0, ctx->t64);
+  } else {
+    TRY(pc_symbol<double>(ctx, v19));
+    TRY(pc_apply<double>(ctx, v19, ctx->r64, ctx->t64));
+  }
+  CHY_HIP(hipMemcpyAsync(z, ctx->t64, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream));
+  return sync(ctx);
 }
 
 int chy_get_potential(chy_ctx *ctx, double *phi)

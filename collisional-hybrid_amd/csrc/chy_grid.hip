@@ -800,12 +800,146 @@ void launch_efield_adj(chy_ctx *ctx, const double *lex, const double *ley, const
   hipLaunchKernelGGL(k_efield_adj, nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g, lex, ley, lez, lphi);
 }
 
+// ---- z-line block Jacobi and Chebyshev-Jacobi preconditioners --------------------
+// z-line: one thread per (i, j) column solves the column's z-line block of
+// A(n) -- diagonal v18, -z entry v5, +z entry v4, periodic -- by the
+// Sherman-Morrison formula around two Thomas sweeps:
+//   A = T + u w^T, u = (g, 0.., 0, c_{m-1}), w = (1, 0.., 0, a_0 / g), g = -b_0,
+// T = A without the wrap entries and with b_0 - g, b_{m-1} - c_{m-1} a_0 / g.
+// y = T^-1 r and q = T^-1 u share the forward sweep; z = y - q (w.y)/(1 + w.q).
+// Scratch: cp and q (nc each); consecutive threads own consecutive x, so every
+// sweep step is a coalesced row of the plane.
+template <typename T>
+__global__ __launch_bounds__(256) void k_zline(Grid g, int lhs_type, SCoef s, LapRow lap, const T *__restrict__ n,
+                                               const T *__restrict__ r, T *__restrict__ z, T *__restrict__ cp,
+                                               T *__restrict__ q)
+{
+  using C = double;
+  const int64_t plane = (int64_t)g.nx * g.ny;
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= plane) return;
+  const int i = (int)(col % g.nx), j = (int)(col / g.nx), m = g.nz;
+  C v[19];
+  row_coeffs<C, T>(g, lhs_type, s, lap.v, n, i, j, 0, v);
+  const C a0 = v[5], b0 = v[18], c0 = v[4];
+  if (m == 1) { z[col] = (T)((C)r[col] / b0); return; }
+  row_coeffs<C, T>(g, lhs_type, s, lap.v, n, i, j, m - 1, v);
+  const C al = v[5], bl = v[18], cl = v[4];
+  if (m == 2) {  // both neighbours are the other node
+    const C a01 = a0 + c0, a10 = al + cl, det = b0 * bl - a01 * a10;
+    const C r0 = (C)r[col], r1 = (C)r[col + plane];
+    z[col] = (T)((r0 * bl - a01 * r1) / det);
+    z[col + plane] = (T)((b0 * r1 - a10 * r0) / det);
+    return;
+  }
+  const C gm = -b0, ag = a0 / gm;
+  C cpp = 0, yp = 0, qp = 0;
+  for (int k = 0; k < m; k++) {
+    C a, b, c;
+    if (k == 0) { a = a0; b = b0 - gm; c = c0; }
+    else if (k == m - 1) { a = al; b = bl - cl * a0 / gm; c = cl; }
+    else {
+      row_coeffs<C, T>(g, lhs_type, s, lap.v, n, i, j, k, v);
+      a = v[5]; b = v[18]; c = v[4];
+    }
+    const int64_t e = col + plane * k;
+    const C uk = k == 0 ? gm : (k == m - 1 ? cl : (C)0);
+    const C den = k == 0 ? b : b - a * cpp;
+    const C cpk = c / den;
+    const C yk = k == 0 ? (C)r[e] / den : ((C)r[e] - a * yp) / den;
+    const C qk = k == 0 ? uk / den : (uk - a * qp) / den;
+    cp[e] = (T)cpk; z[e] = (T)yk; q[e] = (T)qk;
+    cpp = cpk; yp = yk; qp = qk;
+  }
+  const C ylast = yp, qlast = qp;
+  C yn = yp, qn = qp;
+  for (int k = m - 2; k >= 0; k--) {
+    const int64_t e = col + plane * k;
+    const C ck = (C)cp[e];
+    yn = (C)z[e] - ck * yn;
+    qn = (C)q[e] - ck * qn;
+    z[e] = (T)yn; q[e] = (T)qn;
+  }
+  const C f = (yn + ag * ylast) / ((C)1 + qn + ag * qlast);
+  for (int k = 0; k < m; k++) {
+    const int64_t e = col + plane * k;
+    z[e] = (T)((C)z[e] - f * (C)q[e]);
+  }
+}
+
+template <typename T>
+void launch_zline(chy_ctx *ctx, int lhs_type, const T *n, const T *r, T *z, T *cp, T *q)
+{
+  const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
+  const LapRow L = lap_row(ctx, lhs_type);
+  const int64_t plane = (int64_t)ctx->g.nx * ctx->g.ny;
+  hipLaunchKernelGGL((k_zline<T>), nblocks(plane), kBlock, 0, ctx->stream, ctx->g, lhs_type, s, L, n, r, z, cp, q);
+}
+
+// Chebyshev iteration on D^-1 A z = D^-1 r from z = 0 (three-term recurrence);
+// the diagonal is rebuilt from n in each pass.
+template <typename T>
+__global__ __launch_bounds__(256) void k_cheby_init(Grid g, int lhs_type, SCoef s, LapRow lap, const T *__restrict__ n,
+                                                    const T *__restrict__ r, T *__restrict__ res, T *__restrict__ dv,
+                                                    T *__restrict__ z, double inv_theta)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  double v[19];
+  row_coeffs<double, T>(g, lhs_type, s, lap.v, n, i, j, k, v);
+  const double rr = (double)r[c] / v[18];
+  const double d = rr * inv_theta;
+  res[c] = (T)rr; dv[c] = (T)d; z[c] = (T)d;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_cheby_step(Grid g, int lhs_type, SCoef s, LapRow lap, const T *__restrict__ n,
+                                                    const T *__restrict__ Ad, T *__restrict__ res, T *__restrict__ dv,
+                                                    T *__restrict__ z, double c1, double c2)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  double v[19];
+  row_coeffs<double, T>(g, lhs_type, s, lap.v, n, i, j, k, v);
+  const double rr = (double)res[c] - (double)Ad[c] / v[18];
+  const double d = c1 * (double)dv[c] + c2 * rr;
+  res[c] = (T)rr; dv[c] = (T)d; z[c] = (T)((double)z[c] + d);
+}
+
+template <typename T>
+void launch_cheby(chy_ctx *ctx, int lhs_type, bool transpose, int degree, const T *n, const T *r, T *z, T *res,
+                  T *dv, T *Ad)
+{
+  const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
+  const LapRow L = lap_row(ctx, lhs_type);
+  const double lo = 0.2, hi = 2.2, theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+  const unsigned nb = nblocks(ctx->g.nc);
+  hipLaunchKernelGGL((k_cheby_init<T>), nb, kBlock, 0, ctx->stream, ctx->g, lhs_type, s, L, n, r, res, dv, z,
+                     1.0 / theta);
+  double rho = 1.0 / sigma;
+  for (int k = 1; k < degree; k++) {
+    launch_lhs_apply<T, T, T>(ctx, lhs_type, transpose, n, dv, Ad);
+    const double rho1 = 1.0 / (2.0 * sigma - rho);
+    hipLaunchKernelGGL((k_cheby_step<T>), nb, kBlock, 0, ctx->stream, ctx->g, lhs_type, s, L, n, Ad, res, dv, z,
+                       rho1 * rho, 2.0 * rho1 / delta);
+    rho = rho1;
+  }
+}
+
 // explicit instantiations
 template void launch_lhs_apply<double, double, double>(chy_ctx *, int, bool, const double *, const double *, double *);
 template void launch_lhs_apply<float, float, float>(chy_ctx *, int, bool, const float *, const float *, float *);
 template void launch_lhs_apply<float, double, double>(chy_ctx *, int, bool, const float *, const double *, double *);
 template void launch_residual<double, double>(chy_ctx *, int, const double *, const double *, const double *, double *);
 template void launch_residual<float, double>(chy_ctx *, int, const float *, const double *, const double *, double *);
+template void launch_zline<double>(chy_ctx *, int, const double *, const double *, double *, double *, double *);
+template void launch_zline<float>(chy_ctx *, int, const float *, const float *, float *, float *, float *);
+template void launch_cheby<double>(chy_ctx *, int, bool, int, const double *, const double *, double *, double *,
+                                   double *, double *);
+template void launch_cheby<float>(chy_ctx *, int, bool, int, const float *, const float *, float *, float *, float *,
+                                  float *);
 template void launch_efield<double>(chy_ctx *, const double *, double *);
 template void launch_efield<float>(chy_ctx *, const double *, float *);
 template void launch_rhs<double>(chy_ctx *, int, const double *, const double *, const double *, const double *, double *);

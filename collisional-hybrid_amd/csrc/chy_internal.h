@@ -93,6 +93,7 @@ struct chy_ctx {
   double *h_red = nullptr;      // pinned host mirror of red_out
   // FFT preconditioner
   void *fft_buf = nullptr;      // complex work array
+  void *pc_buf = nullptr;       // z-line / Chebyshev scratch, 3 vectors (T)
   double *tw[3] = {nullptr, nullptr, nullptr};   // twiddles per axis (complex)
   double *trig[3] = {nullptr, nullptr, nullptr}; // cos/sin of 2 pi k / N per axis
   bool fft_ok = false;
@@ -191,6 +192,13 @@ template <typename T>
 void launch_fill(chy_ctx *ctx, int64_t n, T *v, T val);
 template <typename T>
 void launch_jacobi(chy_ctx *ctx, int lhs_type, const T *n, const T *r, T *z);
+// z-line block Jacobi (scratch cp, q: nc each) and Chebyshev-Jacobi (scratch
+// res, dv, Ad: nc each) preconditioners
+template <typename T>
+void launch_zline(chy_ctx *ctx, int lhs_type, const T *n, const T *r, T *z, T *cp, T *q);
+template <typename T>
+void launch_cheby(chy_ctx *ctx, int lhs_type, bool transpose, int degree, const T *n, const T *r, T *z, T *res,
+                  T *dv, T *Ad);
 void launch_combine_sum(chy_ctx *ctx, const double *partials, int nblk, int nv, double *out);
 void launch_copy(chy_ctx *ctx, int64_t n16, const void *a, void *b);
 RhsCoef rhs_coef(const chy_ctx *ctx);

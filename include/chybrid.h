@@ -49,7 +49,7 @@ enum { CHY_BC_PERIODIC = 0, CHY_BC_INJECTION, CHY_BC_REFLECTION, CHY_BC_DIRICHLE
 /* build additions */
 enum { CHY_PREC_FP64 = 0, CHY_PREC_MIXED = 1 };
 enum { CHY_KSP_GMRES = 0, CHY_KSP_BCGS = 1 };
-enum { CHY_PC_NONE = 0, CHY_PC_JACOBI = 1, CHY_PC_FFT = 2 };
+enum { CHY_PC_NONE = 0, CHY_PC_JACOBI = 1, CHY_PC_FFT = 2, CHY_PC_ZLINE = 3, CHY_PC_CHEBY = 4 };
 enum { CHY_PERT_NONE = 0, CHY_PERT_SINUSOIDAL_X = 1, CHY_PERT_UNIFORM = 2, CHY_PERT_GAUSSIAN = 3 };
 
 /* ---- run parameters: the scalars of the reference Context (hybrid.h:106-138)
@@ -85,6 +85,7 @@ typedef struct {
   double  ksp_rtol, ksp_atol;
   int64_t ksp_max_it, ksp_restart;
   int32_t ksp_monitor, ksp_converged_reason, ksp_error_if_not_converged;
+  int32_t pc_cheby_degree; /* Chebyshev-Jacobi preconditioner: polynomial degree */
   /* build additions: seed, particles per cell, perturbation, precision, output cadence */
   uint64_t seed;
   int64_t ppc;
@@ -192,6 +193,14 @@ int chy_solve(chy_ctx *ctx, const double *b, double *phi, chy_solve_stats *stats
 int chy_get_potential(chy_ctx *ctx, double *phi);
 int chy_set_potential(chy_ctx *ctx, const double *phi);
 int chy_get_rhs(chy_ctx *ctx, double *b);
+/* PCApply of the selected preconditioner (the apply callback of a PETSc
+ * PCShell, for a host that keeps its own KSP): z = M^-1 r for the current
+ * moments, [Nz][Ny][Nx] FP64 in and out (the mean of z is arbitrary: the
+ * operator annihilates constants).  none: z = r; jacobi: D^-1 r; fft:
+ * A(mean n)^+ r; zline: cyclic tridiagonal solves of every z-line block of
+ * A(n); cheby: pc_cheby_degree steps of Chebyshev iteration on D^-1 A from
+ * zero, eigenvalues in [0.2, 2.2]. */
+int chy_pc_apply(chy_ctx *ctx, const double *r, double *z);
 /* One time step of the loop body hybrid.c:108-117 (velocities, positions,
  * moments, potential), fused on the device. */
 int chy_step(chy_ctx *ctx, chy_step_stats *stats);

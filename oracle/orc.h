@@ -151,11 +151,21 @@ void orc_init_particles(const int64_t N[3], const double p0[3], const double d[3
                         uint32_t *pid, double *w);
 
 /* ---- potential solve (reference potential.c:6-16 + PETSc KSP) ----------- */
-enum { ORC_PC_NONE = 0, ORC_PC_JACOBI = 1, ORC_PC_FFT = 2 };
+enum { ORC_PC_NONE = 0, ORC_PC_JACOBI = 1, ORC_PC_FFT = 2, ORC_PC_ZLINE = 3, ORC_PC_CHEBY = 4 };
 enum { ORC_KSP_GMRES = 0, ORC_KSP_BCGS = 1 };
-/* Apply the preconditioner: z = M^{-1} r (mean removed). */
+#define ORC_CHEBY_DEGREE 4
+/* Apply the preconditioner: z = M^{-1} r (mean removed).
+ *   jacobi  z = D^-1 r
+ *   fft     z = A(mean n)^+ r by 3-D FFT
+ *   zline   block Jacobi over z-lines: the cyclic tridiagonal system of each
+ *           (i, j) column (diagonal and +-z entries of A(n)) solved exactly
+ *   cheby   `degree` steps of Chebyshev iteration on D^-1 A z = D^-1 r from
+ *           z = 0, for eigenvalues in [0.2, 2.2] (0.1 and 1.1 times the
+ *           Gershgorin bound 2 of D^-1 A) */
 void orc_pc_apply(int pc, int lhs_type, const int64_t N[3], const double d[3], const double K[3],
                   const double *n, const double *r, double *z);
+void orc_pc_apply_deg(int pc, int lhs_type, const int64_t N[3], const double d[3], const double K[3],
+                      const double *n, const double *r, double *z, int degree);
 /* Solve A phi = b (b consistent, mean-free) starting from phi; right-
  * preconditioned restarted GMRES(m) with two-pass classical Gram-Schmidt,
  * or BiCGStab.  Converged when ||b - A phi|| <= max(rtol ||b||, atol).

@@ -17,7 +17,8 @@ I64 = np.int64
 
 LHS_IDENTITY, LHS_LAPLACIAN, LHS_FULL = 0, 1, 2
 RHS_CONSTANT, RHS_SINUSOIDAL, RHS_FULL = 0, 1, 2
-PC_NONE, PC_JACOBI, PC_FFT = 0, 1, 2
+PC_NONE, PC_JACOBI, PC_FFT, PC_ZLINE, PC_CHEBY = 0, 1, 2, 3, 4
+CHEBY_DEGREE = 4
 KSP_GMRES, KSP_BCGS = 0, 1
 
 Q, MP, ME, KB = 1.6022e-19, 1.6726e-27, 9.1094e-31, 1.3807e-23
@@ -65,6 +66,7 @@ def _declare(L):
     L.orc_deposit_scatter.argtypes = [P, P, P, C.c_int64] + [P] * 7 + [P] * 4
     L.orc_init_particles.argtypes = [P, P, P, C.c_int64, C.c_uint64, P, P, C.c_int, C.c_double] + [P] * 8
     L.orc_pc_apply.argtypes = [C.c_int, C.c_int, P, P, P, P, P, P]
+    L.orc_pc_apply_deg.argtypes = [C.c_int, C.c_int, P, P, P, P, P, P, C.c_int]
     L.orc_solve.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P, P, P, P,
                             C.c_double, C.c_double, C.c_int64, C.c_int64, P, P]
     L.orc_fft3d.argtypes = [P, P, C.c_int]
@@ -289,10 +291,10 @@ def init_particles(g, ppc, seed, v0, vT, pert=0, amp=0.0):
     return Particles(*arrs, pid=pid, w=w)
 
 
-def pc_apply(g, K, n, r, pc=PC_FFT, lhs_type=LHS_FULL):
+def pc_apply(g, K, n, r, pc=PC_FFT, lhs_type=LHS_FULL, degree=CHEBY_DEGREE):
     z = np.zeros(g.nc)
     K, n, r = _c(K), _c(n), _c(r)
-    lib().orc_pc_apply(pc, lhs_type, ptr(g.N), ptr(g.d), ptr(K), ptr(n), ptr(r), ptr(z))
+    lib().orc_pc_apply_deg(pc, lhs_type, ptr(g.N), ptr(g.d), ptr(K), ptr(n), ptr(r), ptr(z), int(degree))
     return z
 
 

@@ -109,10 +109,114 @@ static void uniform_row(int lhs_type, const int64_t N[3], const double d[3], con
   }
 }
 
+/* Row c of the operator (any type). */
+static void row_of(int lhs_type, const int64_t N[3], const double d[3], const double K[3], const double *n,
+                   int64_t c, double v[19])
+{
+  if (lhs_type == ORC_LHS_FULL) orc_lhs_full_row(N, d, K, n, c, v);
+  else uniform_row(lhs_type, N, d, K, 0.0, v);
+}
+
+/* Tridiagonal solve (Thomas algorithm): sub a, diagonal b, super c. */
+static void thomas(int64_t m, const double *a, const double *b, const double *c, const double *r, double *x,
+                   double *cp)
+{
+  double den = b[0];
+  cp[0] = c[0] / den;
+  x[0] = r[0] / den;
+  for (int64_t k = 1; k < m; k++) {
+    den = b[k] - a[k] * cp[k - 1];
+    cp[k] = c[k] / den;
+    x[k] = (r[k] - a[k] * x[k - 1]) / den;
+  }
+  for (int64_t k = m - 2; k >= 0; k--) x[k] -= cp[k] * x[k + 1];
+}
+
+/* Periodic (cyclic) tridiagonal solve by the Sherman-Morrison formula:
+ * A = T + u v^T with u = (g, 0.., 0, c[m-1]), v = (1, 0.., 0, a[0]/g),
+ * g = -b[0]; T differs from A in its two corner diagonal entries. */
+static void cyclic_solve(int64_t m, const double *a, const double *b, const double *c, const double *r, double *x)
+{
+  if (m == 1) { x[0] = r[0] / b[0]; return; }
+  if (m == 2) {  /* both neighbours are the other node */
+    const double a00 = b[0], a01 = a[0] + c[0], a10 = a[1] + c[1], a11 = b[1];
+    const double det = a00 * a11 - a01 * a10;
+    x[0] = (r[0] * a11 - a01 * r[1]) / det;
+    x[1] = (a00 * r[1] - a10 * r[0]) / det;
+    return;
+  }
+  double *bb = malloc(sizeof(double) * (size_t)m * 4), *u = bb + m, *q = bb + 2 * m, *cp = bb + 3 * m;
+  const double g = -b[0];
+  for (int64_t k = 0; k < m; k++) { bb[k] = b[k]; u[k] = 0.0; }
+  bb[0] = b[0] - g;
+  bb[m - 1] = b[m - 1] - c[m - 1] * a[0] / g;
+  u[0] = g;
+  u[m - 1] = c[m - 1];
+  thomas(m, a, bb, c, r, x, cp);
+  thomas(m, a, bb, c, u, q, cp);
+  const double f = (x[0] + a[0] / g * x[m - 1]) / (1.0 + q[0] + a[0] / g * q[m - 1]);
+  for (int64_t k = 0; k < m; k++) x[k] -= f * q[k];
+  free(bb);
+}
+
 void orc_pc_apply(int pc, int lhs_type, const int64_t N[3], const double d[3], const double K[3],
                   const double *n, const double *r, double *z)
 {
+  orc_pc_apply_deg(pc, lhs_type, N, d, K, n, r, z, ORC_CHEBY_DEGREE);
+}
+
+void orc_pc_apply_deg(int pc, int lhs_type, const int64_t N[3], const double d[3], const double K[3],
+                      const double *n, const double *r, double *z, int degree)
+{
   const int64_t nc = N[0] * N[1] * N[2];
+  if (pc == ORC_PC_ZLINE) {
+    const int64_t m = N[2], plane = N[0] * N[1];
+    double *buf = malloc(sizeof(double) * (size_t)m * 5);
+    double *a = buf, *b = buf + m, *c = buf + 2 * m, *rr = buf + 3 * m, *x = buf + 4 * m;
+    for (int64_t col = 0; col < plane; col++) {
+      for (int64_t k = 0; k < m; k++) {
+        double v[19];
+        row_of(lhs_type, N, d, K, n, col + plane * k, v);
+        a[k] = v[5];   /* (0, 0, -1) */
+        b[k] = v[18];
+        c[k] = v[4];   /* (0, 0, +1) */
+        rr[k] = r[col + plane * k];
+      }
+      cyclic_solve(m, a, b, c, rr, x);
+      for (int64_t k = 0; k < m; k++) z[col + plane * k] = x[k];
+    }
+    free(buf);
+    orc_remove_mean(nc, z);
+    return;
+  }
+  if (pc == ORC_PC_CHEBY) {
+    /* Chebyshev iteration (three-term recurrence) for D^-1 A on [lo, hi] */
+    const double lo = 0.2, hi = 2.2, theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    double *dg = malloc(sizeof(double) * (size_t)nc * 4), *res = dg + nc, *dv = dg + 2 * nc, *Ad = dg + 3 * nc;
+    for (int64_t c = 0; c < nc; c++) {
+      double v[19];
+      row_of(lhs_type, N, d, K, n, c, v);
+      dg[c] = v[18];
+      res[c] = r[c] / dg[c];
+      dv[c] = res[c] / theta;
+      z[c] = dv[c];
+    }
+    double rho = 1.0 / sigma;
+    for (int k = 1; k < degree; k++) {
+      orc_lhs_apply(lhs_type, N, d, K, n, dv, Ad);
+      const double rho1 = 1.0 / (2.0 * sigma - rho);
+      const double c1 = rho1 * rho, c2 = 2.0 * rho1 / delta;
+      for (int64_t c = 0; c < nc; c++) {
+        res[c] -= Ad[c] / dg[c];
+        dv[c] = c1 * dv[c] + c2 * res[c];
+        z[c] += dv[c];
+      }
+      rho = rho1;
+    }
+    free(dg);
+    orc_remove_mean(nc, z);
+    return;
+  }
   if (pc == ORC_PC_JACOBI) {
     for (int64_t c = 0; c < nc; c++) {
       double v[19];

@@ -165,3 +165,68 @@ def test_fft_preconditioner_exact_for_uniform_density(N, extra):
     assert st.iterations <= (2 if ctx.prm.precision == chybrid.PREC_FP64 else 6), st.iterations
     ref = orc.pc_apply(g, list(ctx.prm.kappa_e), n, b, lhs_type=ctx.prm.lhs_type)
     assert relmax(phi, ref) <= (1e-9 if ctx.prm.precision == chybrid.PREC_FP64 else 1e-8)
+
+
+PCS = {"jacobi": orc.PC_JACOBI, "fft": orc.PC_FFT, "zline": orc.PC_ZLINE, "cheby": orc.PC_CHEBY}
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("pc,N", [("jacobi", (16, 8, 12)), ("fft", (16, 8, 32)), ("zline", (16, 8, 12)),
+                                  ("zline", (8, 6, 3)), ("zline", (6, 4, 5)), ("cheby", (16, 8, 12))])
+def test_pc_apply_matches_oracle(pc, N, prec):
+    """chy_pc_apply (the PCShell apply) against the oracle's preconditioners,
+    modulo the constant that A(n) annihilates."""
+    rng = np.random.default_rng(len(pc) + N[2])
+    with context(opts(N, extra=f"-pc_type {pc} -precision {prec} -B0x 1e-5 -B0y -2e-5")) as ctx:
+        g = grid_of(ctx.prm)
+        mom = _moments(g, rng, amp=0.1)
+        ctx.set_moments(vlasov_from(mom))
+        r = rng.standard_normal(g.nc)
+        z = ctx.pc_apply(r)
+        K = list(ctx.prm.kappa_e)
+        deg = ctx.prm.pc_cheby_degree
+    n = mom[0].astype(np.float32).astype(np.float64) if prec == "mixed" else mom[0]
+    ref = orc.pc_apply(g, K, n, r, pc=PCS[pc], degree=deg)
+    z = z - z.mean()
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    assert relmax(z, ref) <= tol, relmax(z, ref)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("pc", ["zline", "cheby"])
+def test_zline_and_chebyshev_preconditioned_solves(pc, prec):
+    """GMRES with the z-line or Chebyshev-Jacobi preconditioner converges to
+    the tolerance on the anisotropic operator, in about the oracle's number of
+    iterations in FP64."""
+    rng = np.random.default_rng(23)
+    with context(opts((16, 16, 16), extra=f"-ksp_rtol 1e-9 -pc_type {pc} -ksp_max_it 20000 -precision {prec}")) as ctx:
+        g = grid_of(ctx.prm)
+        mom = _moments(g, rng, amp=0.01)
+        ctx.set_moments(vlasov_from(mom))
+        phi, st = ctx.solve_potential()
+        b = ctx.get_rhs()
+        K = list(ctx.prm.kappa_e)
+    assert st.converged == 1 and st.residual <= 1e-9, (st.iterations, st.outer, st.residual)
+    n = mom[0].astype(np.float32).astype(np.float64) if prec == "mixed" else mom[0]
+    r = b - orc.lhs_apply(g, K, n, phi)
+    assert np.linalg.norm(r) <= 2e-9 * np.linalg.norm(b)
+    _, its, conv, _ = orc.solve(g, K, n, b, pc=PCS[pc], rtol=1e-9, max_it=20000)
+    assert conv
+    if prec == "fp64":
+        assert st.iterations <= 1.2 * its + 5, (st.iterations, its)
+
+
+def test_non_power_of_two_grid_falls_back_to_zline():
+    rng = np.random.default_rng(29)
+    with context(opts((12, 10, 9), extra="-ksp_rtol 1e-10")) as ctx:
+        g = grid_of(ctx.prm)
+        mom = _moments(g, rng, amp=0.05)
+        ctx.set_moments(vlasov_from(mom))
+        phi, st = ctx.solve_potential()
+        b = ctx.get_rhs()
+        K = list(ctx.prm.kappa_e)
+    assert st.converged == 1
+    r = b - orc.lhs_apply(g, K, mom[0], phi)
+    assert np.linalg.norm(r) <= 2e-10 * np.linalg.norm(b)
+    _, its, _, _ = orc.solve(g, K, mom[0], b, pc=orc.PC_ZLINE, rtol=1e-10, max_it=20000)
+    assert st.iterations <= 1.2 * its + 5

@@ -164,10 +164,9 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
       // two-pass classical Gram-Schmidt (the preconditioned operator is close
       // to the identity, so one pass cancels heavily); the norm of the result
       // is fused into the second update and everything is read back at once
-      launch_mdot<T>(ctx, (int)(j + 1), V, nc, Vn, nc, hdev);
-      launch_mgs_update<T>(ctx, (int)(j + 1), V, nc, hdev, Vn, nc);
-      launch_mdot<T>(ctx, (int)(j + 1), V, nc, Vn, nc, h2dev);
-      launch_mgs_norm<T>(ctx, (int)(j + 1), V, nc, h2dev, Vn, nc, ndev);
+      launch_gs_dots<T>(ctx, (int)(j + 1), V, nc, Vn, nc, hdev);
+      launch_gs_update_dots<T>(ctx, (int)(j + 1), V, nc, hdev, Vn, nc, h2dev);
+      launch_gs_update_norm<T>(ctx, (int)(j + 1), V, nc, h2dev, Vn, nc, ndev);
       TRY(fetch(ctx, ctx->red_out, (int)(2 * (m + 2) + 1), ctx->h_red));
       for (int64_t q = 0; q <= j; q++) h[q] = ctx->h_red[q] + ctx->h_red[(m + 2) + q];
       const double wn = sqrt(ctx->h_red[2 * (m + 2)]);

@@ -174,26 +174,11 @@ __global__ __launch_bounds__(256) void k_residual(Grid g, int lhs_type, SCoef s,
 constexpr int kTX = 64, kTY = 16, kRows = kTY / 4, kZChunk = 16;  // 64 x 4 threads, 4 rows each
 constexpr int kHX = kTX + 2, kHY = kTY + 2;
 
-template <typename TN, typename TX, typename C>
-__device__ __forceinline__ void load_plane(const Grid &g, int x0, int y0, int k, const TN *__restrict__ n,
-                                           const TX *__restrict__ x, C (*xs)[kHX], C (*ns)[kHX], bool need_n)
-{
-  const int kk = wrapi(k, g.nz);  // k in [-1, nz]
-  for (int e = threadIdx.x; e < kHX * kHY; e += blockDim.x) {
-    const int hy = e / kHX, hx = e - hy * kHX;
-    // tiles may overhang small grids by more than one period: full modulo
-    int i = x0 + hx - 1, j = y0 + hy - 1;
-    if (i < 0) i += g.nx;
-    if (i >= g.nx) i -= g.nx;
-    if (i >= g.nx) i %= g.nx;
-    if (j < 0) j += g.ny;
-    if (j >= g.ny) j -= g.ny;
-    if (j >= g.ny) j %= g.ny;
-    const int64_t a = (int64_t)i + (int64_t)g.nx * ((int64_t)j + (int64_t)g.ny * kk);
-    xs[hy][hx] = (C)x[a];
-    if (need_n) ns[hy][hx] = (C)n[a];
-  }
-}
+// Each thread owns up to kPer elements of the (kHY x kHX) halo plane; their
+// in-plane offsets are computed once (with the periodic wrap), and the next
+// plane but one is prefetched into registers while the current one is
+// computed from LDS, so global latency overlaps the arithmetic.
+constexpr int kPer = (kHX * kHY + 255) / 256;
 
 template <typename TN, typename TX, typename TY, typename C, bool DIFF, bool RESID>
 __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s, LapRow lap,
@@ -207,11 +192,51 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
   const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, z0 = blockIdx.z * kZChunk;
   const int z1 = z0 + kZChunk < g.nz ? z0 + kZChunk : g.nz;
   const int i = x0 + tx;
-  load_plane<TN, TX, C>(g, x0, y0, z0 - 1, n, x, xs[(z0 + 2) % 3], ns[(z0 + 2) % 3], full);
-  load_plane<TN, TX, C>(g, x0, y0, z0, n, x, xs[z0 % 3], ns[z0 % 3], full);
+  const int64_t pstride = (int64_t)g.nx * g.ny;
+  // in-plane offsets and LDS slots of this thread's halo elements
+  int64_t off[kPer];
+  int slot[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int e = threadIdx.x + 256 * q;
+    slot[q] = e < kHX * kHY ? e : -1;
+    const int hy = e / kHX, hx = e - hy * kHX;
+    // tiles may overhang small grids by more than one period: full modulo
+    int ii = x0 + hx - 1, jj = y0 + hy - 1;
+    ii = ((ii % g.nx) + g.nx) % g.nx;
+    jj = ((jj % g.ny) + g.ny) % g.ny;
+    off[q] = (int64_t)ii + (int64_t)g.nx * jj;
+  }
+  C rx[kPer], rn[kPer];
+  auto fetch = [&](int k) {
+    const int64_t base = pstride * wrapi(k, g.nz);  // k in [-1, nz]
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      if (slot[q] >= 0) {
+        rx[q] = (C)x[base + off[q]];
+        if (full) rn[q] = (C)n[base + off[q]];
+      }
+    }
+  };
+  auto store = [&](int buf) {
+    C *xb = &xs[buf][0][0], *nb = &ns[buf][0][0];
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      if (slot[q] >= 0) {
+        xb[slot[q]] = rx[q];
+        if (full) nb[slot[q]] = rn[q];
+      }
+    }
+  };
+  fetch(z0 - 1);
+  store((z0 + 2) % 3);
+  fetch(z0);
+  store(z0 % 3);
+  fetch(z0 + 1);
   for (int k = z0; k < z1; k++) {
-    load_plane<TN, TX, C>(g, x0, y0, k + 1, n, x, xs[(k + 1) % 3], ns[(k + 1) % 3], full);
+    store((k + 1) % 3);
     __syncthreads();
+    if (k + 1 < z1) fetch(k + 2);  // in flight while plane k is computed
     const int pm = (k + 2) % 3, pc = k % 3, pp = (k + 1) % 3;
 #pragma unroll
     for (int rr = 0; rr < kRows; rr++) {
@@ -609,6 +634,137 @@ void launch_mgs_norm(chy_ctx *ctx, int nv, const T *V, int64_t stride, const dou
   hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, 1, norm2_dev);
 }
 
+// ---- Gram-Schmidt passes with 16-byte vector loads (n % 4 == 0) -------------------
+// Four consecutive elements per lane and grid-stride step; every block covers
+// a fixed set of quads, so the FP64 partial sums (and the results) do not
+// depend on timing.
+template <typename T> struct alignas(4 * sizeof(T)) Q4 { T v[4]; };
+
+template <typename T, int NV>
+__device__ __forceinline__ void block_partials(const double (&acc)[NV], int nv, double (*sh)[NV], double *partial)
+{
+  const int wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    if (k < nv) {
+      const double s = wave_sum(acc[k]);
+      if ((threadIdx.x & 63) == 0) sh[wid][k] = s;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    const int k = threadIdx.x;
+    partial[(int64_t)blockIdx.x * nv + k] = (sh[0][k] + sh[1][k]) + (sh[2][k] + sh[3][k]);
+  }
+}
+
+// partial[blk*nv + k] = A_k . b over the block's quads
+template <typename T, int NV>
+__global__ __launch_bounds__(256) void k_mdot4(int nv, const T *__restrict__ A, int64_t stride, const T *__restrict__ b,
+                                               int64_t n4, double *__restrict__ partial)
+{
+  __shared__ double sh[4][NV];
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) acc[k] = 0.0;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += (int64_t)gridDim.x * blockDim.x) {
+    const Q4<T> bq = reinterpret_cast<const Q4<T> *>(b)[q];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      if (k < nv) {
+        const Q4<T> a = reinterpret_cast<const Q4<T> *>(A + k * stride)[q];
+        acc[k] += ((double)a.v[0] * (double)bq.v[0] + (double)a.v[1] * (double)bq.v[1]) +
+                  ((double)a.v[2] * (double)bq.v[2] + (double)a.v[3] * (double)bq.v[3]);
+      }
+    }
+  }
+  block_partials<T, NV>(acc, nv, sh, partial);
+}
+
+// w -= sum_k h_k V_k, then partial[blk*nv + k] = V_k . w (of the new w), or
+// with NORM the partial of ||w||^2 instead
+template <typename T, int NV, bool NORM>
+__global__ __launch_bounds__(256) void k_gs4(int nv, const T *__restrict__ V, int64_t stride,
+                                             const double *__restrict__ h, T *__restrict__ w, int64_t n4,
+                                             double *__restrict__ partial)
+{
+  __shared__ double hs[NV];
+  __shared__ double sh[4][NORM ? 1 : NV];
+  if (threadIdx.x < nv) hs[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  double acc[NORM ? 1 : NV];
+#pragma unroll
+  for (int k = 0; k < (NORM ? 1 : NV); k++) acc[k] = 0.0;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += (int64_t)gridDim.x * blockDim.x) {
+    Q4<T> wq = reinterpret_cast<const Q4<T> *>(w)[q];
+    Q4<T> vq[NV];
+    double r[4] = {(double)wq.v[0], (double)wq.v[1], (double)wq.v[2], (double)wq.v[3]};
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      if (k < nv) {
+        vq[k] = reinterpret_cast<const Q4<T> *>(V + k * stride)[q];
+#pragma unroll
+        for (int e = 0; e < 4; e++) r[e] -= hs[k] * (double)vq[k].v[e];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) wq.v[e] = (T)r[e];
+    reinterpret_cast<Q4<T> *>(w)[q] = wq;
+    if (NORM) {
+      acc[0] += ((double)wq.v[0] * (double)wq.v[0] + (double)wq.v[1] * (double)wq.v[1]) +
+                ((double)wq.v[2] * (double)wq.v[2] + (double)wq.v[3] * (double)wq.v[3]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < (NORM ? 1 : NV); k++)
+        if (k < nv)
+          acc[k] += ((double)vq[k].v[0] * (double)wq.v[0] + (double)vq[k].v[1] * (double)wq.v[1]) +
+                    ((double)vq[k].v[2] * (double)wq.v[2] + (double)vq[k].v[3] * (double)wq.v[3]);
+    }
+  }
+  block_partials<T, NORM ? 1 : NV>(acc, NORM ? 1 : nv, sh, partial);
+}
+
+static inline bool quad_ok(int64_t n, int64_t stride, const void *a, const void *b)
+{
+  return n % 4 == 0 && stride % 4 == 0 && ((uintptr_t)a % 16) == 0 && ((uintptr_t)b % 16) == 0;
+}
+
+template <typename T>
+void launch_gs_dots(chy_ctx *ctx, int nv, const T *V, int64_t stride, const T *w, int64_t n, double *out_dev)
+{
+  if (nv > 8 || !quad_ok(n, stride, V, w)) { launch_mdot<T>(ctx, nv, V, stride, w, n, out_dev); return; }
+  const int64_t n4 = n / 4;
+  const unsigned nb = (unsigned)std::min<int64_t>(kRedBlocks, (n4 + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((k_mdot4<T, 8>), nb, kBlock, 0, ctx->stream, nv, V, stride, w, n4, ctx->red);
+  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, nv, out_dev);
+}
+
+template <typename T>
+void launch_gs_update_dots(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n,
+                           double *out_dev)
+{
+  if (nv > 8 || !quad_ok(n, stride, V, w)) {
+    launch_mgs_update<T>(ctx, nv, V, stride, h, w, n);
+    launch_mdot<T>(ctx, nv, V, stride, w, n, out_dev);
+    return;
+  }
+  const int64_t n4 = n / 4;
+  const unsigned nb = (unsigned)std::min<int64_t>(kRedBlocks, (n4 + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((k_gs4<T, 8, false>), nb, kBlock, 0, ctx->stream, nv, V, stride, h, w, n4, ctx->red);
+  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, nv, out_dev);
+}
+
+template <typename T>
+void launch_gs_update_norm(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n,
+                           double *norm2_dev)
+{
+  if (nv > 8 || !quad_ok(n, stride, V, w)) { launch_mgs_norm<T>(ctx, nv, V, stride, h, w, n, norm2_dev); return; }
+  const int64_t n4 = n / 4;
+  const unsigned nb = (unsigned)std::min<int64_t>(kRedBlocks, (n4 + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((k_gs4<T, 8, true>), nb, kBlock, 0, ctx->stream, nv, V, stride, h, w, n4, ctx->red);
+  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, 1, norm2_dev);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_axpby(int64_t n, double a, const T *__restrict__ x, double b, T *__restrict__ y)
 {
@@ -952,6 +1108,16 @@ template void launch_mgs_update<double>(chy_ctx *, int, const double *, int64_t,
 template void launch_mgs_update<float>(chy_ctx *, int, const float *, int64_t, const double *, float *, int64_t);
 template void launch_mgs_norm<double>(chy_ctx *, int, const double *, int64_t, const double *, double *, int64_t, double *);
 template void launch_mgs_norm<float>(chy_ctx *, int, const float *, int64_t, const double *, float *, int64_t, double *);
+template void launch_gs_dots<double>(chy_ctx *, int, const double *, int64_t, const double *, int64_t, double *);
+template void launch_gs_dots<float>(chy_ctx *, int, const float *, int64_t, const float *, int64_t, double *);
+template void launch_gs_update_dots<double>(chy_ctx *, int, const double *, int64_t, const double *, double *, int64_t,
+                                            double *);
+template void launch_gs_update_dots<float>(chy_ctx *, int, const float *, int64_t, const double *, float *, int64_t,
+                                           double *);
+template void launch_gs_update_norm<double>(chy_ctx *, int, const double *, int64_t, const double *, double *, int64_t,
+                                            double *);
+template void launch_gs_update_norm<float>(chy_ctx *, int, const float *, int64_t, const double *, float *, int64_t,
+                                           double *);
 template void launch_axpby<double>(chy_ctx *, int64_t, double, const double *, double, double *);
 template void launch_axpby<float>(chy_ctx *, int64_t, double, const float *, double, float *);
 template void launch_convert<double, float>(chy_ctx *, int64_t, const double *, double, float *);

@@ -182,6 +182,17 @@ void launch_mgs_update(chy_ctx *ctx, int nv, const T *V, int64_t stride, const d
 template <typename T>
 void launch_mgs_norm(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n,
                      double *norm2_dev);
+// Gram-Schmidt passes (vector loads; fall back to the kernels above for
+// nv > 8 or n not a multiple of 4): dots V^T w; w -= V h then V^T w;
+// w -= V h then ||w||^2
+template <typename T>
+void launch_gs_dots(chy_ctx *ctx, int nv, const T *V, int64_t stride, const T *w, int64_t n, double *out_dev);
+template <typename T>
+void launch_gs_update_dots(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n,
+                           double *out_dev);
+template <typename T>
+void launch_gs_update_norm(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n,
+                           double *norm2_dev);
 template <typename T>
 void launch_axpby(chy_ctx *ctx, int64_t n, double a, const T *x, double b, T *y);
 template <typename TS, typename TD>

@@ -68,7 +68,9 @@ def algorithmic_bytes(kernel, prec, np_, nc):
     if kernel == "lhs_apply":
         return nc * 3 * s
     if kernel == "pc":
-        return nc * (s + 2 * s + 4 * (2 * 2 * s) + 2 * s + s)   # 5 FFT passes, complex work array
+        # 5 FFT passes; the x pass is real-to-complex, so each pass reads and
+        # writes one real-field-sized array (half spectrum of nx/2 complex)
+        return nc * 5 * (s + s)
     if kernel == "efield":
         return nc * (8 + 3 * s)
     if kernel == "rhs":

@@ -4,12 +4,14 @@
 // periodic stencil with a_o = a_{-o}, hence diagonalised by the 3-D DFT with
 // real symbol lam(k) = sum_o a_o cos(k.o).  z = P^+ r is computed as
 // IFFT(FFT(r) / lam) with the k = 0 mode removed (the operator's constant
-// null space).  All passes are LDS-resident batched radix-2 transforms:
+// null space).  All passes are LDS-resident batched radix-4 transforms:
 // forward passes use decimation in frequency (natural in, bit-reversed out)
 // and inverse passes decimation in time (bit-reversed in, natural out), so
 // no permutation pass is needed; the z pass fuses forward FFT, the symbol
-// division and the inverse FFT.  Five HBM passes: x-fwd (real in), y-fwd,
-// z-fwd/scale/z-inv, y-inv, x-inv (real out).  Grid sizes: powers of two.
+// division and the inverse FFT.  The x pass is real-to-complex (half
+// spectrum, nx/2 complex values per line), so every pass moves as many bytes
+// as the real field: x-fwd, y-fwd, z-fwd/scale/z-inv, y-inv, x-inv.  Grid
+// sizes: powers of two.
 #include <cmath>
 
 #include "chy_internal.h"
@@ -47,7 +49,7 @@ static size_t fft_lds(int n, int B) { return ((size_t)B * (n + 1) + (size_t)n / 
 // radix-4 stages (two radix-2 stages fused), then a final radix-2 stage when
 // log2 n is odd.
 template <typename T>
-__device__ __forceinline__ void dif_fwd(cpx<T> *s, int B, int logn, const cpx<T> *tw)
+__device__ __forceinline__ void dif_fwd(cpx<T> *s, int B, int logn, const cpx<T> *tw, int tsh = 0)
 {
   const int n = 1 << logn;
   int lg = logn;
@@ -59,7 +61,7 @@ __device__ __forceinline__ void dif_fwd(cpx<T> *s, int B, int logn, const cpx<T>
       const int grp = r >> lq, k = r & (q - 1);
       const int i0 = b * (n + 1) + grp * len + k;
       const cpx<T> x0 = s[i0], x1 = s[i0 + q], x2 = s[i0 + 2 * q], x3 = s[i0 + 3 * q];
-      const cpx<T> w1 = tw[k * step], w2 = tw[2 * k * step];
+      const cpx<T> w1 = tw[(k * step) << tsh], w2 = tw[(2 * k * step) << tsh];
       const cpx<T> a0 = {x0.x + x2.x, x0.y + x2.y};
       const cpx<T> a1 = {x1.x + x3.x, x1.y + x3.y};
       const cpx<T> d02 = {x0.x - x2.x, x0.y - x2.y};
@@ -89,7 +91,7 @@ __device__ __forceinline__ void dif_fwd(cpx<T> *s, int B, int logn, const cpx<T>
 // Inverse, decimation in time (bit-reversed in, natural order out, unscaled):
 // a radix-2 first stage when log2 n is odd, then radix-4 stages.
 template <typename T>
-__device__ __forceinline__ void dit_inv(cpx<T> *s, int B, int logn, const cpx<T> *tw)
+__device__ __forceinline__ void dit_inv(cpx<T> *s, int B, int logn, const cpx<T> *tw, int tsh = 0)
 {
   const int n = 1 << logn;
   int lg = 0;
@@ -113,7 +115,7 @@ __device__ __forceinline__ void dit_inv(cpx<T> *s, int B, int logn, const cpx<T>
       const int grp = r >> lq, k = r & (q - 1);
       const int i0 = b * (n + 1) + grp * len + k;
       const cpx<T> x0 = s[i0], x1 = s[i0 + q], x2 = s[i0 + 2 * q], x3 = s[i0 + 3 * q];
-      const cpx<T> t1 = tw[k * step], t2 = tw[2 * k * step];
+      const cpx<T> t1 = tw[(k * step) << tsh], t2 = tw[(2 * k * step) << tsh];
       const cpx<T> W = {t1.x, -t1.y}, V = {t2.x, -t2.y};  // conjugates: +2 pi i
       const cpx<T> v1 = cmul<T>(x1, V), v3 = cmul<T>(x3, V);
       const cpx<T> a0 = {x0.x + v1.x, x0.y + v1.y}, a1 = {x0.x - v1.x, x0.y - v1.y};
@@ -136,34 +138,89 @@ __device__ __forceinline__ void load_twiddles(cpx<T> *tws, const double *tw, int
   for (int j = threadIdx.x; j < (n >> 1); j += blockDim.x) tws[j] = {(T)tw[2 * j], (T)tw[2 * j + 1]};
 }
 
-// Pass along x (contiguous lines).  MODE 0: real in -> forward -> complex out.
-// MODE 1: complex in -> inverse -> real out (real part times scale).
+// Pass along x (contiguous real lines of length n = 2h) as half-length
+// complex transforms.  MODE 0 (forward): z_t = r_2t + i r_2t+1, Z = FFT_h(z),
+// then X_k = E_k + W_n^k O_k with E_k = (Z_k + conj Z_h-k)/2,
+// O_k = (Z_k - conj Z_h-k)/(2i), written for k < h in natural order; X_0 and
+// X_h are real and share slot 0 as X_0 + i X_h.  MODE 1 (inverse): the exact
+// reverse (E_k = X_k + conj X_h-k, O_k = (X_k - conj X_h-k) W_n^-k,
+// Z = E + i O, z = IFFT_h(Z), r_2t + i r_2t+1 = z_t), which returns n times the
+// input like an unscaled length-n inverse; the real result is multiplied by
+// `scale`.  LDS: twiddles W_n^j (j < h; the half-length FFT uses every other
+// one) and B padded lines of h + 1 complex values.
+template <typename T>
+static int rfft_batch(int h, int64_t avail)
+{
+  const int64_t E = 65536 / (int64_t)(2 * sizeof(T)) - h;
+  int B = 1;
+  while ((int64_t)(2 * B) * (h + 1) <= E && 2 * B <= avail) B *= 2;
+  return B;
+}
+
+template <typename T>
+static size_t rfft_lds(int h, int B) { return ((size_t)B * (h + 1) + (size_t)h) * 2 * sizeof(T); }
+
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_fft_x(int logn, int nlines, int B, const T *__restrict__ rin,
-                                               cpx<T> *__restrict__ buf, T *__restrict__ rout, const double *__restrict__ tw,
-                                               double scale)
+__global__ __launch_bounds__(256) void k_rfft_x(int logn, int nlines, int B, const T *__restrict__ rin,
+                                                cpx<T> *__restrict__ buf, T *__restrict__ rout,
+                                                const double *__restrict__ tw, double scale)
 {
   extern __shared__ unsigned char smem_raw[];
-  const int n = 1 << logn;
+  const int n = 1 << logn, lh = logn - 1, h = n >> 1;
   cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
-  cpx<T> *s = tws + (n >> 1);
+  cpx<T> *s = tws + h;
   load_twiddles<T>(tws, tw, n);
   const int64_t l0 = (int64_t)blockIdx.x * B;
   const int nb = (int)((nlines - l0) < B ? (nlines - l0) : B);
-  for (int e = threadIdx.x; e < (nb << logn); e += blockDim.x) {
-    const int b = e >> logn, t = e & (n - 1);
-    const int64_t a = (l0 + b) * n + t;
-    if (MODE == 0) s[b * (n + 1) + t] = {rin[a], (T)0};
-    else s[b * (n + 1) + t] = buf[a];
-  }
-  __syncthreads();
-  if (MODE == 0) dif_fwd<T>(s, nb, logn, tws);
-  else dit_inv<T>(s, nb, logn, tws);
-  for (int e = threadIdx.x; e < (nb << logn); e += blockDim.x) {
-    const int b = e >> logn, t = e & (n - 1);
-    const int64_t a = (l0 + b) * n + t;
-    if (MODE == 0) buf[a] = s[b * (n + 1) + t];
-    else rout[a] = (T)((double)s[b * (n + 1) + t].x * scale);
+  const cpx<T> *rin2 = reinterpret_cast<const cpx<T> *>(rin);
+  if (MODE == 0) {
+    for (int e = threadIdx.x; e < (nb << lh); e += blockDim.x) {
+      const int b = e >> lh, t = e & (h - 1);
+      s[b * (h + 1) + t] = rin2[(l0 + b) * h + t];
+    }
+    __syncthreads();
+    dif_fwd<T>(s, nb, lh, tws, 1);
+    for (int e = threadIdx.x; e < (nb << lh); e += blockDim.x) {
+      const int b = e >> lh, k = e & (h - 1);
+      const cpx<T> Z = s[b * (h + 1) + (lh ? bitrev(k, lh) : 0)];
+      cpx<T> X;
+      if (k == 0) {
+        X = {Z.x + Z.y, Z.x - Z.y};
+      } else {
+        const cpx<T> M = s[b * (h + 1) + bitrev(h - k, lh)];
+        const cpx<T> E = {(T)0.5 * (Z.x + M.x), (T)0.5 * (Z.y - M.y)};
+        const cpx<T> O = {(T)0.5 * (Z.y + M.y), (T)0.5 * (M.x - Z.x)};
+        const cpx<T> WO = cmul<T>(O, tws[k]);
+        X = {E.x + WO.x, E.y + WO.y};
+      }
+      buf[(l0 + b) * h + k] = X;
+    }
+  } else {
+    __syncthreads();  // the unpacking below reads twiddles loaded by other threads
+    for (int e = threadIdx.x; e < (nb << lh); e += blockDim.x) {
+      const int b = e >> lh, k = e & (h - 1);
+      const cpx<T> X = buf[(l0 + b) * h + k];
+      cpx<T> Z;
+      if (k == 0) {
+        Z = {X.x + X.y, X.x - X.y};
+      } else {
+        const cpx<T> M = buf[(l0 + b) * h + (h - k)];
+        const cpx<T> E = {X.x + M.x, X.y - M.y};
+        const cpx<T> D = {X.x - M.x, X.y + M.y};
+        const cpx<T> w = tws[k];
+        const cpx<T> O = cmul<T>(D, {w.x, -w.y});
+        Z = {E.x - O.y, E.y + O.x};  // E + i O
+      }
+      s[b * (h + 1) + (lh ? bitrev(k, lh) : 0)] = Z;
+    }
+    __syncthreads();
+    dit_inv<T>(s, nb, lh, tws, 1);
+    cpx<T> *rout2 = reinterpret_cast<cpx<T> *>(rout);
+    for (int e = threadIdx.x; e < (nb << lh); e += blockDim.x) {
+      const int b = e >> lh, t = e & (h - 1);
+      const cpx<T> z = s[b * (h + 1) + t];
+      rout2[(l0 + b) * h + t] = {(T)((double)z.x * scale), (T)((double)z.y * scale)};
+    }
   }
 }
 
@@ -176,9 +233,31 @@ struct Symbol {
   double inv_scale;             // 1 / (nx ny nz)
 };
 
+__device__ __forceinline__ double symbol_at(const Symbol &sym, int kx, int ky, int kz, const double *__restrict__ trx,
+                                            const double *__restrict__ try_, const double *__restrict__ trz)
+{
+  const double cx = trx[2 * kx], sx = trx[2 * kx + 1];
+  const double cy = try_[2 * ky], sy = try_[2 * ky + 1];
+  const double cz = trz[2 * kz], sz = trz[2 * kz + 1];
+  return sym.A0 + sym.F[0] * cx + sym.F[1] * cy + sym.F[2] * cz + (sym.P[0] * cx * cy - sym.M[0] * sx * sy) +
+         (sym.P[1] * cx * cz - sym.M[1] * sx * sz) + (sym.P[2] * cy * cz - sym.M[2] * sy * sz);
+}
+
+// inv_scale / lam, 0 for the null mode
+__device__ __forceinline__ double symbol_factor(const Symbol &sym, int kx, int ky, int kz, const double *trx,
+                                                const double *try_, const double *trz)
+{
+  if (kx == 0 && ky == 0 && kz == 0) return 0.0;
+  const double lam = symbol_at(sym, kx, ky, kz, trx, try_, trz);
+  return lam == 0.0 ? 0.0 : sym.inv_scale / lam;
+}
+
 // Pass along a strided axis: line (a, c) at base a + c*cstride, element t at
 // + t*estride; a block holds B consecutive a.  MODE 0 forward, 1 inverse,
-// 2 forward + symbol division + inverse (z axis).
+// 2 forward + symbol division + inverse (z axis) for the half-spectrum lines
+// a = kx + (nx/2) * py with kx >= 1 in natural order (slot 0, which packs
+// kx = 0 and nx/2, is left to k_fft_col0) and py the bit-reversed y
+// frequency.
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_fft_s(int logn, int64_t A, int64_t estride, int64_t cstride, int B,
                                                cpx<T> *__restrict__ buf, const double *__restrict__ tw, Symbol sym,
@@ -194,9 +273,12 @@ __global__ __launch_bounds__(256) void k_fft_s(int logn, int64_t A, int64_t estr
   const int64_t c = blockIdx.x / blocks_per_c;
   const int64_t a0 = (blockIdx.x % blocks_per_c) * B;
   const int64_t base = a0 + c * cstride;
+  const int hx = sym.nx >> 1;
+  // in MODE 2 the lines of the packed slot (kx = 0) are not touched
+  auto live = [&](int b) { return MODE != 2 || ((a0 + b) & (hx - 1)) != 0; };
   for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
     const int t = e / B, b = e - t * B;
-    s[b * (n + 1) + t] = buf[base + b + (int64_t)t * estride];
+    if (live(b)) s[b * (n + 1) + t] = buf[base + b + (int64_t)t * estride];
   }
   __syncthreads();
   if (MODE == 1) {
@@ -209,22 +291,12 @@ __global__ __launch_bounds__(256) void k_fft_s(int logn, int64_t A, int64_t estr
       for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
         const int t = e / B, b = e - t * B;
         const int64_t a = a0 + b;
-        const int kx = bitrev((int)(a & (sym.nx - 1)), sym.lx);
-        const int ky = bitrev((int)(a >> sym.lx), sym.ly);
+        const int kx = (int)(a & (hx - 1));
+        const int ky = bitrev((int)(a >> (sym.lx - 1)), sym.ly);
         const int kz = bitrev(t, sym.lz);
-        const double cx = trx[2 * kx], sx = trx[2 * kx + 1];
-        const double cy = try_[2 * ky], sy = try_[2 * ky + 1];
-        const double cz = trz[2 * kz], sz = trz[2 * kz + 1];
-        const double lam = sym.A0 + sym.F[0] * cx + sym.F[1] * cy + sym.F[2] * cz +
-                           (sym.P[0] * cx * cy - sym.M[0] * sx * sy) + (sym.P[1] * cx * cz - sym.M[1] * sx * sz) +
-                           (sym.P[2] * cy * cz - sym.M[2] * sy * sz);
+        const double f = symbol_factor(sym, kx, ky, kz, trx, try_, trz);
         cpx<T> &z = s[b * (n + 1) + t];
-        if (lam == 0.0 || (kx == 0 && ky == 0 && kz == 0)) {
-          z = {(T)0, (T)0};
-        } else {
-          const double f = sym.inv_scale / lam;
-          z = {(T)((double)z.x * f), (T)((double)z.y * f)};
-        }
+        z = {(T)((double)z.x * f), (T)((double)z.y * f)};
       }
       __syncthreads();
       dit_inv<T>(s, B, logn, tws);
@@ -232,7 +304,72 @@ __global__ __launch_bounds__(256) void k_fft_s(int logn, int64_t A, int64_t estr
   }
   for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
     const int t = e / B, b = e - t * B;
-    buf[base + b + (int64_t)t * estride] = s[b * (n + 1) + t];
+    if (live(b)) buf[base + b + (int64_t)t * estride] = s[b * (n + 1) + t];
+  }
+}
+
+// The packed slot kx = 0 of the half spectrum holds X_0 + i X_h, both real in
+// x, so after the y and z transforms Z = P + i Q with P = F(X_0), Q = F(X_h)
+// Hermitian: P(k) = (Z(k) + conj Z(-k))/2, Q(k) = (Z(k) - conj Z(-k))/(2i).
+// Block p takes the z-lines of bit-reversed y positions p and p' (ky' = -ky;
+// p' == p for ky = 0, ny/2), transforms both, scales P by the symbol at
+// kx = 0 and Q by the symbol at kx = nx/2, recombines and transforms back.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fft_col0(int logn, int64_t estride, int hx, cpx<T> *__restrict__ buf,
+                                                  const double *__restrict__ tw, Symbol sym,
+                                                  const double *__restrict__ trx, const double *__restrict__ try_,
+                                                  const double *__restrict__ trz)
+{
+  extern __shared__ unsigned char smem_raw[];
+  const int n = 1 << logn;
+  cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
+  cpx<T> *s = tws + (n >> 1);     // 2 lines, padded
+  const int ny = 1 << sym.ly;
+  const int p = blockIdx.x;
+  const int ky = bitrev(p, sym.ly);
+  const int pp = bitrev((ny - ky) & (ny - 1), sym.ly);
+  if (pp < p) return;             // the pair is done by block pp
+  const int nl = pp == p ? 1 : 2;
+  load_twiddles<T>(tws, tw, n);
+  const int64_t base[2] = {(int64_t)hx * p, (int64_t)hx * pp};
+  for (int e = threadIdx.x; e < nl * n; e += blockDim.x) {
+    const int b = e >> logn, t = e & (n - 1);
+    s[b * (n + 1) + t] = buf[base[b] + (int64_t)t * estride];
+  }
+  __syncthreads();
+  dif_fwd<T>(s, nl, logn, tws);
+  // every new value needs its partner's old one: compute into registers, then write
+  constexpr int kPerThread = 2 * kFftMaxN / 256;
+  cpx<T> nv[kPerThread];
+#pragma unroll
+  for (int q = 0; q < kPerThread; q++) {
+    const int e = threadIdx.x + 256 * q;
+    if (e >= nl * n) break;
+    const int b = e >> logn, t = e & (n - 1);
+    const int kyb = b == 0 ? ky : (ny - ky) & (ny - 1);
+    const int kz = bitrev(t, sym.lz);
+    const int tm = bitrev((n - kz) & (n - 1), sym.lz);
+    const int bm = nl == 1 ? 0 : 1 - b;
+    const cpx<T> Z = s[b * (n + 1) + t], M = s[bm * (n + 1) + tm];
+    const double Px = 0.5 * ((double)Z.x + (double)M.x), Py = 0.5 * ((double)Z.y - (double)M.y);
+    const double Qx = 0.5 * ((double)Z.y + (double)M.y), Qy = 0.5 * ((double)M.x - (double)Z.x);
+    const double f0 = symbol_factor(sym, 0, kyb, kz, trx, try_, trz);
+    const double fh = symbol_factor(sym, hx, kyb, kz, trx, try_, trz);
+    // P f0 + i Q fh
+    nv[q] = {(T)(Px * f0 - Qy * fh), (T)(Py * f0 + Qx * fh)};
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kPerThread; q++) {
+    const int e = threadIdx.x + 256 * q;
+    if (e >= nl * n) break;
+    s[(e >> logn) * (n + 1) + (e & (n - 1))] = nv[q];
+  }
+  __syncthreads();
+  dit_inv<T>(s, nl, logn, tws);
+  for (int e = threadIdx.x; e < nl * n; e += blockDim.x) {
+    const int b = e >> logn, t = e & (n - 1);
+    buf[base[b] + (int64_t)t * estride] = s[b * (n + 1) + t];
   }
 }
 
@@ -278,34 +415,38 @@ int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
   sym.nx = nx;
   sym.lx = ilog2(nx); sym.ly = ilog2(ny); sym.lz = ilog2(nz);
   sym.inv_scale = 1.0 / ((double)nx * ny * nz);
-  {  // x forward (real in)
-    const int n = nx, B = fft_batch<T>(n, (int64_t)ny * nz), nlines = ny * nz;
-    hipLaunchKernelGGL((k_fft_x<T, 0>), (unsigned)((nlines + B - 1) / B), kBlock, fft_lds<T>(n, B), ctx->stream,
-                       ilog2(n), nlines, B, r, buf, (T *)nullptr, ctx->tw[0], 1.0);
+  const int hx = nx / 2;
+  {  // x forward: real lines -> half spectra (kx < nx/2, slot 0 packs kx = 0, nx/2)
+    const int h = hx, nlines = ny * nz, B = rfft_batch<T>(h, nlines);
+    hipLaunchKernelGGL((k_rfft_x<T, 0>), (unsigned)((nlines + B - 1) / B), kBlock, rfft_lds<T>(h, B), ctx->stream,
+                       ilog2(nx), nlines, B, r, buf, (T *)nullptr, ctx->tw[0], 1.0);
   }
-  {  // y forward: lines (ix, iz); A = nx contiguous, c = iz with stride nx*ny
-    const int n = ny, B = fft_batch<T>(n, nx);
-    hipLaunchKernelGGL((k_fft_s<T, 0>), (unsigned)((nx / B) * nz), kBlock, fft_lds<T>(n, B), ctx->stream, ilog2(n),
-                       (int64_t)nx, (int64_t)nx, (int64_t)nx * ny, B, buf, ctx->tw[1], sym, ctx->trig[0],
+  {  // y forward: lines (kx, iz); A = nx/2 contiguous, c = iz with stride (nx/2)*ny
+    const int n = ny, B = fft_batch<T>(n, hx);
+    hipLaunchKernelGGL((k_fft_s<T, 0>), (unsigned)((hx / B) * nz), kBlock, fft_lds<T>(n, B), ctx->stream, ilog2(n),
+                       (int64_t)hx, (int64_t)hx, (int64_t)hx * ny, B, buf, ctx->tw[1], sym, ctx->trig[0],
                        ctx->trig[1], ctx->trig[2]);
   }
-  {  // z forward + symbol + inverse: lines a = ix + nx*iy, estride nx*ny
+  {  // z forward + symbol + inverse: lines a = kx + (nx/2)*py, estride (nx/2)*ny
     const int n = nz;
-    const int64_t A = (int64_t)nx * ny;
+    const int64_t A = (int64_t)hx * ny;
     const int B = fft_batch<T>(n, A);
     hipLaunchKernelGGL((k_fft_s<T, 2>), (unsigned)(A / B), kBlock, fft_lds<T>(n, B), ctx->stream, ilog2(n), A, A,
                        (int64_t)0, B, buf, ctx->tw[2], sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
+    const size_t lds = ((size_t)(n >> 1) + 2 * (size_t)(n + 1)) * 2 * sizeof(T);
+    hipLaunchKernelGGL((k_fft_col0<T>), (unsigned)ny, kBlock, lds, ctx->stream, ilog2(n), A, hx, buf, ctx->tw[2],
+                       sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
   }
   {  // y inverse
-    const int n = ny, B = fft_batch<T>(n, nx);
-    hipLaunchKernelGGL((k_fft_s<T, 1>), (unsigned)((nx / B) * nz), kBlock, fft_lds<T>(n, B), ctx->stream, ilog2(n),
-                       (int64_t)nx, (int64_t)nx, (int64_t)nx * ny, B, buf, ctx->tw[1], sym, ctx->trig[0],
+    const int n = ny, B = fft_batch<T>(n, hx);
+    hipLaunchKernelGGL((k_fft_s<T, 1>), (unsigned)((hx / B) * nz), kBlock, fft_lds<T>(n, B), ctx->stream, ilog2(n),
+                       (int64_t)hx, (int64_t)hx, (int64_t)hx * ny, B, buf, ctx->tw[1], sym, ctx->trig[0],
                        ctx->trig[1], ctx->trig[2]);
   }
-  {  // x inverse (real out)
-    const int n = nx, B = fft_batch<T>(n, (int64_t)ny * nz), nlines = ny * nz;
-    hipLaunchKernelGGL((k_fft_x<T, 1>), (unsigned)((nlines + B - 1) / B), kBlock, fft_lds<T>(n, B), ctx->stream,
-                       ilog2(n), nlines, B, (const T *)nullptr, buf, z, ctx->tw[0], 1.0);
+  {  // x inverse: half spectra -> real lines
+    const int h = hx, nlines = ny * nz, B = rfft_batch<T>(h, nlines);
+    hipLaunchKernelGGL((k_rfft_x<T, 1>), (unsigned)((nlines + B - 1) / B), kBlock, rfft_lds<T>(h, B), ctx->stream,
+                       ilog2(nx), nlines, B, (const T *)nullptr, buf, z, ctx->tw[0], 1.0);
   }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("fft: ") + hipGetErrorString(e));

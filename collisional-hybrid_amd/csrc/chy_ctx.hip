@@ -421,6 +421,8 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   ctx->mixed = p.precision == CHY_PREC_MIXED;
   Grid &g = ctx->g;
   g.nx = (int)p.N[0]; g.ny = (int)p.N[1]; g.nz = (int)p.N[2];
+  g.mx = fastdiv_magic((uint32_t)g.nx);
+  g.my = fastdiv_magic((uint32_t)g.ny);
   g.nc = p.N[0] * p.N[1] * p.N[2];
   for (int a = 0; a < 3; a++) { g.p0[a] = p.p0[a]; g.p1[a] = p.p1[a]; g.d[a] = p.d[a]; g.L[a] = p.L[a]; }
   if (g.nc >= (int64_t)1 << 31) { delete ctx; return CHY_ERR_ARG_RANGE; }
@@ -441,7 +443,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   A(dalloc(ctx, &ctx->cursor, nc + 1));
   A(dalloc(ctx, &ctx->scan_tmp, nc / 4096 + 2));
   A(dalloc(ctx, &ctx->maxcount, 1));
-  A(dalloc(ctx, &ctx->qlen, 1));
+  A(dalloc(ctx, &ctx->qlen, kMaxPushWaves));
   {
     unsigned char *pp = nullptr;
     A(dalloc(ctx, &pp, (int64_t)(32 * ts) * nc));

@@ -14,7 +14,23 @@ struct Grid {
   int nx, ny, nz;
   int64_t nc;
   double p0[3], p1[3], d[3], L[3];
+  // ceil(2^64 / nx) and ceil(2^64 / ny): q = mulhi64(m, c) is c / n exactly for
+  // every 32-bit c and n >= 2 (Lemire, Kaser and Kurz, "Faster remainder by
+  // direct computation", 2019)
+  uint64_t mx, my;
 };
+
+inline uint64_t fastdiv_magic(uint32_t n) { return ~0ull / n + 1; }
+
+// cell -> (i, j, k) without runtime integer division
+__device__ __forceinline__ void split_cell(const Grid &g, int c, int &i, int &j, int &k)
+{
+  const uint32_t q = (uint32_t)__umul64hi(g.mx, (uint64_t)(uint32_t)c);
+  i = c - (int)q * g.nx;
+  const uint32_t r = (uint32_t)__umul64hi(g.my, (uint64_t)q);
+  j = (int)q - (int)r * g.ny;
+  k = (int)r;
+}
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 

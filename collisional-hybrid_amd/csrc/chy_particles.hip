@@ -40,57 +40,22 @@ __device__ __forceinline__ void count_flags(unsigned long long *cs, int slot, bo
   if (m && lane_id() == 0) atomicAdd(&cs[slot], (unsigned long long)__popcll(m));
 }
 
-// Collision queue: lanes with `flag` append their particle index to a block
-// buffer in LDS (LDS atomics only); the buffer is flushed to the global queue
-// with one global atomic per flush, so the single global counter sees a few
-// thousand atomics per launch instead of one per wave.
-constexpr uint32_t kQBuf = 4096;
-struct QueueLds { uint32_t buf[kQBuf]; uint32_t n, base; };
-
-__device__ __forceinline__ void queue_flush(QueueLds &q, uint32_t *queue, uint32_t *qlen)
+// Collision queue: every wave of the push owns a segment of the queue whose
+// length is the number of live ions it visits (so the segments tile [0, np)
+// in wave order), and appends its colliders there with a ballot -- no atomics.
+// An entry holds the ion's index (qidx) and its state after the Boris step
+// (in the other particle buffer, free until the sort), so the collider
+// kernel reads the queue coalesced.  Wave w of a grid with `stride` threads
+// visits 64 * full + clamp(rem - 64 w, 0, 64) live ions (full = np / stride,
+// rem = np % stride): its segment starts at 64 w full + min(rem, 64 w).
+__device__ __forceinline__ int64_t queue_segment(int64_t w, int64_t full, int64_t rem)
 {
-  // block-uniform: every thread calls this after a barrier
-  if (threadIdx.x == 0) q.base = q.n ? atomicAdd(qlen, q.n) : 0u;
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < q.n; i += blockDim.x) queue[q.base + i] = q.buf[i];
-  __syncthreads();
-  if (threadIdx.x == 0) q.n = 0;
-  __syncthreads();
+  return 64 * w * full + (rem < 64 * w ? rem : 64 * w);
 }
 
-__device__ __forceinline__ void queue_push(QueueLds &q, uint32_t *queue, uint32_t *qlen, bool flag, uint32_t p)
+__device__ __forceinline__ uint32_t lane_prefix(uint64_t m)
 {
-  const uint64_t m = __ballot(flag);
-  if (m) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(&q.n, (uint32_t)__popcll(m));
-    base = __shfl(base, 0, 64);
-    if (flag) q.buf[base + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull))] = p;
-  }
-  __syncthreads();
-  if (q.n > kQBuf - 256) queue_flush(q, queue, qlen);  // room for one more block of appends
-}
-
-// Mixed move: in-cell offset += v dt / d with an exact integer carry.
-__device__ __forceinline__ int move_mixed(const Grid &g, const PushConst &pc, int c, float &fx, float &fy, float &fz,
-                                          float vx, float vy, float vz)
-{
-  float o[3] = {fmaf(vx, (float)pc.cdt[0], fx), fmaf(vy, (float)pc.cdt[1], fy), fmaf(vz, (float)pc.cdt[2], fz)};
-  int ci[3] = {c % g.nx, (c / g.nx) % g.ny, c / (g.nx * g.ny)};
-  const int nn[3] = {g.nx, g.ny, g.nz};
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    const float fl = floorf(o[a]);
-    float f = o[a] - fl;
-    int sh = (int)fl;
-    if (f >= 1.0f) { f -= 1.0f; sh += 1; }
-    int cc = (ci[a] + sh) % nn[a];
-    if (cc < 0) cc += nn[a];
-    ci[a] = cc;
-    o[a] = f;
-  }
-  fx = o[0]; fy = o[1]; fz = o[2];
-  return (int)(ci[0] + (int64_t)g.nx * (ci[1] + (int64_t)g.ny * ci[2]));
+  return (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
 }
 
 // ---- main push: gather + Boris + collision decision + move ----------------------
@@ -98,118 +63,164 @@ __device__ __forceinline__ int move_mixed(const Grid &g, const PushConst &pc, in
 template <bool COLLIDE, bool MOVE>
 __global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t np, PState<double> st,
                                                    const Vec4<double> *__restrict__ E4, const double *__restrict__ Sdev,
-                                                   uint32_t *__restrict__ counts, uint32_t *__restrict__ queue,
-                                                   uint32_t *__restrict__ qlen, unsigned long long *__restrict__ cstats)
+                                                   uint32_t *__restrict__ counts, PState<double> q,
+                                                   uint32_t *__restrict__ qidx, uint32_t *__restrict__ qcount,
+                                                   unsigned long long *__restrict__ cstats)
 {
   const double S = COLLIDE ? Sdev[0] : 0.0;
-  __shared__ QueueLds qs;
-  if (COLLIDE) {
-    if (threadIdx.x == 0) qs.n = 0;
-    __syncthreads();
-  }
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   // every lane iterates the same number of times so wave-level collectives stay convergent
   const int64_t iters = (np + stride - 1) / stride;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t seg = queue_segment(wave, np / stride, np % stride);
+  uint32_t wn = 0;
   for (int64_t it = 0; it < iters; it++) {
     const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = p < np;
     int key = -1, capped = 0;
     bool coll = false, bad = false;
+    double x = 0, y = 0, z = 0, vx = 0, vy = 0, vz = 0;
+    uint32_t pid = 0;
     if (live) {
-      double x = st.x[p], y = st.y[p], z = st.z[p];
-      double vx = st.vx[p], vy = st.vy[p], vz = st.vz[p];
+      x = st.x[p]; y = st.y[p]; z = st.z[p];
+      vx = st.vx[p]; vy = st.vy[p]; vz = st.vz[p];
       int i, j, k;
       double fx, fy, fz, e0, e1, e2;
       cell_coord(x, g.p0[0], g.d[0], g.nx, i, fx);
       cell_coord(y, g.p0[1], g.d[1], g.ny, j, fy);
       cell_coord(z, g.p0[2], g.d[2], g.nz, k, fz);
-      gather<double>(g, E4, i, j, k, fx, fy, fz, e0, e1, e2);
+      gather_c<double>(g, E4, (int64_t)i + (int64_t)g.nx * ((int64_t)j + (int64_t)g.ny * k), i, j, k, fx, fy, fz, e0,
+                       e1, e2);
       boris<double>(pc, e0, e1, e2, vx, vy, vz);
-      if (COLLIDE) coll = collide_decide<double>(pc, S, st.pid[p], vx, vy, vz, capped);
-      st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
-      if (MOVE && !coll) {
-        key = move_fp64(g, pc.dt, x, y, z, vx, vy, vz);
-        st.x[p] = x; st.y[p] = y; st.z[p] = z;
-        st.cell[p] = key;
+      if (COLLIDE) {
+        pid = st.pid[p];
+        coll = collide_decide<double>(pc, S, pid, vx, vy, vz, capped);
+      }
+      if (!coll) {
+        st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
+        if (MOVE) {
+          key = move_fp64(g, pc.dt, x, y, z, vx, vy, vz);
+          st.x[p] = x; st.y[p] = y; st.z[p] = z;
+          st.cell[p] = key;
+        }
       }
       bad = !isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(x) || !isfinite(y) || !isfinite(z);
     }
-    if (COLLIDE) queue_push(qs, queue, qlen, coll, (uint32_t)p);
+    if (COLLIDE) {
+      const uint64_t m = __ballot(coll);
+      if (coll) {
+        const int64_t e = seg + wn + lane_prefix(m);
+        qidx[e] = (uint32_t)p;
+        q.x[e] = x; q.y[e] = y; q.z[e] = z;
+        q.vx[e] = vx; q.vy[e] = vy; q.vz[e] = vz;
+        q.pid[e] = pid;
+      }
+      wn += (uint32_t)__popcll(m);
+    }
     if (MOVE) count_key(counts, key);
     if (COLLIDE) count_flags(cstats, 2, capped != 0);
     count_flags(cstats, 3, bad);
   }
-  if (COLLIDE) queue_flush(qs, queue, qlen);
+  if (COLLIDE && lane_id() == 0) qcount[wave] = wn;
 }
 
 template <bool COLLIDE, bool MOVE>
 __global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConst pc, int64_t np, PState<float> st,
                                                     const Vec4<float> *__restrict__ E4, const double *__restrict__ Sdev,
-                                                    uint32_t *__restrict__ counts, uint32_t *__restrict__ queue,
-                                                    uint32_t *__restrict__ qlen, unsigned long long *__restrict__ cstats)
+                                                    uint32_t *__restrict__ counts, PState<float> q,
+                                                    uint32_t *__restrict__ qidx, uint32_t *__restrict__ qcount,
+                                                    unsigned long long *__restrict__ cstats)
 {
   const float S = COLLIDE ? (float)Sdev[0] : 0.f;
-  __shared__ QueueLds qs;
-  if (COLLIDE) {
-    if (threadIdx.x == 0) qs.n = 0;
-    __syncthreads();
-  }
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t iters = (np + stride - 1) / stride;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t seg = queue_segment(wave, np / stride, np % stride);
+  uint32_t wn = 0;
   for (int64_t it = 0; it < iters; it++) {
     const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = p < np;
-    int key = -1, capped = 0;
+    int key = -1, capped = 0, c = 0;
     bool coll = false, bad = false;
+    float fx = 0, fy = 0, fz = 0, vx = 0, vy = 0, vz = 0;
+    uint32_t pid = 0;
     if (live) {
-      const int c = st.cell[p];
-      float fx = st.x[p], fy = st.y[p], fz = st.z[p];
-      float vx = st.vx[p], vy = st.vy[p], vz = st.vz[p];
+      c = st.cell[p];
+      int i, j, k;
+      split_cell(g, c, i, j, k);
+      fx = st.x[p]; fy = st.y[p]; fz = st.z[p];
+      vx = st.vx[p]; vy = st.vy[p]; vz = st.vz[p];
       float e0, e1, e2;
-      gather<float>(g, E4, c % g.nx, (c / g.nx) % g.ny, c / (g.nx * g.ny), fx, fy, fz, e0, e1, e2);
+      gather_c<float>(g, E4, c, i, j, k, fx, fy, fz, e0, e1, e2);
       boris<float>(pc, e0, e1, e2, vx, vy, vz);
-      if (COLLIDE) coll = collide_decide<float>(pc, S, st.pid[p], vx, vy, vz, capped);
-      st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
-      if (MOVE && !coll) {
-        key = move_mixed(g, pc, c, fx, fy, fz, vx, vy, vz);
-        st.x[p] = fx; st.y[p] = fy; st.z[p] = fz;
-        st.cell[p] = key;
+      if (COLLIDE) {
+        pid = st.pid[p];
+        coll = collide_decide<float>(pc, S, pid, vx, vy, vz, capped);
       }
-      bad = !isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(fx) || !isfinite(fy) || !isfinite(fz);
+      if (!coll) {
+        st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
+        if (MOVE) {
+          float ox = fx, oy = fy, oz = fz;
+          key = move_mixed_ijk(g, pc, i, j, k, ox, oy, oz, vx, vy, vz);
+          st.x[p] = ox; st.y[p] = oy; st.z[p] = oz;
+          st.cell[p] = key;
+          bad = !isfinite(ox) || !isfinite(oy) || !isfinite(oz);
+        }
+      }
+      bad = bad || !isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(fx) || !isfinite(fy) || !isfinite(fz);
     }
-    if (COLLIDE) queue_push(qs, queue, qlen, coll, (uint32_t)p);
+    if (COLLIDE) {
+      const uint64_t m = __ballot(coll);
+      if (coll) {
+        const int64_t e = seg + wn + lane_prefix(m);
+        qidx[e] = (uint32_t)p;
+        q.x[e] = fx; q.y[e] = fy; q.z[e] = fz;
+        q.vx[e] = vx; q.vy[e] = vy; q.vz[e] = vz;
+        q.pid[e] = pid;
+        q.cell[e] = c;
+      }
+      wn += (uint32_t)__popcll(m);
+    }
     if (MOVE) count_key(counts, key);
     if (COLLIDE) count_flags(cstats, 2, capped != 0);
     count_flags(cstats, 3, bad);
   }
-  if (COLLIDE) queue_flush(qs, queue, qlen);
+  if (COLLIDE && lane_id() == 0) qcount[wave] = wn;
 }
 
-// Scattering, move and key of the queued ions.
+// Scattering, move and key of the queued ions: wave v of this grid takes the
+// queue segments of push waves v, v + W, ... (W waves in this grid).
 template <bool MOVE>
-__global__ __launch_bounds__(256) void k_collide_fp64(Grid g, PushConst pc, PState<double> st,
-                                                      const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qlen,
-                                                      uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats)
+__global__ __launch_bounds__(256) void k_collide_fp64(Grid g, PushConst pc, int64_t np, int64_t pstride,
+                                                      int64_t nwaves, PState<double> st, PState<double> q,
+                                                      const uint32_t *__restrict__ qidx,
+                                                      const uint32_t *__restrict__ qcount, uint32_t *__restrict__ counts,
+                                                      unsigned long long *__restrict__ cstats)
 {
   __shared__ unsigned int sacc[4];
   if (threadIdx.x < 4) sacc[threadIdx.x] = 0u;
   __syncthreads();
   unsigned int nacc = 0, nref = 0, nbad = 0;
-  const uint32_t n = *qlen;
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const uint32_t p = queue[q];
-    double vx = st.vx[p], vy = st.vy[p], vz = st.vz[p];
-    const int r = scatter<double>(pc, st.pid[p], vx, vy, vz);
-    st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
-    if (r == 1) nacc++; else nref++;
-    if (MOVE) {
-      double x = st.x[p], y = st.y[p], z = st.z[p];
-      const int key = move_fp64(g, pc.dt, x, y, z, vx, vy, vz);
-      st.x[p] = x; st.y[p] = y; st.z[p] = z;
-      st.cell[p] = key;
-      atomicAdd(&counts[key], 1u);
-      if (!isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(x) || !isfinite(y) || !isfinite(z))
-        nbad++;
+  const int64_t W = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nwaves; w += W) {
+    const int64_t seg = queue_segment(w, np / pstride, np % pstride);
+    const uint32_t n = qcount[w];
+    for (uint32_t e = (uint32_t)lane_id(); e < n; e += 64) {
+      const int64_t s = seg + e;
+      const uint32_t p = qidx[s];
+      double vx = q.vx[s], vy = q.vy[s], vz = q.vz[s];
+      const int r = scatter<double>(pc, q.pid[s], vx, vy, vz);
+      st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
+      if (r == 1) nacc++; else nref++;
+      if (MOVE) {
+        double x = q.x[s], y = q.y[s], z = q.z[s];
+        const int key = move_fp64(g, pc.dt, x, y, z, vx, vy, vz);
+        st.x[p] = x; st.y[p] = y; st.z[p] = z;
+        st.cell[p] = key;
+        atomicAdd(&counts[key], 1u);
+        if (!isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(x) || !isfinite(y) || !isfinite(z))
+          nbad++;
+      }
     }
   }
   if (nacc) atomicAdd(&sacc[0], nacc);
@@ -220,29 +231,38 @@ __global__ __launch_bounds__(256) void k_collide_fp64(Grid g, PushConst pc, PSta
 }
 
 template <bool MOVE>
-__global__ __launch_bounds__(256) void k_collide_mixed(Grid g, PushConst pc, PState<float> st,
-                                                       const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qlen,
-                                                       uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats)
+__global__ __launch_bounds__(256) void k_collide_mixed(Grid g, PushConst pc, int64_t np, int64_t pstride,
+                                                       int64_t nwaves, PState<float> st, PState<float> q,
+                                                       const uint32_t *__restrict__ qidx,
+                                                       const uint32_t *__restrict__ qcount, uint32_t *__restrict__ counts,
+                                                       unsigned long long *__restrict__ cstats)
 {
   __shared__ unsigned int sacc[4];
   if (threadIdx.x < 4) sacc[threadIdx.x] = 0u;
   __syncthreads();
   unsigned int nacc = 0, nref = 0, nbad = 0;
-  const uint32_t n = *qlen;
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const uint32_t p = queue[q];
-    float vx = st.vx[p], vy = st.vy[p], vz = st.vz[p];
-    const int r = scatter<float>(pc, st.pid[p], vx, vy, vz);
-    st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
-    if (r == 1) nacc++; else nref++;
-    if (MOVE) {
-      float fx = st.x[p], fy = st.y[p], fz = st.z[p];
-      const int key = move_mixed(g, pc, st.cell[p], fx, fy, fz, vx, vy, vz);
-      st.x[p] = fx; st.y[p] = fy; st.z[p] = fz;
-      st.cell[p] = key;
-      atomicAdd(&counts[key], 1u);
-      if (!isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(fx) || !isfinite(fy) || !isfinite(fz))
-        nbad++;
+  const int64_t W = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nwaves; w += W) {
+    const int64_t seg = queue_segment(w, np / pstride, np % pstride);
+    const uint32_t n = qcount[w];
+    for (uint32_t e = (uint32_t)lane_id(); e < n; e += 64) {
+      const int64_t s = seg + e;
+      const uint32_t p = qidx[s];
+      float vx = q.vx[s], vy = q.vy[s], vz = q.vz[s];
+      const int r = scatter<float>(pc, q.pid[s], vx, vy, vz);
+      st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
+      if (r == 1) nacc++; else nref++;
+      if (MOVE) {
+        float fx = q.x[s], fy = q.y[s], fz = q.z[s];
+        int i, j, k;
+        split_cell(g, q.cell[s], i, j, k);
+        const int key = move_mixed_ijk(g, pc, i, j, k, fx, fy, fz, vx, vy, vz);
+        st.x[p] = fx; st.y[p] = fy; st.z[p] = fz;
+        st.cell[p] = key;
+        atomicAdd(&counts[key], 1u);
+        if (!isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(fx) || !isfinite(fy) || !isfinite(fz))
+          nbad++;
+      }
     }
   }
   if (nacc) atomicAdd(&sacc[0], nacc);
@@ -285,7 +305,9 @@ __global__ __launch_bounds__(256) void k_move_mixed(Grid g, PushConst pc, int64_
     bool bad = false;
     if (p < np) {
       float fx = st.x[p], fy = st.y[p], fz = st.z[p];
-      key = move_mixed(g, pc, st.cell[p], fx, fy, fz, st.vx[p], st.vy[p], st.vz[p]);
+      int i, j, k;
+      split_cell(g, st.cell[p], i, j, k);
+      key = move_mixed_ijk(g, pc, i, j, k, fx, fy, fz, st.vx[p], st.vy[p], st.vz[p]);
       st.x[p] = fx; st.y[p] = fy; st.z[p] = fz;
       st.cell[p] = key;
       bad = !isfinite(fx) || !isfinite(fy) || !isfinite(fz);
@@ -953,41 +975,45 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
   pc.Nc = (double)Nc;
   if (collide && Nc == 0) collide = false;
   launch_fill<unsigned long long>(ctx, 4, ctx->cstats, 0ull);
-  launch_fill<uint32_t>(ctx, 1, ctx->qlen, 0u);
   if (move) launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
   const unsigned nb = grid_for(ctx->np);
-  // the queue of colliding ions lives in the sort's permutation buffer, which
-  // is free until the sort that follows
-  uint32_t *queue = ctx->perm;
+  const int64_t pstride = (int64_t)nb * kBlock, nwaves = pstride / 64;
+  // the collision queue: indices in the sort's permutation buffer and states in
+  // the other particle buffer, both free until the sort that follows
+  uint32_t *qidx = ctx->perm;
   timer_start(ctx, "push");
-#define LAUNCH(KER, T, C, M)                                                                                  \
+#define LAUNCH(KER, T, C, M, q)                                                                               \
   hipLaunchKernelGGL((KER<C, M>), nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, st,                       \
-                     (const Vec4<T> *)ctx->efield[0], ctx->S_dev, ctx->counts, queue, ctx->qlen, ctx->cstats)
+                     (const Vec4<T> *)ctx->efield[0], ctx->S_dev, ctx->counts, q, qidx, ctx->qlen, ctx->cstats)
   if (ctx->mixed) {
-    PState<float> st = ctx->pf[ctx->cur];
-    if (collide && move) LAUNCH(k_push_mixed, float, true, true);
-    else if (collide) LAUNCH(k_push_mixed, float, true, false);
-    else if (move) LAUNCH(k_push_mixed, float, false, true);
-    else LAUNCH(k_push_mixed, float, false, false);
+    PState<float> st = ctx->pf[ctx->cur], q = ctx->pf[1 - ctx->cur];
+    if (collide && move) LAUNCH(k_push_mixed, float, true, true, q);
+    else if (collide) LAUNCH(k_push_mixed, float, true, false, q);
+    else if (move) LAUNCH(k_push_mixed, float, false, true, q);
+    else LAUNCH(k_push_mixed, float, false, false, q);
   } else {
-    PState<double> st = ctx->pd[ctx->cur];
-    if (collide && move) LAUNCH(k_push_fp64, double, true, true);
-    else if (collide) LAUNCH(k_push_fp64, double, true, false);
-    else if (move) LAUNCH(k_push_fp64, double, false, true);
-    else LAUNCH(k_push_fp64, double, false, false);
+    PState<double> st = ctx->pd[ctx->cur], q = ctx->pd[1 - ctx->cur];
+    if (collide && move) LAUNCH(k_push_fp64, double, true, true, q);
+    else if (collide) LAUNCH(k_push_fp64, double, true, false, q);
+    else if (move) LAUNCH(k_push_fp64, double, false, true, q);
+    else LAUNCH(k_push_fp64, double, false, false, q);
   }
 #undef LAUNCH
   timer_stop(ctx, "push");
   if (collide) {
     timer_start(ctx, "collide");
-    const unsigned nq = 1024;
+    const unsigned nq = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (nwaves + 3) / 4));
+#define COLL(KER, M, st, q)                                                                                 \
+  hipLaunchKernelGGL((KER<M>), nq, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, pstride, nwaves, st, q, qidx, \
+                     ctx->qlen, ctx->counts, ctx->cstats)
     if (ctx->mixed) {
-      if (move) hipLaunchKernelGGL((k_collide_mixed<true>), nq, kBlock, 0, ctx->stream, ctx->g, pc, ctx->pf[ctx->cur], queue, ctx->qlen, ctx->counts, ctx->cstats);
-      else hipLaunchKernelGGL((k_collide_mixed<false>), nq, kBlock, 0, ctx->stream, ctx->g, pc, ctx->pf[ctx->cur], queue, ctx->qlen, ctx->counts, ctx->cstats);
+      if (move) COLL(k_collide_mixed, true, ctx->pf[ctx->cur], ctx->pf[1 - ctx->cur]);
+      else COLL(k_collide_mixed, false, ctx->pf[ctx->cur], ctx->pf[1 - ctx->cur]);
     } else {
-      if (move) hipLaunchKernelGGL((k_collide_fp64<true>), nq, kBlock, 0, ctx->stream, ctx->g, pc, ctx->pd[ctx->cur], queue, ctx->qlen, ctx->counts, ctx->cstats);
-      else hipLaunchKernelGGL((k_collide_fp64<false>), nq, kBlock, 0, ctx->stream, ctx->g, pc, ctx->pd[ctx->cur], queue, ctx->qlen, ctx->counts, ctx->cstats);
+      if (move) COLL(k_collide_fp64, true, ctx->pd[ctx->cur], ctx->pd[1 - ctx->cur]);
+      else COLL(k_collide_fp64, false, ctx->pd[ctx->cur], ctx->pd[1 - ctx->cur]);
     }
+#undef COLL
     timer_stop(ctx, "collide");
   }
   unsigned long long cs[4];

@@ -177,6 +177,53 @@ __device__ __forceinline__ void gather(const Grid &g, const Vec4<T> *__restrict_
   }
 }
 
+// The same gather for an ion of cell c = (i, j, k): corner indices from
+// c and the periodic +1 offsets along each axis (no per-corner index math).
+template <typename T>
+__device__ __forceinline__ void gather_c(const Grid &g, const Vec4<T> *__restrict__ E4, int64_t c, int i, int j, int k,
+                                         T fx, T fy, T fz, T &e0, T &e1, T &e2)
+{
+  T W[8];
+  corner_weights<T>(fx, fy, fz, W);
+  const int64_t ox = i + 1 == g.nx ? 1 - g.nx : 1;
+  const int64_t oy = (j + 1 == g.ny ? 1 - g.ny : 1) * (int64_t)g.nx;
+  const int64_t oz = (k + 1 == g.nz ? 1 - g.nz : 1) * (int64_t)g.nx * g.ny;
+  e0 = (T)0; e1 = (T)0; e2 = (T)0;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const Vec4<T> e = E4[c + ((q & 1) ? ox : 0) + ((q & 2) ? oy : 0) + ((q & 4) ? oz : 0)];
+    e0 += W[q] * e.x;
+    e1 += W[q] * e.y;
+    e2 += W[q] * e.z;
+  }
+}
+
+// Mixed move: in-cell offset += v dt / d with an exact integer carry, for an
+// ion of cell (i, j, k); returns the new cell.
+__device__ __forceinline__ int move_mixed_ijk(const Grid &g, const PushConst &pc, int i, int j, int k, float &fx,
+                                              float &fy, float &fz, float vx, float vy, float vz)
+{
+  float o[3] = {fmaf(vx, (float)pc.cdt[0], fx), fmaf(vy, (float)pc.cdt[1], fy), fmaf(vz, (float)pc.cdt[2], fz)};
+  int ci[3] = {i, j, k};
+  const int nn[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float fl = floorf(o[a]);
+    float f = o[a] - fl;
+    int sh = (int)fl;
+    if (f >= 1.0f) { f -= 1.0f; sh += 1; }
+    int cc = ci[a] + sh;
+    if (cc < 0 || cc >= nn[a]) {  // more than one period away: full modulo
+      cc %= nn[a];
+      if (cc < 0) cc += nn[a];
+    }
+    ci[a] = cc;
+    o[a] = f;
+  }
+  fx = o[0]; fy = o[1]; fz = o[2];
+  return (int)(ci[0] + (int64_t)g.nx * (ci[1] + (int64_t)g.ny * ci[2]));
+}
+
 // FP64 absolute move and cell key.
 __device__ __forceinline__ int move_fp64(const Grid &g, double dt, double &x, double &y, double &z, double vx,
                                          double vy, double vz)

@@ -23,7 +23,7 @@ LHS_IDENTITY, LHS_LAPLACIAN, LHS_FULL = 0, 1, 2
 RHS_CONSTANT, RHS_SINUSOIDAL, RHS_FULL = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 KSP_GMRES, KSP_BCGS = 0, 1
-PC_NONE, PC_JACOBI, PC_FFT = 0, 1, 2
+PC_NONE, PC_JACOBI, PC_FFT, PC_ZLINE, PC_CHEBY, PC_FFT2 = 0, 1, 2, 3, 4, 5
 PERT_NONE, PERT_SINUSOIDAL_X, PERT_UNIFORM, PERT_GAUSSIAN = 0, 1, 2, 3
 
 D3 = C.c_double * 3
@@ -49,6 +49,7 @@ class Params(C.Structure):
         ("ksp_max_it", C.c_int64), ("ksp_restart", C.c_int64),
         ("ksp_monitor", C.c_int32), ("ksp_converged_reason", C.c_int32),
         ("ksp_error_if_not_converged", C.c_int32), ("pc_cheby_degree", C.c_int32),
+        ("ksp_inner_rtol", C.c_double),
         ("seed", C.c_uint64), ("ppc", C.c_int64), ("perturbation", C.c_int32),
         ("perturbation_amplitude", C.c_double), ("weighted", C.c_int32),
         ("precision", C.c_int32), ("output_interval", C.c_int64),
@@ -109,6 +110,7 @@ SIGNATURES = {
     "chy_copy_kernel": (C.c_int, [P, C.c_int64, C.c_int]),
     "chy_adjoint_forward": (C.c_int, [P, C.c_int64, C.c_int64, P]),
     "chy_pc_apply": (C.c_int, [P, P, P]),
+    "chy_timing_select": (C.c_int, [P, C.c_char_p]),
     "chy_adjoint_backward": (C.c_int, [P, P, P]),
 }
 
@@ -327,6 +329,11 @@ class Context:
         n = C.c_int64()
         self._chk(lib().chy_timing_get(self._p, name.encode(), C.byref(t), C.byref(n)))
         return t.value, n.value
+
+    def timing_select(self, names=None):
+        """Record only the given timers (an iterable of names; None: all)."""
+        arg = None if names is None else ",".join(names).encode()
+        self._chk(lib().chy_timing_select(self._p, arg))
 
     def timing_reset(self):
         self._chk(lib().chy_timing_reset(self._p))

@@ -20,9 +20,17 @@ using namespace chy;
 
 namespace chy {
 
+static bool timer_on(const chy_ctx *ctx, const char *name)
+{
+  if (!ctx->timing) return false;
+  if (ctx->timing_only.empty()) return true;
+  for (const std::string &s : ctx->timing_only) if (s == name) return true;
+  return false;
+}
+
 void timer_start(chy_ctx *ctx, const char *name)
 {
-  if (!ctx->timing) return;
+  if (!timer_on(ctx, name)) return;
   Timer &t = ctx->timers[name];
   if (t.used == t.a.size()) {
     hipEvent_t a = nullptr, b = nullptr;
@@ -37,7 +45,7 @@ void timer_start(chy_ctx *ctx, const char *name)
 
 void timer_stop(chy_ctx *ctx, const char *name)
 {
-  if (!ctx->timing) return;
+  if (!timer_on(ctx, name)) return;
   Timer &t = ctx->timers[name];
   if (!t.open) return;
   (void)hipEventRecord(t.b[t.used], ctx->stream);
@@ -86,9 +94,11 @@ int pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
   int rc = CHY_OK;
   const int pc = ctx->prm.pc_type;
   T *sb = (T *)ctx->pc_buf;
-  if (pc == CHY_PC_FFT && ctx->fft_ok && ctx->prm.lhs_type != CHY_LHS_IDENTITY) {
-    rc = fft_pc_apply<T>(ctx, v19, r, z);
-  } else if (pc == CHY_PC_ZLINE || (pc == CHY_PC_FFT && !ctx->fft_ok)) {
+  const bool fftlike = pc == CHY_PC_FFT || pc == CHY_PC_FFT2;
+  if (fftlike && ctx->fft_ok && ctx->prm.lhs_type != CHY_LHS_IDENTITY) {
+    if (pc == CHY_PC_FFT2) rc = fft2_pc_apply<T>(ctx, v19, r, z, sb);
+    else rc = fft_pc_apply<T>(ctx, v19, r, z);
+  } else if (pc == CHY_PC_ZLINE || (fftlike && !ctx->fft_ok)) {
     // grids the FFT does not take fall back to the z-line blocks
     launch_zline<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), r, z, sb, sb + nc);
   } else if (pc == CHY_PC_CHEBY) {
@@ -288,6 +298,9 @@ int pc_symbol(chy_ctx *ctx, double v19[19])
   double s;
   TRY(fetch(ctx, ctx->red_out, 1, &s));
   uniform_row(ctx->prm.lhs_type, ctx->prm.d, ctx->prm.kappa_e, s / (double)nc, v19);
+  // coefficients of the z-averaged operator for the two-level preconditioner
+  if (ctx->prm.pc_type == CHY_PC_FFT2 && ctx->fft_ok && ctx->pc2_buf)
+    launch_zcoef<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), (T *)ctx->pc2_buf);
   return CHY_OK;
 }
 
@@ -328,9 +341,11 @@ int solve_impl(chy_ctx *ctx, chy_solve_stats *st)
     if (!std::isfinite(rn)) return set_err(ctx, CHY_ERR_FP, "non-finite residual in the potential solve");
     if (rn <= tol) { converged = 1; break; }
     if (its >= p.ksp_max_it || outer >= 64) break;
-    // FP32 inner solves stop at 2e-4 (above their rounding floor, which is
-    // about eps_32 times the operator's conditioning); FP64 solves go to tol
-    const double inner_rtol = ctx->mixed ? std::max(2e-4, 0.5 * tol / rn) : std::min(1.0, 0.5 * tol / rn);
+    // FP32 inner solves stop at a floor above their rounding limit (about
+    // eps_32 times the preconditioned operator's conditioning); FP64 solves go
+    // to tol
+    const double inner_rtol =
+        ctx->mixed ? std::max(p.ksp_inner_rtol, 0.5 * tol / rn) : std::min(1.0, 0.5 * tol / rn);
     launch_convert<double, T>(ctx, nc, ctx->r64, 1.0 / rn, rhsT);
     int64_t it_in = 0;
     double rel = 1.0;
@@ -473,6 +488,12 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
     return CHY_ERR_MEM;
   }
   A(fft_setup(ctx));
+  if (p.pc_type == CHY_PC_FFT2 && ctx->fft_ok) {
+    unsigned char *q = nullptr;
+    A(dalloc(ctx, &q, (int64_t)ts * fft2_buf_elems(ctx)));
+    ctx->pc2_buf = q;
+    A(dalloc(ctx, &ctx->pc2_h, 64));
+  }
   launch_fill<double>(ctx, nc, ctx->phi, 0.0);
   launch_fill<double>(ctx, 1, ctx->S_dev, 0.0);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) { chy_destroy(ctx); return CHY_ERR_LIB; }
@@ -491,7 +512,7 @@ int chy_destroy(chy_ctx *ctx)
                 ctx->partials,
                 ctx->sred, ctx->S_dev, ctx->cstats, ctx->mom[0], ctx->mom[1], ctx->mom[2], ctx->mom[3],
                 ctx->efield[0], ctx->phi, ctx->b, ctx->r64, ctx->t64, ctx->V,
-                ctx->Z, ctx->wv, ctx->fft_buf, ctx->pc_buf, ctx->red, ctx->red_out, ctx->tw[0], ctx->tw[1], ctx->tw[2],
+                ctx->Z, ctx->wv, ctx->fft_buf, ctx->pc_buf, ctx->pc2_buf, ctx->pc2_h, ctx->red, ctx->red_out, ctx->tw[0], ctx->tw[1], ctx->tw[2],
                 ctx->trig[0], ctx->trig[1], ctx->trig[2]};
   for (void *p : ps) if (p) (void)hipFree(p);
   if (ctx->h_red) (void)hipHostFree(ctx->h_red);
@@ -770,6 +791,24 @@ int chy_timing_enable(chy_ctx *ctx, int on)
 {
   if (!ctx) return CHY_ERR_ARG_WRONG;
   ctx->timing = on != 0;
+  return CHY_OK;
+}
+
+int chy_timing_select(chy_ctx *ctx, const char *names)
+{
+  if (!ctx) return CHY_ERR_ARG_WRONG;
+  ctx->timing_only.clear();
+  if (!names) return CHY_OK;
+  std::string cur;
+  for (const char *c = names;; c++) {
+    if (*c == ',' || *c == '\0') {
+      if (!cur.empty()) ctx->timing_only.push_back(cur);
+      cur.clear();
+      if (*c == '\0') break;
+    } else if (*c != ' ') {
+      cur += *c;
+    }
+  }
   return CHY_OK;
 }
 

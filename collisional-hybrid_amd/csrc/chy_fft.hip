@@ -26,6 +26,7 @@ __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) { return {a.x * b.x -
 __device__ __forceinline__ int bitrev(int v, int logn) { return (int)(__brev((unsigned)v) >> (32 - logn)); }
 
 constexpr int kFftMaxN = 2048;  // longest axis handled by one LDS-resident line
+constexpr int kU = 8;           // global loads batched per thread before their LDS stores
 
 // Lines per block: the largest power of two whose padded LDS footprint
 // B*(n+1) complex values stays within 64 KB, and at most `avail`.
@@ -174,9 +175,19 @@ __global__ __launch_bounds__(256) void k_rfft_x(int logn, int nlines, int B, con
   const int nb = (int)((nlines - l0) < B ? (nlines - l0) : B);
   const cpx<T> *rin2 = reinterpret_cast<const cpx<T> *>(rin);
   if (MODE == 0) {
-    for (int e = threadIdx.x; e < (nb << lh); e += blockDim.x) {
-      const int b = e >> lh, t = e & (h - 1);
-      s[b * (h + 1) + t] = rin2[(l0 + b) * h + t];
+    // kU independent loads in flight per thread before the LDS stores
+    for (int e0 = threadIdx.x; e0 < (nb << lh); e0 += kU * 256) {
+      cpx<T> v[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        const int e = e0 + 256 * u;
+        if (e < (nb << lh)) v[u] = rin2[(l0 + (e >> lh)) * h + (e & (h - 1))];
+      }
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        const int e = e0 + 256 * u;
+        if (e < (nb << lh)) s[(e >> lh) * (h + 1) + (e & (h - 1))] = v[u];
+      }
     }
     __syncthreads();
     dif_fwd<T>(s, nb, lh, tws, 1);
@@ -197,21 +208,35 @@ __global__ __launch_bounds__(256) void k_rfft_x(int logn, int nlines, int B, con
     }
   } else {
     __syncthreads();  // the unpacking below reads twiddles loaded by other threads
-    for (int e = threadIdx.x; e < (nb << lh); e += blockDim.x) {
-      const int b = e >> lh, k = e & (h - 1);
-      const cpx<T> X = buf[(l0 + b) * h + k];
-      cpx<T> Z;
-      if (k == 0) {
-        Z = {X.x + X.y, X.x - X.y};
-      } else {
-        const cpx<T> M = buf[(l0 + b) * h + (h - k)];
-        const cpx<T> E = {X.x + M.x, X.y - M.y};
-        const cpx<T> D = {X.x - M.x, X.y + M.y};
-        const cpx<T> w = tws[k];
-        const cpx<T> O = cmul<T>(D, {w.x, -w.y});
-        Z = {E.x - O.y, E.y + O.x};  // E + i O
+    for (int e0 = threadIdx.x; e0 < (nb << lh); e0 += kU * 256) {
+      cpx<T> X[kU], M[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        const int e = e0 + 256 * u;
+        if (e < (nb << lh)) {
+          const int b = e >> lh, k = e & (h - 1);
+          X[u] = buf[(l0 + b) * h + k];
+          M[u] = buf[(l0 + b) * h + ((h - k) & (h - 1))];
+        }
       }
-      s[b * (h + 1) + (lh ? bitrev(k, lh) : 0)] = Z;
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        const int e = e0 + 256 * u;
+        if (e < (nb << lh)) {
+          const int b = e >> lh, k = e & (h - 1);
+          cpx<T> Z;
+          if (k == 0) {
+            Z = {X[u].x + X[u].y, X[u].x - X[u].y};
+          } else {
+            const cpx<T> E = {X[u].x + M[u].x, X[u].y - M[u].y};
+            const cpx<T> D = {X[u].x - M[u].x, X[u].y + M[u].y};
+            const cpx<T> w = tws[k];
+            const cpx<T> O = cmul<T>(D, {w.x, -w.y});
+            Z = {E.x - O.y, E.y + O.x};  // E + i O
+          }
+          s[b * (h + 1) + (lh ? bitrev(k, lh) : 0)] = Z;
+        }
+      }
     }
     __syncthreads();
     dit_inv<T>(s, nb, lh, tws, 1);
@@ -274,22 +299,47 @@ __global__ __launch_bounds__(256) void k_fft_s(int logn, int64_t A, int64_t estr
   const int64_t a0 = (blockIdx.x % blocks_per_c) * B;
   const int64_t base = a0 + c * cstride;
   const int hx = sym.nx >> 1;
+  const int lb = __ffs(B) - 1;  // B is a power of two
   // in MODE 2 the lines of the packed slot (kx = 0) are not touched
   auto live = [&](int b) { return MODE != 2 || ((a0 + b) & (hx - 1)) != 0; };
-  for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-    const int t = e / B, b = e - t * B;
-    if (live(b)) s[b * (n + 1) + t] = buf[base + b + (int64_t)t * estride];
+  for (int e0 = threadIdx.x; e0 < B * n; e0 += kU * 256) {
+    cpx<T> v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int e = e0 + 256 * u;
+      const int t = e >> lb, b = e & (B - 1);
+      if (e < B * n && live(b)) v[u] = buf[base + b + (int64_t)t * estride];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int e = e0 + 256 * u;
+      const int t = e >> lb, b = e & (B - 1);
+      if (e < B * n && live(b)) s[b * (n + 1) + t] = v[u];
+    }
   }
   __syncthreads();
   if (MODE == 1) {
     dit_inv<T>(s, B, logn, tws);
   } else {
     dif_fwd<T>(s, B, logn, tws);
+    if (MODE == 3) {
+      // 2-D solve: line a is the bit-reversed x frequency, element t the
+      // bit-reversed y frequency, kz = 0
+      for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
+        const int t = e >> lb, b = e & (B - 1);
+        const int kx = bitrev((int)(a0 + b), sym.lx), ky = bitrev(t, sym.ly);
+        const double f = symbol_factor(sym, kx, ky, 0, trx, try_, trz);
+        cpx<T> &z = s[b * (n + 1) + t];
+        z = {(T)((double)z.x * f), (T)((double)z.y * f)};
+      }
+      __syncthreads();
+      dit_inv<T>(s, B, logn, tws);
+    }
     if (MODE == 2) {
       // line a = ix + nx*iy holds bit-reversed x and y frequencies, element t
       // the bit-reversed z frequency
       for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-        const int t = e / B, b = e - t * B;
+        const int t = e >> lb, b = e & (B - 1);
         const int64_t a = a0 + b;
         const int kx = (int)(a & (hx - 1));
         const int ky = bitrev((int)(a >> (sym.lx - 1)), sym.ly);
@@ -303,7 +353,7 @@ __global__ __launch_bounds__(256) void k_fft_s(int logn, int64_t A, int64_t estr
     }
   }
   for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-    const int t = e / B, b = e - t * B;
+    const int t = e >> lb, b = e & (B - 1);
     if (live(b)) buf[base + b + (int64_t)t * estride] = s[b * (n + 1) + t];
   }
 }
@@ -399,13 +449,9 @@ int fft_setup(chy_ctx *ctx)
   return CHY_OK;
 }
 
-template <typename T>
-int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
+static Symbol make_symbol(const chy_ctx *ctx, const double v19[19])
 {
   const int nx = ctx->g.nx, ny = ctx->g.ny, nz = ctx->g.nz;
-  cpx<T> *buf = (cpx<T> *)ctx->fft_buf;
-  // symbol coefficients from the uniform-density row (column order of
-  // lhs.c:11-104: faces 0..5, xy edges 6..9, xz 10..13, yz 14..17, centre 18)
   Symbol sym;
   sym.A0 = v19[18];
   sym.F[0] = 2.0 * v19[0]; sym.F[1] = 2.0 * v19[2]; sym.F[2] = 2.0 * v19[4];
@@ -415,6 +461,17 @@ int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
   sym.nx = nx;
   sym.lx = ilog2(nx); sym.ly = ilog2(ny); sym.lz = ilog2(nz);
   sym.inv_scale = 1.0 / ((double)nx * ny * nz);
+  return sym;
+}
+
+template <typename T>
+int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
+{
+  const int nx = ctx->g.nx, ny = ctx->g.ny, nz = ctx->g.nz;
+  cpx<T> *buf = (cpx<T> *)ctx->fft_buf;
+  // symbol coefficients from the uniform-density row (column order of
+  // lhs.c:11-104: faces 0..5, xy edges 6..9, xz 10..13, yz 14..17, centre 18)
+  const Symbol sym = make_symbol(ctx, v19);
   const int hx = nx / 2;
   {  // x forward: real lines -> half spectra (kx < nx/2, slot 0 packs kx = 0, nx/2)
     const int h = hx, nlines = ny * nz, B = rfft_batch<T>(h, nlines);
@@ -452,6 +509,268 @@ int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
   if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("fft: ") + hipGetErrorString(e));
   return CHY_OK;
 }
+
+// ---- two-level preconditioner: kz = 0 correction -------------------------------------
+// x-y fields of the 2-D problem in pc2_buf: C (9), V (kFft2Its + 1), Z
+// (kFft2Its), w, r2, e2, and a complex work plane (2).
+int64_t fft2_buf_elems(const chy_ctx *ctx)
+{
+  return (int64_t)(9 + (kFft2Its + 1) + kFft2Its + 3 + 2) * ctx->g.nx * ctx->g.ny;
+}
+
+// pc2_h layout: H[(m+1) m], g[m+1], cs[m], sn[m], y[m], jm, stopped
+constexpr int kHm = kFft2Its;
+constexpr int kHG = (kHm + 1) * kHm, kHCs = kHG + kHm + 1, kHSn = kHCs + kHm, kHY = kHSn + kHm, kHJm = kHY + kHm,
+              kHStop = kHJm + 1;
+
+// 2-D complex pass along x: MODE 0 real in -> forward; MODE 1 inverse -> real
+// part times scale
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_fft2_x(int logn, int nlines, int B, const T *__restrict__ rin,
+                                                cpx<T> *__restrict__ buf, T *__restrict__ rout,
+                                                const double *__restrict__ tw, double scale)
+{
+  extern __shared__ unsigned char smem_raw[];
+  const int n = 1 << logn;
+  cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
+  cpx<T> *s = tws + (n >> 1);
+  load_twiddles<T>(tws, tw, n);
+  const int64_t l0 = (int64_t)blockIdx.x * B;
+  const int nb = (int)((nlines - l0) < B ? (nlines - l0) : B);
+  for (int e = threadIdx.x; e < (nb << logn); e += blockDim.x) {
+    const int b = e >> logn, t = e & (n - 1);
+    const int64_t a = (l0 + b) * n + t;
+    if (MODE == 0) s[b * (n + 1) + t] = {rin[a], (T)0};
+    else s[b * (n + 1) + t] = buf[a];
+  }
+  __syncthreads();
+  if (MODE == 0) dif_fwd<T>(s, nb, logn, tws);
+  else dit_inv<T>(s, nb, logn, tws);
+  for (int e = threadIdx.x; e < (nb << logn); e += blockDim.x) {
+    const int b = e >> logn, t = e & (n - 1);
+    const int64_t a = (l0 + b) * n + t;
+    if (MODE == 0) buf[a] = s[b * (n + 1) + t];
+    else rout[a] = (T)((double)s[b * (n + 1) + t].x * scale);
+  }
+}
+
+// r2[col] = mean over k of (r - w)
+template <typename T>
+__global__ __launch_bounds__(256) void k_zavg_diff(int64_t nxy, int nz, const T *__restrict__ r, const T *__restrict__ w,
+                                                   T *__restrict__ r2)
+{
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= nxy) return;
+  double s = 0.0;
+  for (int k = 0; k < nz; k++) s += (double)r[col + nxy * k] - (double)w[col + nxy * k];
+  r2[col] = (T)(s / (double)nz);
+}
+
+// y = A2 e with the z-averaged 9-point coefficients
+template <typename T>
+__global__ __launch_bounds__(256) void k_a2(int nx, int ny, const T *__restrict__ C, const T *__restrict__ e,
+                                            T *__restrict__ y)
+{
+  const int64_t nxy = (int64_t)nx * ny;
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= nxy) return;
+  const int i = (int)(col % nx), j = (int)(col / nx);
+  const int ip = i + 1 == nx ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
+  const int jp = j + 1 == ny ? 0 : j + 1, jm = j == 0 ? ny - 1 : j - 1;
+  const int xs[3] = {im, i, ip}, ys[3] = {jm, j, jp};
+  double acc = 0.0;
+#pragma unroll
+  for (int o = 0; o < 9; o++) acc += (double)C[o * nxy + col] * (double)e[xs[o % 3] + (int64_t)nx * ys[o / 3]];
+  y[col] = (T)acc;
+}
+
+// z[k * nxy + col] += e2[col]
+template <typename T>
+__global__ __launch_bounds__(256) void k_add_ext(int64_t nxy, const T *__restrict__ e2, T *__restrict__ z)
+{
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= nxy) return;
+  const int64_t a = (int64_t)blockIdx.y * nxy + col;
+  z[a] = (T)((double)z[a] + (double)e2[col]);
+}
+
+// Block-wide sum of nv values per thread (1024 threads): out[q] for q < nv.
+template <int NV>
+__device__ __forceinline__ void block_sums(double (&v)[NV], int nv, double *out)
+{
+  __shared__ double sh[16][NV];
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    if (q < nv) {
+      double t = v[q];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+      if ((threadIdx.x & 63) == 0) sh[w][q] = t;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    double t = 0.0;
+    for (int k = 0; k < 16; k++) t += sh[k][threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
+// Start of the inner GMRES (one block): r2 -= mean(r2), beta = |r2|,
+// V0 = r2 / beta, g = (beta, 0, ...).
+template <typename T>
+__global__ __launch_bounds__(1024) void k2_start(int64_t nxy, const T *__restrict__ r2, T *__restrict__ V0,
+                                                 double *__restrict__ hs)
+{
+  __shared__ double red[2];
+  double a[1] = {0.0};
+  for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) a[0] += (double)r2[c];
+  block_sums<1>(a, 1, red);
+  const double mean = red[0] / (double)nxy;
+  a[0] = 0.0;
+  for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) {
+    const double x = (double)r2[c] - mean;
+    a[0] += x * x;
+  }
+  block_sums<1>(a, 1, red + 1);
+  const double beta = sqrt(red[1]);
+  const double inv = beta > 0.0 ? 1.0 / beta : 0.0;
+  for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) V0[c] = (T)(((double)r2[c] - mean) * inv);
+  if (threadIdx.x == 0) {
+    for (int q = 0; q <= kHm; q++) hs[kHG + q] = 0.0;
+    hs[kHG] = beta;
+    hs[kHJm] = 0.0;
+    hs[kHStop] = beta > 0.0 ? 0.0 : 1.0;
+  }
+}
+
+// Arnoldi step j (one block): w against V_0..V_j twice (classical
+// Gram-Schmidt), V_{j+1} = w / |w|, then the Givens update of column j.
+template <typename T>
+__global__ __launch_bounds__(1024) void k2_gs(int j, int64_t nxy, T *__restrict__ V, T *__restrict__ w,
+                                              double *__restrict__ hs)
+{
+  __shared__ double h1[kHm + 2], h2[kHm + 2];
+  T *Vn = V + (int64_t)(j + 1) * nxy;
+  if (hs[kHStop] != 0.0) {
+    for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) Vn[c] = (T)0;
+    return;
+  }
+  const int nv = j + 1;
+  for (int pass = 0; pass < 2; pass++) {
+    double a[kHm + 1];
+#pragma unroll
+    for (int q = 0; q <= kHm; q++) a[q] = 0.0;
+    for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) {
+      const double wc = (double)w[c];
+#pragma unroll
+      for (int q = 0; q <= kHm; q++)
+        if (q < nv) a[q] += (double)V[(int64_t)q * nxy + c] * wc;
+    }
+    double *h = pass == 0 ? h1 : h2;
+    block_sums<kHm + 1>(a, nv, h);
+    for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) {
+      double x = (double)w[c];
+      for (int q = 0; q < nv; q++) x -= h[q] * (double)V[(int64_t)q * nxy + c];
+      w[c] = (T)x;
+    }
+    __syncthreads();
+  }
+  double a[1] = {0.0};
+  for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) a[0] += (double)w[c] * (double)w[c];
+  __shared__ double nn[1];
+  block_sums<1>(a, 1, nn);
+  const double hn = sqrt(nn[0]);
+  const double inv = hn > 0.0 ? 1.0 / hn : 0.0;
+  for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) Vn[c] = (T)((double)w[c] * inv);
+  if (threadIdx.x == 0) {
+    double h[kHm + 2];
+    for (int q = 0; q < nv; q++) h[q] = h1[q] + h2[q];
+    h[nv] = hn;
+    double *cs = hs + kHCs, *sn = hs + kHSn, *g = hs + kHG;
+    for (int q = 0; q < j; q++) {
+      const double t = cs[q] * h[q] + sn[q] * h[q + 1];
+      h[q + 1] = -sn[q] * h[q] + cs[q] * h[q + 1];
+      h[q] = t;
+    }
+    const double rr = sqrt(h[j] * h[j] + h[j + 1] * h[j + 1]);
+    cs[j] = rr > 0.0 ? h[j] / rr : 1.0;
+    sn[j] = rr > 0.0 ? h[j + 1] / rr : 0.0;
+    h[j] = rr;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+    for (int q = 0; q <= j; q++) hs[q * kHm + j] = h[q];
+    hs[kHJm] = (double)(j + 1);
+    if (!(hn > 0.0)) hs[kHStop] = 1.0;
+  }
+}
+
+// Least-squares solution of the Givens-reduced system and e2 = sum_q y_q Z_q.
+template <typename T>
+__global__ __launch_bounds__(1024) void k2_finish(int64_t nxy, const T *__restrict__ Z, double *__restrict__ hs,
+                                                  T *__restrict__ e2)
+{
+  __shared__ double y[kHm];
+  __shared__ int jm;
+  if (threadIdx.x == 0) {
+    jm = (int)hs[kHJm];
+    for (int q = jm - 1; q >= 0; q--) {
+      double s = hs[kHG + q];
+      for (int r = q + 1; r < jm; r++) s -= hs[q * kHm + r] * y[r];
+      const double d = hs[q * kHm + q];
+      y[q] = d != 0.0 ? s / d : 0.0;
+    }
+  }
+  __syncthreads();
+  for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) {
+    double acc = 0.0;
+    for (int q = 0; q < jm; q++) acc += y[q] * (double)Z[(int64_t)q * nxy + c];
+    e2[c] = (T)acc;
+  }
+}
+
+template <typename T>
+int fft2_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z, T *w3)
+{
+  TRY(fft_pc_apply<T>(ctx, v19, r, z));
+  if (ctx->solve_transpose) return CHY_OK;  // the correction is built for A(n), not its transpose
+  const int nx = ctx->g.nx, ny = ctx->g.ny, nz = ctx->g.nz;
+  const int64_t nxy = (int64_t)nx * ny;
+  T *C = (T *)ctx->pc2_buf, *V = C + 9 * nxy, *Z = V + (kFft2Its + 1) * nxy, *w = Z + kFft2Its * nxy,
+    *r2 = w + nxy, *e2 = r2 + nxy;
+  cpx<T> *cb = (cpx<T> *)(e2 + nxy);
+  double *hs = ctx->pc2_h;
+  const unsigned nb2 = (unsigned)((nxy + kBlock - 1) / kBlock);
+  launch_lhs_apply<T, T, T>(ctx, ctx->prm.lhs_type, false, (const T *)ctx->mom[0], z, w3);
+  hipLaunchKernelGGL((k_zavg_diff<T>), nb2, kBlock, 0, ctx->stream, nxy, nz, r, (const T *)w3, r2);
+  hipLaunchKernelGGL((k2_start<T>), 1, 1024, 0, ctx->stream, nxy, (const T *)r2, V, hs);
+  Symbol sym = make_symbol(ctx, v19);
+  sym.inv_scale = 1.0 / (double)nxy;
+  const int Bx = fft_batch<T>(nx, ny), By = fft_batch<T>(ny, nx);
+  for (int j = 0; j < kFft2Its; j++) {
+    T *Vj = V + (int64_t)j * nxy, *Zj = Z + (int64_t)j * nxy;
+    // Zj = F2 Vj: 2-D FFT solve with the kz = 0 symbol
+    hipLaunchKernelGGL((k_fft2_x<T, 0>), (unsigned)((ny + Bx - 1) / Bx), kBlock, fft_lds<T>(nx, Bx), ctx->stream,
+                       ilog2(nx), ny, Bx, (const T *)Vj, cb, (T *)nullptr, ctx->tw[0], 1.0);
+    hipLaunchKernelGGL((k_fft_s<T, 3>), (unsigned)(nx / By), kBlock, fft_lds<T>(ny, By), ctx->stream, ilog2(ny),
+                       (int64_t)nx, (int64_t)nx, (int64_t)0, By, cb, ctx->tw[1], sym, ctx->trig[0], ctx->trig[1],
+                       ctx->trig[2]);
+    hipLaunchKernelGGL((k_fft2_x<T, 1>), (unsigned)((ny + Bx - 1) / Bx), kBlock, fft_lds<T>(nx, Bx), ctx->stream,
+                       ilog2(nx), ny, Bx, (const T *)nullptr, cb, Zj, ctx->tw[0], 1.0);
+    hipLaunchKernelGGL((k_a2<T>), nb2, kBlock, 0, ctx->stream, nx, ny, (const T *)C, (const T *)Zj, w);
+    hipLaunchKernelGGL((k2_gs<T>), 1, 1024, 0, ctx->stream, j, nxy, V, w, hs);
+  }
+  hipLaunchKernelGGL((k2_finish<T>), 1, 1024, 0, ctx->stream, nxy, (const T *)Z, hs, e2);
+  hipLaunchKernelGGL((k_add_ext<T>), dim3(nb2, (unsigned)nz), kBlock, 0, ctx->stream, nxy, (const T *)e2, z);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("fft2: ") + hipGetErrorString(e));
+  return CHY_OK;
+}
+
+template int fft2_pc_apply<double>(chy_ctx *, const double *, const double *, double *, double *);
+template int fft2_pc_apply<float>(chy_ctx *, const double *, const float *, float *, float *);
 
 template int fft_pc_apply<double>(chy_ctx *, const double *, const double *, double *);
 template int fft_pc_apply<float>(chy_ctx *, const double *, const float *, float *);

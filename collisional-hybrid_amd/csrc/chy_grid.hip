@@ -956,6 +956,40 @@ void launch_efield_adj(chy_ctx *ctx, const double *lex, const double *ley, const
   hipLaunchKernelGGL(k_efield_adj, nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g, lex, ley, lez, lphi);
 }
 
+// ---- z-averaged operator (two-level FFT preconditioner) -------------------------------
+// C[o * nxy + col] = mean over k of the entries of row (col, k) whose column
+// lies at x-y offset o = (dx + 1) + 3 (dy + 1); summed over k, then entries in
+// row order, as the oracle.
+__constant__ int kGroup19[19] = {5, 3, 7, 1, 4, 4, 8, 2, 6, 0, 5, 5, 3, 3, 7, 7, 1, 1, 4};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_zcoef(Grid g, int lhs_type, SCoef s, LapRow lap, const T *__restrict__ n,
+                                               T *__restrict__ C)
+{
+  const int64_t nxy = (int64_t)g.nx * g.ny;
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= nxy) return;
+  const int i = (int)(col % g.nx), j = (int)(col / g.nx);
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < g.nz; k++) {
+    double v[19];
+    row_coeffs<double, T>(g, lhs_type, s, lap.v, n, i, j, k, v);
+#pragma unroll
+    for (int q = 0; q < 19; q++) acc[kGroup19[q]] += v[q];
+  }
+#pragma unroll
+  for (int o = 0; o < 9; o++) C[o * nxy + col] = (T)(acc[o] / (double)g.nz);
+}
+
+template <typename T>
+void launch_zcoef(chy_ctx *ctx, int lhs_type, const T *n, T *C)
+{
+  const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
+  const LapRow L = lap_row(ctx, lhs_type);
+  const int64_t nxy = (int64_t)ctx->g.nx * ctx->g.ny;
+  hipLaunchKernelGGL((k_zcoef<T>), nblocks(nxy), kBlock, 0, ctx->stream, ctx->g, lhs_type, s, L, n, C);
+}
+
 // ---- z-line block Jacobi and Chebyshev-Jacobi preconditioners --------------------
 // z-line: one thread per (i, j) column solves the column's z-line block of
 // A(n) -- diagonal v18, -z entry v5, +z entry v4, periodic -- by the
@@ -1090,6 +1124,8 @@ template void launch_lhs_apply<float, float, float>(chy_ctx *, int, bool, const 
 template void launch_lhs_apply<float, double, double>(chy_ctx *, int, bool, const float *, const double *, double *);
 template void launch_residual<double, double>(chy_ctx *, int, const double *, const double *, const double *, double *);
 template void launch_residual<float, double>(chy_ctx *, int, const float *, const double *, const double *, double *);
+template void launch_zcoef<double>(chy_ctx *, int, const double *, double *);
+template void launch_zcoef<float>(chy_ctx *, int, const float *, float *);
 template void launch_zline<double>(chy_ctx *, int, const double *, const double *, double *, double *, double *);
 template void launch_zline<float>(chy_ctx *, int, const float *, const float *, float *, float *, float *);
 template void launch_cheby<double>(chy_ctx *, int, bool, int, const double *, const double *, double *, double *,

@@ -93,12 +93,15 @@ struct chy_ctx {
   double *h_red = nullptr;      // pinned host mirror of red_out
   // FFT preconditioner
   void *fft_buf = nullptr;      // complex work array
-  void *pc_buf = nullptr;       // z-line / Chebyshev scratch, 3 vectors (T)
+  void *pc_buf = nullptr;       // z-line / Chebyshev / two-level scratch, 3 vectors (T)
+  void *pc2_buf = nullptr;      // two-level FFT preconditioner: x-y fields (T)
+  double *pc2_h = nullptr;      // its inner GMRES: Hessenberg, Givens, state
   double *tw[3] = {nullptr, nullptr, nullptr};   // twiddles per axis (complex)
   double *trig[3] = {nullptr, nullptr, nullptr}; // cos/sin of 2 pi k / N per axis
   bool fft_ok = false;
 
   bool timing = false;
+  std::vector<std::string> timing_only;  // when not empty, the only timers recorded
   std::map<std::string, chy::Timer> timers;
 
   bool solve_transpose = false; // the Krylov solve runs on A(n)^T (adjoint)
@@ -204,6 +207,9 @@ template <typename T>
 void launch_fill(chy_ctx *ctx, int64_t n, T *v, T val);
 template <typename T>
 void launch_jacobi(chy_ctx *ctx, int lhs_type, const T *n, const T *r, T *z);
+// coefficients of the z-averaged operator, C[9][nx*ny] (two-level FFT preconditioner)
+template <typename T>
+void launch_zcoef(chy_ctx *ctx, int lhs_type, const T *n, T *C);
 // z-line block Jacobi (scratch cp, q: nc each) and Chebyshev-Jacobi (scratch
 // res, dv, Ad: nc each) preconditioners
 template <typename T>
@@ -246,5 +252,12 @@ void adjoint_release(chy_ctx *ctx);              // frees the adjoint tape
 int fft_setup(chy_ctx *ctx);
 template <typename T>
 int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z);
+// two-level: fft, then kFft2Its steps of flexible GMRES on the z-averaged
+// residual with the z-averaged operator (coefficients from launch_zcoef in
+// pc2_buf); w3 is 3-D scratch
+constexpr int kFft2Its = 4;
+int64_t fft2_buf_elems(const chy_ctx *ctx);  // elements of T in pc2_buf
+template <typename T>
+int fft2_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z, T *w3);
 
 }  // namespace chy

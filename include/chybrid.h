@@ -49,7 +49,7 @@ enum { CHY_BC_PERIODIC = 0, CHY_BC_INJECTION, CHY_BC_REFLECTION, CHY_BC_DIRICHLE
 /* build additions */
 enum { CHY_PREC_FP64 = 0, CHY_PREC_MIXED = 1 };
 enum { CHY_KSP_GMRES = 0, CHY_KSP_BCGS = 1 };
-enum { CHY_PC_NONE = 0, CHY_PC_JACOBI = 1, CHY_PC_FFT = 2, CHY_PC_ZLINE = 3, CHY_PC_CHEBY = 4 };
+enum { CHY_PC_NONE = 0, CHY_PC_JACOBI = 1, CHY_PC_FFT = 2, CHY_PC_ZLINE = 3, CHY_PC_CHEBY = 4, CHY_PC_FFT2 = 5 };
 enum { CHY_PERT_NONE = 0, CHY_PERT_SINUSOIDAL_X = 1, CHY_PERT_UNIFORM = 2, CHY_PERT_GAUSSIAN = 3 };
 
 /* ---- run parameters: the scalars of the reference Context (hybrid.h:106-138)
@@ -86,6 +86,7 @@ typedef struct {
   int64_t ksp_max_it, ksp_restart;
   int32_t ksp_monitor, ksp_converged_reason, ksp_error_if_not_converged;
   int32_t pc_cheby_degree; /* Chebyshev-Jacobi preconditioner: polynomial degree */
+  double  ksp_inner_rtol;  /* mixed precision: floor of the FP32 inner relative tolerance */
   /* build additions: seed, particles per cell, perturbation, precision, output cadence */
   uint64_t seed;
   int64_t ppc;
@@ -199,7 +200,9 @@ int chy_get_rhs(chy_ctx *ctx, double *b);
  * operator annihilates constants).  none: z = r; jacobi: D^-1 r; fft:
  * A(mean n)^+ r; zline: cyclic tridiagonal solves of every z-line block of
  * A(n); cheby: pc_cheby_degree steps of Chebyshev iteration on D^-1 A from
- * zero, eigenvalues in [0.2, 2.2]. */
+ * zero, eigenvalues in [0.2, 2.2]; fft2 (default): fft, then 4 steps of
+ * flexible GMRES on the z-averaged residual with the z-averaged operator (the
+ * x-y coupling, which the Hall term makes density dependent). */
 int chy_pc_apply(chy_ctx *ctx, const double *r, double *z);
 /* One time step of the loop body hybrid.c:108-117 (velocities, positions,
  * moments, potential), fused on the device. */
@@ -235,6 +238,9 @@ int chy_output_write(chy_ctx *ctx, const char *stem, const char *insert);
  * "deposit_cells", "deposit_nodes", "rhs", "pc", "lhs_apply", "krylov".
  * Events are resolved when read, so timing adds no synchronisation. */
 int chy_timing_enable(chy_ctx *ctx, int on);
+/* Restrict timing to the comma-separated timer names (NULL: all), so a timed
+ * run can keep event records on one kernel only. */
+int chy_timing_select(chy_ctx *ctx, const char *names);
 int chy_timing_get(chy_ctx *ctx, const char *name, double *total_ms, int64_t *count);
 int chy_timing_reset(chy_ctx *ctx);
 /* Plain device-to-device copy of nbytes through a simple vector kernel on the

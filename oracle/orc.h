@@ -151,9 +151,10 @@ void orc_init_particles(const int64_t N[3], const double p0[3], const double d[3
                         uint32_t *pid, double *w);
 
 /* ---- potential solve (reference potential.c:6-16 + PETSc KSP) ----------- */
-enum { ORC_PC_NONE = 0, ORC_PC_JACOBI = 1, ORC_PC_FFT = 2, ORC_PC_ZLINE = 3, ORC_PC_CHEBY = 4 };
+enum { ORC_PC_NONE = 0, ORC_PC_JACOBI = 1, ORC_PC_FFT = 2, ORC_PC_ZLINE = 3, ORC_PC_CHEBY = 4, ORC_PC_FFT2 = 5 };
 enum { ORC_KSP_GMRES = 0, ORC_KSP_BCGS = 1 };
 #define ORC_CHEBY_DEGREE 4
+#define ORC_FFT2_ITS 4
 /* Apply the preconditioner: z = M^{-1} r (mean removed).
  *   jacobi  z = D^-1 r
  *   fft     z = A(mean n)^+ r by 3-D FFT
@@ -161,7 +162,15 @@ enum { ORC_KSP_GMRES = 0, ORC_KSP_BCGS = 1 };
  *           (i, j) column (diagonal and +-z entries of A(n)) solved exactly
  *   cheby   `degree` steps of Chebyshev iteration on D^-1 A z = D^-1 r from
  *           z = 0, for eigenvalues in [0.2, 2.2] (0.1 and 1.1 times the
- *           Gershgorin bound 2 of D^-1 A) */
+ *           Gershgorin bound 2 of D^-1 A)
+ *   fft2    fft, then a correction of the z-averaged (kz = 0) residual:
+ *           r2 = mean_z(r - A z), less its mean (constants are not in the
+ *           range of the 2-D operator), is solved approximately on the x-y grid with
+ *           the z-averaged operator A2 (a 9-point stencil whose coefficients
+ *           are the z-means of the rows' entries grouped by x-y offset) by
+ *           ORC_FFT2_ITS steps of flexible GMRES (classical Gram-Schmidt twice,
+ *           Givens least squares) preconditioned by the kz = 0 symbol of the
+ *           fft preconditioner; the correction, constant along z, is added to z */
 void orc_pc_apply(int pc, int lhs_type, const int64_t N[3], const double d[3], const double K[3],
                   const double *n, const double *r, double *z);
 void orc_pc_apply_deg(int pc, int lhs_type, const int64_t N[3], const double d[3], const double K[3],

@@ -159,6 +159,115 @@ static void cyclic_solve(int64_t m, const double *a, const double *b, const doub
   free(bb);
 }
 
+static double dot(int64_t n, const double *a, const double *b);
+
+/* x-y offset group (dx + 1) + 3 (dy + 1) of each of the 19 row entries */
+static const int kGroup19[19] = {5, 3, 7, 1, 4, 4, 8, 2, 6, 0, 5, 5, 3, 3, 7, 7, 1, 1, 4};
+
+/* z-means of the row entries grouped by x-y offset: C[o * nxy + col] */
+static void zavg_coefficients(int lhs_type, const int64_t N[3], const double d[3], const double K[3],
+                              const double *n, double *C)
+{
+  const int64_t nxy = N[0] * N[1];
+  for (int64_t col = 0; col < nxy; col++) {
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t k = 0; k < N[2]; k++) {
+      double v[19];
+      row_of(lhs_type, N, d, K, n, col + nxy * k, v);
+      for (int q = 0; q < 19; q++) acc[kGroup19[q]] += v[q];
+    }
+    for (int o = 0; o < 9; o++) C[o * nxy + col] = acc[o] / (double)N[2];
+  }
+}
+
+static void a2_apply(const int64_t N[3], const double *C, const double *e, double *y)
+{
+  const int64_t nx = N[0], ny = N[1], nxy = nx * ny;
+  for (int64_t col = 0; col < nxy; col++) {
+    const int64_t i = col % nx, j = col / nx;
+    double acc = 0.0;
+    for (int o = 0; o < 9; o++) {
+      const int64_t ii = (i + (o % 3) - 1 + nx) % nx, jj = (j + (o / 3) - 1 + ny) % ny;
+      acc += C[o * nxy + col] * e[ii + nx * jj];
+    }
+    y[col] = acc;
+  }
+}
+
+/* 2-D FFT solve with the kz = 0 symbol of the mean-density operator */
+static void f2_apply(const int64_t N[3], const double v[19], const double *r, double *z)
+{
+  const int64_t nx = N[0], ny = N[1], nxy = nx * ny;
+  const int64_t N2[3] = {nx, ny, 1};
+  int off[19][3];
+  orc_stencil_offsets(off);
+  double *buf = malloc(sizeof(double) * 2 * nxy);
+  for (int64_t c = 0; c < nxy; c++) { buf[2 * c] = r[c]; buf[2 * c + 1] = 0.0; }
+  orc_fft3d(N2, buf, 0);
+  for (int64_t c = 0; c < nxy; c++) {
+    const int64_t kx = c % nx, ky = c / nx;
+    double lam = 0.0;
+    for (int q = 0; q < 19; q++)
+      lam += v[q] * cos(2.0 * M_PI * ((double)(kx * off[q][0]) / (double)nx + (double)(ky * off[q][1]) / (double)ny));
+    if (c == 0 || lam == 0.0) { buf[2 * c] = buf[2 * c + 1] = 0.0; }
+    else { buf[2 * c] /= lam; buf[2 * c + 1] /= lam; }
+  }
+  orc_fft3d(N2, buf, 1);
+  for (int64_t c = 0; c < nxy; c++) z[c] = buf[2 * c] / (double)nxy;
+  free(buf);
+}
+
+/* m steps of flexible GMRES from zero on A2 e = r2 (no restart) */
+static void gmres2_fixed(int m, const int64_t N[3], const double *C, const double v19[19], const double *r2,
+                         double *e)
+{
+  const int64_t nxy = N[0] * N[1];
+  double *V = calloc((size_t)(m + 1) * nxy, sizeof(double)), *Z = calloc((size_t)m * nxy, sizeof(double));
+  double *w = malloc(sizeof(double) * nxy);
+  double *H = calloc((size_t)(m + 1) * m, sizeof(double)), *h = malloc(sizeof(double) * (m + 1));
+  double *h2 = malloc(sizeof(double) * (m + 1)), *g = calloc((size_t)m + 1, sizeof(double));
+  double *cs = malloc(sizeof(double) * m), *sn = malloc(sizeof(double) * m), *y = malloc(sizeof(double) * m);
+  for (int64_t c = 0; c < nxy; c++) e[c] = 0.0;
+  double beta = sqrt(dot(nxy, r2, r2));
+  int jm = 0;
+  if (beta > 0.0) {
+    for (int64_t c = 0; c < nxy; c++) V[c] = r2[c] / beta;
+    g[0] = beta;
+    for (int j = 0; j < m; j++) {
+      f2_apply(N, v19, V + nxy * j, Z + nxy * j);
+      a2_apply(N, C, Z + nxy * j, w);
+      for (int q = 0; q <= j; q++) h[q] = dot(nxy, V + nxy * q, w);
+      for (int q = 0; q <= j; q++) for (int64_t c = 0; c < nxy; c++) w[c] -= h[q] * V[nxy * q + c];
+      for (int q = 0; q <= j; q++) h2[q] = dot(nxy, V + nxy * q, w);
+      for (int q = 0; q <= j; q++) for (int64_t c = 0; c < nxy; c++) w[c] -= h2[q] * V[nxy * q + c];
+      for (int q = 0; q <= j; q++) h[q] += h2[q];
+      h[j + 1] = sqrt(dot(nxy, w, w));
+      for (int64_t c = 0; c < nxy; c++) V[nxy * (j + 1) + c] = h[j + 1] > 0.0 ? w[c] / h[j + 1] : 0.0;
+      for (int q = 0; q < j; q++) {
+        const double t = cs[q] * h[q] + sn[q] * h[q + 1];
+        h[q + 1] = -sn[q] * h[q] + cs[q] * h[q + 1];
+        h[q] = t;
+      }
+      const double rr = sqrt(h[j] * h[j] + h[j + 1] * h[j + 1]);
+      cs[j] = rr > 0.0 ? h[j] / rr : 1.0;
+      sn[j] = rr > 0.0 ? h[j + 1] / rr : 0.0;
+      h[j] = rr;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      for (int q = 0; q <= j; q++) H[q * m + j] = h[q];
+      jm = j + 1;
+      if (!(h[j + 1] > 0.0)) break;
+    }
+    for (int q = jm - 1; q >= 0; q--) {
+      double s = g[q];
+      for (int r = q + 1; r < jm; r++) s -= H[q * m + r] * y[r];
+      y[q] = H[q * m + q] != 0.0 ? s / H[q * m + q] : 0.0;
+    }
+    for (int q = 0; q < jm; q++) for (int64_t c = 0; c < nxy; c++) e[c] += y[q] * Z[nxy * q + c];
+  }
+  free(V); free(Z); free(w); free(H); free(h); free(h2); free(g); free(cs); free(sn); free(y);
+}
+
 void orc_pc_apply(int pc, int lhs_type, const int64_t N[3], const double d[3], const double K[3],
                   const double *n, const double *r, double *z)
 {
@@ -169,6 +278,30 @@ void orc_pc_apply_deg(int pc, int lhs_type, const int64_t N[3], const double d[3
                       const double *n, const double *r, double *z, int degree)
 {
   const int64_t nc = N[0] * N[1] * N[2];
+  if (pc == ORC_PC_FFT2) {
+    const int64_t nxy = N[0] * N[1];
+    orc_pc_apply_deg(ORC_PC_FFT, lhs_type, N, d, K, n, r, z, degree);
+    double *w = malloc(sizeof(double) * nc), *C = malloc(sizeof(double) * 9 * nxy);
+    double *r2 = calloc((size_t)nxy, sizeof(double)), *e2 = malloc(sizeof(double) * nxy);
+    orc_lhs_apply(lhs_type, N, d, K, n, z, w);
+    for (int64_t col = 0; col < nxy; col++) {
+      double s = 0.0;
+      for (int64_t k = 0; k < N[2]; k++) s += r[col + nxy * k] - w[col + nxy * k];
+      r2[col] = s / (double)N[2];
+    }
+    /* constants are not in the range of A2 (its columns sum to zero) */
+    orc_remove_mean(nxy, r2);
+    zavg_coefficients(lhs_type, N, d, K, n, C);
+    double sn = 0.0;
+    for (int64_t c = 0; c < nc; c++) sn += n[c];
+    double v19[19];
+    uniform_row(lhs_type, N, d, K, sn / (double)nc, v19);
+    gmres2_fixed(ORC_FFT2_ITS, N, C, v19, r2, e2);
+    for (int64_t c = 0; c < nc; c++) z[c] += e2[c % nxy];
+    free(w); free(C); free(r2); free(e2);
+    orc_remove_mean(nc, z);
+    return;
+  }
   if (pc == ORC_PC_ZLINE) {
     const int64_t m = N[2], plane = N[0] * N[1];
     double *buf = malloc(sizeof(double) * (size_t)m * 5);

@@ -492,7 +492,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
     unsigned char *q = nullptr;
     A(dalloc(ctx, &q, (int64_t)ts * fft2_buf_elems(ctx)));
     ctx->pc2_buf = q;
-    A(dalloc(ctx, &ctx->pc2_h, 64));
+    A(dalloc(ctx, &ctx->pc2_h, fft2_h_elems(ctx)));
   }
   launch_fill<double>(ctx, nc, ctx->phi, 0.0);
   launch_fill<double>(ctx, 1, ctx->S_dev, 0.0);

@@ -12,6 +12,7 @@
 // spectrum, nx/2 complex values per line), so every pass moves as many bytes
 // as the real field: x-fwd, y-fwd, z-fwd/scale/z-inv, y-inv, x-inv.  Grid
 // sizes: powers of two.
+#include <algorithm>
 #include <cmath>
 
 #include "chy_internal.h"
@@ -511,95 +512,63 @@ int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
 }
 
 // ---- two-level preconditioner: kz = 0 correction -------------------------------------
-// x-y fields of the 2-D problem in pc2_buf: C (9), V (kFft2Its + 1), Z
-// (kFft2Its), w, r2, e2, and a complex work plane (2).
+// The 2-D inner GMRES runs on up to kNB2 blocks of 256 threads.  Every global
+// sum is left as one partial per block; the kernel that needs it adds the
+// partials itself (every block in the same fixed order, so all blocks agree
+// bit for bit), which keeps one launch per Gram-Schmidt pass.
+constexpr int kHm = kFft2Its;
+constexpr int kNB2 = 256;
+constexpr int kExtZ = 16;  // planes per block when the correction is added
+
+static int nb2_of(int64_t nxy) { return (int)std::min<int64_t>(kNB2, (nxy + 255) / 256); }
+
+// x-y fields in pc2_buf (T): C (9), V (kFft2Its + 1), Z (kFft2Its), w, and a
+// complex work plane (2).
 int64_t fft2_buf_elems(const chy_ctx *ctx)
 {
-  return (int64_t)(9 + (kFft2Its + 1) + kFft2Its + 3 + 2) * ctx->g.nx * ctx->g.ny;
+  return (int64_t)(9 + (kFft2Its + 1) + kFft2Its + 1 + 2) * ctx->g.nx * ctx->g.ny;
 }
 
-// pc2_h layout: H[(m+1) m], g[m+1], cs[m], sn[m], y[m], jm, stopped
-constexpr int kHm = kFft2Its;
-constexpr int kHG = (kHm + 1) * kHm, kHCs = kHG + kHm + 1, kHSn = kHCs + kHm, kHY = kHSn + kHm, kHJm = kHY + kHm,
-              kHStop = kHJm + 1;
+// Double workspace in pc2_h: the z-chunk column sums part[zc][nxy], the
+// stencil-block sums pl[nlb], then per-block partials of the inner GMRES:
+// pn[j][nb] (|V_j|^2 before normalisation, j <= m), p1[j][q][nb] and
+// p2[j][q][nb] (V_q . w in the two Gram-Schmidt passes of step j), and the
+// least-squares coefficients y[m].
+struct H2 {
+  double *part, *pl, *pn, *p1, *p2, *y;
+  int64_t nlb;
+  int nb;
+};
 
-// 2-D complex pass along x: MODE 0 real in -> forward; MODE 1 inverse -> real
-// part times scale
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_fft2_x(int logn, int nlines, int B, const T *__restrict__ rin,
-                                                cpx<T> *__restrict__ buf, T *__restrict__ rout,
-                                                const double *__restrict__ tw, double scale)
+static H2 h2_layout(const chy_ctx *ctx)
 {
-  extern __shared__ unsigned char smem_raw[];
-  const int n = 1 << logn;
-  cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
-  cpx<T> *s = tws + (n >> 1);
-  load_twiddles<T>(tws, tw, n);
-  const int64_t l0 = (int64_t)blockIdx.x * B;
-  const int nb = (int)((nlines - l0) < B ? (nlines - l0) : B);
-  for (int e = threadIdx.x; e < (nb << logn); e += blockDim.x) {
-    const int b = e >> logn, t = e & (n - 1);
-    const int64_t a = (l0 + b) * n + t;
-    if (MODE == 0) s[b * (n + 1) + t] = {rin[a], (T)0};
-    else s[b * (n + 1) + t] = buf[a];
-  }
-  __syncthreads();
-  if (MODE == 0) dif_fwd<T>(s, nb, logn, tws);
-  else dit_inv<T>(s, nb, logn, tws);
-  for (int e = threadIdx.x; e < (nb << logn); e += blockDim.x) {
-    const int b = e >> logn, t = e & (n - 1);
-    const int64_t a = (l0 + b) * n + t;
-    if (MODE == 0) buf[a] = s[b * (n + 1) + t];
-    else rout[a] = (T)((double)s[b * (n + 1) + t].x * scale);
-  }
+  const int64_t nxy = (int64_t)ctx->g.nx * ctx->g.ny;
+  H2 h;
+  h.nb = nb2_of(nxy);
+  h.nlb = lhs_zsum_blocks(ctx->g);
+  h.part = ctx->pc2_h;
+  h.pl = h.part + (int64_t)lhs_zchunks(ctx->g) * nxy;
+  h.pn = h.pl + h.nlb;
+  h.p1 = h.pn + (int64_t)(kHm + 1) * h.nb;
+  h.p2 = h.p1 + (int64_t)kHm * kHm * h.nb;
+  h.y = h.p2 + (int64_t)kHm * kHm * h.nb;
+  return h;
 }
 
-// r2[col] = mean over k of (r - w)
-template <typename T>
-__global__ __launch_bounds__(256) void k_zavg_diff(int64_t nxy, int nz, const T *__restrict__ r, const T *__restrict__ w,
-                                                   T *__restrict__ r2)
+int64_t fft2_h_elems(const chy_ctx *ctx)
 {
-  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= nxy) return;
-  double s = 0.0;
-  for (int k = 0; k < nz; k++) s += (double)r[col + nxy * k] - (double)w[col + nxy * k];
-  r2[col] = (T)(s / (double)nz);
+  const int64_t nxy = (int64_t)ctx->g.nx * ctx->g.ny;
+  const int nb = nb2_of(nxy);
+  return (int64_t)lhs_zchunks(ctx->g) * nxy + lhs_zsum_blocks(ctx->g) + (int64_t)(kHm + 1) * nb +
+         2 * (int64_t)kHm * kHm * nb + kHm;
 }
 
-// y = A2 e with the z-averaged 9-point coefficients
-template <typename T>
-__global__ __launch_bounds__(256) void k_a2(int nx, int ny, const T *__restrict__ C, const T *__restrict__ e,
-                                            T *__restrict__ y)
-{
-  const int64_t nxy = (int64_t)nx * ny;
-  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= nxy) return;
-  const int i = (int)(col % nx), j = (int)(col / nx);
-  const int ip = i + 1 == nx ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
-  const int jp = j + 1 == ny ? 0 : j + 1, jm = j == 0 ? ny - 1 : j - 1;
-  const int xs[3] = {im, i, ip}, ys[3] = {jm, j, jp};
-  double acc = 0.0;
-#pragma unroll
-  for (int o = 0; o < 9; o++) acc += (double)C[o * nxy + col] * (double)e[xs[o % 3] + (int64_t)nx * ys[o / 3]];
-  y[col] = (T)acc;
-}
-
-// z[k * nxy + col] += e2[col]
-template <typename T>
-__global__ __launch_bounds__(256) void k_add_ext(int64_t nxy, const T *__restrict__ e2, T *__restrict__ z)
-{
-  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= nxy) return;
-  const int64_t a = (int64_t)blockIdx.y * nxy + col;
-  z[a] = (T)((double)z[a] + (double)e2[col]);
-}
-
-// Block-wide sum of nv values per thread (1024 threads): out[q] for q < nv.
+// Block-wide sums of nv values per thread: out[q] (LDS) for q < nv.
 template <int NV>
 __device__ __forceinline__ void block_sums(double (&v)[NV], int nv, double *out)
 {
   __shared__ double sh[16][NV];
-  const int w = threadIdx.x >> 6;
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
   for (int q = 0; q < NV; q++) {
     if (q < nv) {
@@ -612,158 +581,276 @@ __device__ __forceinline__ void block_sums(double (&v)[NV], int nv, double *out)
   __syncthreads();
   if (threadIdx.x < nv) {
     double t = 0.0;
-    for (int k = 0; k < 16; k++) t += sh[k][threadIdx.x];
+    for (int k = 0; k < nw; k++) t += sh[k][threadIdx.x];
     out[threadIdx.x] = t;
   }
   __syncthreads();
 }
 
-// Start of the inner GMRES (one block): r2 -= mean(r2), beta = |r2|,
-// V0 = r2 / beta, g = (beta, 0, ...).
-template <typename T>
-__global__ __launch_bounds__(1024) void k2_start(int64_t nxy, const T *__restrict__ r2, T *__restrict__ V0,
-                                                 double *__restrict__ hs)
+// out[q] = sum over b < nb of p[q * nb + b], q < nv (LDS; same value in every block)
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *__restrict__ p, int nb, int nv, double *out)
 {
-  __shared__ double red[2];
+  double a[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) a[q] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < NV; q++)
+      if (q < nv) a[q] += p[(int64_t)q * nb + b];
+  }
+  block_sums<NV>(a, nv, out);
+}
+
+// V0 = r2 - mean(r2), with r2[col] = (1/nz) sum_k (r - A z) from the z-chunk
+// column sums; pn[0][blk] = the block's part of |V0|^2
+template <typename T>
+__global__ __launch_bounds__(256) void k2_r2(int64_t nxy, int nz, int nzc, const double *__restrict__ part,
+                                             const double *__restrict__ pl, int64_t nlb, T *__restrict__ V0,
+                                             double *__restrict__ pn)
+{
+  __shared__ double m[1], q2[1];
   double a[1] = {0.0};
-  for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) a[0] += (double)r2[c];
-  block_sums<1>(a, 1, red);
-  const double mean = red[0] / (double)nxy;
+  for (int64_t b = threadIdx.x; b < nlb; b += blockDim.x) a[0] += pl[b];
+  block_sums<1>(a, 1, m);
+  const double inz = 1.0 / (double)nz, mean = m[0] * inz / (double)nxy;
   a[0] = 0.0;
-  for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) {
-    const double x = (double)r2[c] - mean;
-    a[0] += x * x;
+  for (int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; col < nxy; col += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int zc = 0; zc < nzc; zc++) s += part[(int64_t)zc * nxy + col];
+    const T x = (T)(s * inz - mean);
+    V0[col] = x;
+    a[0] += (double)x * (double)x;
   }
-  block_sums<1>(a, 1, red + 1);
-  const double beta = sqrt(red[1]);
-  const double inv = beta > 0.0 ? 1.0 / beta : 0.0;
-  for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) V0[c] = (T)(((double)r2[c] - mean) * inv);
-  if (threadIdx.x == 0) {
-    for (int q = 0; q <= kHm; q++) hs[kHG + q] = 0.0;
-    hs[kHG] = beta;
-    hs[kHJm] = 0.0;
-    hs[kHStop] = beta > 0.0 ? 0.0 : 1.0;
-  }
+  block_sums<1>(a, 1, q2);
+  if (threadIdx.x == 0) pn[blockIdx.x] = q2[0];
 }
 
-// Arnoldi step j (one block): w against V_0..V_j twice (classical
-// Gram-Schmidt), V_{j+1} = w / |w|, then the Givens update of column j.
-template <typename T>
-__global__ __launch_bounds__(1024) void k2_gs(int j, int64_t nxy, T *__restrict__ V, T *__restrict__ w,
-                                              double *__restrict__ hs)
+// 2-D complex pass along x.  MODE 0: V_j (before normalisation, |V_j|^2 in
+// the partials pn) is scaled to unit norm (zero when the norm is zero),
+// written back, and transformed forward.  MODE 1: inverse transform, real
+// part to io.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_fft2_x(int logn, int nlines, int B, T *__restrict__ io,
+                                                cpx<T> *__restrict__ buf, const double *__restrict__ tw,
+                                                const double *__restrict__ pn, int nb)
 {
-  __shared__ double h1[kHm + 2], h2[kHm + 2];
-  T *Vn = V + (int64_t)(j + 1) * nxy;
-  if (hs[kHStop] != 0.0) {
-    for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) Vn[c] = (T)0;
-    return;
+  extern __shared__ unsigned char smem_raw[];
+  const int n = 1 << logn;
+  cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
+  cpx<T> *s = tws + (n >> 1);
+  __shared__ double hn[1];
+  double inv = 1.0;
+  if (MODE == 0) {
+    sum_partials<1>(pn, nb, 1, hn);
+    const double h = sqrt(hn[0]);
+    inv = h > 0.0 ? 1.0 / h : 0.0;
   }
-  const int nv = j + 1;
-  for (int pass = 0; pass < 2; pass++) {
-    double a[kHm + 1];
-#pragma unroll
-    for (int q = 0; q <= kHm; q++) a[q] = 0.0;
-    for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) {
-      const double wc = (double)w[c];
-#pragma unroll
-      for (int q = 0; q <= kHm; q++)
-        if (q < nv) a[q] += (double)V[(int64_t)q * nxy + c] * wc;
-    }
-    double *h = pass == 0 ? h1 : h2;
-    block_sums<kHm + 1>(a, nv, h);
-    for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) {
-      double x = (double)w[c];
-      for (int q = 0; q < nv; q++) x -= h[q] * (double)V[(int64_t)q * nxy + c];
-      w[c] = (T)x;
-    }
-    __syncthreads();
-  }
-  double a[1] = {0.0};
-  for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) a[0] += (double)w[c] * (double)w[c];
-  __shared__ double nn[1];
-  block_sums<1>(a, 1, nn);
-  const double hn = sqrt(nn[0]);
-  const double inv = hn > 0.0 ? 1.0 / hn : 0.0;
-  for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) Vn[c] = (T)((double)w[c] * inv);
-  if (threadIdx.x == 0) {
-    double h[kHm + 2];
-    for (int q = 0; q < nv; q++) h[q] = h1[q] + h2[q];
-    h[nv] = hn;
-    double *cs = hs + kHCs, *sn = hs + kHSn, *g = hs + kHG;
-    for (int q = 0; q < j; q++) {
-      const double t = cs[q] * h[q] + sn[q] * h[q + 1];
-      h[q + 1] = -sn[q] * h[q] + cs[q] * h[q + 1];
-      h[q] = t;
-    }
-    const double rr = sqrt(h[j] * h[j] + h[j + 1] * h[j + 1]);
-    cs[j] = rr > 0.0 ? h[j] / rr : 1.0;
-    sn[j] = rr > 0.0 ? h[j + 1] / rr : 0.0;
-    h[j] = rr;
-    g[j + 1] = -sn[j] * g[j];
-    g[j] = cs[j] * g[j];
-    for (int q = 0; q <= j; q++) hs[q * kHm + j] = h[q];
-    hs[kHJm] = (double)(j + 1);
-    if (!(hn > 0.0)) hs[kHStop] = 1.0;
-  }
-}
-
-// Least-squares solution of the Givens-reduced system and e2 = sum_q y_q Z_q.
-template <typename T>
-__global__ __launch_bounds__(1024) void k2_finish(int64_t nxy, const T *__restrict__ Z, double *__restrict__ hs,
-                                                  T *__restrict__ e2)
-{
-  __shared__ double y[kHm];
-  __shared__ int jm;
-  if (threadIdx.x == 0) {
-    jm = (int)hs[kHJm];
-    for (int q = jm - 1; q >= 0; q--) {
-      double s = hs[kHG + q];
-      for (int r = q + 1; r < jm; r++) s -= hs[q * kHm + r] * y[r];
-      const double d = hs[q * kHm + q];
-      y[q] = d != 0.0 ? s / d : 0.0;
+  load_twiddles<T>(tws, tw, n);
+  const int64_t l0 = (int64_t)blockIdx.x * B;
+  const int nbl = (int)((nlines - l0) < B ? (nlines - l0) : B);
+  for (int e = threadIdx.x; e < (nbl << logn); e += blockDim.x) {
+    const int b = e >> logn, t = e & (n - 1);
+    const int64_t a = (l0 + b) * n + t;
+    if (MODE == 0) {
+      const T v = (T)((double)io[a] * inv);
+      io[a] = v;
+      s[b * (n + 1) + t] = {v, (T)0};
+    } else {
+      s[b * (n + 1) + t] = buf[a];
     }
   }
   __syncthreads();
-  for (int64_t c = threadIdx.x; c < nxy; c += blockDim.x) {
-    double acc = 0.0;
-    for (int q = 0; q < jm; q++) acc += y[q] * (double)Z[(int64_t)q * nxy + c];
-    e2[c] = (T)acc;
+  if (MODE == 0) dif_fwd<T>(s, nbl, logn, tws);
+  else dit_inv<T>(s, nbl, logn, tws);
+  for (int e = threadIdx.x; e < (nbl << logn); e += blockDim.x) {
+    const int b = e >> logn, t = e & (n - 1);
+    const int64_t a = (l0 + b) * n + t;
+    if (MODE == 0) buf[a] = s[b * (n + 1) + t];
+    else io[a] = s[b * (n + 1) + t].x;
   }
+}
+
+// w = A2 Z_j with the z-averaged 9-point coefficients; p1[q][blk] = the
+// block's part of V_q . w, q <= j
+template <typename T>
+__global__ __launch_bounds__(256) void k2_a2dots(int nx, int ny, int j, const T *__restrict__ C,
+                                                 const T *__restrict__ Zj, const T *__restrict__ V,
+                                                 T *__restrict__ w, double *__restrict__ p1, int nb)
+{
+  __shared__ double h[kHm];
+  const int64_t nxy = (int64_t)nx * ny;
+  double a[kHm];
+#pragma unroll
+  for (int q = 0; q < kHm; q++) a[q] = 0.0;
+  for (int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; col < nxy; col += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(col % nx), jy = (int)(col / nx);
+    const int ip = i + 1 == nx ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
+    const int jp = jy + 1 == ny ? 0 : jy + 1, jm = jy == 0 ? ny - 1 : jy - 1;
+    const int xs[3] = {im, i, ip}, ys[3] = {jm, jy, jp};
+    double acc = 0.0;
+#pragma unroll
+    for (int o = 0; o < 9; o++) acc += (double)C[o * nxy + col] * (double)Zj[xs[o % 3] + (int64_t)nx * ys[o / 3]];
+    const T wt = (T)acc;
+    w[col] = wt;
+#pragma unroll
+    for (int q = 0; q < kHm; q++)
+      if (q <= j) a[q] += (double)V[(int64_t)q * nxy + col] * (double)wt;
+  }
+  block_sums<kHm>(a, j + 1, h);
+  if (threadIdx.x <= j) p1[(int64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// One classical Gram-Schmidt pass of step j: h = the sums of pin,
+// x = w - sum_q h_q V_q.  PASS 1: w = x, pout[q][blk] = parts of V_q . x.
+// PASS 2: V_{j+1} = x (normalised by the next pass along x), pout[blk] = part
+// of |x|^2.
+template <typename T, int PASS>
+__global__ __launch_bounds__(256) void k2_gs_pass(int64_t nxy, int j, T *__restrict__ V, T *__restrict__ w,
+                                                  const double *__restrict__ pin, double *__restrict__ pout, int nb)
+{
+  __shared__ double h[kHm], o[kHm];
+  sum_partials<kHm>(pin, nb, j + 1, h);
+  double a[kHm];
+#pragma unroll
+  for (int q = 0; q < kHm; q++) a[q] = 0.0;
+  T *Vn = V + (int64_t)(j + 1) * nxy;
+  for (int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; col < nxy; col += (int64_t)gridDim.x * blockDim.x) {
+    double x = (double)w[col];
+#pragma unroll
+    for (int q = 0; q < kHm; q++)
+      if (q <= j) x -= h[q] * (double)V[(int64_t)q * nxy + col];
+    const T xt = (T)x;
+    if (PASS == 1) {
+      w[col] = xt;
+#pragma unroll
+      for (int q = 0; q < kHm; q++)
+        if (q <= j) a[q] += (double)V[(int64_t)q * nxy + col] * (double)xt;
+    } else {
+      Vn[col] = xt;
+      a[0] += (double)xt * (double)xt;
+    }
+  }
+  const int nv = PASS == 1 ? j + 1 : 1;
+  block_sums<kHm>(a, nv, o);
+  if (threadIdx.x < nv) pout[(int64_t)threadIdx.x * nb + blockIdx.x] = o[threadIdx.x];
+}
+
+// Hessenberg columns from the partials, Givens QR and back substitution
+// (one block): y[q], q < kHm (zero past a breakdown).
+__global__ __launch_bounds__(256) void k2_solve(const double *__restrict__ pn, const double *__restrict__ p1,
+                                                const double *__restrict__ p2, int nb, double *__restrict__ y)
+{
+  __shared__ double n2[kHm + 1], h1[kHm][kHm], h2[kHm][kHm];
+  for (int j = 0; j <= kHm; j++) sum_partials<1>(pn + (int64_t)j * nb, nb, 1, n2 + j);
+  for (int j = 0; j < kHm; j++) {
+    sum_partials<kHm>(p1 + (int64_t)j * kHm * nb, nb, j + 1, h1[j]);
+    sum_partials<kHm>(p2 + (int64_t)j * kHm * nb, nb, j + 1, h2[j]);
+  }
+  if (threadIdx.x != 0) return;
+  double H[kHm][kHm], g[kHm + 1], cs[kHm], sn[kHm], yy[kHm];
+  for (int q = 0; q <= kHm; q++) g[q] = 0.0;
+  for (int q = 0; q < kHm; q++) yy[q] = 0.0;
+  const double beta = sqrt(n2[0]);
+  int jm = 0;
+  if (beta > 0.0) {
+    g[0] = beta;
+    for (int j = 0; j < kHm; j++) {
+      double h[kHm + 1];
+      for (int q = 0; q <= j; q++) h[q] = h1[j][q] + h2[j][q];
+      h[j + 1] = sqrt(n2[j + 1]);
+      const bool more = h[j + 1] > 0.0;
+      for (int q = 0; q < j; q++) {
+        const double t = cs[q] * h[q] + sn[q] * h[q + 1];
+        h[q + 1] = -sn[q] * h[q] + cs[q] * h[q + 1];
+        h[q] = t;
+      }
+      const double rr = sqrt(h[j] * h[j] + h[j + 1] * h[j + 1]);
+      cs[j] = rr > 0.0 ? h[j] / rr : 1.0;
+      sn[j] = rr > 0.0 ? h[j + 1] / rr : 0.0;
+      h[j] = rr;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      for (int q = 0; q <= j; q++) H[q][j] = h[q];
+      jm = j + 1;
+      if (!more) break;
+    }
+    for (int q = jm - 1; q >= 0; q--) {
+      double s = g[q];
+      for (int r = q + 1; r < jm; r++) s -= H[q][r] * yy[r];
+      yy[q] = H[q][q] != 0.0 ? s / H[q][q] : 0.0;
+    }
+  }
+  for (int q = 0; q < kHm; q++) y[q] = yy[q];
+}
+
+// z[k * nxy + col] += sum_q y_q Z_q[col] over kExtZ planes per block
+template <typename T>
+__global__ __launch_bounds__(256) void k2_ext(int64_t nxy, int nz, const T *__restrict__ Z, const double *__restrict__ y,
+                                              T *__restrict__ z)
+{
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= nxy) return;
+  double e = 0.0;
+#pragma unroll
+  for (int q = 0; q < kHm; q++) e += y[q] * (double)Z[(int64_t)q * nxy + col];
+  const int k0 = blockIdx.y * kExtZ;
+  T v[kExtZ];
+#pragma unroll
+  for (int u = 0; u < kExtZ; u++)
+    if (k0 + u < nz) v[u] = z[(int64_t)(k0 + u) * nxy + col];
+#pragma unroll
+  for (int u = 0; u < kExtZ; u++)
+    if (k0 + u < nz) z[(int64_t)(k0 + u) * nxy + col] = (T)((double)v[u] + e);
+}
+
+// Lines per block of a 2-D pass: at least 64 blocks when there are lines enough
+template <typename T>
+static int fft2_batch(int n, int64_t lines)
+{
+  return fft_batch<T>(n, std::max<int64_t>(1, lines / 64));
 }
 
 template <typename T>
 int fft2_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z, T *w3)
 {
+  (void)w3;
   TRY(fft_pc_apply<T>(ctx, v19, r, z));
   if (ctx->solve_transpose) return CHY_OK;  // the correction is built for A(n), not its transpose
   const int nx = ctx->g.nx, ny = ctx->g.ny, nz = ctx->g.nz;
   const int64_t nxy = (int64_t)nx * ny;
-  T *C = (T *)ctx->pc2_buf, *V = C + 9 * nxy, *Z = V + (kFft2Its + 1) * nxy, *w = Z + kFft2Its * nxy,
-    *r2 = w + nxy, *e2 = r2 + nxy;
-  cpx<T> *cb = (cpx<T> *)(e2 + nxy);
-  double *hs = ctx->pc2_h;
-  const unsigned nb2 = (unsigned)((nxy + kBlock - 1) / kBlock);
-  launch_lhs_apply<T, T, T>(ctx, ctx->prm.lhs_type, false, (const T *)ctx->mom[0], z, w3);
-  hipLaunchKernelGGL((k_zavg_diff<T>), nb2, kBlock, 0, ctx->stream, nxy, nz, r, (const T *)w3, r2);
-  hipLaunchKernelGGL((k2_start<T>), 1, 1024, 0, ctx->stream, nxy, (const T *)r2, V, hs);
+  T *C = (T *)ctx->pc2_buf, *V = C + 9 * nxy, *Z = V + (kFft2Its + 1) * nxy, *w = Z + kFft2Its * nxy;
+  cpx<T> *cb = (cpx<T> *)(w + nxy);
+  const H2 h = h2_layout(ctx);
+  const int nb = h.nb;
+  // z-averaged residual of the first level, mean removed, into V0
+  launch_lhs_zsum<T>(ctx, ctx->prm.lhs_type, (const T *)ctx->mom[0], z, r, h.part, h.pl);
+  hipLaunchKernelGGL((k2_r2<T>), nb, 256, 0, ctx->stream, nxy, nz, lhs_zchunks(ctx->g), (const double *)h.part,
+                     (const double *)h.pl, h.nlb, V, h.pn);
   Symbol sym = make_symbol(ctx, v19);
   sym.inv_scale = 1.0 / (double)nxy;
-  const int Bx = fft_batch<T>(nx, ny), By = fft_batch<T>(ny, nx);
+  const int Bx = fft2_batch<T>(nx, ny), By = fft2_batch<T>(ny, nx);
   for (int j = 0; j < kFft2Its; j++) {
     T *Vj = V + (int64_t)j * nxy, *Zj = Z + (int64_t)j * nxy;
     // Zj = F2 Vj: 2-D FFT solve with the kz = 0 symbol
     hipLaunchKernelGGL((k_fft2_x<T, 0>), (unsigned)((ny + Bx - 1) / Bx), kBlock, fft_lds<T>(nx, Bx), ctx->stream,
-                       ilog2(nx), ny, Bx, (const T *)Vj, cb, (T *)nullptr, ctx->tw[0], 1.0);
+                       ilog2(nx), ny, Bx, Vj, cb, ctx->tw[0], (const double *)(h.pn + (int64_t)j * nb), nb);
     hipLaunchKernelGGL((k_fft_s<T, 3>), (unsigned)(nx / By), kBlock, fft_lds<T>(ny, By), ctx->stream, ilog2(ny),
                        (int64_t)nx, (int64_t)nx, (int64_t)0, By, cb, ctx->tw[1], sym, ctx->trig[0], ctx->trig[1],
                        ctx->trig[2]);
     hipLaunchKernelGGL((k_fft2_x<T, 1>), (unsigned)((ny + Bx - 1) / Bx), kBlock, fft_lds<T>(nx, Bx), ctx->stream,
-                       ilog2(nx), ny, Bx, (const T *)nullptr, cb, Zj, ctx->tw[0], 1.0);
-    hipLaunchKernelGGL((k_a2<T>), nb2, kBlock, 0, ctx->stream, nx, ny, (const T *)C, (const T *)Zj, w);
-    hipLaunchKernelGGL((k2_gs<T>), 1, 1024, 0, ctx->stream, j, nxy, V, w, hs);
+                       ilog2(nx), ny, Bx, Zj, cb, ctx->tw[0], (const double *)nullptr, 0);
+    double *p1 = h.p1 + (int64_t)j * kHm * nb, *p2 = h.p2 + (int64_t)j * kHm * nb;
+    hipLaunchKernelGGL((k2_a2dots<T>), nb, 256, 0, ctx->stream, nx, ny, j, (const T *)C, (const T *)Zj,
+                       (const T *)V, w, p1, nb);
+    hipLaunchKernelGGL((k2_gs_pass<T, 1>), nb, 256, 0, ctx->stream, nxy, j, V, w, (const double *)p1, p2, nb);
+    hipLaunchKernelGGL((k2_gs_pass<T, 2>), nb, 256, 0, ctx->stream, nxy, j, V, w, (const double *)p2,
+                       h.pn + (int64_t)(j + 1) * nb, nb);
   }
-  hipLaunchKernelGGL((k2_finish<T>), 1, 1024, 0, ctx->stream, nxy, (const T *)Z, hs, e2);
-  hipLaunchKernelGGL((k_add_ext<T>), dim3(nb2, (unsigned)nz), kBlock, 0, ctx->stream, nxy, (const T *)e2, z);
+  hipLaunchKernelGGL(k2_solve, 1, 256, 0, ctx->stream, (const double *)h.pn, (const double *)h.p1,
+                     (const double *)h.p2, nb, h.y);
+  hipLaunchKernelGGL((k2_ext<T>), dim3((unsigned)((nxy + 255) / 256), (unsigned)((nz + kExtZ - 1) / kExtZ)), 256, 0,
+                     ctx->stream, nxy, nz, (const T *)Z, (const double *)h.y, z);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("fft2: ") + hipGetErrorString(e));
   return CHY_OK;

@@ -170,7 +170,9 @@ __global__ __launch_bounds__(256) void k_residual(Grid g, int lhs_type, SCoef s,
 // over kZChunk planes.
 // It keeps three z-planes of x and n, each with a one-node halo in x and y
 // ((64+2) x (4+2) values), in LDS and marches up in z, loading one plane per
-// step with coalesced x-fastest reads.  Output: y = A x, or r = b - A x.
+// step with coalesced x-fastest reads.  Output (OUT): 0 y = A x, 1 r = b - A x,
+// 2 the column sums over the block's planes of b - A x, y[zc * nx * ny + col]
+// for z chunk zc, plus the block's total in pl[block] (both in double).
 constexpr int kTX = 64, kTY = 16, kRows = kTY / 4, kZChunk = 16;  // 64 x 4 threads, 4 rows each
 constexpr int kHX = kTX + 2, kHY = kTY + 2;
 
@@ -180,10 +182,11 @@ constexpr int kHX = kTX + 2, kHY = kTY + 2;
 // computed from LDS, so global latency overlaps the arithmetic.
 constexpr int kPer = (kHX * kHY + 255) / 256;
 
-template <typename TN, typename TX, typename TY, typename C, bool DIFF, bool RESID>
+template <typename TN, typename TX, typename TY, typename C, bool DIFF, int OUT, typename TB = double>
 __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s, LapRow lap,
                                                    const TN *__restrict__ n, const TX *__restrict__ x,
-                                                   const double *__restrict__ b, TY *__restrict__ y)
+                                                   const TB *__restrict__ b, TY *__restrict__ y,
+                                                   double *__restrict__ pl)
 {
   __shared__ C xs[3][kHY][kHX];
   __shared__ C ns[3][kHY][kHX];
@@ -228,6 +231,9 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
       }
     }
   };
+  double zs[kRows];
+#pragma unroll
+  for (int rr = 0; rr < kRows; rr++) zs[rr] = 0.0;
   fetch(z0 - 1);
   store((z0 + 2) % 3);
   fetch(z0);
@@ -266,11 +272,31 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
           for (int q = 0; q < 19; q++) acc += v[q] * xv[q];
         }
         const int64_t c = (int64_t)i + (int64_t)g.nx * ((int64_t)j + (int64_t)g.ny * k);
-        if (RESID) y[c] = (TY)(b[c] - (double)acc);
+        if (OUT == 1) y[c] = (TY)((double)b[c] - (double)acc);
+        else if (OUT == 2) zs[rr] += (double)b[c] - (double)acc;
         else y[c] = (TY)acc;
       }
     }
     __syncthreads();
+  }
+  if (OUT == 2) {
+    __shared__ double wsum[4];
+    double t = 0.0;
+#pragma unroll
+    for (int rr = 0; rr < kRows; rr++) {
+      const int j = y0 + ty0 + 4 * rr;
+      if (i < g.nx && j < g.ny) {
+        y[(int64_t)blockIdx.z * pstride + i + (int64_t)g.nx * j] = (TY)zs[rr];
+        t += zs[rr];
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      pl[blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z)] =
+          ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
   }
 }
 
@@ -300,8 +326,26 @@ void launch_lhs_apply(chy_ctx *ctx, int lhs_type, bool transpose, const TN *n, c
     hipLaunchKernelGGL((k_lhs_apply_t<TN, TX, TY, C>), nblocks(ctx->g.nc), kBlock, 0, ctx->stream, ctx->g,
                        lhs_type, s, L, n, x, y);
   else
-    hipLaunchKernelGGL((k_lhs_tiled<TN, TX, TY, C, fp32, false>), tiled_grid(ctx->g), kBlock, 0, ctx->stream, ctx->g,
-                       lhs_type, s, L, n, x, (const double *)nullptr, y);
+    hipLaunchKernelGGL((k_lhs_tiled<TN, TX, TY, C, fp32, 0>), tiled_grid(ctx->g), kBlock, 0, ctx->stream, ctx->g,
+                       lhs_type, s, L, n, x, (const double *)nullptr, y, (double *)nullptr);
+}
+
+int lhs_zchunks(const Grid &g) { return (g.nz + kZChunk - 1) / kZChunk; }
+int64_t lhs_zsum_blocks(const Grid &g)
+{
+  const dim3 d = tiled_grid(g);
+  return (int64_t)d.x * d.y * d.z;
+}
+
+template <typename T>
+void launch_lhs_zsum(chy_ctx *ctx, int lhs_type, const T *n, const T *x, const T *b, double *part, double *pl)
+{
+  const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
+  const LapRow L = lap_row(ctx, lhs_type);
+  constexpr bool fp32 = sizeof(T) == 4;
+  using C = typename std::conditional<fp32, float, double>::type;
+  hipLaunchKernelGGL((k_lhs_tiled<T, T, double, C, fp32, 2, T>), tiled_grid(ctx->g), kBlock, 0, ctx->stream, ctx->g,
+                     lhs_type, s, L, n, x, b, part, pl);
 }
 
 template <typename TN, typename TX>
@@ -309,8 +353,8 @@ void launch_residual(chy_ctx *ctx, int lhs_type, const TN *n, const double *b, c
 {
   const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
   const LapRow L = lap_row(ctx, lhs_type);
-  hipLaunchKernelGGL((k_lhs_tiled<TN, TX, double, double, false, true>), tiled_grid(ctx->g), kBlock, 0, ctx->stream,
-                     ctx->g, lhs_type, s, L, n, x, b, r);
+  hipLaunchKernelGGL((k_lhs_tiled<TN, TX, double, double, false, 1>), tiled_grid(ctx->g), kBlock, 0, ctx->stream,
+                     ctx->g, lhs_type, s, L, n, x, b, r, (double *)nullptr);
 }
 
 template <typename T>
@@ -1122,6 +1166,9 @@ void launch_cheby(chy_ctx *ctx, int lhs_type, bool transpose, int degree, const 
 template void launch_lhs_apply<double, double, double>(chy_ctx *, int, bool, const double *, const double *, double *);
 template void launch_lhs_apply<float, float, float>(chy_ctx *, int, bool, const float *, const float *, float *);
 template void launch_lhs_apply<float, double, double>(chy_ctx *, int, bool, const float *, const double *, double *);
+template void launch_lhs_zsum<double>(chy_ctx *, int, const double *, const double *, const double *, double *,
+                                      double *);
+template void launch_lhs_zsum<float>(chy_ctx *, int, const float *, const float *, const float *, double *, double *);
 template void launch_residual<double, double>(chy_ctx *, int, const double *, const double *, const double *, double *);
 template void launch_residual<float, double>(chy_ctx *, int, const float *, const double *, const double *, double *);
 template void launch_zcoef<double>(chy_ctx *, int, const double *, double *);

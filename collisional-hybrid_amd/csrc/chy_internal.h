@@ -95,7 +95,7 @@ struct chy_ctx {
   void *fft_buf = nullptr;      // complex work array
   void *pc_buf = nullptr;       // z-line / Chebyshev / two-level scratch, 3 vectors (T)
   void *pc2_buf = nullptr;      // two-level FFT preconditioner: x-y fields (T)
-  double *pc2_h = nullptr;      // its inner GMRES: Hessenberg, Givens, state
+  double *pc2_h = nullptr;      // its double workspace: column sums, partial sums
   double *tw[3] = {nullptr, nullptr, nullptr};   // twiddles per axis (complex)
   double *trig[3] = {nullptr, nullptr, nullptr}; // cos/sin of 2 pi k / N per axis
   bool fft_ok = false;
@@ -171,6 +171,12 @@ template <typename TN, typename TX, typename TY>
 void launch_lhs_apply(chy_ctx *ctx, int lhs_type, bool transpose, const TN *n, const TX *x, TY *y);
 template <typename TN, typename TX>
 void launch_residual(chy_ctx *ctx, int lhs_type, const TN *n, const double *b, const TX *x, double *r);
+// part[zc * nx * ny + col] = sum over the planes of z chunk zc of (b - A x) at
+// the column; pl[blk] = the sum over each stencil block (lhs_zsum_blocks of them)
+template <typename T>
+void launch_lhs_zsum(chy_ctx *ctx, int lhs_type, const T *n, const T *x, const T *b, double *part, double *pl);
+int lhs_zchunks(const Grid &g);
+int64_t lhs_zsum_blocks(const Grid &g);
 template <typename T>
 void launch_efield(chy_ctx *ctx, const double *phi, T *E4);
 template <typename T>
@@ -257,6 +263,7 @@ int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z);
 // pc2_buf); w3 is 3-D scratch
 constexpr int kFft2Its = 4;
 int64_t fft2_buf_elems(const chy_ctx *ctx);  // elements of T in pc2_buf
+int64_t fft2_h_elems(const chy_ctx *ctx);    // doubles in pc2_h
 template <typename T>
 int fft2_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z, T *w3);
 

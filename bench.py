@@ -45,10 +45,14 @@ PHYS = ("-mi 4.9816e-26 -mn 4.8156e-26 -Ti 300 -Tn 300 -B0z 5e-5 -E0x 0.05 -nui 
         "-seed 0 --perturbation uniform --perturbation-amplitude 0.01 -ksp_rtol 1e-10")
 
 
+# extra solver options (--solver-opts) appended to every timed workload
+SOLVER_OPTS = []
+
+
 def workload_args(N, ppc, prec):
     L = 0.05 * N
     return (f"-Nx {N} -Ny {N} -Nz {N} -x1 {L} -y1 {L} -z1 {L} -ppc {ppc} -Nt 200 -precision {prec} "
-            + PHYS).split()
+            + PHYS).split() + SOLVER_OPTS
 
 
 # Algorithmic bytes per launch of each kernel: every distinct array argument
@@ -376,7 +380,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain-path comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradient check")
+    ap.add_argument("--solver-opts", default="", help="extra options for the timed runs, e.g. '-ksp_inner_rtol 1e-4'")
     a = ap.parse_args()
+    SOLVER_OPTS[:] = a.solver_opts.split()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the hot path on one GPU; --gpus must be 1 "
               "(multi-GPU sharding is designed, not built)", file=sys.stderr)
@@ -424,6 +430,7 @@ def main():
             "workload": f"{a.grid}^3 periodic grid, {a.ppc} ions/cell ({mixed['np']} ions), mixed precision, "
                         "collisions on, B0z 5e-5 T, E0x 0.05 V/m, dt 1e-5 s, ksp_rtol 1e-10",
             "grid": a.grid, "ions_per_cell": a.ppc, "ions": mixed["np"], "precision": "mixed",
+            **({"solver_opts": a.solver_opts} if a.solver_opts else {}),
         },
         "phi_solve_ms_per_step": mixed["phi_solve_ms_per_step"],
         "particles_ms_per_step": mixed["particles_ms_per_step"],

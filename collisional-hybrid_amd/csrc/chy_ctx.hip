@@ -478,7 +478,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
     A(dalloc(ctx, &q, (int64_t)ts * nc * nvec_V(ctx))); ctx->V = q;
     A(dalloc(ctx, &q, (int64_t)ts * nc * ctx->restart)); ctx->Z = q;
     A(dalloc(ctx, &q, (int64_t)ts * nc)); ctx->wv = q;
-    A(dalloc(ctx, &q, (int64_t)ts * 2 * nc)); ctx->fft_buf = q;
+    A(dalloc(ctx, &q, (int64_t)ts * 2 * (nc + 32 * (int64_t)ctx->g.nz))); ctx->fft_buf = q;  // half spectrum, padded planes
     A(dalloc(ctx, &q, (int64_t)ts * 3 * nc)); ctx->pc_buf = q;
   }
   A(dalloc(ctx, &ctx->red, (int64_t)kRedBlocks * 32));

@@ -43,6 +43,18 @@ static int fft_batch(int n, int64_t avail)
 template <typename T>
 static size_t fft_lds(int n, int B) { return ((size_t)B * (n + 1) + (size_t)n / 2) * 2 * sizeof(T); }
 
+// The z pass also stages 3 doubles per line and the z table (2n doubles).
+template <typename T>
+static size_t fft_lds_z(int n, int B) { return fft_lds<T>(n, B) + (3 * (size_t)B + 2 * (size_t)n) * sizeof(double); }
+
+template <typename T>
+static int fft_batch_z(int n, int64_t avail)
+{
+  int B = fft_batch<T>(n, avail);
+  while (B > 1 && fft_lds_z<T>(n, B) > 65536) B >>= 1;
+  return B;
+}
+
 // One batch of B lines in LDS: s[b*(n+1) + t] (padded rows).  Twiddles
 // w[j] = exp(-2 pi i j / n), j < n/2, are staged in LDS (tw).  Butterfly
 // indices use shifts and masks (n is a power of two).
@@ -165,8 +177,10 @@ static size_t rfft_lds(int h, int B) { return ((size_t)B * (h + 1) + (size_t)h) 
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_rfft_x(int logn, int nlines, int B, const T *__restrict__ rin,
                                                 cpx<T> *__restrict__ buf, T *__restrict__ rout,
-                                                const double *__restrict__ tw, double scale)
+                                                const double *__restrict__ tw, double scale, int ly, int64_t plane)
 {
+  // half spectrum of line l = iy + ny*iz at iy*h + iz*plane
+  auto at = [&](int64_t l) { return (l & ((1 << ly) - 1)) * (int64_t)(1 << (logn - 1)) + (l >> ly) * plane; };
   extern __shared__ unsigned char smem_raw[];
   const int n = 1 << logn, lh = logn - 1, h = n >> 1;
   cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
@@ -205,7 +219,7 @@ __global__ __launch_bounds__(256) void k_rfft_x(int logn, int nlines, int B, con
         const cpx<T> WO = cmul<T>(O, tws[k]);
         X = {E.x + WO.x, E.y + WO.y};
       }
-      buf[(l0 + b) * h + k] = X;
+      buf[at(l0 + b) + k] = X;
     }
   } else {
     __syncthreads();  // the unpacking below reads twiddles loaded by other threads
@@ -216,8 +230,9 @@ __global__ __launch_bounds__(256) void k_rfft_x(int logn, int nlines, int B, con
         const int e = e0 + 256 * u;
         if (e < (nb << lh)) {
           const int b = e >> lh, k = e & (h - 1);
-          X[u] = buf[(l0 + b) * h + k];
-          M[u] = buf[(l0 + b) * h + ((h - k) & (h - 1))];
+          const int64_t o = at(l0 + b);
+          X[u] = buf[o + k];
+          M[u] = buf[o + ((h - k) & (h - 1))];
         }
       }
 #pragma unroll
@@ -256,6 +271,7 @@ __global__ __launch_bounds__(256) void k_rfft_x(int logn, int nlines, int B, con
 struct Symbol {
   double A0, F[3], P[3], M[3];  // pairs ordered xy, xz, yz
   int nx, lx, ly, lz;
+  int ynat;                     // y frequencies of the spectrum in natural (1) or bit-reversed (0) order
   double inv_scale;             // 1 / (nx ny nz)
 };
 
@@ -337,15 +353,30 @@ __global__ __launch_bounds__(256) void k_fft_s(int logn, int64_t A, int64_t estr
       dit_inv<T>(s, B, logn, tws);
     }
     if (MODE == 2) {
-      // line a = ix + nx*iy holds bit-reversed x and y frequencies, element t
-      // the bit-reversed z frequency
-      for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
-        const int t = e >> lb, b = e & (B - 1);
+      // line a = kx + (nx/2)*py holds the natural x and bit-reversed y
+      // frequencies, element t the bit-reversed z frequency.  Along a line the
+      // symbol is la + lb cos(kz) - lc sin(kz) (the z terms of symbol_at
+      // grouped): the three line constants and the z table are staged in LDS
+      // after the lines (kx >= 1 here, so the null mode does not occur).
+      double *lcn = reinterpret_cast<double *>(s + B * (n + 1));
+      double *tz = lcn + 3 * B;
+      for (int b = threadIdx.x; b < B; b += blockDim.x) {
         const int64_t a = a0 + b;
         const int kx = (int)(a & (hx - 1));
-        const int ky = bitrev((int)(a >> (sym.lx - 1)), sym.ly);
+        const int py = (int)(a >> (sym.lx - 1));
+        const int ky = sym.ynat ? py : bitrev(py, sym.ly);
+        const double cx = trx[2 * kx], sx = trx[2 * kx + 1], cy = try_[2 * ky], sy = try_[2 * ky + 1];
+        lcn[3 * b] = sym.A0 + sym.F[0] * cx + sym.F[1] * cy + (sym.P[0] * cx * cy - sym.M[0] * sx * sy);
+        lcn[3 * b + 1] = sym.F[2] + sym.P[1] * cx + sym.P[2] * cy;
+        lcn[3 * b + 2] = sym.M[1] * sx + sym.M[2] * sy;
+      }
+      for (int t = threadIdx.x; t < 2 * n; t += blockDim.x) tz[t] = trz[t];
+      __syncthreads();
+      for (int e = threadIdx.x; e < B * n; e += blockDim.x) {
+        const int t = e >> lb, b = e & (B - 1);
         const int kz = bitrev(t, sym.lz);
-        const double f = symbol_factor(sym, kx, ky, kz, trx, try_, trz);
+        const double lam = lcn[3 * b] + lcn[3 * b + 1] * tz[2 * kz] - lcn[3 * b + 2] * tz[2 * kz + 1];
+        const double f = lam == 0.0 ? 0.0 : sym.inv_scale / lam;
         cpx<T> &z = s[b * (n + 1) + t];
         z = {(T)((double)z.x * f), (T)((double)z.y * f)};
       }
@@ -377,8 +408,8 @@ __global__ __launch_bounds__(256) void k_fft_col0(int logn, int64_t estride, int
   cpx<T> *s = tws + (n >> 1);     // 2 lines, padded
   const int ny = 1 << sym.ly;
   const int p = blockIdx.x;
-  const int ky = bitrev(p, sym.ly);
-  const int pp = bitrev((ny - ky) & (ny - 1), sym.ly);
+  const int ky = sym.ynat ? p : bitrev(p, sym.ly);
+  const int pp = sym.ynat ? (ny - ky) & (ny - 1) : bitrev((ny - ky) & (ny - 1), sym.ly);
   if (pp < p) return;             // the pair is done by block pp
   const int nl = pp == p ? 1 : 2;
   load_twiddles<T>(tws, tw, n);
@@ -424,7 +455,232 @@ __global__ __launch_bounds__(256) void k_fft_col0(int logn, int64_t estride, int
   }
 }
 
+// ---- 256-point lines: register-resident four-step transforms -------------------------
+// A length-256 line is held by 16 threads, thread j owning the elements
+// j + 16 m (m < 16) in registers.  256 = 16 x 16: each thread transforms its
+// 16 values (a radix-4 x radix-4 DFT in registers), multiplies by the twiddles
+// W_256^(j k1), and one transpose through LDS hands thread k1 the values of all
+// 16 threads, whose 16-point DFT gives X[k1 + 16 k2].  Input and output use the
+// same register pattern in natural order, so forward and inverse compose
+// without permutations, and LDS carries each value twice per transform.  The
+// transpose strides are chosen so that the 64 lanes of a wave touch 64
+// distinct banks (values split into real and imaginary arrays).
+constexpr int kF256Sk = 17;   // LDS stride between the k1 rows of a line
+constexpr int kF256SbS = 324; // line stride, strided passes (lanes: 16 lines x 4 threads)
+constexpr int kF256SbX = 272; // line stride, x pass (lanes: 16 threads x 4 line pairs)
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft4(cpx<T> &a0, cpx<T> &a1, cpx<T> &a2, cpx<T> &a3)
+{
+  const cpx<T> t0 = {a0.x + a2.x, a0.y + a2.y}, t1 = {a0.x - a2.x, a0.y - a2.y};
+  const cpx<T> t2 = {a1.x + a3.x, a1.y + a3.y}, t3 = {a1.x - a3.x, a1.y - a3.y};
+  a0 = {t0.x + t2.x, t0.y + t2.y};
+  a2 = {t0.x - t2.x, t0.y - t2.y};
+  if (!INV) {  // X1 = t1 - i t3, X3 = t1 + i t3
+    a1 = {t1.x + t3.y, t1.y - t3.x};
+    a3 = {t1.x - t3.y, t1.y + t3.x};
+  } else {
+    a1 = {t1.x - t3.y, t1.y + t3.x};
+    a3 = {t1.x + t3.y, t1.y - t3.x};
+  }
+}
+
+// In-place 16-point DFT, natural order in and out: n = 4a + b, k = c + 4d.
+template <typename T, bool INV>
+__device__ __forceinline__ void dft16(cpx<T> (&v)[16])
+{
+  constexpr double c1 = 0.92387953251128674, s1 = 0.38268343236508978, h = 0.70710678118654752;  // cos, sin pi/8; sqrt 1/2
+  // W_16^m = cos(2 pi m / 16) - i sin(2 pi m / 16) for the products m = b c
+  constexpr double wr[10] = {1.0, c1, h, s1, 0.0, 0.0, -h, 0.0, 0.0, -c1};
+  constexpr double wi[10] = {0.0, -s1, -h, -c1, -1.0, 0.0, -h, 0.0, 0.0, s1};
+#pragma unroll
+  for (int b = 0; b < 4; b++) dft4<T, INV>(v[b], v[4 + b], v[8 + b], v[12 + b]);
+#pragma unroll
+  for (int b = 1; b < 4; b++) {
+#pragma unroll
+    for (int c = 1; c < 4; c++) {
+      const int m = b * c;
+      const T x = (T)wr[m], y = INV ? (T)(-wi[m]) : (T)wi[m];
+      v[4 * c + b] = cmul<T>(v[4 * c + b], {x, y});
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) dft4<T, INV>(v[4 * c], v[4 * c + 1], v[4 * c + 2], v[4 * c + 3]);
+  // position 4c + d holds X[c + 4d]
+  cpx<T> o[16];
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int d = 0; d < 4; d++) o[c + 4 * d] = v[4 * c + d];
+#pragma unroll
+  for (int q = 0; q < 16; q++) v[q] = o[q];
+}
+
+// The four-step transform of the line held by this thread's group (thread j of
+// 16; its LDS rows start at gbase).  tws: W_256^m, m < 128 (forward).  Every
+// thread of the block calls it (two barriers).
+template <typename T, bool INV>
+__device__ __forceinline__ void fft256(cpx<T> (&v)[16], int j, T *lre, T *lim, int gbase, const cpx<T> *tws)
+{
+  dft16<T, INV>(v);
+#pragma unroll
+  for (int k1 = 1; k1 < 16; k1++) {
+    const int m = j * k1;  // < 256: W^m = -W^(m-128) past 128
+    cpx<T> w = tws[m & 127];
+    if (m & 128) w = {-w.x, -w.y};
+    if (INV) w.y = -w.y;
+    v[k1] = cmul<T>(v[k1], w);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 16; k1++) {
+    lre[gbase + k1 * kF256Sk + j] = v[k1].x;
+    lim[gbase + k1 * kF256Sk + j] = v[k1].y;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int n2 = 0; n2 < 16; n2++) v[n2] = {lre[gbase + j * kF256Sk + n2], lim[gbase + j * kF256Sk + n2]};
+  __syncthreads();
+  dft16<T, INV>(v);
+}
+
+// Strided 256-point pass: lines a = a0 .. a0+15 (16 per block) of the set c,
+// line (a, c) at a + c*cstride, element t at + t*estride; thread (b, j) =
+// (threadIdx & 15, threadIdx >> 4).  MODE 0 forward, 1 inverse, 2 forward +
+// symbol division + inverse (z axis, lines a = kx + (nx/2)*py; the packed
+// kx = 0 slot is left to k_fft_col0).
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_fft256_s(int64_t A, int64_t estride, int64_t cstride,
+                                                  cpx<T> *__restrict__ buf, const double *__restrict__ tw, Symbol sym,
+                                                  const double *__restrict__ trx, const double *__restrict__ try_,
+                                                  const double *__restrict__ trz)
+{
+  __shared__ T lre[16 * kF256SbS], lim[16 * kF256SbS];
+  __shared__ cpx<T> tws[128];
+  load_twiddles<T>(tws, tw, 256);
+  __syncthreads();
+  const int b = threadIdx.x & 15, j = threadIdx.x >> 4;
+  const int64_t blocks_per_c = A / 16;
+  const int64_t c = blockIdx.x / blocks_per_c, a = (blockIdx.x % blocks_per_c) * 16 + b;
+  const int64_t base = a + c * cstride;
+  const int hx = sym.nx >> 1;
+  const bool live = MODE != 2 || (a & (hx - 1)) != 0;
+  cpx<T> v[16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) v[m] = live ? buf[base + (int64_t)(j + 16 * m) * estride] : cpx<T>{(T)0, (T)0};
+  fft256<T, MODE == 1>(v, j, lre, lim, b * kF256SbS, tws);
+  if (MODE == 2) {
+    // symbol along the line: la + lb cos(kz) - lc sin(kz), kz = j + 16 m
+    const int kx = (int)(a & (hx - 1));
+    const int py = (int)(a >> (sym.lx - 1));
+    const int ky = sym.ynat ? py : bitrev(py, sym.ly);
+    const double cx = trx[2 * kx], sx = trx[2 * kx + 1], cy = try_[2 * ky], sy = try_[2 * ky + 1];
+    const double la = sym.A0 + sym.F[0] * cx + sym.F[1] * cy + (sym.P[0] * cx * cy - sym.M[0] * sx * sy);
+    const double lb = sym.F[2] + sym.P[1] * cx + sym.P[2] * cy;
+    const double lc = sym.M[1] * sx + sym.M[2] * sy;
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const int kz = j + 16 * m;
+      const double lam = la + lb * trz[2 * kz] - lc * trz[2 * kz + 1];
+      const double f = lam == 0.0 ? 0.0 : sym.inv_scale / lam;
+      v[m] = {(T)((double)v[m].x * f), (T)((double)v[m].y * f)};
+    }
+    fft256<T, true>(v, j, lre, lim, b * kF256SbS, tws);
+  }
+  if (live) {
+#pragma unroll
+    for (int m = 0; m < 16; m++) buf[base + (int64_t)(j + 16 * m) * estride] = v[m];
+  }
+}
+
+// x pass for nx = 256, two real lines per 16-thread group (32 lines per
+// block): z = x_a + i x_b is transformed once, and X_a = (Z_k + conj Z_-k)/2,
+// X_b = (Z_k - conj Z_-k)/(2i) are written for k < 128 (slot 0 packs X_0 and
+// X_128 as X_0 + i X_128), to the same layout as k_rfft_x.  MODE 1 is the
+// exact reverse: Z_k = X_a + i X_b over the Hermitian extension, inverse
+// transform, x_a = Re, x_b = Im (256 times the input), times scale.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_rfft256_x(int64_t nlines, const T *__restrict__ rin,
+                                                   cpx<T> *__restrict__ buf, T *__restrict__ rout,
+                                                   const double *__restrict__ tw, double scale, int ly, int64_t plane)
+{
+  __shared__ T lre[16 * kF256SbX], lim[16 * kF256SbX];
+  __shared__ cpx<T> tws[128];
+  load_twiddles<T>(tws, tw, 256);
+  __syncthreads();
+  const int j = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const int64_t la = (int64_t)blockIdx.x * 32 + 2 * g, lb = la + 1;
+  const bool ok = lb < nlines;
+  auto at = [&](int64_t l) { return (l & ((1 << ly) - 1)) * (int64_t)128 + (l >> ly) * plane; };
+  const int gb = g * kF256SbX;
+  cpx<T> v[16];
+  if (MODE == 0) {
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      v[m] = ok ? cpx<T>{rin[la * 256 + j + 16 * m], rin[lb * 256 + j + 16 * m]} : cpx<T>{(T)0, (T)0};
+    fft256<T, false>(v, j, lre, lim, gb, tws);
+    // Z in natural order in LDS, for the partner values Z_-k
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      lre[gb + j + 16 * m] = v[m].x;
+      lim[gb + j + 16 * m] = v[m].y;
+    }
+    __syncthreads();
+    if (ok) {
+      const int64_t oa = at(la), ob = at(lb);
+#pragma unroll
+      for (int m = 0; m < 8; m++) {
+        const int k = j + 16 * m;
+        const T zx = lre[gb + k], zy = lim[gb + k];
+        cpx<T> xa, xb;
+        if (k == 0) {
+          xa = {zx, lre[gb + 128]};
+          xb = {zy, lim[gb + 128]};
+        } else {
+          const T mx = lre[gb + 256 - k], my = lim[gb + 256 - k];
+          xa = {(T)0.5 * (zx + mx), (T)0.5 * (zy - my)};
+          xb = {(T)0.5 * (zy + my), (T)0.5 * (mx - zx)};
+        }
+        buf[oa + k] = xa;
+        buf[ob + k] = xb;
+      }
+    }
+  } else {
+    const int64_t oa = ok ? at(la) : 0, ob = ok ? at(lb) : 0;
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const int t = j + 16 * m;
+      cpx<T> z = {(T)0, (T)0};
+      if (ok) {
+        if (t == 0 || t == 128) {
+          const cpx<T> sa = buf[oa], sb = buf[ob];
+          z = t == 0 ? cpx<T>{sa.x, sb.x} : cpx<T>{sa.y, sb.y};
+        } else if (t < 128) {
+          const cpx<T> A = buf[oa + t], B = buf[ob + t];
+          z = {A.x - B.y, A.y + B.x};
+        } else {
+          const cpx<T> A = buf[oa + 256 - t], B = buf[ob + 256 - t];
+          z = {A.x + B.y, B.x - A.y};
+        }
+      }
+      v[m] = z;
+    }
+    fft256<T, true>(v, j, lre, lim, gb, tws);
+    if (ok) {
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        rout[la * 256 + j + 16 * m] = (T)((double)v[m].x * scale);
+        rout[lb * 256 + j + 16 * m] = (T)((double)v[m].y * scale);
+      }
+    }
+  }
+}
+
 static bool pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// Complex elements between z planes of the half spectrum: (nx/2) ny plus
+// 256 bytes
+template <typename T>
+static int64_t spec_plane(int nx, int ny) { return (int64_t)(nx / 2) * ny + 256 / (int64_t)sizeof(cpx<T>); }
 static int ilog2(int64_t n) { int l = 0; while ((1ll << l) < n) l++; return l; }
 
 int fft_setup(chy_ctx *ctx)
@@ -462,6 +718,7 @@ static Symbol make_symbol(const chy_ctx *ctx, const double v19[19])
   sym.nx = nx;
   sym.lx = ilog2(nx); sym.ly = ilog2(ny); sym.lz = ilog2(nz);
   sym.inv_scale = 1.0 / ((double)nx * ny * nz);
+  sym.ynat = 0;
   return sym;
 }
 
@@ -472,39 +729,65 @@ int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
   cpx<T> *buf = (cpx<T> *)ctx->fft_buf;
   // symbol coefficients from the uniform-density row (column order of
   // lhs.c:11-104: faces 0..5, xy edges 6..9, xz 10..13, yz 14..17, centre 18)
-  const Symbol sym = make_symbol(ctx, v19);
+  Symbol sym = make_symbol(ctx, v19);
   const int hx = nx / 2;
-  {  // x forward: real lines -> half spectra (kx < nx/2, slot 0 packs kx = 0, nx/2)
-    const int h = hx, nlines = ny * nz, B = rfft_batch<T>(h, nlines);
-    hipLaunchKernelGGL((k_rfft_x<T, 0>), (unsigned)((nlines + B - 1) / B), kBlock, rfft_lds<T>(h, B), ctx->stream,
-                       ilog2(nx), nlines, B, r, buf, (T *)nullptr, ctx->tw[0], 1.0);
+  // z planes of the half spectrum are spaced P = (nx/2) ny + padding apart, so
+  // that the rows of a z line do not all fall at one power-of-two stride
+  const int64_t P = spec_plane<T>(nx, ny);
+  // 256-point axes use the four-step kernels (16 lines per block); their y
+  // pass leaves the y frequencies in natural order
+  const bool x256 = nx == 256, y256 = ny == 256 && hx % 16 == 0, z256 = nz == 256 && ((int64_t)hx * ny) % 16 == 0;
+  sym.ynat = y256 ? 1 : 0;
+  const int nlines = ny * nz;
+  if (x256) {  // x forward: real lines -> half spectra (kx < nx/2, slot 0 packs kx = 0, nx/2)
+    hipLaunchKernelGGL((k_rfft256_x<T, 0>), (unsigned)((nlines + 31) / 32), 256, 0, ctx->stream, (int64_t)nlines, r,
+                       buf, (T *)nullptr, ctx->tw[0], 1.0, ilog2(ny), P);
+  } else {
+    const int B = rfft_batch<T>(hx, nlines);
+    hipLaunchKernelGGL((k_rfft_x<T, 0>), (unsigned)((nlines + B - 1) / B), kBlock, rfft_lds<T>(hx, B), ctx->stream,
+                       ilog2(nx), nlines, B, r, buf, (T *)nullptr, ctx->tw[0], 1.0, ilog2(ny), P);
   }
-  {  // y forward: lines (kx, iz); A = nx/2 contiguous, c = iz with stride (nx/2)*ny
+  // y forward: lines (kx, iz); A = nx/2 contiguous, c = iz with stride P
+  if (y256) {
+    hipLaunchKernelGGL((k_fft256_s<T, 0>), (unsigned)((hx / 16) * nz), 256, 0, ctx->stream, (int64_t)hx, (int64_t)hx,
+                       P, buf, ctx->tw[1], sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
+  } else {
     const int n = ny, B = fft_batch<T>(n, hx);
     hipLaunchKernelGGL((k_fft_s<T, 0>), (unsigned)((hx / B) * nz), kBlock, fft_lds<T>(n, B), ctx->stream, ilog2(n),
-                       (int64_t)hx, (int64_t)hx, (int64_t)hx * ny, B, buf, ctx->tw[1], sym, ctx->trig[0],
-                       ctx->trig[1], ctx->trig[2]);
+                       (int64_t)hx, (int64_t)hx, P, B, buf, ctx->tw[1], sym, ctx->trig[0], ctx->trig[1],
+                       ctx->trig[2]);
   }
-  {  // z forward + symbol + inverse: lines a = kx + (nx/2)*py, estride (nx/2)*ny
+  {  // z forward + symbol + inverse: lines a = kx + (nx/2)*py, estride P
     const int n = nz;
     const int64_t A = (int64_t)hx * ny;
-    const int B = fft_batch<T>(n, A);
-    hipLaunchKernelGGL((k_fft_s<T, 2>), (unsigned)(A / B), kBlock, fft_lds<T>(n, B), ctx->stream, ilog2(n), A, A,
-                       (int64_t)0, B, buf, ctx->tw[2], sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
+    if (z256) {
+      hipLaunchKernelGGL((k_fft256_s<T, 2>), (unsigned)(A / 16), 256, 0, ctx->stream, A, P, (int64_t)0, buf,
+                         ctx->tw[2], sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
+    } else {
+      const int B = fft_batch_z<T>(n, A);
+      hipLaunchKernelGGL((k_fft_s<T, 2>), (unsigned)(A / B), kBlock, fft_lds_z<T>(n, B), ctx->stream, ilog2(n), A, P,
+                         (int64_t)0, B, buf, ctx->tw[2], sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
+    }
     const size_t lds = ((size_t)(n >> 1) + 2 * (size_t)(n + 1)) * 2 * sizeof(T);
-    hipLaunchKernelGGL((k_fft_col0<T>), (unsigned)ny, kBlock, lds, ctx->stream, ilog2(n), A, hx, buf, ctx->tw[2],
+    hipLaunchKernelGGL((k_fft_col0<T>), (unsigned)ny, kBlock, lds, ctx->stream, ilog2(n), P, hx, buf, ctx->tw[2],
                        sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
   }
-  {  // y inverse
+  if (y256) {  // y inverse
+    hipLaunchKernelGGL((k_fft256_s<T, 1>), (unsigned)((hx / 16) * nz), 256, 0, ctx->stream, (int64_t)hx, (int64_t)hx,
+                       P, buf, ctx->tw[1], sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
+  } else {
     const int n = ny, B = fft_batch<T>(n, hx);
     hipLaunchKernelGGL((k_fft_s<T, 1>), (unsigned)((hx / B) * nz), kBlock, fft_lds<T>(n, B), ctx->stream, ilog2(n),
-                       (int64_t)hx, (int64_t)hx, (int64_t)hx * ny, B, buf, ctx->tw[1], sym, ctx->trig[0],
-                       ctx->trig[1], ctx->trig[2]);
+                       (int64_t)hx, (int64_t)hx, P, B, buf, ctx->tw[1], sym, ctx->trig[0], ctx->trig[1],
+                       ctx->trig[2]);
   }
-  {  // x inverse: half spectra -> real lines
-    const int h = hx, nlines = ny * nz, B = rfft_batch<T>(h, nlines);
-    hipLaunchKernelGGL((k_rfft_x<T, 1>), (unsigned)((nlines + B - 1) / B), kBlock, rfft_lds<T>(h, B), ctx->stream,
-                       ilog2(nx), nlines, B, (const T *)nullptr, buf, z, ctx->tw[0], 1.0);
+  if (x256) {  // x inverse: half spectra -> real lines
+    hipLaunchKernelGGL((k_rfft256_x<T, 1>), (unsigned)((nlines + 31) / 32), 256, 0, ctx->stream, (int64_t)nlines,
+                       (const T *)nullptr, buf, z, ctx->tw[0], 1.0, ilog2(ny), P);
+  } else {
+    const int B = rfft_batch<T>(hx, nlines);
+    hipLaunchKernelGGL((k_rfft_x<T, 1>), (unsigned)((nlines + B - 1) / B), kBlock, rfft_lds<T>(hx, B), ctx->stream,
+                       ilog2(nx), nlines, B, (const T *)nullptr, buf, z, ctx->tw[0], 1.0, ilog2(ny), P);
   }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("fft: ") + hipGetErrorString(e));
@@ -738,16 +1021,40 @@ __global__ __launch_bounds__(256) void k2_gs_pass(int64_t nxy, int j, T *__restr
 
 // Hessenberg columns from the partials, Givens QR and back substitution
 // (one block): y[q], q < kHm (zero past a breakdown).
+// (Every sum is formed as sum_partials forms it, so the values are those the
+// other kernels used.)
+constexpr int kNSums = (kHm + 1) + kHm * (kHm + 1);
 __global__ __launch_bounds__(256) void k2_solve(const double *__restrict__ pn, const double *__restrict__ p1,
                                                 const double *__restrict__ p2, int nb, double *__restrict__ y)
 {
-  __shared__ double n2[kHm + 1], h1[kHm][kHm], h2[kHm][kHm];
-  for (int j = 0; j <= kHm; j++) sum_partials<1>(pn + (int64_t)j * nb, nb, 1, n2 + j);
-  for (int j = 0; j < kHm; j++) {
-    sum_partials<kHm>(p1 + (int64_t)j * kHm * nb, nb, j + 1, h1[j]);
-    sum_partials<kHm>(p2 + (int64_t)j * kHm * nb, nb, j + 1, h2[j]);
+  __shared__ double red[kNSums];
+  double a[kNSums];
+#pragma unroll
+  for (int t = 0; t < kNSums; t++) a[t] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    int t = 0;
+#pragma unroll
+    for (int j = 0; j <= kHm; j++) a[t++] += pn[(int64_t)j * nb + b];
+#pragma unroll
+    for (int j = 0; j < kHm; j++)
+#pragma unroll
+      for (int q = 0; q <= j; q++) {
+        a[t++] += p1[(int64_t)(j * kHm + q) * nb + b];
+        a[t++] += p2[(int64_t)(j * kHm + q) * nb + b];
+      }
   }
+  block_sums<kNSums>(a, kNSums, red);
   if (threadIdx.x != 0) return;
+  double n2[kHm + 1], h1[kHm][kHm], h2[kHm][kHm];
+  {
+    int t = 0;
+    for (int j = 0; j <= kHm; j++) n2[j] = red[t++];
+    for (int j = 0; j < kHm; j++)
+      for (int q = 0; q <= j; q++) {
+        h1[j][q] = red[t++];
+        h2[j][q] = red[t++];
+      }
+  }
   double H[kHm][kHm], g[kHm + 1], cs[kHm], sn[kHm], yy[kHm];
   for (int q = 0; q <= kHm; q++) g[q] = 0.0;
   for (int q = 0; q < kHm; q++) yy[q] = 0.0;

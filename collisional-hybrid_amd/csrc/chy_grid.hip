@@ -234,6 +234,16 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
   double zs[kRows];
 #pragma unroll
   for (int rr = 0; rr < kRows; rr++) zs[rr] = 0.0;
+  // b at this thread's nodes of the current plane, and of the next one in flight
+  TB rb[kRows], rbn[kRows];
+  auto fetch_b = [&](int k, TB(&dst)[kRows]) {
+#pragma unroll
+    for (int rr = 0; rr < kRows; rr++) {
+      const int j = y0 + ty0 + 4 * rr;
+      if (i < g.nx && j < g.ny) dst[rr] = b[(int64_t)i + (int64_t)g.nx * ((int64_t)j + (int64_t)g.ny * k)];
+    }
+  };
+  if (OUT != 0) fetch_b(z0, rb);
   fetch(z0 - 1);
   store((z0 + 2) % 3);
   fetch(z0);
@@ -242,7 +252,10 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
   for (int k = z0; k < z1; k++) {
     store((k + 1) % 3);
     __syncthreads();
-    if (k + 1 < z1) fetch(k + 2);  // in flight while plane k is computed
+    if (k + 1 < z1) {  // in flight while plane k is computed
+      fetch(k + 2);
+      if (OUT != 0) fetch_b(k + 1, rbn);
+    }
     const int pm = (k + 2) % 3, pc = k % 3, pp = (k + 1) % 3;
 #pragma unroll
     for (int rr = 0; rr < kRows; rr++) {
@@ -272,10 +285,14 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
           for (int q = 0; q < 19; q++) acc += v[q] * xv[q];
         }
         const int64_t c = (int64_t)i + (int64_t)g.nx * ((int64_t)j + (int64_t)g.ny * k);
-        if (OUT == 1) y[c] = (TY)((double)b[c] - (double)acc);
-        else if (OUT == 2) zs[rr] += (double)b[c] - (double)acc;
+        if (OUT == 1) y[c] = (TY)((double)rb[rr] - (double)acc);
+        else if (OUT == 2) zs[rr] += (double)rb[rr] - (double)acc;
         else y[c] = (TY)acc;
       }
+    }
+    if (OUT != 0) {
+#pragma unroll
+      for (int rr = 0; rr < kRows; rr++) rb[rr] = rbn[rr];
     }
     __syncthreads();
   }

@@ -261,7 +261,7 @@ int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z);
 // two-level: fft, then kFft2Its steps of flexible GMRES on the z-averaged
 // residual with the z-averaged operator (coefficients from launch_zcoef in
 // pc2_buf); w3 is 3-D scratch
-constexpr int kFft2Its = 4;
+constexpr int kFft2Its = 2;
 int64_t fft2_buf_elems(const chy_ctx *ctx);  // elements of T in pc2_buf
 int64_t fft2_h_elems(const chy_ctx *ctx);    // doubles in pc2_h
 template <typename T>

@@ -154,7 +154,7 @@ void orc_init_particles(const int64_t N[3], const double p0[3], const double d[3
 enum { ORC_PC_NONE = 0, ORC_PC_JACOBI = 1, ORC_PC_FFT = 2, ORC_PC_ZLINE = 3, ORC_PC_CHEBY = 4, ORC_PC_FFT2 = 5 };
 enum { ORC_KSP_GMRES = 0, ORC_KSP_BCGS = 1 };
 #define ORC_CHEBY_DEGREE 4
-#define ORC_FFT2_ITS 4
+#define ORC_FFT2_ITS 2
 /* Apply the preconditioner: z = M^{-1} r (mean removed).
  *   jacobi  z = D^-1 r
  *   fft     z = A(mean n)^+ r by 3-D FFT

@@ -195,6 +195,46 @@ def test_pc_apply_matches_oracle(pc, N, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("N", [(256, 256, 8), (8, 256, 256), (64, 256, 4), (32, 16, 256), (256, 8, 4)])
+def test_fft_pc_with_256_point_axes(N, prec):
+    """256-point axes (four-step transforms), alone and beside the radix-4
+    passes, against the oracle's FFT preconditioner.  At these lengths the
+    rounding of any FFT, amplified by the small eigenvalues of the anisotropic
+    symbol, moves z by up to ~1e-10 of its size in FP64 (the radix-4 kernels
+    at 128 points measure 1e-10 on the same kind of case), hence the
+    tolerances."""
+    rng = np.random.default_rng(N[0] + 3 * N[2])
+    with context(opts(N, extra=f"-pc_type fft -precision {prec} -B0x 1e-5 -B0y -2e-5")) as ctx:
+        g = grid_of(ctx.prm)
+        mom = _moments(g, rng, amp=0.1)
+        ctx.set_moments(vlasov_from(mom))
+        r = rng.standard_normal(g.nc)
+        z = ctx.pc_apply(r)
+        K = list(ctx.prm.kappa_e)
+    n = mom[0].astype(np.float32).astype(np.float64) if prec == "mixed" else mom[0]
+    ref = orc.pc_apply(g, K, n, r, pc=orc.PC_FFT)
+    assert relmax(z - z.mean(), ref) <= (1e-8 if prec == "fp64" else 5e-6), relmax(z - z.mean(), ref)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_fft2_pc_with_256_point_axes(prec):
+    """The two-level preconditioner on a grid whose x and y passes take the
+    four-step kernels, against the oracle."""
+    rng = np.random.default_rng(41)
+    N = (256, 256, 4)
+    with context(opts(N, extra=f"-pc_type fft2 -precision {prec} -B0x 1e-5 -B0y -2e-5")) as ctx:
+        g = grid_of(ctx.prm)
+        mom = _moments(g, rng, amp=0.1)
+        ctx.set_moments(vlasov_from(mom))
+        r = rng.standard_normal(g.nc)
+        z = ctx.pc_apply(r)
+        K = list(ctx.prm.kappa_e)
+    n = mom[0].astype(np.float32).astype(np.float64) if prec == "mixed" else mom[0]
+    ref = orc.pc_apply(g, K, n, r, pc=orc.PC_FFT2)
+    assert relmax(z - z.mean(), ref) <= (1e-8 if prec == "fp64" else 2e-5), relmax(z - z.mean(), ref)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("pc", ["zline", "cheby"])
 def test_zline_and_chebyshev_preconditioned_solves(pc, prec):
     """GMRES with the z-line or Chebyshev-Jacobi preconditioner converges to

@@ -171,15 +171,38 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
       launch_lhs_apply<T, T, T>(ctx, lt, ctx->solve_transpose, n, Zj, Vn);  // w built in place in slot j+1
       timer_stop(ctx, "lhs_apply");
       timer_start(ctx, "krylov");
-      // two-pass classical Gram-Schmidt (the preconditioned operator is close
-      // to the identity, so one pass cancels heavily); the norm of the result
-      // is fused into the second update and everything is read back at once
+      // classical Gram-Schmidt; the norm of the result is fused into the last
+      // update and everything is read back at once.  REFINE_ALWAYS makes a
+      // second pass (the preconditioned operator is close to the identity, so
+      // one pass can cancel heavily); REFINE_IFNEEDED makes it when the first
+      // pass removed more than half of the squared norm (sum h^2 > |w'|^2,
+      // with |w|^2 = |w'|^2 + sum h^2)
+      const int refine = ctx->prm.ksp_gmres_cgs_refinement;
+      double wn;
       launch_gs_dots<T>(ctx, (int)(j + 1), V, nc, Vn, nc, hdev);
-      launch_gs_update_dots<T>(ctx, (int)(j + 1), V, nc, hdev, Vn, nc, h2dev);
-      launch_gs_update_norm<T>(ctx, (int)(j + 1), V, nc, h2dev, Vn, nc, ndev);
-      TRY(fetch(ctx, ctx->red_out, (int)(2 * (m + 2) + 1), ctx->h_red));
-      for (int64_t q = 0; q <= j; q++) h[q] = ctx->h_red[q] + ctx->h_red[(m + 2) + q];
-      const double wn = sqrt(ctx->h_red[2 * (m + 2)]);
+      if (refine == CHY_CGS_REFINE_ALWAYS) {
+        launch_gs_update_dots<T>(ctx, (int)(j + 1), V, nc, hdev, Vn, nc, h2dev);
+        launch_gs_update_norm<T>(ctx, (int)(j + 1), V, nc, h2dev, Vn, nc, ndev);
+        TRY(fetch(ctx, ctx->red_out, (int)(2 * (m + 2) + 1), ctx->h_red));
+        for (int64_t q = 0; q <= j; q++) h[q] = ctx->h_red[q] + ctx->h_red[(m + 2) + q];
+        wn = sqrt(ctx->h_red[2 * (m + 2)]);
+      } else {
+        launch_gs_update_norm<T>(ctx, (int)(j + 1), V, nc, hdev, Vn, nc, ndev);
+        TRY(fetch(ctx, ctx->red_out, (int)(2 * (m + 2) + 1), ctx->h_red));
+        double proj = 0.0;
+        for (int64_t q = 0; q <= j; q++) {
+          h[q] = ctx->h_red[q];
+          proj += h[q] * h[q];
+        }
+        wn = sqrt(ctx->h_red[2 * (m + 2)]);
+        if (refine == CHY_CGS_REFINE_IFNEEDED && proj > wn * wn) {
+          launch_gs_dots<T>(ctx, (int)(j + 1), V, nc, Vn, nc, h2dev);
+          launch_gs_update_norm<T>(ctx, (int)(j + 1), V, nc, h2dev, Vn, nc, ndev);
+          TRY(fetch(ctx, ctx->red_out, (int)(2 * (m + 2) + 1), ctx->h_red));
+          for (int64_t q = 0; q <= j; q++) h[q] += ctx->h_red[(m + 2) + q];
+          wn = sqrt(ctx->h_red[2 * (m + 2)]);
+        }
+      }
       timer_stop(ctx, "krylov");
       h[j + 1] = wn;
       if (!std::isfinite(h[j + 1])) return set_err(ctx, CHY_ERR_FP, "non-finite Arnoldi vector");

@@ -167,6 +167,8 @@ static const char *const DensityNames[] = {"flat-sobol", "flat-reverse", "flat-n
 static const char *const BCNames[] = {"periodic", "injection", "reflection", "dirichlet", "neumann", NULL};
 static const char *const PertNames[] = {"none", "sinusoidal-x", "uniform", "gaussian", NULL};
 static const char *const PrecNames[] = {"fp64", "mixed", NULL};
+/* PETSc KSPGMRESCGSRefinementType names */
+static const char *const CgsRefineNames[] = {"REFINE_NEVER", "REFINE_IFNEEDED", "REFINE_ALWAYS", NULL};
 
 int chy_params_default(chy_params *p)
 {
@@ -200,6 +202,7 @@ int chy_params_default(chy_params *p)
   p->ksp_restart = 30;
   p->pc_cheby_degree = 4;
   p->ksp_inner_rtol = 2e-4;
+  p->ksp_gmres_cgs_refinement = CHY_CGS_REFINE_NEVER;  /* PETSc default */
   p->precision = CHY_PREC_FP64;
   p->output_interval = 1;
   p->perturbation = CHY_PERT_NONE;
@@ -385,6 +388,7 @@ int chy_params_from_options(int argc, const char *const *argv, chy_params *p, ch
   get_int(&db, "-ksp_max_it", &p->ksp_max_it, &e);
   get_int(&db, "-ksp_gmres_restart", &p->ksp_restart, &e);
   get_real(&db, "-ksp_inner_rtol", &p->ksp_inner_rtol, &e);
+  get_enum(&db, "-ksp_gmres_cgs_refinement_type", CgsRefineNames, &p->ksp_gmres_cgs_refinement, &e);
   {
     int64_t deg = p->pc_cheby_degree;
     if (get_int(&db, "-pc_cheby_degree", &deg, &e)) {

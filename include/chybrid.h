@@ -50,6 +50,9 @@ enum { CHY_BC_PERIODIC = 0, CHY_BC_INJECTION, CHY_BC_REFLECTION, CHY_BC_DIRICHLE
 enum { CHY_PREC_FP64 = 0, CHY_PREC_MIXED = 1 };
 enum { CHY_KSP_GMRES = 0, CHY_KSP_BCGS = 1 };
 enum { CHY_PC_NONE = 0, CHY_PC_JACOBI = 1, CHY_PC_FFT = 2, CHY_PC_ZLINE = 3, CHY_PC_CHEBY = 4, CHY_PC_FFT2 = 5 };
+/* -ksp_gmres_cgs_refinement_type: REFINE_NEVER, REFINE_IFNEEDED (when the
+ * first pass removes more than half of the vector's squared norm), REFINE_ALWAYS */
+enum { CHY_CGS_REFINE_NEVER = 0, CHY_CGS_REFINE_IFNEEDED = 1, CHY_CGS_REFINE_ALWAYS = 2 };
 enum { CHY_PERT_NONE = 0, CHY_PERT_SINUSOIDAL_X = 1, CHY_PERT_UNIFORM = 2, CHY_PERT_GAUSSIAN = 3 };
 
 /* ---- run parameters: the scalars of the reference Context (hybrid.h:106-138)
@@ -87,6 +90,7 @@ typedef struct {
   int32_t ksp_monitor, ksp_converged_reason, ksp_error_if_not_converged;
   int32_t pc_cheby_degree; /* Chebyshev-Jacobi preconditioner: polynomial degree */
   double  ksp_inner_rtol;  /* mixed precision: floor of the FP32 inner relative tolerance */
+  int32_t ksp_gmres_cgs_refinement; /* CHY_CGS_REFINE_*: second Gram-Schmidt pass (PETSc KSPGMRESCGSRefinementType) */
   /* build additions: seed, particles per cell, perturbation, precision, output cadence */
   uint64_t seed;
   int64_t ppc;

@@ -110,6 +110,14 @@ def test_solver_option_mapping():
     assert p.pc_cheby_degree == 7 and p.ksp_inner_rtol == 1e-5
     p, _ = chybrid.params_from_options([])
     assert p.pc_type == chybrid.PC_FFT2 and p.pc_cheby_degree == 4 and p.ksp_inner_rtol == 2e-4
+    # classical Gram-Schmidt without refinement unless asked (PETSc's default)
+    assert p.ksp_gmres_cgs_refinement == chybrid.CGS_REFINE_NEVER
+    for name, v in (("REFINE_NEVER", chybrid.CGS_REFINE_NEVER), ("refine_ifneeded", chybrid.CGS_REFINE_IFNEEDED),
+                    ("REFINE_ALWAYS", chybrid.CGS_REFINE_ALWAYS)):
+        p, _ = chybrid.params_from_options(["-ksp_gmres_cgs_refinement_type", name])
+        assert p.ksp_gmres_cgs_refinement == v, name
+    with pytest.raises(chybrid.ChyError):
+        chybrid.params_from_options("-ksp_gmres_cgs_refinement_type sometimes".split())
     with pytest.raises(chybrid.ChyError):
         chybrid.params_from_options("-pc_cheby_degree 0".split())
 

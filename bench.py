@@ -34,11 +34,11 @@ sys.path.insert(0, os.path.join(ROOT, "collisional-hybrid_amd"))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 METRIC = "particle-steps/s + \u03c6-solve ms/step at 256\u00b3, mixed vs FP64 GPU; % HBM peak"
 # timers that bracket exactly one kernel launch (the others cover several)
-SINGLE_KERNEL = ("efield", "push", "collide", "sort_scatter", "sort_reorder", "deposit_cells", "deposit_nodes",
+SINGLE_KERNEL = ("efield", "push", "sort_scatter", "sort_reorder", "deposit_cells", "deposit_nodes",
                  "lhs_apply")
 COPY_BYTES = 1 << 30
 # kernels launched once per step: timed with HIP events during the timed run
-ONCE_PER_STEP = ("efield", "push", "collide", "sort_scatter", "sort_reorder", "deposit_cells", "deposit_nodes")
+ONCE_PER_STEP = ("efield", "push", "sort_scatter", "sort_reorder", "deposit_cells", "deposit_nodes")
 BREAKDOWN_STEPS = 10
 
 PHYS = ("-mi 4.9816e-26 -mn 4.8156e-26 -Ti 300 -Tn 300 -B0z 5e-5 -E0x 0.05 -nui 1e3 -nue 1e5 -dt 1e-5 "
@@ -134,7 +134,7 @@ def run_gpu(N, ppc, prec, steps, warmup, keep=False, cross=None):
         roof_kernel = {"name": dom, "avg_ms": once[dom]["avg_ms"], "launches": once[dom]["launches"]}
         # per-kernel breakdown: every timer on, over BREAKDOWN_STEPS further
         # steps (the event records add launch overhead, so not the headline)
-        names = ("efield", "push", "collide", "sort_scan", "sort_scatter", "sort_reorder", "deposit_cells",
+        names = ("efield", "push", "sort_scan", "sort_scatter", "sort_reorder", "deposit_cells",
                  "deposit_nodes", "rhs", "pc", "lhs_apply", "krylov")
         ctx.timing_select(None)
         ctx.timing_reset()

@@ -72,7 +72,6 @@ struct chy_ctx {
   uint32_t *cursor = nullptr;   // nc
   uint32_t *scan_tmp = nullptr; // block sums for the scan
   uint32_t *maxcount = nullptr; // largest cell occupancy of the last sort
-  uint32_t *qlen = nullptr;     // colliders queued by each wave of the push (kMaxPushWaves)
   void *partials = nullptr;     // per-cell corner partials [32][nc] (T)
   double *sred = nullptr;       // collision-normaliser block partials
   double *S_dev = nullptr;      // collision normaliser (device scalar)
@@ -111,7 +110,6 @@ struct chy_ctx {
 namespace chy {
 
 constexpr int kRedBlocks = 1024;  // fixed reduction grid: deterministic sums
-constexpr int kMaxPushWaves = 4096 * 4;  // particle kernels: at most 4096 blocks of 4 waves
 constexpr int kBlock = 256;
 
 inline int set_err(chy_ctx *c, int code, const std::string &msg)

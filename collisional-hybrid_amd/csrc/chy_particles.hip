@@ -40,50 +40,49 @@ __device__ __forceinline__ void count_flags(unsigned long long *cs, int slot, bo
   if (m && lane_id() == 0) atomicAdd(&cs[slot], (unsigned long long)__popcll(m));
 }
 
-// Collision queue: every wave of the push owns a segment of the queue whose
-// length is the number of live ions it visits (so the segments tile [0, np)
-// in wave order), and appends its colliders there with a ballot -- no atomics.
-// An entry holds the ion's index (qidx) and its state after the Boris step
-// (in the other particle buffer, free until the sort), so the collider
-// kernel reads the queue coalesced.  Wave w of a grid with `stride` threads
-// visits 64 * full + clamp(rem - 64 w, 0, 64) live ions (full = np / stride,
-// rem = np % stride): its segment starts at 64 w full + min(rem, 64 w).
-__device__ __forceinline__ int64_t queue_segment(int64_t w, int64_t full, int64_t rem)
+// Accepted / refused scatterings of the block, one global atomic per block.
+__device__ __forceinline__ void flush_scatter_counts(unsigned long long *cs, uint32_t nacc, uint32_t nref,
+                                                     unsigned int *sacc)
 {
-  return 64 * w * full + (rem < 64 * w ? rem : 64 * w);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    nacc += __shfl_down(nacc, off, 64);
+    nref += __shfl_down(nref, off, 64);
+  }
+  if (lane_id() == 0) {
+    if (nacc) atomicAdd(&sacc[0], nacc);
+    if (nref) atomicAdd(&sacc[1], nref);
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 && sacc[threadIdx.x]) atomicAdd(&cs[threadIdx.x], (unsigned long long)sacc[threadIdx.x]);
 }
 
-__device__ __forceinline__ uint32_t lane_prefix(uint64_t m)
-{
-  return (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
-}
-
-// ---- main push: gather + Boris + collision decision + move ----------------------
-// Ions that collide are queued; their scattering and move run in k_collide.
+// ---- main push: gather + Boris + collision (decision and scattering) + move -----------
+// One neutral draw per ion serves both the decision and, for the ions that
+// collide, the scattering, which runs in place: every ion's new state is
+// written by the same store instructions as its neighbours' (no scattered
+// write-back of the colliders).
 template <bool COLLIDE, bool MOVE>
 __global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t np, PState<double> st,
                                                    const Vec4<double> *__restrict__ E4, const double *__restrict__ Sdev,
-                                                   uint32_t *__restrict__ counts, PState<double> q,
-                                                   uint32_t *__restrict__ qidx, uint32_t *__restrict__ qcount,
-                                                   unsigned long long *__restrict__ cstats)
+                                                   uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats)
 {
+  __shared__ unsigned int sacc[2];
+  if (threadIdx.x < 2) sacc[threadIdx.x] = 0u;
+  __syncthreads();
   const double S = COLLIDE ? Sdev[0] : 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   // every lane iterates the same number of times so wave-level collectives stay convergent
   const int64_t iters = (np + stride - 1) / stride;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t seg = queue_segment(wave, np / stride, np % stride);
-  uint32_t wn = 0;
+  uint32_t nacc = 0, nref = 0;
   for (int64_t it = 0; it < iters; it++) {
     const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = p < np;
     int key = -1, capped = 0;
-    bool coll = false, bad = false;
-    double x = 0, y = 0, z = 0, vx = 0, vy = 0, vz = 0;
-    uint32_t pid = 0;
+    bool bad = false;
     if (live) {
-      x = st.x[p]; y = st.y[p]; z = st.z[p];
-      vx = st.vx[p]; vy = st.vy[p]; vz = st.vz[p];
+      double x = st.x[p], y = st.y[p], z = st.z[p];
+      double vx = st.vx[p], vy = st.vy[p], vz = st.vz[p];
       int i, j, k;
       double fx, fy, fz, e0, e1, e2;
       cell_coord(x, g.p0[0], g.d[0], g.nx, i, fx);
@@ -93,183 +92,79 @@ __global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t
                        e1, e2);
       boris<double>(pc, e0, e1, e2, vx, vy, vz);
       if (COLLIDE) {
-        pid = st.pid[p];
-        coll = collide_decide<double>(pc, S, pid, vx, vy, vz, capped);
-      }
-      if (!coll) {
-        st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
-        if (MOVE) {
-          key = move_fp64(g, pc.dt, x, y, z, vx, vy, vz);
-          st.x[p] = x; st.y[p] = y; st.z[p] = z;
-          st.cell[p] = key;
+        const uint32_t pid = st.pid[p];
+        double vnx, vny, vnz, ud;
+        neutral_draw<double>(pc, pid, vnx, vny, vnz, ud);
+        if (collide_decide_n<double>(pc, S, vnx, vny, vnz, ud, vx, vy, vz, capped)) {
+          if (scatter_n<double>(pc, pid, vnx, vny, vnz, vx, vy, vz) == 1) nacc++;
+          else nref++;
         }
       }
-      bad = !isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(x) || !isfinite(y) || !isfinite(z);
-    }
-    if (COLLIDE) {
-      const uint64_t m = __ballot(coll);
-      if (coll) {
-        const int64_t e = seg + wn + lane_prefix(m);
-        qidx[e] = (uint32_t)p;
-        q.x[e] = x; q.y[e] = y; q.z[e] = z;
-        q.vx[e] = vx; q.vy[e] = vy; q.vz[e] = vz;
-        q.pid[e] = pid;
+      st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
+      if (MOVE) {
+        key = move_fp64(g, pc.dt, x, y, z, vx, vy, vz);
+        st.x[p] = x; st.y[p] = y; st.z[p] = z;
+        st.cell[p] = key;
       }
-      wn += (uint32_t)__popcll(m);
+      bad = !isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(x) || !isfinite(y) || !isfinite(z);
     }
     if (MOVE) count_key(counts, key);
     if (COLLIDE) count_flags(cstats, 2, capped != 0);
     count_flags(cstats, 3, bad);
   }
-  if (COLLIDE && lane_id() == 0) qcount[wave] = wn;
+  if (COLLIDE) flush_scatter_counts(cstats, nacc, nref, sacc);
 }
 
 template <bool COLLIDE, bool MOVE>
 __global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConst pc, int64_t np, PState<float> st,
                                                     const Vec4<float> *__restrict__ E4, const double *__restrict__ Sdev,
-                                                    uint32_t *__restrict__ counts, PState<float> q,
-                                                    uint32_t *__restrict__ qidx, uint32_t *__restrict__ qcount,
-                                                    unsigned long long *__restrict__ cstats)
+                                                    uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats)
 {
+  __shared__ unsigned int sacc[2];
+  if (threadIdx.x < 2) sacc[threadIdx.x] = 0u;
+  __syncthreads();
   const float S = COLLIDE ? (float)Sdev[0] : 0.f;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t iters = (np + stride - 1) / stride;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t seg = queue_segment(wave, np / stride, np % stride);
-  uint32_t wn = 0;
+  uint32_t nacc = 0, nref = 0;
   for (int64_t it = 0; it < iters; it++) {
     const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = p < np;
-    int key = -1, capped = 0, c = 0;
-    bool coll = false, bad = false;
-    float fx = 0, fy = 0, fz = 0, vx = 0, vy = 0, vz = 0;
-    uint32_t pid = 0;
+    int key = -1, capped = 0;
+    bool bad = false;
     if (live) {
-      c = st.cell[p];
+      const int c = st.cell[p];
       int i, j, k;
       split_cell(g, c, i, j, k);
-      fx = st.x[p]; fy = st.y[p]; fz = st.z[p];
-      vx = st.vx[p]; vy = st.vy[p]; vz = st.vz[p];
+      const float fx = st.x[p], fy = st.y[p], fz = st.z[p];
+      float vx = st.vx[p], vy = st.vy[p], vz = st.vz[p];
       float e0, e1, e2;
       gather_c<float>(g, E4, c, i, j, k, fx, fy, fz, e0, e1, e2);
       boris<float>(pc, e0, e1, e2, vx, vy, vz);
       if (COLLIDE) {
-        pid = st.pid[p];
-        coll = collide_decide<float>(pc, S, pid, vx, vy, vz, capped);
-      }
-      if (!coll) {
-        st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
-        if (MOVE) {
-          float ox = fx, oy = fy, oz = fz;
-          key = move_mixed_ijk(g, pc, i, j, k, ox, oy, oz, vx, vy, vz);
-          st.x[p] = ox; st.y[p] = oy; st.z[p] = oz;
-          st.cell[p] = key;
-          bad = !isfinite(ox) || !isfinite(oy) || !isfinite(oz);
+        const uint32_t pid = st.pid[p];
+        float vnx, vny, vnz, ud;
+        neutral_draw<float>(pc, pid, vnx, vny, vnz, ud);
+        if (collide_decide_n<float>(pc, S, vnx, vny, vnz, ud, vx, vy, vz, capped)) {
+          if (scatter_n<float>(pc, pid, vnx, vny, vnz, vx, vy, vz) == 1) nacc++;
+          else nref++;
         }
       }
-      bad = bad || !isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(fx) || !isfinite(fy) || !isfinite(fz);
-    }
-    if (COLLIDE) {
-      const uint64_t m = __ballot(coll);
-      if (coll) {
-        const int64_t e = seg + wn + lane_prefix(m);
-        qidx[e] = (uint32_t)p;
-        q.x[e] = fx; q.y[e] = fy; q.z[e] = fz;
-        q.vx[e] = vx; q.vy[e] = vy; q.vz[e] = vz;
-        q.pid[e] = pid;
-        q.cell[e] = c;
+      st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
+      if (MOVE) {
+        float ox = fx, oy = fy, oz = fz;
+        key = move_mixed_ijk(g, pc, i, j, k, ox, oy, oz, vx, vy, vz);
+        st.x[p] = ox; st.y[p] = oy; st.z[p] = oz;
+        st.cell[p] = key;
+        bad = !isfinite(ox) || !isfinite(oy) || !isfinite(oz);
       }
-      wn += (uint32_t)__popcll(m);
+      bad = bad || !isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(fx) || !isfinite(fy) || !isfinite(fz);
     }
     if (MOVE) count_key(counts, key);
     if (COLLIDE) count_flags(cstats, 2, capped != 0);
     count_flags(cstats, 3, bad);
   }
-  if (COLLIDE && lane_id() == 0) qcount[wave] = wn;
-}
-
-// Scattering, move and key of the queued ions: wave v of this grid takes the
-// queue segments of push waves v, v + W, ... (W waves in this grid).
-template <bool MOVE>
-__global__ __launch_bounds__(256) void k_collide_fp64(Grid g, PushConst pc, int64_t np, int64_t pstride,
-                                                      int64_t nwaves, PState<double> st, PState<double> q,
-                                                      const uint32_t *__restrict__ qidx,
-                                                      const uint32_t *__restrict__ qcount, uint32_t *__restrict__ counts,
-                                                      unsigned long long *__restrict__ cstats)
-{
-  __shared__ unsigned int sacc[4];
-  if (threadIdx.x < 4) sacc[threadIdx.x] = 0u;
-  __syncthreads();
-  unsigned int nacc = 0, nref = 0, nbad = 0;
-  const int64_t W = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nwaves; w += W) {
-    const int64_t seg = queue_segment(w, np / pstride, np % pstride);
-    const uint32_t n = qcount[w];
-    for (uint32_t e = (uint32_t)lane_id(); e < n; e += 64) {
-      const int64_t s = seg + e;
-      const uint32_t p = qidx[s];
-      double vx = q.vx[s], vy = q.vy[s], vz = q.vz[s];
-      const int r = scatter<double>(pc, q.pid[s], vx, vy, vz);
-      st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
-      if (r == 1) nacc++; else nref++;
-      if (MOVE) {
-        double x = q.x[s], y = q.y[s], z = q.z[s];
-        const int key = move_fp64(g, pc.dt, x, y, z, vx, vy, vz);
-        st.x[p] = x; st.y[p] = y; st.z[p] = z;
-        st.cell[p] = key;
-        atomicAdd(&counts[key], 1u);
-        if (!isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(x) || !isfinite(y) || !isfinite(z))
-          nbad++;
-      }
-    }
-  }
-  if (nacc) atomicAdd(&sacc[0], nacc);
-  if (nref) atomicAdd(&sacc[1], nref);
-  if (nbad) atomicAdd(&sacc[3], nbad);
-  __syncthreads();
-  if (threadIdx.x < 4 && sacc[threadIdx.x]) atomicAdd(&cstats[threadIdx.x], (unsigned long long)sacc[threadIdx.x]);
-}
-
-template <bool MOVE>
-__global__ __launch_bounds__(256) void k_collide_mixed(Grid g, PushConst pc, int64_t np, int64_t pstride,
-                                                       int64_t nwaves, PState<float> st, PState<float> q,
-                                                       const uint32_t *__restrict__ qidx,
-                                                       const uint32_t *__restrict__ qcount, uint32_t *__restrict__ counts,
-                                                       unsigned long long *__restrict__ cstats)
-{
-  __shared__ unsigned int sacc[4];
-  if (threadIdx.x < 4) sacc[threadIdx.x] = 0u;
-  __syncthreads();
-  unsigned int nacc = 0, nref = 0, nbad = 0;
-  const int64_t W = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nwaves; w += W) {
-    const int64_t seg = queue_segment(w, np / pstride, np % pstride);
-    const uint32_t n = qcount[w];
-    for (uint32_t e = (uint32_t)lane_id(); e < n; e += 64) {
-      const int64_t s = seg + e;
-      const uint32_t p = qidx[s];
-      float vx = q.vx[s], vy = q.vy[s], vz = q.vz[s];
-      const int r = scatter<float>(pc, q.pid[s], vx, vy, vz);
-      st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
-      if (r == 1) nacc++; else nref++;
-      if (MOVE) {
-        float fx = q.x[s], fy = q.y[s], fz = q.z[s];
-        int i, j, k;
-        split_cell(g, q.cell[s], i, j, k);
-        const int key = move_mixed_ijk(g, pc, i, j, k, fx, fy, fz, vx, vy, vz);
-        st.x[p] = fx; st.y[p] = fy; st.z[p] = fz;
-        st.cell[p] = key;
-        atomicAdd(&counts[key], 1u);
-        if (!isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(fx) || !isfinite(fy) || !isfinite(fz))
-          nbad++;
-      }
-    }
-  }
-  if (nacc) atomicAdd(&sacc[0], nacc);
-  if (nref) atomicAdd(&sacc[1], nref);
-  if (nbad) atomicAdd(&sacc[3], nbad);
-  __syncthreads();
-  if (threadIdx.x < 4 && sacc[threadIdx.x]) atomicAdd(&cstats[threadIdx.x], (unsigned long long)sacc[threadIdx.x]);
+  if (COLLIDE) flush_scatter_counts(cstats, nacc, nref, sacc);
 }
 
 // Move only (positions from the current velocities), with keys and counts.
@@ -977,45 +872,25 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
   launch_fill<unsigned long long>(ctx, 4, ctx->cstats, 0ull);
   if (move) launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
   const unsigned nb = grid_for(ctx->np);
-  const int64_t pstride = (int64_t)nb * kBlock, nwaves = pstride / 64;
-  // the collision queue: indices in the sort's permutation buffer and states in
-  // the other particle buffer, both free until the sort that follows
-  uint32_t *qidx = ctx->perm;
   timer_start(ctx, "push");
-#define LAUNCH(KER, T, C, M, q)                                                                               \
+#define LAUNCH(KER, T, C, M)                                                                                  \
   hipLaunchKernelGGL((KER<C, M>), nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, st,                       \
-                     (const Vec4<T> *)ctx->efield[0], ctx->S_dev, ctx->counts, q, qidx, ctx->qlen, ctx->cstats)
+                     (const Vec4<T> *)ctx->efield[0], ctx->S_dev, ctx->counts, ctx->cstats)
   if (ctx->mixed) {
-    PState<float> st = ctx->pf[ctx->cur], q = ctx->pf[1 - ctx->cur];
-    if (collide && move) LAUNCH(k_push_mixed, float, true, true, q);
-    else if (collide) LAUNCH(k_push_mixed, float, true, false, q);
-    else if (move) LAUNCH(k_push_mixed, float, false, true, q);
-    else LAUNCH(k_push_mixed, float, false, false, q);
+    PState<float> st = ctx->pf[ctx->cur];
+    if (collide && move) LAUNCH(k_push_mixed, float, true, true);
+    else if (collide) LAUNCH(k_push_mixed, float, true, false);
+    else if (move) LAUNCH(k_push_mixed, float, false, true);
+    else LAUNCH(k_push_mixed, float, false, false);
   } else {
-    PState<double> st = ctx->pd[ctx->cur], q = ctx->pd[1 - ctx->cur];
-    if (collide && move) LAUNCH(k_push_fp64, double, true, true, q);
-    else if (collide) LAUNCH(k_push_fp64, double, true, false, q);
-    else if (move) LAUNCH(k_push_fp64, double, false, true, q);
-    else LAUNCH(k_push_fp64, double, false, false, q);
+    PState<double> st = ctx->pd[ctx->cur];
+    if (collide && move) LAUNCH(k_push_fp64, double, true, true);
+    else if (collide) LAUNCH(k_push_fp64, double, true, false);
+    else if (move) LAUNCH(k_push_fp64, double, false, true);
+    else LAUNCH(k_push_fp64, double, false, false);
   }
 #undef LAUNCH
   timer_stop(ctx, "push");
-  if (collide) {
-    timer_start(ctx, "collide");
-    const unsigned nq = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (nwaves + 3) / 4));
-#define COLL(KER, M, st, q)                                                                                 \
-  hipLaunchKernelGGL((KER<M>), nq, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, pstride, nwaves, st, q, qidx, \
-                     ctx->qlen, ctx->counts, ctx->cstats)
-    if (ctx->mixed) {
-      if (move) COLL(k_collide_mixed, true, ctx->pf[ctx->cur], ctx->pf[1 - ctx->cur]);
-      else COLL(k_collide_mixed, false, ctx->pf[ctx->cur], ctx->pf[1 - ctx->cur]);
-    } else {
-      if (move) COLL(k_collide_fp64, true, ctx->pd[ctx->cur], ctx->pd[1 - ctx->cur]);
-      else COLL(k_collide_fp64, false, ctx->pd[ctx->cur], ctx->pd[1 - ctx->cur]);
-    }
-#undef COLL
-    timer_stop(ctx, "collide");
-  }
   unsigned long long cs[4];
   CHY_HIP(hipMemcpyAsync(cs, ctx->cstats, sizeof(cs), hipMemcpyDeviceToHost, ctx->stream));
   CHY_HIP(hipStreamSynchronize(ctx->stream));

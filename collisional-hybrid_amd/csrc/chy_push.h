@@ -77,6 +77,19 @@ __device__ __forceinline__ void neutral_draw(const PushConst &pc, uint32_t pid, 
   vnz = (T)pc.vn0[2] + vnT * (r2 * c2);
 }
 
+// Collision decision from a neutral draw (vn, ud) already made.
+template <typename T>
+__device__ __forceinline__ bool collide_decide_n(const PushConst &pc, T S, T vnx, T vny, T vnz, T ud, T vx, T vy,
+                                                 T vz, int &capped)
+{
+  const T vrx = vx - vnx, vry = vy - vny, vrz = vz - vnz;
+  const T vrr = sqrt(vrx * vrx + vry * vry + vrz * vrz);
+  T p = (S > (T)0) ? ((T)pc.Nc * vrr) / S : (T)0;
+  capped = 0;
+  if (p > (T)1) { p = (T)1; capped = 1; }
+  return ud < p;
+}
+
 template <typename T>
 __device__ __forceinline__ bool collide_decide(const PushConst &pc, T S, uint32_t pid, T vx, T vy, T vz, int &capped)
 {
@@ -93,10 +106,20 @@ __device__ __forceinline__ bool collide_decide(const PushConst &pc, T S, uint32_
 // Elastic scattering in the centre-of-mass frame, rotated into the frame of
 // the relative velocity (particles.c:552-611).  Returns 1 accepted, 2 refused.
 template <typename T>
+__device__ __forceinline__ int scatter_n(const PushConst &pc, uint32_t pid, T vnx, T vny, T vnz, T &vx, T &vy, T &vz);
+
+template <typename T>
 __device__ __forceinline__ int scatter(const PushConst &pc, uint32_t pid, T &vx, T &vy, T &vz)
 {
   T vnx, vny, vnz, ud;
   neutral_draw<T>(pc, pid, vnx, vny, vnz, ud);
+  return scatter_n<T>(pc, pid, vnx, vny, vnz, vx, vy, vz);
+}
+
+// scatter with the neutral draw (vn) already made
+template <typename T>
+__device__ __forceinline__ int scatter_n(const PushConst &pc, uint32_t pid, T vnx, T vny, T vnz, T &vx, T &vy, T &vz)
+{
   const T vrx = vx - vnx, vry = vy - vny, vrz = vz - vnz;
   const T vrr = sqrt(vrx * vrx + vry * vry + vrz * vrz);
   const U4 b = philox4x32_10(U4{pid, pc.step, STREAM_COLLIDE_B, 0u}, pc.k0, pc.k1);

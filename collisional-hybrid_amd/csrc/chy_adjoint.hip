@@ -242,6 +242,7 @@ struct Stage {
   double *phi = nullptr;     // potential the next push uses
   double *S = nullptr;       // collision normaliser of st (device scalar)
   int64_t step = 0;          // collision stream counter
+  bool ordered = false;      // st in the stable order of offs (the sort's neighbour-move path)
 };
 
 struct Tape {
@@ -299,6 +300,7 @@ int save_stage(chy_ctx *ctx, Stage &s)
   CHY_HIP(hipMemcpyAsync(s.phi, ctx->phi, sizeof(double) * nc, hipMemcpyDeviceToDevice, ctx->stream));
   CHY_HIP(hipMemcpyAsync(s.S, ctx->S_dev, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
   s.step = ctx->step;
+  s.ordered = ctx->order_valid;
   return CHY_OK;
 }
 
@@ -311,6 +313,7 @@ int restore_stage(chy_ctx *ctx, const Stage &s)
   CHY_HIP(hipMemcpyAsync(ctx->phi, s.phi, sizeof(double) * nc, hipMemcpyDeviceToDevice, ctx->stream));
   CHY_HIP(hipMemcpyAsync(ctx->S_dev, s.S, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
   ctx->step = s.step;
+  ctx->order_valid = s.ordered;
   ctx->phi_valid = true;
   ctx->moments_valid = false;
   return CHY_OK;

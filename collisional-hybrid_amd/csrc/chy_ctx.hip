@@ -481,6 +481,11 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   A(dalloc(ctx, &ctx->cursor, nc + 1));
   A(dalloc(ctx, &ctx->scan_tmp, nc / 4096 + 2));
   A(dalloc(ctx, &ctx->maxcount, 1));
+  A(dalloc(ctx, &ctx->trans, 27 * nc));
+  A(dalloc(ctx, &ctx->info, cap));
+  A(dalloc(ctx, &ctx->nbase, 27 * nc));
+  A(dalloc(ctx, &ctx->offs_alt, nc + 1));
+  A(dalloc(ctx, &ctx->sortflag, 2));
   {
     unsigned char *pp = nullptr;
     A(dalloc(ctx, &pp, (int64_t)(32 * ts) * nc));
@@ -531,6 +536,7 @@ int chy_destroy(chy_ctx *ctx)
   adjoint_release(ctx);
   for (int s = 0; s < 2; s++) { free_state(ctx->pd[s]); free_state(ctx->pf[s]); }
   void *ps[] = {ctx->perm, ctx->slotkey, ctx->counts, ctx->offsets, ctx->cursor, ctx->scan_tmp, ctx->maxcount,
+                ctx->trans, ctx->info, ctx->nbase, ctx->offs_alt, ctx->sortflag,
                 ctx->partials,
                 ctx->sred, ctx->S_dev, ctx->cstats, ctx->mom[0], ctx->mom[1], ctx->mom[2], ctx->mom[3],
                 ctx->efield[0], ctx->phi, ctx->b, ctx->r64, ctx->t64, ctx->V,

@@ -72,6 +72,14 @@ struct chy_ctx {
   uint32_t *cursor = nullptr;   // nc
   uint32_t *scan_tmp = nullptr; // block sums for the scan
   uint32_t *maxcount = nullptr; // largest cell occupancy of the last sort
+  // neighbour-move sort (the particles are in the stable order of `offsets`
+  // and every ion moved at most one cell in each direction)
+  bool order_valid = false;     // particle order = stable cell order of `offsets`
+  uint8_t *trans = nullptr;     // [27][nc] ions of each old cell per move direction
+  uint16_t *info = nullptr;     // per ion: direction << 8 | rank among the cell's ions of that direction
+  uint16_t *nbase = nullptr;    // [27][nc] start of each (old cell, direction) group in its new cell
+  uint32_t *offs_alt = nullptr; // nc + 1: the new offsets while the old ones are read
+  uint32_t *sortflag = nullptr; // set when a move is not to a neighbour or a count overflows
   void *partials = nullptr;     // per-cell corner partials [32][nc] (T)
   double *sred = nullptr;       // collision-normaliser block partials
   double *S_dev = nullptr;      // collision normaliser (device scalar)

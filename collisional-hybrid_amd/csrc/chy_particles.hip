@@ -26,6 +26,17 @@ namespace chy {
 
 template <typename T> void launch_fill(chy_ctx *ctx, int64_t n, T *v, T val);
 
+// Workgroups are dealt to the 8 XCDs round-robin (block b on XCD b % 8).  The
+// grid-stride particle kernels index by this logical block instead, so that
+// the blocks of one XCD cover one contiguous eighth of every grid-wide window
+// and ions, their neighbours and their field corners share that XCD's L2.
+__device__ __forceinline__ int64_t xcd_block()
+{
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  if (nb & 7u) return b;
+  return (int64_t)(b & 7u) * (nb >> 3) + (b >> 3);
+}
+
 // Wave-aggregated integer atomics on per-cell counters for runs of equal keys.
 __device__ __forceinline__ void count_key(uint32_t *counts, int key)
 {
@@ -76,7 +87,7 @@ __global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t
   const int64_t iters = (np + stride - 1) / stride;
   uint32_t nacc = 0, nref = 0;
   for (int64_t it = 0; it < iters; it++) {
-    const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t p = it * stride + xcd_block() * blockDim.x + threadIdx.x;
     const bool live = p < np;
     int key = -1, capped = 0;
     bool bad = false;
@@ -108,7 +119,7 @@ __global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t
       }
       bad = !isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(x) || !isfinite(y) || !isfinite(z);
     }
-    if (MOVE) count_key(counts, key);
+    if (MOVE && counts) count_key(counts, key);
     if (COLLIDE) count_flags(cstats, 2, capped != 0);
     count_flags(cstats, 3, bad);
   }
@@ -128,7 +139,7 @@ __global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConst pc, int64_
   const int64_t iters = (np + stride - 1) / stride;
   uint32_t nacc = 0, nref = 0;
   for (int64_t it = 0; it < iters; it++) {
-    const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t p = it * stride + xcd_block() * blockDim.x + threadIdx.x;
     const bool live = p < np;
     int key = -1, capped = 0;
     bool bad = false;
@@ -160,7 +171,7 @@ __global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConst pc, int64_
       }
       bad = bad || !isfinite(vx) || !isfinite(vy) || !isfinite(vz) || !isfinite(fx) || !isfinite(fy) || !isfinite(fz);
     }
-    if (MOVE) count_key(counts, key);
+    if (MOVE && counts) count_key(counts, key);
     if (COLLIDE) count_flags(cstats, 2, capped != 0);
     count_flags(cstats, 3, bad);
   }
@@ -314,6 +325,7 @@ __global__ __launch_bounds__(256) void k_scan_tiles(int64_t nt, uint32_t *__rest
     if (i < nt) tsum[i] = carry + ex;
     carry += tot;
   }
+  if (threadIdx.x == 0) tsum[nt] = carry;  // the histogram's total, checked against the ion count
 }
 
 // Per-tile scan; writes offsets[0..n) and the cursor copy; offsets[n] = total.
@@ -342,15 +354,15 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t np, const int32_t *__re
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t iters = (np + stride - 1) / stride;
   for (int64_t it = 0; it < iters; it++) {
-    const int64_t p = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t p = it * stride + xcd_block() * blockDim.x + threadIdx.x;
     const int key = p < np ? cell[p] : -1;
     int head, len;
     key_runs(key, head, len);
     uint32_t base = 0;
     if (key >= 0 && lane_id() == head) base = atomicAdd(&cursor[key], (uint32_t)len);
     base = __shfl(base, head, 64);
-    if (key >= 0) {
-      const uint32_t slot = base + (uint32_t)(lane_id() - head);
+    const uint32_t slot = base + (uint32_t)(lane_id() - head);
+    if (key >= 0 && slot < np) {  // (the host has checked the histogram's total)
       perm[slot] = (uint32_t)p;
       slotkey[slot] = key;
     }
@@ -374,7 +386,7 @@ __global__ __launch_bounds__(256) void k_reorder(int64_t np, const uint32_t *__r
   __shared__ uint32_t pw[kWin];
   __shared__ double sh[4];
   double acc = 0.0;
-  for (int64_t s0 = (int64_t)blockIdx.x * 256; s0 < np; s0 += (int64_t)gridDim.x * 256) {
+  for (int64_t s0 = xcd_block() * 256; s0 < np; s0 += (int64_t)gridDim.x * 256) {
     const int64_t wlo = s0 - kRankHalo;
     for (int e = threadIdx.x; e < kWin; e += blockDim.x) {
       const int64_t t = wlo + e;
@@ -422,6 +434,162 @@ __global__ __launch_bounds__(256) void k_reorder(int64_t np, const uint32_t *__r
   }
 }
 
+// ---- neighbour-move sort ----------------------------------------------------------------
+// After a push from the stable cell order every ion is in its old cell or in
+// one of the 26 neighbours.  Its slot in the new stable order (cell, then old
+// index) is then
+//   offs[c] + (ions entering c from old cells numbered below its old cell,
+//              the stayers of c included when c is below)
+//           + (its rank among the ions of its old cell that go to c).
+// k_trans counts, per old cell, the ions going each way and ranks every ion
+// within its direction; k_counts turns the counts into the new histogram and
+// the start of every (old cell, direction) group within its new cell; k_perm
+// places every ion (perm[slot] = old index) and k_gather moves the state in
+// slot order.
+// The atomic scatter and the per-cell ranking of the general sort are not
+// needed.  A move further than one cell, or 255 ions of one cell going one way,
+// sets a flag and the general sort runs instead.
+
+__device__ __forceinline__ int nb_step(int diff, int n)
+{
+  if (diff == 0) return 0;
+  if (diff == 1 || diff == 1 - n) return 1;
+  if (diff == -1 || diff == n - 1) return -1;
+  return 2;  // not a neighbour
+}
+
+__device__ __forceinline__ int64_t nb_cell(const Grid &g, int i, int j, int k)
+{
+  return (int64_t)wrapi(i, g.nx) + (int64_t)g.nx * ((int64_t)wrapi(j, g.ny) + (int64_t)g.ny * wrapi(k, g.nz));
+}
+
+// direction index d = (di + 1) + 3 (dj + 1) + 9 (dk + 1); 13 = stay
+__global__ __launch_bounds__(256) void k_trans(Grid g, const uint32_t *__restrict__ oldoffs,
+                                               const int32_t *__restrict__ cell, uint16_t *__restrict__ info,
+                                               uint8_t *__restrict__ trans, uint32_t *__restrict__ flag)
+{
+  __shared__ uint16_t cnt[27][256];  // column t belongs to thread t
+  const int t = threadIdx.x;
+  for (int d = 0; d < 27; d++) cnt[d][t] = 0;
+  const int64_t c = xcd_block() * 256 + t;
+  if (c >= g.nc) return;
+  int i, j, k;
+  split_cell(g, (int)c, i, j, k);
+  bool bad = false;
+  const uint32_t lo = oldoffs[c], hi = oldoffs[c + 1];
+  constexpr int kB = 8;  // keys in flight per thread
+  for (uint32_t q0 = lo; q0 < hi; q0 += kB) {
+    int key[kB];
+#pragma unroll
+    for (int u = 0; u < kB; u++) key[u] = q0 + u < hi ? cell[q0 + u] : 0;
+#pragma unroll
+    for (int u = 0; u < kB; u++) {
+      if (q0 + u >= hi) break;
+      int i2, j2, k2;
+      split_cell(g, key[u], i2, j2, k2);
+      const int di = nb_step(i2 - i, g.nx), dj = nb_step(j2 - j, g.ny), dk = nb_step(k2 - k, g.nz);
+      if (di == 2 || dj == 2 || dk == 2) { bad = true; continue; }
+      const int d = (di + 1) + 3 * (dj + 1) + 9 * (dk + 1);
+      const uint32_t r = cnt[d][t];
+      if (r >= 255) { bad = true; continue; }
+      info[q0 + u] = (uint16_t)((d << 8) | r);
+      cnt[d][t] = (uint16_t)(r + 1);
+    }
+  }
+  for (int d = 0; d < 27; d++) trans[(int64_t)d * g.nc + c] = (uint8_t)cnt[d][t];
+  if (bad) atomicOr(flag, 1u);
+}
+
+// counts[c] = ions ending in c; base[d][s] = the ions entering c = s + off(d)
+// from old cells numbered below s (the group of ions going from s in
+// direction d starts there within c's segment).  Away from the periodic wrap
+// the source s_d = c - off(d) falls as d rises (the grid has at least 3 cells
+// per axis), so the bases are suffix sums over d; cells on the boundary
+// compare the source numbers.
+__global__ __launch_bounds__(256) void k_counts(Grid g, const uint8_t *__restrict__ trans, uint32_t *__restrict__ counts,
+                                                uint16_t *__restrict__ base)
+{
+  const int64_t c = xcd_block() * 256 + threadIdx.x;
+  if (c >= g.nc) return;
+  int i, j, k;
+  split_cell(g, (int)c, i, j, k);
+  int64_t s[27];
+  uint32_t t[27];
+  uint32_t n = 0;
+#pragma unroll
+  for (int d = 0; d < 27; d++) {
+    s[d] = nb_cell(g, i - (d % 3 - 1), j - ((d / 3) % 3 - 1), k - (d / 9 - 1));  // old cell of an arrival in direction d
+    t[d] = trans[(int64_t)d * g.nc + s[d]];
+    n += t[d];
+  }
+  counts[c] = n;
+  const bool interior = i >= 1 && i <= g.nx - 2 && j >= 1 && j <= g.ny - 2 && k >= 1 && k <= g.nz - 2;
+  if (interior) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int d = 26; d >= 0; d--) {
+      base[(int64_t)d * g.nc + s[d]] = (uint16_t)acc;
+      acc += t[d];
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < 27; d++) {
+      uint32_t b = 0;
+#pragma unroll
+      for (int d2 = 0; d2 < 27; d2++) b += s[d2] < s[d] ? t[d2] : 0u;
+      base[(int64_t)d * g.nc + s[d]] = (uint16_t)b;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_perm(Grid g, int64_t np, const int32_t *__restrict__ cell,
+                                              const uint16_t *__restrict__ info, const uint16_t *__restrict__ base,
+                                              const uint32_t *__restrict__ offs, uint32_t *__restrict__ perm)
+{
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t p = xcd_block() * blockDim.x + threadIdx.x; p < np; p += stride) {
+    const int c = cell[p];
+    const uint32_t inf = info[p];
+    const int d = (int)(inf >> 8);
+    int i, j, k;
+    split_cell(g, c, i, j, k);
+    const int64_t co = nb_cell(g, i - (d % 3 - 1), j - ((d / 3) % 3 - 1), k - (d / 9 - 1));  // old cell
+    const uint32_t slot = offs[c] + base[(int64_t)d * g.nc + co] + (inf & 255u);
+    if (slot < np) perm[slot] = (uint32_t)p;
+  }
+}
+
+// Slot order: b[s] = a[perm[s]]; with SUMS the collision normaliser's block
+// partials over the final velocities in slot order.
+template <typename T, bool SUMS>
+__global__ __launch_bounds__(256) void k_gather(int64_t np, const uint32_t *__restrict__ perm, PState<T> a,
+                                                PState<T> b, bool weighted, double n0x, double n0y, double n0z,
+                                                double vnT, double *__restrict__ sred, uint32_t *__restrict__ src)
+{
+  __shared__ double sh[4];
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t d = xcd_block() * blockDim.x + threadIdx.x; d < np; d += stride) {
+    const uint32_t me = perm[d];
+    if (src) src[d] = me;
+    const T vx = a.vx[me], vy = a.vy[me], vz = a.vz[me];
+    b.x[d] = a.x[me]; b.y[d] = a.y[me]; b.z[d] = a.z[me];
+    b.vx[d] = vx; b.vy[d] = vy; b.vz[d] = vz;
+    b.pid[d] = a.pid[me];
+    b.cell[d] = a.cell[me];
+    if (weighted) b.w[d] = a.w[me];
+    if (SUMS) {
+      if (sizeof(T) == 4) acc += (double)mean_rel_speed_f((float)vx, (float)vy, (float)vz, (float)n0x, (float)n0y,
+                                                         (float)n0z, (float)vnT);
+      else acc += mean_rel_speed((double)vx, (double)vy, (double)vz, n0x, n0y, n0z, vnT);
+    }
+  }
+  if (SUMS) {
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) sred[blockIdx.x] = t;
+  }
+}
+
 // ---- deposit ---------------------------------------------------------------------
 // Per-cell corner partials P[(4q+m)*nc + c], m in {w, w vx, w vy, w vz}.
 // A block owns kDepCells consecutive cells, whose particles are contiguous
@@ -439,7 +607,7 @@ __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *_
   constexpr int L = kDepChunk + kDepChunk / 16;
   __shared__ T sf[7][L];
   __shared__ uint32_t so[kDepCells + 1];
-  const int64_t c0 = (int64_t)blockIdx.x * kDepCells;
+  const int64_t c0 = xcd_block() * kDepCells;
   const int ncell = (int)((g.nc - c0) < kDepCells ? (g.nc - c0) : kDepCells);
   if (threadIdx.x <= ncell) so[threadIdx.x] = offs[c0 + threadIdx.x];
   __syncthreads();
@@ -501,7 +669,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_node_sum(Grid g, const T *__restrict__ P, T *__restrict__ n, T *__restrict__ gx,
                                                   T *__restrict__ gy, T *__restrict__ gz)
 {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
   T m[4] = {(T)0, (T)0, (T)0, (T)0};
@@ -568,9 +736,71 @@ static unsigned grid_for(int64_t n, unsigned cap = 4096)
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, cap));
 }
 
+// The neighbour-move sort (see k_trans); false when it does not apply and the
+// general sort must run (counts[] is then filled here).
+static int sort_neighbour(chy_ctx *ctx, bool &done)
+{
+  done = false;
+  const int64_t nc = ctx->g.nc, np = ctx->np;
+  const int64_t nt = (nc + kScanTile - 1) / kScanTile;
+  const int32_t *cell = ctx->mixed ? ctx->pf[ctx->cur].cell : ctx->pd[ctx->cur].cell;
+  const unsigned ncb = (unsigned)((nc + 255) / 256);
+  launch_fill<uint32_t>(ctx, 2, ctx->sortflag, 0u);
+  launch_fill<uint32_t>(ctx, 1, ctx->maxcount, 0u);
+  timer_start(ctx, "sort_trans");
+  hipLaunchKernelGGL(k_trans, ncb, 256, 0, ctx->stream, ctx->g, ctx->offsets, cell, ctx->info, ctx->trans,
+                     ctx->sortflag);
+  hipLaunchKernelGGL(k_counts, ncb, 256, 0, ctx->stream, ctx->g, ctx->trans, ctx->counts, ctx->nbase);
+  timer_stop(ctx, "sort_trans");
+  timer_start(ctx, "sort_scan");
+  hipLaunchKernelGGL(k_tile_sum, (unsigned)nt, kBlock, 0, ctx->stream, nc, ctx->counts, ctx->scan_tmp, ctx->maxcount);
+  hipLaunchKernelGGL(k_scan_tiles, 1, kBlock, 0, ctx->stream, nt, ctx->scan_tmp);
+  hipLaunchKernelGGL(k_tile_scan, (unsigned)nt, kBlock, 0, ctx->stream, nc, ctx->counts, ctx->scan_tmp, ctx->offs_alt,
+                     ctx->cursor, (uint32_t)np);
+  timer_stop(ctx, "sort_scan");
+  uint32_t flag = 0, mx = 0, total = 0;
+  CHY_HIP(hipMemcpyAsync(&flag, ctx->sortflag, sizeof(flag), hipMemcpyDeviceToHost, ctx->stream));
+  CHY_HIP(hipMemcpyAsync(&mx, ctx->maxcount, sizeof(mx), hipMemcpyDeviceToHost, ctx->stream));
+  CHY_HIP(hipMemcpyAsync(&total, ctx->scan_tmp + nt, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
+  CHY_HIP(hipStreamSynchronize(ctx->stream));
+  if (flag || mx > kMaxCellOccupancy || total != (uint32_t)np) {
+    launch_fill<uint32_t>(ctx, nc + 1, ctx->counts, 0u);
+    hipLaunchKernelGGL(k_keys_count, grid_for(np), kBlock, 0, ctx->stream, np, cell, ctx->counts);
+    return CHY_OK;
+  }
+  const unsigned nb = grid_for(np);
+  timer_start(ctx, "sort_scatter");
+  hipLaunchKernelGGL(k_perm, nb, kBlock, 0, ctx->stream, ctx->g, np, cell, ctx->info, ctx->nbase, ctx->offs_alt,
+                     ctx->perm);
+  timer_stop(ctx, "sort_scatter");
+  const chy_params &p = ctx->prm;
+  const bool sums = p.mn > 0.0;  // the collision normaliser is needed
+  timer_start(ctx, "sort_reorder");
+#define GATHER(T, S, st)                                                                                       \
+  hipLaunchKernelGGL((k_gather<T, S>), nb, kBlock, 0, ctx->stream, np, ctx->perm, st[ctx->cur], st[1 - ctx->cur], \
+                     ctx->weighted, p.vn0[0], p.vn0[1], p.vn0[2], p.vnT, ctx->sred, ctx->sort_src)
+  if (ctx->mixed) { if (sums) GATHER(float, true, ctx->pf); else GATHER(float, false, ctx->pf); }
+  else { if (sums) GATHER(double, true, ctx->pd); else GATHER(double, false, ctx->pd); }
+#undef GATHER
+  timer_stop(ctx, "sort_reorder");
+  if (sums) launch_combine_sum(ctx, ctx->sred, (int)nb, 1, ctx->S_dev);
+  std::swap(ctx->offsets, ctx->offs_alt);
+  ctx->cur = 1 - ctx->cur;
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("sort: ") + hipGetErrorString(e));
+  done = true;
+  return CHY_OK;
+}
+
 int particles_sort(chy_ctx *ctx)
 {
-  // counts[] was filled by the key pass (push or keys kernel)
+  if (ctx->order_valid && ctx->np > 0) {
+    bool done = false;
+    TRY(sort_neighbour(ctx, done));
+    if (done) return CHY_OK;
+  }
+  ctx->order_valid = false;
+  // counts[] was filled by the key pass (push, keys kernel or the fallback above)
   const int64_t nc = ctx->g.nc, np = ctx->np;
   const int64_t nt = (nc + kScanTile - 1) / kScanTile;
   launch_fill<uint32_t>(ctx, 1, ctx->maxcount, 0u);
@@ -580,17 +810,22 @@ int particles_sort(chy_ctx *ctx)
   hipLaunchKernelGGL(k_tile_scan, (unsigned)nt, kBlock, 0, ctx->stream, nc, ctx->counts, ctx->scan_tmp, ctx->offsets,
                      ctx->cursor, (uint32_t)np);
   timer_stop(ctx, "sort_scan");
+  // the stable per-cell rank is quadratic in the cell occupancy: bound it; and
+  // the histogram must account for every ion before the scatter uses it
+  uint32_t mx = 0, total = 0;
+  CHY_HIP(hipMemcpyAsync(&mx, ctx->maxcount, sizeof(mx), hipMemcpyDeviceToHost, ctx->stream));
+  CHY_HIP(hipMemcpyAsync(&total, ctx->scan_tmp + nt, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
+  CHY_HIP(hipStreamSynchronize(ctx->stream));
+  if (total != (uint32_t)np)
+    return set_err(ctx, CHY_ERR_LIB, "sort: the cell histogram holds " + std::to_string(total) + " ions, not " +
+                                         std::to_string(np));
+  if (mx > kMaxCellOccupancy)
+    return set_err(ctx, CHY_ERR_SUP, "a cell holds " + std::to_string(mx) + " particles; at most " +
+                                         std::to_string(kMaxCellOccupancy) + " are supported");
   const int32_t *cell = ctx->mixed ? ctx->pf[ctx->cur].cell : ctx->pd[ctx->cur].cell;
   timer_start(ctx, "sort_scatter");
   hipLaunchKernelGGL(k_scatter, grid_for(np), kBlock, 0, ctx->stream, np, cell, ctx->cursor, ctx->perm, ctx->slotkey);
   timer_stop(ctx, "sort_scatter");
-  // the stable per-cell rank is quadratic in the cell occupancy: bound it
-  uint32_t mx = 0;
-  CHY_HIP(hipMemcpyAsync(&mx, ctx->maxcount, sizeof(mx), hipMemcpyDeviceToHost, ctx->stream));
-  CHY_HIP(hipStreamSynchronize(ctx->stream));
-  if (mx > kMaxCellOccupancy)
-    return set_err(ctx, CHY_ERR_SUP, "a cell holds " + std::to_string(mx) + " particles; at most " +
-                                         std::to_string(kMaxCellOccupancy) + " are supported");
   const unsigned nb = grid_for(np);
   const chy_params &p = ctx->prm;
   const bool sums = p.mn > 0.0;  // the collision normaliser is needed
@@ -607,6 +842,7 @@ int particles_sort(chy_ctx *ctx)
   ctx->cur = 1 - ctx->cur;
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("sort: ") + hipGetErrorString(e));
+  ctx->order_valid = true;
   return CHY_OK;
 }
 
@@ -670,6 +906,7 @@ int particles_normaliser(chy_ctx *ctx)
 // Keys + histogram + sort + normaliser after new particle data.
 static int settle(chy_ctx *ctx)
 {
+  ctx->order_valid = false;  // new data: the general sort
   launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
   if (ctx->mixed)
     hipLaunchKernelGGL(k_keys_count, grid_for(ctx->np), kBlock, 0, ctx->stream, ctx->np, ctx->pf[ctx->cur].cell,
@@ -870,12 +1107,15 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
   pc.Nc = (double)Nc;
   if (collide && Nc == 0) collide = false;
   launch_fill<unsigned long long>(ctx, 4, ctx->cstats, 0ull);
-  if (move) launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
+  // the histogram is only needed by the general sort (the neighbour-move
+  // sort counts per old cell)
+  const bool count = move && !ctx->order_valid;
+  if (count) launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
   const unsigned nb = grid_for(ctx->np);
   timer_start(ctx, "push");
 #define LAUNCH(KER, T, C, M)                                                                                  \
   hipLaunchKernelGGL((KER<C, M>), nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, st,                       \
-                     (const Vec4<T> *)ctx->efield[0], ctx->S_dev, ctx->counts, ctx->cstats)
+                     (const Vec4<T> *)ctx->efield[0], ctx->S_dev, count ? ctx->counts : nullptr, ctx->cstats)
   if (ctx->mixed) {
     PState<float> st = ctx->pf[ctx->cur];
     if (collide && move) LAUNCH(k_push_mixed, float, true, true);

@@ -134,14 +134,20 @@ def test_device_init_matches_oracle():
             assert relmax(getattr(got, a), getattr(ps, a)) <= 1e-14, a
 
 
-@pytest.mark.parametrize("collide", [False, True])
-def test_steps_match_oracle(collide):
+@pytest.mark.parametrize("collide,N,more,rtol", [
+    (False, (8, 8, 8), "", 1e-12), (True, (8, 8, 8), "", 1e-12),
+    # ions crossing more than one cell per step, and 300 ions per cell (more
+    # than 255 going one way): the sort falls back from the neighbour-move
+    # placement to the general counting sort
+    (False, (8, 8, 8), "-dt 4e-4", 1e-10), (False, (4, 4, 4), "-ppc 300", 1e-12)])
+def test_steps_match_oracle(collide, N, more, rtol):
     """Three whole steps (push, sort, deposit, forcing, solve) from a seeded
-    state; trajectories, moments and phi against the oracle step by step."""
-    extra = "-seed 2 -ppc 16 -ksp_rtol 1e-12 --perturbation uniform --perturbation-amplitude 0.01"
+    state; trajectories, moments and phi against the oracle step by step (the
+    particle order is the oracle's stable cell order after every step)."""
+    extra = f"-seed 2 -ppc 16 -ksp_rtol {rtol} --perturbation uniform --perturbation-amplitude 0.01 " + more
     if not collide:
         extra += " -mn 0"
-    with context(opts((8, 8, 8), extra=extra)) as ctx:
+    with context(opts(N, extra=extra)) as ctx:
         ctx.init_particles()
         ctx.collect_moments(download=False)
         phi, _ = ctx.solve_potential()
@@ -165,11 +171,11 @@ def test_steps_match_oracle(collide):
             for a in range(4):
                 assert relmax(mom_gpu[:, a], mom[a]) <= 1e-12, (it, a)
             b = orc.rhs(g, list(prm.kappa_e), list(prm.E0), 0.0, prm.me * prm.nue / orc.Q, *mom)
-            assert st.solve.converged == 1 and st.solve.residual <= 1e-12
+            assert st.solve.converged == 1 and st.solve.residual <= rtol
             r = b - orc.lhs_apply(g, list(prm.kappa_e), mom[0], phi_gpu)
-            assert np.linalg.norm(r) <= 2e-12 * np.linalg.norm(b), it
-            phi_ref, _, conv, _ = orc.solve(g, list(prm.kappa_e), mom[0], b, phi0=phi, rtol=1e-12)
-            assert conv and relmax(phi_gpu, phi_ref) <= 1e-8, (it, relmax(phi_gpu, phi_ref))
+            assert np.linalg.norm(r) <= 2 * rtol * np.linalg.norm(b), it
+            phi_ref, _, conv, _ = orc.solve(g, list(prm.kappa_e), mom[0], b, phi0=phi, rtol=rtol)
+            assert conv and relmax(phi_gpu, phi_ref) <= 1e4 * rtol, (it, relmax(phi_gpu, phi_ref))
             cur, phi = nxt, phi_gpu
 
 

@@ -38,7 +38,7 @@ SINGLE_KERNEL = ("efield", "push", "sort_scatter", "sort_reorder", "deposit_cell
                  "lhs_apply")
 COPY_BYTES = 1 << 30
 # kernels launched once per step: timed with HIP events during the timed run
-ONCE_PER_STEP = ("efield", "push", "sort_scatter", "sort_reorder", "deposit_cells", "deposit_nodes")
+ONCE_PER_STEP = ("efield", "push", "sort_trans", "sort_scatter", "sort_reorder", "deposit_cells", "deposit_nodes")
 BREAKDOWN_STEPS = 10
 
 PHYS = ("-mi 4.9816e-26 -mn 4.8156e-26 -Ti 300 -Tn 300 -B0z 5e-5 -E0x 0.05 -nui 1e3 -nue 1e5 -dt 1e-5 "
@@ -65,11 +65,15 @@ def algorithmic_bytes(kernel, prec, np_, nc):
         # (position 3s + cell 4, v 3s), node E 3s per node
         pos_in = 3 * s + (4 if prec == "mixed" else 0)
         return np_ * (pos_in + 3 * s + 4 + 3 * s + 4 + 3 * s) + nc * 3 * s
+    if kernel == "sort_trans":
+        # per old cell: the ions' keys in, (direction, rank) out; 27 direction
+        # counts out and back in, new counts and 27 group bases out
+        return np_ * (4 + 2) + nc * (4 + 27 + 27 + 4 + 2 * 27)
     if kernel == "sort_scatter":
-        return np_ * (4 + 4 + 4) + nc * 4            # key in, perm + slot key out, cursor
+        return np_ * (4 + 2 + 4) + nc * (4 + 2 * 27)  # key + (direction, rank) in, perm out; offsets, bases
     if kernel == "sort_reorder":
-        payload = 3 * s + 3 * s + s + 4              # position, v, weight, pid
-        return np_ * (4 + 4 + payload + payload + 4) + nc * 4
+        payload = 3 * s + 3 * s + s + 4 + 4          # position, v, weight, pid, cell
+        return np_ * (4 + payload + payload)          # perm in, state gathered, state out
     if kernel == "sort_scan":
         return nc * (4 + 4 + 4)
     if kernel == "deposit_cells":
@@ -134,7 +138,7 @@ def run_gpu(N, ppc, prec, steps, warmup, keep=False, cross=None):
         roof_kernel = {"name": dom, "avg_ms": once[dom]["avg_ms"], "launches": once[dom]["launches"]}
         # per-kernel breakdown: every timer on, over BREAKDOWN_STEPS further
         # steps (the event records add launch overhead, so not the headline)
-        names = ("efield", "push", "sort_scan", "sort_scatter", "sort_reorder", "deposit_cells",
+        names = ("efield", "push", "sort_trans", "sort_scan", "sort_scatter", "sort_reorder", "deposit_cells",
                  "deposit_nodes", "rhs", "pc", "lhs_apply", "krylov")
         ctx.timing_select(None)
         ctx.timing_reset()

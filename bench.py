@@ -72,8 +72,10 @@ def algorithmic_bytes(kernel, prec, np_, nc):
     if kernel == "sort_scatter":
         return np_ * (4 + 2 + 4) + nc * (4 + 2 * 27)  # key + (direction, rank) in, perm out; offsets, bases
     if kernel == "sort_reorder":
+        # perm in, state gathered and written in slot order, and the deposit's
+        # cell partials (8 corners x {w, w v}) made from it
         payload = 3 * s + 3 * s + s + 4 + 4          # position, v, weight, pid, cell
-        return np_ * (4 + payload + payload)          # perm in, state gathered, state out
+        return np_ * (4 + payload + payload) + nc * (4 + 32 * s)
     if kernel == "sort_scan":
         return nc * (4 + 4 + 4)
     if kernel == "deposit_cells":

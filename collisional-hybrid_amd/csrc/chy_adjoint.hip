@@ -314,6 +314,7 @@ int restore_stage(chy_ctx *ctx, const Stage &s)
   CHY_HIP(hipMemcpyAsync(ctx->S_dev, s.S, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
   ctx->step = s.step;
   ctx->order_valid = s.ordered;
+  ctx->partials_fresh = false;
   ctx->phi_valid = true;
   ctx->moments_valid = false;
   return CHY_OK;

@@ -491,7 +491,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
     A(dalloc(ctx, &pp, (int64_t)(32 * ts) * nc));
     ctx->partials = pp;
   }
-  A(dalloc(ctx, &ctx->sred, 4096));
+  A(dalloc(ctx, &ctx->sred, std::max<int64_t>(4096, nc / 64 + 1)));  // normaliser partials, one per block
   A(dalloc(ctx, &ctx->S_dev, 1));
   A(dalloc(ctx, &ctx->cstats, 4));
   for (int a = 0; a < 4; a++) { unsigned char *q = nullptr; A(dalloc(ctx, &q, (int64_t)ts * nc)); ctx->mom[a] = q; }

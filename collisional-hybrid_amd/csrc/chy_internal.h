@@ -75,6 +75,7 @@ struct chy_ctx {
   // neighbour-move sort (the particles are in the stable order of `offsets`
   // and every ion moved at most one cell in each direction)
   bool order_valid = false;     // particle order = stable cell order of `offsets`
+  bool partials_fresh = false;  // the sort made the deposit's cell partials of the current state
   uint8_t *trans = nullptr;     // [27][nc] ions of each old cell per move direction
   uint16_t *info = nullptr;     // per ion: direction << 8 | rank among the cell's ions of that direction
   uint16_t *nbase = nullptr;    // [27][nc] start of each (old cell, direction) group in its new cell

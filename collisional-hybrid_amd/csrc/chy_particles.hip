@@ -444,8 +444,8 @@ __global__ __launch_bounds__(256) void k_reorder(int64_t np, const uint32_t *__r
 // k_trans counts, per old cell, the ions going each way and ranks every ion
 // within its direction; k_counts turns the counts into the new histogram and
 // the start of every (old cell, direction) group within its new cell; k_perm
-// places every ion (perm[slot] = old index) and k_gather moves the state in
-// slot order.
+// places every ion (perm[slot] = old index) and k_cell_partials<GATHER>
+// moves the state in slot order and makes the deposit's cell partials from it.
 // The atomic scatter and the per-cell ranking of the general sort are not
 // needed.  A move further than one cell, or 255 ions of one cell going one way,
 // sets a flag and the general sort runs instead.
@@ -559,36 +559,6 @@ __global__ __launch_bounds__(256) void k_perm(Grid g, int64_t np, const int32_t 
   }
 }
 
-// Slot order: b[s] = a[perm[s]]; with SUMS the collision normaliser's block
-// partials over the final velocities in slot order.
-template <typename T, bool SUMS>
-__global__ __launch_bounds__(256) void k_gather(int64_t np, const uint32_t *__restrict__ perm, PState<T> a,
-                                                PState<T> b, bool weighted, double n0x, double n0y, double n0z,
-                                                double vnT, double *__restrict__ sred, uint32_t *__restrict__ src)
-{
-  __shared__ double sh[4];
-  double acc = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t d = xcd_block() * blockDim.x + threadIdx.x; d < np; d += stride) {
-    const uint32_t me = perm[d];
-    if (src) src[d] = me;
-    const T vx = a.vx[me], vy = a.vy[me], vz = a.vz[me];
-    b.x[d] = a.x[me]; b.y[d] = a.y[me]; b.z[d] = a.z[me];
-    b.vx[d] = vx; b.vy[d] = vy; b.vz[d] = vz;
-    b.pid[d] = a.pid[me];
-    b.cell[d] = a.cell[me];
-    if (weighted) b.w[d] = a.w[me];
-    if (SUMS) {
-      if (sizeof(T) == 4) acc += (double)mean_rel_speed_f((float)vx, (float)vy, (float)vz, (float)n0x, (float)n0y,
-                                                         (float)n0z, (float)vnT);
-      else acc += mean_rel_speed((double)vx, (double)vy, (double)vz, n0x, n0y, n0z, vnT);
-    }
-  }
-  if (SUMS) {
-    const double t = block_sum(acc, sh);
-    if (threadIdx.x == 0) sred[blockIdx.x] = t;
-  }
-}
 
 // ---- deposit ---------------------------------------------------------------------
 // Per-cell corner partials P[(4q+m)*nc + c], m in {w, w vx, w vy, w vz}.
@@ -596,17 +566,27 @@ __global__ __launch_bounds__(256) void k_gather(int64_t np, const uint32_t *__re
 // after the sort; they are staged through LDS in coalesced chunks.  Four
 // threads share a cell, each summing two corners over the cell's particles in
 // sorted order (the oracle's order), so the result is deterministic.
+// GATHER: the same kernel finishes the neighbour-move sort -- slot s of the
+// new order takes ion perm[s] of `st`, whose state is written to `b` (and
+// src[s] = perm[s]), and the ion is deposited from registers, so the sorted
+// state is not read back; with SUMS the block's part of the collision
+// normaliser over the final velocities in slot order goes to sred[block].
 constexpr int kDepCells = 64;
 constexpr int kDepChunk = 1024;
 __device__ __forceinline__ int skew(int e) { return e + (e >> 4); }  // LDS bank spread
 
-template <typename T, bool ABS>
+template <typename T, bool ABS, bool GATHER, bool SUMS>
 __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *__restrict__ offs, PState<T> st,
-                                                       bool weighted, T *__restrict__ P)
+                                                       bool weighted, T *__restrict__ P,
+                                                       const uint32_t *__restrict__ perm, PState<T> b, double n0x,
+                                                       double n0y, double n0z, double vnT, double *__restrict__ sred,
+                                                       uint32_t *__restrict__ src)
 {
   constexpr int L = kDepChunk + kDepChunk / 16;
   __shared__ T sf[7][L];
   __shared__ uint32_t so[kDepCells + 1];
+  __shared__ double sh[4];
+  double sacc = 0.0;
   const int64_t c0 = xcd_block() * kDepCells;
   const int ncell = (int)((g.nc - c0) < kDepCells ? (g.nc - c0) : kDepCells);
   if (threadIdx.x <= ncell) so[threadIdx.x] = offs[c0 + threadIdx.x];
@@ -622,18 +602,34 @@ __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *_
     const int m = (int)((s1 - base) < (uint32_t)kDepChunk ? (s1 - base) : (uint32_t)kDepChunk);
     for (int e = threadIdx.x; e < m; e += blockDim.x) {
       const uint32_t s = base + e;
+      const uint32_t q = GATHER ? perm[s] : s;
       const int k = skew(e);
+      const T x = st.x[q], y = st.y[q], z = st.z[q], vx = st.vx[q], vy = st.vy[q], vz = st.vz[q];
+      const T w = weighted ? st.w[q] : (T)1;
+      if (GATHER) {
+        b.x[s] = x; b.y[s] = y; b.z[s] = z;
+        b.vx[s] = vx; b.vy[s] = vy; b.vz[s] = vz;
+        b.pid[s] = st.pid[q];
+        b.cell[s] = st.cell[q];
+        if (weighted) b.w[s] = w;
+        if (src) src[s] = q;
+        if (SUMS) {
+          if (sizeof(T) == 4) sacc += (double)mean_rel_speed_f((float)vx, (float)vy, (float)vz, (float)n0x,
+                                                                (float)n0y, (float)n0z, (float)vnT);
+          else sacc += mean_rel_speed((double)vx, (double)vy, (double)vz, n0x, n0y, n0z, vnT);
+        }
+      }
       if (ABS) {
         int i;
         double f;
-        cell_coord((double)st.x[s], g.p0[0], g.d[0], g.nx, i, f); sf[0][k] = (T)f;
-        cell_coord((double)st.y[s], g.p0[1], g.d[1], g.ny, i, f); sf[1][k] = (T)f;
-        cell_coord((double)st.z[s], g.p0[2], g.d[2], g.nz, i, f); sf[2][k] = (T)f;
+        cell_coord((double)x, g.p0[0], g.d[0], g.nx, i, f); sf[0][k] = (T)f;
+        cell_coord((double)y, g.p0[1], g.d[1], g.ny, i, f); sf[1][k] = (T)f;
+        cell_coord((double)z, g.p0[2], g.d[2], g.nz, i, f); sf[2][k] = (T)f;
       } else {
-        sf[0][k] = st.x[s]; sf[1][k] = st.y[s]; sf[2][k] = st.z[s];
+        sf[0][k] = x; sf[1][k] = y; sf[2][k] = z;
       }
-      sf[3][k] = st.vx[s]; sf[4][k] = st.vy[s]; sf[5][k] = st.vz[s];
-      sf[6][k] = weighted ? st.w[s] : (T)1;
+      sf[3][k] = vx; sf[4][k] = vy; sf[5][k] = vz;
+      sf[6][k] = w;
     }
     __syncthreads();
     const uint32_t a0 = lo > base ? lo : base;
@@ -661,6 +657,10 @@ __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *_
     for (int h = 0; h < 2; h++)
 #pragma unroll
       for (int mm = 0; mm < 4; mm++) P[(int64_t)(4 * (2 * part + h) + mm) * g.nc + c0 + cl] = acc[4 * h + mm];
+  }
+  if (GATHER && SUMS) {
+    const double t = block_sum(sacc, sh);
+    if (threadIdx.x == 0) sred[blockIdx.x] = t;
   }
 }
 
@@ -775,15 +775,18 @@ static int sort_neighbour(chy_ctx *ctx, bool &done)
   timer_stop(ctx, "sort_scatter");
   const chy_params &p = ctx->prm;
   const bool sums = p.mn > 0.0;  // the collision normaliser is needed
+  const unsigned nbc = (unsigned)((nc + kDepCells - 1) / kDepCells);
   timer_start(ctx, "sort_reorder");
-#define GATHER(T, S, st)                                                                                       \
-  hipLaunchKernelGGL((k_gather<T, S>), nb, kBlock, 0, ctx->stream, np, ctx->perm, st[ctx->cur], st[1 - ctx->cur], \
-                     ctx->weighted, p.vn0[0], p.vn0[1], p.vn0[2], p.vnT, ctx->sred, ctx->sort_src)
-  if (ctx->mixed) { if (sums) GATHER(float, true, ctx->pf); else GATHER(float, false, ctx->pf); }
-  else { if (sums) GATHER(double, true, ctx->pd); else GATHER(double, false, ctx->pd); }
+#define GATHER(T, ABS, S, st)                                                                                     \
+  hipLaunchKernelGGL((k_cell_partials<T, ABS, true, S>), nbc, kBlock, 0, ctx->stream, ctx->g, ctx->offs_alt,       \
+                     st[ctx->cur], ctx->weighted, (T *)ctx->partials, ctx->perm, st[1 - ctx->cur], p.vn0[0],      \
+                     p.vn0[1], p.vn0[2], p.vnT, ctx->sred, ctx->sort_src)
+  if (ctx->mixed) { if (sums) GATHER(float, false, true, ctx->pf); else GATHER(float, false, false, ctx->pf); }
+  else { if (sums) GATHER(double, true, true, ctx->pd); else GATHER(double, true, false, ctx->pd); }
 #undef GATHER
   timer_stop(ctx, "sort_reorder");
-  if (sums) launch_combine_sum(ctx, ctx->sred, (int)nb, 1, ctx->S_dev);
+  if (sums) launch_combine_sum(ctx, ctx->sred, (int)nbc, 1, ctx->S_dev);
+  ctx->partials_fresh = true;  // the deposit's cell partials of the new order are made
   std::swap(ctx->offsets, ctx->offs_alt);
   ctx->cur = 1 - ctx->cur;
   const hipError_t e = hipGetLastError();
@@ -800,6 +803,7 @@ int particles_sort(chy_ctx *ctx)
     if (done) return CHY_OK;
   }
   ctx->order_valid = false;
+  ctx->partials_fresh = false;
   // counts[] was filled by the key pass (push, keys kernel or the fallback above)
   const int64_t nc = ctx->g.nc, np = ctx->np;
   const int64_t nt = (nc + kScanTile - 1) / kScanTile;
@@ -850,22 +854,29 @@ int particles_deposit(chy_ctx *ctx)
 {
   const unsigned nb = (unsigned)((ctx->g.nc + kBlock - 1) / kBlock);
   const unsigned nbc = (unsigned)((ctx->g.nc + kDepCells - 1) / kDepCells);
+  // after the neighbour-move sort the cell partials are already made
+  const bool cells = !ctx->partials_fresh;
+  ctx->partials_fresh = false;
   if (ctx->mixed) {
     float *P = (float *)ctx->partials;
-    timer_start(ctx, "deposit_cells");
-    hipLaunchKernelGGL((k_cell_partials<float, false>), nbc, kBlock, 0, ctx->stream, ctx->g, ctx->offsets,
-                       ctx->pf[ctx->cur], ctx->weighted, P);
-    timer_stop(ctx, "deposit_cells");
+    if (cells) {
+      timer_start(ctx, "deposit_cells");
+      hipLaunchKernelGGL((k_cell_partials<float, false, false, false>), nbc, kBlock, 0, ctx->stream, ctx->g, ctx->offsets,
+                         ctx->pf[ctx->cur], ctx->weighted, P, (const uint32_t *)nullptr, PState<float>(), 0.0, 0.0, 0.0, 0.0, (double *)nullptr, (uint32_t *)nullptr);
+      timer_stop(ctx, "deposit_cells");
+    }
     timer_start(ctx, "deposit_nodes");
     hipLaunchKernelGGL((k_node_sum<float>), nb, kBlock, 0, ctx->stream, ctx->g, P, (float *)ctx->mom[0],
                        (float *)ctx->mom[1], (float *)ctx->mom[2], (float *)ctx->mom[3]);
     timer_stop(ctx, "deposit_nodes");
   } else {
     double *P = (double *)ctx->partials;
-    timer_start(ctx, "deposit_cells");
-    hipLaunchKernelGGL((k_cell_partials<double, true>), nbc, kBlock, 0, ctx->stream, ctx->g, ctx->offsets,
-                       ctx->pd[ctx->cur], ctx->weighted, P);
-    timer_stop(ctx, "deposit_cells");
+    if (cells) {
+      timer_start(ctx, "deposit_cells");
+      hipLaunchKernelGGL((k_cell_partials<double, true, false, false>), nbc, kBlock, 0, ctx->stream, ctx->g, ctx->offsets,
+                         ctx->pd[ctx->cur], ctx->weighted, P, (const uint32_t *)nullptr, PState<double>(), 0.0, 0.0, 0.0, 0.0, (double *)nullptr, (uint32_t *)nullptr);
+      timer_stop(ctx, "deposit_cells");
+    }
     timer_start(ctx, "deposit_nodes");
     hipLaunchKernelGGL((k_node_sum<double>), nb, kBlock, 0, ctx->stream, ctx->g, P, (double *)ctx->mom[0],
                        (double *)ctx->mom[1], (double *)ctx->mom[2], (double *)ctx->mom[3]);
@@ -882,7 +893,7 @@ void deposit_state(chy_ctx *ctx, const PState<double> &st, const uint32_t *offs,
   const unsigned nb = (unsigned)((ctx->g.nc + kBlock - 1) / kBlock);
   const unsigned nbc = (unsigned)((ctx->g.nc + kDepCells - 1) / kDepCells);
   double *P = (double *)ctx->partials;
-  hipLaunchKernelGGL((k_cell_partials<double, true>), nbc, kBlock, 0, ctx->stream, ctx->g, offs, st, weighted, P);
+  hipLaunchKernelGGL((k_cell_partials<double, true, false, false>), nbc, kBlock, 0, ctx->stream, ctx->g, offs, st, weighted, P, (const uint32_t *)nullptr, PState<double>(), 0.0, 0.0, 0.0, 0.0, (double *)nullptr, (uint32_t *)nullptr);
   hipLaunchKernelGGL((k_node_sum<double>), nb, kBlock, 0, ctx->stream, ctx->g, P, out[0], out[1], out[2], out[3]);
 }
 
@@ -907,6 +918,7 @@ int particles_normaliser(chy_ctx *ctx)
 static int settle(chy_ctx *ctx)
 {
   ctx->order_valid = false;  // new data: the general sort
+  ctx->partials_fresh = false;
   launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
   if (ctx->mixed)
     hipLaunchKernelGGL(k_keys_count, grid_for(ctx->np), kBlock, 0, ctx->stream, ctx->np, ctx->pf[ctx->cur].cell,
@@ -1106,6 +1118,7 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
   const int64_t Nc = collide ? collision_count(ctx->prm, ctx->np) : 0;
   pc.Nc = (double)Nc;
   if (collide && Nc == 0) collide = false;
+  ctx->partials_fresh = false;
   launch_fill<unsigned long long>(ctx, 4, ctx->cstats, 0ull);
   // the histogram is only needed by the general sort (the neighbour-move
   // sort counts per old cell)
@@ -1149,6 +1162,7 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
 int particles_move_only(chy_ctx *ctx)
 {
   PushConst pc = push_const(ctx);
+  ctx->partials_fresh = false;
   launch_fill<unsigned long long>(ctx, 4, ctx->cstats, 0ull);
   launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
   const unsigned nb = grid_for(ctx->np);

@@ -25,6 +25,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 KSP_GMRES, KSP_BCGS = 0, 1
 PC_NONE, PC_JACOBI, PC_FFT, PC_ZLINE, PC_CHEBY, PC_FFT2 = 0, 1, 2, 3, 4, 5
 CGS_REFINE_NEVER, CGS_REFINE_IFNEEDED, CGS_REFINE_ALWAYS = 0, 1, 2
+GUESS_PREVIOUS, GUESS_ZERO, GUESS_EXTRAPOLATE = 0, 1, 2
 PERT_NONE, PERT_SINUSOIDAL_X, PERT_UNIFORM, PERT_GAUSSIAN = 0, 1, 2, 3
 
 D3 = C.c_double * 3
@@ -50,7 +51,7 @@ class Params(C.Structure):
         ("ksp_max_it", C.c_int64), ("ksp_restart", C.c_int64),
         ("ksp_monitor", C.c_int32), ("ksp_converged_reason", C.c_int32),
         ("ksp_error_if_not_converged", C.c_int32), ("pc_cheby_degree", C.c_int32),
-        ("ksp_inner_rtol", C.c_double), ("ksp_gmres_cgs_refinement", C.c_int32),
+        ("ksp_inner_rtol", C.c_double), ("ksp_gmres_cgs_refinement", C.c_int32), ("ksp_initial_guess", C.c_int32),
         ("seed", C.c_uint64), ("ppc", C.c_int64), ("perturbation", C.c_int32),
         ("perturbation_amplitude", C.c_double), ("weighted", C.c_int32),
         ("precision", C.c_int32), ("output_interval", C.c_int64),

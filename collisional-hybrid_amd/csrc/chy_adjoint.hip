@@ -315,6 +315,7 @@ int restore_stage(chy_ctx *ctx, const Stage &s)
   ctx->step = s.step;
   ctx->order_valid = s.ordered;
   ctx->partials_fresh = false;
+  ctx->phi_prev_valid = false;
   ctx->phi_valid = true;
   ctx->moments_valid = false;
   return CHY_OK;

@@ -132,6 +132,14 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
   const int lt = ctx->prm.lhs_type;
   double *hdev = ctx->red_out, *h2dev = ctx->red_out + (m + 2), *ndev = ctx->red_out + 2 * (m + 2);
   std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m), h(m + 2), h2(m + 2);
+  // The basis is stored unnormalised: V_q = sc[q] U_q with U_q in the V
+  // slots, so no pass is spent scaling a new vector.  The dot products come
+  // back multiplied by sc[q]^2 (k_combine's scale), which makes them the
+  // coefficients of the U_q in the update; the host turns them into the
+  // Hessenberg entries.  The preconditioned directions follow:
+  // Z_q = sc[q] (M^-1 U_q).
+  std::vector<double> sc(m + 2, 0.0);
+  double *sc2 = ctx->h_scale;  // pinned: sc[q]^2, copied to ctx->gs_scale
   launch_fill<T>(ctx, nc, e, (T)0);
   double beta0 = -1.0, beta_prev = -1.0;
   *its = 0;
@@ -159,7 +167,8 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
     // back to the FP64 outer sweep
     if (sizeof(T) == 4 && beta_prev > 0.0 && beta > 0.95 * beta_prev) return CHY_OK;
     beta_prev = beta;
-    launch_axpby<T>(ctx, nc, 1.0 / beta, V, 0.0, V);
+    sc[0] = 1.0 / beta;
+    sc2[0] = sc[0] * sc[0];
     std::fill(g.begin(), g.end(), 0.0);
     g[0] = beta;
     int64_t j = 0;
@@ -178,35 +187,38 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
       // pass removed more than half of the squared norm (sum h^2 > |w'|^2,
       // with |w|^2 = |w'|^2 + sum h^2)
       const int refine = ctx->prm.ksp_gmres_cgs_refinement;
-      double wn;
-      launch_gs_dots<T>(ctx, (int)(j + 1), V, nc, Vn, nc, hdev);
+      const double sj = sc[j];
+      CHY_HIP(hipMemcpyAsync(ctx->gs_scale, sc2, sizeof(double) * (j + 1), hipMemcpyHostToDevice, ctx->stream));
+      double wn;  // |U_{j+1}| before scaling
+      launch_gs_dots<T>(ctx, (int)(j + 1), V, nc, Vn, nc, hdev, ctx->gs_scale);
       if (refine == CHY_CGS_REFINE_ALWAYS) {
-        launch_gs_update_dots<T>(ctx, (int)(j + 1), V, nc, hdev, Vn, nc, h2dev);
+        launch_gs_update_dots<T>(ctx, (int)(j + 1), V, nc, hdev, Vn, nc, h2dev, ctx->gs_scale);
         launch_gs_update_norm<T>(ctx, (int)(j + 1), V, nc, h2dev, Vn, nc, ndev);
         TRY(fetch(ctx, ctx->red_out, (int)(2 * (m + 2) + 1), ctx->h_red));
-        for (int64_t q = 0; q <= j; q++) h[q] = ctx->h_red[q] + ctx->h_red[(m + 2) + q];
+        for (int64_t q = 0; q <= j; q++) h[q] = sc[q] > 0.0 ? (ctx->h_red[q] + ctx->h_red[(m + 2) + q]) * sj / sc[q] : 0.0;
         wn = sqrt(ctx->h_red[2 * (m + 2)]);
       } else {
         launch_gs_update_norm<T>(ctx, (int)(j + 1), V, nc, hdev, Vn, nc, ndev);
         TRY(fetch(ctx, ctx->red_out, (int)(2 * (m + 2) + 1), ctx->h_red));
         double proj = 0.0;
         for (int64_t q = 0; q <= j; q++) {
-          h[q] = ctx->h_red[q];
+          h[q] = sc[q] > 0.0 ? ctx->h_red[q] * sj / sc[q] : 0.0;
           proj += h[q] * h[q];
         }
         wn = sqrt(ctx->h_red[2 * (m + 2)]);
-        if (refine == CHY_CGS_REFINE_IFNEEDED && proj > wn * wn) {
-          launch_gs_dots<T>(ctx, (int)(j + 1), V, nc, Vn, nc, h2dev);
+        if (refine == CHY_CGS_REFINE_IFNEEDED && proj > wn * sj * wn * sj) {
+          launch_gs_dots<T>(ctx, (int)(j + 1), V, nc, Vn, nc, h2dev, ctx->gs_scale);
           launch_gs_update_norm<T>(ctx, (int)(j + 1), V, nc, h2dev, Vn, nc, ndev);
           TRY(fetch(ctx, ctx->red_out, (int)(2 * (m + 2) + 1), ctx->h_red));
-          for (int64_t q = 0; q <= j; q++) h[q] += ctx->h_red[(m + 2) + q];
+          for (int64_t q = 0; q <= j; q++) h[q] += sc[q] > 0.0 ? ctx->h_red[(m + 2) + q] * sj / sc[q] : 0.0;
           wn = sqrt(ctx->h_red[2 * (m + 2)]);
         }
       }
       timer_stop(ctx, "krylov");
-      h[j + 1] = wn;
+      h[j + 1] = wn * sj;
       if (!std::isfinite(h[j + 1])) return set_err(ctx, CHY_ERR_FP, "non-finite Arnoldi vector");
-      launch_axpby<T>(ctx, nc, h[j + 1] > 0.0 ? 1.0 / h[j + 1] : 0.0, Vn, 0.0, Vn);
+      sc[j + 1] = wn > 0.0 ? 1.0 / wn : 0.0;
+      sc2[j + 1] = sc[j + 1] * sc[j + 1];
       for (int64_t q = 0; q < j; q++) {
         const double t = cs[q] * h[q] + sn[q] * h[q + 1];
         h[q + 1] = -sn[q] * h[q] + cs[q] * h[q + 1];
@@ -230,7 +242,7 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
       y[q] = s / H[q * m + q];
     }
     std::vector<double> negy(j);
-    for (int64_t q = 0; q < j; q++) negy[q] = -y[q];
+    for (int64_t q = 0; q < j; q++) negy[q] = -y[q] * sc[q];
     CHY_HIP(hipMemcpyAsync(hdev, negy.data(), sizeof(double) * j, hipMemcpyHostToDevice, ctx->stream));
     launch_mgs_update<T>(ctx, (int)j, Z, nc, hdev, e, nc);
     CHY_HIP(hipStreamSynchronize(ctx->stream));
@@ -497,6 +509,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   for (int a = 0; a < 4; a++) { unsigned char *q = nullptr; A(dalloc(ctx, &q, (int64_t)ts * nc)); ctx->mom[a] = q; }
   { unsigned char *q = nullptr; A(dalloc(ctx, &q, (int64_t)ts * 4 * nc)); ctx->efield[0] = q; }
   A(dalloc(ctx, &ctx->phi, nc));
+  if (p.ksp_initial_guess == CHY_GUESS_EXTRAPOLATE) A(dalloc(ctx, &ctx->phi_prev, nc));
   A(dalloc(ctx, &ctx->b, nc));
   A(dalloc(ctx, &ctx->r64, nc));
   A(dalloc(ctx, &ctx->t64, nc));
@@ -510,10 +523,12 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   }
   A(dalloc(ctx, &ctx->red, (int64_t)kRedBlocks * 32));
   A(dalloc(ctx, &ctx->red_out, 3 * (ctx->restart + 2) + 16));
-  if (hipHostMalloc((void **)&ctx->h_red, sizeof(double) * (3 * (ctx->restart + 2) + 16)) != hipSuccess) {
+  if (hipHostMalloc((void **)&ctx->h_red, sizeof(double) * (3 * (ctx->restart + 2) + 16)) != hipSuccess ||
+      hipHostMalloc((void **)&ctx->h_scale, sizeof(double) * (ctx->restart + 2)) != hipSuccess) {
     chy_destroy(ctx);
     return CHY_ERR_MEM;
   }
+  A(dalloc(ctx, &ctx->gs_scale, ctx->restart + 2));
   A(fft_setup(ctx));
   if (p.pc_type == CHY_PC_FFT2 && ctx->fft_ok) {
     unsigned char *q = nullptr;
@@ -539,11 +554,12 @@ int chy_destroy(chy_ctx *ctx)
                 ctx->trans, ctx->info, ctx->nbase, ctx->offs_alt, ctx->sortflag,
                 ctx->partials,
                 ctx->sred, ctx->S_dev, ctx->cstats, ctx->mom[0], ctx->mom[1], ctx->mom[2], ctx->mom[3],
-                ctx->efield[0], ctx->phi, ctx->b, ctx->r64, ctx->t64, ctx->V,
-                ctx->Z, ctx->wv, ctx->fft_buf, ctx->pc_buf, ctx->pc2_buf, ctx->pc2_h, ctx->red, ctx->red_out, ctx->tw[0], ctx->tw[1], ctx->tw[2],
+                ctx->efield[0], ctx->phi, ctx->phi_prev, ctx->b, ctx->r64, ctx->t64, ctx->V,
+                ctx->Z, ctx->wv, ctx->fft_buf, ctx->pc_buf, ctx->pc2_buf, ctx->pc2_h, ctx->red, ctx->red_out, ctx->gs_scale, ctx->tw[0], ctx->tw[1], ctx->tw[2],
                 ctx->trig[0], ctx->trig[1], ctx->trig[2]};
   for (void *p : ps) if (p) (void)hipFree(p);
   if (ctx->h_red) (void)hipHostFree(ctx->h_red);
+  if (ctx->h_scale) (void)hipHostFree(ctx->h_scale);
   for (auto &kv : ctx->timers) {
     for (hipEvent_t e : kv.second.a) (void)hipEventDestroy(e);
     for (hipEvent_t e : kv.second.b) (void)hipEventDestroy(e);
@@ -762,6 +778,7 @@ int chy_set_potential(chy_ctx *ctx, const double *phi)
   if (!ctx || !phi) return CHY_ERR_ARG_WRONG;
   CHY_HIP(hipMemcpy(ctx->phi, phi, sizeof(double) * ctx->g.nc, hipMemcpyHostToDevice));
   ctx->phi_valid = true;
+  ctx->phi_prev_valid = false;
   return CHY_OK;
 }
 
@@ -803,6 +820,21 @@ int chy::ctx_step(chy_ctx *ctx, chy_step_stats *st)
   const auto t1 = std::chrono::steady_clock::now();
   rc = do_rhs(ctx);
   if (rc) return rc;
+  // initial phi of the solve (the adjoint's tape records the previous-phi start)
+  const int guess = ctx->adj ? CHY_GUESS_PREVIOUS : ctx->prm.ksp_initial_guess;
+  if (guess == CHY_GUESS_ZERO) {
+    launch_fill<double>(ctx, ctx->g.nc, ctx->phi, 0.0);
+  } else if (guess == CHY_GUESS_EXTRAPOLATE && ctx->phi_prev) {
+    const size_t bytes = sizeof(double) * ctx->g.nc;
+    if (ctx->phi_prev_valid) {  // phi <- 2 phi - phi_prev, phi_prev <- old phi
+      CHY_HIP(hipMemcpyAsync(ctx->t64, ctx->phi, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+      launch_axpby<double>(ctx, ctx->g.nc, -1.0, ctx->phi_prev, 2.0, ctx->phi);
+      CHY_HIP(hipMemcpyAsync(ctx->phi_prev, ctx->t64, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+      CHY_HIP(hipMemcpyAsync(ctx->phi_prev, ctx->phi, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+      ctx->phi_prev_valid = true;
+    }
+  }
   rc = do_solve(ctx, &local.solve);
   if (rc) return rc;
   const auto t2 = std::chrono::steady_clock::now();

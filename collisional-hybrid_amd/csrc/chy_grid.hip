@@ -561,9 +561,10 @@ __global__ __launch_bounds__(256) void k_mdot(int nv, const T *__restrict__ A, i
   }
 }
 
-// out[k] = sum_blk partial[blk*nv + k], fixed tree order.
+// out[k] = sum_blk partial[blk*nv + k], fixed tree order, times scale[k]
+// when a scale is given.
 __global__ __launch_bounds__(256) void k_combine(const double *__restrict__ partial, int nblk, int nv,
-                                                 double *__restrict__ out)
+                                                 double *__restrict__ out, const double *__restrict__ scale)
 {
   __shared__ double sh[4];
   for (int k = 0; k < nv; k++) {
@@ -572,18 +573,22 @@ __global__ __launch_bounds__(256) void k_combine(const double *__restrict__ part
     const double s = wave_sum(acc);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) out[k] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    if (threadIdx.x == 0) {
+      const double t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+      out[k] = scale ? t * scale[k] : t;
+    }
     __syncthreads();
   }
 }
 
 void launch_combine_sum(chy_ctx *ctx, const double *partials, int nblk, int nv, double *out)
 {
-  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, partials, nblk, nv, out);
+  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, partials, nblk, nv, out, (const double *)nullptr);
 }
 
 template <typename T>
-void launch_mdot(chy_ctx *ctx, int nv, const T *A, int64_t stride, const T *b, int64_t n, double *out_dev)
+void launch_mdot(chy_ctx *ctx, int nv, const T *A, int64_t stride, const T *b, int64_t n, double *out_dev,
+                 const double *scale)
 {
   for (int k0 = 0; k0 < nv; k0 += kMaxDots) {
     const int m = (nv - k0 < kMaxDots) ? nv - k0 : kMaxDots;
@@ -595,7 +600,8 @@ void launch_mdot(chy_ctx *ctx, int nv, const T *A, int64_t stride, const T *b, i
     else
       hipLaunchKernelGGL((k_mdot<T, kMaxDots>), nb, kBlock, 0, ctx->stream, m, A + k0 * stride, stride, b, n,
                          ctx->red);
-    hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, m, out_dev + k0);
+    hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, m, out_dev + k0,
+                       scale ? scale + k0 : nullptr);
   }
 }
 
@@ -617,7 +623,7 @@ void launch_sum(chy_ctx *ctx, const T *a, int64_t n, double *out_dev)
 {
   const unsigned nb = (unsigned)std::min<int64_t>(kRedBlocks, (n + kBlock - 1) / kBlock);
   hipLaunchKernelGGL((k_sum<T>), nb, kBlock, 0, ctx->stream, a, n, ctx->red);
-  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, 1, out_dev);
+  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, 1, out_dev, (const double *)nullptr);
 }
 
 // w -= sum_k h_k V_k  (h on the device; accumulation in FP64)
@@ -692,7 +698,7 @@ void launch_mgs_norm(chy_ctx *ctx, int nv, const T *V, int64_t stride, const dou
     hipLaunchKernelGGL((k_mgs_norm<T, 8>), nb, kBlock, 0, ctx->stream, nv, V, stride, h, w, n, ctx->red);
   else
     hipLaunchKernelGGL((k_mgs_norm<T, kMaxDots>), nb, kBlock, 0, ctx->stream, nv, V, stride, h, w, n, ctx->red);
-  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, 1, norm2_dev);
+  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, 1, norm2_dev, (const double *)nullptr);
 }
 
 // ---- Gram-Schmidt passes with 16-byte vector loads (n % 4 == 0) -------------------
@@ -791,28 +797,29 @@ static inline bool quad_ok(int64_t n, int64_t stride, const void *a, const void 
 }
 
 template <typename T>
-void launch_gs_dots(chy_ctx *ctx, int nv, const T *V, int64_t stride, const T *w, int64_t n, double *out_dev)
+void launch_gs_dots(chy_ctx *ctx, int nv, const T *V, int64_t stride, const T *w, int64_t n, double *out_dev,
+                    const double *scale)
 {
-  if (nv > 8 || !quad_ok(n, stride, V, w)) { launch_mdot<T>(ctx, nv, V, stride, w, n, out_dev); return; }
+  if (nv > 8 || !quad_ok(n, stride, V, w)) { launch_mdot<T>(ctx, nv, V, stride, w, n, out_dev, scale); return; }
   const int64_t n4 = n / 4;
   const unsigned nb = (unsigned)std::min<int64_t>(kRedBlocks, (n4 + kBlock - 1) / kBlock);
   hipLaunchKernelGGL((k_mdot4<T, 8>), nb, kBlock, 0, ctx->stream, nv, V, stride, w, n4, ctx->red);
-  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, nv, out_dev);
+  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, nv, out_dev, scale);
 }
 
 template <typename T>
 void launch_gs_update_dots(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n,
-                           double *out_dev)
+                           double *out_dev, const double *scale)
 {
   if (nv > 8 || !quad_ok(n, stride, V, w)) {
     launch_mgs_update<T>(ctx, nv, V, stride, h, w, n);
-    launch_mdot<T>(ctx, nv, V, stride, w, n, out_dev);
+    launch_mdot<T>(ctx, nv, V, stride, w, n, out_dev, scale);
     return;
   }
   const int64_t n4 = n / 4;
   const unsigned nb = (unsigned)std::min<int64_t>(kRedBlocks, (n4 + kBlock - 1) / kBlock);
   hipLaunchKernelGGL((k_gs4<T, 8, false>), nb, kBlock, 0, ctx->stream, nv, V, stride, h, w, n4, ctx->red);
-  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, nv, out_dev);
+  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, nv, out_dev, scale);
 }
 
 template <typename T>
@@ -823,7 +830,7 @@ void launch_gs_update_norm(chy_ctx *ctx, int nv, const T *V, int64_t stride, con
   const int64_t n4 = n / 4;
   const unsigned nb = (unsigned)std::min<int64_t>(kRedBlocks, (n4 + kBlock - 1) / kBlock);
   hipLaunchKernelGGL((k_gs4<T, 8, true>), nb, kBlock, 0, ctx->stream, nv, V, stride, h, w, n4, ctx->red);
-  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, 1, norm2_dev);
+  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, 1, norm2_dev, (const double *)nullptr);
 }
 
 template <typename T>
@@ -1200,20 +1207,24 @@ template void launch_efield<double>(chy_ctx *, const double *, double *);
 template void launch_efield<float>(chy_ctx *, const double *, float *);
 template void launch_rhs<double>(chy_ctx *, int, const double *, const double *, const double *, const double *, double *);
 template void launch_rhs<float>(chy_ctx *, int, const float *, const float *, const float *, const float *, double *);
-template void launch_mdot<double>(chy_ctx *, int, const double *, int64_t, const double *, int64_t, double *);
-template void launch_mdot<float>(chy_ctx *, int, const float *, int64_t, const float *, int64_t, double *);
+template void launch_mdot<double>(chy_ctx *, int, const double *, int64_t, const double *, int64_t, double *,
+                                  const double *);
+template void launch_mdot<float>(chy_ctx *, int, const float *, int64_t, const float *, int64_t, double *,
+                                 const double *);
 template void launch_sum<double>(chy_ctx *, const double *, int64_t, double *);
 template void launch_sum<float>(chy_ctx *, const float *, int64_t, double *);
 template void launch_mgs_update<double>(chy_ctx *, int, const double *, int64_t, const double *, double *, int64_t);
 template void launch_mgs_update<float>(chy_ctx *, int, const float *, int64_t, const double *, float *, int64_t);
 template void launch_mgs_norm<double>(chy_ctx *, int, const double *, int64_t, const double *, double *, int64_t, double *);
 template void launch_mgs_norm<float>(chy_ctx *, int, const float *, int64_t, const double *, float *, int64_t, double *);
-template void launch_gs_dots<double>(chy_ctx *, int, const double *, int64_t, const double *, int64_t, double *);
-template void launch_gs_dots<float>(chy_ctx *, int, const float *, int64_t, const float *, int64_t, double *);
+template void launch_gs_dots<double>(chy_ctx *, int, const double *, int64_t, const double *, int64_t, double *,
+                                     const double *);
+template void launch_gs_dots<float>(chy_ctx *, int, const float *, int64_t, const float *, int64_t, double *,
+                                    const double *);
 template void launch_gs_update_dots<double>(chy_ctx *, int, const double *, int64_t, const double *, double *, int64_t,
-                                            double *);
+                                            double *, const double *);
 template void launch_gs_update_dots<float>(chy_ctx *, int, const float *, int64_t, const double *, float *, int64_t,
-                                           double *);
+                                           double *, const double *);
 template void launch_gs_update_norm<double>(chy_ctx *, int, const double *, int64_t, const double *, double *, int64_t,
                                             double *);
 template void launch_gs_update_norm<float>(chy_ctx *, int, const float *, int64_t, const double *, float *, int64_t,

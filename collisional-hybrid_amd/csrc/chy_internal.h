@@ -76,6 +76,8 @@ struct chy_ctx {
   // and every ion moved at most one cell in each direction)
   bool order_valid = false;     // particle order = stable cell order of `offsets`
   bool partials_fresh = false;  // the sort made the deposit's cell partials of the current state
+  double *phi_prev = nullptr;   // the previous step's phi (-ksp_initial_guess extrapolate)
+  bool phi_prev_valid = false;
   uint8_t *trans = nullptr;     // [27][nc] ions of each old cell per move direction
   uint16_t *info = nullptr;     // per ion: direction << 8 | rank among the cell's ions of that direction
   uint16_t *nbase = nullptr;    // [27][nc] start of each (old cell, direction) group in its new cell
@@ -99,6 +101,8 @@ struct chy_ctx {
   double *red = nullptr;        // reduction partials [nblk][restart+2]
   double *red_out = nullptr;    // reduced values [restart+2]
   double *h_red = nullptr;      // pinned host mirror of red_out
+  double *h_scale = nullptr;    // pinned: squared scales of the Krylov basis (GMRES)
+  double *gs_scale = nullptr;   // their device copy
   // FFT preconditioner
   void *fft_buf = nullptr;      // complex work array
   void *pc_buf = nullptr;       // z-line / Chebyshev / two-level scratch, 3 vectors (T)
@@ -190,7 +194,8 @@ template <typename T>
 void launch_rhs(chy_ctx *ctx, int rhs_type, const T *n, const T *gx, const T *gy, const T *gz, double *b);
 // reductions: out[k] = sum_i a_k[i] * b[i] for k < nv (a_k = A + k*stride)
 template <typename T>
-void launch_mdot(chy_ctx *ctx, int nv, const T *A, int64_t stride, const T *b, int64_t n, double *out_dev);
+void launch_mdot(chy_ctx *ctx, int nv, const T *A, int64_t stride, const T *b, int64_t n, double *out_dev,
+                 const double *scale = nullptr);  // out[k] *= scale[k] when given
 template <typename T>
 void launch_sum(chy_ctx *ctx, const T *a, int64_t n, double *out_dev);
 // y -= sum_k h_k V_k (h on device, FP64) ; optional normalisation
@@ -203,10 +208,11 @@ void launch_mgs_norm(chy_ctx *ctx, int nv, const T *V, int64_t stride, const dou
 // nv > 8 or n not a multiple of 4): dots V^T w; w -= V h then V^T w;
 // w -= V h then ||w||^2
 template <typename T>
-void launch_gs_dots(chy_ctx *ctx, int nv, const T *V, int64_t stride, const T *w, int64_t n, double *out_dev);
+void launch_gs_dots(chy_ctx *ctx, int nv, const T *V, int64_t stride, const T *w, int64_t n, double *out_dev,
+                    const double *scale = nullptr);
 template <typename T>
 void launch_gs_update_dots(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n,
-                           double *out_dev);
+                           double *out_dev, const double *scale = nullptr);
 template <typename T>
 void launch_gs_update_norm(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n,
                            double *norm2_dev);

@@ -169,6 +169,7 @@ static const char *const PertNames[] = {"none", "sinusoidal-x", "uniform", "gaus
 static const char *const PrecNames[] = {"fp64", "mixed", NULL};
 /* PETSc KSPGMRESCGSRefinementType names */
 static const char *const CgsRefineNames[] = {"REFINE_NEVER", "REFINE_IFNEEDED", "REFINE_ALWAYS", NULL};
+static const char *const GuessNames[] = {"previous", "zero", "extrapolate", NULL};
 
 int chy_params_default(chy_params *p)
 {
@@ -389,6 +390,7 @@ int chy_params_from_options(int argc, const char *const *argv, chy_params *p, ch
   get_int(&db, "-ksp_gmres_restart", &p->ksp_restart, &e);
   get_real(&db, "-ksp_inner_rtol", &p->ksp_inner_rtol, &e);
   get_enum(&db, "-ksp_gmres_cgs_refinement_type", CgsRefineNames, &p->ksp_gmres_cgs_refinement, &e);
+  get_enum(&db, "-ksp_initial_guess", GuessNames, &p->ksp_initial_guess, &e);
   {
     int64_t deg = p->pc_cheby_degree;
     if (get_int(&db, "-pc_cheby_degree", &deg, &e)) {

@@ -53,6 +53,10 @@ enum { CHY_PC_NONE = 0, CHY_PC_JACOBI = 1, CHY_PC_FFT = 2, CHY_PC_ZLINE = 3, CHY
 /* -ksp_gmres_cgs_refinement_type: REFINE_NEVER, REFINE_IFNEEDED (when the
  * first pass removes more than half of the vector's squared norm), REFINE_ALWAYS */
 enum { CHY_CGS_REFINE_NEVER = 0, CHY_CGS_REFINE_IFNEEDED = 1, CHY_CGS_REFINE_ALWAYS = 2 };
+/* -ksp_initial_guess: the previous step's phi (default), zero (the reference's
+ * ComputeInitialPhi, potential.c:20-29), or the linear extrapolation
+ * 2 phi_n - phi_(n-1) of the last two steps */
+enum { CHY_GUESS_PREVIOUS = 0, CHY_GUESS_ZERO = 1, CHY_GUESS_EXTRAPOLATE = 2 };
 enum { CHY_PERT_NONE = 0, CHY_PERT_SINUSOIDAL_X = 1, CHY_PERT_UNIFORM = 2, CHY_PERT_GAUSSIAN = 3 };
 
 /* ---- run parameters: the scalars of the reference Context (hybrid.h:106-138)
@@ -91,6 +95,7 @@ typedef struct {
   int32_t pc_cheby_degree; /* Chebyshev-Jacobi preconditioner: polynomial degree */
   double  ksp_inner_rtol;  /* mixed precision: floor of the FP32 inner relative tolerance */
   int32_t ksp_gmres_cgs_refinement; /* CHY_CGS_REFINE_*: second Gram-Schmidt pass (PETSc KSPGMRESCGSRefinementType) */
+  int32_t ksp_initial_guess; /* CHY_GUESS_*: initial phi of each step's solve */
   /* build additions: seed, particles per cell, perturbation, precision, output cadence */
   uint64_t seed;
   int64_t ppc;

@@ -197,3 +197,24 @@ def test_mixed_step_tracks_fp64_step():
     assert np.array_equal(a[0].pid, b[0].pid) or np.mean(a[0].pid == b[0].pid) > 0.999
     assert relmax(b[1][:, 0], a[1][:, 0]) <= 1e-5
     assert relmax(b[2], a[2]) <= 1e-3
+
+
+def test_initial_guess_modes_reach_the_same_potential():
+    """-ksp_initial_guess previous / zero (the reference's ComputeInitialPhi) /
+    extrapolate: the same steps give the same phi to the solve tolerance."""
+    extra = "-seed 5 -ppc 8 -ksp_rtol 1e-12 --perturbation uniform --perturbation-amplitude 0.01 -mn 0"
+    phis = {}
+    for mode in ("previous", "zero", "extrapolate"):
+        with context(opts((8, 8, 8), extra=extra + f" -ksp_initial_guess {mode}")) as ctx:
+            ctx.init_particles()
+            ctx.collect_moments(download=False)
+            ctx.solve_potential()
+            out = []
+            for _ in range(3):
+                st = ctx.step()
+                assert st.solve.converged == 1, mode
+                out.append(ctx.get_potential())
+            phis[mode] = out
+    for mode in ("zero", "extrapolate"):
+        for a, b in zip(phis[mode], phis["previous"]):
+            assert relmax(a, b) <= 1e-8, (mode, relmax(a, b))

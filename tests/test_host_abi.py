@@ -118,6 +118,11 @@ def test_solver_option_mapping():
         assert p.ksp_gmres_cgs_refinement == v, name
     with pytest.raises(chybrid.ChyError):
         chybrid.params_from_options("-ksp_gmres_cgs_refinement_type sometimes".split())
+    assert p.ksp_initial_guess == chybrid.GUESS_PREVIOUS
+    for name, v in (("previous", chybrid.GUESS_PREVIOUS), ("zero", chybrid.GUESS_ZERO),
+                    ("extrapolate", chybrid.GUESS_EXTRAPOLATE)):
+        p, _ = chybrid.params_from_options(["-ksp_initial_guess", name])
+        assert p.ksp_initial_guess == v, name
     with pytest.raises(chybrid.ChyError):
         chybrid.params_from_options("-pc_cheby_degree 0".split())
 

@@ -159,6 +159,9 @@ def run_gpu(N, ppc, prec, steps, warmup, keep=False, cross=None):
             ctx.set_moments(cross["mom"])
             kept = kept or {}
             kept["cross_phi"], _ = ctx.solve(cross["rhs"])
+            # and a near-exact FP64 answer (relative residual 1e-12) to measure both against
+            ctx.set_tolerances(rtol=1e-12)
+            kept["exact_phi"], _ = ctx.solve(cross["rhs"])
         copy = None
         if prec == "mixed":
             ctx.timing_reset()
@@ -307,7 +310,11 @@ def precision_errors(m, f, N):
     wm, wf = _field_energy(m["phi"], shape, d), _field_energy(f["phi"], shape, d)
     wc = _field_energy(f["cross_phi"], shape, d)
     return {
-        "same_system": {"phi": rel(m["phi"], f["cross_phi"]), "field_energy": abs(wm - wc) / wc},
+        "same_system": {"phi": rel(m["phi"], f["cross_phi"]), "field_energy": abs(wm - wc) / wc,
+                        # both against the FP64 solve to 1e-12: the mixed path's own error, and
+                        # the error of an FP64 solve stopped at the same tolerance
+                        "phi_mixed_vs_fp64_1e-12": rel(m["phi"], f["exact_phi"]),
+                        "phi_fp64_vs_fp64_1e-12": rel(f["cross_phi"], f["exact_phi"])},
         "after_steps": {"phi": rel(m["phi"], f["phi"]), "density": rel(m["mom"][:, 0], f["mom"][:, 0]),
                         "flux": rel(m["mom"][:, 1:], f["mom"][:, 1:]), "field_energy": abs(wm - wf) / wf},
     }

@@ -23,7 +23,7 @@ LHS_IDENTITY, LHS_LAPLACIAN, LHS_FULL = 0, 1, 2
 RHS_CONSTANT, RHS_SINUSOIDAL, RHS_FULL = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 KSP_GMRES, KSP_BCGS = 0, 1
-PC_NONE, PC_JACOBI, PC_FFT, PC_ZLINE, PC_CHEBY, PC_FFT2 = 0, 1, 2, 3, 4, 5
+PC_NONE, PC_JACOBI, PC_FFT, PC_ZLINE, PC_CHEBY, PC_FFT2, PC_FFT2Z = 0, 1, 2, 3, 4, 5, 6
 CGS_REFINE_NEVER, CGS_REFINE_IFNEEDED, CGS_REFINE_ALWAYS = 0, 1, 2
 GUESS_PREVIOUS, GUESS_ZERO, GUESS_EXTRAPOLATE = 0, 1, 2
 PERT_NONE, PERT_SINUSOIDAL_X, PERT_UNIFORM, PERT_GAUSSIAN = 0, 1, 2, 3
@@ -88,6 +88,7 @@ SIGNATURES = {
     "chy_get_params": (C.c_int, [P, P]),
     "chy_num_particles": (C.c_int64, [P]),
     "chy_set_step": (C.c_int, [P, C.c_int64]),
+    "chy_set_tolerances": (C.c_int, [P, C.c_double, C.c_double, C.c_int64]),
     "chy_set_particles": (C.c_int, [P, C.c_int64, P, P, P, P]),
     "chy_get_particles": (C.c_int, [P, P, P, P, P]),
     "chy_init_particles": (C.c_int, [P]),
@@ -235,6 +236,9 @@ class Context:
 
     def set_step(self, it):
         self._chk(lib().chy_set_step(self._p, int(it)))
+
+    def set_tolerances(self, rtol=-1.0, atol=-1.0, max_it=-1):
+        self._chk(lib().chy_set_tolerances(self._p, float(rtol), float(atol), int(max_it)))
 
     # hot path ---------------------------------------------------------------
     def update_velocities(self):

@@ -94,10 +94,18 @@ int pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
   int rc = CHY_OK;
   const int pc = ctx->prm.pc_type;
   T *sb = (T *)ctx->pc_buf;
-  const bool fftlike = pc == CHY_PC_FFT || pc == CHY_PC_FFT2;
+  const bool fftlike = pc == CHY_PC_FFT || pc == CHY_PC_FFT2 || pc == CHY_PC_FFT2Z;
   if (fftlike && ctx->fft_ok && ctx->prm.lhs_type != CHY_LHS_IDENTITY) {
-    if (pc == CHY_PC_FFT2) rc = fft2_pc_apply<T>(ctx, v19, r, z, sb);
-    else rc = fft_pc_apply<T>(ctx, v19, r, z);
+    if (pc == CHY_PC_FFT) {
+      rc = fft_pc_apply<T>(ctx, v19, r, z);
+    } else {
+      rc = fft2_pc_apply<T>(ctx, v19, r, z, sb);
+      // fft2z: z += zline(r - A z) (built for A(n); transposed solves keep fft2)
+      if (rc == CHY_OK && pc == CHY_PC_FFT2Z && !ctx->solve_transpose && ctx->zl_buf) {
+        launch_residual_t<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), r, z, sb);
+        launch_zline_apply<T>(ctx, (const T *)ctx->zl_buf, ctx->zl_col, sb, z);
+      }
+    }
   } else if (pc == CHY_PC_ZLINE || (fftlike && !ctx->fft_ok)) {
     // grids the FFT does not take fall back to the z-line blocks
     launch_zline<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), r, z, sb, sb + nc);
@@ -334,8 +342,11 @@ int pc_symbol(chy_ctx *ctx, double v19[19])
   TRY(fetch(ctx, ctx->red_out, 1, &s));
   uniform_row(ctx->prm.lhs_type, ctx->prm.d, ctx->prm.kappa_e, s / (double)nc, v19);
   // coefficients of the z-averaged operator for the two-level preconditioner
-  if (ctx->prm.pc_type == CHY_PC_FFT2 && ctx->fft_ok && ctx->pc2_buf)
-    launch_zcoef<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), (T *)ctx->pc2_buf);
+  const bool two = ctx->prm.pc_type == CHY_PC_FFT2 || ctx->prm.pc_type == CHY_PC_FFT2Z;
+  if (two && ctx->fft_ok && ctx->pc2_buf) launch_zcoef<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), (T *)ctx->pc2_buf);
+  // and the z-line factors of the fft2z correction
+  if (ctx->prm.pc_type == CHY_PC_FFT2Z && ctx->fft_ok && ctx->zl_buf)
+    launch_zline_factor<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), (T *)ctx->zl_buf, ctx->zl_col);
   return CHY_OK;
 }
 
@@ -530,11 +541,17 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   }
   A(dalloc(ctx, &ctx->gs_scale, ctx->restart + 2));
   A(fft_setup(ctx));
-  if (p.pc_type == CHY_PC_FFT2 && ctx->fft_ok) {
+  if ((p.pc_type == CHY_PC_FFT2 || p.pc_type == CHY_PC_FFT2Z) && ctx->fft_ok) {
     unsigned char *q = nullptr;
     A(dalloc(ctx, &q, (int64_t)ts * fft2_buf_elems(ctx)));
     ctx->pc2_buf = q;
     A(dalloc(ctx, &ctx->pc2_h, fft2_h_elems(ctx)));
+    if (p.pc_type == CHY_PC_FFT2Z && ctx->g.nz >= 3) {
+      unsigned char *zq = nullptr;
+      A(dalloc(ctx, &zq, (int64_t)ts * 4 * nc));
+      ctx->zl_buf = zq;
+      A(dalloc(ctx, &ctx->zl_col, 2 * (int64_t)ctx->g.nx * ctx->g.ny));
+    }
   }
   launch_fill<double>(ctx, nc, ctx->phi, 0.0);
   launch_fill<double>(ctx, 1, ctx->S_dev, 0.0);
@@ -555,7 +572,7 @@ int chy_destroy(chy_ctx *ctx)
                 ctx->partials,
                 ctx->sred, ctx->S_dev, ctx->cstats, ctx->mom[0], ctx->mom[1], ctx->mom[2], ctx->mom[3],
                 ctx->efield[0], ctx->phi, ctx->phi_prev, ctx->b, ctx->r64, ctx->t64, ctx->V,
-                ctx->Z, ctx->wv, ctx->fft_buf, ctx->pc_buf, ctx->pc2_buf, ctx->pc2_h, ctx->red, ctx->red_out, ctx->gs_scale, ctx->tw[0], ctx->tw[1], ctx->tw[2],
+                ctx->Z, ctx->wv, ctx->fft_buf, ctx->pc_buf, ctx->pc2_buf, ctx->pc2_h, ctx->zl_buf, ctx->zl_col, ctx->red, ctx->red_out, ctx->gs_scale, ctx->tw[0], ctx->tw[1], ctx->tw[2],
                 ctx->trig[0], ctx->trig[1], ctx->trig[2]};
   for (void *p : ps) if (p) (void)hipFree(p);
   if (ctx->h_red) (void)hipHostFree(ctx->h_red);
@@ -583,6 +600,16 @@ int chy_get_params(const chy_ctx *ctx, chy_params *prm)
 }
 
 int64_t chy_num_particles(const chy_ctx *ctx) { return ctx ? ctx->np : -1; }
+
+int chy_set_tolerances(chy_ctx *ctx, double rtol, double atol, int64_t max_it)
+{
+  if (!ctx) return CHY_ERR_ARG_WRONG;
+  if (!(rtol < 1.0) || !(atol == atol)) return CHY_ERR_ARG_RANGE;  // NaN or rtol >= 1
+  if (rtol >= 0.0) ctx->prm.ksp_rtol = rtol;
+  if (atol >= 0.0) ctx->prm.ksp_atol = atol;
+  if (max_it >= 0) ctx->prm.ksp_max_it = max_it;
+  return CHY_OK;
+}
 
 int chy_set_step(chy_ctx *ctx, int64_t it)
 {

@@ -1134,6 +1134,156 @@ void launch_zline(chy_ctx *ctx, int lhs_type, const T *n, const T *r, T *z, T *c
   hipLaunchKernelGGL((k_zline<T>), nblocks(plane), kBlock, 0, ctx->stream, ctx->g, lhs_type, s, L, n, r, z, cp, q);
 }
 
+// ---- z-line correction of the two-level preconditioner (fft2z) ----------------------
+// The cyclic tridiagonal system of every z-column (the diagonal and +-z entries
+// of A(n), as k_zline), factored once per solve: za = the -z entry, zi = 1 /
+// pivot, zc = the eliminated +z entry, zq = the Sherman-Morrison vector of the
+// periodic corner; per column cag = a0 / gamma and cden = 1 + q0 + cag q_(m-1).
+// An application is then forward and back substitution and the rank-one
+// correction, three passes up a column with the loads of KC planes in flight.
+template <typename T>
+__global__ __launch_bounds__(256) void k_zline_factor(Grid g, int lhs_type, SCoef s, LapRow lap,
+                                                      const T *__restrict__ n, T *__restrict__ za,
+                                                      T *__restrict__ zi, T *__restrict__ zc, T *__restrict__ zq,
+                                                      double *__restrict__ cag, double *__restrict__ cden)
+{
+  using C = double;
+  const int64_t plane = (int64_t)g.nx * g.ny;
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= plane) return;
+  const int i = (int)(col % g.nx), j = (int)(col / g.nx), m = g.nz;  // m >= 3
+  C v[19];
+  row_coeffs<C, T>(g, lhs_type, s, lap.v, n, i, j, 0, v);
+  const C a0 = v[5], b0 = v[18], c0 = v[4];
+  row_coeffs<C, T>(g, lhs_type, s, lap.v, n, i, j, m - 1, v);
+  const C al = v[5], bl = v[18], cl = v[4];
+  const C gm = -b0, ag = a0 / gm;
+  C cpp = 0, qp = 0;
+  for (int k = 0; k < m; k++) {
+    C a, b, c;
+    if (k == 0) { a = a0; b = b0 - gm; c = c0; }
+    else if (k == m - 1) { a = al; b = bl - cl * a0 / gm; c = cl; }
+    else {
+      row_coeffs<C, T>(g, lhs_type, s, lap.v, n, i, j, k, v);
+      a = v[5]; b = v[18]; c = v[4];
+    }
+    const int64_t e = col + plane * k;
+    const C uk = k == 0 ? gm : (k == m - 1 ? cl : (C)0);
+    const C den = k == 0 ? b : b - a * cpp;
+    const C cpk = c / den;
+    const C qk = k == 0 ? uk / den : (uk - a * qp) / den;
+    za[e] = (T)a; zi[e] = (T)((C)1 / den); zc[e] = (T)cpk; zq[e] = (T)qk;
+    cpp = cpk; qp = qk;
+  }
+  const C qlast = qp;
+  C qn = qp;
+  for (int k = m - 2; k >= 0; k--) {
+    const int64_t e = col + plane * k;
+    qn = (C)zq[e] - (C)zc[e] * qn;
+    zq[e] = (T)qn;
+  }
+  cag[col] = ag;
+  cden[col] = (C)1 + qn + ag * qlast;
+}
+
+// z += the z-line solve of res (res is overwritten by the intermediate y)
+template <typename T>
+__global__ __launch_bounds__(256) void k_zline_apply(Grid g, const T *__restrict__ za, const T *__restrict__ zi,
+                                                     const T *__restrict__ zc, const T *__restrict__ zq,
+                                                     const double *__restrict__ cag, const double *__restrict__ cden,
+                                                     T *__restrict__ res, T *__restrict__ z)
+{
+  constexpr int KC = 16;
+  const int64_t plane = (int64_t)g.nx * g.ny;
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= plane) return;
+  const int m = g.nz;
+  double y = 0.0;
+  for (int k0 = 0; k0 < m; k0 += KC) {  // forward substitution
+    T rr[KC], aa[KC], ii[KC];
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+      const int64_t e = col + plane * (k0 + u);
+      if (k0 + u < m) { rr[u] = res[e]; aa[u] = za[e]; ii[u] = zi[e]; }
+    }
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+      if (k0 + u < m) {
+        y = ((double)rr[u] - (double)aa[u] * y) * (double)ii[u];
+        res[col + plane * (k0 + u)] = (T)y;
+      }
+    }
+  }
+  const double ylast = y;
+  double yn = y;
+  for (int k0 = m - 2; k0 >= 0; k0 -= KC) {  // back substitution, planes k0, k0 - 1, ...
+    T rr[KC], cc[KC];
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+      const int k = k0 - u;
+      if (k >= 0) { rr[u] = res[col + plane * k]; cc[u] = zc[col + plane * k]; }
+    }
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        yn = (double)rr[u] - (double)cc[u] * yn;
+        res[col + plane * k] = (T)yn;
+      }
+    }
+  }
+  const double f = (yn + cag[col] * ylast) / cden[col];
+  for (int k0 = 0; k0 < m; k0 += KC) {  // periodic correction, added to z
+    T rr[KC], qq[KC], zz[KC];
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+      const int64_t e = col + plane * (k0 + u);
+      if (k0 + u < m) { rr[u] = res[e]; qq[u] = zq[e]; zz[u] = z[e]; }
+    }
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+      if (k0 + u < m) z[col + plane * (k0 + u)] = (T)((double)zz[u] + (double)rr[u] - f * (double)qq[u]);
+    }
+  }
+}
+
+template <typename T>
+void launch_zline_factor(chy_ctx *ctx, int lhs_type, const T *n, T *zbuf, double *colbuf)
+{
+  const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
+  const LapRow L = lap_row(ctx, lhs_type);
+  const int64_t nc = ctx->g.nc, plane = (int64_t)ctx->g.nx * ctx->g.ny;
+  hipLaunchKernelGGL((k_zline_factor<T>), nblocks(plane), kBlock, 0, ctx->stream, ctx->g, lhs_type, s, L, n, zbuf,
+                     zbuf + nc, zbuf + 2 * nc, zbuf + 3 * nc, colbuf, colbuf + plane);
+}
+
+template <typename T>
+void launch_zline_apply(chy_ctx *ctx, const T *zbuf, const double *colbuf, T *res, T *z)
+{
+  const int64_t nc = ctx->g.nc, plane = (int64_t)ctx->g.nx * ctx->g.ny;
+  hipLaunchKernelGGL((k_zline_apply<T>), nblocks(plane), kBlock, 0, ctx->stream, ctx->g, zbuf, zbuf + nc,
+                     zbuf + 2 * nc, zbuf + 3 * nc, colbuf, colbuf + plane, res, z);
+}
+
+// res = b - A(n) x in precision T
+template <typename T>
+void launch_residual_t(chy_ctx *ctx, int lhs_type, const T *n, const T *b, const T *x, T *res)
+{
+  const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
+  const LapRow L = lap_row(ctx, lhs_type);
+  constexpr bool fp32 = sizeof(T) == 4;
+  using C = typename std::conditional<fp32, float, double>::type;
+  hipLaunchKernelGGL((k_lhs_tiled<T, T, T, C, fp32, 1, T>), tiled_grid(ctx->g), kBlock, 0, ctx->stream, ctx->g,
+                     lhs_type, s, L, n, x, b, res, (double *)nullptr);
+}
+
+template void launch_zline_factor<double>(chy_ctx *, int, const double *, double *, double *);
+template void launch_zline_factor<float>(chy_ctx *, int, const float *, float *, double *);
+template void launch_zline_apply<double>(chy_ctx *, const double *, const double *, double *, double *);
+template void launch_zline_apply<float>(chy_ctx *, const float *, const double *, float *, float *);
+template void launch_residual_t<double>(chy_ctx *, int, const double *, const double *, const double *, double *);
+template void launch_residual_t<float>(chy_ctx *, int, const float *, const float *, const float *, float *);
+
 // Chebyshev iteration on D^-1 A z = D^-1 r from z = 0 (three-term recurrence);
 // the diagonal is rebuilt from n in each pass.
 template <typename T>

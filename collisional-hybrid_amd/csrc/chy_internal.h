@@ -108,6 +108,8 @@ struct chy_ctx {
   void *pc_buf = nullptr;       // z-line / Chebyshev / two-level scratch, 3 vectors (T)
   void *pc2_buf = nullptr;      // two-level FFT preconditioner: x-y fields (T)
   double *pc2_h = nullptr;      // its double workspace: column sums, partial sums
+  void *zl_buf = nullptr;       // fft2z: z-line factors (4 nc values of T)
+  double *zl_col = nullptr;     // fft2z: per-column Sherman-Morrison scalars
   double *tw[3] = {nullptr, nullptr, nullptr};   // twiddles per axis (complex)
   double *trig[3] = {nullptr, nullptr, nullptr}; // cos/sin of 2 pi k / N per axis
   bool fft_ok = false;
@@ -277,6 +279,16 @@ int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z);
 constexpr int kFft2Its = 2;
 int64_t fft2_buf_elems(const chy_ctx *ctx);  // elements of T in pc2_buf
 int64_t fft2_h_elems(const chy_ctx *ctx);    // doubles in pc2_h
+// fft2z: the z-line correction of the fft2 residual.  zbuf: 4 nc values of T
+// (factors), colbuf: 2 (nx ny) doubles, made by launch_zline_factor once per
+// solve; launch_zline_apply adds the z-line solve of res to z (res is used as
+// scratch); launch_residual_t: res = b - A(n) x in precision T.
+template <typename T>
+void launch_zline_factor(chy_ctx *ctx, int lhs_type, const T *n, T *zbuf, double *colbuf);
+template <typename T>
+void launch_zline_apply(chy_ctx *ctx, const T *zbuf, const double *colbuf, T *res, T *z);
+template <typename T>
+void launch_residual_t(chy_ctx *ctx, int lhs_type, const T *n, const T *b, const T *x, T *res);
 template <typename T>
 int fft2_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z, T *w3);
 

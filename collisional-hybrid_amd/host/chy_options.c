@@ -196,13 +196,13 @@ int chy_params_default(chy_params *p)
   p->rhs_type = CHY_RHS_FULL;
   p->density_type = CHY_DENSITY_FLAT_SOBOL;
   p->ksp_type = CHY_KSP_GMRES;
-  p->pc_type = CHY_PC_FFT2;
+  p->pc_type = CHY_PC_FFT2Z;
   p->ksp_rtol = 1e-5;      /* PETSc KSP defaults */
   p->ksp_atol = 1e-50;
   p->ksp_max_it = 10000;
   p->ksp_restart = 30;
   p->pc_cheby_degree = 4;
-  p->ksp_inner_rtol = 2e-4;
+  p->ksp_inner_rtol = 2e-5;
   p->ksp_gmres_cgs_refinement = CHY_CGS_REFINE_NEVER;  /* PETSc default */
   p->precision = CHY_PREC_FP64;
   p->output_interval = 1;
@@ -286,7 +286,7 @@ int chy_params_derive(chy_params *p, char *err, size_t errlen)
     emsg(&e, CHY_ERR_ARG_WRONG, "unknown precision\n");
   if (p->ksp_restart < 1) p->ksp_restart = 30;
   if (p->pc_cheby_degree < 1) p->pc_cheby_degree = 4;
-  if (!(p->ksp_inner_rtol > 0.0 && p->ksp_inner_rtol < 1.0)) p->ksp_inner_rtol = 2e-4;
+  if (!(p->ksp_inner_rtol > 0.0 && p->ksp_inner_rtol < 1.0)) p->ksp_inner_rtol = 2e-5;
   return e.status;
 }
 
@@ -379,10 +379,11 @@ int chy_params_from_options(int argc, const char *const *argv, chy_params *p, ch
     else if (!strcasecmp(s, "jacobi") || !strcasecmp(s, "pbjacobi")) p->pc_type = CHY_PC_JACOBI;
     else if (!strcasecmp(s, "fft")) p->pc_type = CHY_PC_FFT;
     else if (!strcasecmp(s, "fft2")) p->pc_type = CHY_PC_FFT2;
+    else if (!strcasecmp(s, "fft2z")) p->pc_type = CHY_PC_FFT2Z;
     else if (!strcasecmp(s, "zline") || !strcasecmp(s, "bjacobi") || !strcasecmp(s, "line"))
       p->pc_type = CHY_PC_ZLINE;
     else if (!strcasecmp(s, "cheby") || !strcasecmp(s, "chebyshev")) p->pc_type = CHY_PC_CHEBY;
-    else emsg(&e, 0, "Warning: -pc_type %s mapped to the default GPU preconditioner (fft2)\n", s);
+    else emsg(&e, 0, "Warning: -pc_type %s mapped to the default GPU preconditioner (fft2z)\n", s);
   }
   get_real(&db, "-ksp_rtol", &p->ksp_rtol, &e);
   get_real(&db, "-ksp_atol", &p->ksp_atol, &e);

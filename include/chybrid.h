@@ -49,7 +49,8 @@ enum { CHY_BC_PERIODIC = 0, CHY_BC_INJECTION, CHY_BC_REFLECTION, CHY_BC_DIRICHLE
 /* build additions */
 enum { CHY_PREC_FP64 = 0, CHY_PREC_MIXED = 1 };
 enum { CHY_KSP_GMRES = 0, CHY_KSP_BCGS = 1 };
-enum { CHY_PC_NONE = 0, CHY_PC_JACOBI = 1, CHY_PC_FFT = 2, CHY_PC_ZLINE = 3, CHY_PC_CHEBY = 4, CHY_PC_FFT2 = 5 };
+enum { CHY_PC_NONE = 0, CHY_PC_JACOBI = 1, CHY_PC_FFT = 2, CHY_PC_ZLINE = 3, CHY_PC_CHEBY = 4, CHY_PC_FFT2 = 5,
+       CHY_PC_FFT2Z = 6 };
 /* -ksp_gmres_cgs_refinement_type: REFINE_NEVER, REFINE_IFNEEDED (when the
  * first pass removes more than half of the vector's squared norm), REFINE_ALWAYS */
 enum { CHY_CGS_REFINE_NEVER = 0, CHY_CGS_REFINE_IFNEEDED = 1, CHY_CGS_REFINE_ALWAYS = 2 };
@@ -200,6 +201,10 @@ int chy_lhs_apply_transpose(chy_ctx *ctx, const double *x, double *y);
 int chy_solve_potential(chy_ctx *ctx, double *phi, chy_solve_stats *stats);
 /* Solve A(n) phi = b for a given b (mean removed); phi in/out. */
 int chy_solve(chy_ctx *ctx, const double *b, double *phi, chy_solve_stats *stats);
+/* KSPSetTolerances (the KSP that potential.c:6-16 solves with): relative and
+ * absolute tolerance and iteration limit of the following solves; a negative
+ * value keeps the current setting (PETSC_DEFAULT). */
+int chy_set_tolerances(chy_ctx *ctx, double rtol, double atol, int64_t max_it);
 int chy_get_potential(chy_ctx *ctx, double *phi);
 int chy_set_potential(chy_ctx *ctx, const double *phi);
 int chy_get_rhs(chy_ctx *ctx, double *b);

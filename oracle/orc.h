@@ -151,7 +151,8 @@ void orc_init_particles(const int64_t N[3], const double p0[3], const double d[3
                         uint32_t *pid, double *w);
 
 /* ---- potential solve (reference potential.c:6-16 + PETSc KSP) ----------- */
-enum { ORC_PC_NONE = 0, ORC_PC_JACOBI = 1, ORC_PC_FFT = 2, ORC_PC_ZLINE = 3, ORC_PC_CHEBY = 4, ORC_PC_FFT2 = 5 };
+enum { ORC_PC_NONE = 0, ORC_PC_JACOBI = 1, ORC_PC_FFT = 2, ORC_PC_ZLINE = 3, ORC_PC_CHEBY = 4, ORC_PC_FFT2 = 5,
+       ORC_PC_FFT2Z = 6 };
 enum { ORC_KSP_GMRES = 0, ORC_KSP_BCGS = 1 };
 #define ORC_CHEBY_DEGREE 4
 #define ORC_FFT2_ITS 2
@@ -170,7 +171,11 @@ enum { ORC_KSP_GMRES = 0, ORC_KSP_BCGS = 1 };
  *           are the z-means of the rows' entries grouped by x-y offset) by
  *           ORC_FFT2_ITS steps of flexible GMRES (classical Gram-Schmidt twice,
  *           Givens least squares) preconditioned by the kz = 0 symbol of the
- *           fft preconditioner; the correction, constant along z, is added to z */
+ *           fft preconditioner; the correction, constant along z, is added to z
+ *   fft2z   fft2, then zline applied to its residual: z1 = fft2(r),
+ *           z = z1 + zline(r - A z1) (the two corrections take complementary
+ *           parts of the error: the x-y coupling of the kz = 0 modes and the
+ *           density variation along z) */
 void orc_pc_apply(int pc, int lhs_type, const int64_t N[3], const double d[3], const double K[3],
                   const double *n, const double *r, double *z);
 void orc_pc_apply_deg(int pc, int lhs_type, const int64_t N[3], const double d[3], const double K[3],

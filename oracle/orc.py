@@ -17,7 +17,7 @@ I64 = np.int64
 
 LHS_IDENTITY, LHS_LAPLACIAN, LHS_FULL = 0, 1, 2
 RHS_CONSTANT, RHS_SINUSOIDAL, RHS_FULL = 0, 1, 2
-PC_NONE, PC_JACOBI, PC_FFT, PC_ZLINE, PC_CHEBY, PC_FFT2 = 0, 1, 2, 3, 4, 5
+PC_NONE, PC_JACOBI, PC_FFT, PC_ZLINE, PC_CHEBY, PC_FFT2, PC_FFT2Z = 0, 1, 2, 3, 4, 5, 6
 FFT2_ITS = 2
 CHEBY_DEGREE = 4
 KSP_GMRES, KSP_BCGS = 0, 1

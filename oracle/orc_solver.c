@@ -278,6 +278,17 @@ void orc_pc_apply_deg(int pc, int lhs_type, const int64_t N[3], const double d[3
                       const double *n, const double *r, double *z, int degree)
 {
   const int64_t nc = N[0] * N[1] * N[2];
+  if (pc == ORC_PC_FFT2Z) {
+    double *w = malloc(sizeof(double) * nc), *z2 = malloc(sizeof(double) * nc);
+    orc_pc_apply_deg(ORC_PC_FFT2, lhs_type, N, d, K, n, r, z, degree);
+    orc_lhs_apply(lhs_type, N, d, K, n, z, w);
+    for (int64_t c = 0; c < nc; c++) w[c] = r[c] - w[c];
+    orc_pc_apply_deg(ORC_PC_ZLINE, lhs_type, N, d, K, n, w, z2, degree);
+    for (int64_t c = 0; c < nc; c++) z[c] += z2[c];
+    free(w); free(z2);
+    orc_remove_mean(nc, z);
+    return;
+  }
   if (pc == ORC_PC_FFT2) {
     const int64_t nxy = N[0] * N[1];
     orc_pc_apply_deg(ORC_PC_FFT, lhs_type, N, d, K, n, r, z, degree);

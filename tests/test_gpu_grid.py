@@ -195,12 +195,13 @@ def test_fft_preconditioner_exact_for_uniform_density(N, extra):
 
 
 PCS = {"jacobi": orc.PC_JACOBI, "fft": orc.PC_FFT, "zline": orc.PC_ZLINE, "cheby": orc.PC_CHEBY,
-       "fft2": orc.PC_FFT2}
+       "fft2": orc.PC_FFT2, "fft2z": orc.PC_FFT2Z}
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("pc,N", [("jacobi", (16, 8, 12)), ("fft", (16, 8, 32)), ("fft2", (16, 8, 32)),
-                                  ("fft2", (32, 16, 8)), ("zline", (16, 8, 12)),
+                                  ("fft2", (32, 16, 8)), ("fft2z", (16, 8, 32)), ("fft2z", (32, 16, 8)),
+                                  ("zline", (16, 8, 12)),
                                   ("zline", (8, 6, 3)), ("zline", (6, 4, 5)), ("cheby", (16, 8, 12))])
 def test_pc_apply_matches_oracle(pc, N, prec):
     """chy_pc_apply (the PCShell apply) against the oracle's preconditioners,
@@ -329,3 +330,30 @@ def test_two_level_fft_preconditioner_for_structured_density(prec):
             _, oits, conv, _ = orc.solve(g, K, n, b, pc=PCS[pc], rtol=1e-10, max_it=2000)
             assert conv and st.iterations <= 1.2 * oits + 3, (pc, st.iterations, oits)
     assert its["fft2"] < 0.6 * its["fft"], its
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_fft2z_preconditioner_solves_in_fewer_iterations(prec):
+    """Density varying in all three directions: the z-line correction of the
+    two-level preconditioner converges to the tolerance in fewer iterations
+    than fft2, and in about the oracle's count (FP64)."""
+    rng = np.random.default_rng(43)
+    its = {}
+    for pc in ("fft2", "fft2z"):
+        with context(opts((32, 32, 16), extra=f"-ksp_rtol 1e-10 -pc_type {pc} -precision {prec}")) as ctx:
+            g = grid_of(ctx.prm)
+            n = 16 * (1 + 0.25 * np.random.default_rng(44).uniform(-1, 1, g.nc))
+            n = n.astype(np.float32).astype(np.float64)
+            G = [np.random.default_rng(45 + a).standard_normal(g.nc) * 200 * n for a in range(3)]
+            ctx.set_moments(vlasov_from([n] + G))
+            phi, st = ctx.solve_potential()
+            b = ctx.get_rhs()
+            K = list(ctx.prm.kappa_e)
+        assert st.converged == 1 and st.residual <= 1e-10, (pc, st.iterations, st.residual)
+        r = b - orc.lhs_apply(g, K, n, phi)
+        assert np.linalg.norm(r) <= 2e-10 * np.linalg.norm(b)
+        its[pc] = st.iterations
+        if prec == "fp64":
+            _, oits, conv, _ = orc.solve(g, K, n, b, pc=PCS[pc], rtol=1e-10, max_it=2000)
+            assert conv and st.iterations <= 1.2 * oits + 3, (pc, st.iterations, oits)
+    assert its["fft2z"] < its["fft2"], its

@@ -98,18 +98,19 @@ def test_option_errors_mirror_reference_classes():
 def test_solver_option_mapping():
     p, msg = chybrid.params_from_options("-ksp_type bcgs -pc_type hypre -ksp_rtol 1e-9 -ksp_max_it 77 "
                                          "-ksp_gmres_restart 12 -ksp_monitor_short -precision mixed".split())
-    assert p.ksp_type == chybrid.KSP_BCGS and p.pc_type == chybrid.PC_FFT2 and "hypre" in msg
+    assert p.ksp_type == chybrid.KSP_BCGS and p.pc_type == chybrid.PC_FFT2Z and "hypre" in msg
     assert p.ksp_rtol == 1e-9 and p.ksp_max_it == 77 and p.ksp_restart == 12 and p.ksp_monitor == 1
     assert p.precision == chybrid.PREC_MIXED
     for name, pc in (("none", chybrid.PC_NONE), ("jacobi", chybrid.PC_JACOBI), ("fft", chybrid.PC_FFT),
-                     ("fft2", chybrid.PC_FFT2), ("zline", chybrid.PC_ZLINE), ("bjacobi", chybrid.PC_ZLINE),
+                     ("fft2", chybrid.PC_FFT2), ("fft2z", chybrid.PC_FFT2Z), ("zline", chybrid.PC_ZLINE),
+                     ("bjacobi", chybrid.PC_ZLINE),
                      ("cheby", chybrid.PC_CHEBY), ("chebyshev", chybrid.PC_CHEBY)):
         p, _ = chybrid.params_from_options(["-pc_type", name])
         assert p.pc_type == pc, name
     p, _ = chybrid.params_from_options("-pc_cheby_degree 7 -ksp_inner_rtol 1e-5".split())
     assert p.pc_cheby_degree == 7 and p.ksp_inner_rtol == 1e-5
     p, _ = chybrid.params_from_options([])
-    assert p.pc_type == chybrid.PC_FFT2 and p.pc_cheby_degree == 4 and p.ksp_inner_rtol == 2e-4
+    assert p.pc_type == chybrid.PC_FFT2Z and p.pc_cheby_degree == 4 and p.ksp_inner_rtol == 2e-5
     # classical Gram-Schmidt without refinement unless asked (PETSc's default)
     assert p.ksp_gmres_cgs_refinement == chybrid.CGS_REFINE_NEVER
     for name, v in (("REFINE_NEVER", chybrid.CGS_REFINE_NEVER), ("refine_ifneeded", chybrid.CGS_REFINE_IFNEEDED),

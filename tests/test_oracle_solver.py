@@ -221,3 +221,53 @@ def test_fft2_preconditioner_cuts_iterations_for_structured_density():
         assert conv and res <= 1e-10
         its[pc] = it
     assert its[orc.PC_FFT2] < 0.6 * its[orc.PC_FFT], its
+
+
+def test_fft2z_equals_fft_for_uniform_density():
+    """Uniform density: fft is exact, so both corrections of fft2z vanish."""
+    g = orc.Grid((8, 6, 4), p1=(0.4, 0.3, 0.2))
+    K = (0.5, -0.3, 6.0)
+    rng = np.random.default_rng(35)
+    n = np.full(g.nc, 16.0)
+    r = rng.standard_normal(g.nc)
+    z1 = orc.pc_apply(g, K, n, r, pc=orc.PC_FFT)
+    z2 = orc.pc_apply(g, K, n, r, pc=orc.PC_FFT2Z)
+    assert np.abs(z2 - z1).max() <= 1e-10 * np.abs(z1).max()
+
+
+def test_fft2z_adds_the_z_line_solve_of_the_fft2_residual():
+    """fft2z = fft2 + the z-line block solve of fft2's residual, with the
+    column systems solved densely from the operator's rows here."""
+    g = orc.Grid((5, 4, 6), p1=(0.25, 0.2, 0.3))
+    K = (0.4, -0.7, 9.0)
+    rng = np.random.default_rng(36)
+    n = 16 * (1 + 0.3 * rng.uniform(-1, 1, g.nc))
+    r = rng.standard_normal(g.nc)
+    z1 = orc.pc_apply(g, K, n, r, pc=orc.PC_FFT2)
+    res = r - orc.lhs_apply(g, K, n, z1)
+    plane = int(g.N[0] * g.N[1])
+    y = np.zeros(g.nc)
+    for col in range(plane):
+        y[col::plane] = np.linalg.solve(_line_matrix(g, K, n, col), res[col::plane])
+    ref = z1 + y
+    ref -= ref.mean()
+    got = orc.pc_apply(g, K, n, r, pc=orc.PC_FFT2Z)
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_fft2z_cuts_iterations_below_fft2():
+    """Density varying in all three directions with the workload's
+    anisotropy: adding the z-line correction to the two-level preconditioner
+    cuts the GMRES iterations to 1e-10."""
+    g = orc.Grid((32, 32, 16), p1=(1.6, 1.6, 0.8))
+    K = (17.6, 35.2, KZ)
+    rng = np.random.default_rng(37)
+    n = 16 * (1 + 0.25 * rng.uniform(-1, 1, g.nc))
+    b = rng.standard_normal(g.nc)
+    b -= b.mean()
+    its = {}
+    for pc in (orc.PC_FFT2, orc.PC_FFT2Z):
+        phi, it, conv, res = orc.solve(g, K, n, b, pc=pc, rtol=1e-10, max_it=2000)
+        assert conv and res <= 1e-10
+        its[pc] = it
+    assert its[orc.PC_FFT2Z] < 0.75 * its[orc.PC_FFT2], its

@@ -214,9 +214,10 @@ int chy_get_rhs(chy_ctx *ctx, double *b);
  * operator annihilates constants).  none: z = r; jacobi: D^-1 r; fft:
  * A(mean n)^+ r; zline: cyclic tridiagonal solves of every z-line block of
  * A(n); cheby: pc_cheby_degree steps of Chebyshev iteration on D^-1 A from
- * zero, eigenvalues in [0.2, 2.2]; fft2 (default): fft, then 2 steps of
- * flexible GMRES on the z-averaged residual with the z-averaged operator (the
- * x-y coupling, which the Hall term makes density dependent). */
+ * zero, eigenvalues in [0.2, 2.2]; fft2: fft, then 2 steps of flexible GMRES
+ * on the z-averaged residual with the z-averaged operator (the x-y coupling,
+ * which the Hall term makes density dependent); fft2z (default): fft2, then the
+ * zline solve of its residual added. */
 int chy_pc_apply(chy_ctx *ctx, const double *r, double *z);
 /* One time step of the loop body hybrid.c:108-117 (velocities, positions,
  * moments, potential), fused on the device. */

@@ -488,6 +488,8 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   for (int a = 0; a < 3; a++) { g.p0[a] = p.p0[a]; g.p1[a] = p.p1[a]; g.d[a] = p.d[a]; g.L[a] = p.L[a]; }
   if (g.nc >= (int64_t)1 << 31) { delete ctx; return CHY_ERR_ARG_RANGE; }
   ctx->cap = np_capacity > 0 ? np_capacity : std::max<int64_t>(p.Np, 1);
+  // slots, offsets and sort permutations are 32-bit
+  if (ctx->cap > (int64_t)0xFFFFFFFF) { delete ctx; return CHY_ERR_ARG_RANGE; }
   ctx->weighted = p.perturbation != CHY_PERT_NONE || p.weighted;
   ctx->restart = std::max<int64_t>(1, std::min<int64_t>(p.ksp_restart, 64));
   const int64_t nc = g.nc, cap = ctx->cap;
@@ -514,7 +516,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
     A(dalloc(ctx, &pp, (int64_t)(32 * ts) * nc));
     ctx->partials = pp;
   }
-  A(dalloc(ctx, &ctx->sred, std::max<int64_t>(4096, nc / 64 + 1)));  // normaliser partials, one per block
+  A(dalloc(ctx, &ctx->sred, std::max<int64_t>(4096, nc / 32 + 1)));  // normaliser partials, one per block
   A(dalloc(ctx, &ctx->S_dev, 1));
   A(dalloc(ctx, &ctx->cstats, 4));
   for (int a = 0; a < 4; a++) { unsigned char *q = nullptr; A(dalloc(ctx, &q, (int64_t)ts * nc)); ctx->mom[a] = q; }

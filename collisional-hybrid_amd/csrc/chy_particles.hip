@@ -17,6 +17,7 @@
 // Mixed state: int32 cell + FP32 in-cell offsets, FP32 v, pid, weight.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "chy_internal.h"
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t
 }
 
 template <bool COLLIDE, bool MOVE>
-__global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConst pc, int64_t np, PState<float> st,
+__global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConstF pc, int64_t np, PState<float> st,
                                                     const Vec4<float> *__restrict__ E4, const double *__restrict__ Sdev,
                                                     uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats)
 {
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(256) void k_move_fp64(Grid g, PushConst pc, int64_t
   }
 }
 
-__global__ __launch_bounds__(256) void k_move_mixed(Grid g, PushConst pc, int64_t np, PState<float> st,
+__global__ __launch_bounds__(256) void k_move_mixed(Grid g, PushConstF pc, int64_t np, PState<float> st,
                                                     uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats)
 {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -563,17 +564,22 @@ __global__ __launch_bounds__(256) void k_perm(Grid g, int64_t np, const int32_t 
 // ---- deposit ---------------------------------------------------------------------
 // Per-cell corner partials P[(4q+m)*nc + c], m in {w, w vx, w vy, w vz}.
 // A block owns kDepCells consecutive cells, whose particles are contiguous
-// after the sort; they are staged through LDS in coalesced chunks.  Four
-// threads share a cell, each summing two corners over the cell's particles in
-// sorted order (the oracle's order), so the result is deterministic.
+// after the sort; they are staged through LDS in coalesced chunks of up to
+// kDepUnroll * 256 ions, every load of a chunk issued before the first is
+// used.  Eight threads share a cell, each summing one corner over the cell's
+// particles in sorted order (the oracle's order), so the result is
+// deterministic.
 // GATHER: the same kernel finishes the neighbour-move sort -- slot s of the
 // new order takes ion perm[s] of `st`, whose state is written to `b` (and
-// src[s] = perm[s]), and the ion is deposited from registers, so the sorted
-// state is not read back; with SUMS the block's part of the collision
-// normaliser over the final velocities in slot order goes to sred[block].
-constexpr int kDepCells = 64;
-constexpr int kDepChunk = 1024;
-__device__ __forceinline__ int skew(int e) { return e + (e >> 4); }  // LDS bank spread
+// src[s] = perm[s]), and the ion is deposited from LDS, so the sorted state
+// is not read back; with SUMS the block's part of the collision normaliser
+// over the final velocities in slot order goes to sred[block].
+constexpr int kDepCells = 32;
+constexpr int kDepUnroll = 3;
+template <typename T> constexpr int dep_chunk() { return sizeof(T) == 4 ? 640 : 640; }
+
+// One staged ion: in-cell offsets, velocity and weight.
+template <typename T> struct alignas(8 * sizeof(T)) DepRec { T fx, fy, fz, vx, vy, vz, w, pad; };
 
 template <typename T, bool ABS, bool GATHER, bool SUMS>
 __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *__restrict__ offs, PState<T> st,
@@ -582,8 +588,9 @@ __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *_
                                                        double n0y, double n0z, double vnT, double *__restrict__ sred,
                                                        uint32_t *__restrict__ src)
 {
-  constexpr int L = kDepChunk + kDepChunk / 16;
-  __shared__ T sf[7][L];
+  constexpr int kChunk = dep_chunk<T>();
+  static_assert(kChunk <= kDepUnroll * 256, "a chunk is staged in kDepUnroll passes of the block");
+  __shared__ DepRec<T> rec[kChunk];
   __shared__ uint32_t so[kDepCells + 1];
   __shared__ double sh[4];
   double sacc = 0.0;
@@ -592,71 +599,104 @@ __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *_
   if (threadIdx.x <= ncell) so[threadIdx.x] = offs[c0 + threadIdx.x];
   __syncthreads();
   const uint32_t s0 = so[0], s1 = so[ncell];
-  const int cl = threadIdx.x >> 2, part = threadIdx.x & 3;
+  const int cl = threadIdx.x >> 3, corner = threadIdx.x & 7;
   const bool own = cl < ncell;
   const uint32_t lo = own ? so[cl] : 0, hi = own ? so[cl + 1] : 0;
-  T acc[8];
+  T acc[4] = {(T)0, (T)0, (T)0, (T)0};
+  for (uint32_t base = s0; base < s1; base += kChunk) {
+    const int m = (int)((s1 - base) < (uint32_t)kChunk ? (s1 - base) : (uint32_t)kChunk);
+    // A wave stages the passes u with a slot of the chunk (wave-uniform
+    // count); each variant issues its indices first, then every field load,
+    // then the writes, in straight-line code so all loads are in flight
+    // together.  Slots past the chunk load ion `base` again (always present)
+    // and write nothing.
+    auto stage = [&](auto passes) {
+      constexpr int NU = decltype(passes)::value;
+      uint32_t q[NU];
 #pragma unroll
-  for (int q = 0; q < 8; q++) acc[q] = (T)0;
-  for (uint32_t base = s0; base < s1; base += kDepChunk) {
-    const int m = (int)((s1 - base) < (uint32_t)kDepChunk ? (s1 - base) : (uint32_t)kDepChunk);
-    for (int e = threadIdx.x; e < m; e += blockDim.x) {
-      const uint32_t s = base + e;
-      const uint32_t q = GATHER ? perm[s] : s;
-      const int k = skew(e);
-      const T x = st.x[q], y = st.y[q], z = st.z[q], vx = st.vx[q], vy = st.vy[q], vz = st.vz[q];
-      const T w = weighted ? st.w[q] : (T)1;
-      if (GATHER) {
-        b.x[s] = x; b.y[s] = y; b.z[s] = z;
-        b.vx[s] = vx; b.vy[s] = vy; b.vz[s] = vz;
-        b.pid[s] = st.pid[q];
-        b.cell[s] = st.cell[q];
-        if (weighted) b.w[s] = w;
-        if (src) src[s] = q;
-        if (SUMS) {
-          if (sizeof(T) == 4) sacc += (double)mean_rel_speed_f((float)vx, (float)vy, (float)vz, (float)n0x,
-                                                                (float)n0y, (float)n0z, (float)vnT);
-          else sacc += mean_rel_speed((double)vx, (double)vy, (double)vz, n0x, n0y, n0z, vnT);
+      for (int u = 0; u < NU; u++) {
+        const int e = (int)threadIdx.x + 256 * u;
+        const uint32_t s = base + (uint32_t)(e < m ? e : 0);
+        q[u] = GATHER ? perm[s] : s;
+      }
+      T x[NU], y[NU], z[NU], vx[NU], vy[NU], vz[NU], w[NU];
+      uint32_t pid[NU];
+      int32_t cell[NU];
+#pragma unroll
+      for (int u = 0; u < NU; u++) {
+        x[u] = st.x[q[u]]; y[u] = st.y[q[u]]; z[u] = st.z[q[u]];
+        vx[u] = st.vx[q[u]]; vy[u] = st.vy[q[u]]; vz[u] = st.vz[q[u]];
+        w[u] = weighted ? st.w[q[u]] : (T)1;
+        if (GATHER) { pid[u] = st.pid[q[u]]; cell[u] = st.cell[q[u]]; }
+      }
+#pragma unroll
+      for (int u = 0; u < NU; u++) {
+        const int e = (int)threadIdx.x + 256 * u;
+        if (e < m) {
+          const uint32_t s = base + (uint32_t)e;
+          if (GATHER) {
+            b.x[s] = x[u]; b.y[s] = y[u]; b.z[s] = z[u];
+            b.vx[s] = vx[u]; b.vy[s] = vy[u]; b.vz[s] = vz[u];
+            b.pid[s] = pid[u];
+            b.cell[s] = cell[u];
+            if (weighted) b.w[s] = w[u];
+            if (src) src[s] = q[u];
+            if (SUMS) {
+              if (sizeof(T) == 4)
+                sacc += (double)mean_rel_speed_f((float)vx[u], (float)vy[u], (float)vz[u], (float)n0x, (float)n0y,
+                                                 (float)n0z, (float)vnT);
+              else sacc += mean_rel_speed((double)vx[u], (double)vy[u], (double)vz[u], n0x, n0y, n0z, vnT);
+            }
+          }
+          DepRec<T> r;
+          if (ABS) {
+            int i;
+            double f;
+            cell_coord((double)x[u], g.p0[0], g.d[0], g.nx, i, f); r.fx = (T)f;
+            cell_coord((double)y[u], g.p0[1], g.d[1], g.ny, i, f); r.fy = (T)f;
+            cell_coord((double)z[u], g.p0[2], g.d[2], g.nz, i, f); r.fz = (T)f;
+          } else {
+            r.fx = x[u]; r.fy = y[u]; r.fz = z[u];
+          }
+          r.vx = vx[u]; r.vy = vy[u]; r.vz = vz[u];
+          r.w = w[u];
+          r.pad = (T)0;
+          rec[e] = r;
         }
       }
-      if (ABS) {
-        int i;
-        double f;
-        cell_coord((double)x, g.p0[0], g.d[0], g.nx, i, f); sf[0][k] = (T)f;
-        cell_coord((double)y, g.p0[1], g.d[1], g.ny, i, f); sf[1][k] = (T)f;
-        cell_coord((double)z, g.p0[2], g.d[2], g.nz, i, f); sf[2][k] = (T)f;
-      } else {
-        sf[0][k] = x; sf[1][k] = y; sf[2][k] = z;
-      }
-      sf[3][k] = vx; sf[4][k] = vy; sf[5][k] = vz;
-      sf[6][k] = w;
-    }
+    };
+    const int wave0 = (int)(threadIdx.x & ~63u);
+    const int passes = m > wave0 ? (m - wave0 + 255) / 256 : 0;
+    static_assert(kDepUnroll == 3, "one staging variant per pass count");
+    if (passes >= 3) stage(std::integral_constant<int, 3>());
+    else if (passes == 2) stage(std::integral_constant<int, 2>());
+    else if (passes == 1) stage(std::integral_constant<int, 1>());
     __syncthreads();
     const uint32_t a0 = lo > base ? lo : base;
     const uint32_t a1 = hi < base + m ? hi : base + m;
-    for (uint32_t s = a0; s < a1; s++) {
-      const int k = skew((int)(s - base));
-      const T fx = sf[0][k], fy = sf[1][k], fz = sf[2][k];
-      const T vx = sf[3][k], vy = sf[4][k], vz = sf[5][k], wt = sf[6][k];
-      const T wx[2] = {(T)1 - fx, fx}, wy[2] = {(T)1 - fy, fy}, wz[2] = {(T)1 - fz, fz};
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int q = 2 * part + h;
-        const T W = (wz[q >> 2] * wy[(q >> 1) & 1]) * wx[q & 1];
-        const T a = weighted ? wt * W : W;
-        acc[4 * h + 0] += a;
-        acc[4 * h + 1] += a * vx;
-        acc[4 * h + 2] += a * vy;
-        acc[4 * h + 3] += a * vz;
-      }
+    // the cell's ions in slot order; records read four at a time, summed in order
+    auto add = [&](const DepRec<T> &r) {
+      const T wx = (corner & 1) ? r.fx : (T)1 - r.fx;
+      const T wy = (corner & 2) ? r.fy : (T)1 - r.fy;
+      const T wz = (corner & 4) ? r.fz : (T)1 - r.fz;
+      const T W = (wz * wy) * wx;
+      const T a = weighted ? r.w * W : W;
+      acc[0] += a;
+      acc[1] += a * r.vx;
+      acc[2] += a * r.vy;
+      acc[3] += a * r.vz;
+    };
+    uint32_t s = a0;
+    for (; s + 4 <= a1; s += 4) {
+      const DepRec<T> r0 = rec[s - base], r1 = rec[s + 1 - base], r2 = rec[s + 2 - base], r3 = rec[s + 3 - base];
+      add(r0); add(r1); add(r2); add(r3);
     }
+    for (; s < a1; s++) add(rec[s - base]);
     __syncthreads();
   }
   if (own) {
 #pragma unroll
-    for (int h = 0; h < 2; h++)
-#pragma unroll
-      for (int mm = 0; mm < 4; mm++) P[(int64_t)(4 * (2 * part + h) + mm) * g.nc + c0 + cl] = acc[4 * h + mm];
+    for (int mm = 0; mm < 4; mm++) P[(int64_t)(4 * corner + mm) * g.nc + c0 + cl] = acc[mm];
   }
   if (GATHER && SUMS) {
     const double t = block_sum(sacc, sh);
@@ -1127,15 +1167,17 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
   const unsigned nb = grid_for(ctx->np);
   timer_start(ctx, "push");
 #define LAUNCH(KER, T, C, M)                                                                                  \
-  hipLaunchKernelGGL((KER<C, M>), nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, st,                       \
+  hipLaunchKernelGGL((KER<C, M>), nb, kBlock, 0, ctx->stream, ctx->g, pcT, ctx->np, st,                      \
                      (const Vec4<T> *)ctx->efield[0], ctx->S_dev, count ? ctx->counts : nullptr, ctx->cstats)
   if (ctx->mixed) {
+    const PushConstF pcT = push_const_f(pc);
     PState<float> st = ctx->pf[ctx->cur];
     if (collide && move) LAUNCH(k_push_mixed, float, true, true);
     else if (collide) LAUNCH(k_push_mixed, float, true, false);
     else if (move) LAUNCH(k_push_mixed, float, false, true);
     else LAUNCH(k_push_mixed, float, false, false);
   } else {
+    const PushConst &pcT = pc;
     PState<double> st = ctx->pd[ctx->cur];
     if (collide && move) LAUNCH(k_push_fp64, double, true, true);
     else if (collide) LAUNCH(k_push_fp64, double, true, false);
@@ -1167,7 +1209,7 @@ int particles_move_only(chy_ctx *ctx)
   launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
   const unsigned nb = grid_for(ctx->np);
   if (ctx->mixed)
-    hipLaunchKernelGGL(k_move_mixed, nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, ctx->pf[ctx->cur], ctx->counts,
+    hipLaunchKernelGGL(k_move_mixed, nb, kBlock, 0, ctx->stream, ctx->g, push_const_f(pc), ctx->np, ctx->pf[ctx->cur], ctx->counts,
                        ctx->cstats);
   else
     hipLaunchKernelGGL(k_move_fp64, nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, ctx->pd[ctx->cur], ctx->counts,

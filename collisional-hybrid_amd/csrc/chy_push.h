@@ -34,6 +34,32 @@ inline PushConst push_const(const chy_ctx *ctx)
   return c;
 }
 
+// The same constants in FP32 for the mixed kernels: passed as kernel
+// arguments they sit in scalar registers, where the FP64 set would be
+// converted per wave into vector registers.  Every conversion is the
+// round-to-nearest one the FP32 arithmetic applied to the FP64 values.
+struct PushConstF {
+  float tau, t[3], s[3];
+  float dt, cdt[3];
+  float mi, mn, viT, vnT, vn0[3];
+  float Nc;
+  uint32_t k0, k1, step;
+};
+
+inline PushConstF push_const_f(const PushConst &c)
+{
+  PushConstF f;
+  f.tau = (float)c.tau;
+  for (int a = 0; a < 3; a++) {
+    f.t[a] = (float)c.t[a]; f.s[a] = (float)c.s[a]; f.cdt[a] = (float)c.cdt[a]; f.vn0[a] = (float)c.vn0[a];
+  }
+  f.dt = (float)c.dt;
+  f.mi = (float)c.mi; f.mn = (float)c.mn; f.viT = (float)c.viT; f.vnT = (float)c.vnT;
+  f.Nc = (float)c.Nc;
+  f.k0 = c.k0; f.k1 = c.k1; f.step = c.step;
+  return f;
+}
+
 inline int64_t collision_count(const chy_params &p, int64_t Np)
 {
   // particles.c:510-519
@@ -48,12 +74,27 @@ inline int64_t collision_count(const chy_params &p, int64_t Np)
   return (int64_t)x;
 }
 
+// Arithmetic of the collision kinematics.  FP64 (the reference's
+// arithmetic): IEEE square root, division, sine and cosine.  FP32 (the mixed
+// path): the hardware square root, reciprocal and sine/cosine, each within a
+// few ulp -- the neutral draw and the scattering are Monte Carlo samples whose
+// FP32 rounding is far below their sampling noise -- with the square root
+// clamped at zero against rounding that pushes 1 - cos^2 below it.
+template <typename T> __device__ __forceinline__ T kin_sqrt(T x) { return sqrt(x); }
+template <> __device__ __forceinline__ float kin_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(fmaxf(x, 0.0f)); }
+template <typename T> __device__ __forceinline__ T kin_div(T a, T b) { return a / b; }
+template <> __device__ __forceinline__ float kin_div<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <typename T> __device__ __forceinline__ T kin_cos(T a) { return cos(a); }
+template <> __device__ __forceinline__ float kin_cos<float>(float a) { return __cosf(a); }
+template <typename T> __device__ __forceinline__ T kin_sin(T a) { return sin(a); }
+template <> __device__ __forceinline__ float kin_sin<float>(float a) { return __sinf(a); }
+
 // ---- ion-neutral collisions (reference particles.c:529-611) ---------------------
 // Neutral draw and Bernoulli decision from one Philox block: Box-Muller on the
 // top 24 bits of the four words, the decision uniform from their low bytes.
 // The decision is "collide iff u < Nc |v_r| / S" (probability clipped at 1).
-template <typename T>
-__device__ __forceinline__ void neutral_draw(const PushConst &pc, uint32_t pid, T &vnx, T &vny, T &vnz, T &udec)
+template <typename T, typename PC>
+__device__ __forceinline__ void neutral_draw(const PC &pc, uint32_t pid, T &vnx, T &vny, T &vnz, T &udec)
 {
   const U4 a = philox4x32_10(U4{pid, pc.step, STREAM_COLLIDE_A, 0u}, pc.k0, pc.k1);
   const T u0 = u24<T>(a.x), u1 = u24<T>(a.y), u2 = u24<T>(a.z), u3 = u24<T>(a.w);
@@ -61,8 +102,8 @@ __device__ __forceinline__ void neutral_draw(const PushConst &pc, uint32_t pid, 
   T r1, r2, c1, s1, c2;
   if (sizeof(T) == 4) {
     // hardware transcendentals: ample accuracy for a Monte Carlo neutral draw
-    r1 = __fsqrt_rn(-2.0f * __logf((float)u0));
-    r2 = __fsqrt_rn(-2.0f * __logf((float)u2));
+    r1 = kin_sqrt<float>(-2.0f * __logf((float)u0));
+    r2 = kin_sqrt<float>(-2.0f * __logf((float)u2));
     const float a1 = 6.28318530717958647692f * (float)u1, a2 = 6.28318530717958647692f * (float)u3;
     c1 = __cosf(a1); s1 = __sinf(a1); c2 = __cosf(a2);
   } else {
@@ -78,26 +119,26 @@ __device__ __forceinline__ void neutral_draw(const PushConst &pc, uint32_t pid, 
 }
 
 // Collision decision from a neutral draw (vn, ud) already made.
-template <typename T>
-__device__ __forceinline__ bool collide_decide_n(const PushConst &pc, T S, T vnx, T vny, T vnz, T ud, T vx, T vy,
+template <typename T, typename PC>
+__device__ __forceinline__ bool collide_decide_n(const PC &pc, T S, T vnx, T vny, T vnz, T ud, T vx, T vy,
                                                  T vz, int &capped)
 {
   const T vrx = vx - vnx, vry = vy - vny, vrz = vz - vnz;
-  const T vrr = sqrt(vrx * vrx + vry * vry + vrz * vrz);
-  T p = (S > (T)0) ? ((T)pc.Nc * vrr) / S : (T)0;
+  const T vrr = kin_sqrt<T>(vrx * vrx + vry * vry + vrz * vrz);
+  T p = (S > (T)0) ? kin_div<T>((T)pc.Nc * vrr, S) : (T)0;
   capped = 0;
   if (p > (T)1) { p = (T)1; capped = 1; }
   return ud < p;
 }
 
-template <typename T>
-__device__ __forceinline__ bool collide_decide(const PushConst &pc, T S, uint32_t pid, T vx, T vy, T vz, int &capped)
+template <typename T, typename PC>
+__device__ __forceinline__ bool collide_decide(const PC &pc, T S, uint32_t pid, T vx, T vy, T vz, int &capped)
 {
   T vnx, vny, vnz, ud;
   neutral_draw<T>(pc, pid, vnx, vny, vnz, ud);
   const T vrx = vx - vnx, vry = vy - vny, vrz = vz - vnz;
-  const T vrr = sqrt(vrx * vrx + vry * vry + vrz * vrz);
-  T p = (S > (T)0) ? ((T)pc.Nc * vrr) / S : (T)0;
+  const T vrr = kin_sqrt<T>(vrx * vrx + vry * vry + vrz * vrz);
+  T p = (S > (T)0) ? kin_div<T>((T)pc.Nc * vrr, S) : (T)0;
   capped = 0;
   if (p > (T)1) { p = (T)1; capped = 1; }
   return ud < p;
@@ -105,11 +146,11 @@ __device__ __forceinline__ bool collide_decide(const PushConst &pc, T S, uint32_
 
 // Elastic scattering in the centre-of-mass frame, rotated into the frame of
 // the relative velocity (particles.c:552-611).  Returns 1 accepted, 2 refused.
-template <typename T>
-__device__ __forceinline__ int scatter_n(const PushConst &pc, uint32_t pid, T vnx, T vny, T vnz, T &vx, T &vy, T &vz);
+template <typename T, typename PC>
+__device__ __forceinline__ int scatter_n(const PC &pc, uint32_t pid, T vnx, T vny, T vnz, T &vx, T &vy, T &vz);
 
-template <typename T>
-__device__ __forceinline__ int scatter(const PushConst &pc, uint32_t pid, T &vx, T &vy, T &vz)
+template <typename T, typename PC>
+__device__ __forceinline__ int scatter(const PC &pc, uint32_t pid, T &vx, T &vy, T &vz)
 {
   T vnx, vny, vnz, ud;
   neutral_draw<T>(pc, pid, vnx, vny, vnz, ud);
@@ -117,35 +158,35 @@ __device__ __forceinline__ int scatter(const PushConst &pc, uint32_t pid, T &vx,
 }
 
 // scatter with the neutral draw (vn) already made
-template <typename T>
-__device__ __forceinline__ int scatter_n(const PushConst &pc, uint32_t pid, T vnx, T vny, T vnz, T &vx, T &vy, T &vz)
+template <typename T, typename PC>
+__device__ __forceinline__ int scatter_n(const PC &pc, uint32_t pid, T vnx, T vny, T vnz, T &vx, T &vy, T &vz)
 {
   const T vrx = vx - vnx, vry = vy - vny, vrz = vz - vnz;
-  const T vrr = sqrt(vrx * vrx + vry * vry + vrz * vrz);
+  const T vrr = kin_sqrt<T>(vrx * vrx + vry * vry + vrz * vrz);
   const U4 b = philox4x32_10(U4{pid, pc.step, STREAM_COLLIDE_B, 0u}, pc.k0, pc.k1);
   const T twopi = (T)(2.0 * M_PI);
   const T mi = (T)pc.mi, mn = (T)pc.mn, M = mn + mi;
-  const T vcx = (vx * mi + vnx * mn) / M;
-  const T vcy = (vy * mi + vny * mn) / M;
-  const T vcz = (vz * mi + vnz * mn) / M;
+  const T vcx = kin_div<T>(vx * mi + vnx * mn, M);
+  const T vcy = kin_div<T>(vy * mi + vny * mn, M);
+  const T vcz = kin_div<T>(vz * mi + vnz * mn, M);
   const T ex = vx - vcx, ey = vy - vcy, ez = vz - vcz;
-  const T vcr = sqrt(ex * ex + ey * ey + ez * ez);
+  const T vcr = kin_sqrt<T>(ex * ex + ey * ey + ez * ez);
   const T uz = (T)2.0 * u24<T>(b.x) - (T)1.0;
-  const T uperp = sqrt((T)1.0 - uz * uz);
+  const T uperp = kin_sqrt<T>((T)1.0 - uz * uz);
   const T uphi = twopi * u24<T>(b.y);
-  const T ux = uperp * cos(uphi), uy = uperp * sin(uphi);
-  const T costht = vrz / vrr;
-  const T sintht = sqrt((T)1.0 - costht * costht);
+  const T ux = uperp * kin_cos<T>(uphi), uy = uperp * kin_sin<T>(uphi);
+  const T costht = kin_div<T>(vrz, vrr);
+  const T sintht = kin_sqrt<T>((T)1.0 - costht * costht);
   T cosphi = (T)1, sinphi = (T)0;
   if (sintht > (T)1e-12) {
-    cosphi = vrx / (vrr * sintht);
-    sinphi = vry / (vrr * sintht);
+    cosphi = kin_div<T>(vrx, vrr * sintht);
+    sinphi = kin_div<T>(vry, vrr * sintht);
   }
   const T ux1 = uz * sintht + ux * costht, uy1 = uy, uz1 = uz * costht - ux * sintht;
   const T ux2 = ux1 * cosphi - uy1 * sinphi, uy2 = ux1 * sinphi + uy1 * cosphi, uz2 = uz1;
   const T vfx = vcx + vcr * ux2, vfy = vcy + vcr * uy2, vfz = vcz + vcr * uz2;
-  const T vfr = sqrt(vfx * vfx + vfy * vfy + vfz * vfz);
-  if (vfr / (T)pc.viT > (T)10.0) return 2;
+  const T vfr = kin_sqrt<T>(vfx * vfx + vfy * vfy + vfz * vfz);
+  if (kin_div<T>(vfr, (T)pc.viT) > (T)10.0) return 2;
   vx = vfx; vy = vfy; vz = vfz;
   return 1;
 }
@@ -166,8 +207,8 @@ template <typename T> struct alignas(4 * sizeof(T)) Vec4 { T x, y, z, w; };
 
 // Boris rotation (particles.c:374-447 with the intended cross products and
 // per-component v-minus); operation order shared with the oracle.
-template <typename T>
-__device__ __forceinline__ void boris(const PushConst &pc, T e0, T e1, T e2, T &vx, T &vy, T &vz)
+template <typename T, typename PC>
+__device__ __forceinline__ void boris(const PC &pc, T e0, T e1, T e2, T &vx, T &vy, T &vz)
 {
   const T tau = (T)pc.tau, t0 = (T)pc.t[0], t1 = (T)pc.t[1], t2 = (T)pc.t[2];
   const T s0 = (T)pc.s[0], s1 = (T)pc.s[1], s2 = (T)pc.s[2];
@@ -223,7 +264,8 @@ __device__ __forceinline__ void gather_c(const Grid &g, const Vec4<T> *__restric
 
 // Mixed move: in-cell offset += v dt / d with an exact integer carry, for an
 // ion of cell (i, j, k); returns the new cell.
-__device__ __forceinline__ int move_mixed_ijk(const Grid &g, const PushConst &pc, int i, int j, int k, float &fx,
+template <typename PC>
+__device__ __forceinline__ int move_mixed_ijk(const Grid &g, const PC &pc, int i, int j, int k, float &fx,
                                               float &fy, float &fz, float vx, float vy, float vz)
 {
   float o[3] = {fmaf(vx, (float)pc.cdt[0], fx), fmaf(vy, (float)pc.cdt[1], fy), fmaf(vz, (float)pc.cdt[2], fz)};

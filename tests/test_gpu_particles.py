@@ -179,6 +179,39 @@ def test_steps_match_oracle(collide, N, more, rtol):
             cur, phi = nxt, phi_gpu
 
 
+def test_mixed_collisions_track_fp64_statistics():
+    """Collisions on, both precisions from the same seeded state: every
+    step's collision count within 3 sigma of the binomial expectation Nc, and
+    the mixed run's density, drift and temperature within 1e-3 of the FP64
+    run's (the FP32 collision kinematics use the hardware square root,
+    reciprocal, sine and cosine)."""
+    extra = ("-seed 6 -ppc 32 -nui 3e3 -ksp_rtol 1e-10 --perturbation uniform --perturbation-amplitude 0.01")
+    res = {}
+    for prec in ("fp64", "mixed"):
+        with context(opts((16, 16, 16), extra=extra + f" -precision {prec}")) as ctx:
+            ctx.init_particles()
+            ctx.collect_moments(download=False)
+            ctx.solve_potential()
+            for it in range(4):
+                c = ctx.step().collisions
+                n = c.accepted + c.refused
+                assert c.Nc > 100 and abs(n - c.Nc) <= 3 * np.sqrt(c.Nc), (prec, it, n, c.Nc)
+            res[prec] = (as_particles(ctx.get_particles()), ctx.collect_moments())
+    (a, ma), (b, mb) = res["fp64"], res["mixed"]
+
+    def drift_temp(p):
+        v = np.stack([p.vx, p.vy, p.vz])
+        w = p.w if p.w is not None else np.ones(p.np)
+        u = (v * w).sum(axis=1) / w.sum()
+        return u, ((v - u[:, None]) ** 2 * w).sum(axis=1) / w.sum()
+
+    ua, Ta = drift_temp(a)
+    ub, Tb = drift_temp(b)
+    assert relmax(mb[:, 0], ma[:, 0]) <= 1e-3
+    assert np.abs(ub - ua).max() <= 1e-3 * np.linalg.norm(ua), (ua, ub)
+    assert relmax(Tb, Ta) <= 1e-3, (Ta, Tb)
+
+
 def test_mixed_step_tracks_fp64_step():
     """Same seeded initial state in both precisions: after three steps the
     mixed (int cell + FP32 offset) run agrees with FP64 to FP32 accuracy."""

@@ -23,16 +23,30 @@ struct RhsCoef {
   double dx, dy, dz;
 };
 
+template <typename T> struct alignas(4 * sizeof(T)) Vec4 { T x, y, z, w; };
+
 // Device buffers of one particle generation (ping-pong pair in the context).
+// FP64: one array per field -- x,y,z absolute positions, vx,vy,vz, the
+// optional weight, the id and the cell key.
 template <typename T>
 struct PState {
-  // FP64: x,y,z absolute positions, vx,vy,vz.  Mixed: ox,oy,oz in-cell
-  // offsets in [0,1) with the integer cell in `cell`.
   T *x = nullptr, *y = nullptr, *z = nullptr;
   T *vx = nullptr, *vy = nullptr, *vz = nullptr;
   T *w = nullptr;          // optional weights
   uint32_t *pid = nullptr;
-  int32_t *cell = nullptr; // cell key (both modes)
+  int32_t *cell = nullptr; // cell key
+};
+
+// Mixed: per ion a 16-byte record (ox, oy, oz, w) of in-cell offsets in
+// [0,1) and the weight (1 when unweighted), a 16-byte record (vx, vy, vz, id)
+// with the id's bits in the last lane, and the int32 cell key.  An ion moved
+// by the sort in slot order is then three reads (two 16-byte records and the
+// key) instead of nine 4-byte ones; most of the sort's gathered reads serve a
+// few ions each, so their number, not their bytes, sets its speed.
+template <>
+struct PState<float> {
+  Vec4<float> *pos = nullptr, *vel = nullptr;
+  int32_t *cell = nullptr;
 };
 
 // Per-name kernel timer: event pairs recorded on the context stream and
@@ -171,6 +185,22 @@ void free_state(PState<T> &s)
   void *ps[] = {s.x, s.y, s.z, s.vx, s.vy, s.vz, s.w, s.pid, s.cell};
   for (void *p : ps) if (p) (void)hipFree(p);
   s = PState<T>();
+}
+
+template <>
+inline int alloc_state<float>(chy_ctx *ctx, PState<float> &s, int64_t cap, bool)
+{
+  TRY(dalloc(ctx, &s.pos, cap)); TRY(dalloc(ctx, &s.vel, cap));
+  TRY(dalloc(ctx, &s.cell, cap));
+  return CHY_OK;
+}
+
+template <>
+inline void free_state<float>(PState<float> &s)
+{
+  void *ps[] = {s.pos, s.vel, s.cell};
+  for (void *p : ps) if (p) (void)hipFree(p);
+  s = PState<float>();
 }
 
 // timers (no-ops unless ctx->timing)

@@ -17,6 +17,7 @@
 // Mixed state: int32 cell + FP32 in-cell offsets, FP32 v, pid, weight.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -148,13 +149,14 @@ __global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConstF pc, int64
       const int c = st.cell[p];
       int i, j, k;
       split_cell(g, c, i, j, k);
-      const float fx = st.x[p], fy = st.y[p], fz = st.z[p];
-      float vx = st.vx[p], vy = st.vy[p], vz = st.vz[p];
+      const Vec4<float> P = st.pos[p], V = st.vel[p];
+      const float fx = P.x, fy = P.y, fz = P.z;
+      float vx = V.x, vy = V.y, vz = V.z;
       float e0, e1, e2;
       gather_c<float>(g, E4, c, i, j, k, fx, fy, fz, e0, e1, e2);
       boris<float>(pc, e0, e1, e2, vx, vy, vz);
       if (COLLIDE) {
-        const uint32_t pid = st.pid[p];
+        const uint32_t pid = __float_as_uint(V.w);
         float vnx, vny, vnz, ud;
         neutral_draw<float>(pc, pid, vnx, vny, vnz, ud);
         if (collide_decide_n<float>(pc, S, vnx, vny, vnz, ud, vx, vy, vz, capped)) {
@@ -162,11 +164,11 @@ __global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConstF pc, int64
           else nref++;
         }
       }
-      st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
+      st.vel[p] = Vec4<float>{vx, vy, vz, V.w};
       if (MOVE) {
         float ox = fx, oy = fy, oz = fz;
         key = move_mixed_ijk(g, pc, i, j, k, ox, oy, oz, vx, vy, vz);
-        st.x[p] = ox; st.y[p] = oy; st.z[p] = oz;
+        st.pos[p] = Vec4<float>{ox, oy, oz, P.w};
         st.cell[p] = key;
         bad = !isfinite(ox) || !isfinite(oy) || !isfinite(oz);
       }
@@ -211,11 +213,12 @@ __global__ __launch_bounds__(256) void k_move_mixed(Grid g, PushConstF pc, int64
     int key = -1;
     bool bad = false;
     if (p < np) {
-      float fx = st.x[p], fy = st.y[p], fz = st.z[p];
+      const Vec4<float> P = st.pos[p], V = st.vel[p];
+      float fx = P.x, fy = P.y, fz = P.z;
       int i, j, k;
       split_cell(g, st.cell[p], i, j, k);
-      key = move_mixed_ijk(g, pc, i, j, k, fx, fy, fz, st.vx[p], st.vy[p], st.vz[p]);
-      st.x[p] = fx; st.y[p] = fy; st.z[p] = fz;
+      key = move_mixed_ijk(g, pc, i, j, k, fx, fy, fz, V.x, V.y, V.z);
+      st.pos[p] = Vec4<float>{fx, fy, fz, P.w};
       st.cell[p] = key;
       bad = !isfinite(fx) || !isfinite(fy) || !isfinite(fz);
     }
@@ -234,6 +237,19 @@ __global__ __launch_bounds__(256) void k_meanrel(int64_t np, const T *__restrict
   double acc = 0.0;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x)
     acc += mean_rel_speed((double)vx[p], (double)vy[p], (double)vz[p], n0x, n0y, n0z, vnT);
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) sred[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_meanrel_mixed(int64_t np, const Vec4<float> *__restrict__ vel, double n0x,
+                                                       double n0y, double n0z, double vnT, double *__restrict__ sred)
+{
+  __shared__ double sh[4];
+  double acc = 0.0;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+    const Vec4<float> v = vel[p];
+    acc += mean_rel_speed((double)v.x, (double)v.y, (double)v.z, n0x, n0y, n0z, vnT);
+  }
   const double s = block_sum(acc, sh);
   if (threadIdx.x == 0) sred[blockIdx.x] = s;
 }
@@ -415,12 +431,19 @@ __global__ __launch_bounds__(256) void k_reorder(int64_t np, const uint32_t *__r
       }
       const uint32_t d = lo + rank;
       if (src) src[d] = me;  // source index of each slot, when asked for
-      const T vx = a.vx[me], vy = a.vy[me], vz = a.vz[me];
-      b.x[d] = a.x[me]; b.y[d] = a.y[me]; b.z[d] = a.z[me];
-      b.vx[d] = vx; b.vy[d] = vy; b.vz[d] = vz;
-      b.pid[d] = a.pid[me];
+      T vx, vy, vz;
+      if constexpr (sizeof(T) == 4) {
+        const Vec4<float> P = a.pos[me], V = a.vel[me];
+        vx = V.x; vy = V.y; vz = V.z;
+        b.pos[d] = P; b.vel[d] = V;
+      } else {
+        vx = a.vx[me]; vy = a.vy[me]; vz = a.vz[me];
+        b.x[d] = a.x[me]; b.y[d] = a.y[me]; b.z[d] = a.z[me];
+        b.vx[d] = vx; b.vy[d] = vy; b.vz[d] = vz;
+        b.pid[d] = a.pid[me];
+        if (weighted) b.w[d] = a.w[me];
+      }
       b.cell[d] = c;
-      if (weighted) b.w[d] = a.w[me];
       if (SUMS) {
         if (sizeof(T) == 4) acc += (double)mean_rel_speed_f((float)vx, (float)vy, (float)vz, (float)n0x, (float)n0y,
                                                            (float)n0z, (float)vnT);
@@ -619,49 +642,71 @@ __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *_
         const uint32_t s = base + (uint32_t)(e < m ? e : 0);
         q[u] = GATHER ? perm[s] : s;
       }
-      T x[NU], y[NU], z[NU], vx[NU], vy[NU], vz[NU], w[NU];
-      uint32_t pid[NU];
-      int32_t cell[NU];
+      if constexpr (sizeof(T) == 4) {
+        // mixed: two 16-byte records and the key per ion
+        Vec4<float> P[NU], V[NU];
+        int32_t cell[NU];
 #pragma unroll
-      for (int u = 0; u < NU; u++) {
-        x[u] = st.x[q[u]]; y[u] = st.y[q[u]]; z[u] = st.z[q[u]];
-        vx[u] = st.vx[q[u]]; vy[u] = st.vy[q[u]]; vz[u] = st.vz[q[u]];
-        w[u] = weighted ? st.w[q[u]] : (T)1;
-        if (GATHER) { pid[u] = st.pid[q[u]]; cell[u] = st.cell[q[u]]; }
-      }
+        for (int u = 0; u < NU; u++) {
+          P[u] = st.pos[q[u]]; V[u] = st.vel[q[u]];
+          if (GATHER) cell[u] = st.cell[q[u]];
+        }
 #pragma unroll
-      for (int u = 0; u < NU; u++) {
-        const int e = (int)threadIdx.x + 256 * u;
-        if (e < m) {
-          const uint32_t s = base + (uint32_t)e;
-          if (GATHER) {
-            b.x[s] = x[u]; b.y[s] = y[u]; b.z[s] = z[u];
-            b.vx[s] = vx[u]; b.vy[s] = vy[u]; b.vz[s] = vz[u];
-            b.pid[s] = pid[u];
-            b.cell[s] = cell[u];
-            if (weighted) b.w[s] = w[u];
-            if (src) src[s] = q[u];
-            if (SUMS) {
-              if (sizeof(T) == 4)
-                sacc += (double)mean_rel_speed_f((float)vx[u], (float)vy[u], (float)vz[u], (float)n0x, (float)n0y,
-                                                 (float)n0z, (float)vnT);
-              else sacc += mean_rel_speed((double)vx[u], (double)vy[u], (double)vz[u], n0x, n0y, n0z, vnT);
+        for (int u = 0; u < NU; u++) {
+          const int e = (int)threadIdx.x + 256 * u;
+          if (e < m) {
+            const uint32_t s = base + (uint32_t)e;
+            if (GATHER) {
+              b.pos[s] = P[u]; b.vel[s] = V[u];
+              b.cell[s] = cell[u];
+              if (src) src[s] = q[u];
+              if (SUMS)
+                sacc += (double)mean_rel_speed_f(V[u].x, V[u].y, V[u].z, (float)n0x, (float)n0y, (float)n0z,
+                                                 (float)vnT);
             }
+            rec[e] = DepRec<T>{P[u].x, P[u].y, P[u].z, V[u].x, V[u].y, V[u].z, P[u].w, 0.0f};
           }
-          DepRec<T> r;
-          if (ABS) {
-            int i;
-            double f;
-            cell_coord((double)x[u], g.p0[0], g.d[0], g.nx, i, f); r.fx = (T)f;
-            cell_coord((double)y[u], g.p0[1], g.d[1], g.ny, i, f); r.fy = (T)f;
-            cell_coord((double)z[u], g.p0[2], g.d[2], g.nz, i, f); r.fz = (T)f;
-          } else {
-            r.fx = x[u]; r.fy = y[u]; r.fz = z[u];
+        }
+      } else {
+        T x[NU], y[NU], z[NU], vx[NU], vy[NU], vz[NU], w[NU];
+        uint32_t pid[NU];
+        int32_t cell[NU];
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+          x[u] = st.x[q[u]]; y[u] = st.y[q[u]]; z[u] = st.z[q[u]];
+          vx[u] = st.vx[q[u]]; vy[u] = st.vy[q[u]]; vz[u] = st.vz[q[u]];
+          w[u] = weighted ? st.w[q[u]] : (T)1;
+          if (GATHER) { pid[u] = st.pid[q[u]]; cell[u] = st.cell[q[u]]; }
+        }
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+          const int e = (int)threadIdx.x + 256 * u;
+          if (e < m) {
+            const uint32_t s = base + (uint32_t)e;
+            if (GATHER) {
+              b.x[s] = x[u]; b.y[s] = y[u]; b.z[s] = z[u];
+              b.vx[s] = vx[u]; b.vy[s] = vy[u]; b.vz[s] = vz[u];
+              b.pid[s] = pid[u];
+              b.cell[s] = cell[u];
+              if (weighted) b.w[s] = w[u];
+              if (src) src[s] = q[u];
+              if (SUMS) sacc += mean_rel_speed((double)vx[u], (double)vy[u], (double)vz[u], n0x, n0y, n0z, vnT);
+            }
+            DepRec<T> r;
+            if (ABS) {
+              int i;
+              double f;
+              cell_coord((double)x[u], g.p0[0], g.d[0], g.nx, i, f); r.fx = (T)f;
+              cell_coord((double)y[u], g.p0[1], g.d[1], g.ny, i, f); r.fy = (T)f;
+              cell_coord((double)z[u], g.p0[2], g.d[2], g.nz, i, f); r.fz = (T)f;
+            } else {
+              r.fx = x[u]; r.fy = y[u]; r.fz = z[u];
+            }
+            r.vx = vx[u]; r.vy = vy[u]; r.vz = vz[u];
+            r.w = w[u];
+            r.pad = (T)0;
+            rec[e] = r;
           }
-          r.vx = vx[u]; r.vy = vy[u]; r.vz = vz[u];
-          r.w = w[u];
-          r.pad = (T)0;
-          rec[e] = r;
         }
       }
     };
@@ -760,12 +805,11 @@ __global__ __launch_bounds__(256) void k_init(Grid g, int64_t ppc, uint32_t k0, 
     if (pert == CHY_PERT_SINUSOIDAL_X) wgt = 1.0 + amp * sin(2.0 * M_PI * (x - g.p0[0]) / Lx);
     if (sd.w) sd.w[p] = wgt;
   } else {
-    sf.x[p] = (float)u0; sf.y[p] = (float)u1; sf.z[p] = (float)u2;  // exact: 24-bit uniforms
-    sf.cell[p] = (int32_t)c;
-    sf.vx[p] = (float)vx; sf.vy[p] = (float)vy; sf.vz[p] = (float)vz;
-    sf.pid[p] = (uint32_t)p;
     if (pert == CHY_PERT_SINUSOIDAL_X) wgt = 1.0 + amp * sin(2.0 * M_PI * ((double)i + u0) / (double)g.nx);
-    if (sf.w) sf.w[p] = (float)wgt;
+    // offsets exact (24-bit uniforms); the weight is 1 without a perturbation
+    sf.pos[p] = Vec4<float>{(float)u0, (float)u1, (float)u2, (float)wgt};
+    sf.vel[p] = Vec4<float>{(float)vx, (float)vy, (float)vz, __uint_as_float((uint32_t)p)};
+    sf.cell[p] = (int32_t)c;
   }
 }
 
@@ -942,8 +986,7 @@ int particles_normaliser(chy_ctx *ctx)
   const chy_params &p = ctx->prm;
   const unsigned nb = grid_for(ctx->np);
   if (ctx->mixed) {
-    const PState<float> &s = ctx->pf[ctx->cur];
-    hipLaunchKernelGGL((k_meanrel<float>), nb, kBlock, 0, ctx->stream, ctx->np, s.vx, s.vy, s.vz, p.vn0[0], p.vn0[1],
+    hipLaunchKernelGGL(k_meanrel_mixed, nb, kBlock, 0, ctx->stream, ctx->np, ctx->pf[ctx->cur].vel, p.vn0[0], p.vn0[1],
                        p.vn0[2], p.vnT, ctx->sred);
   } else {
     const PState<double> &s = ctx->pd[ctx->cur];
@@ -1015,12 +1058,12 @@ int particles_upload(chy_ctx *ctx, int64_t np, const double *pos, const double *
     CHY_HIP(hipStreamSynchronize(ctx->stream));
   } else {
     PState<float> &s = ctx->pf[ctx->cur];
-    std::vector<float> o[3], v[3];
+    std::vector<Vec4<float>> hp4(np), hv4(np);
     std::vector<int32_t> cell(np);
-    for (int a = 0; a < 3; a++) { o[a].resize(np); v[a].resize(np); }
     const int nn[3] = {g.nx, g.ny, g.nz};
     for (int64_t i = 0; i < np; i++) {
       int ci[3];
+      float o[3];
       for (int a = 0; a < 3; a++) {
         const double p0 = g.p0[a];
         const double x = host_wrap(pos[4 * i + a], p0, g.p1[a], g.L[a]);
@@ -1030,26 +1073,18 @@ int particles_upload(chy_ctx *ctx, int64_t np, const double *pos, const double *
         float f = (float)(xi - (double)ii);
         if (f >= 1.0f) { f = 0.0f; ii = (ii + 1) % nn[a]; }
         ci[a] = ii;
-        o[a][i] = f;
-        v[a][i] = (float)vel[4 * i + a];
+        o[a] = f;
       }
+      uint32_t id = hp[i];
+      float idf;
+      memcpy(&idf, &id, sizeof idf);
+      hp4[i] = Vec4<float>{o[0], o[1], o[2], w ? (float)w[i] : 1.0f};
+      hv4[i] = Vec4<float>{(float)vel[4 * i], (float)vel[4 * i + 1], (float)vel[4 * i + 2], idf};
       cell[i] = ci[0] + g.nx * (ci[1] + g.ny * ci[2]);
     }
-    float *dst[6] = {s.x, s.y, s.z, s.vx, s.vy, s.vz};
-    for (int a = 0; a < 3; a++) {
-      CHY_HIP(hipMemcpyAsync(dst[a], o[a].data(), sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
-      CHY_HIP(hipMemcpyAsync(dst[3 + a], v[a].data(), sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
-    }
+    CHY_HIP(hipMemcpyAsync(s.pos, hp4.data(), sizeof(Vec4<float>) * np, hipMemcpyHostToDevice, ctx->stream));
+    CHY_HIP(hipMemcpyAsync(s.vel, hv4.data(), sizeof(Vec4<float>) * np, hipMemcpyHostToDevice, ctx->stream));
     CHY_HIP(hipMemcpyAsync(s.cell, cell.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, ctx->stream));
-    CHY_HIP(hipMemcpyAsync(s.pid, hp.data(), sizeof(uint32_t) * np, hipMemcpyHostToDevice, ctx->stream));
-    std::vector<float> wf;
-    if (w) {
-      wf.resize(np);
-      for (int64_t i = 0; i < np; i++) wf[i] = (float)w[i];
-      CHY_HIP(hipMemcpyAsync(s.w, wf.data(), sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
-    } else if (ctx->weighted) {
-      launch_fill<float>(ctx, np, s.w, 1.0f);
-    }
     CHY_HIP(hipStreamSynchronize(ctx->stream));
   }
   return settle(ctx);
@@ -1086,39 +1121,29 @@ int particles_download(chy_ctx *ctx, double *pos, double *vel, double *w, uint32
     if (pid) CHY_HIP(hipMemcpy(pid, s.pid, sizeof(uint32_t) * np, hipMemcpyDeviceToHost));
   } else {
     const PState<float> &s = ctx->pf[ctx->cur];
-    std::vector<float> a[6];
+    std::vector<Vec4<float>> hp4(np), hv4(np);
     std::vector<int32_t> cell(np);
-    const float *src[6] = {s.x, s.y, s.z, s.vx, s.vy, s.vz};
-    for (int f = 0; f < 6; f++) {
-      a[f].resize(np);
-      CHY_HIP(hipMemcpyAsync(a[f].data(), src[f], sizeof(float) * np, hipMemcpyDeviceToHost, ctx->stream));
-    }
+    CHY_HIP(hipMemcpyAsync(hp4.data(), s.pos, sizeof(Vec4<float>) * np, hipMemcpyDeviceToHost, ctx->stream));
+    CHY_HIP(hipMemcpyAsync(hv4.data(), s.vel, sizeof(Vec4<float>) * np, hipMemcpyDeviceToHost, ctx->stream));
     CHY_HIP(hipMemcpyAsync(cell.data(), s.cell, sizeof(int32_t) * np, hipMemcpyDeviceToHost, ctx->stream));
     CHY_HIP(hipStreamSynchronize(ctx->stream));
     for (int64_t i = 0; i < np; i++) {
       const int c = cell[i];
       const int ci[3] = {c % g.nx, (c / g.nx) % g.ny, c / (g.nx * g.ny)};
+      const float o[3] = {hp4[i].x, hp4[i].y, hp4[i].z}, v[3] = {hv4[i].x, hv4[i].y, hv4[i].z};
       double r2 = 0.0, v2 = 0.0;
       for (int q = 0; q < 3; q++) {
-        const double x = g.p0[q] + ((double)ci[q] + (double)a[q][i]) * g.d[q];
+        const double x = g.p0[q] + ((double)ci[q] + (double)o[q]) * g.d[q];
         if (pos) pos[4 * i + q] = x;
-        if (vel) vel[4 * i + q] = (double)a[3 + q][i];
+        if (vel) vel[4 * i + q] = (double)v[q];
         r2 += x * x;
-        v2 += (double)a[3 + q][i] * (double)a[3 + q][i];
+        v2 += (double)v[q] * (double)v[q];
       }
       if (pos) pos[4 * i + 3] = sqrt(r2);
       if (vel) vel[4 * i + 3] = sqrt(v2);
+      if (w) w[i] = ctx->weighted ? (double)hp4[i].w : 1.0;
+      if (pid) memcpy(&pid[i], &hv4[i].w, sizeof(uint32_t));
     }
-    if (w) {
-      if (ctx->weighted) {
-        std::vector<float> wf(np);
-        CHY_HIP(hipMemcpy(wf.data(), s.w, sizeof(float) * np, hipMemcpyDeviceToHost));
-        for (int64_t i = 0; i < np; i++) w[i] = wf[i];
-      } else {
-        for (int64_t i = 0; i < np; i++) w[i] = 1.0;
-      }
-    }
-    if (pid) CHY_HIP(hipMemcpy(pid, s.pid, sizeof(uint32_t) * np, hipMemcpyDeviceToHost));
   }
   return CHY_OK;
 }
@@ -1145,10 +1170,7 @@ int particles_init(chy_ctx *ctx)
     hipLaunchKernelGGL((k_init<false>), nb, kBlock, 0, ctx->stream, ctx->g, ppc, (uint32_t)p.seed,
                        (uint32_t)(p.seed >> 32), p.vi0[0], p.vi0[1], p.vi0[2], p.viT[0], p.viT[1], p.viT[2],
                        p.perturbation, p.perturbation_amplitude, sd, sf);
-  if (ctx->weighted && p.perturbation == CHY_PERT_NONE) {
-    if (ctx->mixed) launch_fill<float>(ctx, np, sf.w, 1.0f);
-    else launch_fill<double>(ctx, np, sd.w, 1.0);
-  }
+  if (ctx->weighted && p.perturbation == CHY_PERT_NONE && !ctx->mixed) launch_fill<double>(ctx, np, sd.w, 1.0);
   return settle(ctx);
 }
 

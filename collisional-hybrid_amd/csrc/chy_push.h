@@ -203,7 +203,6 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
   return r;
 }
 
-template <typename T> struct alignas(4 * sizeof(T)) Vec4 { T x, y, z, w; };
 
 // Boris rotation (particles.c:374-447 with the intended cross products and
 // per-component v-minus); operation order shared with the oracle.

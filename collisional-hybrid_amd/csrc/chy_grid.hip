@@ -581,8 +581,31 @@ __global__ __launch_bounds__(256) void k_combine(const double *__restrict__ part
   }
 }
 
+// Block b of 256 sums partials [b * chunk, (b + 1) * chunk) in a fixed order.
+__global__ __launch_bounds__(256) void k_chunk_sum(const double *__restrict__ partial, int64_t n, int64_t chunk,
+                                                   double *__restrict__ out)
+{
+  __shared__ double sh[4];
+  const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+  double acc = 0.0;
+#pragma unroll 8
+  for (int64_t b = lo + threadIdx.x; b < hi; b += blockDim.x) acc += partial[b];
+  const double t = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
 void launch_combine_sum(chy_ctx *ctx, const double *partials, int nblk, int nv, double *out)
 {
+  if (nv == 1 && nblk > 4 * kRedBlocks) {
+    // one partial per deposit block (nc / 32 of them): two levels
+    constexpr int kChunks = 256;
+    const int64_t chunk = ((int64_t)nblk + kChunks - 1) / kChunks;
+    hipLaunchKernelGGL(k_chunk_sum, kChunks, kBlock, 0, ctx->stream, partials, (int64_t)nblk, chunk, ctx->red);
+    hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, kChunks, 1, out, (const double *)nullptr);
+    return;
+  }
   hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, partials, nblk, nv, out, (const double *)nullptr);
 }
 

@@ -633,6 +633,17 @@ __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *_
     // then the writes, in straight-line code so all loads are in flight
     // together.  Slots past the chunk load ion `base` again (always present)
     // and write nothing.
+    // the new cell of slot s is the block's cell whose segment holds s (the
+    // sort put the ion there), so it is not read from the source
+    auto cell_of = [&](uint32_t s) {
+      int lo = 0, hi = ncell;  // so[lo] <= s < so[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (so[mid] <= s) lo = mid;
+        else hi = mid;
+      }
+      return (int32_t)(c0 + lo);
+    };
     auto stage = [&](auto passes) {
       constexpr int NU = decltype(passes)::value;
       uint32_t q[NU];
@@ -645,11 +656,9 @@ __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *_
       if constexpr (sizeof(T) == 4) {
         // mixed: two 16-byte records and the key per ion
         Vec4<float> P[NU], V[NU];
-        int32_t cell[NU];
 #pragma unroll
         for (int u = 0; u < NU; u++) {
           P[u] = st.pos[q[u]]; V[u] = st.vel[q[u]];
-          if (GATHER) cell[u] = st.cell[q[u]];
         }
 #pragma unroll
         for (int u = 0; u < NU; u++) {
@@ -658,7 +667,7 @@ __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *_
             const uint32_t s = base + (uint32_t)e;
             if (GATHER) {
               b.pos[s] = P[u]; b.vel[s] = V[u];
-              b.cell[s] = cell[u];
+              b.cell[s] = cell_of(s);
               if (src) src[s] = q[u];
               if (SUMS)
                 sacc += (double)mean_rel_speed_f(V[u].x, V[u].y, V[u].z, (float)n0x, (float)n0y, (float)n0z,
@@ -670,13 +679,12 @@ __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *_
       } else {
         T x[NU], y[NU], z[NU], vx[NU], vy[NU], vz[NU], w[NU];
         uint32_t pid[NU];
-        int32_t cell[NU];
 #pragma unroll
         for (int u = 0; u < NU; u++) {
           x[u] = st.x[q[u]]; y[u] = st.y[q[u]]; z[u] = st.z[q[u]];
           vx[u] = st.vx[q[u]]; vy[u] = st.vy[q[u]]; vz[u] = st.vz[q[u]];
           w[u] = weighted ? st.w[q[u]] : (T)1;
-          if (GATHER) { pid[u] = st.pid[q[u]]; cell[u] = st.cell[q[u]]; }
+          if (GATHER) pid[u] = st.pid[q[u]];
         }
 #pragma unroll
         for (int u = 0; u < NU; u++) {
@@ -687,7 +695,7 @@ __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *_
               b.x[s] = x[u]; b.y[s] = y[u]; b.z[s] = z[u];
               b.vx[s] = vx[u]; b.vy[s] = vy[u]; b.vz[s] = vz[u];
               b.pid[s] = pid[u];
-              b.cell[s] = cell[u];
+              b.cell[s] = cell_of(s);
               if (weighted) b.w[s] = w[u];
               if (src) src[s] = q[u];
               if (SUMS) sacc += mean_rel_speed((double)vx[u], (double)vy[u], (double)vz[u], n0x, n0y, n0z, vnT);

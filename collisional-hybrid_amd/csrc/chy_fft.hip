@@ -823,12 +823,12 @@ struct H2 {
   int nb;
 };
 
-static H2 h2_layout(const chy_ctx *ctx)
+static H2 h2_layout(const chy_ctx *ctx, bool fp32)
 {
   const int64_t nxy = (int64_t)ctx->g.nx * ctx->g.ny;
   H2 h;
   h.nb = nb2_of(nxy);
-  h.nlb = lhs_zsum_blocks(ctx->g);
+  h.nlb = lhs_zsum_blocks(ctx->g, fp32);
   h.part = ctx->pc2_h;
   h.pl = h.part + (int64_t)lhs_zchunks(ctx->g) * nxy;
   h.pn = h.pl + h.nlb;
@@ -842,7 +842,8 @@ int64_t fft2_h_elems(const chy_ctx *ctx)
 {
   const int64_t nxy = (int64_t)ctx->g.nx * ctx->g.ny;
   const int nb = nb2_of(nxy);
-  return (int64_t)lhs_zchunks(ctx->g) * nxy + lhs_zsum_blocks(ctx->g) + (int64_t)(kHm + 1) * nb +
+  const int64_t nlb = std::max(lhs_zsum_blocks(ctx->g, false), lhs_zsum_blocks(ctx->g, true));
+  return (int64_t)lhs_zchunks(ctx->g) * nxy + nlb + (int64_t)(kHm + 1) * nb +
          2 * (int64_t)kHm * kHm * nb + kHm;
 }
 
@@ -1128,7 +1129,7 @@ int fft2_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z, T *w3)
   const int64_t nxy = (int64_t)nx * ny;
   T *C = (T *)ctx->pc2_buf, *V = C + 9 * nxy, *Z = V + (kFft2Its + 1) * nxy, *w = Z + kFft2Its * nxy;
   cpx<T> *cb = (cpx<T> *)(w + nxy);
-  const H2 h = h2_layout(ctx);
+  const H2 h = h2_layout(ctx, sizeof(T) == 4);
   const int nb = h.nb;
   // z-averaged residual of the first level, mean removed, into V0
   launch_lhs_zsum<T>(ctx, ctx->prm.lhs_type, (const T *)ctx->mom[0], z, r, h.part, h.pl);

@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_residual(Grid g, int lhs_type, SCoef s,
 
 // ---- LDS-tiled (2.5-D) operator apply ------------------------------------------
 // A 64 x 4 thread block owns a (64 x 16) column of nodes (4 rows per thread)
-// over kZChunk planes.
+// over kZChunk planes (TW = 256: a 256 x 1 block owns 256 x 8, 8 rows each).
 // It keeps three z-planes of x and n, each with a one-node halo in x and y
 // ((64+2) x (4+2) values), in LDS and marches up in z, loading one plane per
 // step with coalesced x-fastest reads.  Output (OUT): 0 y = A x, 1 r = b - A x,
@@ -182,39 +182,44 @@ constexpr int kHX = kTX + 2, kHY = kTY + 2;
 // computed from LDS, so global latency overlaps the arithmetic.
 constexpr int kPer = (kHX * kHY + 255) / 256;
 
-template <typename TN, typename TX, typename TY, typename C, bool DIFF, int OUT, typename TB = double>
+template <typename TN, typename TX, typename TY, typename C, bool DIFF, int OUT, typename TB = double, int TW = kTX>
 __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s, LapRow lap,
                                                    const TN *__restrict__ n, const TX *__restrict__ x,
                                                    const TB *__restrict__ b, TY *__restrict__ y,
                                                    double *__restrict__ pl)
 {
-  __shared__ C xs[3][kHY][kHX];
-  __shared__ C ns[3][kHY][kHX];
+  // tile TW x TY_ (TW = 64: 64 x 16; TW = 256, a whole x row of a 256-wide
+  // grid: 256 x 8 with no x halo beyond the row's own periodic wrap), GW row
+  // groups of TW threads, RW rows per thread
+  constexpr int TY_ = TW == 256 ? 8 : kTY, GW = 256 / TW, RW = TY_ / GW;
+  constexpr int HX = TW + 2, HY = TY_ + 2, PER = (HX * HY + 255) / 256;
+  __shared__ C xs[3][HY][HX];
+  __shared__ C ns[3][HY][HX];
   const bool full = lhs_type == CHY_LHS_FULL;
-  const int tx = threadIdx.x & (kTX - 1), ty0 = threadIdx.x >> 6;
-  const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, z0 = blockIdx.z * kZChunk;
+  const int tx = threadIdx.x % TW, ty0 = threadIdx.x / TW;
+  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TY_, z0 = blockIdx.z * kZChunk;
   const int z1 = z0 + kZChunk < g.nz ? z0 + kZChunk : g.nz;
   const int i = x0 + tx;
   const int64_t pstride = (int64_t)g.nx * g.ny;
   // in-plane offsets and LDS slots of this thread's halo elements
-  int64_t off[kPer];
-  int slot[kPer];
+  int64_t off[PER];
+  int slot[PER];
 #pragma unroll
-  for (int q = 0; q < kPer; q++) {
+  for (int q = 0; q < PER; q++) {
     const int e = threadIdx.x + 256 * q;
-    slot[q] = e < kHX * kHY ? e : -1;
-    const int hy = e / kHX, hx = e - hy * kHX;
+    slot[q] = e < HX * HY ? e : -1;
+    const int hy = e / HX, hx = e - hy * HX;
     // tiles may overhang small grids by more than one period: full modulo
     int ii = x0 + hx - 1, jj = y0 + hy - 1;
     ii = ((ii % g.nx) + g.nx) % g.nx;
     jj = ((jj % g.ny) + g.ny) % g.ny;
     off[q] = (int64_t)ii + (int64_t)g.nx * jj;
   }
-  C rx[kPer], rn[kPer];
+  C rx[PER], rn[PER];
   auto fetch = [&](int k) {
     const int64_t base = pstride * wrapi(k, g.nz);  // k in [-1, nz]
 #pragma unroll
-    for (int q = 0; q < kPer; q++) {
+    for (int q = 0; q < PER; q++) {
       if (slot[q] >= 0) {
         rx[q] = (C)x[base + off[q]];
         if (full) rn[q] = (C)n[base + off[q]];
@@ -224,22 +229,22 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
   auto store = [&](int buf) {
     C *xb = &xs[buf][0][0], *nb = &ns[buf][0][0];
 #pragma unroll
-    for (int q = 0; q < kPer; q++) {
+    for (int q = 0; q < PER; q++) {
       if (slot[q] >= 0) {
         xb[slot[q]] = rx[q];
         if (full) nb[slot[q]] = rn[q];
       }
     }
   };
-  double zs[kRows];
+  double zs[RW];
 #pragma unroll
-  for (int rr = 0; rr < kRows; rr++) zs[rr] = 0.0;
+  for (int rr = 0; rr < RW; rr++) zs[rr] = 0.0;
   // b at this thread's nodes of the current plane, and of the next one in flight
-  TB rb[kRows], rbn[kRows];
-  auto fetch_b = [&](int k, TB(&dst)[kRows]) {
+  TB rb[RW], rbn[RW];
+  auto fetch_b = [&](int k, TB(&dst)[RW]) {
 #pragma unroll
-    for (int rr = 0; rr < kRows; rr++) {
-      const int j = y0 + ty0 + 4 * rr;
+    for (int rr = 0; rr < RW; rr++) {
+      const int j = y0 + ty0 + GW * rr;
       if (i < g.nx && j < g.ny) dst[rr] = b[(int64_t)i + (int64_t)g.nx * ((int64_t)j + (int64_t)g.ny * k)];
     }
   };
@@ -258,8 +263,8 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
     }
     const int pm = (k + 2) % 3, pc = k % 3, pp = (k + 1) % 3;
 #pragma unroll
-    for (int rr = 0; rr < kRows; rr++) {
-      const int ty = ty0 + 4 * rr, j = y0 + ty;
+    for (int rr = 0; rr < RW; rr++) {
+      const int ty = ty0 + GW * rr, j = y0 + ty;
       if (i < g.nx && j < g.ny) {
         const int hx = tx + 1, hy = ty + 1;
         C v[19];
@@ -292,7 +297,7 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
     }
     if (OUT != 0) {
 #pragma unroll
-      for (int rr = 0; rr < kRows; rr++) rb[rr] = rbn[rr];
+      for (int rr = 0; rr < RW; rr++) rb[rr] = rbn[rr];
     }
     __syncthreads();
   }
@@ -300,8 +305,8 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
     __shared__ double wsum[4];
     double t = 0.0;
 #pragma unroll
-    for (int rr = 0; rr < kRows; rr++) {
-      const int j = y0 + ty0 + 4 * rr;
+    for (int rr = 0; rr < RW; rr++) {
+      const int j = y0 + ty0 + GW * rr;
       if (i < g.nx && j < g.ny) {
         y[(int64_t)blockIdx.z * pstride + i + (int64_t)g.nx * j] = (TY)zs[rr];
         t += zs[rr];
@@ -317,11 +322,19 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
   }
 }
 
-static inline dim3 tiled_grid(const Grid &g)
+static inline dim3 tiled_grid(const Grid &g, int tw = kTX)
 {
-  return dim3((unsigned)((g.nx + kTX - 1) / kTX), (unsigned)((g.ny + kTY - 1) / kTY),
+  const int ty = tw == 256 ? 8 : kTY;
+  return dim3((unsigned)((g.nx + tw - 1) / tw), (unsigned)((g.ny + ty - 1) / ty),
               (unsigned)((g.nz + kZChunk - 1) / kZChunk));
 }
+
+// The FP32 residual passes of the preconditioner (b - A x, and its z-chunk
+// column sums) on a 256-wide grid take whole rows as tiles: the one-node x
+// halo of a 64-wide tile costs two extra cache lines per row (twice the x
+// traffic), a whole row's halo is its own wrap.  (A x alone, with a third
+// less traffic, ran faster at the 64-wide tiles' higher occupancy.)
+static inline bool wide_rows(const Grid &g, bool fp32) { return fp32 && g.nx == 256; }
 
 static LapRow lap_row(chy_ctx *ctx, int lhs_type)
 {
@@ -348,9 +361,9 @@ void launch_lhs_apply(chy_ctx *ctx, int lhs_type, bool transpose, const TN *n, c
 }
 
 int lhs_zchunks(const Grid &g) { return (g.nz + kZChunk - 1) / kZChunk; }
-int64_t lhs_zsum_blocks(const Grid &g)
+int64_t lhs_zsum_blocks(const Grid &g, bool fp32)
 {
-  const dim3 d = tiled_grid(g);
+  const dim3 d = tiled_grid(g, wide_rows(g, fp32) ? 256 : kTX);
   return (int64_t)d.x * d.y * d.z;
 }
 
@@ -361,6 +374,13 @@ void launch_lhs_zsum(chy_ctx *ctx, int lhs_type, const T *n, const T *x, const T
   const LapRow L = lap_row(ctx, lhs_type);
   constexpr bool fp32 = sizeof(T) == 4;
   using C = typename std::conditional<fp32, float, double>::type;
+  if constexpr (fp32) {
+    if (wide_rows(ctx->g, fp32)) {
+      hipLaunchKernelGGL((k_lhs_tiled<T, T, double, C, fp32, 2, T, 256>), tiled_grid(ctx->g, 256), kBlock, 0,
+                         ctx->stream, ctx->g, lhs_type, s, L, n, x, b, part, pl);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_lhs_tiled<T, T, double, C, fp32, 2, T>), tiled_grid(ctx->g), kBlock, 0, ctx->stream, ctx->g,
                      lhs_type, s, L, n, x, b, part, pl);
 }
@@ -1296,6 +1316,13 @@ void launch_residual_t(chy_ctx *ctx, int lhs_type, const T *n, const T *b, const
   const LapRow L = lap_row(ctx, lhs_type);
   constexpr bool fp32 = sizeof(T) == 4;
   using C = typename std::conditional<fp32, float, double>::type;
+  if constexpr (fp32) {
+    if (wide_rows(ctx->g, fp32)) {
+      hipLaunchKernelGGL((k_lhs_tiled<T, T, T, C, fp32, 1, T, 256>), tiled_grid(ctx->g, 256), kBlock, 0,
+                         ctx->stream, ctx->g, lhs_type, s, L, n, x, b, res, (double *)nullptr);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_lhs_tiled<T, T, T, C, fp32, 1, T>), tiled_grid(ctx->g), kBlock, 0, ctx->stream, ctx->g,
                      lhs_type, s, L, n, x, b, res, (double *)nullptr);
 }

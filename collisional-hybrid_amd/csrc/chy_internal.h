@@ -219,7 +219,7 @@ void launch_residual(chy_ctx *ctx, int lhs_type, const TN *n, const double *b, c
 template <typename T>
 void launch_lhs_zsum(chy_ctx *ctx, int lhs_type, const T *n, const T *x, const T *b, double *part, double *pl);
 int lhs_zchunks(const Grid &g);
-int64_t lhs_zsum_blocks(const Grid &g);
+int64_t lhs_zsum_blocks(const Grid &g, bool fp32);  // FP32 or FP64 stencil's grid
 template <typename T>
 void launch_efield(chy_ctx *ctx, const double *phi, T *E4);
 template <typename T>

@@ -13,8 +13,10 @@
 //           partial sums in sorted order, then per-node sums over the 8
 //           adjacent cells in fixed corner order (particles.c:136-255).
 //
-// FP64 state: absolute x,y,z, v (f64), pid, optional weight, cell key.
-// Mixed state: int32 cell + FP32 in-cell offsets, FP32 v, pid, weight.
+// FP64 state: one array per field -- absolute x,y,z, v (f64), pid, optional
+// weight, cell key.  Mixed state: per ion the 16-byte records
+// (ox, oy, oz, w) and (vx, vy, vz, id) of FP32 in-cell offsets, weight,
+// velocity and id bits, and the int32 cell key.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -597,9 +599,9 @@ __global__ __launch_bounds__(256) void k_perm(Grid g, int64_t np, const int32_t 
 // src[s] = perm[s]), and the ion is deposited from LDS, so the sorted state
 // is not read back; with SUMS the block's part of the collision normaliser
 // over the final velocities in slot order goes to sred[block].
-constexpr int kDepCells = 32;
-constexpr int kDepUnroll = 3;
-template <typename T> constexpr int dep_chunk() { return sizeof(T) == 4 ? 640 : 640; }
+constexpr int kDepCells = 32;   // about 512 ions at 16 per cell: one chunk
+constexpr int kDepUnroll = 3;   // staging passes of the block per chunk
+constexpr int kDepChunk = 640;  // ions per chunk (LDS records)
 
 // One staged ion: in-cell offsets, velocity and weight.
 template <typename T> struct alignas(8 * sizeof(T)) DepRec { T fx, fy, fz, vx, vy, vz, w, pad; };
@@ -611,7 +613,7 @@ __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *_
                                                        double n0y, double n0z, double vnT, double *__restrict__ sred,
                                                        uint32_t *__restrict__ src)
 {
-  constexpr int kChunk = dep_chunk<T>();
+  constexpr int kChunk = kDepChunk;
   static_assert(kChunk <= kDepUnroll * 256, "a chunk is staged in kDepUnroll passes of the block");
   __shared__ DepRec<T> rec[kChunk];
   __shared__ uint32_t so[kDepCells + 1];

@@ -189,6 +189,7 @@ def run_gpu(N, ppc, prec, steps, warmup, keep=False, cross=None):
         "solve_outer_per_step": sum(s.solve.outer for s in stats) / steps,
         "max_residual": max(s.solve.residual for s in stats),
         "collisions_per_step": sum(s.collisions.accepted for s in stats) / steps,
+        "refused_per_step": sum(s.collisions.refused for s in stats) / steps,
         "collisions_target": stats[-1].collisions.Nc,
         "kernels": kern,
         "roofline_kernel": roof_kernel,
@@ -451,6 +452,7 @@ def main():
         "solve_outer_per_step": mixed["solve_outer_per_step"],
         "max_solve_residual": mixed["max_residual"],
         "collisions_per_step": mixed["collisions_per_step"],
+        "refused_per_step": mixed["refused_per_step"],
         "collisions_target_per_step": mixed["collisions_target"],
         "kernels": {k: {q: round(v, 4) if isinstance(v, float) else v for q, v in d.items()}
                     for k, d in kern.items()},

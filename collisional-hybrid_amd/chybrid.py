@@ -54,7 +54,7 @@ class Params(C.Structure):
         ("ksp_inner_rtol", C.c_double), ("ksp_gmres_cgs_refinement", C.c_int32), ("ksp_initial_guess", C.c_int32),
         ("seed", C.c_uint64), ("ppc", C.c_int64), ("perturbation", C.c_int32),
         ("perturbation_amplitude", C.c_double), ("weighted", C.c_int32),
-        ("precision", C.c_int32), ("output_interval", C.c_int64),
+        ("precision", C.c_int32), ("output_interval", C.c_int64), ("collision_refusal_limit", C.c_int64),
         ("flux_scale", D3), ("input", C.c_char * 4096), ("output", C.c_char * 4096),
     ]
 

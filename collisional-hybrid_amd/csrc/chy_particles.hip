@@ -1228,7 +1228,8 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
     stats->Nc = Nc;
   }
   if (cs[3]) return set_err(ctx, CHY_ERR_FP, "non-finite particle position or velocity after the push");
-  if (cs[1] >= 10)
+  const int64_t lim = ctx->prm.collision_refusal_limit;  // particles.c:603 (10)
+  if (lim > 0 && (int64_t)cs[1] >= lim)
     return set_err(ctx, CHY_ERR_SIG, "Failed to collide " + std::to_string(cs[1]) + " ion-neutral pairs. Aborting.");
   return CHY_OK;
 }

@@ -203,6 +203,7 @@ int chy_params_default(chy_params *p)
   p->ksp_restart = 30;
   p->pc_cheby_degree = 4;
   p->ksp_inner_rtol = 2e-5;
+  p->collision_refusal_limit = 10;  /* particles.c:603 */
   p->ksp_gmres_cgs_refinement = CHY_CGS_REFINE_NEVER;  /* PETSc default */
   p->precision = CHY_PREC_FP64;
   p->output_interval = 1;
@@ -392,6 +393,16 @@ int chy_params_from_options(int argc, const char *const *argv, chy_params *p, ch
   get_real(&db, "-ksp_inner_rtol", &p->ksp_inner_rtol, &e);
   get_enum(&db, "-ksp_gmres_cgs_refinement_type", CgsRefineNames, &p->ksp_gmres_cgs_refinement, &e);
   get_enum(&db, "-ksp_initial_guess", GuessNames, &p->ksp_initial_guess, &e);
+  {
+    /* the reference aborts a step at its 10th refused scattering; with ~1e9
+       ions and tens of millions of scatterings per step a few refusals are
+       routine, so the limit is a flag (0: never abort) */
+    int64_t lim = p->collision_refusal_limit;
+    if (get_int(&db, "--collision-refusal-limit", &lim, &e)) {
+      if (lim < 0) emsg(&e, CHY_ERR_ARG_RANGE, "--collision-refusal-limit must be >= 0\n");
+      else p->collision_refusal_limit = lim;
+    }
+  }
   {
     int64_t deg = p->pc_cheby_degree;
     if (get_int(&db, "-pc_cheby_degree", &deg, &e)) {

@@ -105,6 +105,7 @@ typedef struct {
   int32_t weighted;      /* allocate per-particle weights (implied by a perturbation) */
   int32_t precision;
   int64_t output_interval;
+  int64_t collision_refusal_limit; /* refused scatterings in one step that abort it (particles.c:603: 10); 0 = never */
   /* solver binary (parameters.c:419-440) */
   double  flux_scale[3];
   char    input[4096];

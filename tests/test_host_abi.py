@@ -124,6 +124,12 @@ def test_solver_option_mapping():
                     ("extrapolate", chybrid.GUESS_EXTRAPOLATE)):
         p, _ = chybrid.params_from_options(["-ksp_initial_guess", name])
         assert p.ksp_initial_guess == v, name
+    # the reference's abort at the 10th refused scattering of a step, as a flag
+    assert chybrid.params_from_options([])[0].collision_refusal_limit == 10
+    assert chybrid.params_from_options("--collision-refusal-limit 0".split())[0].collision_refusal_limit == 0
+    assert chybrid.params_from_options("--collision-refusal-limit 1000".split())[0].collision_refusal_limit == 1000
+    with pytest.raises(chybrid.ChyError):
+        chybrid.params_from_options("--collision-refusal-limit -1".split())
     with pytest.raises(chybrid.ChyError):
         chybrid.params_from_options("-pc_cheby_degree 0".split())
 

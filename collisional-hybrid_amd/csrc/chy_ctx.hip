@@ -371,13 +371,17 @@ int solve_impl(chy_ctx *ctx, chy_solve_stats *st)
   T *rhsT = (T *)ctx->t64;  // FP64-sized buffer holds the inner right-hand side
   T *ev = (T *)ctx->V + (nvec_V(ctx) - 1) * nc;
   for (;;) {
+    // r = b - A phi in FP64 arithmetic, stored in the inner precision as the
+    // inner solve's right-hand side (unscaled), and |r|^2 in FP64
     if (ctx->solve_transpose) {
       launch_lhs_apply<T, double, double>(ctx, p.lhs_type, true, n, ctx->phi, ctx->r64);
       launch_axpby<double>(ctx, nc, 1.0, ctx->b, -1.0, ctx->r64);
+      launch_mdot<double>(ctx, 1, ctx->r64, nc, ctx->r64, nc, ctx->red_out);
+      launch_convert<double, T>(ctx, nc, ctx->r64, 1.0, rhsT);
     } else {
-      launch_residual<T, double>(ctx, p.lhs_type, n, ctx->b, ctx->phi, ctx->r64);
+      launch_residual_norm<T, T>(ctx, p.lhs_type, n, ctx->b, ctx->phi, rhsT, ctx->rpl);
+      launch_combine_sum(ctx, ctx->rpl, (int)lhs_zsum_blocks(ctx->g, false), 1, ctx->red_out);
     }
-    launch_mdot<double>(ctx, 1, ctx->r64, nc, ctx->r64, nc, ctx->red_out);
     double rr;
     TRY(fetch(ctx, ctx->red_out, 1, &rr));
     rn = sqrt(rr);
@@ -392,7 +396,6 @@ int solve_impl(chy_ctx *ctx, chy_solve_stats *st)
     // to tol
     const double inner_rtol =
         ctx->mixed ? std::max(p.ksp_inner_rtol, 0.5 * tol / rn) : std::min(1.0, 0.5 * tol / rn);
-    launch_convert<double, T>(ctx, nc, ctx->r64, 1.0 / rn, rhsT);
     int64_t it_in = 0;
     double rel = 1.0;
     if (p.ksp_type == CHY_KSP_BCGS)
@@ -401,9 +404,8 @@ int solve_impl(chy_ctx *ctx, chy_solve_stats *st)
       TRY(gmres_inner<T>(ctx, v19, rhsT, ev, inner_rtol, p.ksp_max_it - its, &it_in, &rel));
     its += it_in;
     outer++;
-    // phi += rn * e
-    launch_convert<T, double>(ctx, nc, ev, rn, ctx->r64);
-    launch_axpby<double>(ctx, nc, 1.0, ctx->r64, 1.0, ctx->phi);
+    // phi += e
+    launch_add_scaled<T>(ctx, nc, ev, 1.0, ctx->phi);
     if (it_in == 0) break;
   }
   // mean projection of phi (null space)
@@ -535,6 +537,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
     A(dalloc(ctx, &q, (int64_t)ts * 3 * nc)); ctx->pc_buf = q;
   }
   A(dalloc(ctx, &ctx->red, (int64_t)kRedBlocks * 32));
+  A(dalloc(ctx, &ctx->rpl, lhs_zsum_blocks(g, false)));
   A(dalloc(ctx, &ctx->red_out, 3 * (ctx->restart + 2) + 16));
   if (hipHostMalloc((void **)&ctx->h_red, sizeof(double) * (3 * (ctx->restart + 2) + 16)) != hipSuccess ||
       hipHostMalloc((void **)&ctx->h_scale, sizeof(double) * (ctx->restart + 2)) != hipSuccess) {
@@ -574,7 +577,7 @@ int chy_destroy(chy_ctx *ctx)
                 ctx->partials,
                 ctx->sred, ctx->S_dev, ctx->cstats, ctx->mom[0], ctx->mom[1], ctx->mom[2], ctx->mom[3],
                 ctx->efield[0], ctx->phi, ctx->phi_prev, ctx->b, ctx->r64, ctx->t64, ctx->V,
-                ctx->Z, ctx->wv, ctx->fft_buf, ctx->pc_buf, ctx->pc2_buf, ctx->pc2_h, ctx->zl_buf, ctx->zl_col, ctx->red, ctx->red_out, ctx->gs_scale, ctx->tw[0], ctx->tw[1], ctx->tw[2],
+                ctx->Z, ctx->wv, ctx->fft_buf, ctx->pc_buf, ctx->pc2_buf, ctx->pc2_h, ctx->zl_buf, ctx->zl_col, ctx->red, ctx->rpl, ctx->red_out, ctx->gs_scale, ctx->tw[0], ctx->tw[1], ctx->tw[2],
                 ctx->trig[0], ctx->trig[1], ctx->trig[2]};
   for (void *p : ps) if (p) (void)hipFree(p);
   if (ctx->h_red) (void)hipHostFree(ctx->h_red);

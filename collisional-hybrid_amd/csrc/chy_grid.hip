@@ -172,7 +172,8 @@ __global__ __launch_bounds__(256) void k_residual(Grid g, int lhs_type, SCoef s,
 // ((64+2) x (4+2) values), in LDS and marches up in z, loading one plane per
 // step with coalesced x-fastest reads.  Output (OUT): 0 y = A x, 1 r = b - A x,
 // 2 the column sums over the block's planes of b - A x, y[zc * nx * ny + col]
-// for z chunk zc, plus the block's total in pl[block] (both in double).
+// for z chunk zc, plus the block's total in pl[block] (both in double),
+// 3 r = b - A x and the block's sum of r^2 (in double) in pl[block].
 constexpr int kTX = 64, kTY = 16, kRows = kTY / 4, kZChunk = 16;  // 64 x 4 threads, 4 rows each
 constexpr int kHX = kTX + 2, kHY = kTY + 2;
 
@@ -292,7 +293,11 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
         const int64_t c = (int64_t)i + (int64_t)g.nx * ((int64_t)j + (int64_t)g.ny * k);
         if (OUT == 1) y[c] = (TY)((double)rb[rr] - (double)acc);
         else if (OUT == 2) zs[rr] += (double)rb[rr] - (double)acc;
-        else y[c] = (TY)acc;
+        else if (OUT == 3) {
+          const double rv = (double)rb[rr] - (double)acc;
+          y[c] = (TY)rv;
+          zs[rr] += rv * rv;
+        } else y[c] = (TY)acc;
       }
     }
     if (OUT != 0) {
@@ -301,14 +306,14 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
     }
     __syncthreads();
   }
-  if (OUT == 2) {
+  if (OUT == 2 || OUT == 3) {
     __shared__ double wsum[4];
     double t = 0.0;
 #pragma unroll
     for (int rr = 0; rr < RW; rr++) {
       const int j = y0 + ty0 + GW * rr;
       if (i < g.nx && j < g.ny) {
-        y[(int64_t)blockIdx.z * pstride + i + (int64_t)g.nx * j] = (TY)zs[rr];
+        if (OUT == 2) y[(int64_t)blockIdx.z * pstride + i + (int64_t)g.nx * j] = (TY)zs[rr];
         t += zs[rr];
       }
     }
@@ -383,6 +388,33 @@ void launch_lhs_zsum(chy_ctx *ctx, int lhs_type, const T *n, const T *x, const T
   }
   hipLaunchKernelGGL((k_lhs_tiled<T, T, double, C, fp32, 2, T>), tiled_grid(ctx->g), kBlock, 0, ctx->stream, ctx->g,
                      lhs_type, s, L, n, x, b, part, pl);
+}
+
+// r = b - A x in FP64 arithmetic, stored as TY, and one partial of |r|^2 per
+// stencil block in pl (lhs_zsum_blocks(g, false) of them).
+template <typename TN, typename TY>
+void launch_residual_norm(chy_ctx *ctx, int lhs_type, const TN *n, const double *b, const double *x, TY *r,
+                          double *pl)
+{
+  const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
+  const LapRow L = lap_row(ctx, lhs_type);
+  hipLaunchKernelGGL((k_lhs_tiled<TN, double, TY, double, false, 3>), tiled_grid(ctx->g), kBlock, 0, ctx->stream,
+                     ctx->g, lhs_type, s, L, n, x, b, r, pl);
+}
+
+// y += a x, x of any precision, y FP64
+template <typename TS>
+__global__ __launch_bounds__(256) void k_add_scaled(int64_t n, const TS *__restrict__ x, double a, double *__restrict__ y)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (double)x[i] * a + y[i];
+}
+
+template <typename TS>
+void launch_add_scaled(chy_ctx *ctx, int64_t n, const TS *x, double a, double *y)
+{
+  const unsigned nb = (unsigned)std::min<int64_t>(8192, (n + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((k_add_scaled<TS>), nb, kBlock, 0, ctx->stream, n, x, a, y);
 }
 
 template <typename TN, typename TX>
@@ -1395,6 +1427,12 @@ template void launch_lhs_zsum<double>(chy_ctx *, int, const double *, const doub
 template void launch_lhs_zsum<float>(chy_ctx *, int, const float *, const float *, const float *, double *, double *);
 template void launch_residual<double, double>(chy_ctx *, int, const double *, const double *, const double *, double *);
 template void launch_residual<float, double>(chy_ctx *, int, const float *, const double *, const double *, double *);
+template void launch_residual_norm<double, double>(chy_ctx *, int, const double *, const double *, const double *,
+                                                   double *, double *);
+template void launch_residual_norm<float, float>(chy_ctx *, int, const float *, const double *, const double *, float *,
+                                                 double *);
+template void launch_add_scaled<double>(chy_ctx *, int64_t, const double *, double, double *);
+template void launch_add_scaled<float>(chy_ctx *, int64_t, const float *, double, double *);
 template void launch_zcoef<double>(chy_ctx *, int, const double *, double *);
 template void launch_zcoef<float>(chy_ctx *, int, const float *, float *);
 template void launch_zline<double>(chy_ctx *, int, const double *, const double *, double *, double *, double *);

@@ -114,6 +114,7 @@ struct chy_ctx {
   void *V = nullptr, *Z = nullptr, *wv = nullptr;
   double *red = nullptr;        // reduction partials [nblk][restart+2]
   double *red_out = nullptr;    // reduced values [restart+2]
+  double *rpl = nullptr;        // per-stencil-block partials of the outer residual's |r|^2
   double *h_red = nullptr;      // pinned host mirror of red_out
   double *h_scale = nullptr;    // pinned: squared scales of the Krylov basis (GMRES)
   double *gs_scale = nullptr;   // their device copy
@@ -214,6 +215,11 @@ template <typename TN, typename TX, typename TY>
 void launch_lhs_apply(chy_ctx *ctx, int lhs_type, bool transpose, const TN *n, const TX *x, TY *y);
 template <typename TN, typename TX>
 void launch_residual(chy_ctx *ctx, int lhs_type, const TN *n, const double *b, const TX *x, double *r);
+template <typename TN, typename TY>
+void launch_residual_norm(chy_ctx *ctx, int lhs_type, const TN *n, const double *b, const double *x, TY *r,
+                          double *pl);
+template <typename TS>
+void launch_add_scaled(chy_ctx *ctx, int64_t n, const TS *x, double a, double *y);
 // part[zc * nx * ny + col] = sum over the planes of z chunk zc of (b - A x) at
 // the column; pl[blk] = the sum over each stencil block (lhs_zsum_blocks of them)
 template <typename T>

@@ -130,9 +130,12 @@ int fetch(chy_ctx *ctx, const double *dev, int nv, double *host)
 }
 
 // ---- inner GMRES on A e = rhs (e starts at zero), precision T -------------------
+// rhs == V (basis slot 0): the caller wrote the right-hand side there, and
+// beta_hint > 0 is its norm; the first cycle then starts without a copy or a
+// norm pass, and rhs is saved to `save` before a restart overwrites slot 0.
 template <typename T>
 int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double rtol, int64_t max_it,
-                int64_t *its, double *relres)
+                int64_t *its, double *relres, double beta_hint = -1.0, T *save = nullptr)
 {
   const int64_t nc = ctx->g.nc, m = ctx->restart;
   T *V = (T *)ctx->V, *Z = (T *)ctx->Z, *w = (T *)ctx->wv;
@@ -152,18 +155,28 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
   double beta0 = -1.0, beta_prev = -1.0;
   *its = 0;
   *relres = 1.0;
+  const bool in_place = rhs == V;
   for (;;) {
     // V0 = rhs - A e
+    double nn;
     if (*its == 0) {
-      CHY_HIP(hipMemcpyAsync(V, rhs, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+      if (!in_place) CHY_HIP(hipMemcpyAsync(V, rhs, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
     } else {
+      if (in_place && rhs == V) {  // keep the right-hand side before slot 0 changes
+        if (!save) return set_err(ctx, CHY_ERR_LIB, "gmres: no buffer to keep the right-hand side for a restart");
+        CHY_HIP(hipMemcpyAsync(save, V, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+        rhs = save;
+      }
       launch_lhs_apply<T, T, T>(ctx, lt, ctx->solve_transpose, n, e, w);
       CHY_HIP(hipMemcpyAsync(V, rhs, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
       launch_axpby<T>(ctx, nc, -1.0, w, 1.0, V);
     }
-    launch_mdot<T>(ctx, 1, V, nc, V, nc, ndev);
-    double nn;
-    TRY(fetch(ctx, ndev, 1, &nn));
+    if (*its == 0 && beta_hint > 0.0) {
+      nn = beta_hint * beta_hint;
+    } else {
+      launch_mdot<T>(ctx, 1, V, nc, V, nc, ndev);
+      TRY(fetch(ctx, ndev, 1, &nn));
+    }
     const double beta = sqrt(nn);
     if (beta0 < 0) beta0 = beta;
     if (!(beta0 > 0.0)) { *relres = 0.0; return CHY_OK; }
@@ -369,6 +382,7 @@ int solve_impl(chy_ctx *ctx, chy_solve_stats *st)
   double rn = 0.0;
   int converged = 0;
   T *rhsT = (T *)ctx->t64;  // FP64-sized buffer holds the inner right-hand side
+  const bool gm_in_place = !ctx->solve_transpose && p.ksp_type != CHY_KSP_BCGS;
   T *ev = (T *)ctx->V + (nvec_V(ctx) - 1) * nc;
   for (;;) {
     // r = b - A phi in FP64 arithmetic, stored in the inner precision as the
@@ -379,7 +393,8 @@ int solve_impl(chy_ctx *ctx, chy_solve_stats *st)
       launch_mdot<double>(ctx, 1, ctx->r64, nc, ctx->r64, nc, ctx->red_out);
       launch_convert<double, T>(ctx, nc, ctx->r64, 1.0, rhsT);
     } else {
-      launch_residual_norm<T, T>(ctx, p.lhs_type, n, ctx->b, ctx->phi, rhsT, ctx->rpl);
+      // GMRES takes it in its first basis slot (no copy); BiCGStab in rhsT
+      launch_residual_norm<T, T>(ctx, p.lhs_type, n, ctx->b, ctx->phi, gm_in_place ? (T *)ctx->V : rhsT, ctx->rpl);
       launch_combine_sum(ctx, ctx->rpl, (int)lhs_zsum_blocks(ctx->g, false), 1, ctx->red_out);
     }
     double rr;
@@ -401,7 +416,8 @@ int solve_impl(chy_ctx *ctx, chy_solve_stats *st)
     if (p.ksp_type == CHY_KSP_BCGS)
       TRY(bicgstab_inner<T>(ctx, v19, rhsT, ev, inner_rtol, p.ksp_max_it - its, &it_in, &rel));
     else
-      TRY(gmres_inner<T>(ctx, v19, rhsT, ev, inner_rtol, p.ksp_max_it - its, &it_in, &rel));
+      TRY(gm_in_place ? gmres_inner<T>(ctx, v19, (T *)ctx->V, ev, inner_rtol, p.ksp_max_it - its, &it_in, &rel, rn, rhsT)
+                      : gmres_inner<T>(ctx, v19, rhsT, ev, inner_rtol, p.ksp_max_it - its, &it_in, &rel));
     its += it_in;
     outer++;
     // phi += e

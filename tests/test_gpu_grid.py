@@ -135,6 +135,27 @@ def test_mixed_solve_reaches_fp64_accuracy():
     assert relmax(out["mixed"][0], out["fp64"][0]) <= 1e-9
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_gmres_restarts_reach_the_same_potential(prec):
+    """GMRES(2) against GMRES(30) on the same system (one-level FFT
+    preconditioner, so that each outer sweep takes several cycles): the outer
+    sweep writes the inner right-hand side straight into the first basis
+    slot, and a restart has to keep it before that slot is rebuilt."""
+    out = {}
+    for restart in (30, 2):
+        extra = f"-ksp_rtol 1e-10 -pc_type fft -ksp_gmres_restart {restart} -precision {prec}"
+        with context(opts((32, 32, 32), extra=extra)) as ctx:
+            g = grid_of(ctx.prm)
+            mom = _moments(g, np.random.default_rng(9), amp=0.1)
+            mom = [m.astype(np.float32).astype(np.float64) for m in mom]
+            ctx.set_moments(vlasov_from(mom))
+            phi, st = ctx.solve_potential()
+            assert st.converged == 1 and st.residual <= 1e-10, (restart, st.iterations, st.outer, st.residual)
+            out[restart] = (phi, st)
+    assert out[2][1].iterations > 2 * out[2][1].outer  # cycles restarted within the outer sweeps
+    assert relmax(out[2][0], out[30][0]) <= 1e-8
+
+
 def test_solve_warm_start_and_fft_iteration_counts():
     rng = np.random.default_rng(9)
     with context(opts((32, 32, 32), extra="-ksp_rtol 1e-10")) as ctx:

@@ -116,7 +116,7 @@ struct chy_ctx {
   double *red_out = nullptr;    // reduced values [restart+2]
   double *rpl = nullptr;        // per-stencil-block partials of the outer residual's |r|^2
   double *h_red = nullptr;      // pinned host mirror of red_out
-  double *h_scale = nullptr;    // pinned: squared scales of the Krylov basis (GMRES)
+  double *h_scale = nullptr;    // pinned: squared scales of the Krylov basis, then the update coefficients (GMRES)
   double *gs_scale = nullptr;   // their device copy
   // FFT preconditioner
   void *fft_buf = nullptr;      // complex work array

@@ -167,14 +167,19 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
         CHY_HIP(hipMemcpyAsync(save, V, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
         rhs = save;
       }
-      launch_lhs_apply<T, T, T>(ctx, lt, ctx->solve_transpose, n, e, w);
-      CHY_HIP(hipMemcpyAsync(V, rhs, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
-      launch_axpby<T>(ctx, nc, -1.0, w, 1.0, V);
+      if (ctx->solve_transpose) {
+        launch_lhs_apply<T, T, T>(ctx, lt, true, n, e, w);
+        CHY_HIP(hipMemcpyAsync(V, rhs, sizeof(T) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+        launch_axpby<T>(ctx, nc, -1.0, w, 1.0, V);
+      } else {  // one stencil pass: V0 and its squared norm's partials
+        const int64_t nb = launch_residual_sq<T>(ctx, lt, n, rhs, e, V, ctx->rpl);
+        launch_combine_sum(ctx, ctx->rpl, (int)nb, 1, ndev);
+      }
     }
     if (*its == 0 && beta_hint > 0.0) {
       nn = beta_hint * beta_hint;
     } else {
-      launch_mdot<T>(ctx, 1, V, nc, V, nc, ndev);
+      if (*its == 0 || ctx->solve_transpose) launch_mdot<T>(ctx, 1, V, nc, V, nc, ndev);
       TRY(fetch(ctx, ndev, 1, &nn));
     }
     const double beta = sqrt(nn);

@@ -402,6 +402,28 @@ void launch_residual_norm(chy_ctx *ctx, int lhs_type, const TN *n, const double 
                      ctx->g, lhs_type, s, L, n, x, b, r, pl);
 }
 
+// r = b - A x in the inner precision (a GMRES restart's residual; the same
+// arithmetic as A x followed by b - y), with one FP64 partial of |r|^2 per
+// stencil block in pl; returns the number of partials
+template <typename T>
+int64_t launch_residual_sq(chy_ctx *ctx, int lhs_type, const T *n, const T *b, const T *x, T *r, double *pl)
+{
+  const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
+  const LapRow L = lap_row(ctx, lhs_type);
+  constexpr bool fp32 = sizeof(T) == 4;
+  using C = typename std::conditional<fp32, float, double>::type;
+  if constexpr (fp32) {
+    if (wide_rows(ctx->g, fp32)) {
+      hipLaunchKernelGGL((k_lhs_tiled<T, T, T, C, fp32, 3, T, 256>), tiled_grid(ctx->g, 256), kBlock, 0, ctx->stream,
+                         ctx->g, lhs_type, s, L, n, x, b, r, pl);
+      return lhs_zsum_blocks(ctx->g, fp32);
+    }
+  }
+  hipLaunchKernelGGL((k_lhs_tiled<T, T, T, C, fp32, 3, T>), tiled_grid(ctx->g), kBlock, 0, ctx->stream, ctx->g,
+                     lhs_type, s, L, n, x, b, r, pl);
+  return lhs_zsum_blocks(ctx->g, fp32);
+}
+
 // y += a x, x of any precision, y FP64
 template <typename TS>
 __global__ __launch_bounds__(256) void k_add_scaled(int64_t n, const TS *__restrict__ x, double a, double *__restrict__ y)
@@ -1431,6 +1453,10 @@ template void launch_residual_norm<double, double>(chy_ctx *, int, const double 
                                                    double *, double *);
 template void launch_residual_norm<float, float>(chy_ctx *, int, const float *, const double *, const double *, float *,
                                                  double *);
+template int64_t launch_residual_sq<double>(chy_ctx *, int, const double *, const double *, const double *, double *,
+                                            double *);
+template int64_t launch_residual_sq<float>(chy_ctx *, int, const float *, const float *, const float *, float *,
+                                           double *);
 template void launch_add_scaled<double>(chy_ctx *, int64_t, const double *, double, double *);
 template void launch_add_scaled<float>(chy_ctx *, int64_t, const float *, double, double *);
 template void launch_zcoef<double>(chy_ctx *, int, const double *, double *);

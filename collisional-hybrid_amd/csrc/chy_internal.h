@@ -220,6 +220,8 @@ void launch_residual_norm(chy_ctx *ctx, int lhs_type, const TN *n, const double 
                           double *pl);
 template <typename TS>
 void launch_add_scaled(chy_ctx *ctx, int64_t n, const TS *x, double a, double *y);
+template <typename T>
+int64_t launch_residual_sq(chy_ctx *ctx, int lhs_type, const T *n, const T *b, const T *x, T *r, double *pl);
 // part[zc * nx * ny + col] = sum over the planes of z chunk zc of (b - A x) at
 // the column; pl[blk] = the sum over each stencil block (lhs_zsum_blocks of them)
 template <typename T>

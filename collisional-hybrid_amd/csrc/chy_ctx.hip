@@ -364,10 +364,17 @@ int pc_symbol(chy_ctx *ctx, double v19[19])
   uniform_row(ctx->prm.lhs_type, ctx->prm.d, ctx->prm.kappa_e, s / (double)nc, v19);
   // coefficients of the z-averaged operator for the two-level preconditioner
   const bool two = ctx->prm.pc_type == CHY_PC_FFT2 || ctx->prm.pc_type == CHY_PC_FFT2Z;
-  if (two && ctx->fft_ok && ctx->pc2_buf) launch_zcoef<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), (T *)ctx->pc2_buf);
-  // and the z-line factors of the fft2z correction
-  if (ctx->prm.pc_type == CHY_PC_FFT2Z && ctx->fft_ok && ctx->zl_buf)
-    launch_zline_factor<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), (T *)ctx->zl_buf, ctx->zl_col);
+  // and the z-line factors of the fft2z correction (for the full operator,
+  // both from one walk up the columns)
+  const bool coef = two && ctx->fft_ok && ctx->pc2_buf;
+  const bool fact = ctx->prm.pc_type == CHY_PC_FFT2Z && ctx->fft_ok && ctx->zl_buf;
+  if (ctx->prm.lhs_type == CHY_LHS_FULL && (coef || fact)) {
+    launch_zcolumn<T>(ctx, mom<T>(ctx, 0), coef ? (T *)ctx->pc2_buf : nullptr, fact ? (T *)ctx->zl_buf : nullptr,
+                      fact ? ctx->zl_col : nullptr);
+  } else {
+    if (coef) launch_zcoef<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), (T *)ctx->pc2_buf);
+    if (fact) launch_zline_factor<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), (T *)ctx->zl_buf, ctx->zl_col);
+  }
   return CHY_OK;
 }
 

@@ -1344,6 +1344,113 @@ __global__ __launch_bounds__(256) void k_zline_apply(Grid g, const T *__restrict
   }
 }
 
+// Both once-per-solve column walks of the two-level preconditioners for the
+// full operator (COEF: k_zcoef's z-means, FACT: k_zline_factor's factors,
+// the same arithmetic in the same order), in one walk up each column with the
+// density star of KC planes loaded before any is used; the back sweep of the
+// Sherman-Morrison vector reads KC planes at a time too.
+template <typename T, bool COEF, bool FACT>
+__global__ __launch_bounds__(256) void k_zcolumn(Grid g, SCoef s, const T *__restrict__ n, T *__restrict__ Cz,
+                                                 T *__restrict__ za, T *__restrict__ zi, T *__restrict__ zc,
+                                                 T *__restrict__ zq, double *__restrict__ cag,
+                                                 double *__restrict__ cden)
+{
+  using C = double;
+  constexpr int KC = 8;
+  const int64_t plane = (int64_t)g.nx * g.ny;
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= plane) return;
+  const int i = (int)(col % g.nx), j = (int)(col / g.nx), m = g.nz;  // m >= 3
+  const int64_t oxp = lin(g, i + 1, j, 0), oxm = lin(g, i - 1, j, 0), oyp = lin(g, i, j + 1, 0),
+                oym = lin(g, i, j - 1, 0);
+  auto nat = [&](int64_t o, int k) { return (C)n[o + plane * (int64_t)wrapi(k, m)]; };
+  // rows at k = 0 and k = m - 1 (the periodic corner)
+  C v[19];
+  row_from_star<C>(s, nat(col, 0), nat(oxp, 0), nat(oxm, 0), nat(oyp, 0), nat(oym, 0), nat(col, 1), nat(col, -1), v);
+  const C a0 = v[5], b0 = v[18], c0 = v[4];
+  row_from_star<C>(s, nat(col, m - 1), nat(oxp, m - 1), nat(oxm, m - 1), nat(oyp, m - 1), nat(oym, m - 1),
+                   nat(col, m), nat(col, m - 2), v);
+  const C al = v[5], bl = v[18], cl = v[4];
+  const C gm = -b0, ag = a0 / gm;
+  C acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  C cpp = 0, qp = 0;
+  for (int k0 = 0; k0 < m; k0 += KC) {
+    // planes k0 - 1 .. k0 + KC of the column, and the x-y neighbours of k0 .. k0 + KC - 1
+    T nc_[KC + 2], xp[KC], xm[KC], yp[KC], ym[KC];
+#pragma unroll
+    for (int u = 0; u < KC + 2; u++) nc_[u] = n[col + plane * (int64_t)wrapi(k0 - 1 + u < m + 1 ? k0 - 1 + u : m, m)];
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+      const int64_t pk = plane * (int64_t)(k0 + u < m ? k0 + u : m - 1);
+      xp[u] = n[oxp + pk]; xm[u] = n[oxm + pk]; yp[u] = n[oyp + pk]; ym[u] = n[oym + pk];
+    }
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+      const int k = k0 + u;
+      if (k >= m) break;
+      row_from_star<C>(s, (C)nc_[u + 1], (C)xp[u], (C)xm[u], (C)yp[u], (C)ym[u], (C)nc_[u + 2], (C)nc_[u], v);
+      if (COEF) {
+#pragma unroll
+        for (int q = 0; q < 19; q++) acc[kGroup19[q]] += v[q];
+      }
+      if (FACT) {
+        C a, b, c;
+        if (k == 0) { a = a0; b = b0 - gm; c = c0; }
+        else if (k == m - 1) { a = al; b = bl - cl * a0 / gm; c = cl; }
+        else { a = v[5]; b = v[18]; c = v[4]; }
+        const int64_t e = col + plane * k;
+        const C uk = k == 0 ? gm : (k == m - 1 ? cl : (C)0);
+        const C den = k == 0 ? b : b - a * cpp;
+        const C cpk = c / den;
+        const C qk = k == 0 ? uk / den : (uk - a * qp) / den;
+        za[e] = (T)a; zi[e] = (T)((C)1 / den); zc[e] = (T)cpk; zq[e] = (T)qk;
+        cpp = cpk; qp = qk;
+      }
+    }
+  }
+  if (COEF) {
+#pragma unroll
+    for (int o = 0; o < 9; o++) Cz[o * plane + col] = (T)(acc[o] / (double)m);
+  }
+  if (FACT) {
+    const C qlast = qp;
+    C qn = qp;
+    for (int k0 = m - 2; k0 >= 0; k0 -= KC) {  // planes k0, k0 - 1, ...
+      T qq[KC], cc[KC];
+#pragma unroll
+      for (int u = 0; u < KC; u++) {
+        const int k = k0 - u >= 0 ? k0 - u : 0;
+        qq[u] = zq[col + plane * k]; cc[u] = zc[col + plane * k];
+      }
+#pragma unroll
+      for (int u = 0; u < KC; u++) {
+        const int k = k0 - u;
+        if (k < 0) break;
+        qn = (C)qq[u] - (C)cc[u] * qn;
+        zq[col + plane * k] = (T)qn;
+      }
+    }
+    cag[col] = ag;
+    cden[col] = (C)1 + qn + ag * qlast;
+  }
+}
+
+template <typename T>
+void launch_zcolumn(chy_ctx *ctx, const T *n, T *Cz, T *zbuf, double *colbuf)
+{
+  const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
+  const int64_t nc = ctx->g.nc, plane = (int64_t)ctx->g.nx * ctx->g.ny;
+  T *z4[4] = {zbuf, zbuf ? zbuf + nc : nullptr, zbuf ? zbuf + 2 * nc : nullptr, zbuf ? zbuf + 3 * nc : nullptr};
+  double *c2[2] = {colbuf, colbuf ? colbuf + plane : nullptr};
+#define ZCOL(A, B)                                                                                              \
+  hipLaunchKernelGGL((k_zcolumn<T, A, B>), nblocks(plane), kBlock, 0, ctx->stream, ctx->g, s, n, Cz, z4[0], z4[1], \
+                     z4[2], z4[3], c2[0], c2[1])
+  if (Cz && zbuf) ZCOL(true, true);
+  else if (Cz) ZCOL(true, false);
+  else if (zbuf) ZCOL(false, true);
+#undef ZCOL
+}
+
 template <typename T>
 void launch_zline_factor(chy_ctx *ctx, int lhs_type, const T *n, T *zbuf, double *colbuf)
 {
@@ -1381,6 +1488,8 @@ void launch_residual_t(chy_ctx *ctx, int lhs_type, const T *n, const T *b, const
                      lhs_type, s, L, n, x, b, res, (double *)nullptr);
 }
 
+template void launch_zcolumn<double>(chy_ctx *, const double *, double *, double *, double *);
+template void launch_zcolumn<float>(chy_ctx *, const float *, float *, float *, double *);
 template void launch_zline_factor<double>(chy_ctx *, int, const double *, double *, double *);
 template void launch_zline_factor<float>(chy_ctx *, int, const float *, float *, double *);
 template void launch_zline_apply<double>(chy_ctx *, const double *, const double *, double *, double *);

@@ -323,6 +323,11 @@ int64_t fft2_h_elems(const chy_ctx *ctx);    // doubles in pc2_h
 // scratch); launch_residual_t: res = b - A(n) x in precision T.
 template <typename T>
 void launch_zline_factor(chy_ctx *ctx, int lhs_type, const T *n, T *zbuf, double *colbuf);
+// the full operator's z-averaged coefficients (Cz, as launch_zcoef) and/or
+// z-line factors (zbuf, colbuf, as launch_zline_factor) in one column walk;
+// a null Cz or zbuf skips that part
+template <typename T>
+void launch_zcolumn(chy_ctx *ctx, const T *n, T *Cz, T *zbuf, double *colbuf);
 template <typename T>
 void launch_zline_apply(chy_ctx *ctx, const T *zbuf, const double *colbuf, T *res, T *z);
 template <typename T>

@@ -537,7 +537,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   A(dalloc(ctx, &ctx->counts, nc + 1));
   A(dalloc(ctx, &ctx->offsets, nc + 1));
   A(dalloc(ctx, &ctx->cursor, nc + 1));
-  A(dalloc(ctx, &ctx->scan_tmp, nc / 4096 + 2));
+  A(dalloc(ctx, &ctx->scan_tmp, 2 * (nc / 4096 + 2)));  // tile sums (+ total), then tile maxima
   A(dalloc(ctx, &ctx->maxcount, 1));
   A(dalloc(ctx, &ctx->trans, 27 * nc));
   A(dalloc(ctx, &ctx->info, cap));

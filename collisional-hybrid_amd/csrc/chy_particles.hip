@@ -292,24 +292,35 @@ __global__ __launch_bounds__(256) void k_keys_count(int64_t np, const int32_t *_
 constexpr int kScanTile = 4096;  // 256 threads x 16
 constexpr uint32_t kMaxCellOccupancy = 8192;
 
+// Tile sums and tile maxima of the cell counts (no atomics: the single-block
+// scan below takes the largest of the tile maxima).
 __global__ __launch_bounds__(256) void k_tile_sum(int64_t n, const uint32_t *__restrict__ in, uint32_t *__restrict__ tsum,
-                                                  uint32_t *__restrict__ maxcount)
+                                                  uint32_t *__restrict__ tmax)
 {
-  __shared__ uint32_t sh[4];
+  __shared__ uint32_t sh[4], shm[4];
   const int64_t base = (int64_t)blockIdx.x * kScanTile;
-  uint32_t acc = 0, mx = 0;
-  for (int q = threadIdx.x; q < kScanTile; q += blockDim.x) {
-    const int64_t i = base + q;
-    if (i < n) { acc += in[i]; mx = in[i] > mx ? in[i] : mx; }
+  uint32_t v[kScanTile / 256];
+#pragma unroll
+  for (int u = 0; u < kScanTile / 256; u++) {
+    const int64_t i = base + threadIdx.x + 256 * u;
+    v[u] = i < n ? in[i] : 0u;
   }
+  uint32_t acc = 0, mx = 0;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) { const uint32_t o = __shfl_down(mx, off, 64); mx = o > mx ? o : mx; }
-  if ((threadIdx.x & 63) == 0 && mx) atomicMax(maxcount, mx);
+  for (int u = 0; u < kScanTile / 256; u++) { acc += v[u]; mx = v[u] > mx ? v[u] : mx; }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_down(mx, off, 64);
+    mx = o > mx ? o : mx;
+    acc += __shfl_down(acc, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) { sh[threadIdx.x >> 6] = acc; shm[threadIdx.x >> 6] = mx; }
   __syncthreads();
-  if (threadIdx.x == 0) tsum[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+  if (threadIdx.x == 0) {
+    tsum[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    const uint32_t m01 = shm[0] > shm[1] ? shm[0] : shm[1], m23 = shm[2] > shm[3] ? shm[2] : shm[3];
+    tmax[blockIdx.x] = m01 > m23 ? m01 : m23;
+  }
 }
 
 // Block-wide exclusive scan of 256 values (one per thread); returns the total.
@@ -331,10 +342,26 @@ __device__ __forceinline__ uint32_t block_exscan(uint32_t v, uint32_t &total, ui
   return wbase + x - v;
 }
 
-// Single block: exclusive scan of the tile sums in place.
-__global__ __launch_bounds__(256) void k_scan_tiles(int64_t nt, uint32_t *__restrict__ tsum)
+// Single block: exclusive scan of the tile sums in place, and the largest
+// tile maximum (the largest cell occupancy) into maxcount[0].
+__global__ __launch_bounds__(256) void k_scan_tiles(int64_t nt, uint32_t *__restrict__ tsum,
+                                                    const uint32_t *__restrict__ tmax, uint32_t *__restrict__ maxcount)
 {
   __shared__ uint32_t sh[4];
+  uint32_t mx = 0;
+  for (int64_t i = threadIdx.x; i < nt; i += 256) mx = tmax[i] > mx ? tmax[i] : mx;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_down(mx, off, 64);
+    mx = o > mx ? o : mx;
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t m01 = sh[0] > sh[1] ? sh[0] : sh[1], m23 = sh[2] > sh[3] ? sh[2] : sh[3];
+    maxcount[0] = m01 > m23 ? m01 : m23;
+  }
+  __syncthreads();
   uint32_t carry = 0;
   for (int64_t b0 = 0; b0 < nt; b0 += 256) {
     const int64_t i = b0 + threadIdx.x;
@@ -847,8 +874,9 @@ static int sort_neighbour(chy_ctx *ctx, bool &done)
   hipLaunchKernelGGL(k_counts, ncb, 256, 0, ctx->stream, ctx->g, ctx->trans, ctx->counts, ctx->nbase);
   timer_stop(ctx, "sort_trans");
   timer_start(ctx, "sort_scan");
-  hipLaunchKernelGGL(k_tile_sum, (unsigned)nt, kBlock, 0, ctx->stream, nc, ctx->counts, ctx->scan_tmp, ctx->maxcount);
-  hipLaunchKernelGGL(k_scan_tiles, 1, kBlock, 0, ctx->stream, nt, ctx->scan_tmp);
+  hipLaunchKernelGGL(k_tile_sum, (unsigned)nt, kBlock, 0, ctx->stream, nc, ctx->counts, ctx->scan_tmp,
+                     ctx->scan_tmp + nt + 1);
+  hipLaunchKernelGGL(k_scan_tiles, 1, kBlock, 0, ctx->stream, nt, ctx->scan_tmp, ctx->scan_tmp + nt + 1, ctx->maxcount);
   hipLaunchKernelGGL(k_tile_scan, (unsigned)nt, kBlock, 0, ctx->stream, nc, ctx->counts, ctx->scan_tmp, ctx->offs_alt,
                      ctx->cursor, (uint32_t)np);
   timer_stop(ctx, "sort_scan");
@@ -903,8 +931,9 @@ int particles_sort(chy_ctx *ctx)
   const int64_t nt = (nc + kScanTile - 1) / kScanTile;
   launch_fill<uint32_t>(ctx, 1, ctx->maxcount, 0u);
   timer_start(ctx, "sort_scan");
-  hipLaunchKernelGGL(k_tile_sum, (unsigned)nt, kBlock, 0, ctx->stream, nc, ctx->counts, ctx->scan_tmp, ctx->maxcount);
-  hipLaunchKernelGGL(k_scan_tiles, 1, kBlock, 0, ctx->stream, nt, ctx->scan_tmp);
+  hipLaunchKernelGGL(k_tile_sum, (unsigned)nt, kBlock, 0, ctx->stream, nc, ctx->counts, ctx->scan_tmp,
+                     ctx->scan_tmp + nt + 1);
+  hipLaunchKernelGGL(k_scan_tiles, 1, kBlock, 0, ctx->stream, nt, ctx->scan_tmp, ctx->scan_tmp + nt + 1, ctx->maxcount);
   hipLaunchKernelGGL(k_tile_scan, (unsigned)nt, kBlock, 0, ctx->stream, nc, ctx->counts, ctx->scan_tmp, ctx->offsets,
                      ctx->cursor, (uint32_t)np);
   timer_stop(ctx, "sort_scan");

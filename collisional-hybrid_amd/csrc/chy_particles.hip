@@ -1225,7 +1225,9 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
   // sort counts per old cell)
   const bool count = move && !ctx->order_valid;
   if (count) launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
-  const unsigned nb = grid_for(ctx->np);
+  // 65536 blocks (16 ions per thread at 256^3 x 16): 3.96 ms, against 4.19
+  // with 4096 blocks and 4.04 with 262144
+  const unsigned nb = grid_for(ctx->np, 65536);
   timer_start(ctx, "push");
 #define LAUNCH(KER, T, C, M)                                                                                  \
   hipLaunchKernelGGL((KER<C, M>), nb, kBlock, 0, ctx->stream, ctx->g, pcT, ctx->np, st,                      \

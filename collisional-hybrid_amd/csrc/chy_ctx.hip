@@ -519,7 +519,8 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   g.my = fastdiv_magic((uint32_t)g.ny);
   g.nc = p.N[0] * p.N[1] * p.N[2];
   for (int a = 0; a < 3; a++) { g.p0[a] = p.p0[a]; g.p1[a] = p.p1[a]; g.d[a] = p.d[a]; g.L[a] = p.L[a]; }
-  if (g.nc >= (int64_t)1 << 31) { delete ctx; return CHY_ERR_ARG_RANGE; }
+  // node counts index 32-bit; row launches put ny and nz on the y and z grid axes
+  if (g.nc >= (int64_t)1 << 31 || g.ny > 65535 || g.nz > 65535) { delete ctx; return CHY_ERR_ARG_RANGE; }
   ctx->cap = np_capacity > 0 ? np_capacity : std::max<int64_t>(p.Np, 1);
   // slots, offsets and sort permutations are 32-bit
   if (ctx->cap > (int64_t)0xFFFFFFFF) { delete ctx; return CHY_ERR_ARG_RANGE; }

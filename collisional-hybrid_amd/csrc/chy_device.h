@@ -34,6 +34,23 @@ __device__ __forceinline__ void split_cell(const Grid &g, int c, int &i, int &j,
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// Logical tile of this workgroup in a 3-D launch.  Workgroups are dealt to the
+// 8 XCDs round-robin in linear order (x fastest); relabelling them so that XCD
+// q takes the q-th contiguous eighth of the tiles keeps a tile's y and z
+// neighbours, whose halo rows it reads, in the same L2.  Returns the logical
+// linear index; (tx, ty, tz) are its coordinates.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t &tx, uint32_t &ty, uint32_t &tz)
+{
+  const uint32_t gx = gridDim.x, gy = gridDim.y, nb = gx * gy * gridDim.z;
+  uint32_t b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  if ((nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+  const uint32_t p = b / gx;
+  tx = b - p * gx;
+  ty = p % gy;
+  tz = p / gy;
+  return b;
+}
+
 __device__ __forceinline__ int64_t lin(const Grid &g, int i, int j, int k)
 {
   return (int64_t)wrapi(i, g.nx) + (int64_t)g.nx * ((int64_t)wrapi(j, g.ny) + (int64_t)g.ny * wrapi(k, g.nz));

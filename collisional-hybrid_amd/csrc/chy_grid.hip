@@ -198,7 +198,9 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
   __shared__ C ns[3][HY][HX];
   const bool full = lhs_type == CHY_LHS_FULL;
   const int tx = threadIdx.x % TW, ty0 = threadIdx.x / TW;
-  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TY_, z0 = blockIdx.z * kZChunk;
+  uint32_t bx, by, bz;
+  const uint32_t tile = xcd_tile(bx, by, bz);
+  const int x0 = bx * TW, y0 = by * TY_, z0 = bz * kZChunk;
   const int z1 = z0 + kZChunk < g.nz ? z0 + kZChunk : g.nz;
   const int i = x0 + tx;
   const int64_t pstride = (int64_t)g.nx * g.ny;
@@ -313,7 +315,7 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
     for (int rr = 0; rr < RW; rr++) {
       const int j = y0 + ty0 + GW * rr;
       if (i < g.nx && j < g.ny) {
-        if (OUT == 2) y[(int64_t)blockIdx.z * pstride + i + (int64_t)g.nx * j] = (TY)zs[rr];
+        if (OUT == 2) y[(int64_t)bz * pstride + i + (int64_t)g.nx * j] = (TY)zs[rr];
         t += zs[rr];
       }
     }
@@ -321,9 +323,7 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
     for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = t;
     __syncthreads();
-    if (threadIdx.x == 0)
-      pl[blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z)] =
-          ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+    if (threadIdx.x == 0) pl[tile] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
   }
 }
 
@@ -497,35 +497,46 @@ __global__ __launch_bounds__(256) void k_rhs_full(Grid g, RhsCoef rc, const T *_
                                                   const T *__restrict__ gx, const T *__restrict__ gy,
                                                   const T *__restrict__ gz, double *__restrict__ b)
 {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.nc) return;
-  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
-  const double dx = rc.dx, dy = rc.dy, dz = rc.dz;
-#define F(a, di, dj, dk) ((double)a[lin(g, i + (di), j + (dj), k + (dk))])
-  const double dndx = (F(n, 1, 0, 0) - F(n, -1, 0, 0)) / (2.0 * dx);
-  const double dndy = (F(n, 0, 1, 0) - F(n, 0, -1, 0)) / (2.0 * dy);
-  const double dndz = (F(n, 0, 0, 1) - F(n, 0, 0, -1)) / (2.0 * dz);
-  const double dGxdx = (F(gx, 1, 0, 0) - F(gx, -1, 0, 0)) / (2.0 * dx);
-  const double dGydy = (F(gy, 0, 1, 0) - F(gy, 0, -1, 0)) / (2.0 * dy);
-  const double dGzdz = (F(gz, 0, 0, 1) - F(gz, 0, 0, -1)) / (2.0 * dz);
+  // one block per 256-node segment of an x row; the (segments x ny x nz)
+  // launch is relabelled XCD-contiguously, so the y and z neighbour rows come
+  // from the L2 that the neighbouring rows' blocks filled
+  uint32_t seg, j_, k_;
+  xcd_tile(seg, j_, k_);
+  const int i = (int)seg * 256 + threadIdx.x, j = (int)j_, k = (int)k_;
+  if (i >= g.nx) return;
+  const int ip = i + 1 == g.nx ? 0 : i + 1, im = i == 0 ? g.nx - 1 : i - 1;
+  const int jp = j + 1 == g.ny ? 0 : j + 1, jm = j == 0 ? g.ny - 1 : j - 1;
+  const int kp = k + 1 == g.nz ? 0 : k + 1, km = k == 0 ? g.nz - 1 : k - 1;
+  auto row = [&](int jj, int kk) { return (int64_t)g.nx * ((int64_t)jj + (int64_t)g.ny * kk); };
+  const int64_t r0 = row(j, k), ryp = row(jp, k), rym = row(jm, k), rzp = row(j, kp), rzm = row(j, km);
+  const double n0 = (double)n[r0 + i];
+  const double nxp = (double)n[r0 + ip], nxm = (double)n[r0 + im];
+  const double nyp = (double)n[ryp + i], nym = (double)n[rym + i];
+  const double nzp = (double)n[rzp + i], nzm = (double)n[rzm + i];
+  const double dndx = (nxp - nxm) * rc.h1[0];
+  const double dndy = (nyp - nym) * rc.h1[1];
+  const double dndz = (nzp - nzm) * rc.h1[2];
+  const double dGxdx = ((double)gx[r0 + ip] - (double)gx[r0 + im]) * rc.h1[0];
+  const double dGydy = ((double)gy[ryp + i] - (double)gy[rym + i]) * rc.h1[1];
+  const double dGzdz = ((double)gz[rzp + i] - (double)gz[rzm + i]) * rc.h1[2];
   double Pterm = 0.0;
   if (rc.cth != 0.0) {
-    const double d2xx = (F(n, 1, 0, 0) - 2.0 * F(n, 0, 0, 0) + F(n, -1, 0, 0)) / (dx * dx);
-    const double d2yy = (F(n, 0, 1, 0) - 2.0 * F(n, 0, 0, 0) + F(n, 0, -1, 0)) / (dy * dy);
-    const double d2zz = (F(n, 0, 0, 1) - 2.0 * F(n, 0, 0, 0) + F(n, 0, 0, -1)) / (dz * dz);
-    const double d2xy = (F(n, 1, 1, 0) - F(n, -1, 1, 0) - F(n, 1, -1, 0) + F(n, -1, -1, 0)) / (4.0 * dx * dy);
-    const double d2xz = (F(n, 1, 0, 1) - F(n, -1, 0, 1) - F(n, 1, 0, -1) + F(n, -1, 0, -1)) / (4.0 * dx * dz);
-    const double d2yz = (F(n, 0, 1, 1) - F(n, 0, -1, 1) - F(n, 0, 1, -1) + F(n, 0, -1, -1)) / (4.0 * dy * dz);
+    const int64_t rpp = row(jp, kp), rmp = row(jm, kp), rpm = row(jp, km), rmm = row(jm, km);
+    const double d2xx = (nxp - 2.0 * n0 + nxm) * rc.h2[0];
+    const double d2yy = (nyp - 2.0 * n0 + nym) * rc.h2[1];
+    const double d2zz = (nzp - 2.0 * n0 + nzm) * rc.h2[2];
+    const double d2xy =
+        ((double)n[ryp + ip] - (double)n[ryp + im] - (double)n[rym + ip] + (double)n[rym + im]) * rc.hm[0];
+    const double d2xz =
+        ((double)n[rzp + ip] - (double)n[rzp + im] - (double)n[rzm + ip] + (double)n[rzm + im]) * rc.hm[1];
+    const double d2yz = ((double)n[rpp + i] - (double)n[rmp + i] - (double)n[rpm + i] + (double)n[rmm + i]) * rc.hm[2];
     Pterm = rc.cth * (rc.r[0][0] * d2xx + rc.r[0][1] * d2xy + rc.r[0][2] * d2xz +
                       rc.r[1][0] * d2xy + rc.r[1][1] * d2yy + rc.r[1][2] * d2yz +
                       rc.r[2][0] * d2xz + rc.r[2][1] * d2yz + rc.r[2][2] * d2zz);
   }
-#undef F
-  const double Eterm = (rc.r[0][0] * rc.E0[0] + rc.r[0][1] * rc.E0[1] + rc.r[0][2] * rc.E0[2]) * dndx +
-                       (rc.r[1][0] * rc.E0[0] + rc.r[1][1] * rc.E0[1] + rc.r[1][2] * rc.E0[2]) * dndy +
-                       (rc.r[2][0] * rc.E0[0] + rc.r[2][1] * rc.E0[1] + rc.r[2][2] * rc.E0[2]) * dndz;
+  const double Eterm = rc.re[0] * dndx + rc.re[1] * dndy + rc.re[2] * dndz;
   const double Gterm = rc.detA * rc.cG * (dGxdx + dGydy + dGzdz);
-  b[c] = rc.scale * (Eterm + Pterm + Gterm);
+  b[r0 + i] = rc.scale * (Eterm + Pterm + Gterm);
 }
 
 // Sinusoidal forcing (rhs.c:98-112); n0 is a device scalar (sum of n).
@@ -565,6 +576,14 @@ RhsCoef rhs_coef(const chy_ctx *ctx)
   rc.cG = p.me * p.nue / 1.6022e-19;
   for (int a = 0; a < 3; a++) rc.E0[a] = p.E0[a];
   rc.dx = p.d[0]; rc.dy = p.d[1]; rc.dz = p.d[2];
+  for (int a = 0; a < 3; a++) {
+    rc.h1[a] = 1.0 / (2.0 * p.d[a]);
+    rc.h2[a] = 1.0 / (p.d[a] * p.d[a]);
+    rc.re[a] = rc.r[a][0] * p.E0[0] + rc.r[a][1] * p.E0[1] + rc.r[a][2] * p.E0[2];
+  }
+  rc.hm[0] = 1.0 / (4.0 * p.d[0] * p.d[1]);
+  rc.hm[1] = 1.0 / (4.0 * p.d[0] * p.d[2]);
+  rc.hm[2] = 1.0 / (4.0 * p.d[1] * p.d[2]);
   return rc;
 }
 
@@ -577,7 +596,8 @@ void launch_rhs(chy_ctx *ctx, int rhs_type, const T *n, const T *gx, const T *gy
   const int64_t nc = ctx->g.nc;
   if (rhs_type == CHY_RHS_FULL) {
     const RhsCoef rc = rhs_coef(ctx);
-    hipLaunchKernelGGL((k_rhs_full<T>), nblocks(nc), kBlock, 0, ctx->stream, ctx->g, rc, n, gx, gy, gz, b);
+    const dim3 grid((unsigned)((ctx->g.nx + 255) / 256), (unsigned)ctx->g.ny, (unsigned)ctx->g.nz);
+    hipLaunchKernelGGL((k_rhs_full<T>), grid, kBlock, 0, ctx->stream, ctx->g, rc, n, gx, gy, gz, b);
   } else {
     launch_sum<T>(ctx, n, nc, ctx->red_out);
     const double fac = 2.0 * p.d[0] * p.d[1] * p.d[2] / detA;

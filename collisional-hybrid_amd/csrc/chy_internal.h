@@ -21,6 +21,8 @@ struct RhsCoef {
   double r[3][3];
   double detA, scale, cth, cG, E0[3];
   double dx, dy, dz;
+  // 1/(2 d_a), 1/d_a^2, 1/(4 dx dy), 1/(4 dx dz), 1/(4 dy dz), and (R E0)_a
+  double h1[3], h2[3], hm[3], re[3];
 };
 
 template <typename T> struct alignas(4 * sizeof(T)) Vec4 { T x, y, z, w; };

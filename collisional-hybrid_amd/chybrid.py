@@ -98,6 +98,7 @@ SIGNATURES = {
     "chy_set_moments": (C.c_int, [P, P]),
     "chy_get_moments": (C.c_int, [P, P]),
     "chy_output_write": (C.c_int, [P, C.c_char_p, C.c_char_p]),
+    "chy_read_vlasov": (C.c_int, [P, C.c_char_p, P, C.c_char_p, C.c_size_t]),
     "chy_compute_rhs": (C.c_int, [P, P]),
     "chy_lhs_apply": (C.c_int, [P, P, P]),
     "chy_lhs_apply_transpose": (C.c_int, [P, P, P]),

@@ -248,6 +248,16 @@ int chy_adjoint_backward(chy_ctx *ctx, double dJ_dE0[3], double *dJ_ddelta);
  * files and the call returns CHY_ERR_SUP after writing them. */
 int chy_output_write(chy_ctx *ctx, const char *stem, const char *insert);
 
+/* ---- input (LoadVlasovQuantities, reference fileio.c:6-70) ------------------
+ * Reads the density dataset "density-kji" (nc = Nx Ny Nz f64 values, x
+ * fastest) of the HDF5 file at path into vlasov [Nz][Ny][Nx][4] as
+ * {n, s_x n, s_y n, s_z n}, s = prm->flux_scale (fluxes made from the density,
+ * fileio.c:44-53).  A path ending in ".f64" is read instead as nc raw f64
+ * values.  HDF5 is loaded at run time; without it an HDF5 path is
+ * CHY_ERR_SUP.  A missing file or dataset, or one whose size is not nc, is
+ * CHY_ERR_ARG_WRONG; msg (may be NULL) receives a description of any failure. */
+int chy_read_vlasov(const chy_params *prm, const char *path, double *vlasov, char *msg, size_t msg_len);
+
 /* ---- instrumentation -------------------------------------------------------
  * Per-kernel HIP-event timing on the context stream (off by default).
  * Names: "efield", "push", "sort_scan", "sort_scatter", "sort_reorder",

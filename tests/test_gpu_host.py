@@ -1,6 +1,7 @@
 """The host side around the GPU path: HDF5 output (reference fileio.c:73-126)
-read back through libhdf5, and the `hybrid` driver binary run end to end as
-run.sh would (reference hybrid.c:18-144)."""
+read back through libhdf5, the `hybrid` driver binary run end to end as
+run.sh would (reference hybrid.c:18-144), and the `solver` program on an input
+density (reference solver.c, fileio.c:6-70)."""
 import ctypes as C
 import os
 import subprocess
@@ -104,3 +105,58 @@ def test_hybrid_driver_runs_like_run_sh(tmp_path):
     env["OMPI_COMM_WORLD_RANK"] = "1"
     p = subprocess.run([HYBRID] + args, cwd=tmp_path, capture_output=True, text=True, timeout=60, env=env)
     assert p.returncode == 0 and "exits" in p.stdout
+
+
+SOLVER = os.path.join(ROOT, "collisional-hybrid_amd", "bin", "solver")
+
+
+def _fields(h5, stem):
+    if h5 is not None:
+        return {k: v.ravel() for k, v in _read(h5, stem + ".hdf").items()}
+    return {nm: np.fromfile(f"{stem}.{nm.replace(' ', '_')}.f64") for nm in NAMES}
+
+
+def test_solver_program_matches_oracle(tmp_path):
+    """bin/solver as the reference tool runs (solver.c:18-130): density from
+    --input with the fluxes made from it (fileio.c:6-70), one potential solve,
+    the fields written.  The forcing and potential it writes are the oracle's
+    for that density; with no input it solves the sinusoidal forcing of zero
+    moments (fileio.c:21-26), whose potential is zero."""
+    import orc
+    from gpu_cases import grid_of, relmax
+
+    assert os.path.exists(SOLVER), "bin/solver is not built"
+    rng = np.random.default_rng(13)
+    stem = str(tmp_path / "sol")
+    args = opts((16, 12, 8), extra="-ksp_rtol 1e-12 --x-flux-scale 3000 --y-flux-scale -2000 "
+                "--z-flux-scale 1000 --output " + stem)
+    prm, _ = chybrid.params_from_options(args)
+    g = grid_of(prm)
+    n = 16 * (1 + 0.1 * rng.uniform(-1, 1, g.nc))
+    n.tofile(tmp_path / "density.f64")
+    print("solver program: 16x12x8, raw density input", flush=True)
+    p = subprocess.run([SOLVER] + args + ["--input", str(tmp_path / "density.f64")], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    assert "Loaded density" in p.stdout and "converged" in p.stdout and "END" in p.stdout
+    d = _fields(_hdf5(), stem)
+    assert np.array_equal(d["density"], n)
+    for nm, s in zip(NAMES[1:4], (3000.0, -2000.0, 1000.0)):
+        assert np.array_equal(d[nm], s * n), nm
+    mom = [n, 3000.0 * n, -2000.0 * n, 1000.0 * n]
+    cth = prm.gamma_e * orc.KB * prm.Te / orc.Q
+    cG = prm.me * prm.nue / orc.Q
+    b_ref = orc.rhs(g, list(prm.kappa_e), list(prm.E0), cth, cG, *mom, rhs_type=prm.rhs_type)
+    assert relmax(d["rhs"], b_ref) <= 1e-13
+    ref, its, conv, res = orc.solve(g, list(prm.kappa_e), n, d["rhs"], lhs_type=prm.lhs_type, rtol=1e-12,
+                                    max_it=20000)
+    assert conv
+    assert relmax(d["potential"], ref) <= 1e-8, relmax(d["potential"], ref)
+    # no input: zero moments, sinusoidal forcing n0 cos cos cos = 0
+    print("solver program: no input", flush=True)
+    args0 = opts((8, 8, 8), extra="--output " + str(tmp_path / "zero"))
+    p = subprocess.run([SOLVER] + args0, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    assert "empty path to density file" in p.stdout
+    d0 = _fields(_hdf5(), str(tmp_path / "zero"))
+    assert not d0["potential"].any() and not d0["rhs"].any()

@@ -143,3 +143,88 @@ def test_options_txt_is_parseable(tmp_path):
     # scripts/tools.py reads "-key value" lines of the option table
     opts = dict(line.split(None, 1) for line in text.splitlines() if line.startswith("-") and " " in line)
     assert opts["-Nx"] == "32" and opts["-B0z"] == "5e-5"
+
+
+# ---- solver input (LoadVlasovQuantities, fileio.c:6-70) ----------------------
+SMALL = ("-Nx 4 -Ny 5 -Nz 6 -x1 0.2 -y1 0.25 -z1 0.3 -mi 4.9816e-26 -mn 4.8156e-26 -Ti 300 -Tn 300 "
+         "-B0z 5e-5 -E0x 0.05 -nui 1e3 -nue 1e5 -dt 1e-5").split()
+
+
+def _h5_lib():
+    import ctypes as C
+    for name in ("libhdf5.so", "libhdf5.so.103", "/opt/conda/lib/libhdf5.so"):
+        try:
+            h = C.CDLL(name)
+        except OSError:
+            continue
+        for f in ("H5Fcreate", "H5Screate_simple", "H5Dcreate2"):
+            getattr(h, f).restype = C.c_int64
+        h.H5Fcreate.argtypes = [C.c_char_p, C.c_uint, C.c_int64, C.c_int64]
+        h.H5Screate_simple.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+        h.H5Dcreate2.argtypes = [C.c_int64, C.c_char_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_int64]
+        h.H5Dwrite.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_void_p]
+        for f in ("H5Dclose", "H5Sclose", "H5Fclose"):
+            getattr(h, f).argtypes = [C.c_int64]
+        h.H5open()
+        return h
+    return None
+
+
+def _h5_write(h, path, name, arr):
+    """One f64 dataset `name` of arr's shape in a new HDF5 file."""
+    import ctypes as C
+    dbl = C.c_int64.in_dll(h, "H5T_NATIVE_DOUBLE_g").value
+    f = h.H5Fcreate(str(path).encode(), 2, 0, 0)  # H5F_ACC_TRUNC
+    dims = np.array(arr.shape, dtype=np.uint64)
+    sp = h.H5Screate_simple(arr.ndim, dims.ctypes.data, None)
+    ds = h.H5Dcreate2(f, name.encode(), dbl, sp, 0, 0, 0)
+    assert f >= 0 and sp >= 0 and ds >= 0
+    assert h.H5Dwrite(ds, dbl, 0, 0, 0, np.ascontiguousarray(arr, dtype=np.float64).ctypes.data) >= 0
+    h.H5Dclose(ds)
+    h.H5Sclose(sp)
+    h.H5Fclose(f)
+
+
+def _read_vlasov(prm, path):
+    nc = prm.N[0] * prm.N[1] * prm.N[2]
+    vl = np.full((nc, 4), np.nan)
+    msg = chybrid.C.create_string_buffer(512)
+    rc = chybrid.lib().chy_read_vlasov(chybrid.C.byref(prm), str(path).encode(), vl.ctypes.data, msg, len(msg))
+    return rc, vl, msg.value.decode()
+
+
+def test_read_vlasov_raw_density_makes_the_fluxes(tmp_path):
+    """fileio.c:34-53: the density is read and each flux is its flux scale
+    times the density (VecAXPY onto zero), so all four are exact."""
+    prm, _ = chybrid.params_from_options(SMALL + "--x-flux-scale 3 --y-flux-scale -0.5 --z-flux-scale 7".split())
+    n = np.random.default_rng(3).uniform(1, 2, 4 * 5 * 6)
+    n.tofile(tmp_path / "n.f64")
+    rc, vl, msg = _read_vlasov(prm, tmp_path / "n.f64")
+    assert rc == 0, msg
+    assert np.array_equal(vl[:, 0], n)
+    for a, s in enumerate((3.0, -0.5, 7.0)):
+        assert np.array_equal(vl[:, 1 + a], s * n)
+    # a density of the wrong size, or no file, is refused with a message
+    n[:-1].tofile(tmp_path / "short.f64")
+    rc, _, msg = _read_vlasov(prm, tmp_path / "short.f64")
+    assert rc == chybrid.ERR_ARG_WRONG and "size" in msg
+    rc, _, msg = _read_vlasov(prm, tmp_path / "none.f64")
+    assert rc == chybrid.ERR_ARG_WRONG and "none.f64" in msg
+
+
+def test_read_vlasov_hdf5_density_kji(tmp_path):
+    """The reference loads the Vec named "density-kji" (fileio.c:39-41), shape
+    (Nz, Ny, Nx) with x fastest."""
+    h = _h5_lib()
+    if h is None:
+        pytest.skip("no libhdf5 to write the input file")
+    prm, _ = chybrid.params_from_options(SMALL)
+    n = np.random.default_rng(4).uniform(1, 2, (6, 5, 4))
+    _h5_write(h, tmp_path / "in.hdf", "density-kji", n)
+    rc, vl, msg = _read_vlasov(prm, tmp_path / "in.hdf")
+    assert rc == 0, msg
+    assert np.array_equal(vl[:, 0], n.ravel())
+    assert np.array_equal(vl[:, 1], n.ravel())  # flux scales default to 1
+    _h5_write(h, tmp_path / "other.hdf", "density", n)
+    rc, _, msg = _read_vlasov(prm, tmp_path / "other.hdf")
+    assert rc == chybrid.ERR_ARG_WRONG and "density-kji" in msg

@@ -1,0 +1,49 @@
+"""The benchmark workload itself -- 256^3 nodes, 16 ions per cell (268 M ions),
+mixed precision, collisions on, -ksp_rtol 1e-10 -- checked through properties
+that hold at any size, since the oracle cannot step it in reasonable time:
+every ion survives the push, the neighbour-move sort and the deposit (the
+deposited weight is conserved: CIC weights partition unity), every solve
+reaches the asked residual with a mean-free potential, and the collision
+count matches its target in expectation (the mean-matched Bernoulli draw of
+particles.c:504-530, in the intended semantics)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import chybrid
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def test_full_size_steps_conserve_ions_and_converge():
+    import bench
+
+    args = bench.workload_args(256, 16, "mixed")
+    prm, _ = chybrid.params_from_options(args)
+    print("full size: 256^3 x 16 ions/cell, mixed", flush=True)
+    with chybrid.Context(prm) as ctx:
+        ctx.init_particles()
+        ctx.collect_moments(download=False)
+        ctx.solve_potential()
+        np_ = ctx.num_particles()
+        w0 = ctx.get_moments()[:, 0].sum()
+        for it in range(3):
+            st = ctx.step()
+            assert st.solve.converged == 1 and st.solve.residual <= 1e-10, (it, st.solve.residual)
+            c = st.collisions
+            assert c.Nc > 0 and abs(c.accepted - c.Nc) <= 6 * np.sqrt(c.Nc), (it, c.accepted, c.Nc)
+            assert c.refused <= 10
+        assert ctx.num_particles() == np_ == 256 ** 3 * 16
+        w = ctx.get_moments()[:, 0].sum()
+        phi = ctx.get_potential()
+    # one ion lost or doubled moves the total by about 1 / 2.7e8 = 3.7e-9 of
+    # it; FP32 node sums of 268 M weights round far below that (random walk)
+    assert abs(w - w0) <= 1e-9 * w0, (w, w0)
+    assert np.isfinite(phi).all()
+    assert abs(phi.mean()) <= 1e-12 * np.abs(phi).max()

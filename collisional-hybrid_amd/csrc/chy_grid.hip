@@ -183,8 +183,15 @@ constexpr int kHX = kTX + 2, kHY = kTY + 2;
 // computed from LDS, so global latency overlaps the arithmetic.
 constexpr int kPer = (kHX * kHY + 255) / 256;
 
+// Waves per SIMD asked of the compiler: the FP64 passes over an FP32 density
+// fit three 64-wide tiles per CU in LDS, so their registers are held to three
+// waves; the others keep the compiler's choice.
+template <typename TN, typename C, int TW>
+constexpr int tiled_waves() { return (sizeof(C) == 8 && sizeof(TN) == 4 && TW == 64) ? 3 : 1; }
+
 template <typename TN, typename TX, typename TY, typename C, bool DIFF, int OUT, typename TB = double, int TW = kTX>
-__global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s, LapRow lap,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(tiled_waves<TN, C, TW>())))
+void k_lhs_tiled(Grid g, int lhs_type, SCoef s, LapRow lap,
                                                    const TN *__restrict__ n, const TX *__restrict__ x,
                                                    const TB *__restrict__ b, TY *__restrict__ y,
                                                    double *__restrict__ pl)
@@ -195,7 +202,10 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
   constexpr int TY_ = TW == 256 ? 8 : kTY, GW = 256 / TW, RW = TY_ / GW;
   constexpr int HX = TW + 2, HY = TY_ + 2, PER = (HX * HY + 255) / 256;
   __shared__ C xs[3][HY][HX];
-  __shared__ C ns[3][HY][HX];
+  // the density stays in its stored precision in LDS (FP32 density of an FP64
+  // pass: half the bytes, exact on conversion at use)
+  using NS = typename std::conditional<(sizeof(TN) < sizeof(C)), TN, C>::type;
+  __shared__ NS ns[3][HY][HX];
   const bool full = lhs_type == CHY_LHS_FULL;
   const int tx = threadIdx.x % TW, ty0 = threadIdx.x / TW;
   uint32_t bx, by, bz;
@@ -218,19 +228,21 @@ __global__ __launch_bounds__(256) void k_lhs_tiled(Grid g, int lhs_type, SCoef s
     jj = ((jj % g.ny) + g.ny) % g.ny;
     off[q] = (int64_t)ii + (int64_t)g.nx * jj;
   }
-  C rx[PER], rn[PER];
+  C rx[PER];
+  NS rn[PER];
   auto fetch = [&](int k) {
     const int64_t base = pstride * wrapi(k, g.nz);  // k in [-1, nz]
 #pragma unroll
     for (int q = 0; q < PER; q++) {
       if (slot[q] >= 0) {
         rx[q] = (C)x[base + off[q]];
-        if (full) rn[q] = (C)n[base + off[q]];
+        if (full) rn[q] = (NS)n[base + off[q]];
       }
     }
   };
   auto store = [&](int buf) {
-    C *xb = &xs[buf][0][0], *nb = &ns[buf][0][0];
+    C *xb = &xs[buf][0][0];
+    NS *nb = &ns[buf][0][0];
 #pragma unroll
     for (int q = 0; q < PER; q++) {
       if (slot[q] >= 0) {

@@ -1,4 +1,4 @@
-/* chy_output.h -- declaration shared by the driver (the entry point itself is
+/* chy_output.h -- declaration shared by the hybrid and solver programs (the entry point itself is
  * part of the C ABI in chybrid.h). */
 #ifndef CHY_OUTPUT_H
 #define CHY_OUTPUT_H

@@ -98,6 +98,8 @@ SIGNATURES = {
     "chy_set_moments": (C.c_int, [P, P]),
     "chy_get_moments": (C.c_int, [P, P]),
     "chy_output_write": (C.c_int, [P, C.c_char_p, C.c_char_p]),
+    "chy_output_write_async": (C.c_int, [P, C.c_char_p, C.c_char_p]),
+    "chy_output_wait": (C.c_int, [P]),
     "chy_read_vlasov": (C.c_int, [P, C.c_char_p, P, C.c_char_p, C.c_size_t]),
     "chy_compute_rhs": (C.c_int, [P, P]),
     "chy_lhs_apply": (C.c_int, [P, P, P]),
@@ -270,6 +272,17 @@ class Context:
     def output_write(self, stem, insert):
         """Returns 0 when HDF5 was written, ERR_SUP when raw files were written."""
         rc = lib().chy_output_write(self._p, stem.encode(), insert.encode())
+        if rc not in (0, ERR_SUP):
+            self._chk(rc)
+        return rc
+
+    def output_write_async(self, stem, insert):
+        """Starts the same output behind the computation; output_wait() ends it."""
+        self._chk(lib().chy_output_write_async(self._p, stem.encode(), insert.encode()))
+
+    def output_wait(self):
+        """Result of the output in flight: 0 (HDF5) or ERR_SUP (raw files)."""
+        rc = lib().chy_output_wait(self._p)
         if rc not in (0, ERR_SUP):
             self._chk(rc)
         return rc

@@ -598,10 +598,19 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   return CHY_OK;
 }
 
+static int out_join(chy_ctx *ctx);
+
 int chy_destroy(chy_ctx *ctx)
 {
   if (!ctx) return CHY_OK;
+  (void)out_join(ctx);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->out_stream) (void)hipStreamSynchronize(ctx->out_stream);
+  if (ctx->out_host) (void)hipHostFree(ctx->out_host);
+  if (ctx->out_dev) (void)hipFree(ctx->out_dev);
+  if (ctx->out_snap) (void)hipEventDestroy(ctx->out_snap);
+  if (ctx->out_copied) (void)hipEventDestroy(ctx->out_copied);
+  if (ctx->out_stream) (void)hipStreamDestroy(ctx->out_stream);
   adjoint_release(ctx);
   for (int s = 0; s < 2; s++) { free_state(ctx->pd[s]); free_state(ctx->pf[s]); }
   void *ps[] = {ctx->perm, ctx->slotkey, ctx->counts, ctx->offsets, ctx->cursor, ctx->scan_tmp, ctx->maxcount,
@@ -844,6 +853,74 @@ int chy_set_potential(chy_ctx *ctx, const double *phi)
   ctx->phi_valid = true;
   ctx->phi_prev_valid = false;
   return CHY_OK;
+}
+
+// Wait for the asynchronous writer, if one runs; its result.
+static int out_join(chy_ctx *ctx)
+{
+  if (!ctx->out_thread.joinable()) return CHY_OK;
+  ctx->out_thread.join();
+  const int rc = ctx->out_rc;
+  ctx->out_rc = CHY_OK;
+  return rc;
+}
+
+int chy_output_write_async(chy_ctx *ctx, const char *stem, const char *insert)
+{
+  if (!ctx || !stem || !insert) return CHY_ERR_ARG_WRONG;
+  const int prev = out_join(ctx);
+  if (prev != CHY_OK && prev != CHY_ERR_SUP) return set_err(ctx, prev, "the previous asynchronous output failed");
+  const int64_t nc = ctx->g.nc;
+  if (!ctx->out_dev) {
+    if (hipMalloc(&ctx->out_dev, sizeof(double) * 6 * nc) != hipSuccess) {
+      ctx->out_dev = nullptr;
+      return set_err(ctx, CHY_ERR_MEM, "no device memory for the output snapshot");
+    }
+    if (hipHostMalloc(&ctx->out_host, sizeof(double) * 6 * nc) != hipSuccess) {
+      ctx->out_host = nullptr;
+      return set_err(ctx, CHY_ERR_MEM, "no pinned host memory for the output");
+    }
+    CHY_HIP(hipStreamCreateWithFlags(&ctx->out_stream, hipStreamNonBlocking));
+    CHY_HIP(hipEventCreateWithFlags(&ctx->out_snap, hipEventDisableTiming));
+    CHY_HIP(hipEventCreateWithFlags(&ctx->out_copied, hipEventDisableTiming));
+  }
+  // snapshot in stream order (the next step's kernels then overwrite the
+  // fields freely): moments in f64, rhs, potential; zero where not yet made
+  double *d = ctx->out_dev;
+  for (int a = 0; a < 4; a++) {
+    if (!ctx->moments_valid) launch_fill<double>(ctx, nc, d + a * nc, 0.0);
+    else if (ctx->mixed) launch_convert<float, double>(ctx, nc, (const float *)ctx->mom[a], 1.0, d + a * nc);
+    else CHY_HIP(hipMemcpyAsync(d + a * nc, ctx->mom[a], sizeof(double) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  CHY_HIP(hipMemcpyAsync(d + 4 * nc, ctx->b, sizeof(double) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+  if (ctx->phi_valid)
+    CHY_HIP(hipMemcpyAsync(d + 5 * nc, ctx->phi, sizeof(double) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+  else
+    launch_fill<double>(ctx, nc, d + 5 * nc, 0.0);
+  CHY_HIP(hipEventRecord(ctx->out_snap, ctx->stream));
+  // the copy to the host overlaps whatever the context stream runs next
+  CHY_HIP(hipStreamWaitEvent(ctx->out_stream, ctx->out_snap, 0));
+  CHY_HIP(hipMemcpyAsync(ctx->out_host, d, sizeof(double) * 6 * nc, hipMemcpyDeviceToHost, ctx->out_stream));
+  CHY_HIP(hipEventRecord(ctx->out_copied, ctx->out_stream));
+  const std::string st(stem), in(insert);
+  const int dev = ctx->device;
+  ctx->out_thread = std::thread([ctx, st, in, dev, nc]() {
+    int rc = hipSetDevice(dev) == hipSuccess && hipEventSynchronize(ctx->out_copied) == hipSuccess ? CHY_OK
+                                                                                                   : CHY_ERR_LIB;
+    if (rc == CHY_OK) {
+      const double *f[6];
+      for (int a = 0; a < 6; a++) f[a] = ctx->out_host + a * nc;
+      rc = chyo_write_fields(&ctx->prm, st.c_str(), in.c_str(), f);
+    }
+    ctx->out_rc = rc;
+  });
+  return CHY_OK;
+}
+
+int chy_output_wait(chy_ctx *ctx)
+{
+  if (!ctx) return CHY_ERR_ARG_WRONG;
+  return out_join(ctx);
 }
 
 int chy_get_rhs(chy_ctx *ctx, double *b)

@@ -6,10 +6,13 @@
 
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "chy_device.h"
 #include "chybrid.h"
+
+extern "C" int chyo_write_fields(const chy_params *p, const char *stem, const char *insert, const double *const f[6]);
 
 namespace chy {
 
@@ -127,6 +130,13 @@ struct chy_ctx {
   double *pc2_h = nullptr;      // its double workspace: column sums, partial sums
   void *zl_buf = nullptr;       // fft2z: z-line factors (4 nc values of T)
   double *zl_col = nullptr;     // fft2z: per-column Sherman-Morrison scalars
+  // asynchronous field output: the six fields snapshotted on the device (6 nc
+  // f64), copied on out_stream into pinned out_host, written by out_thread
+  double *out_dev = nullptr, *out_host = nullptr;
+  hipStream_t out_stream = nullptr;
+  hipEvent_t out_snap = nullptr, out_copied = nullptr;
+  std::thread out_thread;
+  int out_rc = 0;
   double *tw[3] = {nullptr, nullptr, nullptr};   // twiddles per axis (complex)
   double *trig[3] = {nullptr, nullptr, nullptr}; // cos/sin of 2 pi k / N per axis
   bool fft_ok = false;

@@ -10,6 +10,7 @@
 #include "chybrid.h"
 
 #include <dlfcn.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -37,7 +38,12 @@ static struct {
   herr_t (*dread)(hid_t, hid_t, hid_t, hid_t, hid_t, void *);
 } H5;
 
-static void h5_load(void)
+static void h5_load_once(void);
+static pthread_once_t h5_once = PTHREAD_ONCE_INIT;
+/* resolved once per process, whichever thread writes first */
+static void h5_load(void) { pthread_once(&h5_once, h5_load_once); }
+
+static void h5_load_once(void)
 {
   if (H5.tried) return;
   H5.tried = 1;
@@ -118,10 +124,50 @@ int chy_read_vlasov(const chy_params *p, const char *path, double *vl, char *msg
   return rc;
 }
 
+/* The six fields (f[0..3] moments, f[4] rhs, f[5] potential, each nc f64,
+ * x fastest) to <stem><insert>.hdf, or to raw files without HDF5 (then
+ * CHY_ERR_SUP after writing them).  Host memory only: the synchronous and the
+ * asynchronous output share it. */
+int chyo_write_fields(const chy_params *p, const char *stem, const char *insert, const double *const f[6])
+{
+  const int64_t nc = p->N[0] * p->N[1] * p->N[2];
+  const char *names[6] = {"density", "x flux", "y flux", "z flux", "rhs", "potential"};
+  char path[8300];
+  int rc = CHY_OK;
+  h5_load();
+  if (H5.ok) {
+    snprintf(path, sizeof path, "%s%s.hdf", stem, insert);
+    const hid_t file = H5.fcreate(path, 0x0002u /* H5F_ACC_TRUNC */, 0, 0);
+    if (file < 0) rc = CHY_ERR_LIB;
+    const hsize_t dims[3] = {(hsize_t)p->N[2], (hsize_t)p->N[1], (hsize_t)p->N[0]};
+    for (int a = 0; a < 6 && !rc; a++) {
+      const hid_t sp = H5.screate(3, dims, NULL);
+      const hid_t ds = H5.dcreate(file, names[a], *H5.native_double, sp, 0, 0, 0);
+      if (sp < 0 || ds < 0 || H5.dwrite(ds, *H5.native_double, 0, 0, 0, f[a]) < 0) rc = CHY_ERR_LIB;
+      if (ds >= 0) H5.dclose(ds);
+      if (sp >= 0) H5.sclose(sp);
+    }
+    if (file >= 0) H5.fclose(file);
+  } else {
+    for (int a = 0; a < 6 && !rc; a++) {
+      snprintf(path, sizeof path, "%s%s.%s.f64", stem, insert, names[a]);
+      for (char *q = path + strlen(stem); *q; q++) if (*q == ' ') *q = '_';
+      FILE *fp = fopen(path, "wb");
+      if (!fp || fwrite(f[a], sizeof(double), (size_t)nc, fp) != (size_t)nc) rc = CHY_ERR_LIB;
+      if (fp) fclose(fp);
+    }
+    if (!rc) rc = CHY_ERR_SUP;
+  }
+  return rc;
+}
+
 int chy_output_write(chy_ctx *ctx, const char *stem, const char *insert)
 {
   chy_params p;
-  int rc = chy_get_params(ctx, &p);
+  /* one writer at a time: finish an asynchronous output first */
+  int rc = chy_output_wait(ctx);
+  if (rc && rc != CHY_ERR_SUP) return rc;
+  rc = chy_get_params(ctx, &p);
   if (rc) return rc;
   const int64_t nc = p.N[0] * p.N[1] * p.N[2];
   double *vl = malloc(sizeof(double) * 4 * nc), *f[6];
@@ -131,34 +177,7 @@ int chy_output_write(chy_ctx *ctx, const char *stem, const char *insert)
   if (!rc) rc = chy_get_potential(ctx, f[5]);
   for (int64_t c = 0; c < nc && !rc; c++)
     for (int a = 0; a < 4; a++) f[a][c] = vl[4 * c + a];
-  const char *names[6] = {"density", "x flux", "y flux", "z flux", "rhs", "potential"};
-  char path[8300];
-  if (!rc) {
-    h5_load();
-    if (H5.ok) {
-      snprintf(path, sizeof path, "%s%s.hdf", stem, insert);
-      const hid_t file = H5.fcreate(path, 0x0002u /* H5F_ACC_TRUNC */, 0, 0);
-      if (file < 0) rc = CHY_ERR_LIB;
-      const hsize_t dims[3] = {(hsize_t)p.N[2], (hsize_t)p.N[1], (hsize_t)p.N[0]};
-      for (int a = 0; a < 6 && !rc; a++) {
-        const hid_t sp = H5.screate(3, dims, NULL);
-        const hid_t ds = H5.dcreate(file, names[a], *H5.native_double, sp, 0, 0, 0);
-        if (sp < 0 || ds < 0 || H5.dwrite(ds, *H5.native_double, 0, 0, 0, f[a]) < 0) rc = CHY_ERR_LIB;
-        if (ds >= 0) H5.dclose(ds);
-        if (sp >= 0) H5.sclose(sp);
-      }
-      if (file >= 0) H5.fclose(file);
-    } else {
-      for (int a = 0; a < 6 && !rc; a++) {
-        snprintf(path, sizeof path, "%s%s.%s.f64", stem, insert, names[a]);
-        for (char *q = path + strlen(stem); *q; q++) if (*q == ' ') *q = '_';
-        FILE *fp = fopen(path, "wb");
-        if (!fp || fwrite(f[a], sizeof(double), (size_t)nc, fp) != (size_t)nc) rc = CHY_ERR_LIB;
-        if (fp) fclose(fp);
-      }
-      if (!rc) rc = CHY_ERR_SUP;
-    }
-  }
+  if (!rc) rc = chyo_write_fields(&p, stem, insert, (const double *const *)f);
   free(vl);
   for (int a = 0; a < 6; a++) free(f[a]);
   return rc;

@@ -70,7 +70,9 @@ int main(int argc, char **argv)
   CHECK(chy_solve_potential(ctx, NULL, &ss));
   int width = 1 + (int)log10((double)(prm.Nt > 0 ? prm.Nt : 1));
   if (width > 19) width = 19;
-  if (prm.output_interval > 0) CHECK(chy_output_write(ctx, prm.output, "-initial"));
+  /* output runs behind the following steps (snapshot on the device, copy and
+   * file write overlapped); each call first finishes the previous one */
+  if (prm.output_interval > 0) CHECK(chy_output_write_async(ctx, prm.output, "-initial"));
   printf("\n*** Main time-step loop ***\n\n");
   double total_ms = 0.0, solve_ms = 0.0, part_ms = 0.0;
   for (int64_t it = 0; it < prm.Nt; it++) {
@@ -91,9 +93,10 @@ int main(int argc, char **argv)
     if (prm.output_interval > 0 && (it + 1) % prm.output_interval == 0) {
       char insert[96];
       snprintf(insert, sizeof insert, "-%0*lld", width, (long long)it);
-      CHECK(chy_output_write(ctx, prm.output, insert));
+      CHECK(chy_output_write_async(ctx, prm.output, insert));
     }
   }
+  CHECK(chy_output_wait(ctx));
   if (prm.Nt > 0)
     printf("\nparticle-steps/s %.4e, mean step %.3f ms (particles %.3f, potential %.3f)\n",
            (double)prm.Np * (double)prm.Nt / (total_ms * 1e-3), total_ms / prm.Nt, part_ms / prm.Nt,

@@ -247,6 +247,17 @@ int chy_adjoint_backward(chy_ctx *ctx, double dJ_dE0[3], double *dJ_ddelta);
  * run time; without it the six arrays go to <stem><insert>.<name>.f64 raw
  * files and the call returns CHY_ERR_SUP after writing them. */
 int chy_output_write(chy_ctx *ctx, const char *stem, const char *insert);
+/* The same files written behind the computation (OutputHDF5 at scale): the
+ * six fields are snapshotted on the device in stream order, so the following
+ * calls may change them at once; the copy to pinned host memory runs on a
+ * second stream and a host thread writes the file.  Returns without waiting.
+ * One output is in flight at a time: the next call (or chy_output_write)
+ * first waits for the previous one and returns its error, if any.  Holds
+ * 6 nc f64 of device and of pinned host memory from the first call on. */
+int chy_output_write_async(chy_ctx *ctx, const char *stem, const char *insert);
+/* Wait for the asynchronous output in flight, if any; its result (CHY_ERR_SUP
+ * after writing raw files without HDF5, as chy_output_write). */
+int chy_output_wait(chy_ctx *ctx);
 
 /* ---- input (LoadVlasovQuantities, reference fileio.c:6-70) ------------------
  * Reads the density dataset "density-kji" (nc = Nx Ny Nz f64 values, x

@@ -160,3 +160,30 @@ def test_solver_program_matches_oracle(tmp_path):
     assert "empty path to density file" in p.stdout
     d0 = _fields(_hdf5(), str(tmp_path / "zero"))
     assert not d0["potential"].any() and not d0["rhs"].any()
+
+
+def test_async_output_matches_synchronous(tmp_path):
+    """chy_output_write_async (OutputHDF5 behind the computation): its files
+    equal the synchronous writer's for the same state, although the next step
+    changes every field while the copy may still run; two in a row, the
+    second joined by chy_output_write and the last by chy_output_wait."""
+    h5 = _hdf5()
+    a, s_ = str(tmp_path / "a"), str(tmp_path / "s")
+    with context(opts((16, 8, 8), extra="-seed 3 -ppc 4")) as ctx:
+        print("async output: 16x8x8, two outputs", flush=True)
+        ctx.init_particles()
+        ctx.collect_moments(download=False)
+        ctx.solve_potential()
+        ctx.output_write(s_, "-0")
+        ctx.output_write_async(a, "-0")
+        ctx.step()
+        ctx.output_write(s_, "-1")          # waits for the asynchronous -0 first
+        ctx.output_write_async(a, "-1")
+        ctx.step()                          # the fields change behind the copy of -1
+        assert ctx.output_wait() in (0, chybrid.ERR_SUP)
+        assert ctx.output_wait() == 0       # nothing in flight any more
+    got = {ins: (_fields(h5, a + ins), _fields(h5, s_ + ins)) for ins in ("-0", "-1")}
+    for ins, (fa, fs) in got.items():
+        for nm in NAMES:
+            assert np.array_equal(fa[nm], fs[nm]), (ins, nm)
+    assert not np.array_equal(got["-0"][0]["potential"], got["-1"][0]["potential"])

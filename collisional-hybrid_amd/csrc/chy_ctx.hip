@@ -858,8 +858,7 @@ int chy_set_potential(chy_ctx *ctx, const double *phi)
 // Wait for the asynchronous writer, if one runs; its result.
 static int out_join(chy_ctx *ctx)
 {
-  if (!ctx->out_thread.joinable()) return CHY_OK;
-  ctx->out_thread.join();
+  if (ctx->out_thread.joinable()) ctx->out_thread.join();
   const int rc = ctx->out_rc;
   ctx->out_rc = CHY_OK;
   return rc;
@@ -871,18 +870,23 @@ int chy_output_write_async(chy_ctx *ctx, const char *stem, const char *insert)
   const int prev = out_join(ctx);
   if (prev != CHY_OK && prev != CHY_ERR_SUP) return set_err(ctx, prev, "the previous asynchronous output failed");
   const int64_t nc = ctx->g.nc;
-  if (!ctx->out_dev) {
-    if (hipMalloc(&ctx->out_dev, sizeof(double) * 6 * nc) != hipSuccess) {
-      ctx->out_dev = nullptr;
-      return set_err(ctx, CHY_ERR_MEM, "no device memory for the output snapshot");
+  if (!ctx->out_copied) {  // first call: all of the resources, or none
+    bool ok = hipMalloc(&ctx->out_dev, sizeof(double) * 6 * nc) == hipSuccess;
+    ok = ok && hipHostMalloc(&ctx->out_host, sizeof(double) * 6 * nc) == hipSuccess;
+    ok = ok && hipStreamCreateWithFlags(&ctx->out_stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&ctx->out_snap, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&ctx->out_copied, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      if (ctx->out_snap) (void)hipEventDestroy(ctx->out_snap);
+      if (ctx->out_stream) (void)hipStreamDestroy(ctx->out_stream);
+      if (ctx->out_host) (void)hipHostFree(ctx->out_host);
+      if (ctx->out_dev) (void)hipFree(ctx->out_dev);
+      ctx->out_snap = ctx->out_copied = nullptr;
+      ctx->out_stream = nullptr;
+      ctx->out_host = ctx->out_dev = nullptr;
+      (void)hipGetLastError();
+      return set_err(ctx, CHY_ERR_MEM, "no memory for the asynchronous output (6 nc f64 on the device and pinned)");
     }
-    if (hipHostMalloc(&ctx->out_host, sizeof(double) * 6 * nc) != hipSuccess) {
-      ctx->out_host = nullptr;
-      return set_err(ctx, CHY_ERR_MEM, "no pinned host memory for the output");
-    }
-    CHY_HIP(hipStreamCreateWithFlags(&ctx->out_stream, hipStreamNonBlocking));
-    CHY_HIP(hipEventCreateWithFlags(&ctx->out_snap, hipEventDisableTiming));
-    CHY_HIP(hipEventCreateWithFlags(&ctx->out_copied, hipEventDisableTiming));
   }
   // snapshot in stream order (the next step's kernels then overwrite the
   // fields freely): moments in f64, rhs, potential; zero where not yet made
@@ -904,7 +908,7 @@ int chy_output_write_async(chy_ctx *ctx, const char *stem, const char *insert)
   CHY_HIP(hipEventRecord(ctx->out_copied, ctx->out_stream));
   const std::string st(stem), in(insert);
   const int dev = ctx->device;
-  ctx->out_thread = std::thread([ctx, st, in, dev, nc]() {
+  auto write = [ctx, st, in, dev, nc]() {
     int rc = hipSetDevice(dev) == hipSuccess && hipEventSynchronize(ctx->out_copied) == hipSuccess ? CHY_OK
                                                                                                    : CHY_ERR_LIB;
     if (rc == CHY_OK) {
@@ -913,7 +917,12 @@ int chy_output_write_async(chy_ctx *ctx, const char *stem, const char *insert)
       rc = chyo_write_fields(&ctx->prm, st.c_str(), in.c_str(), f);
     }
     ctx->out_rc = rc;
-  });
+  };
+  try {
+    ctx->out_thread = std::thread(write);
+  } catch (...) {  // no thread to be had: write now, the result waits for chy_output_wait
+    write();
+  }
   return CHY_OK;
 }
 

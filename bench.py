@@ -63,12 +63,16 @@ def algorithmic_bytes(kernel, prec, np_, nc):
     if kernel == "push":
         # state in (position 3s [+ cell 4 in mixed], v 3s, pid 4), state out
         # (position 3s + cell 4, v 3s), node E 3s per node
+        # (+ the move direction, 1 B, in mixed)
         pos_in = 3 * s + (4 if prec == "mixed" else 0)
-        return np_ * (pos_in + 3 * s + 4 + 3 * s + 4 + 3 * s) + nc * 3 * s
+        dir_out = 1 if prec == "mixed" else 0
+        return np_ * (pos_in + 3 * s + 4 + 3 * s + 4 + 3 * s + dir_out) + nc * 3 * s
     if kernel == "sort_trans":
-        # per old cell: the ions' keys in, (direction, rank) out; 27 direction
-        # counts out and back in, new counts and 27 group bases out
-        return np_ * (4 + 2) + nc * (4 + 27 + 27 + 4 + 2 * 27)
+        # per old cell: the ions' move directions (mixed, written by the push)
+        # or new keys (FP64) in, (direction, rank) out; 27 direction counts out
+        # and back in, new counts and 27 group bases out
+        per_ion_in = 1 if prec == "mixed" else 4
+        return np_ * (per_ion_in + 2) + nc * (4 + 27 + 27 + 4 + 2 * 27)
     if kernel == "sort_scatter":
         return np_ * (4 + 2 + 4) + nc * (4 + 2 * 27)  # key + (direction, rank) in, perm out; offsets, bases
     if kernel == "sort_reorder":

@@ -314,6 +314,8 @@ int restore_stage(chy_ctx *ctx, const Stage &s)
   CHY_HIP(hipMemcpyAsync(ctx->S_dev, s.S, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
   ctx->step = s.step;
   ctx->order_valid = s.ordered;
+  ctx->dir_valid = false;
+  ctx->keys_moved = true;  // not known: the next push writes no directions
   ctx->partials_fresh = false;
   ctx->phi_prev_valid = false;
   ctx->phi_valid = true;

@@ -542,6 +542,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   A(dalloc(ctx, &ctx->maxcount, 1));
   A(dalloc(ctx, &ctx->trans, 27 * nc));
   A(dalloc(ctx, &ctx->info, cap));
+  if (ctx->mixed) A(dalloc(ctx, &ctx->dir, cap));
   A(dalloc(ctx, &ctx->nbase, 27 * nc));
   A(dalloc(ctx, &ctx->offs_alt, nc + 1));
   A(dalloc(ctx, &ctx->sortflag, 2));
@@ -614,7 +615,7 @@ int chy_destroy(chy_ctx *ctx)
   adjoint_release(ctx);
   for (int s = 0; s < 2; s++) { free_state(ctx->pd[s]); free_state(ctx->pf[s]); }
   void *ps[] = {ctx->perm, ctx->slotkey, ctx->counts, ctx->offsets, ctx->cursor, ctx->scan_tmp, ctx->maxcount,
-                ctx->trans, ctx->info, ctx->nbase, ctx->offs_alt, ctx->sortflag,
+                ctx->trans, ctx->info, ctx->dir, ctx->nbase, ctx->offs_alt, ctx->sortflag,
                 ctx->partials,
                 ctx->sred, ctx->S_dev, ctx->cstats, ctx->mom[0], ctx->mom[1], ctx->mom[2], ctx->mom[3],
                 ctx->efield[0], ctx->phi, ctx->phi_prev, ctx->b, ctx->r64, ctx->t64, ctx->V,

@@ -94,6 +94,12 @@ struct chy_ctx {
   // neighbour-move sort (the particles are in the stable order of `offsets`
   // and every ion moved at most one cell in each direction)
   bool order_valid = false;     // particle order = stable cell order of `offsets`
+  // mixed: move direction of each ion (0..26, 255 beyond a neighbour), written
+  // by the moving push from the stable order; dir_valid while it matches the
+  // keys (any other change of keys or order clears it)
+  uint8_t *dir = nullptr;
+  bool dir_valid = false;
+  bool keys_moved = false;      // keys changed since the order was made (directions then unknown)
   bool partials_fresh = false;  // the sort made the deposit's cell partials of the current state
   double *phi_prev = nullptr;   // the previous step's phi (-ksp_initial_guess extrapolate)
   bool phi_prev_valid = false;

@@ -41,6 +41,16 @@ __device__ __forceinline__ int64_t xcd_block()
   return (int64_t)(b & 7u) * (nb >> 3) + (b >> 3);
 }
 
+// Step between two cell indices along an axis of n cells: 0, +1, -1 (with
+// the periodic wrap), or 2 when they are not neighbours.
+__device__ __forceinline__ int nb_step(int diff, int n)
+{
+  if (diff == 0) return 0;
+  if (diff == 1 || diff == 1 - n) return 1;
+  if (diff == -1 || diff == n - 1) return -1;
+  return 2;  // not a neighbour
+}
+
 // Wave-aggregated integer atomics on per-cell counters for runs of equal keys.
 __device__ __forceinline__ void count_key(uint32_t *counts, int key)
 {
@@ -133,7 +143,8 @@ __global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t
 template <bool COLLIDE, bool MOVE>
 __global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConstF pc, int64_t np, PState<float> st,
                                                     const Vec4<float> *__restrict__ E4, const double *__restrict__ Sdev,
-                                                    uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats)
+                                                    uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats,
+                                                    uint8_t *__restrict__ dir)
 {
   __shared__ unsigned int sacc[2];
   if (threadIdx.x < 2) sacc[threadIdx.x] = 0u;
@@ -169,7 +180,13 @@ __global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConstF pc, int64
       st.vel[p] = Vec4<float>{vx, vy, vz, V.w};
       if (MOVE) {
         float ox = fx, oy = fy, oz = fz;
+        const int i0 = i, j0 = j, k0 = k;
         key = move_mixed_ijk(g, pc, i, j, k, ox, oy, oz, vx, vy, vz);
+        if (dir) {  // the move direction for the neighbour-move sort (k_trans_dir)
+          const int di = nb_step(i - i0, g.nx), dj = nb_step(j - j0, g.ny), dk = nb_step(k - k0, g.nz);
+          dir[p] = (di == 2 || dj == 2 || dk == 2) ? (uint8_t)255
+                                                   : (uint8_t)((di + 1) + 3 * (dj + 1) + 9 * (dk + 1));
+        }
         st.pos[p] = Vec4<float>{ox, oy, oz, P.w};
         st.cell[p] = key;
         bad = !isfinite(ox) || !isfinite(oy) || !isfinite(oz);
@@ -503,13 +520,6 @@ __global__ __launch_bounds__(256) void k_reorder(int64_t np, const uint32_t *__r
 // needed.  A move further than one cell, or 255 ions of one cell going one way,
 // sets a flag and the general sort runs instead.
 
-__device__ __forceinline__ int nb_step(int diff, int n)
-{
-  if (diff == 0) return 0;
-  if (diff == 1 || diff == 1 - n) return 1;
-  if (diff == -1 || diff == n - 1) return -1;
-  return 2;  // not a neighbour
-}
 
 __device__ __forceinline__ int64_t nb_cell(const Grid &g, int i, int j, int k)
 {
@@ -543,6 +553,39 @@ __global__ __launch_bounds__(256) void k_trans(Grid g, const uint32_t *__restric
       const int di = nb_step(i2 - i, g.nx), dj = nb_step(j2 - j, g.ny), dk = nb_step(k2 - k, g.nz);
       if (di == 2 || dj == 2 || dk == 2) { bad = true; continue; }
       const int d = (di + 1) + 3 * (dj + 1) + 9 * (dk + 1);
+      const uint32_t r = cnt[d][t];
+      if (r >= 255) { bad = true; continue; }
+      info[q0 + u] = (uint16_t)((d << 8) | r);
+      cnt[d][t] = (uint16_t)(r + 1);
+    }
+  }
+  for (int d = 0; d < 27; d++) trans[(int64_t)d * g.nc + c] = (uint8_t)cnt[d][t];
+  if (bad) atomicOr(flag, 1u);
+}
+
+// k_trans with the move directions the push wrote (dir, one byte per ion in
+// the old order) in place of the new keys: the same counts and ranks.
+__global__ __launch_bounds__(256) void k_trans_dir(Grid g, const uint32_t *__restrict__ oldoffs,
+                                                   const uint8_t *__restrict__ dir, uint16_t *__restrict__ info,
+                                                   uint8_t *__restrict__ trans, uint32_t *__restrict__ flag)
+{
+  __shared__ uint16_t cnt[27][256];  // column t belongs to thread t
+  const int t = threadIdx.x;
+  for (int d = 0; d < 27; d++) cnt[d][t] = 0;
+  const int64_t c = xcd_block() * 256 + t;
+  if (c >= g.nc) return;
+  bool bad = false;
+  const uint32_t lo = oldoffs[c], hi = oldoffs[c + 1];
+  constexpr int kB = 8;  // directions in flight per thread
+  for (uint32_t q0 = lo; q0 < hi; q0 += kB) {
+    int dd[kB];
+#pragma unroll
+    for (int u = 0; u < kB; u++) dd[u] = q0 + u < hi ? (int)dir[q0 + u] : 0;
+#pragma unroll
+    for (int u = 0; u < kB; u++) {
+      if (q0 + u >= hi) break;
+      const int d = dd[u];
+      if (d > 26) { bad = true; continue; }
       const uint32_t r = cnt[d][t];
       if (r >= 255) { bad = true; continue; }
       info[q0 + u] = (uint16_t)((d << 8) | r);
@@ -869,8 +912,12 @@ static int sort_neighbour(chy_ctx *ctx, bool &done)
   launch_fill<uint32_t>(ctx, 2, ctx->sortflag, 0u);
   launch_fill<uint32_t>(ctx, 1, ctx->maxcount, 0u);
   timer_start(ctx, "sort_trans");
-  hipLaunchKernelGGL(k_trans, ncb, 256, 0, ctx->stream, ctx->g, ctx->offsets, cell, ctx->info, ctx->trans,
-                     ctx->sortflag);
+  if (ctx->dir_valid)
+    hipLaunchKernelGGL(k_trans_dir, ncb, 256, 0, ctx->stream, ctx->g, ctx->offsets, (const uint8_t *)ctx->dir,
+                       ctx->info, ctx->trans, ctx->sortflag);
+  else
+    hipLaunchKernelGGL(k_trans, ncb, 256, 0, ctx->stream, ctx->g, ctx->offsets, cell, ctx->info, ctx->trans,
+                       ctx->sortflag);
   hipLaunchKernelGGL(k_counts, ncb, 256, 0, ctx->stream, ctx->g, ctx->trans, ctx->counts, ctx->nbase);
   timer_stop(ctx, "sort_trans");
   timer_start(ctx, "sort_scan");
@@ -911,6 +958,8 @@ static int sort_neighbour(chy_ctx *ctx, bool &done)
   ctx->partials_fresh = true;  // the deposit's cell partials of the new order are made
   std::swap(ctx->offsets, ctx->offs_alt);
   ctx->cur = 1 - ctx->cur;
+  ctx->dir_valid = false;  // the order is new: directions and keys agree with it
+  ctx->keys_moved = false;
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("sort: ") + hipGetErrorString(e));
   done = true;
@@ -925,6 +974,8 @@ int particles_sort(chy_ctx *ctx)
     if (done) return CHY_OK;
   }
   ctx->order_valid = false;
+  ctx->dir_valid = false;
+  ctx->keys_moved = false;
   ctx->partials_fresh = false;
   // counts[] was filled by the key pass (push, keys kernel or the fallback above)
   const int64_t nc = ctx->g.nc, np = ctx->np;
@@ -970,6 +1021,8 @@ int particles_sort(chy_ctx *ctx)
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("sort: ") + hipGetErrorString(e));
   ctx->order_valid = true;
+  ctx->dir_valid = false;
+  ctx->keys_moved = false;
   return CHY_OK;
 }
 
@@ -1040,6 +1093,8 @@ int particles_normaliser(chy_ctx *ctx)
 static int settle(chy_ctx *ctx)
 {
   ctx->order_valid = false;  // new data: the general sort
+  ctx->dir_valid = false;
+  ctx->keys_moved = false;
   ctx->partials_fresh = false;
   launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
   if (ctx->mixed)
@@ -1229,17 +1284,23 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
   // with 4096 blocks and 4.04 with 262144
   const unsigned nb = grid_for(ctx->np, 65536);
   timer_start(ctx, "push");
-#define LAUNCH(KER, T, C, M)                                                                                  \
+#define LAUNCH(KER, T, C, M, ...)                                                                             \
   hipLaunchKernelGGL((KER<C, M>), nb, kBlock, 0, ctx->stream, ctx->g, pcT, ctx->np, st,                      \
-                     (const Vec4<T> *)ctx->efield[0], ctx->S_dev, count ? ctx->counts : nullptr, ctx->cstats)
+                     (const Vec4<T> *)ctx->efield[0], ctx->S_dev, count ? ctx->counts : nullptr, ctx->cstats,   \
+                     ##__VA_ARGS__)
   if (ctx->mixed) {
     const PushConstF pcT = push_const_f(pc);
     PState<float> st = ctx->pf[ctx->cur];
-    if (collide && move) LAUNCH(k_push_mixed, float, true, true);
-    else if (collide) LAUNCH(k_push_mixed, float, true, false);
-    else if (move) LAUNCH(k_push_mixed, float, false, true);
-    else LAUNCH(k_push_mixed, float, false, false);
+    // from the stable order the moving push also writes each ion's move
+    // direction, which the neighbour-move sort then reads instead of the keys
+    uint8_t *dir = move && ctx->order_valid && !ctx->keys_moved ? ctx->dir : nullptr;
+    if (collide && move) LAUNCH(k_push_mixed, float, true, true, dir);
+    else if (collide) LAUNCH(k_push_mixed, float, true, false, dir);
+    else if (move) LAUNCH(k_push_mixed, float, false, true, dir);
+    else LAUNCH(k_push_mixed, float, false, false, dir);
+    if (move) ctx->dir_valid = dir != nullptr;
   } else {
+    ctx->dir_valid = false;
     const PushConst &pcT = pc;
     PState<double> st = ctx->pd[ctx->cur];
     if (collide && move) LAUNCH(k_push_fp64, double, true, true);
@@ -1248,6 +1309,7 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
     else LAUNCH(k_push_fp64, double, false, false);
   }
 #undef LAUNCH
+  if (move) ctx->keys_moved = true;
   timer_stop(ctx, "push");
   unsigned long long cs[4];
   CHY_HIP(hipMemcpyAsync(cs, ctx->cstats, sizeof(cs), hipMemcpyDeviceToHost, ctx->stream));
@@ -1268,6 +1330,8 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
 int particles_move_only(chy_ctx *ctx)
 {
   PushConst pc = push_const(ctx);
+  ctx->dir_valid = false;  // keys change without directions
+  ctx->keys_moved = true;
   ctx->partials_fresh = false;
   launch_fill<unsigned long long>(ctx, 4, ctx->cstats, 0ull);
   launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);

@@ -262,9 +262,9 @@ __device__ __forceinline__ void gather_c(const Grid &g, const Vec4<T> *__restric
 }
 
 // Mixed move: in-cell offset += v dt / d with an exact integer carry, for an
-// ion of cell (i, j, k); returns the new cell.
+// ion of cell (i, j, k); returns the new cell and leaves its indices in (i, j, k).
 template <typename PC>
-__device__ __forceinline__ int move_mixed_ijk(const Grid &g, const PC &pc, int i, int j, int k, float &fx,
+__device__ __forceinline__ int move_mixed_ijk(const Grid &g, const PC &pc, int &i, int &j, int &k, float &fx,
                                               float &fy, float &fz, float vx, float vy, float vz)
 {
   float o[3] = {fmaf(vx, (float)pc.cdt[0], fx), fmaf(vy, (float)pc.cdt[1], fy), fmaf(vz, (float)pc.cdt[2], fz)};
@@ -285,6 +285,7 @@ __device__ __forceinline__ int move_mixed_ijk(const Grid &g, const PC &pc, int i
     o[a] = f;
   }
   fx = o[0]; fy = o[1]; fz = o[2];
+  i = ci[0]; j = ci[1]; k = ci[2];
   return (int)(ci[0] + (int64_t)g.nx * (ci[1] + (int64_t)g.ny * ci[2]));
 }
 

@@ -251,3 +251,29 @@ def test_initial_guess_modes_reach_the_same_potential():
     for mode in ("zero", "extrapolate"):
         for a, b in zip(phis[mode], phis["previous"]):
             assert relmax(a, b) <= 1e-8, (mode, relmax(a, b))
+
+
+def test_mixed_sort_from_push_directions_matches_key_sort():
+    """The moving push writes each ion's move direction and the neighbour-move
+    sort ranks by it (k_trans_dir); a separate move (update_positions) leaves
+    the sort to the new keys (k_trans).  From the same state the arithmetic is
+    the same, so one step gives bitwise equal orders, states and moments."""
+    extra = ("-seed 6 -ppc 16 -ksp_rtol 1e-10 --perturbation uniform --perturbation-amplitude 0.01 -mn 0 "
+             "-precision mixed")
+    out = []
+    for fused in (True, False):
+        with context(opts((16, 16, 16), extra=extra)) as ctx:
+            ctx.init_particles()
+            ctx.collect_moments(download=False)
+            ctx.solve_potential()
+            if fused:
+                ctx.step()
+            else:
+                ctx.update_velocities()
+                ctx.update_positions()
+            out.append((as_particles(ctx.get_particles()), ctx.collect_moments()))
+    a, b = out
+    assert np.array_equal(a[0].pid, b[0].pid)
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(getattr(a[0], f), getattr(b[0], f)), f
+    assert np.array_equal(a[1], b[1])

@@ -228,3 +228,16 @@ def test_read_vlasov_hdf5_density_kji(tmp_path):
     _h5_write(h, tmp_path / "other.hdf", "density", n)
     rc, _, msg = _read_vlasov(prm, tmp_path / "other.hdf")
     assert rc == chybrid.ERR_ARG_WRONG and "density-kji" in msg
+
+
+def test_entry_points_refuse_a_null_context():
+    """Every context call returns PETSC_ERR_ARG_WRONG for a null context
+    without touching a device (no exception or abort crosses the boundary)."""
+    L = chybrid.lib()
+    assert L.chy_output_write_async(None, b"x", b"") == chybrid.ERR_ARG_WRONG
+    assert L.chy_output_wait(None) == chybrid.ERR_ARG_WRONG
+    assert L.chy_get_rhs(None, None) == chybrid.ERR_ARG_WRONG
+    assert L.chy_step(None, None) == chybrid.ERR_ARG_WRONG
+    prm, _ = chybrid.params_from_options(SMALL)
+    msg = chybrid.C.create_string_buffer(64)
+    assert L.chy_read_vlasov(chybrid.C.byref(prm), None, None, msg, len(msg)) == chybrid.ERR_ARG_WRONG

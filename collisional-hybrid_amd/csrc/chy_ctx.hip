@@ -354,14 +354,18 @@ int bicgstab_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *x, doubl
 
 // Row of the constant-coefficient operator at the mean density (the FFT
 // preconditioner's symbol).
+// With bb given, |b|^2 is reduced beside the density sum and read back with it
+// (one host round trip for both).
 template <typename T>
-int pc_symbol(chy_ctx *ctx, double v19[19])
+int pc_symbol(chy_ctx *ctx, double v19[19], double *bb = nullptr)
 {
   const int64_t nc = ctx->g.nc;
   launch_sum<T>(ctx, mom<T>(ctx, 0), nc, ctx->red_out);
-  double s;
-  TRY(fetch(ctx, ctx->red_out, 1, &s));
-  uniform_row(ctx->prm.lhs_type, ctx->prm.d, ctx->prm.kappa_e, s / (double)nc, v19);
+  if (bb) launch_mdot<double>(ctx, 1, ctx->b, nc, ctx->b, nc, ctx->red_out + 1);
+  double sv[2] = {0.0, 0.0};
+  TRY(fetch(ctx, ctx->red_out, bb ? 2 : 1, sv));
+  if (bb) *bb = sv[1];
+  uniform_row(ctx->prm.lhs_type, ctx->prm.d, ctx->prm.kappa_e, sv[0] / (double)nc, v19);
   // coefficients of the z-averaged operator for the two-level preconditioner
   const bool two = ctx->prm.pc_type == CHY_PC_FFT2 || ctx->prm.pc_type == CHY_PC_FFT2Z;
   // and the z-line factors of the fft2z correction (for the full operator,
@@ -386,11 +390,8 @@ int solve_impl(chy_ctx *ctx, chy_solve_stats *st)
   const int64_t nc = ctx->g.nc;
   const chy_params &p = ctx->prm;
   const T *n = mom<T>(ctx, 0);
-  double v19[19];
-  TRY(pc_symbol<T>(ctx, v19));
-  launch_mdot<double>(ctx, 1, ctx->b, nc, ctx->b, nc, ctx->red_out);
-  double bb;
-  TRY(fetch(ctx, ctx->red_out, 1, &bb));
+  double v19[19], bb = 0.0;
+  TRY(pc_symbol<T>(ctx, v19, &bb));
   const double bn = sqrt(bb);
   const double tol = std::max(p.ksp_rtol * bn, p.ksp_atol);
   int64_t its = 0, outer = 0;

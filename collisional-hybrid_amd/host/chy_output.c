@@ -6,6 +6,8 @@
  *
  * HDF5's C library is opened with dlopen so libchybrid.so has no link-time
  * dependency on it; the handful of entry points used are resolved by name.
+ * CHY_NO_HDF5=1 in the environment skips the library (raw-file output), as on
+ * a host without HDF5.
  */
 #include "chybrid.h"
 
@@ -47,6 +49,8 @@ static void h5_load_once(void)
 {
   if (H5.tried) return;
   H5.tried = 1;
+  const char *off = getenv("CHY_NO_HDF5");
+  if (off && off[0] && strcmp(off, "0") != 0) return;
   const char *names[] = {"libhdf5.so", "libhdf5.so.103", "libhdf5_serial.so", "/opt/conda/lib/libhdf5.so", NULL};
   void *h = NULL;
   for (int i = 0; names[i] && !h; i++) h = dlopen(names[i], RTLD_NOW | RTLD_LOCAL);

@@ -107,6 +107,24 @@ def test_hybrid_driver_runs_like_run_sh(tmp_path):
     assert p.returncode == 0 and "exits" in p.stdout
 
 
+def test_hybrid_driver_without_hdf5_writes_raw_files(tmp_path):
+    """Without an HDF5 library the driver writes each field as a raw .f64 file
+    and still exits 0 (the output calls report CHY_ERR_SUP, which is not a
+    failure of the run)."""
+    args = opts((8, 8, 8), extra="-Np 4096 -Nt 2 -output_interval 1 -seed 2 --output " + str(tmp_path / "raw"))
+    env = dict(os.environ)
+    env["OMPI_COMM_WORLD_RANK"] = "0"
+    env["CHY_NO_HDF5"] = "1"
+    p = subprocess.run([HYBRID] + args, cwd=tmp_path, capture_output=True, text=True, timeout=120, env=env)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    assert "No HDF5 library" in p.stdout and "END" in p.stdout
+    for insert in ("-initial", "-0", "-1"):
+        n = np.fromfile(str(tmp_path / f"raw{insert}.density.f64"))
+        assert n.size == 512 and n.sum() == pytest.approx(4096, rel=1e-12)
+        phi = np.fromfile(str(tmp_path / f"raw{insert}.potential.f64"))
+        assert phi.size == 512 and np.all(np.isfinite(phi))
+
+
 SOLVER = os.path.join(ROOT, "collisional-hybrid_amd", "bin", "solver")
 
 

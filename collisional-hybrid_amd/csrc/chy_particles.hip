@@ -893,6 +893,144 @@ __global__ __launch_bounds__(256) void k_init(Grid g, int64_t ppc, uint32_t k0, 
   }
 }
 
+// ---- the reference's initial densities (particles.c:26-45, distributions.c) --------
+// Direction integers of the first three dimensions of the reference's Sobol
+// generator (random.c:234-290), scaled to 30 bits: V[j][k] for bit j, dimension k.
+struct SobolDirs { uint32_t V[30][3]; };
+
+// 53-bit uniform in [0, 1) from two Philox words.
+__device__ __forceinline__ double u53(uint32_t hi, uint32_t lo)
+{
+  return ((double)(hi >> 5) * 67108864.0 + (double)(lo >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+constexpr uint32_t STREAM_PLACE = 4;
+
+// Ion p of a reference placement: flat-sobol (point p of the Sobol sequence,
+// the call order of random.c:273-287 given in closed form: the running XOR
+// after call p is the XOR of the direction integers of the set bits of the
+// Gray code of p + 1; positions as distributions.c:124-136), uniform (cell
+// centres of all but the last cell along each axis, cell index k fastest,
+// distributions.c:71-86) or sinusoidal (rejection against 1 + 0.25 sin(2 pi x)
+// with the density's maximum 1.25, distributions.c:144-232; the candidates of
+// ion p from its own Philox blocks, the last one kept after 64 rejections).
+// Velocities and weights as k_init.
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_init_density(Grid g, int kind, int64_t np, SobolDirs sob, uint32_t k0,
+                                                      uint32_t k1, double v0x, double v0y, double v0z, double vTx,
+                                                      double vTy, double vTz, int pert, double amp,
+                                                      PState<double> sd, PState<float> sf)
+{
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  double q[3];
+  if (kind == CHY_DENSITY_FLAT_SOBOL) {
+    const uint32_t gc = (uint32_t)(p + 1) ^ (uint32_t)((p + 1) >> 1);
+    uint32_t ix[3] = {0u, 0u, 0u};
+#pragma unroll 1
+    for (int j = 0; j < 30; j++) {
+      if ((gc >> j) & 1u) {
+        ix[0] ^= sob.V[j][0]; ix[1] ^= sob.V[j][1]; ix[2] ^= sob.V[j][2];
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const double r = (double)ix[a] * (1.0 / 1073741824.0);
+      double v = g.p0[a] + (r * (g.L[a] + g.d[a]) - 0.5 * g.d[a]);
+      if (v < g.p0[a]) v = g.p0[a] + 0.01 * g.d[a];
+      else if (v > g.p1[a]) v = g.p1[a] - 0.01 * g.d[a];
+      q[a] = wrap_pos(v, g.p0[a], g.p1[a], g.L[a]);
+    }
+  } else if (kind == CHY_DENSITY_UNIFORM) {
+    const int64_t mx = g.nx - 1, my = g.ny - 1, mz = g.nz - 1, m = mx * my * mz;
+    const int64_t r = p % m;
+    const int64_t k = r % mz, j = (r / mz) % my, i = r / (mz * my);
+    q[0] = g.p0[0] + g.d[0] * ((double)(i + 1) - 0.5);
+    q[1] = g.p0[1] + g.d[1] * ((double)(j + 1) - 0.5);
+    q[2] = g.p0[2] + g.d[2] * ((double)(k + 1) - 0.5);
+  } else {
+    q[0] = g.p0[0]; q[1] = g.p0[1]; q[2] = g.p0[2];
+#pragma unroll 1
+    for (uint32_t a = 0; a < 64; a++) {
+      const U4 b0 = philox4x32_10(U4{(uint32_t)p, a, STREAM_PLACE, 0u}, k0, k1);
+      const U4 b1 = philox4x32_10(U4{(uint32_t)p, a, STREAM_PLACE, 1u}, k0, k1);
+      const double u[3] = {u53(b0.x, b0.y), u53(b0.z, b0.w), u53(b1.x, b1.y)};
+#pragma unroll
+      for (int e = 0; e < 3; e++) q[e] = wrap_pos(g.p0[e] + u[e] * g.L[e], g.p0[e], g.p1[e], g.L[e]);
+      const double dens = 1.0 + 0.25 * sin(2.0 * M_PI * q[0]);
+      if (dens > u24<double>(b1.z) * 1.25) break;
+    }
+  }
+  const U4 b = philox4x32_10(U4{(uint32_t)p, 0u, STREAM_INIT, 1u}, k0, k1);
+  const double r1 = sqrt(-2.0 * log(u24<double>(b.x))), a1 = 2.0 * M_PI * u24<double>(b.y);
+  const double r2 = sqrt(-2.0 * log(u24<double>(b.z))), a2 = 2.0 * M_PI * u24<double>(b.w);
+  const double vx = v0x + vTx * (r1 * cos(a1));
+  const double vy = v0y + vTy * (r1 * sin(a1));
+  const double vz = v0z + vTz * (r2 * cos(a2));
+  // cell of the position (the clamped truncation of cell_coord)
+  int ci[3];
+  double fr[3];
+  cell_coord(q[0], g.p0[0], g.d[0], g.nx, ci[0], fr[0]);
+  cell_coord(q[1], g.p0[1], g.d[1], g.ny, ci[1], fr[1]);
+  cell_coord(q[2], g.p0[2], g.d[2], g.nz, ci[2], fr[2]);
+  const int64_t c = ci[0] + (int64_t)g.nx * (ci[1] + (int64_t)g.ny * ci[2]);
+  double wgt = 1.0;
+  if (pert == CHY_PERT_SINUSOIDAL_X) {
+    wgt = 1.0 + amp * sin(2.0 * M_PI * (q[0] - g.p0[0]) / (g.d[0] * (double)g.nx));
+  } else if (pert == CHY_PERT_UNIFORM || pert == CHY_PERT_GAUSSIAN) {
+    const U4 wv = philox4x32_10(U4{(uint32_t)c, 0u, STREAM_WEIGHT, 0u}, k0, k1);
+    const double w0 = u24<double>(wv.x), w1 = u24<double>(wv.y);
+    wgt = (pert == CHY_PERT_UNIFORM) ? 1.0 + amp * (2.0 * w0 - 1.0)
+                                     : 1.0 + amp * (sqrt(-2.0 * log(w0)) * cos(2.0 * M_PI * w1));
+  }
+  if (!MIXED) {
+    sd.x[p] = q[0]; sd.y[p] = q[1]; sd.z[p] = q[2];
+    sd.vx[p] = vx; sd.vy[p] = vy; sd.vz[p] = vz;
+    sd.pid[p] = (uint32_t)p;
+    if (sd.w) sd.w[p] = wgt;
+  } else {
+    // cell + FP32 offset as an upload of the FP64 position makes them
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    float o[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      float f = (float)fr[a];
+      if (f >= 1.0f) { f = 0.0f; ci[a] = (ci[a] + 1) % nn[a]; }
+      o[a] = f;
+    }
+    sf.pos[p] = Vec4<float>{o[0], o[1], o[2], (float)wgt};
+    sf.vel[p] = Vec4<float>{(float)vx, (float)vy, (float)vz, __uint_as_float((uint32_t)p)};
+    sf.cell[p] = (int32_t)(ci[0] + g.nx * (ci[1] + g.ny * ci[2]));
+  }
+}
+
+// Direction integers of the reference's Sobol generator (random.c:241-270):
+// degrees 1, 2, 3, polynomial codes 0, 1, 1, leading odd integers {1},
+// {1, 1}, {1, 3, 7}, and the recurrence as random.c:260-269 writes it (one
+// XOR of the previous integer per set polynomial bit).
+static SobolDirs sobol_dirs()
+{
+  SobolDirs s;
+  const int deg[3] = {1, 2, 3};
+  const unsigned code[3] = {0u, 1u, 1u};
+  const uint32_t lead[3][3] = {{1u, 0u, 0u}, {1u, 1u, 0u}, {1u, 3u, 7u}};
+  for (int k = 0; k < 3; k++) {
+    const int dg = deg[k];
+    for (int j = 0; j < dg; j++) s.V[j][k] = lead[k][j] << (29 - j);
+    for (int j = dg; j < 30; j++) {
+      uint32_t v = s.V[j - dg][k];
+      v ^= v >> dg;
+      unsigned bits = code[k];
+      for (int l = dg - 1; l >= 1; l--) {
+        if (bits & 1u) v ^= s.V[j - 1][k];
+        bits >>= 1;
+      }
+      s.V[j][k] = v;
+    }
+  }
+  return s;
+}
+
 // ---- host orchestration ------------------------------------------------------------
 static unsigned grid_for(int64_t n, unsigned cap = 4096)
 {
@@ -1242,12 +1380,56 @@ int particles_download(chy_ctx *ctx, double *pos, double *vel, double *w, uint32
   return CHY_OK;
 }
 
+// The reference's placements (InitializePositions, particles.c:26-45).
+static int particles_init_density(chy_ctx *ctx)
+{
+  const chy_params &p = ctx->prm;
+  const int64_t nc = ctx->g.nc;
+  const int kind = p.density_type;
+  static const char *const names[] = {"flat-sobol", "flat-reverse", "flat-normal", "uniform", "sinusoidal",
+                                      "gaussian"};
+  if (kind == CHY_DENSITY_FLAT_NORMAL || kind == CHY_DENSITY_FLAT_REVERSE || kind == CHY_DENSITY_GAUSSIAN)
+    return set_err(ctx, CHY_ERR_ARG_WRONG, std::string("Not implemented: ") + names[kind] + " density");
+  if (kind != CHY_DENSITY_FLAT_SOBOL && kind != CHY_DENSITY_UNIFORM && kind != CHY_DENSITY_SINUSOIDAL)
+    return set_err(ctx, CHY_ERR_ARG_WRONG, "unknown density type");
+  int64_t np = p.Np;
+  if (kind == CHY_DENSITY_UNIFORM) {
+    // Np / nc ions in each cell but the last along every axis (the
+    // reference's swarm keeps the unset slots of the other cells at undefined
+    // positions; here they are not created)
+    np = (p.Np / nc) * (int64_t)(ctx->g.nx - 1) * (ctx->g.ny - 1) * (ctx->g.nz - 1);
+  }
+  // the reference's Sobol generator stops at its 2^30-th call (random.c:279-281)
+  if (kind == CHY_DENSITY_FLAT_SOBOL && np >= (int64_t)1 << 30)
+    return set_err(ctx, CHY_ERR_ARG_WRONG, "MAXBIT too small in the Sobol sequence: at most 2^30 - 1 ions");
+  if (np > ctx->cap) return set_err(ctx, CHY_ERR_ARG_RANGE, "the ion count exceeds the context capacity");
+  if (p.perturbation != CHY_PERT_NONE && !ctx->weighted)
+    return set_err(ctx, CHY_ERR_ARG_INCOMP, "perturbation needs weight buffers");
+  ctx->np = np;
+  if (np > 0) {
+    const SobolDirs sob = sobol_dirs();
+    const unsigned nb = (unsigned)((np + kBlock - 1) / kBlock);
+    PState<double> sd = ctx->pd[ctx->cur];
+    PState<float> sf = ctx->pf[ctx->cur];
+    if (ctx->mixed)
+      hipLaunchKernelGGL((k_init_density<true>), nb, kBlock, 0, ctx->stream, ctx->g, kind, np, sob,
+                         (uint32_t)p.seed, (uint32_t)(p.seed >> 32), p.vi0[0], p.vi0[1], p.vi0[2], p.viT[0],
+                         p.viT[1], p.viT[2], p.perturbation, p.perturbation_amplitude, sd, sf);
+    else
+      hipLaunchKernelGGL((k_init_density<false>), nb, kBlock, 0, ctx->stream, ctx->g, kind, np, sob,
+                         (uint32_t)p.seed, (uint32_t)(p.seed >> 32), p.vi0[0], p.vi0[1], p.vi0[2], p.viT[0],
+                         p.viT[1], p.viT[2], p.perturbation, p.perturbation_amplitude, sd, sf);
+  }
+  return settle(ctx);
+}
+
 int particles_init(chy_ctx *ctx)
 {
   const chy_params &p = ctx->prm;
   const int64_t nc = ctx->g.nc;
+  if (p.density_type != CHY_DENSITY_LATTICE) return particles_init_density(ctx);
   int64_t ppc = p.ppc > 0 ? p.ppc : (p.Np % nc == 0 ? p.Np / nc : 0);
-  if (ppc <= 0) return set_err(ctx, CHY_ERR_ARG_INCOMP, "device initialisation needs Np to be a multiple of the cell count (-ppc)");
+  if (ppc <= 0) return set_err(ctx, CHY_ERR_ARG_INCOMP, "the per-cell lattice placement needs Np to be a multiple of the cell count (-ppc)");
   const int64_t np = ppc * nc;
   if (np > ctx->cap) return set_err(ctx, CHY_ERR_ARG_RANGE, "ppc x cells exceeds the context capacity");
   if (p.perturbation != CHY_PERT_NONE && !ctx->weighted)

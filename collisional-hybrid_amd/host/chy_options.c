@@ -163,7 +163,7 @@ static int get_enum(const optdb *db, const char *name, const char *const *names,
 static const char *const RHSNames[] = {"constant", "sinusoidal", "full", NULL};
 static const char *const LHSNames[] = {"identity", "laplacian", "full", NULL};
 static const char *const DensityNames[] = {"flat-sobol", "flat-reverse", "flat-normal", "uniform",
-                                           "sinusoidal", "gaussian", NULL};
+                                           "sinusoidal", "gaussian", "lattice", NULL};
 static const char *const BCNames[] = {"periodic", "injection", "reflection", "dirichlet", "neumann", NULL};
 static const char *const PertNames[] = {"none", "sinusoidal-x", "uniform", "gaussian", NULL};
 static const char *const PrecNames[] = {"fp64", "mixed", NULL};
@@ -315,7 +315,7 @@ int chy_params_from_options(int argc, const char *const *argv, chy_params *p, ch
   get_bool(&db, "--view-lhs", &p->view_lhs, &e);
   get_enum(&db, "--rhs-type", RHSNames, &p->rhs_type, &e);
   get_enum(&db, "--lhs-type", LHSNames, &p->lhs_type, &e);
-  get_enum(&db, "--density-type", DensityNames, &p->density_type, &e);
+  const int density_given = get_enum(&db, "--density-type", DensityNames, &p->density_type, &e);
   const char *nn[3] = {"-Nx", "-Ny", "-Nz"};
   const char *n0[3] = {"-x0", "-y0", "-z0"}, *n1[3] = {"-x1", "-y1", "-z1"};
   const char *b0[3] = {"--x0-bc", "--y0-bc", "--z0-bc"}, *b1[3] = {"--x1-bc", "--y1-bc", "--z1-bc"};
@@ -362,6 +362,14 @@ int chy_params_from_options(int argc, const char *const *argv, chy_params *p, ch
   int64_t iv;
   if (get_int(&db, "-seed", &iv, &e)) p->seed = (uint64_t)iv;
   get_int(&db, "-ppc", &p->ppc, &e);
+  get_int(&db, "-diagnostics_interval", &p->diagnostics_interval, &e);
+  /* -ppc selects the per-cell lattice placement (a build addition) */
+  if (p->ppc > 0) {
+    if (density_given && p->density_type != CHY_DENSITY_LATTICE)
+      emsg(&e, CHY_ERR_ARG_INCOMP, "-ppc places ions per cell; it does not combine with --density-type %s\n",
+           DensityNames[p->density_type]);
+    p->density_type = CHY_DENSITY_LATTICE;
+  }
   get_enum(&db, "--perturbation", PertNames, &p->perturbation, &e);
   get_real(&db, "--perturbation-amplitude", &p->perturbation_amplitude, &e);
   get_enum(&db, "-precision", PrecNames, &p->precision, &e);

@@ -150,6 +150,21 @@ void orc_init_particles(const int64_t N[3], const double p0[3], const double d[3
                         double *x, double *y, double *z, double *vx, double *vy, double *vz,
                         uint32_t *pid, double *w);
 
+/* The reference's initial densities (particles.c:26-45, distributions.c):
+ * flat-sobol (its Sobol sequence, distributions.c:95-141), uniform (cell
+ * centres, distributions.c:33-92) and sinusoidal (rejection against
+ * 1 + 0.25 sin(2 pi x), distributions.c:144-232).  Velocities and weights as
+ * orc_init_particles.  Returns the ion count placed, -1 when the Sobol
+ * sequence runs out of bits, -2 for another kind. */
+enum { ORC_DENSITY_FLAT_SOBOL = 0, ORC_DENSITY_UNIFORM = 3, ORC_DENSITY_SINUSOIDAL = 4 };
+void orc_sobol_dirs(uint32_t V[30][3]);
+int orc_sobol_points(int64_t np, double *r);
+int64_t orc_density_count(int kind, const int64_t N[3], int64_t Np);
+int64_t orc_init_density(int kind, const int64_t N[3], const double p0[3], const double p1[3], const double L[3],
+                         const double d[3], int64_t Np, uint64_t seed, const double v0[3], const double vT[3],
+                         int pert, double amp, double *x, double *y, double *z, double *vx, double *vy,
+                         double *vz, uint32_t *pid, double *w);
+
 /* ---- potential solve (reference potential.c:6-16 + PETSc KSP) ----------- */
 enum { ORC_PC_NONE = 0, ORC_PC_JACOBI = 1, ORC_PC_FFT = 2, ORC_PC_ZLINE = 3, ORC_PC_CHEBY = 4, ORC_PC_FFT2 = 5,
        ORC_PC_FFT2Z = 6 };

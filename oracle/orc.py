@@ -66,6 +66,12 @@ def _declare(L):
     L.orc_deposit_sorted.argtypes = [P, P, P, C.c_int64] + [P] * 7 + [P] + [P] * 4
     L.orc_deposit_scatter.argtypes = [P, P, P, C.c_int64] + [P] * 7 + [P] * 4
     L.orc_init_particles.argtypes = [P, P, P, C.c_int64, C.c_uint64, P, P, C.c_int, C.c_double] + [P] * 8
+    L.orc_sobol_dirs.argtypes = [P]
+    L.orc_sobol_points.argtypes = [C.c_int64, P]
+    L.orc_density_count.argtypes = [C.c_int, P, C.c_int64]
+    L.orc_density_count.restype = C.c_int64
+    L.orc_init_density.argtypes = [C.c_int, P, P, P, P, P, C.c_int64, C.c_uint64, P, P, C.c_int, C.c_double] + [P] * 8
+    L.orc_init_density.restype = C.c_int64
     L.orc_pc_apply.argtypes = [C.c_int, C.c_int, P, P, P, P, P, P]
     L.orc_pc_apply_deg.argtypes = [C.c_int, C.c_int, P, P, P, P, P, P, C.c_int]
     L.orc_solve.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P, P, P, P,
@@ -290,6 +296,37 @@ def init_particles(g, ppc, seed, v0, vT, pert=0, amp=0.0):
                              ptr(v0), ptr(vT), int(pert), float(amp),
                              *[ptr(a) for a in arrs], ptr(pid), ptr(w))
     return Particles(*arrs, pid=pid, w=w)
+
+
+DENSITY_FLAT_SOBOL, DENSITY_UNIFORM, DENSITY_SINUSOIDAL = 0, 3, 4
+
+
+def sobol_dirs():
+    """[30][3] direction integers of the reference's Sobol generator."""
+    V = np.zeros((30, 3), np.uint32)
+    lib().orc_sobol_dirs(ptr(V))
+    return V
+
+
+def sobol_points(n):
+    r = np.zeros((int(n), 3))
+    if lib().orc_sobol_points(int(n), ptr(r)) != 0:
+        raise ValueError("the Sobol sequence needs more than 30 bits")
+    return r
+
+
+def init_density(g, kind, Np, seed, v0, vT, pert=0, amp=0.0):
+    """The reference's density placements (flat-sobol, uniform, sinusoidal)."""
+    n = int(lib().orc_density_count(int(kind), ptr(g.N), int(Np)))
+    arrs = [np.zeros(max(n, 1)) for _ in range(6)]
+    pid = np.zeros(max(n, 1), np.uint32)
+    w = np.zeros(max(n, 1)) if pert else None
+    v0, vT = _c(v0), _c(vT)
+    rc = lib().orc_init_density(int(kind), ptr(g.N), ptr(g.p0), ptr(g.p1), ptr(g.L), ptr(g.d), int(Np), int(seed),
+                                ptr(v0), ptr(vT), int(pert), float(amp), *[ptr(a) for a in arrs], ptr(pid), ptr(w))
+    if rc < 0:
+        raise ValueError(f"orc_init_density failed ({rc})")
+    return Particles(*[a[:n] for a in arrs], pid=pid[:n], w=None if w is None else w[:n])
 
 
 def pc_apply(g, K, n, r, pc=PC_FFT, lhs_type=LHS_FULL, degree=CHEBY_DEGREE):

@@ -252,3 +252,175 @@ void orc_init_particles(const int64_t N[3], const double p0[3], const double d[3
     }
   }
 }
+
+/* ---- the reference's initial densities (particles.c:26-45) -------------------
+ * Positions as distributions.c places them; velocities v0 + vT N(0,1) and the
+ * weight perturbation from the same Philox blocks as orc_init_particles (the
+ * reference draws velocities from gasdev, particles.c:100-126, a stream this
+ * build does not reproduce).  Ion p gets id p. */
+
+/* Direction integers, scaled to MAXBIT = 30 bits, of the first three
+ * dimensions of the Sobol generator of random.c:234-290: degrees 1, 2, 3
+ * (random.c:241), polynomial codes 0, 1, 1 (random.c:242) and the leading odd
+ * integers {1}, {1, 1}, {1, 3, 7} (random.c:244).  The later integers follow
+ * the recurrence of random.c:260-269 as written there: it XORs in the
+ * previous integer V[j-1] once for each set polynomial bit.  For the third
+ * dimension this is the recurrence of the primitive polynomial x^3 + x^2 + 1
+ * (not the x^3 + x + 1 that code 1 names in the Numerical Recipes table). */
+void orc_sobol_dirs(uint32_t V[30][3])
+{
+  static const int deg[3] = {1, 2, 3};
+  static const unsigned code[3] = {0u, 1u, 1u};
+  static const uint32_t lead[3][3] = {{1u, 0u, 0u}, {1u, 1u, 0u}, {1u, 3u, 7u}};
+  for (int k = 0; k < 3; k++) {
+    const int s = deg[k];
+    for (int j = 0; j < s; j++) V[j][k] = lead[k][j] << (29 - j);
+    for (int j = s; j < 30; j++) {
+      uint32_t v = V[j - s][k];
+      v ^= v >> s;
+      unsigned bits = code[k];
+      for (int l = s - 1; l >= 1; l--) {
+        if (bits & 1u) v ^= V[j - 1][k];
+        bits >>= 1;
+      }
+      V[j][k] = v;
+    }
+  }
+}
+
+/* The first np points of the sequence in the order successive calls return
+ * them (random.c:273-287): call t XORs the direction integer of the lowest
+ * zero bit of t into the running integers.  r is [np][3] in [0, 1).  Returns
+ * -1 when a call would need a 31st bit (the reference's "MAXBIT too small"). */
+int orc_sobol_points(int64_t np, double *r)
+{
+  uint32_t V[30][3];
+  orc_sobol_dirs(V);
+  uint32_t ix[3] = {0u, 0u, 0u};
+  for (int64_t t = 0; t < np; t++) {
+    uint64_t im = (uint64_t)t;
+    int j = 0;
+    while (im & 1u) { im >>= 1; j++; }
+    if (j >= 30) return -1;
+    for (int k = 0; k < 3; k++) {
+      ix[k] ^= V[j][k];
+      r[3 * t + k] = (double)ix[k] * (1.0 / 1073741824.0);
+    }
+  }
+  return 0;
+}
+
+/* 53-bit uniform in [0, 1) from two Philox words. */
+static double u53(uint32_t hi, uint32_t lo)
+{
+  return ((double)(hi >> 5) * 67108864.0 + (double)(lo >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+enum { STREAM_PLACE = 4 };
+
+/* Velocity, weight and id of ion p at position (x, y, z). */
+static void finish_ion(int64_t p, const int64_t N[3], const double p0[3], const double d[3], uint64_t seed,
+                       const double v0[3], const double vT[3], int pert, double amp, double *vx, double *vy,
+                       double *vz, uint32_t *pid, double *w, double x, double y, double z)
+{
+  double ub[4];
+  philox_u((uint32_t)p, 0, STREAM_INIT, 1, seed, ub);
+  const double r1 = sqrt(-2.0 * log(ub[0])), a1 = 2.0 * M_PI * ub[1];
+  const double r2 = sqrt(-2.0 * log(ub[2])), a2 = 2.0 * M_PI * ub[3];
+  vx[p] = v0[0] + vT[0] * (r1 * cos(a1));
+  vy[p] = v0[1] + vT[1] * (r1 * sin(a1));
+  vz[p] = v0[2] + vT[2] * (r2 * cos(a2));
+  pid[p] = (uint32_t)p;
+  if (!w) return;
+  if (pert == 1) {
+    w[p] = 1.0 + amp * sin(2.0 * M_PI * (x - p0[0]) / (d[0] * (double)N[0]));
+  } else if (pert == 2 || pert == 3) {
+    int64_t ci[3];
+    const double q[3] = {x, y, z};
+    for (int a = 0; a < 3; a++) {
+      int64_t i = (int64_t)((q[a] - p0[a]) / d[a]);
+      if (i > N[a] - 1) i = N[a] - 1;
+      if (i < 0) i = 0;
+      ci[a] = i;
+    }
+    const int64_t c = ci[0] + N[0] * (ci[1] + N[1] * ci[2]);
+    double u[4];
+    philox_u((uint32_t)c, 0, STREAM_WEIGHT, 0, seed, u);
+    w[p] = (pert == 2) ? 1.0 + amp * (2.0 * u[0] - 1.0)
+                       : 1.0 + amp * (sqrt(-2.0 * log(u[0])) * cos(2.0 * M_PI * u[1]));
+  } else {
+    w[p] = 1.0;
+  }
+}
+
+int64_t orc_density_count(int kind, const int64_t N[3], int64_t Np)
+{
+  if (kind == ORC_DENSITY_UNIFORM) {
+    const int64_t nc = N[0] * N[1] * N[2];
+    return (Np / nc) * (N[0] - 1) * (N[1] - 1) * (N[2] - 1);
+  }
+  return Np;
+}
+
+int64_t orc_init_density(int kind, const int64_t N[3], const double p0[3], const double p1[3], const double L[3],
+                         const double d[3], int64_t Np, uint64_t seed, const double v0[3], const double vT[3],
+                         int pert, double amp, double *x, double *y, double *z, double *vx, double *vy,
+                         double *vz, uint32_t *pid, double *w)
+{
+  const int64_t np = orc_density_count(kind, N, Np);
+  if (kind == ORC_DENSITY_FLAT_SOBOL) {
+    /* distributions.c:95-141: v = r (L + d) - d/2, pulled in by 0.01 d when it
+     * falls outside [p0, p1] */
+    double *r = malloc(sizeof(double) * 3 * (size_t)(np > 0 ? np : 1));
+    if (orc_sobol_points(np, r)) { free(r); return -1; }
+    for (int64_t p = 0; p < np; p++) {
+      double q[3];
+      for (int a = 0; a < 3; a++) {
+        double v = p0[a] + (r[3 * p + a] * (L[a] + d[a]) - 0.5 * d[a]);
+        if (v < p0[a]) v = p0[a] + 0.01 * d[a];
+        else if (v > p1[a]) v = p1[a] - 0.01 * d[a];
+        q[a] = orc_wrap(v, p0[a], p1[a], L[a]);
+      }
+      x[p] = q[0]; y[p] = q[1]; z[p] = q[2];
+      finish_ion(p, N, p0, d, seed, v0, vT, pert, amp, vx, vy, vz, pid, w, q[0], q[1], q[2]);
+    }
+    free(r);
+  } else if (kind == ORC_DENSITY_UNIFORM) {
+    /* distributions.c:33-92: Np / nc ions at the centre of every cell except
+     * the last one along each axis, cell index k fastest */
+    const int64_t mx = N[0] - 1, my = N[1] - 1, mz = N[2] - 1, m = mx * my * mz;
+    for (int64_t p = 0; p < np; p++) {
+      const int64_t q = p % m;
+      const int64_t k = q % mz, j = (q / mz) % my, i = q / (mz * my);
+      x[p] = p0[0] + d[0] * ((double)(i + 1) - 0.5);
+      y[p] = p0[1] + d[1] * ((double)(j + 1) - 0.5);
+      z[p] = p0[2] + d[2] * ((double)(k + 1) - 0.5);
+      finish_ion(p, N, p0, d, seed, v0, vT, pert, amp, vx, vy, vz, pid, w, x[p], y[p], z[p]);
+    }
+  } else if (kind == ORC_DENSITY_SINUSOIDAL) {
+    /* distributions.c:144-232 with SinusoidalDistribution (152-162): accept a
+     * uniform point where 1 + 0.25 sin(2 pi x) > u * 1.25, the density's
+     * maximum; the candidates of ion p come from its own Philox blocks
+     * (attempt a: {p, a, PLACE, 0/1}); after 64 rejections the last candidate
+     * is kept (probability below 0.4^64) */
+    for (int64_t p = 0; p < np; p++) {
+      double q[3] = {p0[0], p0[1], p0[2]};
+      for (uint32_t a = 0; a < 64; a++) {
+        const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+        const uint32_t c0[4] = {(uint32_t)p, a, STREAM_PLACE, 0u}, c1[4] = {(uint32_t)p, a, STREAM_PLACE, 1u};
+        uint32_t b0[4], b1[4];
+        orc_philox4x32_10(c0, key, b0);
+        orc_philox4x32_10(c1, key, b1);
+        const double u[3] = {u53(b0[0], b0[1]), u53(b0[2], b0[3]), u53(b1[0], b1[1])};
+        for (int e = 0; e < 3; e++) q[e] = orc_wrap(p0[e] + u[e] * L[e], p0[e], p1[e], L[e]);
+        const double dens = 1.0 + 0.25 * sin(2.0 * M_PI * q[0]);
+        if (dens > orc_u24(b1[2]) * 1.25) break;
+      }
+      x[p] = q[0]; y[p] = q[1]; z[p] = q[2];
+      finish_ion(p, N, p0, d, seed, v0, vT, pert, amp, vx, vy, vz, pid, w, q[0], q[1], q[2]);
+    }
+  } else {
+    return -2;
+  }
+  return np;
+}

@@ -134,6 +134,47 @@ def test_device_init_matches_oracle():
             assert relmax(getattr(got, a), getattr(ps, a)) <= 1e-14, a
 
 
+@pytest.mark.parametrize("kind,name", [(orc.DENSITY_FLAT_SOBOL, "flat-sobol"), (orc.DENSITY_UNIFORM, "uniform"),
+                                       (orc.DENSITY_SINUSOIDAL, "sinusoidal")])
+def test_reference_density_placements_match_oracle(kind, name):
+    """--density-type flat-sobol / uniform / sinusoidal (particles.c:26-45,
+    distributions.c) on the device against the oracle's placement: the cell
+    order and positions bitwise (FP64), velocities and weights to 1e-14; the
+    mixed state holds the same cells and offsets to FP32 rounding."""
+    Np = 3001
+    for pert, pname in ((0, ""), (2, " --perturbation uniform --perturbation-amplitude 0.01")):
+        args = opts((8, 4, 6), extra=f"-seed 3 -Np {Np} --density-type {name}" + pname)
+        with context(args) as ctx:
+            ctx.init_particles()
+            got = as_particles(ctx.get_particles())
+            g = grid_of(ctx.prm)
+            vi0, viT = list(ctx.prm.vi0), list(ctx.prm.viT)
+        ref = orc.init_density(g, kind, Np, 3, vi0, viT, pert=pert, amp=0.01)
+        assert got.np == ref.np == (Np if kind != orc.DENSITY_UNIFORM else (Np // g.nc) * 7 * 3 * 5)
+        ps, _ = oracle_sorted(g, ref)
+        assert np.array_equal(got.pid, ps.pid)
+        for a in ("x", "y", "z"):
+            assert np.array_equal(getattr(got, a), getattr(ps, a)), a
+        for a in ("vx", "vy", "vz") + (("w",) if pert else ()):
+            assert relmax(getattr(got, a), getattr(ps, a)) <= 1e-14, a
+        with context(args + ["-precision", "mixed"]) as ctx:
+            ctx.init_particles()
+            m = as_particles(ctx.get_particles())
+        assert np.array_equal(m.pid, ps.pid)
+        for a in ("x", "y", "z"):
+            assert np.abs(getattr(m, a) - getattr(ps, a)).max() <= 1e-7 * g.d[0], a
+
+
+def test_unimplemented_density_types_are_refused():
+    """particles.c:28-45: flat-normal, flat-reverse and gaussian raise
+    PETSC_ERR_ARG_WRONG ("Not implemented")."""
+    for name in ("flat-normal", "flat-reverse", "gaussian"):
+        with context(opts((4, 4, 4), extra=f"-Np 64 --density-type {name}")) as ctx:
+            with pytest.raises(chybrid.ChyError) as e:
+                ctx.init_particles()
+            assert e.value.code == chybrid.ERR_ARG_WRONG and "Not implemented" in str(e.value), name
+
+
 @pytest.mark.parametrize("collide,N,more,rtol", [
     (False, (8, 8, 8), "", 1e-12), (True, (8, 8, 8), "", 1e-12),
     # ions crossing more than one cell per step, and 300 ions per cell (more

@@ -95,6 +95,30 @@ def test_option_errors_mirror_reference_classes():
         chybrid.params_from_options("-Nx 2".split())
 
 
+def test_density_type_options():
+    """--density-type names (parameters.c:14-16, default flat-sobol); -ppc
+    selects the per-cell lattice and refuses another explicit density type."""
+    p, _ = chybrid.params_from_options([])
+    assert p.density_type == chybrid.DENSITY_FLAT_SOBOL
+    for name, v in (("flat-sobol", 0), ("flat-reverse", 1), ("flat-normal", 2), ("uniform", 3), ("sinusoidal", 4),
+                    ("gaussian", 5), ("lattice", chybrid.DENSITY_LATTICE)):
+        p, _ = chybrid.params_from_options(["--density-type", name])
+        assert p.density_type == v, name
+    p, _ = chybrid.params_from_options("-Nx 4 -Ny 4 -Nz 4 -ppc 2".split())
+    assert p.density_type == chybrid.DENSITY_LATTICE and p.Np == 128
+    p, _ = chybrid.params_from_options("-Nx 4 -Ny 4 -Nz 4 -ppc 2 --density-type lattice".split())
+    assert p.density_type == chybrid.DENSITY_LATTICE
+    with pytest.raises(chybrid.ChyError) as e:
+        chybrid.params_from_options("-Nx 4 -Ny 4 -Nz 4 -ppc 2 --density-type sinusoidal".split())
+    assert e.value.code == chybrid.ERR_ARG_INCOMP
+    with pytest.raises(chybrid.ChyError) as e:
+        chybrid.params_from_options("--density-type bogus".split())
+    assert e.value.code == chybrid.ERR_ARG_WRONG
+    # any -Np without -ppc
+    p, _ = chybrid.params_from_options("-Nx 4 -Ny 4 -Nz 4 -Np 1001".split())
+    assert p.Np == 1001 and p.density_type == chybrid.DENSITY_FLAT_SOBOL
+
+
 def test_solver_option_mapping():
     p, msg = chybrid.params_from_options("-ksp_type bcgs -pc_type hypre -ksp_rtol 1e-9 -ksp_max_it 77 "
                                          "-ksp_gmres_restart 12 -ksp_monitor_short -precision mixed".split())

@@ -281,3 +281,122 @@ def test_init_particles_layout_and_moments():
     assert np.array_equal(p.x, q.x) and np.array_equal(p.w, q.w)
     r = orc.init_particles(g, 16, 1, (10.0, 0.0, 0.0), vT, pert=2, amp=0.01)
     assert not np.array_equal(p.x, r.x)
+
+
+# ---- the reference's density placements (particles.c:26-45, distributions.c) ----
+
+def _textbook_sobol(npts, s, a, m_lead, bits=30):
+    """Points 1..npts of a one-dimensional Sobol sequence by Bratley & Fox
+    (ACM TOMS 14, 1988): primitive polynomial x^s + a_1 x^(s-1) + ... +
+    a_(s-1) x + 1, odd leading integers m_1..m_s, then
+    m_j = 2 a_1 m_(j-1) ^ 4 a_2 m_(j-2) ^ ... ^ 2^s m_(j-s) ^ m_(j-s), and
+    point n = XOR of v_i = m_i / 2^i over the set bits i of the Gray code of n
+    (Antonov & Saleev)."""
+    m = list(m_lead)
+    for j in range(s, bits):
+        v = m[j - s] ^ (m[j - s] << s)
+        for i in range(1, s):
+            if a[i - 1]:
+                v ^= m[j - i] << i
+        m.append(v)
+    V = [m[j] << (bits - 1 - j) for j in range(bits)]
+    out = np.zeros(npts)
+    for n in range(1, npts + 1):
+        g = n ^ (n >> 1)
+        x, i = 0, 0
+        while g:
+            if g & 1:
+                x ^= V[i]
+            g >>= 1
+            i += 1
+        out[n - 1] = x / 2.0**bits
+    return out
+
+
+def test_sobol_sequence_is_the_textbook_generator():
+    """The reference's generator (random.c:234-290) is the Bratley-Fox Sobol
+    sequence of the polynomials x + 1, x^2 + x + 1 and, because its
+    recurrence XORs the previous integer for each polynomial bit
+    (random.c:265), x^3 + x^2 + 1 (code 1 names x^3 + x + 1 in the table it
+    comes from; that generator differs from the third point of dimension 3 on)."""
+    n = 1 << 12
+    r = orc.sobol_points(n)
+    assert r[0].tolist() == [0.5, 0.5, 0.5]
+    assert r[1].tolist() == [0.25, 0.75, 0.25]
+    assert np.array_equal(r[:, 0], _textbook_sobol(n, 1, [], [1]))
+    assert np.array_equal(r[:, 1], _textbook_sobol(n, 2, [1], [1, 1]))
+    assert np.array_equal(r[:, 2], _textbook_sobol(n, 3, [1, 0], [1, 3, 7]))
+    assert not np.array_equal(r[:, 2], _textbook_sobol(n, 3, [0, 1], [1, 3, 7]))
+
+
+def test_sobol_points_stratify_every_axis():
+    """With x_0 = 0 before them, the first 2^m - 1 calls fill each of the 2^m
+    dyadic intervals of every axis exactly once ((0, m, 1)-nets)."""
+    r = orc.sobol_points((1 << 14) - 1)
+    for m in range(1, 15):
+        for k in range(3):
+            pts = np.concatenate([[0.0], r[:(1 << m) - 1, k]])
+            assert np.array_equal(np.sort(np.floor(pts * (1 << m))), np.arange(1 << m)), (m, k)
+
+
+def test_sobol_needs_at_most_30_bits():
+    # call 2^30 - 1 would need the 31st direction integer (random.c:279-281)
+    V = orc.sobol_dirs()
+    assert V.shape == (30, 3) and np.all(V[0] == 1 << 29)
+    # the leading 1 of integer j sits at bit 29 - j, so each is odd once scaled back
+    for j in range(30):
+        assert np.all((V[j] >> (29 - j)) & 1 == 1)
+
+
+def test_flat_sobol_placement():
+    g = orc.Grid((4, 5, 6), p1=(0.2, 0.25, 0.3))
+    n = 3000
+    p = orc.init_density(g, orc.DENSITY_FLAT_SOBOL, n, 7, (0.0, 0.0, 0.0), (100.0, 100.0, 100.0))
+    r = orc.sobol_points(n)
+    for a, x in enumerate((p.x, p.y, p.z)):
+        v = r[:, a] * (g.L[a] + g.d[a]) - 0.5 * g.d[a]
+        v = np.where(v < 0, 0.01 * g.d[a], np.where(v > g.p1[a], g.p1[a] - 0.01 * g.d[a], v))
+        assert np.array_equal(x, v), a
+        assert np.all((x >= g.p0[a]) & (x < g.p1[a]))
+    assert np.array_equal(p.pid, np.arange(n))
+    # the pulled-in points pile up 0.01 d inside both faces
+    assert np.sum(p.x == 0.01 * g.d[0]) > 0 and np.sum(p.x == g.p1[0] - 0.01 * g.d[0]) > 0
+
+
+def test_uniform_placement_fills_all_but_the_last_cell_of_each_axis():
+    g = orc.Grid((5, 4, 6), p1=(0.25, 0.2, 0.3))
+    Np = 3 * g.nc + 7
+    p = orc.init_density(g, orc.DENSITY_UNIFORM, Np, 1, (0.0, 0.0, 0.0), (100.0, 100.0, 100.0))
+    assert p.np == 3 * 4 * 3 * 5
+    key = orc.cell_keys(g, p)
+    cnt = np.bincount(key, minlength=g.nc).reshape(g.shape())  # [z][y][x]
+    assert np.all(cnt[:5, :3, :4] == 3) and cnt.sum() == p.np
+    # cell centres (i + 1/2) d
+    for a, x in enumerate((p.x, p.y, p.z)):
+        idx = np.rint(x / g.d[a] - 0.5)
+        assert np.allclose(x, (idx + 0.5) * g.d[a], rtol=0, atol=1e-15)
+    # k (z index) fastest, then j, then i; ion blocks repeat per ion slot
+    assert p.z[1] > p.z[0] and p.y[1] == p.y[0] and p.x[1] == p.x[0]
+    assert np.array_equal(p.x[:60], p.x[60:120])
+
+
+def test_sinusoidal_placement_follows_its_density():
+    """Rejection against 1 + 0.25 sin(2 pi x) (distributions.c:152-162) over
+    x in [0, 1.6): chi-square of the x histogram against the integral of the
+    density per bin, and uniform y and z."""
+    from scipy import stats
+    g = orc.Grid((32, 8, 8), p1=(1.6, 0.4, 0.4))
+    n = 200000
+    p = orc.init_density(g, orc.DENSITY_SINUSOIDAL, n, 5, (0.0, 0.0, 0.0), (100.0, 100.0, 100.0))
+    edges = np.linspace(0.0, 1.6, 33)
+    F = edges - 0.25 / (2 * np.pi) * np.cos(2 * np.pi * edges)
+    expect = np.diff(F) / (F[-1] - F[0]) * n
+    obs, _ = np.histogram(p.x, edges)
+    chi2 = np.sum((obs - expect) ** 2 / expect)
+    assert stats.chi2.sf(chi2, 31) > 1e-4, chi2
+    for x, L in ((p.y, 0.4), (p.z, 0.4)):
+        o, _ = np.histogram(x, np.linspace(0, L, 17))
+        assert stats.chi2.sf(np.sum((o - n / 16) ** 2 / (n / 16)), 15) > 1e-4
+    # a flat density would fail the same test
+    flat = np.full(32, n / 32)
+    assert stats.chi2.sf(np.sum((obs - flat) ** 2 / flat), 31) < 1e-10

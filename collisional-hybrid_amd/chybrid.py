@@ -21,6 +21,8 @@ ERR_FP, ERR_ARG_INCOMP, ERR_LIB, ERR_NOT_CONVERGED = -72, -75, -76, -91
 
 LHS_IDENTITY, LHS_LAPLACIAN, LHS_FULL = 0, 1, 2
 RHS_CONSTANT, RHS_SINUSOIDAL, RHS_FULL = 0, 1, 2
+(DENSITY_FLAT_SOBOL, DENSITY_FLAT_REVERSE, DENSITY_FLAT_NORMAL, DENSITY_UNIFORM, DENSITY_SINUSOIDAL,
+ DENSITY_GAUSSIAN, DENSITY_LATTICE) = range(7)
 PREC_FP64, PREC_MIXED = 0, 1
 KSP_GMRES, KSP_BCGS = 0, 1
 PC_NONE, PC_JACOBI, PC_FFT, PC_ZLINE, PC_CHEBY, PC_FFT2, PC_FFT2Z = 0, 1, 2, 3, 4, 5, 6
@@ -56,6 +58,7 @@ class Params(C.Structure):
         ("perturbation_amplitude", C.c_double), ("weighted", C.c_int32),
         ("precision", C.c_int32), ("output_interval", C.c_int64), ("collision_refusal_limit", C.c_int64),
         ("flux_scale", D3), ("input", C.c_char * 4096), ("output", C.c_char * 4096),
+        ("diagnostics_interval", C.c_int64),
     ]
 
 
@@ -68,9 +71,14 @@ class SolveStats(C.Structure):
                 ("converged", C.c_int32), ("reason", C.c_int32), ("ms", C.c_double)]
 
 
+class Diagnostics(C.Structure):
+    _fields_ = [("ions", C.c_int64), ("step", C.c_int64), ("weight", C.c_double), ("momentum", D3),
+                ("kinetic_energy", C.c_double), ("field_energy", C.c_double), ("valid", C.c_int32)]
+
+
 class StepStats(C.Structure):
     _fields_ = [("collisions", CollisionStats), ("solve", SolveStats),
-                ("ms_particles", C.c_double), ("ms_total", C.c_double)]
+                ("ms_particles", C.c_double), ("ms_total", C.c_double), ("diag", Diagnostics)]
 
 
 # every entry point of include/chybrid.h with its ctypes signature
@@ -118,6 +126,7 @@ SIGNATURES = {
     "chy_pc_apply": (C.c_int, [P, P, P]),
     "chy_timing_select": (C.c_int, [P, C.c_char_p]),
     "chy_adjoint_backward": (C.c_int, [P, P, P]),
+    "chy_diagnostics": (C.c_int, [P, P]),
 }
 
 _LIB = None
@@ -336,6 +345,12 @@ class Context:
         st = StepStats()
         self._chk(lib().chy_step(self._p, C.byref(st)))
         return st
+
+    def diagnostics(self):
+        """FP64 device sums: ions, weight, momentum, kinetic and field energy."""
+        d = Diagnostics()
+        self._chk(lib().chy_diagnostics(self._p, C.byref(d)))
+        return d
 
     def sync(self):
         self._chk(lib().chy_sync(self._p))

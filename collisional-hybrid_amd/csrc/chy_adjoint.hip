@@ -403,6 +403,13 @@ int moments_adjoint(chy_ctx *ctx, Tape *t, const Stage &A, bool final, const dou
     const int rc = ctx_solve(ctx, &st);
     ctx->solve_transpose = false;
     if (rc) return rc;
+    // an unconverged mu would enter every later adjoint silently
+    if (!st.converged) {
+      char m[160];
+      snprintf(m, sizeof m, "adjoint: the transposed potential solve did not converge (relative residual %.3e after "
+               "%lld iterations)", st.residual, (long long)st.iterations);
+      return set_err(ctx, CHY_ERR_NOT_CONVERGED, m);
+    }
     const double *mu = ctx->phi;
     // phi depends on n through A(n): lambda_n -= d(mu^T A(n) phi_A)/dn
     launch_lhs_dn(ctx, mu, A.phi, -1.0, t->ln);

@@ -579,6 +579,8 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
     return CHY_ERR_MEM;
   }
   A(dalloc(ctx, &ctx->gs_scale, ctx->restart + 2));
+  A(dalloc(ctx, &ctx->diag_part, diag_elems()));
+  ctx->diag_out = ctx->diag_part + diag_parts();
   A(fft_setup(ctx));
   if ((p.pc_type == CHY_PC_FFT2 || p.pc_type == CHY_PC_FFT2Z) && ctx->fft_ok) {
     unsigned char *q = nullptr;
@@ -620,7 +622,7 @@ int chy_destroy(chy_ctx *ctx)
                 ctx->partials,
                 ctx->sred, ctx->S_dev, ctx->cstats, ctx->mom[0], ctx->mom[1], ctx->mom[2], ctx->mom[3],
                 ctx->efield[0], ctx->phi, ctx->phi_prev, ctx->b, ctx->r64, ctx->t64, ctx->V,
-                ctx->Z, ctx->wv, ctx->fft_buf, ctx->pc_buf, ctx->pc2_buf, ctx->pc2_h, ctx->zl_buf, ctx->zl_col, ctx->red, ctx->rpl, ctx->red_out, ctx->gs_scale, ctx->tw[0], ctx->tw[1], ctx->tw[2],
+                ctx->Z, ctx->wv, ctx->fft_buf, ctx->pc_buf, ctx->pc2_buf, ctx->pc2_h, ctx->zl_buf, ctx->zl_col, ctx->red, ctx->rpl, ctx->red_out, ctx->gs_scale, ctx->diag_part, ctx->tw[0], ctx->tw[1], ctx->tw[2],
                 ctx->trig[0], ctx->trig[1], ctx->trig[2]};
   for (void *p : ps) if (p) (void)hipFree(p);
   if (ctx->h_red) (void)hipHostFree(ctx->h_red);
@@ -811,6 +813,14 @@ int chy_solve(chy_ctx *ctx, const double *b, double *phi, chy_solve_stats *st)
   const int64_t nc = ctx->g.nc;
   CHY_HIP(hipMemcpy(ctx->b, b, sizeof(double) * nc, hipMemcpyHostToDevice));
   CHY_HIP(hipMemcpy(ctx->phi, phi, sizeof(double) * nc, hipMemcpyHostToDevice));
+  // The full and Laplacian operators carry the constant null space
+  // (lhs.c:334-335,530-531), which the solve projects out of b as PETSc's KSP
+  // does for a matrix with an attached null space; a b with a non-zero mean
+  // would otherwise be inconsistent
+  if (ctx->prm.lhs_type != CHY_LHS_IDENTITY) {
+    launch_sum<double>(ctx, ctx->b, nc, ctx->red_out);
+    launch_shift<double>(ctx, nc, ctx->b, ctx->red_out, -1.0 / (double)nc);
+  }
   ctx->phi_valid = true;
   int rc = do_solve(ctx, st);
   if (rc) return rc;
@@ -993,6 +1003,9 @@ int chy::ctx_step(chy_ctx *ctx, chy_step_stats *st)
   local.solve.ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
   local.ms_particles = std::chrono::duration<double, std::milli>(t1 - t0).count();
   local.ms_total = std::chrono::duration<double, std::milli>(t2 - t0).count();
+  // global diagnostics of the new state, outside the step's timings
+  const int64_t di = ctx->prm.diagnostics_interval;
+  if (di > 0 && ctx->step % di == 0) TRY(diagnostics(ctx, &local.diag));
   if (st) *st = local;
   return CHY_OK;
 }

@@ -151,6 +151,8 @@ struct chy_ctx {
   std::vector<std::string> timing_only;  // when not empty, the only timers recorded
   std::map<std::string, chy::Timer> timers;
 
+  double *diag_part = nullptr;  // diagnostics: block partials, then the sums (diag_out)
+  double *diag_out = nullptr;
   bool solve_transpose = false; // the Krylov solve runs on A(n)^T (adjoint)
   void *adj = nullptr;          // adjoint tape (chy_adjoint.hip)
 };
@@ -318,6 +320,11 @@ int particles_normaliser(chy_ctx *ctx);
 // deposit of an arbitrary cell-sorted FP64 state (segments `offs`) into
 // out[4] = {sum w W, sum w W vx, sum w W vy, sum w W vz}
 void deposit_state(chy_ctx *ctx, const PState<double> &st, const uint32_t *offs, bool weighted, double *const out[4]);
+
+// ---- diagnostics (chy_diag.hip) ----
+int diagnostics(chy_ctx *ctx, chy_diag_stats *d);
+int64_t diag_elems();  // device doubles for diag_part + diag_out
+int diag_parts();      // of which block partials (diag_out follows them)
 
 // ---- context-level operations (chy_ctx.hip) ----
 int ctx_step(chy_ctx *ctx, chy_step_stats *st);  // one step of the loop body

@@ -43,8 +43,10 @@ extern "C" {
 /* ---- enumerations (reference lhs.h:8-12, rhs.h:8-12, hybrid.h:84-102) ---- */
 enum { CHY_LHS_IDENTITY = 0, CHY_LHS_LAPLACIAN = 1, CHY_LHS_FULL = 2 };
 enum { CHY_RHS_CONSTANT = 0, CHY_RHS_SINUSOIDAL = 1, CHY_RHS_FULL = 2 };
+/* --density-type (parameters.c:14-16); CHY_DENSITY_LATTICE is a build
+ * addition: ppc ions per cell at seeded uniform offsets, selected by -ppc */
 enum { CHY_DENSITY_FLAT_SOBOL = 0, CHY_DENSITY_FLAT_REVERSE, CHY_DENSITY_FLAT_NORMAL,
-       CHY_DENSITY_UNIFORM, CHY_DENSITY_SINUSOIDAL, CHY_DENSITY_GAUSSIAN };
+       CHY_DENSITY_UNIFORM, CHY_DENSITY_SINUSOIDAL, CHY_DENSITY_GAUSSIAN, CHY_DENSITY_LATTICE };
 enum { CHY_BC_PERIODIC = 0, CHY_BC_INJECTION, CHY_BC_REFLECTION, CHY_BC_DIRICHLET, CHY_BC_NEUMANN };
 /* build additions */
 enum { CHY_PREC_FP64 = 0, CHY_PREC_MIXED = 1 };
@@ -110,6 +112,8 @@ typedef struct {
   double  flux_scale[3];
   char    input[4096];
   char    output[4096];
+  /* build addition: chy_step fills its diagnostics every this many steps (0 = never) */
+  int64_t diagnostics_interval;
 } chy_params;
 
 typedef struct chy_ctx chy_ctx;
@@ -130,11 +134,23 @@ typedef struct {
   double  ms;            /* wall time of the solve (RHS + Krylov + projections) */
 } chy_solve_stats;
 
+/* Global diagnostics, summed in FP64 on the device (chy_diagnostics). */
+typedef struct {
+  int64_t ions;            /* live ions */
+  int64_t step;            /* collision step counter when taken */
+  double  weight;          /* sum of the ion weights */
+  double  momentum[3];     /* m_i sum w v  [kg m/s] */
+  double  kinetic_energy;  /* 1/2 m_i sum w |v|^2  [J] */
+  double  field_energy;    /* 1/2 eps0 sum |grad phi|^2 dV over the nodes  [J] */
+  int32_t valid;           /* 1 when filled */
+} chy_diag_stats;
+
 typedef struct {
   chy_collision_stats collisions;
   chy_solve_stats     solve;
   double              ms_particles;  /* push + sort + deposit */
   double              ms_total;
+  chy_diag_stats      diag;          /* after the step, when -diagnostics_interval selects it */
 } chy_step_stats;
 
 /* ---- parameters ----------------------------------------------------------
@@ -172,9 +188,23 @@ int chy_set_particles(chy_ctx *ctx, int64_t np, const double *pos, const double 
                       const double *w, const uint32_t *pid);
 /* Any output may be NULL.  Order is the current cell-sorted order. */
 int chy_get_particles(chy_ctx *ctx, double *pos, double *vel, double *w, uint32_t *pid);
-/* Seeded placement on the device: ppc per cell, offsets U[0,1)^3, velocities
- * vi0 + viT N(0,1); weights from prm perturbation (replaces the once-per-run
- * InitializePositions / InitializeVelocities, particles.c:9-132). */
+/* Seeded placement on the device, replacing the once-per-run
+ * InitializePositions / InitializeVelocities (particles.c:9-132), by
+ * prm->density_type:
+ *   flat-sobol (default)  Np ions at the points of the reference's Sobol
+ *                         sequence, v = r (L + d) - d/2 pulled inside the
+ *                         domain (distributions.c:95-141; at most 2^30 - 1);
+ *   uniform               Np / nc ions at the centre of every cell except the
+ *                         last along each axis (distributions.c:33-92): the
+ *                         ion count becomes Np/nc (Nx-1)(Ny-1)(Nz-1);
+ *   sinusoidal            rejection sampling of 1 + 0.25 sin(2 pi x)
+ *                         (distributions.c:144-232), against its maximum;
+ *   lattice (-ppc P)      P ions per cell at uniform offsets (build addition);
+ *   flat-normal, flat-reverse, gaussian: CHY_ERR_ARG_WRONG "Not implemented",
+ *                         as particles.c:28-45.
+ * Velocities vi0 + viT N(0,1) and the weight perturbation come from Philox
+ * streams keyed by the seed (the reference's gasdev stream is not
+ * reproduced); weights from prm perturbation. */
 int chy_init_particles(chy_ctx *ctx);
 
 /* ---- the hot path --------------------------------------------------------
@@ -200,7 +230,10 @@ int chy_lhs_apply_transpose(chy_ctx *ctx, const double *x, double *y);
  * Krylov solve warm-started from the previous phi (zero on the first call,
  * potential.c:20-29).  phi may be NULL. */
 int chy_solve_potential(chy_ctx *ctx, double *phi, chy_solve_stats *stats);
-/* Solve A(n) phi = b for a given b (mean removed); phi in/out. */
+/* Solve A(n) phi = b for a given b; phi in/out (the initial guess).  For the
+ * full and Laplacian operators, whose null space is the constants, the mean
+ * of b is removed first, as PETSc's KSP projects the right-hand side of a
+ * matrix with an attached null space (lhs.c:334-335,530-531). */
 int chy_solve(chy_ctx *ctx, const double *b, double *phi, chy_solve_stats *stats);
 /* KSPSetTolerances (the KSP that potential.c:6-16 solves with): relative and
  * absolute tolerance and iteration limit of the following solves; a negative
@@ -223,6 +256,14 @@ int chy_pc_apply(chy_ctx *ctx, const double *r, double *z);
 /* One time step of the loop body hybrid.c:108-117 (velocities, positions,
  * moments, potential), fused on the device. */
 int chy_step(chy_ctx *ctx, chy_step_stats *stats);
+
+/* Global diagnostics of the current state (no reference counterpart; the
+ * reference reports collision counts only, particles.c:525,620): ion count,
+ * total weight, momentum and kinetic energy of the ions, and the energy of
+ * the field of the current potential (periodic central differences, the node
+ * field the push gathers, without the imposed uniform E0), each summed in
+ * FP64 over a fixed grid, so repeated calls agree bitwise. */
+int chy_diagnostics(chy_ctx *ctx, chy_diag_stats *d);
 
 /* ---- adjoint of the deterministic step map --------------------------------
  * No reference counterpart: the reference has no derivatives.  The forward

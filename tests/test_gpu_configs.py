@@ -1,0 +1,186 @@
+"""The five workloads of the benchmark definition on the device, each at its
+full size:
+
+cfg0  32^3 x 8 ions/cell (262,144), FP64, sinusoidal-x weights, collisions
+      on, all 100 steps; every step compared with one oracle step from the
+      same state and potential: the stable cell order bitwise, trajectories,
+      moments and the accepted collisions exactly the oracle's, counts
+      within 3 sigma of the binomial expectation, and the new potential
+      certified by the oracle's own operator and forcing.
+cfg1  128^3 x 32 ions/cell (67.1 M), FP64, collisions off (-mn 0), 200 steps;
+      one-step oracle comparisons at steps 0, 1, 100 and 199 (the cell keys
+      and order bitwise, trajectories to 1e-12), with the number of ions the
+      two sides put in different cells reported.
+cfg4  512^3 x 8 ions/cell (1.07 B), mixed precision, nu_in 5e3 s^-1,
+      E0x 0.08 V/m, seed 7: the refusal rule of particles.c:597-605 (a step
+      with more than 10 refused scatterings aborts with PETSC_ERR_SIG), then
+      3 steps with the rule off, checked by properties that hold at any size.
+The 256^3 workload (cfg2) is tests/test_gpu_fullsize.py; the 64^3
+differentiation workload (cfg3) is tests/test_gpu_adjoint.py."""
+import numpy as np
+import pytest
+
+import chybrid
+import orc
+from gpu_cases import as_particles, grid_of, push_params_of, relmax
+
+pytestmark = pytest.mark.gpu
+
+PHYS = ("-mi 4.9816e-26 -mn 4.8156e-26 -Ti 300 -Tn 300 -B0z 5e-5 -E0x 0.05 -nui 1e3 -nue 1e5 -dt 1e-5 ")
+
+
+def cfg_args(N, ppc, extra):
+    L = 0.05 * N
+    return (f"-Nx {N} -Ny {N} -Nz {N} -x1 {L} -y1 {L} -z1 {L} -ppc {ppc} " + PHYS + extra).split()
+
+
+def _oracle_K(prm):
+    return list(prm.kappa_e), prm.gamma_e * orc.KB * prm.Te / orc.Q, prm.me * prm.nue / orc.Q
+
+
+def _normaliser(p, prm):
+    return float(sum(orc.mean_rel_speed((p.vx[s], p.vy[s], p.vz[s]), list(prm.vn0), prm.vnT) for s in range(p.np)))
+
+
+def _compare_step(ctx, prm, g, it, S, collide, rtol):
+    """One device step against one oracle step from the same state and phi
+    (the potential solved to rtol).  Returns (next S, ions placed in
+    different cells, step stats)."""
+    before = as_particles(ctx.get_particles())
+    phi = ctx.get_potential()
+    _, P = push_params_of(prm, step=it, collide=collide)
+    st = ctx.step()
+    got = as_particles(ctx.get_particles())
+    mom = ctx.get_moments()
+    nxt, key, ref_mom, Sn, stats = orc.step(g, P, before, phi, S, list(prm.E0))
+    moved = int(np.count_nonzero(orc.cell_keys(g, got) != key))
+    assert np.array_equal(got.pid, nxt.pid), f"step {it}: cell order differs from the oracle's stable sort"
+    for a in ("x", "y", "z", "vx", "vy", "vz"):
+        assert relmax(getattr(got, a), getattr(nxt, a)) <= 1e-12, (it, a)
+    for m in range(4):
+        assert relmax(mom[:, m], ref_mom[m]) <= 1e-12, (it, m)
+    if collide:
+        assert st.collisions.accepted == stats[0] and st.collisions.refused == stats[1], (it, st.collisions.accepted,
+                                                                                          stats[:2])
+    # the new potential, certified with the oracle's operator and forcing
+    K, cth, cG = _oracle_K(prm)
+    b = orc.rhs(g, K, list(prm.E0), cth, cG, *ref_mom)
+    phin = ctx.get_potential()
+    r = b - orc.lhs_apply(g, K, ref_mom[0], phin)
+    assert st.solve.converged == 1, (it, st.solve.residual, st.solve.iterations)
+    assert np.linalg.norm(r) <= 2 * rtol * np.linalg.norm(b), (it, np.linalg.norm(r) / np.linalg.norm(b))
+    return Sn, moved, st
+
+
+@pytest.mark.timeout(900)
+def test_cfg0_every_step_matches_the_oracle():
+    args = cfg_args(32, 8, "-Nt 100 -seed 0 --perturbation sinusoidal-x --perturbation-amplitude 0.01 "
+                           "-precision fp64 -ksp_rtol 1e-12")
+    prm, _ = chybrid.params_from_options(args)
+    g = grid_of(prm)
+    print("cfg0: 32^3 x 8 ions/cell, FP64, sinusoidal-x, collisions on, 100 steps", flush=True)
+    counts, targets = [], []
+    with chybrid.Context(prm) as ctx:
+        ctx.init_particles()
+        ctx.collect_moments(download=False)
+        ctx.solve_potential()
+        S = _normaliser(as_particles(ctx.get_particles()), prm)
+        moved_total = 0
+        for it in range(100):
+            S, moved, st = _compare_step(ctx, prm, g, it, S, True, 1e-12)
+            moved_total += moved
+            counts.append(st.collisions.accepted + st.collisions.refused)
+            targets.append(st.collisions.Nc)
+            if it == 50:
+                # and once a full oracle solve of the same system
+                K, cth, cG = _oracle_K(prm)
+                mom = ctx.get_moments()
+                b = orc.rhs(g, K, list(prm.E0), cth, cG, *[mom[:, m] for m in range(4)])
+                phi_o, _, conv, _ = orc.solve(g, K, mom[:, 0], b, rtol=1e-12)
+                assert conv and relmax(ctx.get_potential(), phi_o) <= 1e-8
+        assert ctx.num_particles() == 262144
+    assert moved_total == 0
+    Np = 262144
+    counts, targets = np.array(counts), np.array(targets)
+    assert np.all(targets == targets[0]) and targets[0] == 6053   # particles.c:514 at these parameters
+    sig = np.sqrt(targets * (1 - targets / Np))
+    z = (counts - targets) / sig
+    print(f"cfg0 collision counts: mean {counts.mean():.1f} (target {targets[0]}), |z| max {np.abs(z).max():.2f}",
+          flush=True)
+    assert np.all(np.abs(z) <= 3.0), z
+    assert abs(z.mean()) * np.sqrt(len(z)) <= 3.0
+
+
+@pytest.mark.timeout(900)
+def test_cfg1_collisionless_trajectories_match_the_oracle_over_200_steps():
+    # 1e-11: at 128^3 the FP64 residual of the 19-term sum can stall just
+    # above 1e-12 of |b|; the trajectory comparisons take the device's phi as
+    # the oracle's input, so they do not depend on it
+    args = cfg_args(128, 32, "-Nt 200 -seed 0 --perturbation uniform --perturbation-amplitude 0.01 "
+                             "-precision fp64 -ksp_rtol 1e-11 -mn 0")
+    prm, _ = chybrid.params_from_options(args)
+    g = grid_of(prm)
+    print("cfg1: 128^3 x 32 ions/cell (67.1 M), FP64, collisions off, 200 steps", flush=True)
+    disagree = {}
+    with chybrid.Context(prm) as ctx:
+        ctx.init_particles()
+        ctx.collect_moments(download=False)
+        ctx.solve_potential()
+        for it in range(200):
+            if it in (0, 1, 100, 199):
+                _, moved, st = _compare_step(ctx, prm, g, it, 1.0, False, 1e-11)
+                disagree[it] = moved
+                print(f"cfg1 step {it}: oracle comparison done, {moved} ions in other cells", flush=True)
+            else:
+                st = ctx.step()
+                assert st.solve.converged == 1 and st.solve.residual <= 1e-11, (it, st.solve.residual,
+                                                                                st.solve.iterations)
+        assert ctx.num_particles() == 128 ** 3 * 32
+    assert all(v == 0 for v in disagree.values()), disagree
+
+
+@pytest.mark.timeout(900)
+def test_cfg4_scale_stress_refusal_rule_and_properties():
+    args = cfg_args(512, 8, "-Nt 100 -seed 7 -nui 5e3 -E0x 0.08 --perturbation uniform "
+                            "--perturbation-amplitude 0.01 -precision mixed -ksp_rtol 1e-10")
+    # the reference aborts a step at its 10th refused scattering (final speed
+    # above 10 v_T, particles.c:597-605); at 1.07 B ions and nu_in = 5e3 a step
+    # holds about 70 of them among 124 M scatterings
+    prm, _ = chybrid.params_from_options(args)
+    print("cfg4: 512^3 x 8 ions/cell (1.07 B), mixed, refusal limit 10", flush=True)
+    with chybrid.Context(prm) as ctx:
+        ctx.init_particles()
+        ctx.collect_moments(download=False)
+        ctx.solve_potential()
+        with pytest.raises(chybrid.ChyError) as e:
+            ctx.step()
+        assert e.value.code == chybrid.ERR_SIG
+    prm, _ = chybrid.params_from_options(args + ["--collision-refusal-limit", "0"])
+    print("cfg4: 3 steps with the refusal rule off", flush=True)
+    with chybrid.Context(prm) as ctx:
+        ctx.init_particles()
+        ctx.collect_moments(download=False)
+        ctx.solve_potential()
+        np_ = ctx.num_particles()
+        assert np_ == 512 ** 3 * 8
+        w0 = ctx.diagnostics().weight
+        n0 = ctx.get_moments()[:, 0].sum()
+        assert abs(n0 - w0) <= 1e-9 * w0
+        for it in range(3):
+            st = ctx.step()
+            assert st.solve.converged == 1 and st.solve.residual <= 1e-10, (it, st.solve.residual)
+            c = st.collisions
+            q = c.Nc / np_
+            assert c.Nc > 0 and abs(c.accepted + c.refused - c.Nc) <= 3 * np.sqrt(c.Nc * (1 - q)), (it, c.accepted,
+                                                                                                 c.Nc)
+            assert 0 < c.refused <= 1e-5 * c.Nc
+            print(f"cfg4 step {it}: {st.solve.iterations} its, {c.accepted} scatterings, {c.refused} refused",
+                  flush=True)
+        assert ctx.num_particles() == np_
+        d = ctx.diagnostics()
+        n = ctx.get_moments()[:, 0].sum()
+        phi = ctx.get_potential()
+    # weights are carried unchanged, and every ion is deposited once
+    assert abs(d.weight - w0) <= 1e-12 * w0
+    assert abs(n - w0) <= 1e-9 * w0
+    assert np.isfinite(phi).all() and abs(phi.mean()) <= 1e-12 * np.abs(phi).max()

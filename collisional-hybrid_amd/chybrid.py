@@ -25,7 +25,7 @@ RHS_CONSTANT, RHS_SINUSOIDAL, RHS_FULL = 0, 1, 2
  DENSITY_GAUSSIAN, DENSITY_LATTICE) = range(7)
 PREC_FP64, PREC_MIXED = 0, 1
 KSP_GMRES, KSP_BCGS = 0, 1
-PC_NONE, PC_JACOBI, PC_FFT, PC_ZLINE, PC_CHEBY, PC_FFT2, PC_FFT2Z = 0, 1, 2, 3, 4, 5, 6
+PC_NONE, PC_JACOBI, PC_FFT, PC_ZLINE, PC_CHEBY, PC_FFT2, PC_FFT2Z, PC_MG = 0, 1, 2, 3, 4, 5, 6, 7
 CGS_REFINE_NEVER, CGS_REFINE_IFNEEDED, CGS_REFINE_ALWAYS = 0, 1, 2
 GUESS_PREVIOUS, GUESS_ZERO, GUESS_EXTRAPOLATE = 0, 1, 2
 PERT_NONE, PERT_SINUSOIDAL_X, PERT_UNIFORM, PERT_GAUSSIAN = 0, 1, 2, 3
@@ -63,7 +63,8 @@ class Params(C.Structure):
 
 
 class CollisionStats(C.Structure):
-    _fields_ = [("accepted", C.c_int64), ("refused", C.c_int64), ("capped", C.c_int64), ("Nc", C.c_int64)]
+    _fields_ = [("accepted", C.c_int64), ("refused", C.c_int64), ("capped", C.c_int64), ("Nc", C.c_int64),
+                ("expected", C.c_double), ("variance", C.c_double)]
 
 
 class SolveStats(C.Structure):

@@ -95,7 +95,14 @@ int pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
   const int pc = ctx->prm.pc_type;
   T *sb = (T *)ctx->pc_buf;
   const bool fftlike = pc == CHY_PC_FFT || pc == CHY_PC_FFT2 || pc == CHY_PC_FFT2Z;
-  if (fftlike && ctx->fft_ok && ctx->prm.lhs_type != CHY_LHS_IDENTITY) {
+  if (pc == CHY_PC_MG && ctx->prm.lhs_type != CHY_LHS_IDENTITY && !ctx->solve_transpose) {
+    rc = mg_apply<T>(ctx, r, z);
+  } else if (pc == CHY_PC_MG && ctx->solve_transpose) {
+    // the z-line smoother is built for A(n); transposed solves (the adjoint)
+    // take the FFT preconditioner of the mean density, symmetric, or Jacobi
+    if (ctx->fft_ok) rc = fft_pc_apply<T>(ctx, v19, r, z);
+    else launch_jacobi<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), r, z);
+  } else if (fftlike && ctx->fft_ok && ctx->prm.lhs_type != CHY_LHS_IDENTITY) {
     if (pc == CHY_PC_FFT) {
       rc = fft_pc_apply<T>(ctx, v19, r, z);
     } else {
@@ -372,6 +379,8 @@ int pc_symbol(chy_ctx *ctx, double v19[19], double *bb = nullptr)
   // both from one walk up the columns)
   const bool coef = two && ctx->fft_ok && ctx->pc2_buf;
   const bool fact = ctx->prm.pc_type == CHY_PC_FFT2Z && ctx->fft_ok && ctx->zl_buf;
+  if (ctx->prm.pc_type == CHY_PC_MG && ctx->mg && ctx->prm.lhs_type != CHY_LHS_IDENTITY)
+    TRY(mg_prepare<T>(ctx, mom<T>(ctx, 0)));
   if (ctx->prm.lhs_type == CHY_LHS_FULL && (coef || fact)) {
     launch_zcolumn<T>(ctx, mom<T>(ctx, 0), coef ? (T *)ctx->pc2_buf : nullptr, fact ? (T *)ctx->zl_buf : nullptr,
                       fact ? ctx->zl_col : nullptr);
@@ -554,7 +563,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   }
   A(dalloc(ctx, &ctx->sred, std::max<int64_t>(4096, nc / 32 + 1)));  // normaliser partials, one per block
   A(dalloc(ctx, &ctx->S_dev, 1));
-  A(dalloc(ctx, &ctx->cstats, 4));
+  A(dalloc(ctx, &ctx->cstats, 6));
   for (int a = 0; a < 4; a++) { unsigned char *q = nullptr; A(dalloc(ctx, &q, (int64_t)ts * nc)); ctx->mom[a] = q; }
   { unsigned char *q = nullptr; A(dalloc(ctx, &q, (int64_t)ts * 4 * nc)); ctx->efield[0] = q; }
   A(dalloc(ctx, &ctx->phi, nc));
@@ -579,6 +588,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
     return CHY_ERR_MEM;
   }
   A(dalloc(ctx, &ctx->gs_scale, ctx->restart + 2));
+  if (p.pc_type == CHY_PC_MG) A(mg_setup(ctx));
   A(dalloc(ctx, &ctx->diag_part, diag_elems()));
   ctx->diag_out = ctx->diag_part + diag_parts();
   A(fft_setup(ctx));
@@ -616,6 +626,7 @@ int chy_destroy(chy_ctx *ctx)
   if (ctx->out_copied) (void)hipEventDestroy(ctx->out_copied);
   if (ctx->out_stream) (void)hipStreamDestroy(ctx->out_stream);
   adjoint_release(ctx);
+  mg_release(ctx);
   for (int s = 0; s < 2; s++) { free_state(ctx->pd[s]); free_state(ctx->pf[s]); }
   void *ps[] = {ctx->perm, ctx->slotkey, ctx->counts, ctx->offsets, ctx->cursor, ctx->scan_tmp, ctx->maxcount,
                 ctx->trans, ctx->info, ctx->dir, ctx->nbase, ctx->offs_alt, ctx->sortflag,

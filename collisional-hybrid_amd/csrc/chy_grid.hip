@@ -445,18 +445,26 @@ static inline bool kz_operator(int lhs_type, const SCoef &s)
   return lhs_type == CHY_LHS_FULL && s.sxz == 0.0 && s.syz == 0.0 && s.szx == 0.0 && s.szy == 0.0;
 }
 
-// One tiled stencil pass, the B-along-z instantiation when it applies.
+// One tiled stencil pass on grid g, the B-along-z instantiation when it applies.
+template <typename TN, typename TX, typename TY, typename C, bool DIFF, int OUT, typename TB = double, int TW = kTX>
+static void launch_tiled_on(chy_ctx *ctx, const Grid &g, const SCoef &s, const LapRow &L, int lhs_type, const TN *n,
+                            const TX *x, const TB *b, TY *y, double *pl)
+{
+  if (kz_operator(lhs_type, s))
+    hipLaunchKernelGGL((k_lhs_tiled<TN, TX, TY, C, DIFF, OUT, TB, TW, true>), tiled_grid(g, TW), kBlock, 0,
+                       ctx->stream, g, lhs_type, s, L, n, x, b, y, pl);
+  else
+    hipLaunchKernelGGL((k_lhs_tiled<TN, TX, TY, C, DIFF, OUT, TB, TW, false>), tiled_grid(g, TW), kBlock, 0,
+                       ctx->stream, g, lhs_type, s, L, n, x, b, y, pl);
+}
+
+// The same on the context's grid.
 template <typename TN, typename TX, typename TY, typename C, bool DIFF, int OUT, typename TB = double, int TW = kTX>
 static void launch_tiled(chy_ctx *ctx, int lhs_type, const TN *n, const TX *x, const TB *b, TY *y, double *pl)
 {
   const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
   const LapRow L = lap_row(ctx, lhs_type);
-  if (kz_operator(lhs_type, s))
-    hipLaunchKernelGGL((k_lhs_tiled<TN, TX, TY, C, DIFF, OUT, TB, TW, true>), tiled_grid(ctx->g, TW), kBlock, 0,
-                       ctx->stream, ctx->g, lhs_type, s, L, n, x, b, y, pl);
-  else
-    hipLaunchKernelGGL((k_lhs_tiled<TN, TX, TY, C, DIFF, OUT, TB, TW, false>), tiled_grid(ctx->g, TW), kBlock, 0,
-                       ctx->stream, ctx->g, lhs_type, s, L, n, x, b, y, pl);
+  launch_tiled_on<TN, TX, TY, C, DIFF, OUT, TB, TW>(ctx, ctx->g, s, L, lhs_type, n, x, b, y, pl);
 }
 
 template <typename TN, typename TX, typename TY>
@@ -1405,12 +1413,13 @@ __global__ __launch_bounds__(256) void k_zline_factor(Grid g, int lhs_type, SCoe
   cden[col] = (C)1 + qn + ag * qlast;
 }
 
-// z += the z-line solve of res (res is overwritten by the intermediate y)
+// z += scale times the z-line solve of res (res is overwritten by the
+// intermediate y)
 template <typename T>
 __global__ __launch_bounds__(256) void k_zline_apply(Grid g, const T *__restrict__ za, const T *__restrict__ zi,
                                                      const T *__restrict__ zc, const T *__restrict__ zq,
                                                      const double *__restrict__ cag, const double *__restrict__ cden,
-                                                     T *__restrict__ res, T *__restrict__ z)
+                                                     T *__restrict__ res, T *__restrict__ z, double scale)
 {
   constexpr int KC = 16;
   const int64_t plane = (int64_t)g.nx * g.ny;
@@ -1461,7 +1470,7 @@ __global__ __launch_bounds__(256) void k_zline_apply(Grid g, const T *__restrict
     }
 #pragma unroll
     for (int u = 0; u < KC; u++) {
-      if (k0 + u < m) z[col + plane * (k0 + u)] = (T)((double)zz[u] + (double)rr[u] - f * (double)qq[u]);
+      if (k0 + u < m) z[col + plane * (k0 + u)] = (T)((double)zz[u] + scale * ((double)rr[u] - f * (double)qq[u]));
     }
   }
 }
@@ -1586,9 +1595,41 @@ void launch_zline_factor(chy_ctx *ctx, int lhs_type, const T *n, T *zbuf, double
 template <typename T>
 void launch_zline_apply(chy_ctx *ctx, const T *zbuf, const double *colbuf, T *res, T *z)
 {
-  const int64_t nc = ctx->g.nc, plane = (int64_t)ctx->g.nx * ctx->g.ny;
-  hipLaunchKernelGGL((k_zline_apply<T>), nblocks(plane), kBlock, 0, ctx->stream, ctx->g, zbuf, zbuf + nc,
-                     zbuf + 2 * nc, zbuf + 3 * nc, colbuf, colbuf + plane, res, z);
+  launch_zline_apply_g<T>(ctx, ctx->g, zbuf, colbuf, res, z, 1.0);
+}
+
+template <typename T>
+void launch_zline_apply_g(chy_ctx *ctx, const Grid &g, const T *zbuf, const double *colbuf, T *res, T *z,
+                          double scale)
+{
+  const int64_t nc = g.nc, plane = (int64_t)g.nx * g.ny;
+  hipLaunchKernelGGL((k_zline_apply<T>), nblocks(plane), kBlock, 0, ctx->stream, g, zbuf, zbuf + nc,
+                     zbuf + 2 * nc, zbuf + 3 * nc, colbuf, colbuf + plane, res, z, scale);
+}
+
+template <typename T>
+void launch_zline_factor_g(chy_ctx *ctx, const Grid &g, const double d[3], int lhs_type, const T *n, T *zbuf,
+                           double *colbuf)
+{
+  const SCoef s = full_scoef(d, ctx->prm.kappa_e);
+  LapRow L;
+  uniform_row(lhs_type == CHY_LHS_FULL ? CHY_LHS_IDENTITY : lhs_type, d, ctx->prm.kappa_e, 0.0, L.v);
+  const int64_t nc = g.nc, plane = (int64_t)g.nx * g.ny;
+  hipLaunchKernelGGL((k_zline_factor<T>), nblocks(plane), kBlock, 0, ctx->stream, g, lhs_type, s, L, n, zbuf,
+                     zbuf + nc, zbuf + 2 * nc, zbuf + 3 * nc, colbuf, colbuf + plane);
+}
+
+// res = b - A(n) x on grid g with spacings d, precision T (multigrid levels)
+template <typename T>
+void launch_residual_g(chy_ctx *ctx, const Grid &g, const double d[3], int lhs_type, const T *n, const T *b,
+                       const T *x, T *res)
+{
+  const SCoef s = full_scoef(d, ctx->prm.kappa_e);
+  LapRow L;
+  uniform_row(lhs_type == CHY_LHS_FULL ? CHY_LHS_IDENTITY : lhs_type, d, ctx->prm.kappa_e, 0.0, L.v);
+  constexpr bool fp32 = sizeof(T) == 4;
+  using C = typename std::conditional<fp32, float, double>::type;
+  launch_tiled_on<T, T, T, C, fp32, 1, T>(ctx, g, s, L, lhs_type, n, x, b, res, (double *)nullptr);
 }
 
 // res = b - A(n) x in precision T
@@ -1612,6 +1653,18 @@ template void launch_zcolumn<double>(chy_ctx *, const double *, double *, double
 template void launch_zcolumn<float>(chy_ctx *, const float *, float *, float *, double *);
 template void launch_zline_factor<double>(chy_ctx *, int, const double *, double *, double *);
 template void launch_zline_factor<float>(chy_ctx *, int, const float *, float *, double *);
+template void launch_zline_apply_g<double>(chy_ctx *, const Grid &, const double *, const double *, double *,
+                                           double *, double);
+template void launch_zline_apply_g<float>(chy_ctx *, const Grid &, const float *, const double *, float *, float *,
+                                          double);
+template void launch_zline_factor_g<double>(chy_ctx *, const Grid &, const double *, int, const double *, double *,
+                                            double *);
+template void launch_zline_factor_g<float>(chy_ctx *, const Grid &, const double *, int, const float *, float *,
+                                           double *);
+template void launch_residual_g<double>(chy_ctx *, const Grid &, const double *, int, const double *, const double *,
+                                        const double *, double *);
+template void launch_residual_g<float>(chy_ctx *, const Grid &, const double *, int, const float *, const float *,
+                                       const float *, float *);
 template void launch_zline_apply<double>(chy_ctx *, const double *, const double *, double *, double *);
 template void launch_zline_apply<float>(chy_ctx *, const float *, const double *, float *, float *);
 template void launch_residual_t<double>(chy_ctx *, int, const double *, const double *, const double *, double *);

@@ -153,6 +153,7 @@ struct chy_ctx {
 
   double *diag_part = nullptr;  // diagnostics: block partials, then the sums (diag_out)
   double *diag_out = nullptr;
+  void *mg = nullptr;           // multigrid levels (chy_mg.hip), -pc_type mg
   bool solve_transpose = false; // the Krylov solve runs on A(n)^T (adjoint)
   void *adj = nullptr;          // adjoint tape (chy_adjoint.hip)
 };
@@ -326,6 +327,15 @@ int diagnostics(chy_ctx *ctx, chy_diag_stats *d);
 int64_t diag_elems();  // device doubles for diag_part + diag_out
 int diag_parts();      // of which block partials (diag_out follows them)
 
+// ---- multigrid preconditioner (chy_mg.hip) ----
+int mg_setup(chy_ctx *ctx);    // levels and their buffers (context creation)
+void mg_release(chy_ctx *ctx);
+int mg_levels(const chy_ctx *ctx);
+template <typename T>
+int mg_prepare(chy_ctx *ctx, const T *n);  // per solve: level densities, z-line factors
+template <typename T>
+int mg_apply(chy_ctx *ctx, const T *r, T *z);  // z = one V-cycle on r
+
 // ---- context-level operations (chy_ctx.hip) ----
 int ctx_step(chy_ctx *ctx, chy_step_stats *st);  // one step of the loop body
 int ctx_rhs(chy_ctx *ctx);                       // forcing from the current moments
@@ -357,6 +367,17 @@ template <typename T>
 void launch_zline_apply(chy_ctx *ctx, const T *zbuf, const double *colbuf, T *res, T *z);
 template <typename T>
 void launch_residual_t(chy_ctx *ctx, int lhs_type, const T *n, const T *b, const T *x, T *res);
+// the same pieces on an explicit grid g with spacings d (multigrid levels):
+// z-line factors, z += scale zline^-1 res, res = b - A(n) x
+template <typename T>
+void launch_zline_factor_g(chy_ctx *ctx, const Grid &g, const double d[3], int lhs_type, const T *n, T *zbuf,
+                           double *colbuf);
+template <typename T>
+void launch_zline_apply_g(chy_ctx *ctx, const Grid &g, const T *zbuf, const double *colbuf, T *res, T *z,
+                          double scale);
+template <typename T>
+void launch_residual_g(chy_ctx *ctx, const Grid &g, const double d[3], int lhs_type, const T *n, const T *b,
+                       const T *x, T *res);
 template <typename T>
 int fft2_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z, T *w3);
 

@@ -65,6 +65,23 @@ __device__ __forceinline__ void count_flags(unsigned long long *cs, int slot, bo
   if (m && lane_id() == 0) atomicAdd(&cs[slot], (unsigned long long)__popcll(m));
 }
 
+// The block's sums of the collision probabilities p and of p (1 - p) (the
+// mean and variance of its Bernoulli count), one FP64 atomic each per block.
+__device__ __forceinline__ void flush_collision_moments(double *cexp, double pm, double pv, double *sh)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    pm += __shfl_down(pm, off, 64);
+    pv += __shfl_down(pv, off, 64);
+  }
+  if (lane_id() == 0) { sh[2 * (threadIdx.x >> 6)] = pm; sh[2 * (threadIdx.x >> 6) + 1] = pv; }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const int q = threadIdx.x;
+    atomicAdd(&cexp[q], (sh[q] + sh[2 + q]) + (sh[4 + q] + sh[6 + q]));
+  }
+}
+
 // Accepted / refused scatterings of the block, one global atomic per block.
 __device__ __forceinline__ void flush_scatter_counts(unsigned long long *cs, uint32_t nacc, uint32_t nref,
                                                      unsigned int *sacc)
@@ -90,9 +107,12 @@ __device__ __forceinline__ void flush_scatter_counts(unsigned long long *cs, uin
 template <bool COLLIDE, bool MOVE>
 __global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t np, PState<double> st,
                                                    const Vec4<double> *__restrict__ E4, const double *__restrict__ Sdev,
-                                                   uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats)
+                                                   uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats,
+                                                   double *__restrict__ cexp)
 {
   __shared__ unsigned int sacc[2];
+  __shared__ double smom[8];
+  double pm = 0.0, pv = 0.0;  // sums of the collision probabilities p and of p (1 - p)
   if (threadIdx.x < 2) sacc[threadIdx.x] = 0u;
   __syncthreads();
   const double S = COLLIDE ? Sdev[0] : 0.0;
@@ -120,7 +140,10 @@ __global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t
         const uint32_t pid = st.pid[p];
         double vnx, vny, vnz, ud;
         neutral_draw<double>(pc, pid, vnx, vny, vnz, ud);
-        if (collide_decide_n<double>(pc, S, vnx, vny, vnz, ud, vx, vy, vz, capped)) {
+        const double pr = collide_prob_n<double>(pc, S, vnx, vny, vnz, vx, vy, vz, capped);
+        pm += pr;
+        pv += pr * (1.0 - pr);
+        if (ud < pr) {
           if (scatter_n<double>(pc, pid, vnx, vny, vnz, vx, vy, vz) == 1) nacc++;
           else nref++;
         }
@@ -137,16 +160,21 @@ __global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t
     if (COLLIDE) count_flags(cstats, 2, capped != 0);
     count_flags(cstats, 3, bad);
   }
-  if (COLLIDE) flush_scatter_counts(cstats, nacc, nref, sacc);
+  if (COLLIDE) {
+    flush_scatter_counts(cstats, nacc, nref, sacc);
+    flush_collision_moments(cexp, pm, pv, smom);
+  }
 }
 
 template <bool COLLIDE, bool MOVE>
 __global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConstF pc, int64_t np, PState<float> st,
                                                     const Vec4<float> *__restrict__ E4, const double *__restrict__ Sdev,
                                                     uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats,
-                                                    uint8_t *__restrict__ dir)
+                                                    double *__restrict__ cexp, uint8_t *__restrict__ dir)
 {
   __shared__ unsigned int sacc[2];
+  __shared__ double smom[8];
+  float pm = 0.f, pv = 0.f;  // this thread's sums of p and p (1 - p) (a few ions each)
   if (threadIdx.x < 2) sacc[threadIdx.x] = 0u;
   __syncthreads();
   const float S = COLLIDE ? (float)Sdev[0] : 0.f;
@@ -172,7 +200,10 @@ __global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConstF pc, int64
         const uint32_t pid = __float_as_uint(V.w);
         float vnx, vny, vnz, ud;
         neutral_draw<float>(pc, pid, vnx, vny, vnz, ud);
-        if (collide_decide_n<float>(pc, S, vnx, vny, vnz, ud, vx, vy, vz, capped)) {
+        const float pr = collide_prob_n<float>(pc, S, vnx, vny, vnz, vx, vy, vz, capped);
+        pm += pr;
+        pv += pr * (1.0f - pr);
+        if (ud < pr) {
           if (scatter_n<float>(pc, pid, vnx, vny, vnz, vx, vy, vz) == 1) nacc++;
           else nref++;
         }
@@ -197,7 +228,10 @@ __global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConstF pc, int64
     if (COLLIDE) count_flags(cstats, 2, capped != 0);
     count_flags(cstats, 3, bad);
   }
-  if (COLLIDE) flush_scatter_counts(cstats, nacc, nref, sacc);
+  if (COLLIDE) {
+    flush_scatter_counts(cstats, nacc, nref, sacc);
+    flush_collision_moments(cexp, (double)pm, (double)pv, smom);
+  }
 }
 
 // Move only (positions from the current velocities), with keys and counts.
@@ -1457,7 +1491,9 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
   pc.Nc = (double)Nc;
   if (collide && Nc == 0) collide = false;
   ctx->partials_fresh = false;
-  launch_fill<unsigned long long>(ctx, 4, ctx->cstats, 0ull);
+  // slots 0-3 counts, 4-5 the FP64 sums of p and p (1 - p) (zero bits are 0.0)
+  launch_fill<unsigned long long>(ctx, 6, ctx->cstats, 0ull);
+  double *cexp = (double *)(ctx->cstats + 4);
   // the histogram is only needed by the general sort (the neighbour-move
   // sort counts per old cell)
   const bool count = move && !ctx->order_valid;
@@ -1469,7 +1505,7 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
 #define LAUNCH(KER, T, C, M, ...)                                                                             \
   hipLaunchKernelGGL((KER<C, M>), nb, kBlock, 0, ctx->stream, ctx->g, pcT, ctx->np, st,                      \
                      (const Vec4<T> *)ctx->efield[0], ctx->S_dev, count ? ctx->counts : nullptr, ctx->cstats,   \
-                     ##__VA_ARGS__)
+                     cexp, ##__VA_ARGS__)
   if (ctx->mixed) {
     const PushConstF pcT = push_const_f(pc);
     PState<float> st = ctx->pf[ctx->cur];
@@ -1493,7 +1529,7 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
 #undef LAUNCH
   if (move) ctx->keys_moved = true;
   timer_stop(ctx, "push");
-  unsigned long long cs[4];
+  unsigned long long cs[6];
   CHY_HIP(hipMemcpyAsync(cs, ctx->cstats, sizeof(cs), hipMemcpyDeviceToHost, ctx->stream));
   CHY_HIP(hipStreamSynchronize(ctx->stream));
   if (stats) {
@@ -1501,6 +1537,8 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
     stats->refused = (int64_t)cs[1];
     stats->capped = (int64_t)cs[2];
     stats->Nc = Nc;
+    memcpy(&stats->expected, &cs[4], sizeof(double));
+    memcpy(&stats->variance, &cs[5], sizeof(double));
   }
   if (cs[3]) return set_err(ctx, CHY_ERR_FP, "non-finite particle position or velocity after the push");
   const int64_t lim = ctx->prm.collision_refusal_limit;  // particles.c:603 (10)

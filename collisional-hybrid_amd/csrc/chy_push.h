@@ -118,17 +118,24 @@ __device__ __forceinline__ void neutral_draw(const PC &pc, uint32_t pid, T &vnx,
   vnz = (T)pc.vn0[2] + vnT * (r2 * c2);
 }
 
-// Collision decision from a neutral draw (vn, ud) already made.
+// Collision probability Nc |v_r| / S (clipped at 1) for a neutral draw vn.
 template <typename T, typename PC>
-__device__ __forceinline__ bool collide_decide_n(const PC &pc, T S, T vnx, T vny, T vnz, T ud, T vx, T vy,
-                                                 T vz, int &capped)
+__device__ __forceinline__ T collide_prob_n(const PC &pc, T S, T vnx, T vny, T vnz, T vx, T vy, T vz, int &capped)
 {
   const T vrx = vx - vnx, vry = vy - vny, vrz = vz - vnz;
   const T vrr = kin_sqrt<T>(vrx * vrx + vry * vry + vrz * vrz);
   T p = (S > (T)0) ? kin_div<T>((T)pc.Nc * vrr, S) : (T)0;
   capped = 0;
   if (p > (T)1) { p = (T)1; capped = 1; }
-  return ud < p;
+  return p;
+}
+
+// Collision decision from a neutral draw (vn, ud) already made.
+template <typename T, typename PC>
+__device__ __forceinline__ bool collide_decide_n(const PC &pc, T S, T vnx, T vny, T vnz, T ud, T vx, T vy,
+                                                 T vz, int &capped)
+{
+  return ud < collide_prob_n<T>(pc, S, vnx, vny, vnz, vx, vy, vz, capped);
 }
 
 template <typename T, typename PC>

@@ -392,6 +392,8 @@ int chy_params_from_options(int argc, const char *const *argv, chy_params *p, ch
     else if (!strcasecmp(s, "zline") || !strcasecmp(s, "bjacobi") || !strcasecmp(s, "line"))
       p->pc_type = CHY_PC_ZLINE;
     else if (!strcasecmp(s, "cheby") || !strcasecmp(s, "chebyshev")) p->pc_type = CHY_PC_CHEBY;
+    else if (!strcasecmp(s, "mg") || !strcasecmp(s, "gamg") || !strcasecmp(s, "hypre") || !strcasecmp(s, "ml"))
+      p->pc_type = CHY_PC_MG;  /* PETSc's multigrid preconditioners: x-y semicoarsening multigrid */
     else emsg(&e, 0, "Warning: -pc_type %s mapped to the default GPU preconditioner (fft2z)\n", s);
   }
   get_real(&db, "-ksp_rtol", &p->ksp_rtol, &e);

@@ -52,7 +52,7 @@ enum { CHY_BC_PERIODIC = 0, CHY_BC_INJECTION, CHY_BC_REFLECTION, CHY_BC_DIRICHLE
 enum { CHY_PREC_FP64 = 0, CHY_PREC_MIXED = 1 };
 enum { CHY_KSP_GMRES = 0, CHY_KSP_BCGS = 1 };
 enum { CHY_PC_NONE = 0, CHY_PC_JACOBI = 1, CHY_PC_FFT = 2, CHY_PC_ZLINE = 3, CHY_PC_CHEBY = 4, CHY_PC_FFT2 = 5,
-       CHY_PC_FFT2Z = 6 };
+       CHY_PC_FFT2Z = 6, CHY_PC_MG = 7 };
 /* -ksp_gmres_cgs_refinement_type: REFINE_NEVER, REFINE_IFNEEDED (when the
  * first pass removes more than half of the vector's squared norm), REFINE_ALWAYS */
 enum { CHY_CGS_REFINE_NEVER = 0, CHY_CGS_REFINE_IFNEEDED = 1, CHY_CGS_REFINE_ALWAYS = 2 };
@@ -123,6 +123,9 @@ typedef struct {
   int64_t refused;       /* refused (final speed > 10 vT), particles.c:597-605 */
   int64_t capped;        /* Bernoulli probabilities clipped at 1 */
   int64_t Nc;            /* target count, particles.c:514 */
+  double  expected;      /* sum over the ions of their collision probability p: the
+                            mean of accepted + refused (Nc up to the step's drift of S) */
+  double  variance;      /* sum of p (1 - p): its variance */
 } chy_collision_stats;
 
 typedef struct {
@@ -251,7 +254,9 @@ int chy_get_rhs(chy_ctx *ctx, double *b);
  * zero, eigenvalues in [0.2, 2.2]; fft2: fft, then 2 steps of flexible GMRES
  * on the z-averaged residual with the z-averaged operator (the x-y coupling,
  * which the Hall term makes density dependent); fft2z (default): fft2, then the
- * zline solve of its residual added. */
+ * zline solve of its residual added; mg (-pc_type gamg, hypre and ml map
+ * here): one V(2,2) cycle of x-y semicoarsening multigrid with the damped
+ * z-line smoother, any axis lengths. */
 int chy_pc_apply(chy_ctx *ctx, const double *r, double *z);
 /* One time step of the loop body hybrid.c:108-117 (velocities, positions,
  * moments, potential), fused on the device. */

@@ -45,6 +45,8 @@ extern "C" {
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* 24-bit uniform in (0,1): ((w>>8)+0.5)*2^-24, exact in FP32 and FP64. */
 double orc_u24(uint32_t w);
+double orc_normaliser(int64_t np, const double *vx, const double *vy, const double *vz, const double vn0[3],
+                      double vnT);
 
 /* ---- operator and forcing (reference lhs.c, rhs.c, calculus.c) ---------- */
 enum { ORC_LHS_IDENTITY = 0, ORC_LHS_LAPLACIAN = 1, ORC_LHS_FULL = 2 };
@@ -167,7 +169,16 @@ int64_t orc_init_density(int kind, const int64_t N[3], const double p0[3], const
 
 /* ---- potential solve (reference potential.c:6-16 + PETSc KSP) ----------- */
 enum { ORC_PC_NONE = 0, ORC_PC_JACOBI = 1, ORC_PC_FFT = 2, ORC_PC_ZLINE = 3, ORC_PC_CHEBY = 4, ORC_PC_FFT2 = 5,
-       ORC_PC_FFT2Z = 6 };
+       ORC_PC_FFT2Z = 6, ORC_PC_MG = 7 };
+/* multigrid (ORC_PC_MG): levels, smoothing sweeps before and after the
+ * coarse correction, sweeps on the coarsest level, damping */
+#define ORC_MG_MAX_LEVELS 8
+#define ORC_MG_SWEEPS 2
+#define ORC_MG_COARSE_SWEEPS 8
+#define ORC_MG_OMEGA 0.8
+int orc_mg_levels(const int64_t N[3], int64_t out[ORC_MG_MAX_LEVELS][3]);
+void orc_mg_restrict(const int64_t Nf[3], const double *f, double *c);
+void orc_mg_prolong_add(const int64_t Nf[3], const double *c, double *f);
 enum { ORC_KSP_GMRES = 0, ORC_KSP_BCGS = 1 };
 #define ORC_CHEBY_DEGREE 4
 #define ORC_FFT2_ITS 2

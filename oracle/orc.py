@@ -17,7 +17,7 @@ I64 = np.int64
 
 LHS_IDENTITY, LHS_LAPLACIAN, LHS_FULL = 0, 1, 2
 RHS_CONSTANT, RHS_SINUSOIDAL, RHS_FULL = 0, 1, 2
-PC_NONE, PC_JACOBI, PC_FFT, PC_ZLINE, PC_CHEBY, PC_FFT2, PC_FFT2Z = 0, 1, 2, 3, 4, 5, 6
+PC_NONE, PC_JACOBI, PC_FFT, PC_ZLINE, PC_CHEBY, PC_FFT2, PC_FFT2Z, PC_MG = 0, 1, 2, 3, 4, 5, 6, 7
 FFT2_ITS = 2
 CHEBY_DEGREE = 4
 KSP_GMRES, KSP_BCGS = 0, 1
@@ -57,6 +57,8 @@ def _declare(L):
     L.orc_gather.argtypes = [P, P, P, P, P, P, C.c_double, C.c_double, C.c_double, P]
     L.orc_mean_rel_speed.argtypes = [P, P, C.c_double]
     L.orc_mean_rel_speed.restype = C.c_double
+    L.orc_normaliser.argtypes = [C.c_int64, P, P, P, P, C.c_double]
+    L.orc_normaliser.restype = C.c_double
     L.orc_collision_count.argtypes = [C.c_int64] + [C.c_double] * 3 + [P, P] + [C.c_double] * 3
     L.orc_collision_count.restype = C.c_int64
     L.orc_push.argtypes = [P, P, P, P, C.c_int64, P, P, P, P, P, P, P, C.c_double, P, P, P]
@@ -182,6 +184,12 @@ def wrap(x, p0, p1, L):
 def mean_rel_speed(v, vn0, vnT):
     v, vn0 = _c(v), _c(vn0)
     return lib().orc_mean_rel_speed(ptr(v), ptr(vn0), float(vnT))
+
+
+def normaliser(parts, vn0, vnT):
+    """Sum over the ions of the mean relative speed (collision normaliser)."""
+    vn0 = _c(vn0)
+    return lib().orc_normaliser(parts.np, ptr(parts.vx), ptr(parts.vy), ptr(parts.vz), ptr(vn0), float(vnT))
 
 
 def collision_count(Np, nui, dt, viT, vi0, vn0, vnT, mi, mn):

@@ -74,6 +74,19 @@ double orc_mean_rel_speed(const double v[3], const double vn0[3], double vnT)
   return vnT * (c * exp(-0.5 * lam * lam) + (lam + 1.0 / lam) * erf(lam * 0.70710678118654752440));
 }
 
+/* Collision normaliser of a state: sum over the ions of the mean relative
+ * speed (the S that the next step's collision probabilities divide by). */
+double orc_normaliser(int64_t np, const double *vx, const double *vy, const double *vz, const double vn0[3],
+                      double vnT)
+{
+  double S = 0.0;
+  for (int64_t p = 0; p < np; p++) {
+    const double v[3] = {vx[p], vy[p], vz[p]};
+    S += orc_mean_rel_speed(v, vn0, vnT);
+  }
+  return S;
+}
+
 int64_t orc_collision_count(int64_t Np, double nui, double dt, double viT,
                             const double vi0[3], const double vn0[3], double vnT,
                             double mi, double mn)

@@ -160,6 +160,8 @@ static void cyclic_solve(int64_t m, const double *a, const double *b, const doub
 }
 
 static double dot(int64_t n, const double *a, const double *b);
+static void residual(int lhs_type, const int64_t N[3], const double d[3], const double K[3],
+                     const double *n, const double *b, const double *phi, double *r);
 
 /* x-y offset group (dx + 1) + 3 (dy + 1) of each of the 19 row entries */
 static const int kGroup19[19] = {5, 3, 7, 1, 4, 4, 8, 2, 6, 0, 5, 5, 3, 3, 7, 7, 1, 1, 4};
@@ -268,6 +270,126 @@ static void gmres2_fixed(int m, const int64_t N[3], const double *C, const doubl
   free(V); free(Z); free(w); free(H); free(h); free(h2); free(g); free(cs); free(sn); free(y);
 }
 
+/* ---- xy-semicoarsening multigrid with a z-line smoother ---------------------
+ * A V-cycle for the strongly z-coupled operator (parallel mobility about
+ * 7.7e3 times the perpendicular one): the z-line block Jacobi smoother solves
+ * every z-line exactly and damps the high x-y frequencies, and the coarse
+ * grids halve x and y only (semicoarsening), keeping all z planes.  Level l
+ * has N_l = (N_x / 2^l, N_y / 2^l, N_z) nodes and spacings (2^l dx, 2^l dy, dz);
+ * its operator is the same 19-point operator rediscretised from the density
+ * restricted to it.  Restriction is x-y full weighting (1/4, 1/2, 1/4 per
+ * axis, periodic), prolongation its bilinear counterpart.  One V(2,2) cycle
+ * from zero, damping 0.8, and 8 smoothing sweeps on the coarsest level. */
+typedef struct {
+  int nlev;
+  int64_t N[ORC_MG_MAX_LEVELS][3];
+  double d[ORC_MG_MAX_LEVELS][3];
+  double *n[ORC_MG_MAX_LEVELS];
+} mg_levels;
+
+int orc_mg_levels(const int64_t N[3], int64_t out[ORC_MG_MAX_LEVELS][3])
+{
+  int64_t nx = N[0], ny = N[1];
+  int L = 1;
+  out[0][0] = nx; out[0][1] = ny; out[0][2] = N[2];
+  while (L < ORC_MG_MAX_LEVELS && nx % 2 == 0 && ny % 2 == 0 && nx / 2 >= 3 && ny / 2 >= 3) {
+    nx /= 2; ny /= 2;
+    out[L][0] = nx; out[L][1] = ny; out[L][2] = N[2];
+    L++;
+  }
+  return L;
+}
+
+/* c = x-y full weighting of f (f on Nf, c on Nc = (Nf_x / 2, Nf_y / 2, Nf_z)) */
+void orc_mg_restrict(const int64_t Nf[3], const double *f, double *c)
+{
+  static const double w[3] = {0.25, 0.5, 0.25};
+  const int64_t nx = Nf[0], ny = Nf[1], nz = Nf[2], cx = nx / 2, cy = ny / 2;
+  for (int64_t k = 0; k < nz; k++)
+    for (int64_t J = 0; J < cy; J++)
+      for (int64_t I = 0; I < cx; I++) {
+        double s = 0.0;
+        for (int b = 0; b < 3; b++) {
+          const int64_t j = (2 * J + b - 1 + ny) % ny;
+          for (int a = 0; a < 3; a++) {
+            const int64_t i = (2 * I + a - 1 + nx) % nx;
+            s += (w[b] * w[a]) * f[i + nx * (j + ny * k)];
+          }
+        }
+        c[I + cx * (J + cy * k)] = s;
+      }
+}
+
+/* f += bilinear x-y interpolation of c (f on Nf) */
+void orc_mg_prolong_add(const int64_t Nf[3], const double *c, double *f)
+{
+  const int64_t nx = Nf[0], ny = Nf[1], nz = Nf[2], cx = nx / 2, cy = ny / 2;
+  for (int64_t k = 0; k < nz; k++)
+    for (int64_t j = 0; j < ny; j++)
+      for (int64_t i = 0; i < nx; i++) {
+        const int64_t I0 = i / 2, J0 = j / 2;
+        const int64_t I1 = (i % 2) ? (I0 + 1) % cx : I0, J1 = (j % 2) ? (J0 + 1) % cy : J0;
+        const double *ck = c + cx * cy * k;
+        const double v = 0.25 * ((ck[I0 + cx * J0] + ck[I1 + cx * J0]) + (ck[I0 + cx * J1] + ck[I1 + cx * J1]));
+        f[i + nx * (j + ny * k)] += v;
+      }
+}
+
+/* x += omega zline^-1 (b - A x), nsweep times */
+static void mg_smooth(int lhs_type, const int64_t N[3], const double d[3], const double K[3], const double *n,
+                      const double *b, double *x, int nsweep, double *r, double *e)
+{
+  const int64_t nc = N[0] * N[1] * N[2];
+  for (int s = 0; s < nsweep; s++) {
+    residual(lhs_type, N, d, K, n, b, x, r);
+    orc_pc_apply(ORC_PC_ZLINE, lhs_type, N, d, K, n, r, e);
+    for (int64_t c = 0; c < nc; c++) x[c] += ORC_MG_OMEGA * e[c];
+  }
+}
+
+static void mg_vcycle(int lhs_type, const mg_levels *M, const double K[3], int l, const double *b, double *x)
+{
+  const int64_t *N = M->N[l];
+  const int64_t nc = N[0] * N[1] * N[2];
+  double *r = malloc(sizeof(double) * nc), *e = malloc(sizeof(double) * nc);
+  for (int64_t c = 0; c < nc; c++) x[c] = 0.0;
+  if (l == M->nlev - 1) {
+    mg_smooth(lhs_type, N, M->d[l], K, M->n[l], b, x, ORC_MG_COARSE_SWEEPS, r, e);
+  } else {
+    mg_smooth(lhs_type, N, M->d[l], K, M->n[l], b, x, ORC_MG_SWEEPS, r, e);
+    residual(lhs_type, N, M->d[l], K, M->n[l], b, x, r);
+    const int64_t *Nc = M->N[l + 1];
+    const int64_t ncc = Nc[0] * Nc[1] * Nc[2];
+    double *bc = malloc(sizeof(double) * ncc), *xc = malloc(sizeof(double) * ncc);
+    orc_mg_restrict(N, r, bc);
+    mg_vcycle(lhs_type, M, K, l + 1, bc, xc);
+    orc_mg_prolong_add(N, xc, x);
+    free(bc); free(xc);
+    mg_smooth(lhs_type, N, M->d[l], K, M->n[l], b, x, ORC_MG_SWEEPS, r, e);
+  }
+  free(r); free(e);
+}
+
+static void mg_apply(int lhs_type, const int64_t N[3], const double d[3], const double K[3], const double *n,
+                     const double *r, double *z)
+{
+  mg_levels M;
+  int64_t NL[ORC_MG_MAX_LEVELS][3];
+  M.nlev = orc_mg_levels(N, NL);
+  for (int l = 0; l < M.nlev; l++) {
+    for (int a = 0; a < 3; a++) M.N[l][a] = NL[l][a];
+    M.d[l][0] = d[0] * (double)(N[0] / NL[l][0]);
+    M.d[l][1] = d[1] * (double)(N[1] / NL[l][1]);
+    M.d[l][2] = d[2];
+    M.n[l] = malloc(sizeof(double) * NL[l][0] * NL[l][1] * NL[l][2]);
+    if (l == 0) memcpy(M.n[0], n, sizeof(double) * N[0] * N[1] * N[2]);
+    else orc_mg_restrict(M.N[l - 1], M.n[l - 1], M.n[l]);
+  }
+  mg_vcycle(lhs_type, &M, K, 0, r, z);
+  for (int l = 0; l < M.nlev; l++) free(M.n[l]);
+  orc_remove_mean(N[0] * N[1] * N[2], z);
+}
+
 void orc_pc_apply(int pc, int lhs_type, const int64_t N[3], const double d[3], const double K[3],
                   const double *n, const double *r, double *z)
 {
@@ -278,6 +400,10 @@ void orc_pc_apply_deg(int pc, int lhs_type, const int64_t N[3], const double d[3
                       const double *n, const double *r, double *z, int degree)
 {
   const int64_t nc = N[0] * N[1] * N[2];
+  if (pc == ORC_PC_MG) {
+    mg_apply(lhs_type, N, d, K, n, r, z);
+    return;
+  }
   if (pc == ORC_PC_FFT2Z) {
     double *w = malloc(sizeof(double) * nc), *z2 = malloc(sizeof(double) * nc);
     orc_pc_apply_deg(ORC_PC_FFT2, lhs_type, N, d, K, n, r, z, degree);

@@ -69,3 +69,15 @@ def relmax(a, b):
     a, b = np.asarray(a), np.asarray(b)
     s = np.abs(b).max()
     return float(np.abs(a - b).max() / (s if s > 0 else 1.0))
+
+
+def expectation_bar(Nc, Np):
+    """Bound on |sum_i p_i - Nc| for one step's collision probabilities
+    p_i = Nc |v_r,i| / S (particles.c:529-551).  S holds the expected relative
+    speeds, from the velocities before the push, while each p_i uses one
+    neutral draw and the velocity after it: the draws make sum_i |v_r,i|
+    deviate from S by sd(|v_r|) / mean(|v_r|) / sqrt(Np), at most
+    sqrt(3 pi / 8 - 1) = 0.422 / sqrt(Np) (the Maxwell speed distribution's;
+    a drift lowers it), here allowed 5 of those, and the push moves the mean
+    by under 1e-3 at the benchmark's field and time step."""
+    return Nc * (1e-3 + 5 * 0.43 / np.sqrt(Np))

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import chybrid
+from gpu_cases import expectation_bar
 
 pytestmark = pytest.mark.gpu
 
@@ -37,10 +38,11 @@ def test_full_size_steps_conserve_ions_and_converge():
             st = ctx.step()
             assert st.solve.converged == 1 and st.solve.residual <= 1e-10, (it, st.solve.residual)
             c = st.collisions
-            # 3 sigma of the binomial count (success probability Nc / Np per ion)
-            q = c.Nc / np_
-            assert c.Nc > 0 and abs(c.accepted + c.refused - c.Nc) <= 3 * np.sqrt(c.Nc * (1 - q)), (it, c.accepted,
-                                                                                                 c.Nc)
+            # 3 sigma of the Poisson-binomial count (sum of p and of p (1 - p) over
+            # the ions), whose mean matches the target Nc of particles.c:514
+            assert c.Nc > 0 and abs(c.accepted + c.refused - c.expected) <= 3 * np.sqrt(c.variance), (
+                it, c.accepted, c.refused, c.expected, c.variance)
+            assert abs(c.expected - c.Nc) <= expectation_bar(c.Nc, np_), (it, c.expected, c.Nc)
             assert c.refused <= 10
         assert ctx.num_particles() == np_ == 256 ** 3 * 16
         w = ctx.get_moments()[:, 0].sum()

@@ -120,15 +120,18 @@ def test_density_type_options():
 
 
 def test_solver_option_mapping():
-    p, msg = chybrid.params_from_options("-ksp_type bcgs -pc_type hypre -ksp_rtol 1e-9 -ksp_max_it 77 "
+    p, msg = chybrid.params_from_options("-ksp_type bcgs -pc_type ilu -ksp_rtol 1e-9 -ksp_max_it 77 "
                                          "-ksp_gmres_restart 12 -ksp_monitor_short -precision mixed".split())
-    assert p.ksp_type == chybrid.KSP_BCGS and p.pc_type == chybrid.PC_FFT2Z and "hypre" in msg
+    assert p.ksp_type == chybrid.KSP_BCGS and p.pc_type == chybrid.PC_FFT2Z and "ilu" in msg
     assert p.ksp_rtol == 1e-9 and p.ksp_max_it == 77 and p.ksp_restart == 12 and p.ksp_monitor == 1
     assert p.precision == chybrid.PREC_MIXED
     for name, pc in (("none", chybrid.PC_NONE), ("jacobi", chybrid.PC_JACOBI), ("fft", chybrid.PC_FFT),
                      ("fft2", chybrid.PC_FFT2), ("fft2z", chybrid.PC_FFT2Z), ("zline", chybrid.PC_ZLINE),
                      ("bjacobi", chybrid.PC_ZLINE),
-                     ("cheby", chybrid.PC_CHEBY), ("chebyshev", chybrid.PC_CHEBY)):
+                     ("cheby", chybrid.PC_CHEBY), ("chebyshev", chybrid.PC_CHEBY),
+                     # PETSc's multigrid preconditioners (run-case gamg-basic, basic-physics)
+                     ("mg", chybrid.PC_MG), ("gamg", chybrid.PC_MG), ("hypre", chybrid.PC_MG),
+                     ("ml", chybrid.PC_MG)):
         p, _ = chybrid.params_from_options(["-pc_type", name])
         assert p.pc_type == pc, name
     p, _ = chybrid.params_from_options("-pc_cheby_degree 7 -ksp_inner_rtol 1e-5".split())

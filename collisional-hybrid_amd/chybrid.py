@@ -128,6 +128,7 @@ SIGNATURES = {
     "chy_timing_select": (C.c_int, [P, C.c_char_p]),
     "chy_adjoint_backward": (C.c_int, [P, P, P]),
     "chy_diagnostics": (C.c_int, [P, P]),
+    "chy_diagnostics_noalloc": (C.c_int, [P, P]),
 }
 
 _LIB = None

@@ -591,6 +591,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   if (p.pc_type == CHY_PC_MG) A(mg_setup(ctx));
   A(dalloc(ctx, &ctx->diag_part, diag_elems()));
   ctx->diag_out = ctx->diag_part + diag_parts();
+  A(diag_scratch_init(ctx));
   A(fft_setup(ctx));
   if ((p.pc_type == CHY_PC_FFT2 || p.pc_type == CHY_PC_FFT2Z) && ctx->fft_ok) {
     unsigned char *q = nullptr;

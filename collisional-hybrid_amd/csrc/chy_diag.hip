@@ -90,6 +90,26 @@ __global__ __launch_bounds__(256) void k_diag_field(Grid g, double hx, double hy
   block_store<1>(a, part);
 }
 
+// out[0..5] = {weight, momentum (3), kinetic energy, field energy} from the
+// six sums, with the scalings the host applies in diagnostics().
+__global__ void k_diag_scale(const double *__restrict__ sums, double mi, double fscale, double *__restrict__ out)
+{
+  if (threadIdx.x != 0) return;
+  out[0] = sums[0];
+  for (int a = 0; a < 3; a++) out[1 + a] = mi * sums[1 + a];
+  out[4] = (0.5 * mi) * sums[4];
+  out[5] = fscale * sums[5];
+}
+
+// The allocation-free entry point's scratch: the block partials of the six
+// sums and the sums themselves (kDiagBlocks * 6 + 6 doubles, 48 KiB), one
+// copy per GPU for the life of the process.  Host code uses the per-device
+// address from hipGetSymbolAddress, never the symbol itself.
+constexpr int kScratchDoubles = kDiagBlocks * (kIonTerms + 1) + kIonTerms + 1;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double *scratch_ptr[kMaxDevices];
+
 // Enqueues the ion and field sums of the context's state: block partials in
 // part (kDiagBlocks * 6), the six sums in sums.
 void diag_sums(chy_ctx *ctx, double *part, double *sums)
@@ -118,6 +138,21 @@ double field_scale(const chy_params &p) { return (0.5 * kEps0) * (p.d[0] * p.d[1
 
 }  // namespace
 
+// Address of this device's scratch array (once per device, that device
+// current: the context's creation).
+int diag_scratch_init(chy_ctx *ctx)
+{
+  if (ctx->device < 0 || ctx->device >= kMaxDevices)
+    return set_err(ctx, CHY_ERR_ARG_RANGE, "device index beyond the scratch table");
+  if (scratch_ptr[ctx->device]) return CHY_OK;
+  double *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return set_err(ctx, CHY_ERR_LIB, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  scratch_ptr[ctx->device] = ptr;
+  return CHY_OK;
+}
+
 // Fills d from the current ions and potential (the potential term is zero
 // before the first solve).  One host read-back.
 int diagnostics(chy_ctx *ctx, chy_diag_stats *d)
@@ -139,6 +174,20 @@ int diagnostics(chy_ctx *ctx, chy_diag_stats *d)
   return CHY_OK;
 }
 
+// The same figures into caller device memory, with the partials in this
+// device's static scratch: no allocation, no host synchronisation.
+int diagnostics_noalloc(chy_ctx *ctx, double *d_out)
+{
+  double *base = scratch_ptr[ctx->device];
+  if (!base) return set_err(ctx, CHY_ERR_LIB, "the scratch array of this device has no address");
+  double *part = base, *sums = base + (int64_t)kDiagBlocks * (kIonTerms + 1);
+  diag_sums(ctx, part, sums);
+  hipLaunchKernelGGL(k_diag_scale, 1, 64, 0, ctx->stream, (const double *)sums, ctx->prm.mi, field_scale(ctx->prm),
+                     d_out);
+  CHY_HIP(hipGetLastError());
+  return CHY_OK;
+}
+
 // Device doubles the diagnostics need: block partials and the sums.
 int64_t diag_elems() { return (int64_t)kDiagBlocks * (kIonTerms + 1) + kIonTerms + 1; }
 int diag_parts() { return kDiagBlocks * (kIonTerms + 1); }
@@ -151,4 +200,11 @@ extern "C" int chy_diagnostics(chy_ctx *ctx, chy_diag_stats *d)
 {
   if (!ctx || !d) return CHY_ERR_ARG_WRONG;
   return diagnostics(ctx, d);
+}
+
+extern "C" int chy_diagnostics_noalloc(chy_ctx *ctx, double *d_out)
+{
+  if (!ctx) return CHY_ERR_ARG_WRONG;
+  if (!d_out) return set_err(ctx, CHY_ERR_ARG_WRONG, "chy_diagnostics_noalloc: no device output array");
+  return diagnostics_noalloc(ctx, d_out);
 }

@@ -324,6 +324,7 @@ void deposit_state(chy_ctx *ctx, const PState<double> &st, const uint32_t *offs,
 
 // ---- diagnostics (chy_diag.hip) ----
 int diagnostics(chy_ctx *ctx, chy_diag_stats *d);
+int diag_scratch_init(chy_ctx *ctx);  // this device's static scratch address (context creation)
 int64_t diag_elems();  // device doubles for diag_part + diag_out
 int diag_parts();      // of which block partials (diag_out follows them)
 

@@ -269,6 +269,17 @@ int chy_step(chy_ctx *ctx, chy_step_stats *stats);
  * field the push gathers, without the imposed uniform E0), each summed in
  * FP64 over a fixed grid, so repeated calls agree bitwise. */
 int chy_diagnostics(chy_ctx *ctx, chy_diag_stats *d);
+/* The same diagnostics without any allocation or host synchronisation, for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops).  No reference counterpart.  Enqueued on the context stream; writes
+ * d_out[0..5] = {weight, momentum x, y, z, kinetic energy, field energy}, the
+ * values chy_diagnostics returns bit for bit, to caller device memory.  The
+ * block partials live in a static 48 KiB device array, one per GPU, set up
+ * by chy_create; its reduction grid is fixed at 1024 blocks, so every ion and
+ * node count fits.  Each call overwrites every partial it reads, so nothing
+ * needs clearing, but calls that overlap on one GPU (two contexts' streams
+ * at once) race on that array. */
+int chy_diagnostics_noalloc(chy_ctx *ctx, double *d_out);
 
 /* ---- adjoint of the deterministic step map --------------------------------
  * No reference counterpart: the reference has no derivatives.  The forward

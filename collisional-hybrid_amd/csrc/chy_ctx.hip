@@ -552,7 +552,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   A(dalloc(ctx, &ctx->maxcount, 1));
   A(dalloc(ctx, &ctx->trans, 27 * nc));
   A(dalloc(ctx, &ctx->info, cap));
-  if (ctx->mixed) A(dalloc(ctx, &ctx->dir, cap));
+  A(dalloc(ctx, &ctx->dir, cap + 16));  // the push's move directions; read as aligned words
   A(dalloc(ctx, &ctx->nbase, 27 * nc));
   A(dalloc(ctx, &ctx->offs_alt, nc + 1));
   A(dalloc(ctx, &ctx->sortflag, 2));

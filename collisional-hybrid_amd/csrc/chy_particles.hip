@@ -108,7 +108,7 @@ template <bool COLLIDE, bool MOVE>
 __global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t np, PState<double> st,
                                                    const Vec4<double> *__restrict__ E4, const double *__restrict__ Sdev,
                                                    uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats,
-                                                   double *__restrict__ cexp)
+                                                   double *__restrict__ cexp, uint8_t *__restrict__ dir)
 {
   __shared__ unsigned int sacc[2];
   __shared__ double smom[8];
@@ -151,6 +151,13 @@ __global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t
       st.vx[p] = vx; st.vy[p] = vy; st.vz[p] = vz;
       if (MOVE) {
         key = move_fp64(g, pc.dt, x, y, z, vx, vy, vz);
+        if (dir) {  // the move direction for the neighbour-move sort (k_trans_dir)
+          int i2, j2, k2;
+          split_cell(g, key, i2, j2, k2);
+          const int di = nb_step(i2 - i, g.nx), dj = nb_step(j2 - j, g.ny), dk = nb_step(k2 - k, g.nz);
+          dir[p] = (di == 2 || dj == 2 || dk == 2) ? (uint8_t)255
+                                                   : (uint8_t)((di + 1) + 3 * (dj + 1) + 9 * (dk + 1));
+        }
         st.x[p] = x; st.y[p] = y; st.z[p] = z;
         st.cell[p] = key;
       }
@@ -545,14 +552,15 @@ __global__ __launch_bounds__(256) void k_reorder(int64_t np, const uint32_t *__r
 //   offs[c] + (ions entering c from old cells numbered below its old cell,
 //              the stayers of c included when c is below)
 //           + (its rank among the ions of its old cell that go to c).
-// k_trans counts, per old cell, the ions going each way and ranks every ion
-// within its direction; k_counts turns the counts into the new histogram and
+// From the new keys, k_trans counts, per old cell, the ions going each way and
+// ranks every ion within its direction; k_counts turns the counts into the new histogram and
 // the start of every (old cell, direction) group within its new cell; k_perm
 // places every ion (perm[slot] = old index) and k_cell_partials<GATHER>
 // moves the state in slot order and makes the deposit's cell partials from it.
 // The atomic scatter and the per-cell ranking of the general sort are not
 // needed.  A move further than one cell, or 255 ions of one cell going one way,
-// sets a flag and the general sort runs instead.
+// sets a flag and the general sort runs instead.  When the push wrote the move
+// directions, k_trans_dir (below) reads them in place of the keys.
 
 
 __device__ __forceinline__ int64_t nb_cell(const Grid &g, int i, int j, int k)
@@ -597,45 +605,21 @@ __global__ __launch_bounds__(256) void k_trans(Grid g, const uint32_t *__restric
   if (bad) atomicOr(flag, 1u);
 }
 
-// k_trans with the move directions the push wrote (dir, one byte per ion in
-// the old order) in place of the new keys: the same counts and ranks.
-__global__ __launch_bounds__(256) void k_trans_dir(Grid g, const uint32_t *__restrict__ oldoffs,
-                                                   const uint8_t *__restrict__ dir, uint16_t *__restrict__ info,
-                                                   uint8_t *__restrict__ trans, uint32_t *__restrict__ flag)
+// The three source coordinates of new coordinate c along an axis of n cells,
+// in ascending wrapped order, as moves m (source = c - m, m in {-1, 0, 1}).
+// Cell numbers are ordered z, then y, then x, so nesting the three axes'
+// orders lists the 27 sources of a cell in ascending cell number.
+__device__ __forceinline__ void src_order(int c, int n, int m[3])
 {
-  __shared__ uint16_t cnt[27][256];  // column t belongs to thread t
-  const int t = threadIdx.x;
-  for (int d = 0; d < 27; d++) cnt[d][t] = 0;
-  const int64_t c = xcd_block() * 256 + t;
-  if (c >= g.nc) return;
-  bool bad = false;
-  const uint32_t lo = oldoffs[c], hi = oldoffs[c + 1];
-  constexpr int kB = 8;  // directions in flight per thread
-  for (uint32_t q0 = lo; q0 < hi; q0 += kB) {
-    int dd[kB];
-#pragma unroll
-    for (int u = 0; u < kB; u++) dd[u] = q0 + u < hi ? (int)dir[q0 + u] : 0;
-#pragma unroll
-    for (int u = 0; u < kB; u++) {
-      if (q0 + u >= hi) break;
-      const int d = dd[u];
-      if (d > 26) { bad = true; continue; }
-      const uint32_t r = cnt[d][t];
-      if (r >= 255) { bad = true; continue; }
-      info[q0 + u] = (uint16_t)((d << 8) | r);
-      cnt[d][t] = (uint16_t)(r + 1);
-    }
-  }
-  for (int d = 0; d < 27; d++) trans[(int64_t)d * g.nc + c] = (uint8_t)cnt[d][t];
-  if (bad) atomicOr(flag, 1u);
+  if (c == 0 && n > 1) { m[0] = 0; m[1] = -1; m[2] = 1; }        // c, c + 1, then c - 1 = n - 1
+  else if (c == n - 1 && n > 1) { m[0] = -1; m[1] = 1; m[2] = 0; } // c + 1 = 0, c - 1, c
+  else { m[0] = 1; m[1] = 0; m[2] = -1; }                       // c - 1, c, c + 1
 }
 
 // counts[c] = ions ending in c; base[d][s] = the ions entering c = s + off(d)
 // from old cells numbered below s (the group of ions going from s in
-// direction d starts there within c's segment).  Away from the periodic wrap
-// the source s_d = c - off(d) falls as d rises (the grid has at least 3 cells
-// per axis), so the bases are suffix sums over d; cells on the boundary
-// compare the source numbers.
+// direction d starts there within c's segment): a running sum over the 27
+// sources in ascending cell number, their 27 group sizes loaded first.
 __global__ __launch_bounds__(256) void k_counts(Grid g, const uint8_t *__restrict__ trans, uint32_t *__restrict__ counts,
                                                 uint16_t *__restrict__ base)
 {
@@ -643,33 +627,26 @@ __global__ __launch_bounds__(256) void k_counts(Grid g, const uint8_t *__restric
   if (c >= g.nc) return;
   int i, j, k;
   split_cell(g, (int)c, i, j, k);
-  int64_t s[27];
+  int mx[3], my[3], mz[3];
+  src_order(i, g.nx, mx);
+  src_order(j, g.ny, my);
+  src_order(k, g.nz, mz);
   uint32_t t[27];
-  uint32_t n = 0;
 #pragma unroll
-  for (int d = 0; d < 27; d++) {
-    s[d] = nb_cell(g, i - (d % 3 - 1), j - ((d / 3) % 3 - 1), k - (d / 9 - 1));  // old cell of an arrival in direction d
-    t[d] = trans[(int64_t)d * g.nc + s[d]];
-    n += t[d];
+  for (int p = 0; p < 27; p++) {
+    const int di = mx[p % 3], dj = my[(p / 3) % 3], dk = mz[p / 9];
+    const int d = (di + 1) + 3 * (dj + 1) + 9 * (dk + 1);
+    t[p] = trans[(int64_t)d * g.nc + nb_cell(g, i - di, j - dj, k - dk)];
   }
-  counts[c] = n;
-  const bool interior = i >= 1 && i <= g.nx - 2 && j >= 1 && j <= g.ny - 2 && k >= 1 && k <= g.nz - 2;
-  if (interior) {
-    uint32_t acc = 0;
+  uint32_t acc = 0;
 #pragma unroll
-    for (int d = 26; d >= 0; d--) {
-      base[(int64_t)d * g.nc + s[d]] = (uint16_t)acc;
-      acc += t[d];
-    }
-  } else {
-#pragma unroll
-    for (int d = 0; d < 27; d++) {
-      uint32_t b = 0;
-#pragma unroll
-      for (int d2 = 0; d2 < 27; d2++) b += s[d2] < s[d] ? t[d2] : 0u;
-      base[(int64_t)d * g.nc + s[d]] = (uint16_t)b;
-    }
+  for (int p = 0; p < 27; p++) {
+    const int di = mx[p % 3], dj = my[(p / 3) % 3], dk = mz[p / 9];
+    const int d = (di + 1) + 3 * (dj + 1) + 9 * (dk + 1);
+    base[(int64_t)d * g.nc + nb_cell(g, i - di, j - dj, k - dk)] = (uint16_t)acc;
+    acc += t[p];
   }
+  counts[c] = acc;
 }
 
 __global__ __launch_bounds__(256) void k_perm(Grid g, int64_t np, const int32_t *__restrict__ cell,
@@ -689,6 +666,82 @@ __global__ __launch_bounds__(256) void k_perm(Grid g, int64_t np, const int32_t 
   }
 }
 
+
+// The directions of a block's old cells (one contiguous range, ions s0 ..
+// s1) staged in LDS with aligned 4-byte loads: word w0 = s0 / 4 first, so
+// ion q's byte is at q - 4 w0.  Returns false (nothing staged) when the
+// range does not fit NW words per thread.
+template <int NW>
+__device__ __forceinline__ bool stage_dirs(const uint8_t *__restrict__ dir, uint32_t s0, uint32_t s1,
+                                           uint32_t *__restrict__ sw)
+{
+  const uint32_t w0 = s0 >> 2, w1 = (s1 + 3) >> 2;
+  if (w1 - w0 > (uint32_t)(NW * 256)) return false;
+  const uint32_t *dw = reinterpret_cast<const uint32_t *>(dir);
+  uint32_t v[NW];
+#pragma unroll
+  for (int u = 0; u < NW; u++) {
+    const uint32_t w = w0 + threadIdx.x + 256u * u;
+    v[u] = w < w1 ? dw[w] : 0u;
+  }
+#pragma unroll
+  for (int u = 0; u < NW; u++) {
+    const uint32_t w = w0 + threadIdx.x + 256u * u;
+    if (w < w1) sw[w - w0] = v[u];
+  }
+  return true;
+}
+
+// k_trans from the move directions the push wrote (dir, one byte per ion in
+// the old order): per old cell the 27 group sizes, and per ion its
+// (direction, rank) record for k_perm.  The block's 256 cells' directions
+// (one contiguous range) are staged in LDS by aligned coalesced loads, each
+// thread walks its cell there, and the records go out coalesced from LDS.
+// NW: staged words per thread; a block whose ions do not fit reads and
+// writes global memory directly.
+template <int NW>
+__global__ __launch_bounds__(256) void k_trans_dir(Grid g, const uint32_t *__restrict__ oldoffs,
+                                                   const uint8_t *__restrict__ dir, uint16_t *__restrict__ info,
+                                                   uint8_t *__restrict__ trans, uint32_t *__restrict__ flag)
+{
+  constexpr int CAP = NW * 1024 - 8;  // ions whose directions always fit, whatever their alignment
+  __shared__ uint32_t sw[NW * 256];
+  __shared__ uint16_t sinfo[CAP];
+  __shared__ uint16_t cnt[27][256];  // column t belongs to thread t
+  __shared__ uint32_t so[257];
+  const int t = (int)threadIdx.x;
+  const int64_t c0 = xcd_block() * 256;
+  if (c0 >= g.nc) return;
+  const int ncell = (int)((g.nc - c0) < 256 ? (g.nc - c0) : 256);
+  if (t <= ncell) so[t] = oldoffs[c0 + t];
+  if (t == 0 && ncell == 256) so[256] = oldoffs[c0 + 256];
+  for (int d = 0; d < 27; d++) cnt[d][t] = 0;
+  __syncthreads();
+  const uint32_t s0 = so[0], s1 = so[ncell];
+  const bool staged = s1 - s0 <= (uint32_t)CAP && stage_dirs<NW>(dir, s0, s1, sw);
+  __syncthreads();
+  const uint8_t *sb = reinterpret_cast<const uint8_t *>(sw) + (s0 - 4u * (s0 >> 2));  // ion s0's direction
+  bool bad = false;
+  if (t < ncell) {
+    const uint32_t lo = so[t], hi = so[t + 1];
+    for (uint32_t q = lo; q < hi; q++) {
+      const uint32_t d = staged ? (uint32_t)sb[q - s0] : (uint32_t)dir[q];
+      if (d > 26u) { bad = true; continue; }
+      const uint32_t r = cnt[d][t];
+      if (r >= 255u) { bad = true; continue; }
+      const uint16_t rec = (uint16_t)((d << 8) | r);
+      if (staged) sinfo[q - s0] = rec;
+      else info[q] = rec;
+      cnt[d][t] = (uint16_t)(r + 1);
+    }
+    for (int d = 0; d < 27; d++) trans[(int64_t)d * g.nc + c0 + t] = (uint8_t)cnt[d][t];
+  }
+  if (staged) {
+    __syncthreads();
+    for (uint32_t e = (uint32_t)t; e < s1 - s0; e += 256) info[s0 + e] = sinfo[e];
+  }
+  if (bad) atomicOr(flag, 1u);
+}
 
 // ---- deposit ---------------------------------------------------------------------
 // Per-cell corner partials P[(4q+m)*nc + c], m in {w, w vx, w vy, w vz}.
@@ -1083,13 +1136,22 @@ static int sort_neighbour(chy_ctx *ctx, bool &done)
   const unsigned ncb = (unsigned)((nc + 255) / 256);
   launch_fill<uint32_t>(ctx, 2, ctx->sortflag, 0u);
   launch_fill<uint32_t>(ctx, 1, ctx->maxcount, 0u);
+  // the group sizes and ranks from the push's move directions (k_trans_dir,
+  // its LDS staging sized from the mean occupancy) or from the new keys (k_trans)
+  const bool fromdir = ctx->dir_valid;
+  const bool dense = np > 20 * nc;
   timer_start(ctx, "sort_trans");
-  if (ctx->dir_valid)
-    hipLaunchKernelGGL(k_trans_dir, ncb, 256, 0, ctx->stream, ctx->g, ctx->offsets, (const uint8_t *)ctx->dir,
-                       ctx->info, ctx->trans, ctx->sortflag);
-  else
+  if (fromdir) {
+    if (dense)
+      hipLaunchKernelGGL((k_trans_dir<16>), ncb, 256, 0, ctx->stream, ctx->g, ctx->offsets,
+                         (const uint8_t *)ctx->dir, ctx->info, ctx->trans, ctx->sortflag);
+    else
+      hipLaunchKernelGGL((k_trans_dir<8>), ncb, 256, 0, ctx->stream, ctx->g, ctx->offsets,
+                         (const uint8_t *)ctx->dir, ctx->info, ctx->trans, ctx->sortflag);
+  } else {
     hipLaunchKernelGGL(k_trans, ncb, 256, 0, ctx->stream, ctx->g, ctx->offsets, cell, ctx->info, ctx->trans,
                        ctx->sortflag);
+  }
   hipLaunchKernelGGL(k_counts, ncb, 256, 0, ctx->stream, ctx->g, ctx->trans, ctx->counts, ctx->nbase);
   timer_stop(ctx, "sort_trans");
   timer_start(ctx, "sort_scan");
@@ -1518,13 +1580,14 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
     else LAUNCH(k_push_mixed, float, false, false, dir);
     if (move) ctx->dir_valid = dir != nullptr;
   } else {
-    ctx->dir_valid = false;
     const PushConst &pcT = pc;
     PState<double> st = ctx->pd[ctx->cur];
-    if (collide && move) LAUNCH(k_push_fp64, double, true, true);
-    else if (collide) LAUNCH(k_push_fp64, double, true, false);
-    else if (move) LAUNCH(k_push_fp64, double, false, true);
-    else LAUNCH(k_push_fp64, double, false, false);
+    uint8_t *dir = move && ctx->order_valid && !ctx->keys_moved ? ctx->dir : nullptr;
+    if (collide && move) LAUNCH(k_push_fp64, double, true, true, dir);
+    else if (collide) LAUNCH(k_push_fp64, double, true, false, dir);
+    else if (move) LAUNCH(k_push_fp64, double, false, true, dir);
+    else LAUNCH(k_push_fp64, double, false, false, dir);
+    if (move) ctx->dir_valid = dir != nullptr;
   }
 #undef LAUNCH
   if (move) ctx->keys_moved = true;

@@ -160,7 +160,7 @@ def test_cfg3_adjoint_at_full_size():
     within 1e-5 of the gradient's norm, and the differences at 1e-7 and
     1e-6 |E0| no closer to the adjoint than 1e-8 (they average more pieces);
     six cells against central differences at h = 1e-5, within 1e-5 of
-    themselves or the difference to h = 1e-6."""
+    themselves, their own bar or the difference to h = 1e-6."""
     import os
     import sys
 
@@ -181,4 +181,5 @@ def test_cfg3_adjoint_at_full_size():
     assert c8["max_err_rel_to_norm"] <= 1e-5
     for hr in ("1e-07", "1e-06"):
         assert res["E0"][f"central_h={hr}|E0|"]["max_err_rel_to_norm"] >= 0.5 * c8["max_err_rel_to_norm"]
-    assert res["delta"]["within_1e-5_or_two_h_bar"] == "6/6", res["delta"]
+    assert res["delta"]["within_1e-5_or_bar"] == "6/6", res["delta"]
+    assert res["delta"]["central_h=1e-05"]["max_err_rel_to_max"] <= 1e-5, res["delta"]

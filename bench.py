@@ -191,7 +191,9 @@ def run_gpu(N, ppc, prec, steps, warmup, keep=False, cross=None):
             ctx.timing_reset()
             ctx.copy_kernel(COPY_BYTES, repeat=5)
             tot, cnt = ctx.timing_get("copy")
-            copy = {"bytes": 2 * COPY_BYTES, "avg_ms": tot / cnt, "GBs": 2 * COPY_BYTES / (tot / cnt * 1e-3) / 1e9}
+            tot1, cnt1 = ctx.timing_get("copy1")
+            copy = {"bytes": 2 * COPY_BYTES, "avg_ms": tot / cnt, "GBs": 2 * COPY_BYTES / (tot / cnt * 1e-3) / 1e9,
+                    "one_per_thread": {"avg_ms": tot1 / cnt1, "GBs": 2 * COPY_BYTES / (tot1 / cnt1 * 1e-3) / 1e9}}
     finally:
         ctx.close()
     nc = N ** 3
@@ -426,7 +428,11 @@ def measured_traffic(kernel, N, ppc):
 
 
 def cpu_baseline(Ns=64, ppc=16, steps=2):
-    """Oracle (1 thread) on a bounded sample: Ns^3 grid, same physics."""
+    """The oracle (FP64, the reference's formulas and loop order) on a
+    bounded sample of the workload: Ns^3 grid, same ions per cell and
+    physics, `steps` full steps, timed with every host thread (the reported
+    value; its independent loops run in parallel, every sum in serial order)
+    and with one thread."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -439,36 +445,61 @@ def cpu_baseline(Ns=64, ppc=16, steps=2):
     g = orc.Grid((Ns,) * 3, p1=(0.05 * Ns,) * 3)
     K = (0.0, 0.0, Q * 5e-5 / ME / 1e5)
     E0 = (0.05, 0.0, 0.0)
-    p = orc.init_particles(g, ppc, 0, (0, 0, 0), (vT, vT, vT), pert=2, amp=0.01)
-    perm, offs = orc.sort_perm(g.nc, orc.cell_keys(g, p))
-    p = p.permute(perm)
-    mom = orc.deposit_sorted(g, p, offs)
-    b = orc.rhs(g, K, E0, 0.0, ME * 1e5 / Q, *mom)
-    phi, _, _, _ = orc.solve(g, K, mom[0], b, rtol=1e-10)
-    Nc = orc.collision_count(p.np, 1e3, 1e-5, np.sqrt(3) * vT, (0, 0, 0), (0, 0, 0), vnT, mi, mn)
-    S = sum(orc.mean_rel_speed((p.vx[i], p.vy[i], p.vz[i]), (0, 0, 0), vnT) for i in range(0, p.np, 97)) * 97
-    tp = ts = 0.0
-    for it in range(steps):
-        P = orc.push_params(g, Q / mi, 1e-5, (0, 0, 5e-5), E0, collide=True, mi=mi, mn=mn,
-                            viT=np.sqrt(3) * vT, vnT=vnT, Nc=Nc, seed=0, step=it)
-        t0 = time.perf_counter()
-        p, _, mom, S, _ = orc.step(g, P, p, phi, S, E0)
-        t1 = time.perf_counter()
+
+    def run():
+        p = orc.init_particles(g, ppc, 0, (0, 0, 0), (vT, vT, vT), pert=2, amp=0.01)
+        perm, offs = orc.sort_perm(g.nc, orc.cell_keys(g, p))
+        p = p.permute(perm)
+        mom = orc.deposit_sorted(g, p, offs)
         b = orc.rhs(g, K, E0, 0.0, ME * 1e5 / Q, *mom)
-        phi, _, _, _ = orc.solve(g, K, mom[0], b, phi0=phi, rtol=1e-10)
-        t2 = time.perf_counter()
-        tp += t1 - t0
-        ts += t2 - t1
+        phi, _, _, _ = orc.solve(g, K, mom[0], b, rtol=1e-10)
+        Nc = orc.collision_count(p.np, 1e3, 1e-5, np.sqrt(3) * vT, (0, 0, 0), (0, 0, 0), vnT, mi, mn)
+        S = orc.normaliser(p, [0.0, 0.0, 0.0], vnT)
+        tp = ts = 0.0
+        for it in range(steps):
+            P = orc.push_params(g, Q / mi, 1e-5, (0, 0, 5e-5), E0, collide=True, mi=mi, mn=mn,
+                                viT=np.sqrt(3) * vT, vnT=vnT, Nc=Nc, seed=0, step=it)
+            t0 = time.perf_counter()
+            p, _, mom, S, _ = orc.step(g, P, p, phi, S, E0)
+            t1 = time.perf_counter()
+            b = orc.rhs(g, K, E0, 0.0, ME * 1e5 / Q, *mom)
+            phi, _, _, _ = orc.solve(g, K, mom[0], b, phi0=phi, rtol=1e-10)
+            t2 = time.perf_counter()
+            tp += t1 - t0
+            ts += t2 - t1
+        return p.np, tp, ts
+
+    nthreads = orc.get_threads()
+    npart, tp, ts = run()
+    orc.set_threads(1)
+    try:
+        _, tp1, ts1 = run()
+    finally:
+        orc.set_threads(nthreads)
     return {
-        "value": p.np * steps / (tp + ts),
+        "value": npart * steps / (tp + ts),
         "unit": "particle-steps/s",
-        "cores": 1,
+        "cores": nthreads,
         "kind": "port",
         "sample": f"oracle full step (push+collisions+move, stable sort, deposit, forcing, FFT-GMRES solve to 1e-10) "
-                  f"on {Ns}^3 x {ppc} ions/cell ({p.np} ions), {steps} steps, 1 thread",
+                  f"on {Ns}^3 x {ppc} ions/cell ({npart} ions), {steps} steps, {nthreads} threads",
         "particles_s_per_step": tp / steps,
         "phi_solve_ms_per_step": ts / steps * 1e3,
+        "one_thread": {"value": npart * steps / (tp1 + ts1), "particles_s_per_step": tp1 / steps,
+                       "phi_solve_ms_per_step": ts1 / steps * 1e3},
+        "cpu_model": _cpu_model(),
     }
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():

@@ -217,6 +217,10 @@ void orc_solve(int ksp, int pc, int lhs_type, const int64_t N[3], const double d
                int64_t info[2], double resid[1]);
 /* Complex 3-D DFT helper (in place, interleaved re/im, sign -1 forward). */
 void orc_fft3d(const int64_t N[3], double *data, int inverse);
+/* Host threads of the independent per-ion, per-node and per-line loops; every
+ * sum keeps its serial order, so no result depends on the thread count. */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 
 #ifdef __cplusplus
 }

@@ -58,6 +58,11 @@ def _declare(L):
     L.orc_mean_rel_speed.argtypes = [P, P, C.c_double]
     L.orc_mean_rel_speed.restype = C.c_double
     L.orc_normaliser.argtypes = [C.c_int64, P, P, P, P, C.c_double]
+    L.orc_mg_levels.argtypes = [P, P]
+    L.orc_mg_restrict.argtypes = [P, P, P]
+    L.orc_mg_prolong_add.argtypes = [P, P, P]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_get_threads.restype = C.c_int
     L.orc_normaliser.restype = C.c_double
     L.orc_collision_count.argtypes = [C.c_int64] + [C.c_double] * 3 + [P, P] + [C.c_double] * 3
     L.orc_collision_count.restype = C.c_int64
@@ -353,6 +358,37 @@ def solve(g, K, n, b, phi0=None, ksp=KSP_GMRES, pc=PC_FFT, lhs_type=LHS_FULL,
     lib().orc_solve(ksp, pc, lhs_type, ptr(g.N), ptr(g.d), ptr(K), ptr(n), ptr(b),
                     ptr(phi), rtol, atol, int(max_it), int(restart), ptr(info), ptr(res))
     return phi, int(info[0]), bool(info[1]), float(res[0])
+
+
+def mg_levels(N):
+    """Grid sizes of the multigrid levels (x and y halved per level)."""
+    out = np.zeros((8, 3), I64)
+    L = lib().orc_mg_levels(ptr(_c(N, I64)), ptr(out))
+    return [tuple(int(v) for v in out[l]) for l in range(L)]
+
+
+def mg_restrict(N, f):
+    N = _c(N, I64)
+    c = np.zeros(int(N[0] // 2 * (N[1] // 2) * N[2]))
+    lib().orc_mg_restrict(ptr(N), ptr(_c(f)), ptr(c))
+    return c
+
+
+def mg_prolong_add(N, c, f):
+    N = _c(N, I64)
+    out = _c(f).copy()
+    lib().orc_mg_prolong_add(ptr(N), ptr(_c(c)), ptr(out))
+    return out
+
+
+def set_threads(n):
+    """Host threads of the oracle's independent loops (no result depends on
+    them: every sum keeps its serial order)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().orc_get_threads())
 
 
 def fft3d(g, data, inverse=False):

@@ -113,6 +113,7 @@ void orc_lhs_apply(int type, const int64_t N[3], const double d[3], const double
                    const double *n, const double *x, double *y)
 {
   const int64_t nc = N[0] * N[1] * N[2];
+#pragma omp parallel for schedule(static) if (nc > 16384)
   for (int64_t c = 0; c < nc; c++) {
     const int64_t i = c % N[0], j = (c / N[0]) % N[1], k = c / (N[0] * N[1]);
     double v[19], acc = 0.0;
@@ -194,6 +195,7 @@ void orc_rhs(int type, const int64_t N[3], const double d[3], const double K[3],
   const double ryx = Kx * Ky + Kz, ryy = 1 + Ky * Ky, ryz = Kz * Ky - Kx;
   const double rzx = Kx * Kz - Ky, rzy = Ky * Kz + Kx, rzz = 1 + Kz * Kz;
   const double scale = 1.0 / detA;
+#pragma omp parallel for schedule(static) if (nc > 16384)
   for (int64_t c = 0; c < nc; c++) {
     const int64_t i = c % N[0], j = (c / N[0]) % N[1], k = c / (N[0] * N[1]);
 #define F(a, di, dj, dk) a[lin(N, i + (di), j + (dj), k + (dk))]
@@ -227,6 +229,7 @@ void orc_efield_nodes(const int64_t N[3], const double d[3], const double E0[3],
 {
   const int64_t nc = N[0] * N[1] * N[2];
   const double hx = 0.5 / d[0], hy = 0.5 / d[1], hz = 0.5 / d[2];
+#pragma omp parallel for schedule(static) if (nc > 16384)
   for (int64_t c = 0; c < nc; c++) {
     const int64_t i = c % N[0], j = (c / N[0]) % N[1], k = c / (N[0] * N[1]);
     ex[c] = E0[0] - (phi[lin(N, i + 1, j, k)] - phi[lin(N, i - 1, j, k)]) * hx;
@@ -282,6 +285,8 @@ void orc_deposit_sorted(const int64_t N[3], const double p0[3], const double d[3
   (void)np;
   const int64_t nc = N[0] * N[1] * N[2];
   double *P = (double *)calloc((size_t)nc * 32, sizeof(double));
+  /* cells are independent, each summed in its slot order */
+#pragma omp parallel for schedule(dynamic, 256) if (nc > 4096)
   for (int64_t c = 0; c < nc; c++) {
     double *Pc = P + 32 * c;
     for (int64_t p = offsets[c]; p < offsets[c + 1]; p++) {
@@ -300,6 +305,7 @@ void orc_deposit_sorted(const int64_t N[3], const double p0[3], const double d[3
       }
     }
   }
+#pragma omp parallel for schedule(static) if (nc > 16384)
   for (int64_t c = 0; c < nc; c++) {
     const int64_t i = c % N[0], j = (c / N[0]) % N[1], k = c / (N[0] * N[1]);
     double m[4] = {0.0, 0.0, 0.0, 0.0};

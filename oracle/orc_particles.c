@@ -167,8 +167,11 @@ void orc_push(const orc_push_params *P, const double *ex, const double *ey, cons
   const double t0 = tau * P->B0[0], t1 = tau * P->B0[1], t2 = tau * P->B0[2];
   const double tt = t0 * t0 + t1 * t1 + t2 * t2;
   const double s0 = 2.0 * t0 / (1 + tt), s1 = 2.0 * t1 / (1 + tt), s2 = 2.0 * t2 / (1 + tt);
-  double Sn = 0.0;
-  stats[0] = stats[1] = stats[2] = 0;
+  /* independent ions: threads change nothing (the counts are integers and
+   * the normaliser is summed in ion order after the loop) */
+  double *mrs = (double *)malloc(sizeof(double) * (size_t)(np > 0 ? np : 1));
+  int64_t acc = 0, ref = 0, cap = 0;
+#pragma omp parallel for schedule(static) reduction(+ : acc, ref, cap) if (np > 16384)
   for (int64_t p = 0; p < np; p++) {
     double E[3];
     orc_gather(P->N, P->p0, P->d, ex, ey, ez, x[p], y[p], z[p], E);
@@ -185,9 +188,9 @@ void orc_push(const orc_push_params *P, const double *ex, const double *ey, cons
     if (P->collide) {
       int capped = 0;
       const int r = collide_one(P, pid[p], S, v, &capped);
-      if (r == 1) stats[0]++;
-      if (r == 2) stats[1]++;
-      stats[2] += capped;
+      if (r == 1) acc++;
+      if (r == 2) ref++;
+      cap += capped;
     }
     vx[p] = v[0]; vy[p] = v[1]; vz[p] = v[2];
     /* Move with stored result (intended; particles.c:679-704). */
@@ -200,8 +203,12 @@ void orc_push(const orc_push_params *P, const double *ex, const double *ey, cons
     orc_cell_coord(y[p], P->p0[1], P->d[1], P->N[1], &j, &f);
     orc_cell_coord(z[p], P->p0[2], P->d[2], P->N[2], &k, &f);
     key[p] = (int32_t)(i + P->N[0] * (j + P->N[1] * k));
-    Sn += orc_mean_rel_speed(v, P->vn0, P->vnT);
+    mrs[p] = orc_mean_rel_speed(v, P->vn0, P->vnT);
   }
+  double Sn = 0.0;
+  for (int64_t p = 0; p < np; p++) Sn += mrs[p];
+  free(mrs);
+  stats[0] = acc; stats[1] = ref; stats[2] = cap;
   *S_next = Sn;
 }
 
@@ -218,11 +225,13 @@ void orc_sort_perm(int64_t ncell, int64_t np, const int32_t *key, int64_t *perm,
 
 void orc_permute_d(int64_t np, const int64_t *perm, const double *in, double *out)
 {
+#pragma omp parallel for schedule(static) if (np > 65536)
   for (int64_t s = 0; s < np; s++) out[s] = in[perm[s]];
 }
 
 void orc_permute_u(int64_t np, const int64_t *perm, const uint32_t *in, uint32_t *out)
 {
+#pragma omp parallel for schedule(static) if (np > 65536)
   for (int64_t s = 0; s < np; s++) out[s] = in[perm[s]];
 }
 

@@ -6,8 +6,13 @@
 #include "orc.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Host threads of the parallel loops (results do not depend on them). */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 
 static int is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
@@ -58,27 +63,32 @@ void orc_fft3d(const int64_t N[3], double *data, int inverse)
   int64_t nmax = N[0];
   if (N[1] > nmax) nmax = N[1];
   if (N[2] > nmax) nmax = N[2];
-  double *re = malloc(sizeof(double) * nmax), *im = malloc(sizeof(double) * nmax);
-  double *tr = malloc(sizeof(double) * nmax), *ti = malloc(sizeof(double) * nmax);
   const int64_t str[3] = {1, N[0], N[0] * N[1]};
   const int64_t nc = N[0] * N[1] * N[2];
   for (int ax = 0; ax < 3; ax++) {
     const int64_t n = N[ax];
-    for (int64_t base = 0; base < nc; base++) {
-      /* lines start where the index along ax is 0 */
-      if ((base / str[ax]) % n != 0) continue;
-      for (int64_t t = 0; t < n; t++) {
-        re[t] = data[2 * (base + t * str[ax])];
-        im[t] = data[2 * (base + t * str[ax]) + 1];
+    /* the lines along ax are independent: one scratch set per thread */
+#pragma omp parallel if (nc > 16384)
+    {
+      double *re = malloc(sizeof(double) * nmax), *im = malloc(sizeof(double) * nmax);
+      double *tr = malloc(sizeof(double) * nmax), *ti = malloc(sizeof(double) * nmax);
+#pragma omp for schedule(static)
+      for (int64_t base = 0; base < nc; base++) {
+        /* lines start where the index along ax is 0 */
+        if ((base / str[ax]) % n != 0) continue;
+        for (int64_t t = 0; t < n; t++) {
+          re[t] = data[2 * (base + t * str[ax])];
+          im[t] = data[2 * (base + t * str[ax]) + 1];
+        }
+        dft_line(re, im, n, sign, tr, ti);
+        for (int64_t t = 0; t < n; t++) {
+          data[2 * (base + t * str[ax])] = re[t];
+          data[2 * (base + t * str[ax]) + 1] = im[t];
+        }
       }
-      dft_line(re, im, n, sign, tr, ti);
-      for (int64_t t = 0; t < n; t++) {
-        data[2 * (base + t * str[ax])] = re[t];
-        data[2 * (base + t * str[ax]) + 1] = im[t];
-      }
+      free(re); free(im); free(tr); free(ti);
     }
   }
-  free(re); free(im); free(tr); free(ti);
 }
 
 /* Row of the constant-coefficient operator A(nbar) (uniform density). */
@@ -171,6 +181,7 @@ static void zavg_coefficients(int lhs_type, const int64_t N[3], const double d[3
                               const double *n, double *C)
 {
   const int64_t nxy = N[0] * N[1];
+#pragma omp parallel for schedule(static) if (nxy > 1024)
   for (int64_t col = 0; col < nxy; col++) {
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int64_t k = 0; k < N[2]; k++) {
@@ -185,6 +196,7 @@ static void zavg_coefficients(int lhs_type, const int64_t N[3], const double d[3
 static void a2_apply(const int64_t N[3], const double *C, const double *e, double *y)
 {
   const int64_t nx = N[0], ny = N[1], nxy = nx * ny;
+#pragma omp parallel for schedule(static) if (nxy > 16384)
   for (int64_t col = 0; col < nxy; col++) {
     const int64_t i = col % nx, j = col / nx;
     double acc = 0.0;
@@ -421,6 +433,7 @@ void orc_pc_apply_deg(int pc, int lhs_type, const int64_t N[3], const double d[3
     double *w = malloc(sizeof(double) * nc), *C = malloc(sizeof(double) * 9 * nxy);
     double *r2 = calloc((size_t)nxy, sizeof(double)), *e2 = malloc(sizeof(double) * nxy);
     orc_lhs_apply(lhs_type, N, d, K, n, z, w);
+#pragma omp parallel for schedule(static) if (nxy > 1024)
     for (int64_t col = 0; col < nxy; col++) {
       double s = 0.0;
       for (int64_t k = 0; k < N[2]; k++) s += r[col + nxy * k] - w[col + nxy * k];
@@ -441,21 +454,26 @@ void orc_pc_apply_deg(int pc, int lhs_type, const int64_t N[3], const double d[3
   }
   if (pc == ORC_PC_ZLINE) {
     const int64_t m = N[2], plane = N[0] * N[1];
-    double *buf = malloc(sizeof(double) * (size_t)m * 5);
-    double *a = buf, *b = buf + m, *c = buf + 2 * m, *rr = buf + 3 * m, *x = buf + 4 * m;
-    for (int64_t col = 0; col < plane; col++) {
-      for (int64_t k = 0; k < m; k++) {
-        double v[19];
-        row_of(lhs_type, N, d, K, n, col + plane * k, v);
-        a[k] = v[5];   /* (0, 0, -1) */
-        b[k] = v[18];
-        c[k] = v[4];   /* (0, 0, +1) */
-        rr[k] = r[col + plane * k];
+    /* independent columns: one scratch set per thread */
+#pragma omp parallel if (nc > 16384)
+    {
+      double *buf = malloc(sizeof(double) * (size_t)m * 5);
+      double *a = buf, *b = buf + m, *c = buf + 2 * m, *rr = buf + 3 * m, *x = buf + 4 * m;
+#pragma omp for schedule(static)
+      for (int64_t col = 0; col < plane; col++) {
+        for (int64_t k = 0; k < m; k++) {
+          double v[19];
+          row_of(lhs_type, N, d, K, n, col + plane * k, v);
+          a[k] = v[5];   /* (0, 0, -1) */
+          b[k] = v[18];
+          c[k] = v[4];   /* (0, 0, +1) */
+          rr[k] = r[col + plane * k];
+        }
+        cyclic_solve(m, a, b, c, rr, x);
+        for (int64_t k = 0; k < m; k++) z[col + plane * k] = x[k];
       }
-      cyclic_solve(m, a, b, c, rr, x);
-      for (int64_t k = 0; k < m; k++) z[col + plane * k] = x[k];
+      free(buf);
     }
-    free(buf);
     orc_remove_mean(nc, z);
     return;
   }
@@ -508,6 +526,7 @@ void orc_pc_apply_deg(int pc, int lhs_type, const int64_t N[3], const double d[3
     double *buf = malloc(sizeof(double) * 2 * nc);
     for (int64_t c = 0; c < nc; c++) { buf[2 * c] = r[c]; buf[2 * c + 1] = 0.0; }
     orc_fft3d(N, buf, 0);
+#pragma omp parallel for schedule(static) if (nc > 16384)
     for (int64_t c = 0; c < nc; c++) {
       const int64_t kx = c % N[0], ky = (c / N[0]) % N[1], kz = c / (N[0] * N[1]);
       double lam = 0.0;
@@ -542,6 +561,7 @@ static void residual(int lhs_type, const int64_t N[3], const double d[3], const 
 {
   const int64_t nc = N[0] * N[1] * N[2];
   orc_lhs_apply(lhs_type, N, d, K, n, phi, r);
+#pragma omp parallel for schedule(static) if (nc > 65536)
   for (int64_t c = 0; c < nc; c++) r[c] = b[c] - r[c];
 }
 
@@ -573,10 +593,13 @@ static void gmres(int pc, int lhs_type, const int64_t N[3], const double d[3], c
       orc_pc_apply(pc, lhs_type, N, d, K, n, V + nc * j, zj);
       orc_lhs_apply(lhs_type, N, d, K, n, zj, w);
       /* two-pass classical Gram-Schmidt */
+      /* the updates of each element in q order, elements in parallel */
       for (int64_t q = 0; q <= j; q++) h[q] = dot(nc, V + nc * q, w);
-      for (int64_t q = 0; q <= j; q++) for (int64_t c = 0; c < nc; c++) w[c] -= h[q] * V[nc * q + c];
+#pragma omp parallel for schedule(static) if (nc > 65536)
+      for (int64_t c = 0; c < nc; c++) for (int64_t q = 0; q <= j; q++) w[c] -= h[q] * V[nc * q + c];
       for (int64_t q = 0; q <= j; q++) h2[q] = dot(nc, V + nc * q, w);
-      for (int64_t q = 0; q <= j; q++) for (int64_t c = 0; c < nc; c++) w[c] -= h2[q] * V[nc * q + c];
+#pragma omp parallel for schedule(static) if (nc > 65536)
+      for (int64_t c = 0; c < nc; c++) for (int64_t q = 0; q <= j; q++) w[c] -= h2[q] * V[nc * q + c];
       for (int64_t q = 0; q <= j; q++) h[q] += h2[q];
       h[j + 1] = sqrt(dot(nc, w, w));
       for (int64_t c = 0; c < nc; c++) V[nc * (j + 1) + c] = (h[j + 1] > 0.0) ? w[c] / h[j + 1] : 0.0;
@@ -600,7 +623,8 @@ static void gmres(int pc, int lhs_type, const int64_t N[3], const double d[3], c
       for (int64_t r = q + 1; r < j; r++) s -= H[q * m + r] * y[r];
       y[q] = s / H[q * m + q];
     }
-    for (int64_t q = 0; q < j; q++) for (int64_t c = 0; c < nc; c++) phi[c] += y[q] * Z[nc * q + c];
+#pragma omp parallel for schedule(static) if (nc > 65536)
+    for (int64_t c = 0; c < nc; c++) for (int64_t q = 0; q < j; q++) phi[c] += y[q] * Z[nc * q + c];
   }
   info[0] = it;
   free(V); free(Z); free(w); free(H); free(cs); free(sn); free(g); free(y); free(h); free(h2);

@@ -107,6 +107,36 @@ def test_hybrid_driver_runs_like_run_sh(tmp_path):
     assert p.returncode == 0 and "exits" in p.stdout
 
 
+def test_run_sh_builds_and_runs_the_driver(tmp_path):
+    """The two commands of the reference's run.sh (run.sh:163-188) with
+    srcdir = the package directory and bindir = its bin/: `make hybrid
+    CFLAGS=...` in srcdir, then `mpiexec -n 2 ${bindir}/hybrid --output
+    ${outname} ${extra}` from the run directory.  Rank 0 runs the problem on
+    the GPU, rank 1 exits at once; the run writes options.txt and its
+    outputs."""
+    import shutil
+
+    mpiexec = shutil.which("mpiexec") or ("/opt/conda/bin/mpiexec" if os.path.exists("/opt/conda/bin/mpiexec")
+                                          else None)
+    if mpiexec is None:
+        pytest.skip("no mpiexec on this host")
+    pkg = os.path.dirname(os.path.dirname(HYBRID))
+    p = subprocess.run(["make", "hybrid", "CFLAGS="], cwd=pkg, capture_output=True, text=True, timeout=900)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    extra = opts((8, 8, 8), extra="-Np 4096 -Nt 2 -output_interval 1 -seed 2")
+    p = subprocess.run([mpiexec, "-n", "2", os.path.join(pkg, "bin", "hybrid"), "--output", "run"] + extra,
+                       cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    assert "END" in p.stdout and "rank 1 exits" in p.stdout
+    assert (tmp_path / "options.txt").exists()
+    h5 = _hdf5()
+    for insert in ("-initial", "-0", "-1"):
+        if h5 is not None:
+            assert (tmp_path / f"run{insert}.hdf").exists(), insert
+        else:
+            assert (tmp_path / f"run{insert}.density.f64").exists(), insert
+
+
 def test_hybrid_driver_without_hdf5_writes_raw_files(tmp_path):
     """Without an HDF5 library the driver writes each field as a raw .f64 file
     and still exits 0 (the output calls report CHY_ERR_SUP, which is not a

@@ -1071,6 +1071,7 @@ int chy_copy_kernel(chy_ctx *ctx, int64_t nbytes, int repeat)
 {
   if (!ctx || nbytes <= 0 || repeat < 1) return CHY_ERR_ARG_WRONG;
   const int64_t n16 = (nbytes + 15) / 16;
+  if ((n16 + kBlock - 1) / kBlock > (int64_t)0x7fffffff) return set_err(ctx, CHY_ERR_ARG_RANGE, "copy too large");
   void *a = nullptr, *b = nullptr;
   CHY_HIP(hipMalloc(&a, n16 * 16));
   if (hipMalloc(&b, n16 * 16) != hipSuccess) { (void)hipFree(a); return set_err(ctx, CHY_ERR_MEM, "copy buffers"); }
@@ -1079,6 +1080,9 @@ int chy_copy_kernel(chy_ctx *ctx, int64_t nbytes, int repeat)
     timer_start(ctx, "copy");
     launch_copy(ctx, n16, a, b);
     timer_stop(ctx, "copy");
+    timer_start(ctx, "copy1");
+    launch_copy1(ctx, n16, a, b);
+    timer_stop(ctx, "copy1");
   }
   const hipError_t e = hipStreamSynchronize(ctx->stream);
   (void)hipFree(a);

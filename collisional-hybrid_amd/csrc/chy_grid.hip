@@ -1125,10 +1125,24 @@ __global__ __launch_bounds__(256) void k_copy(int64_t n16, const uint4 *__restri
     b[i] = a[i];
 }
 
+// One 16-byte element per thread, no loop: the device's streaming ceiling
+// (every wave retires after one load and one store).
+__global__ __launch_bounds__(256) void k_copy1(int64_t n16, const uint4 *__restrict__ a, uint4 *__restrict__ b)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n16) b[i] = a[i];
+}
+
 void launch_copy(chy_ctx *ctx, int64_t n16, const void *a, void *b)
 {
   const unsigned nb = (unsigned)std::min<int64_t>(16384, (n16 + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(k_copy, nb, kBlock, 0, ctx->stream, n16, (const uint4 *)a, (uint4 *)b);
+}
+
+void launch_copy1(chy_ctx *ctx, int64_t n16, const void *a, void *b)
+{
+  const unsigned nb = (unsigned)((n16 + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_copy1, nb, kBlock, 0, ctx->stream, n16, (const uint4 *)a, (uint4 *)b);
 }
 
 // ---- adjoint pieces of the grid operators (FP64) ------------------------------------

@@ -298,7 +298,8 @@ template <typename T>
 void launch_cheby(chy_ctx *ctx, int lhs_type, bool transpose, int degree, const T *n, const T *r, T *z, T *res,
                   T *dv, T *Ad);
 void launch_combine_sum(chy_ctx *ctx, const double *partials, int nblk, int nv, double *out);
-void launch_copy(chy_ctx *ctx, int64_t n16, const void *a, void *b);
+void launch_copy(chy_ctx *ctx, int64_t n16, const void *a, void *b);   // grid-stride, 16384 blocks
+void launch_copy1(chy_ctx *ctx, int64_t n16, const void *a, void *b);  // one element per thread
 RhsCoef rhs_coef(const chy_ctx *ctx);
 // adjoint pieces (FP64): out += alpha d(mu^T A(n) phi)/dn; transposed forcing;
 // central gradient of n into g3[3][nc]; transposed E-field stencil

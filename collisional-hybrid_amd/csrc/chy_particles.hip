@@ -104,8 +104,9 @@ __device__ __forceinline__ void flush_scatter_counts(unsigned long long *cs, uin
 // collide, the scattering, which runs in place: every ion's new state is
 // written by the same store instructions as its neighbours' (no scattered
 // write-back of the colliders).
+// At least 3 waves per SIMD: 168 registers (a few spills) against 178 at 2.
 template <bool COLLIDE, bool MOVE>
-__global__ __launch_bounds__(256) void k_push_fp64(Grid g, PushConst pc, int64_t np, PState<double> st,
+__global__ __launch_bounds__(256, 3) void k_push_fp64(Grid g, PushConst pc, int64_t np, PState<double> st,
                                                    const Vec4<double> *__restrict__ E4, const double *__restrict__ Sdev,
                                                    uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats,
                                                    double *__restrict__ cexp, uint8_t *__restrict__ dir)

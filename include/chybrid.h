@@ -337,8 +337,10 @@ int chy_timing_enable(chy_ctx *ctx, int on);
 int chy_timing_select(chy_ctx *ctx, const char *names);
 int chy_timing_get(chy_ctx *ctx, const char *name, double *total_ms, int64_t *count);
 int chy_timing_reset(chy_ctx *ctx);
-/* Plain device-to-device copy of nbytes through a simple vector kernel on the
- * context stream ("copy" timer); used to calibrate HBM byte counters. */
+/* Plain device-to-device copies of nbytes on the context stream, repeat
+ * times each: a grid-stride vector kernel over 16384 blocks ("copy" timer)
+ * and one 16-byte element per thread ("copy1" timer, the streaming
+ * ceiling); used to calibrate HBM byte counters and rooflines. */
 int chy_copy_kernel(chy_ctx *ctx, int64_t nbytes, int repeat);
 
 #ifdef __cplusplus

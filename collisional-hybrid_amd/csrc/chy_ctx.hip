@@ -556,6 +556,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   A(dalloc(ctx, &ctx->nbase, 27 * nc));
   A(dalloc(ctx, &ctx->offs_alt, nc + 1));
   A(dalloc(ctx, &ctx->sortflag, 2));
+  launch_fill<uint32_t>(ctx, 2, ctx->sortflag, 0u);  // [1]: placements lost, sticky until read
   {
     unsigned char *pp = nullptr;
     A(dalloc(ctx, &pp, (int64_t)(32 * ts) * nc));

@@ -113,6 +113,12 @@ __device__ __forceinline__ double wrap_pos(double x, double p0, double p1, doubl
   return x;
 }
 
+// a*b + c: one fused multiply-add in FP32 (the mixed path, compared with the
+// oracle to a tolerance); in FP64 a separate multiply and add, rounded as the
+// oracle rounds them (the build turns off FMA contraction).
+__device__ __forceinline__ float fmad(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fmad(double a, double b, double c) { return a * b + c; }
+
 // Corner weights W_q = (wz*wy)*wx for q = dx + 2 dy + 4 dz.
 template <typename T>
 __device__ __forceinline__ void corner_weights(T fx, T fy, T fz, T W[8])

@@ -454,7 +454,8 @@ __global__ __launch_bounds__(256) void k_tile_scan(int64_t n, const uint32_t *__
 
 // Scatter: slot = cursor[key]++ (run-aggregated), perm[slot] = p.
 __global__ __launch_bounds__(256) void k_scatter(int64_t np, const int32_t *__restrict__ cell, uint32_t *__restrict__ cursor,
-                                                 uint32_t *__restrict__ perm, int32_t *__restrict__ slotkey)
+                                                 uint32_t *__restrict__ perm, int32_t *__restrict__ slotkey,
+                                                 uint32_t *__restrict__ lost)
 {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t iters = (np + stride - 1) / stride;
@@ -470,6 +471,8 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t np, const int32_t *__re
     if (key >= 0 && slot < np) {  // (the host has checked the histogram's total)
       perm[slot] = (uint32_t)p;
       slotkey[slot] = key;
+    } else if (key >= 0) {
+      atomicOr(lost, 1u);  // a slot outside the ion range: reported by the next sort
     }
   }
 }
@@ -652,7 +655,8 @@ __global__ __launch_bounds__(256) void k_counts(Grid g, const uint8_t *__restric
 
 __global__ __launch_bounds__(256) void k_perm(Grid g, int64_t np, const int32_t *__restrict__ cell,
                                               const uint16_t *__restrict__ info, const uint16_t *__restrict__ base,
-                                              const uint32_t *__restrict__ offs, uint32_t *__restrict__ perm)
+                                              const uint32_t *__restrict__ offs, uint32_t *__restrict__ perm,
+                                              uint32_t *__restrict__ lost)
 {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t p = xcd_block() * blockDim.x + threadIdx.x; p < np; p += stride) {
@@ -664,6 +668,7 @@ __global__ __launch_bounds__(256) void k_perm(Grid g, int64_t np, const int32_t 
     const int64_t co = nb_cell(g, i - (d % 3 - 1), j - ((d / 3) % 3 - 1), k - (d / 9 - 1));  // old cell
     const uint32_t slot = offs[c] + base[(int64_t)d * g.nc + co] + (inf & 255u);
     if (slot < np) perm[slot] = (uint32_t)p;
+    else atomicOr(lost, 1u);  // a slot outside the ion range: reported by the next sort
   }
 }
 
@@ -895,9 +900,9 @@ __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *_
       const T W = (wz * wy) * wx;
       const T a = weighted ? r.w * W : W;
       acc[0] += a;
-      acc[1] += a * r.vx;
-      acc[2] += a * r.vy;
-      acc[3] += a * r.vz;
+      acc[1] = fmad(a, r.vx, acc[1]);
+      acc[2] = fmad(a, r.vy, acc[2]);
+      acc[3] = fmad(a, r.vz, acc[3]);
     };
     uint32_t s = a0;
     for (; s + 4 <= a1; s += 4) {
@@ -1126,6 +1131,16 @@ static unsigned grid_for(int64_t n, unsigned cap = 4096)
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, cap));
 }
 
+// A placement kernel (k_perm, k_scatter) computed a slot outside the ion
+// range on an earlier sort: the ion order is not a permutation, so the state
+// is lost.  The flag is read with the next sort's other read-backs.
+static int lost_placement(chy_ctx *ctx)
+{
+  launch_fill<uint32_t>(ctx, 1, ctx->sortflag + 1, 0u);
+  return set_err(ctx, CHY_ERR_LIB, "sort: an earlier placement put an ion outside the ion range; the particle state is "
+                                   "not a permutation of the ions");
+}
+
 // The neighbour-move sort (see k_trans); false when it does not apply and the
 // general sort must run (counts[] is then filled here).
 static int sort_neighbour(chy_ctx *ctx, bool &done)
@@ -1135,7 +1150,7 @@ static int sort_neighbour(chy_ctx *ctx, bool &done)
   const int64_t nt = (nc + kScanTile - 1) / kScanTile;
   const int32_t *cell = ctx->mixed ? ctx->pf[ctx->cur].cell : ctx->pd[ctx->cur].cell;
   const unsigned ncb = (unsigned)((nc + 255) / 256);
-  launch_fill<uint32_t>(ctx, 2, ctx->sortflag, 0u);
+  launch_fill<uint32_t>(ctx, 1, ctx->sortflag, 0u);  // word 1 (placements lost) stays until read
   launch_fill<uint32_t>(ctx, 1, ctx->maxcount, 0u);
   // the group sizes and ranks from the push's move directions (k_trans_dir,
   // its LDS staging sized from the mean occupancy) or from the new keys (k_trans)
@@ -1162,11 +1177,13 @@ static int sort_neighbour(chy_ctx *ctx, bool &done)
   hipLaunchKernelGGL(k_tile_scan, (unsigned)nt, kBlock, 0, ctx->stream, nc, ctx->counts, ctx->scan_tmp, ctx->offs_alt,
                      ctx->cursor, (uint32_t)np);
   timer_stop(ctx, "sort_scan");
-  uint32_t flag = 0, mx = 0, total = 0;
+  uint32_t flag = 0, mx = 0, total = 0, lost = 0;
   CHY_HIP(hipMemcpyAsync(&flag, ctx->sortflag, sizeof(flag), hipMemcpyDeviceToHost, ctx->stream));
+  CHY_HIP(hipMemcpyAsync(&lost, ctx->sortflag + 1, sizeof(lost), hipMemcpyDeviceToHost, ctx->stream));
   CHY_HIP(hipMemcpyAsync(&mx, ctx->maxcount, sizeof(mx), hipMemcpyDeviceToHost, ctx->stream));
   CHY_HIP(hipMemcpyAsync(&total, ctx->scan_tmp + nt, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
   CHY_HIP(hipStreamSynchronize(ctx->stream));
+  if (lost) return lost_placement(ctx);
   if (flag || mx > kMaxCellOccupancy || total != (uint32_t)np) {
     launch_fill<uint32_t>(ctx, nc + 1, ctx->counts, 0u);
     hipLaunchKernelGGL(k_keys_count, grid_for(np), kBlock, 0, ctx->stream, np, cell, ctx->counts);
@@ -1175,7 +1192,7 @@ static int sort_neighbour(chy_ctx *ctx, bool &done)
   const unsigned nb = grid_for(np);
   timer_start(ctx, "sort_scatter");
   hipLaunchKernelGGL(k_perm, nb, kBlock, 0, ctx->stream, ctx->g, np, cell, ctx->info, ctx->nbase, ctx->offs_alt,
-                     ctx->perm);
+                     ctx->perm, ctx->sortflag + 1);
   timer_stop(ctx, "sort_scatter");
   const chy_params &p = ctx->prm;
   const bool sums = p.mn > 0.0;  // the collision normaliser is needed
@@ -1225,10 +1242,12 @@ int particles_sort(chy_ctx *ctx)
   timer_stop(ctx, "sort_scan");
   // the stable per-cell rank is quadratic in the cell occupancy: bound it; and
   // the histogram must account for every ion before the scatter uses it
-  uint32_t mx = 0, total = 0;
+  uint32_t mx = 0, total = 0, lost = 0;
   CHY_HIP(hipMemcpyAsync(&mx, ctx->maxcount, sizeof(mx), hipMemcpyDeviceToHost, ctx->stream));
   CHY_HIP(hipMemcpyAsync(&total, ctx->scan_tmp + nt, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
+  CHY_HIP(hipMemcpyAsync(&lost, ctx->sortflag + 1, sizeof(lost), hipMemcpyDeviceToHost, ctx->stream));
   CHY_HIP(hipStreamSynchronize(ctx->stream));
+  if (lost) return lost_placement(ctx);
   if (total != (uint32_t)np)
     return set_err(ctx, CHY_ERR_LIB, "sort: the cell histogram holds " + std::to_string(total) + " ions, not " +
                                          std::to_string(np));
@@ -1237,7 +1256,8 @@ int particles_sort(chy_ctx *ctx)
                                          std::to_string(kMaxCellOccupancy) + " are supported");
   const int32_t *cell = ctx->mixed ? ctx->pf[ctx->cur].cell : ctx->pd[ctx->cur].cell;
   timer_start(ctx, "sort_scatter");
-  hipLaunchKernelGGL(k_scatter, grid_for(np), kBlock, 0, ctx->stream, np, cell, ctx->cursor, ctx->perm, ctx->slotkey);
+  hipLaunchKernelGGL(k_scatter, grid_for(np), kBlock, 0, ctx->stream, np, cell, ctx->cursor, ctx->perm, ctx->slotkey,
+                     ctx->sortflag + 1);
   timer_stop(ctx, "sort_scatter");
   const unsigned nb = grid_for(np);
   const chy_params &p = ctx->prm;

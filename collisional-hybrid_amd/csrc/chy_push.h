@@ -218,16 +218,17 @@ __device__ __forceinline__ void boris(const PC &pc, T e0, T e1, T e2, T &vx, T &
 {
   const T tau = (T)pc.tau, t0 = (T)pc.t[0], t1 = (T)pc.t[1], t2 = (T)pc.t[2];
   const T s0 = (T)pc.s[0], s1 = (T)pc.s[1], s2 = (T)pc.s[2];
-  const T vm0 = vx + tau * e0, vm1 = vy + tau * e1, vm2 = vz + tau * e2;
-  const T vp0 = vm0 + (vm1 * t2 - vm2 * t1);
-  const T vp1 = vm1 + (vm2 * t0 - vm0 * t2);
-  const T vp2 = vm2 + (vm0 * t1 - vm1 * t0);
-  const T vq0 = vm0 + (vp1 * s2 - vp2 * s1);
-  const T vq1 = vm1 + (vp2 * s0 - vp0 * s2);
-  const T vq2 = vm2 + (vp0 * s1 - vp1 * s0);
-  vx = vq0 + tau * e0;
-  vy = vq1 + tau * e1;
-  vz = vq2 + tau * e2;
+  // fmad(a, b, -c) is a*b - c and fmad(tau, e, v) is v + tau*e exactly in FP64
+  const T vm0 = fmad(tau, e0, vx), vm1 = fmad(tau, e1, vy), vm2 = fmad(tau, e2, vz);
+  const T vp0 = vm0 + fmad(vm1, t2, -(vm2 * t1));
+  const T vp1 = vm1 + fmad(vm2, t0, -(vm0 * t2));
+  const T vp2 = vm2 + fmad(vm0, t1, -(vm1 * t0));
+  const T vq0 = vm0 + fmad(vp1, s2, -(vp2 * s1));
+  const T vq1 = vm1 + fmad(vp2, s0, -(vp0 * s2));
+  const T vq2 = vm2 + fmad(vp0, s1, -(vp1 * s0));
+  vx = fmad(tau, e0, vq0);
+  vy = fmad(tau, e1, vq1);
+  vz = fmad(tau, e2, vq2);
 }
 
 // Trilinear gather from the interleaved node field, corner order q = 0..7.
@@ -241,9 +242,9 @@ __device__ __forceinline__ void gather(const Grid &g, const Vec4<T> *__restrict_
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     const Vec4<T> e = E4[lin(g, i + (q & 1), j + ((q >> 1) & 1), k + (q >> 2))];
-    e0 += W[q] * e.x;
-    e1 += W[q] * e.y;
-    e2 += W[q] * e.z;
+    e0 = fmad(W[q], e.x, e0);
+    e1 = fmad(W[q], e.y, e1);
+    e2 = fmad(W[q], e.z, e2);
   }
 }
 
@@ -262,9 +263,9 @@ __device__ __forceinline__ void gather_c(const Grid &g, const Vec4<T> *__restric
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     const Vec4<T> e = E4[c + ((q & 1) ? ox : 0) + ((q & 2) ? oy : 0) + ((q & 4) ? oz : 0)];
-    e0 += W[q] * e.x;
-    e1 += W[q] * e.y;
-    e2 += W[q] * e.z;
+    e0 = fmad(W[q], e.x, e0);
+    e1 = fmad(W[q], e.y, e1);
+    e2 = fmad(W[q], e.z, e2);
   }
 }
 

@@ -128,6 +128,31 @@ __global__ __launch_bounds__(256) void k_lhs_apply(Grid g, int lhs_type, SCoef s
   y[c] = (TY)acc;
 }
 
+// res = b - A x, one thread per node (grids too small for the tiled kernel's
+// tiles to fill the chip: the multigrid's coarse levels); the row sum as
+// k_lhs_apply forms it.
+template <typename T, typename C, bool DIFF>
+__global__ __launch_bounds__(256) void k_residual_node(Grid g, int lhs_type, SCoef s, LapRow lap,
+                                                       const T *__restrict__ n, const T *__restrict__ b,
+                                                       const T *__restrict__ x, T *__restrict__ res)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  C v[19];
+  row_coeffs<C, T>(g, lhs_type, s, lap.v, n, i, j, k, v);
+  C acc = (C)0;
+  if (DIFF) {
+    const C xc = (C)x[c];
+#pragma unroll
+    for (int q = 0; q < 18; q++) acc += v[q] * ((C)x[lin(g, i + kOff[q][0], j + kOff[q][1], k + kOff[q][2])] - xc);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 19; q++) acc += v[q] * (C)x[lin(g, i + kOff[q][0], j + kOff[q][1], k + kOff[q][2])];
+  }
+  res[c] = (T)((C)b[c] - acc);
+}
+
 // y = A^T x: (A^T x)_c = sum_q A[c - o_q, c] x[c - o_q]; the coefficient of
 // column c in row r = c - o_q is that row's entry q.
 template <typename TN, typename TX, typename TY, typename C>
@@ -1489,6 +1514,99 @@ __global__ __launch_bounds__(256) void k_zline_apply(Grid g, const T *__restrict
   }
 }
 
+// The same z-line solve with one wave per column, for grids of few columns
+// (the multigrid's coarse levels), where one thread per column leaves the
+// chip idle and each call costs the latency of walking the column three
+// times.  The forward and back substitutions are first-order linear
+// recurrences, y_k = alpha_k y_(k-1) + beta_k: each lane composes the affine
+// maps of its P consecutive planes, a wave scan (6 shuffle steps) gives every
+// lane the value entering its planes, and the lane then runs its planes with
+// the sequential kernel's formulas.  Intermediate values are rounded to T
+// where the sequential kernel stores them.  The scan reassociates the
+// products, so results agree with k_zline_apply to rounding, not bitwise.
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_zline_apply_wave(Grid g, const T *__restrict__ za, const T *__restrict__ zi,
+                                                          const T *__restrict__ zc, const T *__restrict__ zq,
+                                                          const double *__restrict__ cag,
+                                                          const double *__restrict__ cden, const T *__restrict__ res,
+                                                          T *__restrict__ z, double scale)
+{
+  const int64_t plane = (int64_t)g.nx * g.ny;
+  const int64_t col = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int L = (int)(threadIdx.x & 63), m = g.nz;
+  if (col >= plane) return;  // the wave's column: the whole wave leaves together
+  double r[P], a[P], iv[P], c[P], y[P];
+#pragma unroll
+  for (int u = 0; u < P; u++) {
+    const int k = L * P + u;
+    r[u] = a[u] = iv[u] = c[u] = 0.0;
+    if (k < m) {
+      const int64_t e = col + plane * k;
+      r[u] = (double)res[e]; a[u] = (double)za[e]; iv[u] = (double)zi[e]; c[u] = (double)zc[e];
+    }
+  }
+  // forward: y_k = (r_k - a_k y_(k-1)) i_k, y_(-1) = 0; the lane's map y -> A y + B
+  double A = 1.0, B = 0.0;
+#pragma unroll
+  for (int u = 0; u < P; u++) {
+    if (L * P + u < m) {
+      const double al = -a[u] * iv[u], be = r[u] * iv[u];
+      A = al * A;
+      B = al * B + be;
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {  // inclusive scan: lower lanes' maps first
+    const double A2 = __shfl_up(A, off, 64), B2 = __shfl_up(B, off, 64);
+    if (L >= off) { B = A * B2 + B; A = A * A2; }
+  }
+  double yin = __shfl_up(B, 1, 64);  // the exclusive prefix applied to y_(-1) = 0
+  if (L == 0) yin = 0.0;
+  double yv = yin;
+#pragma unroll
+  for (int u = 0; u < P; u++) {
+    if (L * P + u < m) {
+      yv = (r[u] - a[u] * yv) * iv[u];
+      y[u] = (double)(T)yv;  // as the sequential kernel stores it
+    } else {
+      y[u] = 0.0;
+    }
+  }
+  const double ylast = __shfl(yv, (m - 1) / P, 64);  // y_(m-1), unrounded
+  // backward: x_(m-1) = y_(m-1), x_k = y_k - c_k x_(k+1); higher planes first
+  A = 1.0; B = 0.0;
+#pragma unroll
+  for (int u = P - 1; u >= 0; u--) {
+    const int k = L * P + u;
+    if (k == m - 1) { A = 0.0; B = ylast; }
+    else if (k < m - 1) { B = -c[u] * B + y[u]; A = -c[u] * A; }
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {  // higher lanes' maps first
+    const double A2 = __shfl_down(A, off, 64), B2 = __shfl_down(B, off, 64);
+    if (L + off < 64) { B = A * B2 + B; A = A * A2; }
+  }
+  double xin = __shfl_down(B, 1, 64);
+  if (L == 63) xin = 0.0;
+  double xv = xin;
+#pragma unroll
+  for (int u = P - 1; u >= 0; u--) {
+    const int k = L * P + u;
+    if (k == m - 1) xv = ylast;  // y[u] keeps y_(m-1) as stored
+    else if (k < m - 1) { xv = y[u] - c[u] * xv; y[u] = (double)(T)xv; }
+  }
+  const double x0 = __shfl(xv, 0, 64);  // x_0, unrounded (lane 0's last step is plane 0)
+  const double f = (x0 + cag[col] * ylast) / cden[col];
+#pragma unroll
+  for (int u = 0; u < P; u++) {
+    const int k = L * P + u;
+    if (k < m) {
+      const int64_t e = col + plane * k;
+      z[e] = (T)((double)z[e] + scale * (y[u] - f * (double)zq[e]));
+    }
+  }
+}
+
 // Both once-per-solve column walks of the two-level preconditioners for the
 // full operator (COEF: k_zcoef's z-means, FACT: k_zline_factor's factors,
 // the same arithmetic in the same order), in one walk up each column with the
@@ -1621,6 +1739,33 @@ void launch_zline_apply_g(chy_ctx *ctx, const Grid &g, const T *zbuf, const doub
                      zbuf + 2 * nc, zbuf + 3 * nc, colbuf, colbuf + plane, res, z, scale);
 }
 
+// The wave-per-column form for grids of at most kZlineWaveCols columns and 3 to
+// 512 planes (the multigrid's coarse levels); otherwise the thread-per-column
+// one.  A wave's loads along its column touch one cache line per value, so
+// above a few thousand columns the thread-per-column form, whose loads are
+// coalesced across columns, is faster despite its serial walk.
+constexpr int64_t kZlineWaveCols = 4096;
+template <typename T>
+void launch_zline_apply_coarse(chy_ctx *ctx, const Grid &g, const T *zbuf, const double *colbuf, T *res, T *z,
+                               double scale)
+{
+  const int64_t nc = g.nc, plane = (int64_t)g.nx * g.ny;
+  const int m = g.nz;
+  if (plane > kZlineWaveCols || m < 3 || m > 512) {
+    launch_zline_apply_g<T>(ctx, g, zbuf, colbuf, res, z, scale);
+    return;
+  }
+  const unsigned nb = (unsigned)((plane + 3) / 4);
+#define ZW(P)                                                                                                      \
+  hipLaunchKernelGGL((k_zline_apply_wave<T, P>), nb, 256, 0, ctx->stream, g, zbuf, zbuf + nc, zbuf + 2 * nc,       \
+                     zbuf + 3 * nc, colbuf, colbuf + plane, (const T *)res, z, scale)
+  if (m <= 64) ZW(1);
+  else if (m <= 128) ZW(2);
+  else if (m <= 256) ZW(4);
+  else ZW(8);
+#undef ZW
+}
+
 template <typename T>
 void launch_zline_factor_g(chy_ctx *ctx, const Grid &g, const double d[3], int lhs_type, const T *n, T *zbuf,
                            double *colbuf)
@@ -1643,6 +1788,11 @@ void launch_residual_g(chy_ctx *ctx, const Grid &g, const double d[3], int lhs_t
   uniform_row(lhs_type == CHY_LHS_FULL ? CHY_LHS_IDENTITY : lhs_type, d, ctx->prm.kappa_e, 0.0, L.v);
   constexpr bool fp32 = sizeof(T) == 4;
   using C = typename std::conditional<fp32, float, double>::type;
+  if ((int64_t)g.nx * g.ny <= 16384) {  // few x-y tiles: one thread per node fills the chip better
+    hipLaunchKernelGGL((k_residual_node<T, C, fp32>), nblocks(g.nc), kBlock, 0, ctx->stream, g, lhs_type, s, L, n,
+                       b, x, res);
+    return;
+  }
   launch_tiled_on<T, T, T, C, fp32, 1, T>(ctx, g, s, L, lhs_type, n, x, b, res, (double *)nullptr);
 }
 
@@ -1671,6 +1821,10 @@ template void launch_zline_apply_g<double>(chy_ctx *, const Grid &, const double
                                            double *, double);
 template void launch_zline_apply_g<float>(chy_ctx *, const Grid &, const float *, const double *, float *, float *,
                                           double);
+template void launch_zline_apply_coarse<double>(chy_ctx *, const Grid &, const double *, const double *, double *,
+                                                double *, double);
+template void launch_zline_apply_coarse<float>(chy_ctx *, const Grid &, const float *, const double *, float *,
+                                               float *, double);
 template void launch_zline_factor_g<double>(chy_ctx *, const Grid &, const double *, int, const double *, double *,
                                             double *);
 template void launch_zline_factor_g<float>(chy_ctx *, const Grid &, const double *, int, const float *, float *,

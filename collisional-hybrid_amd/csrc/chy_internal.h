@@ -377,6 +377,10 @@ void launch_zline_factor_g(chy_ctx *ctx, const Grid &g, const double d[3], int l
 template <typename T>
 void launch_zline_apply_g(chy_ctx *ctx, const Grid &g, const T *zbuf, const double *colbuf, T *res, T *z,
                           double scale);
+// the same, one wave per column on grids of few columns (res is only read)
+template <typename T>
+void launch_zline_apply_coarse(chy_ctx *ctx, const Grid &g, const T *zbuf, const double *colbuf, T *res, T *z,
+                               double scale);
 template <typename T>
 void launch_residual_g(chy_ctx *ctx, const Grid &g, const double d[3], int lhs_type, const T *n, const T *b,
                        const T *x, T *res);

@@ -96,13 +96,14 @@ void vcycle(chy_ctx *ctx, Mg &M, int l, const T *b, T *x)
   const int lt = ctx->prm.lhs_type;
   const T *n = (const T *)L.n;
   T *r = (T *)L.r, *zf = (T *)L.zf;
-  // first sweep from x = 0: x = omega zline^-1 b (the solve overwrites its input)
+  // first sweep from x = 0: x = omega zline^-1 b (the thread-per-column solve
+  // overwrites its input, so it gets a copy)
   launch_fill<T>(ctx, g.nc, x, (T)0);
   (void)hipMemcpyAsync(r, b, sizeof(T) * g.nc, hipMemcpyDeviceToDevice, ctx->stream);
-  launch_zline_apply_g<T>(ctx, g, zf, L.zcol, r, x, kMgOmega);
+  launch_zline_apply_coarse<T>(ctx, g, zf, L.zcol, r, x, kMgOmega);
   auto sweep = [&]() {
     launch_residual_g<T>(ctx, g, L.d, lt, n, b, x, r);
-    launch_zline_apply_g<T>(ctx, g, zf, L.zcol, r, x, kMgOmega);
+    launch_zline_apply_coarse<T>(ctx, g, zf, L.zcol, r, x, kMgOmega);
   };
   const bool coarsest = l == M.nlev - 1;
   for (int s = 1; s < (coarsest ? kMgCoarseSweeps : kMgSweeps); s++) sweep();

@@ -222,10 +222,15 @@ PCS = {"jacobi": orc.PC_JACOBI, "fft": orc.PC_FFT, "zline": orc.PC_ZLINE, "cheby
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("pc,N", [("jacobi", (16, 8, 12)), ("fft", (16, 8, 32)), ("fft2", (16, 8, 32)),
                                   ("fft2", (32, 16, 8)), ("fft2z", (16, 8, 32)), ("fft2z", (32, 16, 8)),
+                                  # z-line correction on long columns: 8 and 16 register segments
+                                  ("fft2z", (16, 8, 256)), ("fft2z", (8, 8, 512)),
                                   ("zline", (16, 8, 12)),
                                   ("zline", (8, 6, 3)), ("zline", (6, 4, 5)), ("cheby", (16, 8, 12)),
                                   # multigrid: 2 levels (16x8, 8x4), 3 levels (24x20, 12x10, 6x5)
-                                  ("mg", (16, 8, 12)), ("mg", (24, 20, 9))])
+                                  ("mg", (16, 8, 12)), ("mg", (24, 20, 9)),
+                                  # a fine level of 4608 columns of 40 planes: register segments,
+                                  # the top one partly filled
+                                  ("mg", (72, 64, 40))])
 def test_pc_apply_matches_oracle(pc, N, prec):
     """chy_pc_apply (the PCShell apply) against the oracle's preconditioners,
     modulo the constant that A(n) annihilates."""
@@ -241,7 +246,9 @@ def test_pc_apply_matches_oracle(pc, N, prec):
     n = mom[0].astype(np.float32).astype(np.float64) if prec == "mixed" else mom[0]
     ref = orc.pc_apply(g, K, n, r, pc=PCS[pc], degree=deg)
     z = z - z.mean()
-    tol = 1e-12 if prec == "fp64" else (1e-4 if pc == "mg" else 2e-5)
+    # FP64: rounding of the FFT and of the z-line recurrences grows with the
+    # line length (512-plane columns: 1.4e-12)
+    tol = (1e-12 if max(N) <= 256 else 4e-12) if prec == "fp64" else (1e-4 if pc == "mg" else 2e-5)
     assert relmax(z, ref) <= tol, relmax(z, ref)
 
 

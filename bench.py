@@ -41,8 +41,10 @@ COPY_BYTES = 1 << 30
 KERNEL_OF = {
     "mixed": {"efield": "k_efield<float>", "push": "k_push_mixed<true, true>",
               "sort_trans": "k_trans_dir + k_counts", "sort_scatter": "k_perm",
-              "sort_reorder": "k_cell_partials<float, false, true, true> (slot-order gather + deposit partials)",
-              "deposit_nodes": "k_node_sum<float>",
+              "sort_reorder": "k_cell_partials<float, false, true, true, true> "
+                              "(slot-order gather + x-combined deposit partials)",
+              "deposit_cells": "k_cell_partials<float, false, false, false, true>",
+              "deposit_nodes": "k_node_sum_xc<float>",
               "lhs_apply": "k_lhs_tiled<float, float, float, float, true, 0, double, 64, true>"},
     "fp64": {"efield": "k_efield<double>", "push": "k_push_fp64<true, true>",
              "sort_trans": "k_trans_dir + k_counts", "sort_scatter": "k_perm",
@@ -69,11 +71,24 @@ def workload_args(N, ppc, prec):
             + PHYS).split() + SOLVER_OPTS
 
 
+def _pc_type():
+    """The preconditioner the timed runs use (fft2z unless --solver-opts names another)."""
+    o = SOLVER_OPTS
+    for i, a in enumerate(o[:-1]):
+        if a == "-pc_type":
+            return o[i + 1]
+    return "fft2z"
+
+
 # Algorithmic bytes per launch of each kernel: every distinct array argument
 # read once and every output written once at its stored precision.
 # (np = ions, nc = cells = nodes); weights are present in this workload.
-def algorithmic_bytes(kernel, prec, np_, nc):
+# nx: the mixed deposit's cell partials are x-combined (16 values per cell and
+# 16 per 32-cell block) when nx is a multiple of 32, else 32 values per cell.
+def algorithmic_bytes(kernel, prec, np_, nc, nx=None):
     s = 4 if prec == "mixed" else 8
+    xc = prec == "mixed" and nx is not None and nx % 32 == 0
+    part = (16 * s + 16 * s / 32) if xc else 32 * s  # cell partials per cell
     if kernel == "push":
         # state in (position 3s [+ cell 4 in mixed], v 3s, pid 4), state out
         # (position 3s + cell 4, v 3s, the move direction 1), node E 3s per node
@@ -91,19 +106,26 @@ def algorithmic_bytes(kernel, prec, np_, nc):
         # perm in, state gathered and written in slot order, and the deposit's
         # cell partials (8 corners x {w, w v}) made from it
         payload = 3 * s + 3 * s + s + 4 + 4          # position, v, weight, pid, cell
-        return np_ * (4 + payload + payload) + nc * (4 + 32 * s)
+        return np_ * (4 + payload + payload) + nc * (4 + part)
     if kernel == "sort_scan":
         return nc * (4 + 4 + 4)
     if kernel == "deposit_cells":
-        return np_ * (3 * s + 3 * s + s) + nc * (4 + 32 * s)
+        return np_ * (3 * s + 3 * s + s) + nc * (4 + part)
     if kernel == "deposit_nodes":
-        return nc * (32 * s + 4 * s)
+        return nc * (part + 4 * s)
     if kernel == "lhs_apply":
         return nc * 3 * s
     if kernel == "pc":
-        # 5 FFT passes; the x pass is real-to-complex, so each pass reads and
-        # writes one real-field-sized array (half spectrum of nx/2 complex)
-        return nc * 5 * (s + s)
+        if _pc_type() != "fft2z":
+            return None
+        # fft2z, per application, the sum of its passes' own reads and writes:
+        # 5 FFT passes (each a real-field-sized array in and out: the half
+        # spectrum of nx/2 complex) 10s; the z-summed residual (n, z, r in) 3s;
+        # the 2-D correction's extension (z in and out) 2s; the residual
+        # (n, z, r in, r' out) 4s; the z-line solve (r, 4 factors, z in; the
+        # intermediate out and in twice; z out) 11s.  The 2-D (x-y plane)
+        # work is left out.
+        return nc * (10 + 3 + 2 + 4 + 11) * s
     if kernel == "efield":
         return nc * (8 + 3 * s)
     if kernel == "rhs":
@@ -198,7 +220,7 @@ def run_gpu(N, ppc, prec, steps, warmup, keep=False, cross=None):
         ctx.close()
     nc = N ** 3
     for nme, k in kern.items():
-        b = algorithmic_bytes(nme, prec, np_, nc)
+        b = algorithmic_bytes(nme, prec, np_, nc, nx=N)
         if b is not None:
             k["bytes"] = b
             k["GBs"] = b / (k["avg_ms"] * 1e-3) / 1e9
@@ -526,7 +548,7 @@ def main():
     kern = mixed["kernels"]
     rk = mixed["roofline_kernel"]
     dom = rk["name"]
-    achieved = algorithmic_bytes(dom, "mixed", mixed["np"], a.grid ** 3) / (rk["avg_ms"] * 1e-3) / 1e9
+    achieved = algorithmic_bytes(dom, "mixed", mixed["np"], a.grid ** 3, nx=a.grid) / (rk["avg_ms"] * 1e-3) / 1e9
     roof = {
         "kernel": KERNEL_OF["mixed"].get(dom, dom),
         "timer": dom,

@@ -128,9 +128,16 @@ int pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
   return rc;
 }
 
-// Read nv doubles from the device reduction buffer (one sync).
+// Read nv doubles from the device reduction buffer (one sync).  Values in
+// red_out are already in its host mirror once the stream has drained.
 int fetch(chy_ctx *ctx, const double *dev, int nv, double *host)
 {
+  if (ctx->red_mirror && dev >= ctx->red_out && dev + nv <= ctx->red_out + ctx->red_out_n) {
+    CHY_HIP(hipStreamSynchronize(ctx->stream));
+    const double *src = ctx->h_red + (dev - ctx->red_out);
+    if (src != host) std::memcpy(host, src, sizeof(double) * nv);
+    return CHY_OK;
+  }
   CHY_HIP(hipMemcpyAsync(host, dev, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream));
   CHY_HIP(hipStreamSynchronize(ctx->stream));
   return CHY_OK;
@@ -157,7 +164,7 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
   // Hessenberg entries.  The preconditioned directions follow:
   // Z_q = sc[q] (M^-1 U_q).
   std::vector<double> sc(m + 2, 0.0);
-  double *sc2 = ctx->h_scale;  // pinned: sc[q]^2, copied to ctx->gs_scale
+  double *sc2 = ctx->h_scale;  // sc[q]^2, passed by value to the combines
   launch_fill<T>(ctx, nc, e, (T)0);
   double beta0 = -1.0, beta_prev = -1.0;
   *its = 0;
@@ -221,11 +228,10 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
       // with |w|^2 = |w'|^2 + sum h^2)
       const int refine = ctx->prm.ksp_gmres_cgs_refinement;
       const double sj = sc[j];
-      CHY_HIP(hipMemcpyAsync(ctx->gs_scale, sc2, sizeof(double) * (j + 1), hipMemcpyHostToDevice, ctx->stream));
       double wn;  // |U_{j+1}| before scaling
-      launch_gs_dots<T>(ctx, (int)(j + 1), V, nc, Vn, nc, hdev, ctx->gs_scale);
+      launch_gs_dots<T>(ctx, (int)(j + 1), V, nc, Vn, nc, hdev, sc2);
       if (refine == CHY_CGS_REFINE_ALWAYS) {
-        launch_gs_update_dots<T>(ctx, (int)(j + 1), V, nc, hdev, Vn, nc, h2dev, ctx->gs_scale);
+        launch_gs_update_dots<T>(ctx, (int)(j + 1), V, nc, hdev, Vn, nc, h2dev, sc2);
         launch_gs_update_norm<T>(ctx, (int)(j + 1), V, nc, h2dev, Vn, nc, ndev);
         TRY(fetch(ctx, ctx->red_out, (int)(2 * (m + 2) + 1), ctx->h_red));
         for (int64_t q = 0; q <= j; q++) h[q] = sc[q] > 0.0 ? (ctx->h_red[q] + ctx->h_red[(m + 2) + q]) * sj / sc[q] : 0.0;
@@ -240,7 +246,7 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
         }
         wn = sqrt(ctx->h_red[2 * (m + 2)]);
         if (refine == CHY_CGS_REFINE_IFNEEDED && proj > wn * sj * wn * sj) {
-          launch_gs_dots<T>(ctx, (int)(j + 1), V, nc, Vn, nc, h2dev, ctx->gs_scale);
+          launch_gs_dots<T>(ctx, (int)(j + 1), V, nc, Vn, nc, h2dev, sc2);
           launch_gs_update_norm<T>(ctx, (int)(j + 1), V, nc, h2dev, Vn, nc, ndev);
           TRY(fetch(ctx, ctx->red_out, (int)(2 * (m + 2) + 1), ctx->h_red));
           for (int64_t q = 0; q <= j; q++) h[q] += sc[q] > 0.0 ? ctx->h_red[(m + 2) + q] * sj / sc[q] : 0.0;
@@ -276,8 +282,10 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
     }
     double *negy = ctx->h_scale + (m + 2);  // pinned, so no wait for the copy is needed
     for (int64_t q = 0; q < j; q++) negy[q] = -y[q] * sc[q];
-    CHY_HIP(hipMemcpyAsync(hdev, negy, sizeof(double) * j, hipMemcpyHostToDevice, ctx->stream));
-    launch_mgs_update<T>(ctx, (int)j, Z, nc, hdev, e, nc);
+    // (to its own device buffer: red_out holds only reduction results, which
+    // its host mirror repeats)
+    CHY_HIP(hipMemcpyAsync(ctx->gs_scale, negy, sizeof(double) * j, hipMemcpyHostToDevice, ctx->stream));
+    launch_mgs_update<T>(ctx, (int)j, Z, nc, ctx->gs_scale, e, nc);
     (void)done;
     // the true residual of the new e is checked at the top of the loop (a
     // cycle that ends on GMRES's own estimate without it leaves phi measurably
@@ -582,8 +590,12 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   }
   A(dalloc(ctx, &ctx->red, (int64_t)kRedBlocks * 32));
   A(dalloc(ctx, &ctx->rpl, lhs_zsum_blocks(g, false)));
-  A(dalloc(ctx, &ctx->red_out, 3 * (ctx->restart + 2) + 16));
-  if (hipHostMalloc((void **)&ctx->h_red, sizeof(double) * (3 * (ctx->restart + 2) + 16)) != hipSuccess ||
+  ctx->red_out_n = 3 * (ctx->restart + 2) + 16;
+  A(dalloc(ctx, &ctx->red_out, ctx->red_out_n));
+  // the mirror is coherent host memory: device writes reach it without a copy
+  if (hipHostMalloc((void **)&ctx->h_red, sizeof(double) * ctx->red_out_n,
+                    hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&ctx->red_mirror, ctx->h_red, 0) != hipSuccess ||
       hipHostMalloc((void **)&ctx->h_scale, sizeof(double) * 2 * (ctx->restart + 2)) != hipSuccess) {
     chy_destroy(ctx);
     return CHY_ERR_MEM;

@@ -790,10 +790,17 @@ __global__ __launch_bounds__(256) void k_mdot(int nv, const T *__restrict__ A, i
   }
 }
 
-// out[k] = sum_blk partial[blk*nv + k], fixed tree order, times scale[k]
-// when a scale is given.
+// Scales of a combine's outputs, passed by value with the launch.
+struct DotScale {
+  int on;
+  double s[kMaxDots];
+};
+
+// out[k] = sum_blk partial[blk*nv + k], fixed tree order, times sc.s[k] when
+// sc.on; each value also goes to mirror[k] when a mirror is given (host
+// memory: the read-back of a reduced value needs no copy).
 __global__ __launch_bounds__(256) void k_combine(const double *__restrict__ partial, int nblk, int nv,
-                                                 double *__restrict__ out, const double *__restrict__ scale)
+                                                 double *__restrict__ out, DotScale sc, double *__restrict__ mirror)
 {
   __shared__ double sh[4];
   for (int k = 0; k < nv; k++) {
@@ -804,10 +811,30 @@ __global__ __launch_bounds__(256) void k_combine(const double *__restrict__ part
     __syncthreads();
     if (threadIdx.x == 0) {
       const double t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-      out[k] = scale ? t * scale[k] : t;
+      const double v = sc.on ? t * sc.s[k] : t;
+      out[k] = v;
+      if (mirror) mirror[k] = v;
     }
     __syncthreads();
   }
+}
+
+// The host mirror of out when out lies in the reduction output buffer.
+static double *mirror_of(chy_ctx *ctx, const double *out, int nv)
+{
+  if (!ctx->red_mirror || out < ctx->red_out || out + nv > ctx->red_out + ctx->red_out_n) return nullptr;
+  return ctx->red_mirror + (out - ctx->red_out);
+}
+
+// One combine launch: nv outputs, scaled by scale_host[k] when given (at most
+// kMaxDots of them).
+static void combine(chy_ctx *ctx, const double *partial, int nblk, int nv, double *out,
+                    const double *scale_host = nullptr)
+{
+  DotScale sc;
+  sc.on = scale_host != nullptr;
+  for (int k = 0; k < kMaxDots; k++) sc.s[k] = (sc.on && k < nv) ? scale_host[k] : 1.0;
+  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, partial, nblk, nv, out, sc, mirror_of(ctx, out, nv));
 }
 
 // Block b of 256 sums partials [b * chunk, (b + 1) * chunk) in a fixed order.
@@ -832,10 +859,10 @@ void launch_combine_sum(chy_ctx *ctx, const double *partials, int nblk, int nv, 
     constexpr int kChunks = 256;
     const int64_t chunk = ((int64_t)nblk + kChunks - 1) / kChunks;
     hipLaunchKernelGGL(k_chunk_sum, kChunks, kBlock, 0, ctx->stream, partials, (int64_t)nblk, chunk, ctx->red);
-    hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, kChunks, 1, out, (const double *)nullptr);
+    combine(ctx, ctx->red, kChunks, 1, out);
     return;
   }
-  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, partials, nblk, nv, out, (const double *)nullptr);
+  combine(ctx, partials, nblk, nv, out);
 }
 
 template <typename T>
@@ -852,8 +879,7 @@ void launch_mdot(chy_ctx *ctx, int nv, const T *A, int64_t stride, const T *b, i
     else
       hipLaunchKernelGGL((k_mdot<T, kMaxDots>), nb, kBlock, 0, ctx->stream, m, A + k0 * stride, stride, b, n,
                          ctx->red);
-    hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, m, out_dev + k0,
-                       scale ? scale + k0 : nullptr);
+    combine(ctx, ctx->red, (int)nb, m, out_dev + k0, scale ? scale + k0 : nullptr);
   }
 }
 
@@ -875,7 +901,7 @@ void launch_sum(chy_ctx *ctx, const T *a, int64_t n, double *out_dev)
 {
   const unsigned nb = (unsigned)std::min<int64_t>(kRedBlocks, (n + kBlock - 1) / kBlock);
   hipLaunchKernelGGL((k_sum<T>), nb, kBlock, 0, ctx->stream, a, n, ctx->red);
-  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, 1, out_dev, (const double *)nullptr);
+  combine(ctx, ctx->red, (int)nb, 1, out_dev);
 }
 
 // w -= sum_k h_k V_k  (h on the device; accumulation in FP64)
@@ -950,7 +976,7 @@ void launch_mgs_norm(chy_ctx *ctx, int nv, const T *V, int64_t stride, const dou
     hipLaunchKernelGGL((k_mgs_norm<T, 8>), nb, kBlock, 0, ctx->stream, nv, V, stride, h, w, n, ctx->red);
   else
     hipLaunchKernelGGL((k_mgs_norm<T, kMaxDots>), nb, kBlock, 0, ctx->stream, nv, V, stride, h, w, n, ctx->red);
-  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, 1, norm2_dev, (const double *)nullptr);
+  combine(ctx, ctx->red, (int)nb, 1, norm2_dev);
 }
 
 // ---- Gram-Schmidt passes with 16-byte vector loads (n % 4 == 0) -------------------
@@ -1056,7 +1082,7 @@ void launch_gs_dots(chy_ctx *ctx, int nv, const T *V, int64_t stride, const T *w
   const int64_t n4 = n / 4;
   const unsigned nb = (unsigned)std::min<int64_t>(kRedBlocks, (n4 + kBlock - 1) / kBlock);
   hipLaunchKernelGGL((k_mdot4<T, 8>), nb, kBlock, 0, ctx->stream, nv, V, stride, w, n4, ctx->red);
-  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, nv, out_dev, scale);
+  combine(ctx, ctx->red, (int)nb, nv, out_dev, scale);
 }
 
 template <typename T>
@@ -1071,7 +1097,7 @@ void launch_gs_update_dots(chy_ctx *ctx, int nv, const T *V, int64_t stride, con
   const int64_t n4 = n / 4;
   const unsigned nb = (unsigned)std::min<int64_t>(kRedBlocks, (n4 + kBlock - 1) / kBlock);
   hipLaunchKernelGGL((k_gs4<T, 8, false>), nb, kBlock, 0, ctx->stream, nv, V, stride, h, w, n4, ctx->red);
-  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, nv, out_dev, scale);
+  combine(ctx, ctx->red, (int)nb, nv, out_dev, scale);
 }
 
 template <typename T>
@@ -1082,7 +1108,7 @@ void launch_gs_update_norm(chy_ctx *ctx, int nv, const T *V, int64_t stride, con
   const int64_t n4 = n / 4;
   const unsigned nb = (unsigned)std::min<int64_t>(kRedBlocks, (n4 + kBlock - 1) / kBlock);
   hipLaunchKernelGGL((k_gs4<T, 8, true>), nb, kBlock, 0, ctx->stream, nv, V, stride, h, w, n4, ctx->red);
-  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, ctx->red, (int)nb, 1, norm2_dev, (const double *)nullptr);
+  combine(ctx, ctx->red, (int)nb, 1, norm2_dev);
 }
 
 template <typename T>

@@ -124,11 +124,17 @@ struct chy_ctx {
   int64_t restart = 30;
   void *V = nullptr, *Z = nullptr, *wv = nullptr;
   double *red = nullptr;        // reduction partials [nblk][restart+2]
-  double *red_out = nullptr;    // reduced values [restart+2]
+  double *red_out = nullptr;    // reduced values [red_out_n]
+  int64_t red_out_n = 0;
   double *rpl = nullptr;        // per-stencil-block partials of the outer residual's |r|^2
-  double *h_red = nullptr;      // pinned host mirror of red_out
+  // Coherent pinned host mirror of red_out: every reduction that writes
+  // red_out also writes its value here (red_mirror is the device's address of
+  // it), so reading a reduced value back is a stream synchronisation, with no
+  // device-to-host copy
+  double *h_red = nullptr;
+  double *red_mirror = nullptr;
   double *h_scale = nullptr;    // pinned: squared scales of the Krylov basis, then the update coefficients (GMRES)
-  double *gs_scale = nullptr;   // their device copy
+  double *gs_scale = nullptr;   // device copy of the update coefficients
   // FFT preconditioner
   void *fft_buf = nullptr;      // complex work array
   void *pc_buf = nullptr;       // z-line / Chebyshev / two-level scratch, 3 vectors (T)
@@ -256,7 +262,7 @@ void launch_rhs(chy_ctx *ctx, int rhs_type, const T *n, const T *gx, const T *gy
 // reductions: out[k] = sum_i a_k[i] * b[i] for k < nv (a_k = A + k*stride)
 template <typename T>
 void launch_mdot(chy_ctx *ctx, int nv, const T *A, int64_t stride, const T *b, int64_t n, double *out_dev,
-                 const double *scale = nullptr);  // out[k] *= scale[k] when given
+                 const double *scale = nullptr);  // out[k] *= scale[k] when given (a host array, passed by value)
 template <typename T>
 void launch_sum(chy_ctx *ctx, const T *a, int64_t n, double *out_dev);
 // y -= sum_k h_k V_k (h on device, FP64) ; optional normalisation
@@ -270,7 +276,7 @@ void launch_mgs_norm(chy_ctx *ctx, int nv, const T *V, int64_t stride, const dou
 // w -= V h then ||w||^2
 template <typename T>
 void launch_gs_dots(chy_ctx *ctx, int nv, const T *V, int64_t stride, const T *w, int64_t n, double *out_dev,
-                    const double *scale = nullptr);
+                    const double *scale = nullptr);  // host scales, as launch_mdot
 template <typename T>
 void launch_gs_update_dots(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n,
                            double *out_dev, const double *scale = nullptr);

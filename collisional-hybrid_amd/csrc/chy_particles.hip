@@ -769,7 +769,13 @@ constexpr int kDepChunk = 640;  // ions per chunk (LDS records)
 // One staged ion: in-cell offsets, velocity and weight.
 template <typename T> struct alignas(8 * sizeof(T)) DepRec { T fx, fy, fz, vx, vy, vz, w, pad; };
 
-template <typename T, bool ABS, bool GATHER, bool SUMS>
+// XC (nx a multiple of kDepCells, so a block's cells lie in one x row): the
+// block adds each cell's x-low corners to the x-high corners of the cell
+// before it in LDS and writes 16 values per cell, P[(4r+m)*nc + c] for the
+// y-z corner r = 2 (z bit) + (y bit), and the x-high corners of its last
+// cell to E[(4r+m)*nblocks + block] (E = P + 16 nc), for the node sum to add
+// at the block boundaries: half the partials' bytes written and read.
+template <typename T, bool ABS, bool GATHER, bool SUMS, bool XC = false>
 __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *__restrict__ offs, PState<T> st,
                                                        bool weighted, T *__restrict__ P,
                                                        const uint32_t *__restrict__ perm, PState<T> b, double n0x,
@@ -912,7 +918,22 @@ __global__ __launch_bounds__(256) void k_cell_partials(Grid g, const uint32_t *_
     for (; s < a1; s++) add(rec[s - base]);
     __syncthreads();
   }
-  if (own) {
+  if constexpr (XC) {
+    // every block holds kDepCells cells (nc is a multiple of it)
+    __shared__ T Ps[kDepCells][8][4];
+#pragma unroll
+    for (int mm = 0; mm < 4; mm++) Ps[cl][corner][mm] = acc[mm];
+    __syncthreads();
+    const int64_t lb = c0 / kDepCells, nbk = g.nc / kDepCells;
+    T *E = P + (int64_t)16 * g.nc;
+    for (int e = (int)threadIdx.x; e < 16 * kDepCells; e += 256) {
+      const int rm = e / kDepCells, c = e % kDepCells, r = rm >> 2, mm = rm & 3;
+      T v = Ps[c][2 * r][mm];
+      if (c > 0) v = v + Ps[c - 1][2 * r + 1][mm];
+      P[(int64_t)rm * g.nc + c0 + c] = v;
+      if (c == kDepCells - 1) E[(int64_t)rm * nbk + lb] = Ps[c][2 * r + 1][mm];
+    }
+  } else if (own) {
 #pragma unroll
     for (int mm = 0; mm < 4; mm++) P[(int64_t)(4 * corner + mm) * g.nc + c0 + cl] = acc[mm];
   }
@@ -936,6 +957,37 @@ __global__ __launch_bounds__(256) void k_node_sum(Grid g, const T *__restrict__ 
     const int64_t s = lin(g, i - (q & 1), j - ((q >> 1) & 1), k - (q >> 2));
 #pragma unroll
     for (int a = 0; a < 4; a++) m[a] += P[(int64_t)(4 * q + a) * g.nc + s];
+  }
+  n[c] = m[0]; gx[c] = m[1]; gy[c] = m[2]; gz[c] = m[3];
+}
+
+// Node moments from the x-combined partials of k_cell_partials<XC>: the four
+// y-z corners r = 0..3 in order, each the x pair made in the block, plus for
+// nodes at a block's first x position the x-high corners of the cell before
+// it (the previous block's last cell, or the row's last across the periodic
+// wrap), added as the block adds them.
+template <typename T>
+__global__ __launch_bounds__(256) void k_node_sum_xc(Grid g, const T *__restrict__ P, T *__restrict__ n,
+                                                     T *__restrict__ gx, T *__restrict__ gy, T *__restrict__ gz)
+{
+  const int64_t c = xcd_block() * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  const int64_t nbk = g.nc / kDepCells;
+  const T *E = P + (int64_t)16 * g.nc;
+  const bool edge = i % kDepCells == 0;
+  T m[4] = {(T)0, (T)0, (T)0, (T)0};
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int jj = j - (r & 1), kk = k - (r >> 1);
+    const int64_t s = lin(g, i, jj, kk);
+    const int64_t bw = edge ? lin(g, i - 1, jj, kk) / kDepCells : 0;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      T v = P[(int64_t)(4 * r + a) * g.nc + s];
+      if (edge) v = v + E[(int64_t)(4 * r + a) * nbk + bw];
+      m[a] += v;
+    }
   }
   n[c] = m[0]; gx[c] = m[1]; gy[c] = m[2]; gz[c] = m[3];
 }
@@ -1134,6 +1186,10 @@ static unsigned grid_for(int64_t n, unsigned cap = 4096)
 // A placement kernel (k_perm, k_scatter) computed a slot outside the ion
 // range on an earlier sort: the ion order is not a permutation, so the state
 // is lost.  The flag is read with the next sort's other read-backs.
+// The mixed path deposits through the x-combined partials when whole blocks
+// of cells lie in one x row.
+static bool x_combined(const chy_ctx *ctx) { return ctx->mixed && ctx->g.nx % kDepCells == 0; }
+
 static int lost_placement(chy_ctx *ctx)
 {
   launch_fill<uint32_t>(ctx, 1, ctx->sortflag + 1, 0u);
@@ -1198,12 +1254,14 @@ static int sort_neighbour(chy_ctx *ctx, bool &done)
   const bool sums = p.mn > 0.0;  // the collision normaliser is needed
   const unsigned nbc = (unsigned)((nc + kDepCells - 1) / kDepCells);
   timer_start(ctx, "sort_reorder");
-#define GATHER(T, ABS, S, st)                                                                                     \
-  hipLaunchKernelGGL((k_cell_partials<T, ABS, true, S>), nbc, kBlock, 0, ctx->stream, ctx->g, ctx->offs_alt,       \
+#define GATHER(T, ABS, S, XC, st)                                                                                 \
+  hipLaunchKernelGGL((k_cell_partials<T, ABS, true, S, XC>), nbc, kBlock, 0, ctx->stream, ctx->g, ctx->offs_alt,   \
                      st[ctx->cur], ctx->weighted, (T *)ctx->partials, ctx->perm, st[1 - ctx->cur], p.vn0[0],      \
                      p.vn0[1], p.vn0[2], p.vnT, ctx->sred, ctx->sort_src)
-  if (ctx->mixed) { if (sums) GATHER(float, false, true, ctx->pf); else GATHER(float, false, false, ctx->pf); }
-  else { if (sums) GATHER(double, true, true, ctx->pd); else GATHER(double, true, false, ctx->pd); }
+  if (ctx->mixed) {
+    if (x_combined(ctx)) { if (sums) GATHER(float, false, true, true, ctx->pf); else GATHER(float, false, false, true, ctx->pf); }
+    else { if (sums) GATHER(float, false, true, false, ctx->pf); else GATHER(float, false, false, false, ctx->pf); }
+  } else { if (sums) GATHER(double, true, true, false, ctx->pd); else GATHER(double, true, false, false, ctx->pd); }
 #undef GATHER
   timer_stop(ctx, "sort_reorder");
   if (sums) launch_combine_sum(ctx, ctx->sred, (int)nbc, 1, ctx->S_dev);
@@ -1290,15 +1348,26 @@ int particles_deposit(chy_ctx *ctx)
   ctx->partials_fresh = false;
   if (ctx->mixed) {
     float *P = (float *)ctx->partials;
+    const bool xc = x_combined(ctx);
     if (cells) {
       timer_start(ctx, "deposit_cells");
-      hipLaunchKernelGGL((k_cell_partials<float, false, false, false>), nbc, kBlock, 0, ctx->stream, ctx->g, ctx->offsets,
-                         ctx->pf[ctx->cur], ctx->weighted, P, (const uint32_t *)nullptr, PState<float>(), 0.0, 0.0, 0.0, 0.0, (double *)nullptr, (uint32_t *)nullptr);
+      if (xc)
+        hipLaunchKernelGGL((k_cell_partials<float, false, false, false, true>), nbc, kBlock, 0, ctx->stream, ctx->g,
+                           ctx->offsets, ctx->pf[ctx->cur], ctx->weighted, P, (const uint32_t *)nullptr,
+                           PState<float>(), 0.0, 0.0, 0.0, 0.0, (double *)nullptr, (uint32_t *)nullptr);
+      else
+        hipLaunchKernelGGL((k_cell_partials<float, false, false, false>), nbc, kBlock, 0, ctx->stream, ctx->g,
+                           ctx->offsets, ctx->pf[ctx->cur], ctx->weighted, P, (const uint32_t *)nullptr,
+                           PState<float>(), 0.0, 0.0, 0.0, 0.0, (double *)nullptr, (uint32_t *)nullptr);
       timer_stop(ctx, "deposit_cells");
     }
     timer_start(ctx, "deposit_nodes");
-    hipLaunchKernelGGL((k_node_sum<float>), nb, kBlock, 0, ctx->stream, ctx->g, P, (float *)ctx->mom[0],
-                       (float *)ctx->mom[1], (float *)ctx->mom[2], (float *)ctx->mom[3]);
+    if (xc)
+      hipLaunchKernelGGL((k_node_sum_xc<float>), nb, kBlock, 0, ctx->stream, ctx->g, P, (float *)ctx->mom[0],
+                         (float *)ctx->mom[1], (float *)ctx->mom[2], (float *)ctx->mom[3]);
+    else
+      hipLaunchKernelGGL((k_node_sum<float>), nb, kBlock, 0, ctx->stream, ctx->g, P, (float *)ctx->mom[0],
+                         (float *)ctx->mom[1], (float *)ctx->mom[2], (float *)ctx->mom[3]);
     timer_stop(ctx, "deposit_nodes");
   } else {
     double *P = (double *)ctx->partials;

@@ -32,6 +32,47 @@ def test_sort_and_deposit_fp64_bitwise(N, n, weights):
         assert np.array_equal(v[:, a], ref[a]), (a, relmax(v[:, a], ref[a]))
 
 
+@pytest.mark.parametrize("N", [(32, 8, 6), (64, 4, 4), (24, 8, 6)])
+def test_mixed_deposit_matches_oracle(N):
+    """Mixed-precision deposit (FP32 in-cell offsets and sums) against the
+    FP64 oracle on the same ions.  Rows of 32 and 64 cells take the
+    x-combined cell partials (each block's x pairs added in LDS, the corners
+    across block edges and the periodic wrap by the node sum); a row of 24
+    takes the per-corner partials.  Repeated deposits are bitwise equal."""
+    n = 40 * N[0] * N[1] * N[2]
+    rng = np.random.default_rng(N[0])
+    with context(opts(N, extra="-precision mixed"), capacity=n) as ctx:
+        g = grid_of(ctx.prm)
+        upload(ctx, random_particles(g, n, rng))
+        got = as_particles(ctx.get_particles())
+        v1 = ctx.collect_moments()
+        v2 = ctx.collect_moments()
+    assert np.array_equal(v1, v2)
+    ps, offs = oracle_sorted(g, got)
+    assert np.array_equal(ps.pid, got.pid)
+    ref = orc.deposit_sorted(g, ps, offs)
+    for a in range(4):
+        assert relmax(v1[:, a], ref[a]) <= 1e-5, (a, relmax(v1[:, a], ref[a]))
+    assert v1[:, 0].sum() == pytest.approx(n, rel=1e-5)
+
+
+def test_mixed_step_deposit_equals_fresh_deposit():
+    """The step deposits inside the neighbour-move sort (the gather variant of
+    the x-combined partials); depositing the same sorted ions again (the
+    plain variant) gives bitwise the same moments."""
+    extra = "-seed 6 -ppc 16 -ksp_rtol 1e-8 --perturbation uniform --perturbation-amplitude 0.01 -precision mixed"
+    with context(opts((32, 16, 8), extra=extra)) as ctx:
+        ctx.init_particles()
+        ctx.collect_moments(download=False)
+        ctx.solve_potential()
+        for _ in range(2):
+            ctx.step()
+        a = ctx.get_moments()
+        b = ctx.collect_moments()
+    assert a[:, 0].sum() > 0
+    assert np.array_equal(a, b)
+
+
 def test_deposit_edge_cases():
     """Empty set, one crowded cell, positions on the upper bound and outside."""
     with context(opts((4, 4, 4)), capacity=4000) as ctx:

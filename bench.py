@@ -439,12 +439,15 @@ def measured_traffic(kernel, N, ppc):
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{N}cubed_ppc{ppc}.md")
     if not os.path.exists(path):
         return None
+    col = None
     for line in open(path):
-        if line.startswith(f"| {kernel} |"):
-            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if line.startswith("| timer |"):
+            col = cells.index("traffic bytes") if "traffic bytes" in cells else None
+        elif col is not None and line.startswith(f"| {kernel} |"):
             try:
-                return float(cells[-1])
-            except ValueError:
+                return float(cells[col])
+            except (ValueError, IndexError):
                 return None
     return None
 

@@ -530,6 +530,10 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
     return CHY_ERR_LIB;
   }
   if (hipSetDevice(0) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) { delete ctx; return CHY_ERR_LIB; }
+  if (hipEventCreate(&ctx->ev_step) != hipSuccess || hipEventCreate(&ctx->ev_particles) != hipSuccess) {
+    chy_destroy(ctx);
+    return CHY_ERR_LIB;
+  }
   ctx->mixed = p.precision == CHY_PREC_MIXED;
   Grid &g = ctx->g;
   g.nx = (int)p.N[0]; g.ny = (int)p.N[1]; g.nz = (int)p.N[2];
@@ -596,7 +600,10 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   if (hipHostMalloc((void **)&ctx->h_red, sizeof(double) * ctx->red_out_n,
                     hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&ctx->red_mirror, ctx->h_red, 0) != hipSuccess ||
-      hipHostMalloc((void **)&ctx->h_scale, sizeof(double) * 2 * (ctx->restart + 2)) != hipSuccess) {
+      hipHostMalloc((void **)&ctx->h_scale, sizeof(double) * 2 * (ctx->restart + 2)) != hipSuccess ||
+      hipHostMalloc((void **)&ctx->h_status, sizeof(uint64_t) * kStatusWords,
+                    hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&ctx->status_dev, ctx->h_status, 0) != hipSuccess) {
     chy_destroy(ctx);
     return CHY_ERR_MEM;
   }
@@ -652,10 +659,13 @@ int chy_destroy(chy_ctx *ctx)
   for (void *p : ps) if (p) (void)hipFree(p);
   if (ctx->h_red) (void)hipHostFree(ctx->h_red);
   if (ctx->h_scale) (void)hipHostFree(ctx->h_scale);
+  if (ctx->h_status) (void)hipHostFree(ctx->h_status);
   for (auto &kv : ctx->timers) {
     for (hipEvent_t e : kv.second.a) (void)hipEventDestroy(e);
     for (hipEvent_t e : kv.second.b) (void)hipEventDestroy(e);
   }
+  if (ctx->ev_step) (void)hipEventDestroy(ctx->ev_step);
+  if (ctx->ev_particles) (void)hipEventDestroy(ctx->ev_particles);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CHY_OK;
@@ -992,18 +1002,21 @@ int chy::ctx_step(chy_ctx *ctx, chy_step_stats *st)
   const auto t0 = std::chrono::steady_clock::now();
   chy_step_stats local;
   memset(&local, 0, sizeof local);
+  CHY_HIP(hipEventRecord(ctx->ev_step, ctx->stream));
   timer_start(ctx, "efield");
   do_efield(ctx);
   timer_stop(ctx, "efield");
-  int rc = particles_push(ctx, true, ctx->prm.mn > 0.0, &local.collisions);
+  // the push's counters are read with the sort's status, in one synchronisation
+  int rc = particles_push(ctx, true, ctx->prm.mn > 0.0, &local.collisions, true);
   ctx->step++;
   if (rc) return rc;
   rc = particles_sort(ctx);
-  if (rc) return rc;
+  if (rc) { ctx->push_pending = false; ctx->push_stats = nullptr; return rc; }
+  TRY(particles_push_check(ctx));
   rc = particles_deposit(ctx);
   if (rc) return rc;
-  rc = sync(ctx);
-  if (rc) return rc;
+  // the particle phase ends on the device at this event (no synchronisation)
+  CHY_HIP(hipEventRecord(ctx->ev_particles, ctx->stream));
   const auto t1 = std::chrono::steady_clock::now();
   rc = do_rhs(ctx);
   if (rc) return rc;
@@ -1024,10 +1037,17 @@ int chy::ctx_step(chy_ctx *ctx, chy_step_stats *st)
   }
   rc = do_solve(ctx, &local.solve);
   if (rc) return rc;
+  CHY_HIP(hipStreamSynchronize(ctx->stream));
   const auto t2 = std::chrono::steady_clock::now();
-  local.solve.ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
-  local.ms_particles = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  (void)t1;
+  // the particle phase: host start of the step to the device's end of the
+  // deposit (the device is idle when a step starts: every step ends on a
+  // synchronisation); the solve: the rest of the step
+  float dev_ms = 0.f;
+  CHY_HIP(hipEventElapsedTime(&dev_ms, ctx->ev_step, ctx->ev_particles));
   local.ms_total = std::chrono::duration<double, std::milli>(t2 - t0).count();
+  local.ms_particles = std::min<double>(dev_ms, local.ms_total);
+  local.solve.ms = local.ms_total - local.ms_particles;
   // global diagnostics of the new state, outside the step's timings
   const int64_t di = ctx->prm.diagnostics_interval;
   if (di > 0 && ctx->step % di == 0) TRY(diagnostics(ctx, &local.diag));

@@ -133,6 +133,15 @@ struct chy_ctx {
   // device-to-host copy
   double *h_red = nullptr;
   double *red_mirror = nullptr;
+  // Coherent pinned status words (the sort's checks, the push's counters),
+  // written by a one-block kernel at the device address status_dev, so a
+  // read-back costs one stream synchronisation and no copies
+  uint64_t *h_status = nullptr;
+  uint64_t *status_dev = nullptr;
+  hipEvent_t ev_step = nullptr, ev_particles = nullptr;  // a step's start and the end of its particle phase
+  bool push_pending = false;    // a push's counters not yet read (deferred to the sort's read-back)
+  int64_t push_Nc = 0;
+  chy_collision_stats *push_stats = nullptr;
   double *h_scale = nullptr;    // pinned: squared scales of the Krylov basis, then the update coefficients (GMRES)
   double *gs_scale = nullptr;   // device copy of the update coefficients
   // FFT preconditioner
@@ -167,6 +176,7 @@ struct chy_ctx {
 namespace chy {
 
 constexpr int kRedBlocks = 1024;  // fixed reduction grid: deterministic sums
+constexpr int kStatusWords = 16;  // chy_ctx::h_status
 constexpr int kBlock = 256;
 
 inline int set_err(chy_ctx *c, int code, const std::string &msg)
@@ -319,7 +329,10 @@ int particles_upload(chy_ctx *ctx, int64_t np, const double *pos, const double *
                      const uint32_t *pid);
 int particles_download(chy_ctx *ctx, double *pos, double *vel, double *w, uint32_t *pid);
 int particles_init(chy_ctx *ctx);
-int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *st);
+// defer: leave the counters to the next status read-back (the sort's), which
+// checks them first; particles_push_check reads them if still pending
+int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *st, bool defer = false);
+int particles_push_check(chy_ctx *ctx);
 int particles_move_only(chy_ctx *ctx);
 int particles_sort(chy_ctx *ctx);
 int particles_deposit(chy_ctx *ctx);

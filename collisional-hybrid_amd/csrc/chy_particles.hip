@@ -1186,6 +1186,43 @@ static unsigned grid_for(int64_t n, unsigned cap = 4096)
 // A placement kernel (k_perm, k_scatter) computed a slot outside the ion
 // range on an earlier sort: the ion order is not a permutation, so the state
 // is lost.  The flag is read with the next sort's other read-backs.
+// Status words to the host: out[q] = *w[q] for the 32-bit words given
+// (q < 4), then out[4 + q] = u[q] for q < nu; out is coherent host memory.
+__global__ __launch_bounds__(64) void k_status(const uint32_t *w0, const uint32_t *w1, const uint32_t *w2,
+                                               const uint32_t *w3, const unsigned long long *u, int nu,
+                                               unsigned long long *out)
+{
+  const int t = (int)threadIdx.x;
+  const uint32_t *w = t == 0 ? w0 : t == 1 ? w1 : t == 2 ? w2 : w3;
+  if (t < 4 && w) out[t] = *w;
+  if (t < nu) out[4 + t] = u[t];
+}
+
+static int push_finish(chy_ctx *ctx, const unsigned long long cs[6]);
+
+// Reads the words (null entries read 0) and nu counters in one
+// synchronisation: w[q] into v32[q], the counters into v64.  A push whose
+// counters are still unread (ctx->push_pending) has them read in the same
+// synchronisation and checked first.
+static int read_status(chy_ctx *ctx, const uint32_t *const w[4], uint32_t v32[4], const unsigned long long *u, int nu,
+                       unsigned long long *v64)
+{
+  const bool push = ctx->push_pending;
+  if (nu > kStatusWords - 4 || (push && nu > 0)) return set_err(ctx, CHY_ERR_LIB, "read_status: bad counter request");
+  if (push) { u = ctx->cstats; nu = 6; }
+  hipLaunchKernelGGL(k_status, 1, 64, 0, ctx->stream, w[0], w[1], w[2], w[3], u, nu,
+                     (unsigned long long *)ctx->status_dev);
+  CHY_HIP(hipStreamSynchronize(ctx->stream));
+  for (int q = 0; q < 4; q++) v32[q] = w[q] ? (uint32_t)ctx->h_status[q] : 0u;
+  if (push) {
+    unsigned long long cs[6];
+    for (int q = 0; q < 6; q++) cs[q] = ctx->h_status[4 + q];
+    return push_finish(ctx, cs);
+  }
+  for (int q = 0; q < nu; q++) v64[q] = ctx->h_status[4 + q];
+  return CHY_OK;
+}
+
 // The mixed path deposits through the x-combined partials when whole blocks
 // of cells lie in one x row.
 static bool x_combined(const chy_ctx *ctx) { return ctx->mixed && ctx->g.nx % kDepCells == 0; }
@@ -1233,12 +1270,12 @@ static int sort_neighbour(chy_ctx *ctx, bool &done)
   hipLaunchKernelGGL(k_tile_scan, (unsigned)nt, kBlock, 0, ctx->stream, nc, ctx->counts, ctx->scan_tmp, ctx->offs_alt,
                      ctx->cursor, (uint32_t)np);
   timer_stop(ctx, "sort_scan");
-  uint32_t flag = 0, mx = 0, total = 0, lost = 0;
-  CHY_HIP(hipMemcpyAsync(&flag, ctx->sortflag, sizeof(flag), hipMemcpyDeviceToHost, ctx->stream));
-  CHY_HIP(hipMemcpyAsync(&lost, ctx->sortflag + 1, sizeof(lost), hipMemcpyDeviceToHost, ctx->stream));
-  CHY_HIP(hipMemcpyAsync(&mx, ctx->maxcount, sizeof(mx), hipMemcpyDeviceToHost, ctx->stream));
-  CHY_HIP(hipMemcpyAsync(&total, ctx->scan_tmp + nt, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
-  CHY_HIP(hipStreamSynchronize(ctx->stream));
+  uint32_t v[4];
+  {
+    const uint32_t *w[4] = {ctx->sortflag, ctx->sortflag + 1, ctx->maxcount, ctx->scan_tmp + nt};
+    TRY(read_status(ctx, w, v, nullptr, 0, nullptr));
+  }
+  const uint32_t flag = v[0], lost = v[1], mx = v[2], total = v[3];
   if (lost) return lost_placement(ctx);
   if (flag || mx > kMaxCellOccupancy || total != (uint32_t)np) {
     launch_fill<uint32_t>(ctx, nc + 1, ctx->counts, 0u);
@@ -1300,11 +1337,12 @@ int particles_sort(chy_ctx *ctx)
   timer_stop(ctx, "sort_scan");
   // the stable per-cell rank is quadratic in the cell occupancy: bound it; and
   // the histogram must account for every ion before the scatter uses it
-  uint32_t mx = 0, total = 0, lost = 0;
-  CHY_HIP(hipMemcpyAsync(&mx, ctx->maxcount, sizeof(mx), hipMemcpyDeviceToHost, ctx->stream));
-  CHY_HIP(hipMemcpyAsync(&total, ctx->scan_tmp + nt, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
-  CHY_HIP(hipMemcpyAsync(&lost, ctx->sortflag + 1, sizeof(lost), hipMemcpyDeviceToHost, ctx->stream));
-  CHY_HIP(hipStreamSynchronize(ctx->stream));
+  uint32_t v[4];
+  {
+    const uint32_t *w[4] = {ctx->maxcount, ctx->scan_tmp + nt, ctx->sortflag + 1, nullptr};
+    TRY(read_status(ctx, w, v, nullptr, 0, nullptr));
+  }
+  const uint32_t mx = v[0], total = v[1], lost = v[2];
   if (lost) return lost_placement(ctx);
   if (total != (uint32_t)np)
     return set_err(ctx, CHY_ERR_LIB, "sort: the cell histogram holds " + std::to_string(total) + " ions, not " +
@@ -1636,8 +1674,39 @@ int particles_init(chy_ctx *ctx)
   return settle(ctx);
 }
 
-int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *stats)
+// The push's counters, once read: collision statistics, non-finite state,
+// the refusal limit.
+static int push_finish(chy_ctx *ctx, const unsigned long long cs[6])
 {
+  ctx->push_pending = false;
+  chy_collision_stats *stats = ctx->push_stats;
+  ctx->push_stats = nullptr;
+  if (stats) {
+    stats->accepted = (int64_t)cs[0];
+    stats->refused = (int64_t)cs[1];
+    stats->capped = (int64_t)cs[2];
+    stats->Nc = ctx->push_Nc;
+    memcpy(&stats->expected, &cs[4], sizeof(double));
+    memcpy(&stats->variance, &cs[5], sizeof(double));
+  }
+  if (cs[3]) return set_err(ctx, CHY_ERR_FP, "non-finite particle position or velocity after the push");
+  const int64_t lim = ctx->prm.collision_refusal_limit;  // particles.c:603 (10)
+  if (lim > 0 && (int64_t)cs[1] >= lim)
+    return set_err(ctx, CHY_ERR_SIG, "Failed to collide " + std::to_string(cs[1]) + " ion-neutral pairs. Aborting.");
+  return CHY_OK;
+}
+
+int particles_push_check(chy_ctx *ctx)
+{
+  if (!ctx->push_pending) return CHY_OK;
+  const uint32_t *w[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t v[4];
+  return read_status(ctx, w, v, nullptr, 0, nullptr);
+}
+
+int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *stats, bool defer)
+{
+  if (ctx->push_pending) TRY(particles_push_check(ctx));
   PushConst pc = push_const(ctx);
   const int64_t Nc = collide ? collision_count(ctx->prm, ctx->np) : 0;
   pc.Nc = (double)Nc;
@@ -1682,22 +1751,12 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
 #undef LAUNCH
   if (move) ctx->keys_moved = true;
   timer_stop(ctx, "push");
-  unsigned long long cs[6];
-  CHY_HIP(hipMemcpyAsync(cs, ctx->cstats, sizeof(cs), hipMemcpyDeviceToHost, ctx->stream));
-  CHY_HIP(hipStreamSynchronize(ctx->stream));
-  if (stats) {
-    stats->accepted = (int64_t)cs[0];
-    stats->refused = (int64_t)cs[1];
-    stats->capped = (int64_t)cs[2];
-    stats->Nc = Nc;
-    memcpy(&stats->expected, &cs[4], sizeof(double));
-    memcpy(&stats->variance, &cs[5], sizeof(double));
-  }
-  if (cs[3]) return set_err(ctx, CHY_ERR_FP, "non-finite particle position or velocity after the push");
-  const int64_t lim = ctx->prm.collision_refusal_limit;  // particles.c:603 (10)
-  if (lim > 0 && (int64_t)cs[1] >= lim)
-    return set_err(ctx, CHY_ERR_SIG, "Failed to collide " + std::to_string(cs[1]) + " ion-neutral pairs. Aborting.");
-  return CHY_OK;
+  // the counters are read now, or (defer) with the next status read-back,
+  // the sort's, and checked before its own results
+  ctx->push_pending = true;
+  ctx->push_Nc = Nc;
+  ctx->push_stats = stats;
+  return defer ? CHY_OK : particles_push_check(ctx);
 }
 
 int particles_move_only(chy_ctx *ctx)
@@ -1716,8 +1775,11 @@ int particles_move_only(chy_ctx *ctx)
     hipLaunchKernelGGL(k_move_fp64, nb, kBlock, 0, ctx->stream, ctx->g, pc, ctx->np, ctx->pd[ctx->cur], ctx->counts,
                        ctx->cstats);
   unsigned long long cs[4];
-  CHY_HIP(hipMemcpyAsync(cs, ctx->cstats, sizeof(cs), hipMemcpyDeviceToHost, ctx->stream));
-  CHY_HIP(hipStreamSynchronize(ctx->stream));
+  {
+    const uint32_t *w[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t v[4];
+    TRY(read_status(ctx, w, v, ctx->cstats, 4, cs));
+  }
   if (cs[3]) return set_err(ctx, CHY_ERR_FP, "non-finite particle position after the move");
   return CHY_OK;
 }

@@ -950,7 +950,8 @@ __global__ __launch_bounds__(256) void k_node_sum(Grid g, const T *__restrict__ 
 {
   const int64_t c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
-  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  int i, j, k;
+  split_cell(g, (int)c, i, j, k);
   T m[4] = {(T)0, (T)0, (T)0, (T)0};
 #pragma unroll
   for (int q = 0; q < 8; q++) {
@@ -972,11 +973,14 @@ __global__ __launch_bounds__(256) void k_node_sum_xc(Grid g, const T *__restrict
 {
   const int64_t c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
-  const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  int i, j, k;
+  split_cell(g, (int)c, i, j, k);
   const int64_t nbk = g.nc / kDepCells;
   const T *E = P + (int64_t)16 * g.nc;
   const bool edge = i % kDepCells == 0;
-  T m[4] = {(T)0, (T)0, (T)0, (T)0};
+  // every lane loads an E value (block 0's for the others, unused), so the
+  // edge lanes' loads go out with the rest instead of after them
+  T pv[16], ev[16];
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int jj = j - (r & 1), kk = k - (r >> 1);
@@ -984,8 +988,16 @@ __global__ __launch_bounds__(256) void k_node_sum_xc(Grid g, const T *__restrict
     const int64_t bw = edge ? lin(g, i - 1, jj, kk) / kDepCells : 0;
 #pragma unroll
     for (int a = 0; a < 4; a++) {
-      T v = P[(int64_t)(4 * r + a) * g.nc + s];
-      if (edge) v = v + E[(int64_t)(4 * r + a) * nbk + bw];
+      pv[4 * r + a] = P[(int64_t)(4 * r + a) * g.nc + s];
+      ev[4 * r + a] = E[(int64_t)(4 * r + a) * nbk + bw];
+    }
+  }
+  T m[4] = {(T)0, (T)0, (T)0, (T)0};
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const T v = edge ? pv[4 * r + a] + ev[4 * r + a] : pv[4 * r + a];
       m[a] += v;
     }
   }

@@ -455,19 +455,22 @@ __global__ __launch_bounds__(256) void k_fft_col0(int logn, int64_t estride, int
   }
 }
 
-// ---- 256-point lines: register-resident four-step transforms -------------------------
-// A length-256 line is held by 16 threads, thread j owning the elements
-// j + 16 m (m < 16) in registers.  256 = 16 x 16: each thread transforms its
-// 16 values (a radix-4 x radix-4 DFT in registers), multiplies by the twiddles
-// W_256^(j k1), and one transpose through LDS hands thread k1 the values of all
-// 16 threads, whose 16-point DFT gives X[k1 + 16 k2].  Input and output use the
-// same register pattern in natural order, so forward and inverse compose
-// without permutations, and LDS carries each value twice per transform.  The
-// transpose strides are chosen so that the 64 lanes of a wave touch 64
-// distinct banks (values split into real and imaginary arrays).
-constexpr int kF256Sk = 17;   // LDS stride between the k1 rows of a line
-constexpr int kF256SbS = 324; // line stride, strided passes (lanes: 16 lines x 4 threads)
-constexpr int kF256SbX = 272; // line stride, x pass (lanes: 16 threads x 4 line pairs)
+// ---- 256- and 512-point lines: register-resident four-step transforms -----------------
+// A line of N = 16 R points (R = 16: 256, R = 32: 512) is held by 16 threads,
+// thread j owning the elements j + 16 m (m < R) in registers.  Each thread
+// transforms its R values (a length-R DFT in registers over m), multiplies by
+// the twiddles W_N^(j k1), and one transpose through LDS hands thread j the
+// 16 values of rows k1 = j + 16 c (c < R/16), whose 16-point DFTs over j
+// give X[k1 + R k2].  That is element j + 16 (c + (R/16) k2): input and output
+// use the same register pattern in natural order, so forward and inverse
+// compose without permutations, and LDS carries each value twice per
+// transform.  The LDS strides (values split into real and imaginary arrays)
+// make the 64 lanes of a wave touch 64 distinct banks in both layouts.
+constexpr int kQSk = 17;  // LDS stride between the k1 rows of a line
+// line strides: strided passes (lanes: 16 lines x 4 threads), x pass (lanes:
+// 16 threads x 4 line pairs); 4 and 16 modulo 64 respectively
+template <int R> constexpr int q_sbs() { return R == 16 ? 324 : 580; }
+template <int R> constexpr int q_sbx() { return R == 16 ? 272 : 592; }
 
 template <typename T, bool INV>
 __device__ __forceinline__ void dft4(cpx<T> &a0, cpx<T> &a1, cpx<T> &a2, cpx<T> &a3)
@@ -516,47 +519,83 @@ __device__ __forceinline__ void dft16(cpx<T> (&v)[16])
   for (int q = 0; q < 16; q++) v[q] = o[q];
 }
 
-// The four-step transform of the line held by this thread's group (thread j of
-// 16; its LDS rows start at gbase).  tws: W_256^m, m < 128 (forward).  Every
-// thread of the block calls it (two barriers).
-template <typename T, bool INV>
-__device__ __forceinline__ void fft256(cpx<T> (&v)[16], int j, T *lre, T *lim, int gbase, const cpx<T> *tws)
+// In-place R-point DFT (R = 16 or 32), natural order in and out; 32 points
+// as two 16-point DFTs of the even and odd elements and one radix-2 stage.
+template <typename T, bool INV, int R>
+__device__ __forceinline__ void dftR(cpx<T> (&v)[R])
 {
-  dft16<T, INV>(v);
+  if constexpr (R == 16) {
+    dft16<T, INV>(v);
+  } else {
+    static_assert(R == 32, "16- or 32-point register DFTs");
+    cpx<T> e[16], o[16];
 #pragma unroll
-  for (int k1 = 1; k1 < 16; k1++) {
-    const int m = j * k1;  // < 256: W^m = -W^(m-128) past 128
-    cpx<T> w = tws[m & 127];
-    if (m & 128) w = {-w.x, -w.y};
+    for (int q = 0; q < 16; q++) { e[q] = v[2 * q]; o[q] = v[2 * q + 1]; }
+    dft16<T, INV>(e);
+    dft16<T, INV>(o);
+    // W_32^k = cos(2 pi k / 32) -+ i sin(2 pi k / 32)
+    constexpr double wc[16] = {1.0, 0.9807852804032304, 0.9238795325112867, 0.8314696123025452, 0.7071067811865476, 0.5555702330196023, 0.38268343236508984, 0.19509032201612833, 0.0, -0.1950903220161282, -0.3826834323650897, -0.555570233019602, -0.7071067811865475, -0.8314696123025453, -0.9238795325112867, -0.9807852804032304};
+    constexpr double ws[16] = {0.0, 0.19509032201612825, 0.3826834323650898, 0.5555702330196022, 0.7071067811865475, 0.8314696123025452, 0.9238795325112867, 0.9807852804032304, 1.0, 0.9807852804032304, 0.9238795325112867, 0.8314696123025455, 0.7071067811865476, 0.5555702330196022, 0.3826834323650899, 0.1950903220161286};
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const T wr = (T)wc[k], wi = INV ? (T)ws[k] : (T)(-ws[k]);
+      const cpx<T> t = cmul<T>(o[k], {wr, wi});
+      v[k] = {e[k].x + t.x, e[k].y + t.y};
+      v[k + 16] = {e[k].x - t.x, e[k].y - t.y};
+    }
+  }
+}
+
+// The four-step transform of the line held by this thread's group (thread j
+// of 16; its LDS rows start at gbase).  tws: W_N^m, m < N/2 (forward).  Every
+// thread of the block calls it (two barriers).
+template <typename T, bool INV, int R>
+__device__ __forceinline__ void fft_line(cpx<T> (&v)[R], int j, T *lre, T *lim, int gbase, const cpx<T> *tws)
+{
+  constexpr int N = 16 * R, H = N / 2;
+  dftR<T, INV, R>(v);
+#pragma unroll
+  for (int k1 = 1; k1 < R; k1++) {
+    const int m = j * k1;  // < N: W^m = -W^(m-N/2) past N/2
+    cpx<T> w = tws[m & (H - 1)];
+    if (m & H) w = {-w.x, -w.y};
     if (INV) w.y = -w.y;
     v[k1] = cmul<T>(v[k1], w);
   }
 #pragma unroll
-  for (int k1 = 0; k1 < 16; k1++) {
-    lre[gbase + k1 * kF256Sk + j] = v[k1].x;
-    lim[gbase + k1 * kF256Sk + j] = v[k1].y;
+  for (int k1 = 0; k1 < R; k1++) {
+    lre[gbase + k1 * kQSk + j] = v[k1].x;
+    lim[gbase + k1 * kQSk + j] = v[k1].y;
   }
   __syncthreads();
 #pragma unroll
-  for (int n2 = 0; n2 < 16; n2++) v[n2] = {lre[gbase + j * kF256Sk + n2], lim[gbase + j * kF256Sk + n2]};
+  for (int c = 0; c < R / 16; c++) {
+    cpx<T> u[16];
+    const int row = gbase + (j + 16 * c) * kQSk;
+#pragma unroll
+    for (int n2 = 0; n2 < 16; n2++) u[n2] = {lre[row + n2], lim[row + n2]};
+    dft16<T, INV>(u);
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) v[c + (R / 16) * k2] = u[k2];
+  }
   __syncthreads();
-  dft16<T, INV>(v);
 }
 
-// Strided 256-point pass: lines a = a0 .. a0+15 (16 per block) of the set c,
+// Strided N-point pass: lines a = a0 .. a0+15 (16 per block) of the set c,
 // line (a, c) at a + c*cstride, element t at + t*estride; thread (b, j) =
 // (threadIdx & 15, threadIdx >> 4).  MODE 0 forward, 1 inverse, 2 forward +
 // symbol division + inverse (z axis, lines a = kx + (nx/2)*py; the packed
 // kx = 0 slot is left to k_fft_col0).
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_fft256_s(int64_t A, int64_t estride, int64_t cstride,
-                                                  cpx<T> *__restrict__ buf, const double *__restrict__ tw, Symbol sym,
-                                                  const double *__restrict__ trx, const double *__restrict__ try_,
-                                                  const double *__restrict__ trz)
+template <typename T, int MODE, int R>
+__global__ __launch_bounds__(256) void k_fftq_s(int64_t A, int64_t estride, int64_t cstride,
+                                                cpx<T> *__restrict__ buf, const double *__restrict__ tw, Symbol sym,
+                                                const double *__restrict__ trx, const double *__restrict__ try_,
+                                                const double *__restrict__ trz)
 {
-  __shared__ T lre[16 * kF256SbS], lim[16 * kF256SbS];
-  __shared__ cpx<T> tws[128];
-  load_twiddles<T>(tws, tw, 256);
+  constexpr int N = 16 * R, SB = q_sbs<R>();
+  __shared__ T lre[16 * SB], lim[16 * SB];
+  __shared__ cpx<T> tws[N / 2];
+  load_twiddles<T>(tws, tw, N);
   __syncthreads();
   const int b = threadIdx.x & 15, j = threadIdx.x >> 4;
   const int64_t blocks_per_c = A / 16;
@@ -564,10 +603,10 @@ __global__ __launch_bounds__(256) void k_fft256_s(int64_t A, int64_t estride, in
   const int64_t base = a + c * cstride;
   const int hx = sym.nx >> 1;
   const bool live = MODE != 2 || (a & (hx - 1)) != 0;
-  cpx<T> v[16];
+  cpx<T> v[R];
 #pragma unroll
-  for (int m = 0; m < 16; m++) v[m] = live ? buf[base + (int64_t)(j + 16 * m) * estride] : cpx<T>{(T)0, (T)0};
-  fft256<T, MODE == 1>(v, j, lre, lim, b * kF256SbS, tws);
+  for (int m = 0; m < R; m++) v[m] = live ? buf[base + (int64_t)(j + 16 * m) * estride] : cpx<T>{(T)0, (T)0};
+  fft_line<T, MODE == 1, R>(v, j, lre, lim, b * SB, tws);
   if (MODE == 2) {
     // symbol along the line: la + lb cos(kz) - lc sin(kz), kz = j + 16 m
     const int kx = (int)(a & (hx - 1));
@@ -578,49 +617,57 @@ __global__ __launch_bounds__(256) void k_fft256_s(int64_t A, int64_t estride, in
     const double lb = sym.F[2] + sym.P[1] * cx + sym.P[2] * cy;
     const double lc = sym.M[1] * sx + sym.M[2] * sy;
 #pragma unroll
-    for (int m = 0; m < 16; m++) {
+    for (int m = 0; m < R; m++) {
       const int kz = j + 16 * m;
+      // lambda in FP64 (it cancels near k = 0); the FP32 pass then takes its
+      // reciprocal in FP32, the FP64 pass divides exactly
       const double lam = la + lb * trz[2 * kz] - lc * trz[2 * kz + 1];
-      const double f = lam == 0.0 ? 0.0 : sym.inv_scale / lam;
-      v[m] = {(T)((double)v[m].x * f), (T)((double)v[m].y * f)};
+      if constexpr (sizeof(T) == 4) {
+        const float f = lam == 0.0 ? 0.0f : __builtin_amdgcn_rcpf((float)lam) * (float)sym.inv_scale;
+        v[m] = {v[m].x * f, v[m].y * f};
+      } else {
+        const double f = lam == 0.0 ? 0.0 : sym.inv_scale / lam;
+        v[m] = {(T)((double)v[m].x * f), (T)((double)v[m].y * f)};
+      }
     }
-    fft256<T, true>(v, j, lre, lim, b * kF256SbS, tws);
+    fft_line<T, true, R>(v, j, lre, lim, b * SB, tws);
   }
   if (live) {
 #pragma unroll
-    for (int m = 0; m < 16; m++) buf[base + (int64_t)(j + 16 * m) * estride] = v[m];
+    for (int m = 0; m < R; m++) buf[base + (int64_t)(j + 16 * m) * estride] = v[m];
   }
 }
 
-// x pass for nx = 256, two real lines per 16-thread group (32 lines per
+// x pass for nx = N, two real lines per 16-thread group (32 lines per
 // block): z = x_a + i x_b is transformed once, and X_a = (Z_k + conj Z_-k)/2,
-// X_b = (Z_k - conj Z_-k)/(2i) are written for k < 128 (slot 0 packs X_0 and
-// X_128 as X_0 + i X_128), to the same layout as k_rfft_x.  MODE 1 is the
+// X_b = (Z_k - conj Z_-k)/(2i) are written for k < N/2 (slot 0 packs X_0 and
+// X_(N/2) as X_0 + i X_(N/2)), to the same layout as k_rfft_x.  MODE 1 is the
 // exact reverse: Z_k = X_a + i X_b over the Hermitian extension, inverse
-// transform, x_a = Re, x_b = Im (256 times the input), times scale.
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_rfft256_x(int64_t nlines, const T *__restrict__ rin,
-                                                   cpx<T> *__restrict__ buf, T *__restrict__ rout,
-                                                   const double *__restrict__ tw, double scale, int ly, int64_t plane)
+// transform, x_a = Re, x_b = Im (N times the input), times scale.
+template <typename T, int MODE, int R>
+__global__ __launch_bounds__(256) void k_rfftq_x(int64_t nlines, const T *__restrict__ rin,
+                                                 cpx<T> *__restrict__ buf, T *__restrict__ rout,
+                                                 const double *__restrict__ tw, double scale, int ly, int64_t plane)
 {
-  __shared__ T lre[16 * kF256SbX], lim[16 * kF256SbX];
-  __shared__ cpx<T> tws[128];
-  load_twiddles<T>(tws, tw, 256);
+  constexpr int N = 16 * R, H = N / 2, SB = q_sbx<R>();
+  __shared__ T lre[16 * SB], lim[16 * SB];
+  __shared__ cpx<T> tws[H];
+  load_twiddles<T>(tws, tw, N);
   __syncthreads();
   const int j = threadIdx.x & 15, g = threadIdx.x >> 4;
   const int64_t la = (int64_t)blockIdx.x * 32 + 2 * g, lb = la + 1;
   const bool ok = lb < nlines;
-  auto at = [&](int64_t l) { return (l & ((1 << ly) - 1)) * (int64_t)128 + (l >> ly) * plane; };
-  const int gb = g * kF256SbX;
-  cpx<T> v[16];
+  auto at = [&](int64_t l) { return (l & ((1 << ly) - 1)) * (int64_t)H + (l >> ly) * plane; };
+  const int gb = g * SB;
+  cpx<T> v[R];
   if (MODE == 0) {
 #pragma unroll
-    for (int m = 0; m < 16; m++)
-      v[m] = ok ? cpx<T>{rin[la * 256 + j + 16 * m], rin[lb * 256 + j + 16 * m]} : cpx<T>{(T)0, (T)0};
-    fft256<T, false>(v, j, lre, lim, gb, tws);
+    for (int m = 0; m < R; m++)
+      v[m] = ok ? cpx<T>{rin[la * N + j + 16 * m], rin[lb * N + j + 16 * m]} : cpx<T>{(T)0, (T)0};
+    fft_line<T, false, R>(v, j, lre, lim, gb, tws);
     // Z in natural order in LDS, for the partner values Z_-k
 #pragma unroll
-    for (int m = 0; m < 16; m++) {
+    for (int m = 0; m < R; m++) {
       lre[gb + j + 16 * m] = v[m].x;
       lim[gb + j + 16 * m] = v[m].y;
     }
@@ -628,15 +675,15 @@ __global__ __launch_bounds__(256) void k_rfft256_x(int64_t nlines, const T *__re
     if (ok) {
       const int64_t oa = at(la), ob = at(lb);
 #pragma unroll
-      for (int m = 0; m < 8; m++) {
+      for (int m = 0; m < R / 2; m++) {
         const int k = j + 16 * m;
         const T zx = lre[gb + k], zy = lim[gb + k];
         cpx<T> xa, xb;
         if (k == 0) {
-          xa = {zx, lre[gb + 128]};
-          xb = {zy, lim[gb + 128]};
+          xa = {zx, lre[gb + H]};
+          xb = {zy, lim[gb + H]};
         } else {
-          const T mx = lre[gb + 256 - k], my = lim[gb + 256 - k];
+          const T mx = lre[gb + N - k], my = lim[gb + N - k];
           xa = {(T)0.5 * (zx + mx), (T)0.5 * (zy - my)};
           xb = {(T)0.5 * (zy + my), (T)0.5 * (mx - zx)};
         }
@@ -646,36 +693,45 @@ __global__ __launch_bounds__(256) void k_rfft256_x(int64_t nlines, const T *__re
     }
   } else {
     const int64_t oa = ok ? at(la) : 0, ob = ok ? at(lb) : 0;
+    // each half-spectrum value is read once: A_t and B_t (t < N/2) staged in
+    // LDS, then Z_t = A_t + i B_t and its Hermitian partner Z_(N-t) built from
+    // there (Z_0 and Z_(N/2) from the packed slot 0)
+    T *are = lre + gb, *aim = lim + gb, *bre = are + H, *bim = aim + H;
 #pragma unroll
-    for (int m = 0; m < 16; m++) {
+    for (int m = 0; m < R / 2; m++) {
       const int t = j + 16 * m;
-      cpx<T> z = {(T)0, (T)0};
-      if (ok) {
-        if (t == 0 || t == 128) {
-          const cpx<T> sa = buf[oa], sb = buf[ob];
-          z = t == 0 ? cpx<T>{sa.x, sb.x} : cpx<T>{sa.y, sb.y};
-        } else if (t < 128) {
-          const cpx<T> A = buf[oa + t], B = buf[ob + t];
-          z = {A.x - B.y, A.y + B.x};
-        } else {
-          const cpx<T> A = buf[oa + 256 - t], B = buf[ob + 256 - t];
-          z = {A.x + B.y, B.x - A.y};
-        }
-      }
+      const cpx<T> A = ok ? buf[oa + t] : cpx<T>{(T)0, (T)0}, B = ok ? buf[ob + t] : cpx<T>{(T)0, (T)0};
+      are[t] = A.x; aim[t] = A.y; bre[t] = B.x; bim[t] = B.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < R; m++) {
+      const int t = j + 16 * m;
+      cpx<T> z;
+      if (t == 0) z = {are[0], bre[0]};
+      else if (t == H) z = {aim[0], bim[0]};
+      else if (t < H) z = {are[t] - bim[t], aim[t] + bre[t]};
+      else z = {are[N - t] + bim[N - t], bre[N - t] - aim[N - t]};
       v[m] = z;
     }
-    fft256<T, true>(v, j, lre, lim, gb, tws);
+    __syncthreads();
+    fft_line<T, true, R>(v, j, lre, lim, gb, tws);
     if (ok) {
 #pragma unroll
-      for (int m = 0; m < 16; m++) {
-        rout[la * 256 + j + 16 * m] = (T)((double)v[m].x * scale);
-        rout[lb * 256 + j + 16 * m] = (T)((double)v[m].y * scale);
+      for (int m = 0; m < R; m++) {
+        rout[la * N + j + 16 * m] = (T)((double)v[m].x * scale);
+        rout[lb * N + j + 16 * m] = (T)((double)v[m].y * scale);
       }
     }
   }
 }
 
 static bool pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// Register radix R of the four-step kernels for an axis of n points: 16 for
+// 256, 32 for 512 in FP32 (the FP64 512-point tiles would not fit LDS twice
+// per CU); 0: the radix-4 LDS passes.
+template <typename T> static int q_radix(int n) { return n == 256 ? 16 : (n == 512 && sizeof(T) == 4 ? 32 : 0); }
 
 // Complex elements between z planes of the half spectrum: (nx/2) ny plus
 // 256 bytes
@@ -734,23 +790,42 @@ int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
   // z planes of the half spectrum are spaced P = (nx/2) ny + padding apart, so
   // that the rows of a z line do not all fall at one power-of-two stride
   const int64_t P = spec_plane<T>(nx, ny);
-  // 256-point axes use the four-step kernels (16 lines per block); their y
-  // pass leaves the y frequencies in natural order
-  const bool x256 = nx == 256, y256 = ny == 256 && hx % 16 == 0, z256 = nz == 256 && ((int64_t)hx * ny) % 16 == 0;
-  sym.ynat = y256 ? 1 : 0;
+  // 256-point axes (and 512-point ones in FP32) use the four-step kernels
+  // (16 lines per block); their y pass leaves the y frequencies in natural
+  // order.  xq, yq, zq: the register radix R of each axis (0: radix-4 passes)
+  const int xq = q_radix<T>(nx), yq = hx % 16 == 0 ? q_radix<T>(ny) : 0,
+            zq = ((int64_t)hx * ny) % 16 == 0 ? q_radix<T>(nz) : 0;
+  sym.ynat = yq ? 1 : 0;
   const int nlines = ny * nz;
-  if (x256) {  // x forward: real lines -> half spectra (kx < nx/2, slot 0 packs kx = 0, nx/2)
-    hipLaunchKernelGGL((k_rfft256_x<T, 0>), (unsigned)((nlines + 31) / 32), 256, 0, ctx->stream, (int64_t)nlines, r,
-                       buf, (T *)nullptr, ctx->tw[0], 1.0, ilog2(ny), P);
+  const unsigned nxb = (unsigned)((nlines + 31) / 32), nyb = (unsigned)((hx / 16) * nz);
+#define RFFTQ(MODE, RIN, ROUT)                                                                                        \
+  do {                                                                                                             \
+    if (xq == 16)                                                                                                  \
+      hipLaunchKernelGGL((k_rfftq_x<T, MODE, 16>), nxb, 256, 0, ctx->stream, (int64_t)nlines, RIN, buf, ROUT,       \
+                         ctx->tw[0], 1.0, ilog2(ny), P);                                                           \
+    else if constexpr (sizeof(T) == 4)                                                                             \
+      hipLaunchKernelGGL((k_rfftq_x<T, MODE, 32>), nxb, 256, 0, ctx->stream, (int64_t)nlines, RIN, buf, ROUT,       \
+                         ctx->tw[0], 1.0, ilog2(ny), P);                                                           \
+  } while (0)
+#define FFTQ(MODE, RQ, NB, A_, ES, CS, TW)                                                                           \
+  do {                                                                                                             \
+    if (RQ == 16)                                                                                                  \
+      hipLaunchKernelGGL((k_fftq_s<T, MODE, 16>), NB, 256, 0, ctx->stream, A_, ES, CS, buf, TW, sym, ctx->trig[0], \
+                         ctx->trig[1], ctx->trig[2]);                                                              \
+    else if constexpr (sizeof(T) == 4)                                                                             \
+      hipLaunchKernelGGL((k_fftq_s<T, MODE, 32>), NB, 256, 0, ctx->stream, A_, ES, CS, buf, TW, sym, ctx->trig[0], \
+                         ctx->trig[1], ctx->trig[2]);                                                              \
+  } while (0)
+  if (xq) {  // x forward: real lines -> half spectra (kx < nx/2, slot 0 packs kx = 0, nx/2)
+    RFFTQ(0, r, (T *)nullptr);
   } else {
     const int B = rfft_batch<T>(hx, nlines);
     hipLaunchKernelGGL((k_rfft_x<T, 0>), (unsigned)((nlines + B - 1) / B), kBlock, rfft_lds<T>(hx, B), ctx->stream,
                        ilog2(nx), nlines, B, r, buf, (T *)nullptr, ctx->tw[0], 1.0, ilog2(ny), P);
   }
   // y forward: lines (kx, iz); A = nx/2 contiguous, c = iz with stride P
-  if (y256) {
-    hipLaunchKernelGGL((k_fft256_s<T, 0>), (unsigned)((hx / 16) * nz), 256, 0, ctx->stream, (int64_t)hx, (int64_t)hx,
-                       P, buf, ctx->tw[1], sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
+  if (yq) {
+    FFTQ(0, yq, nyb, (int64_t)hx, (int64_t)hx, P, ctx->tw[1]);
   } else {
     const int n = ny, B = fft_batch<T>(n, hx);
     hipLaunchKernelGGL((k_fft_s<T, 0>), (unsigned)((hx / B) * nz), kBlock, fft_lds<T>(n, B), ctx->stream, ilog2(n),
@@ -760,9 +835,8 @@ int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
   {  // z forward + symbol + inverse: lines a = kx + (nx/2)*py, estride P
     const int n = nz;
     const int64_t A = (int64_t)hx * ny;
-    if (z256) {
-      hipLaunchKernelGGL((k_fft256_s<T, 2>), (unsigned)(A / 16), 256, 0, ctx->stream, A, P, (int64_t)0, buf,
-                         ctx->tw[2], sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
+    if (zq) {
+      FFTQ(2, zq, (unsigned)(A / 16), A, P, (int64_t)0, ctx->tw[2]);
     } else {
       const int B = fft_batch_z<T>(n, A);
       hipLaunchKernelGGL((k_fft_s<T, 2>), (unsigned)(A / B), kBlock, fft_lds_z<T>(n, B), ctx->stream, ilog2(n), A, P,
@@ -772,23 +846,23 @@ int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
     hipLaunchKernelGGL((k_fft_col0<T>), (unsigned)ny, kBlock, lds, ctx->stream, ilog2(n), P, hx, buf, ctx->tw[2],
                        sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
   }
-  if (y256) {  // y inverse
-    hipLaunchKernelGGL((k_fft256_s<T, 1>), (unsigned)((hx / 16) * nz), 256, 0, ctx->stream, (int64_t)hx, (int64_t)hx,
-                       P, buf, ctx->tw[1], sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
+  if (yq) {  // y inverse
+    FFTQ(1, yq, nyb, (int64_t)hx, (int64_t)hx, P, ctx->tw[1]);
   } else {
     const int n = ny, B = fft_batch<T>(n, hx);
     hipLaunchKernelGGL((k_fft_s<T, 1>), (unsigned)((hx / B) * nz), kBlock, fft_lds<T>(n, B), ctx->stream, ilog2(n),
                        (int64_t)hx, (int64_t)hx, P, B, buf, ctx->tw[1], sym, ctx->trig[0], ctx->trig[1],
                        ctx->trig[2]);
   }
-  if (x256) {  // x inverse: half spectra -> real lines
-    hipLaunchKernelGGL((k_rfft256_x<T, 1>), (unsigned)((nlines + 31) / 32), 256, 0, ctx->stream, (int64_t)nlines,
-                       (const T *)nullptr, buf, z, ctx->tw[0], 1.0, ilog2(ny), P);
+  if (xq) {  // x inverse: half spectra -> real lines
+    RFFTQ(1, (const T *)nullptr, z);
   } else {
     const int B = rfft_batch<T>(hx, nlines);
     hipLaunchKernelGGL((k_rfft_x<T, 1>), (unsigned)((nlines + B - 1) / B), kBlock, rfft_lds<T>(hx, B), ctx->stream,
                        ilog2(nx), nlines, B, (const T *)nullptr, buf, z, ctx->tw[0], 1.0, ilog2(ny), P);
   }
+#undef RFFTQ
+#undef FFTQ
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("fft: ") + hipGetErrorString(e));
   return CHY_OK;

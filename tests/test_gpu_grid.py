@@ -224,6 +224,8 @@ PCS = {"jacobi": orc.PC_JACOBI, "fft": orc.PC_FFT, "zline": orc.PC_ZLINE, "cheby
                                   ("fft2", (32, 16, 8)), ("fft2z", (16, 8, 32)), ("fft2z", (32, 16, 8)),
                                   # z-line correction on long columns: 8 and 16 register segments
                                   ("fft2z", (16, 8, 256)), ("fft2z", (8, 8, 512)),
+                                  # 512-point x and y lines: four-step kernels in FP32
+                                  ("fft", (512, 4, 4)), ("fft", (32, 512, 4)), ("fft2z", (512, 8, 4)),
                                   ("zline", (16, 8, 12)),
                                   ("zline", (8, 6, 3)), ("zline", (6, 4, 5)), ("cheby", (16, 8, 12)),
                                   # multigrid: 2 levels (16x8, 8x4), 3 levels (24x20, 12x10, 6x5)
@@ -246,9 +248,12 @@ def test_pc_apply_matches_oracle(pc, N, prec):
     n = mom[0].astype(np.float32).astype(np.float64) if prec == "mixed" else mom[0]
     ref = orc.pc_apply(g, K, n, r, pc=PCS[pc], degree=deg)
     z = z - z.mean()
-    # FP64: rounding of the FFT and of the z-line recurrences grows with the
-    # line length (512-plane columns: 1.4e-12)
-    tol = (1e-12 if max(N) <= 256 else 4e-12) if prec == "fp64" else (1e-4 if pc == "mg" else 2e-5)
+    # FP64: the symbol lambda(k) = sum_q v_q cos(k . o_q) cancels near k = 0,
+    # where its relative rounding is about eps sum|v_q| / |lambda(k)|, growing
+    # as N^2 for the lowest mode of an N-point axis; the device and the oracle
+    # sum its terms in different orders.  On 512-point axes the two
+    # preconditioners differ by 1.4e-12 (z), 1.1e-11 (y) and 4.4e-11 (x).
+    tol = (1e-12 if max(N) <= 256 else 1e-10) if prec == "fp64" else (1e-4 if pc == "mg" else 2e-5)
     assert relmax(z, ref) <= tol, relmax(z, ref)
 
 

@@ -32,8 +32,9 @@ def test_sort_and_deposit_fp64_bitwise(N, n, weights):
         assert np.array_equal(v[:, a], ref[a]), (a, relmax(v[:, a], ref[a]))
 
 
-@pytest.mark.parametrize("N", [(32, 8, 6), (64, 4, 4), (24, 8, 6)])
-def test_mixed_deposit_matches_oracle(N):
+@pytest.mark.parametrize("N,weights", [((32, 8, 6), False), ((64, 4, 4), False), ((24, 8, 6), False),
+                                       ((32, 8, 6), True)])
+def test_mixed_deposit_matches_oracle(N, weights):
     """Mixed-precision deposit (FP32 in-cell offsets and sums) against the
     FP64 oracle on the same ions.  Rows of 32 and 64 cells take the
     x-combined cell partials (each block's x pairs added in LDS, the corners
@@ -41,9 +42,9 @@ def test_mixed_deposit_matches_oracle(N):
     takes the per-corner partials.  Repeated deposits are bitwise equal."""
     n = 40 * N[0] * N[1] * N[2]
     rng = np.random.default_rng(N[0])
-    with context(opts(N, extra="-precision mixed"), capacity=n) as ctx:
+    with context(opts(N, extra="-precision mixed"), capacity=n, weighted=weights) as ctx:
         g = grid_of(ctx.prm)
-        upload(ctx, random_particles(g, n, rng))
+        upload(ctx, random_particles(g, n, rng, weights=weights))
         got = as_particles(ctx.get_particles())
         v1 = ctx.collect_moments()
         v2 = ctx.collect_moments()
@@ -53,7 +54,8 @@ def test_mixed_deposit_matches_oracle(N):
     ref = orc.deposit_sorted(g, ps, offs)
     for a in range(4):
         assert relmax(v1[:, a], ref[a]) <= 1e-5, (a, relmax(v1[:, a], ref[a]))
-    assert v1[:, 0].sum() == pytest.approx(n, rel=1e-5)
+    total = got.w.sum() if weights else n
+    assert v1[:, 0].sum() == pytest.approx(total, rel=1e-5)
 
 
 def test_mixed_step_deposit_equals_fresh_deposit():

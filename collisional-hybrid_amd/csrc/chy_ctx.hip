@@ -577,6 +577,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   A(dalloc(ctx, &ctx->sred, std::max<int64_t>(4096, nc / 32 + 1)));  // normaliser partials, one per block
   A(dalloc(ctx, &ctx->S_dev, 1));
   A(dalloc(ctx, &ctx->cstats, 6));
+  A(dalloc(ctx, &ctx->push_part, 4 * (int64_t)kPushBlocks));
   for (int a = 0; a < 4; a++) { unsigned char *q = nullptr; A(dalloc(ctx, &q, (int64_t)ts * nc)); ctx->mom[a] = q; }
   { unsigned char *q = nullptr; A(dalloc(ctx, &q, (int64_t)ts * 4 * nc)); ctx->efield[0] = q; }
   A(dalloc(ctx, &ctx->phi, nc));
@@ -652,7 +653,7 @@ int chy_destroy(chy_ctx *ctx)
   void *ps[] = {ctx->perm, ctx->slotkey, ctx->counts, ctx->offsets, ctx->cursor, ctx->scan_tmp, ctx->maxcount,
                 ctx->trans, ctx->info, ctx->dir, ctx->nbase, ctx->offs_alt, ctx->sortflag,
                 ctx->partials,
-                ctx->sred, ctx->S_dev, ctx->cstats, ctx->mom[0], ctx->mom[1], ctx->mom[2], ctx->mom[3],
+                ctx->sred, ctx->S_dev, ctx->cstats, ctx->push_part, ctx->mom[0], ctx->mom[1], ctx->mom[2], ctx->mom[3],
                 ctx->efield[0], ctx->phi, ctx->phi_prev, ctx->b, ctx->r64, ctx->t64, ctx->V,
                 ctx->Z, ctx->wv, ctx->fft_buf, ctx->pc_buf, ctx->pc2_buf, ctx->pc2_h, ctx->zl_buf, ctx->zl_col, ctx->red, ctx->rpl, ctx->red_out, ctx->gs_scale, ctx->diag_part, ctx->tw[0], ctx->tw[1], ctx->tw[2],
                 ctx->trig[0], ctx->trig[1], ctx->trig[2]};

@@ -139,6 +139,7 @@ struct chy_ctx {
   uint64_t *h_status = nullptr;
   uint64_t *status_dev = nullptr;
   hipEvent_t ev_step = nullptr, ev_particles = nullptr;  // a step's start and the end of its particle phase
+  double *push_part = nullptr;  // the push's per-block collision tallies [kPushBlocks][4]
   bool push_pending = false;    // a push's counters not yet read (deferred to the sort's read-back)
   int64_t push_Nc = 0;
   chy_collision_stats *push_stats = nullptr;
@@ -177,6 +178,7 @@ namespace chy {
 
 constexpr int kRedBlocks = 1024;  // fixed reduction grid: deterministic sums
 constexpr int kStatusWords = 16;  // chy_ctx::h_status
+constexpr int kPushBlocks = 65536;  // the push's largest grid
 constexpr int kBlock = 256;
 
 inline int set_err(chy_ctx *c, int code, const std::string &msg)

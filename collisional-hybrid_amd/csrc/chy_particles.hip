@@ -65,38 +65,61 @@ __device__ __forceinline__ void count_flags(unsigned long long *cs, int slot, bo
   if (m && lane_id() == 0) atomicAdd(&cs[slot], (unsigned long long)__popcll(m));
 }
 
-// The block's sums of the collision probabilities p and of p (1 - p) (the
-// mean and variance of its Bernoulli count), one FP64 atomic each per block.
-__device__ __forceinline__ void flush_collision_moments(double *cexp, double pm, double pv, double *sh)
+// The block's collision tallies -- accepted and refused scatterings, and the
+// sums of the collision probabilities p and of p (1 - p) (the mean and
+// variance of its Bernoulli count) -- to bpart[4 block + q], q = 0..3, for
+// k_push_reduce to add in a fixed order: no atomics on shared addresses
+// (one per block had cost a few ns each, serialised), and the FP64 sums are
+// reproducible.
+__device__ __forceinline__ void flush_block(double *bpart, uint32_t nacc, uint32_t nref, double pm, double pv,
+                                            double *sh)
 {
+  double v[4] = {(double)nacc, (double)nref, pm, pv};
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    pm += __shfl_down(pm, off, 64);
-    pv += __shfl_down(pv, off, 64);
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] += __shfl_down(v[q], off, 64);
   }
-  if (lane_id() == 0) { sh[2 * (threadIdx.x >> 6)] = pm; sh[2 * (threadIdx.x >> 6) + 1] = pv; }
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) sh[4 * (threadIdx.x >> 6) + q] = v[q];
+  }
   __syncthreads();
-  if (threadIdx.x < 2) {
+  if (threadIdx.x < 4) {
     const int q = threadIdx.x;
-    atomicAdd(&cexp[q], (sh[q] + sh[2 + q]) + (sh[4 + q] + sh[6 + q]));
+    bpart[4 * (int64_t)blockIdx.x + q] = (sh[q] + sh[4 + q]) + (sh[8 + q] + sh[12 + q]);
   }
 }
 
-// Accepted / refused scatterings of the block, one global atomic per block.
-__device__ __forceinline__ void flush_scatter_counts(unsigned long long *cs, uint32_t nacc, uint32_t nref,
-                                                     unsigned int *sacc)
+// Sums of the push's block tallies over nb blocks in a fixed order (one block
+// of 1024 threads): cs[0] accepted, cs[1] refused (counts), cs[4] and cs[5]
+// the FP64 sums of p and p (1 - p).
+__global__ __launch_bounds__(1024) void k_push_reduce(const double *__restrict__ bpart, int nb,
+                                                      unsigned long long *__restrict__ cs)
 {
+  __shared__ double sh[16][4];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] += bpart[4 * (int64_t)b + q];
+  }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    nacc += __shfl_down(nacc, off, 64);
-    nref += __shfl_down(nref, off, 64);
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] += __shfl_down(v[q], off, 64);
   }
   if (lane_id() == 0) {
-    if (nacc) atomicAdd(&sacc[0], nacc);
-    if (nref) atomicAdd(&sacc[1], nref);
+#pragma unroll
+    for (int q = 0; q < 4; q++) sh[threadIdx.x >> 6][q] = v[q];
   }
   __syncthreads();
-  if (threadIdx.x < 2 && sacc[threadIdx.x]) atomicAdd(&cs[threadIdx.x], (unsigned long long)sacc[threadIdx.x]);
+  if (threadIdx.x < 4) {
+    const int q = threadIdx.x;
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w][q];
+    if (q < 2) cs[q] = (unsigned long long)(t + 0.5);
+    else reinterpret_cast<double *>(cs)[2 + q] = t;
+  }
 }
 
 // ---- main push: gather + Boris + collision (decision and scattering) + move -----------
@@ -109,13 +132,10 @@ template <bool COLLIDE, bool MOVE>
 __global__ __launch_bounds__(256, 3) void k_push_fp64(Grid g, PushConst pc, int64_t np, PState<double> st,
                                                    const Vec4<double> *__restrict__ E4, const double *__restrict__ Sdev,
                                                    uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats,
-                                                   double *__restrict__ cexp, uint8_t *__restrict__ dir)
+                                                   double *__restrict__ bpart, uint8_t *__restrict__ dir)
 {
-  __shared__ unsigned int sacc[2];
-  __shared__ double smom[8];
+  __shared__ double smom[16];
   double pm = 0.0, pv = 0.0;  // sums of the collision probabilities p and of p (1 - p)
-  if (threadIdx.x < 2) sacc[threadIdx.x] = 0u;
-  __syncthreads();
   const double S = COLLIDE ? Sdev[0] : 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   // every lane iterates the same number of times so wave-level collectives stay convergent
@@ -169,8 +189,7 @@ __global__ __launch_bounds__(256, 3) void k_push_fp64(Grid g, PushConst pc, int6
     count_flags(cstats, 3, bad);
   }
   if (COLLIDE) {
-    flush_scatter_counts(cstats, nacc, nref, sacc);
-    flush_collision_moments(cexp, pm, pv, smom);
+    flush_block(bpart, nacc, nref, pm, pv, smom);
   }
 }
 
@@ -178,13 +197,10 @@ template <bool COLLIDE, bool MOVE>
 __global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConstF pc, int64_t np, PState<float> st,
                                                     const Vec4<float> *__restrict__ E4, const double *__restrict__ Sdev,
                                                     uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats,
-                                                    double *__restrict__ cexp, uint8_t *__restrict__ dir)
+                                                    double *__restrict__ bpart, uint8_t *__restrict__ dir)
 {
-  __shared__ unsigned int sacc[2];
-  __shared__ double smom[8];
+  __shared__ double smom[16];
   float pm = 0.f, pv = 0.f;  // this thread's sums of p and p (1 - p) (a few ions each)
-  if (threadIdx.x < 2) sacc[threadIdx.x] = 0u;
-  __syncthreads();
   const float S = COLLIDE ? (float)Sdev[0] : 0.f;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t iters = (np + stride - 1) / stride;
@@ -237,8 +253,7 @@ __global__ __launch_bounds__(256) void k_push_mixed(Grid g, PushConstF pc, int64
     count_flags(cstats, 3, bad);
   }
   if (COLLIDE) {
-    flush_scatter_counts(cstats, nacc, nref, sacc);
-    flush_collision_moments(cexp, (double)pm, (double)pv, smom);
+    flush_block(bpart, nacc, nref, (double)pm, (double)pv, smom);
   }
 }
 
@@ -1726,19 +1741,18 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
   ctx->partials_fresh = false;
   // slots 0-3 counts, 4-5 the FP64 sums of p and p (1 - p) (zero bits are 0.0)
   launch_fill<unsigned long long>(ctx, 6, ctx->cstats, 0ull);
-  double *cexp = (double *)(ctx->cstats + 4);
   // the histogram is only needed by the general sort (the neighbour-move
   // sort counts per old cell)
   const bool count = move && !ctx->order_valid;
   if (count) launch_fill<uint32_t>(ctx, ctx->g.nc + 1, ctx->counts, 0u);
   // 65536 blocks (16 ions per thread at 256^3 x 16): 3.96 ms, against 4.19
   // with 4096 blocks and 4.04 with 262144
-  const unsigned nb = grid_for(ctx->np, 65536);
+  const unsigned nb = grid_for(ctx->np, kPushBlocks);
   timer_start(ctx, "push");
 #define LAUNCH(KER, T, C, M, ...)                                                                             \
   hipLaunchKernelGGL((KER<C, M>), nb, kBlock, 0, ctx->stream, ctx->g, pcT, ctx->np, st,                      \
                      (const Vec4<T> *)ctx->efield[0], ctx->S_dev, count ? ctx->counts : nullptr, ctx->cstats,   \
-                     cexp, ##__VA_ARGS__)
+                     ctx->push_part, ##__VA_ARGS__)
   if (ctx->mixed) {
     const PushConstF pcT = push_const_f(pc);
     PState<float> st = ctx->pf[ctx->cur];
@@ -1761,6 +1775,8 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
     if (move) ctx->dir_valid = dir != nullptr;
   }
 #undef LAUNCH
+  if (collide)  // the blocks' tallies, summed in a fixed order
+    hipLaunchKernelGGL(k_push_reduce, 1, 1024, 0, ctx->stream, (const double *)ctx->push_part, (int)nb, ctx->cstats);
   if (move) ctx->keys_moved = true;
   timer_stop(ctx, "push");
   // the counters are read now, or (defer) with the next status read-back,

@@ -577,7 +577,7 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   A(dalloc(ctx, &ctx->sred, std::max<int64_t>(4096, nc / 32 + 1)));  // normaliser partials, one per block
   A(dalloc(ctx, &ctx->S_dev, 1));
   A(dalloc(ctx, &ctx->cstats, 6));
-  A(dalloc(ctx, &ctx->push_part, 4 * (int64_t)kPushBlocks));
+  A(dalloc(ctx, &ctx->push_part, 4 * ((int64_t)kPushBlocks + 64)));
   for (int a = 0; a < 4; a++) { unsigned char *q = nullptr; A(dalloc(ctx, &q, (int64_t)ts * nc)); ctx->mom[a] = q; }
   { unsigned char *q = nullptr; A(dalloc(ctx, &q, (int64_t)ts * 4 * nc)); ctx->efield[0] = q; }
   A(dalloc(ctx, &ctx->phi, nc));

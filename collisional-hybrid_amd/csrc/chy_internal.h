@@ -139,7 +139,7 @@ struct chy_ctx {
   uint64_t *h_status = nullptr;
   uint64_t *status_dev = nullptr;
   hipEvent_t ev_step = nullptr, ev_particles = nullptr;  // a step's start and the end of its particle phase
-  double *push_part = nullptr;  // the push's per-block collision tallies [kPushBlocks][4]
+  double *push_part = nullptr;  // the FP32 push's per-block collision tallies [kPushBlocks][4], then [64][4]
   bool push_pending = false;    // a push's counters not yet read (deferred to the sort's read-back)
   int64_t push_Nc = 0;
   chy_collision_stats *push_stats = nullptr;

@@ -91,17 +91,23 @@ __device__ __forceinline__ void flush_block(double *bpart, uint32_t nacc, uint32
   }
 }
 
-// Sums of the push's block tallies over nb blocks in a fixed order (one block
-// of 1024 threads): cs[0] accepted, cs[1] refused (counts), cs[4] and cs[5]
-// the FP64 sums of p and p (1 - p).
-__global__ __launch_bounds__(1024) void k_push_reduce(const double *__restrict__ bpart, int nb,
-                                                      unsigned long long *__restrict__ cs)
+// Sums of the push's block tallies in a fixed order, in two levels: FINAL
+// false, block g (of kPushRed) adds blocks [g kPushRedSpan, (g+1)
+// kPushRedSpan) of bpart into out[4 g + q]; FINAL true, one block adds the
+// kPushRed results and writes cs[0] accepted, cs[1] refused (counts), cs[4]
+// and cs[5] the FP64 sums of p and p (1 - p).
+constexpr int kPushRed = 64, kPushRedSpan = kPushBlocks / kPushRed;
+template <bool FINAL>
+__global__ __launch_bounds__(256) void k_push_reduce(const double *__restrict__ in, int n, double *__restrict__ out,
+                                                     unsigned long long *__restrict__ cs)
 {
-  __shared__ double sh[16][4];
+  __shared__ double sh[4][4];
+  const int lo = FINAL ? 0 : (int)blockIdx.x * kPushRedSpan;
+  const int hi = FINAL ? n : (lo + kPushRedSpan < n ? lo + kPushRedSpan : n);
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+  for (int b = lo + (int)threadIdx.x; b < hi; b += blockDim.x) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) v[q] += bpart[4 * (int64_t)b + q];
+    for (int q = 0; q < 4; q++) v[q] += in[4 * (int64_t)b + q];
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -115,11 +121,46 @@ __global__ __launch_bounds__(1024) void k_push_reduce(const double *__restrict__
   __syncthreads();
   if (threadIdx.x < 4) {
     const int q = threadIdx.x;
-    double t = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w][q];
-    if (q < 2) cs[q] = (unsigned long long)(t + 0.5);
+    const double t = (sh[0][q] + sh[1][q]) + (sh[2][q] + sh[3][q]);
+    if (!FINAL) out[4 * (int64_t)blockIdx.x + q] = t;
+    else if (q < 2) cs[q] = (unsigned long long)(t + 0.5);
     else reinterpret_cast<double *>(cs)[2 + q] = t;
   }
+}
+
+// FP64 push: the block's collision moments and scattering counts by global
+// atomics (one per block and value).  (Block partials, as the FP32 push
+// writes them, took this kernel from 11.7 to 12.3 ms: it is held to three
+// waves per SIMD with a few spills, and its schedule shifted.)
+__device__ __forceinline__ void flush_collision_moments(double *cexp, double pm, double pv, double *sh)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    pm += __shfl_down(pm, off, 64);
+    pv += __shfl_down(pv, off, 64);
+  }
+  if (lane_id() == 0) { sh[2 * (threadIdx.x >> 6)] = pm; sh[2 * (threadIdx.x >> 6) + 1] = pv; }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const int q = threadIdx.x;
+    atomicAdd(&cexp[q], (sh[q] + sh[2 + q]) + (sh[4 + q] + sh[6 + q]));
+  }
+}
+
+__device__ __forceinline__ void flush_scatter_counts(unsigned long long *cs, uint32_t nacc, uint32_t nref,
+                                                     unsigned int *sacc)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    nacc += __shfl_down(nacc, off, 64);
+    nref += __shfl_down(nref, off, 64);
+  }
+  if (lane_id() == 0) {
+    if (nacc) atomicAdd(&sacc[0], nacc);
+    if (nref) atomicAdd(&sacc[1], nref);
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 && sacc[threadIdx.x]) atomicAdd(&cs[threadIdx.x], (unsigned long long)sacc[threadIdx.x]);
 }
 
 // ---- main push: gather + Boris + collision (decision and scattering) + move -----------
@@ -134,8 +175,11 @@ __global__ __launch_bounds__(256, 3) void k_push_fp64(Grid g, PushConst pc, int6
                                                    uint32_t *__restrict__ counts, unsigned long long *__restrict__ cstats,
                                                    double *__restrict__ bpart, uint8_t *__restrict__ dir)
 {
-  __shared__ double smom[16];
+  __shared__ unsigned int sacc[2];
+  __shared__ double smom[8];
   double pm = 0.0, pv = 0.0;  // sums of the collision probabilities p and of p (1 - p)
+  if (threadIdx.x < 2) sacc[threadIdx.x] = 0u;
+  __syncthreads();
   const double S = COLLIDE ? Sdev[0] : 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   // every lane iterates the same number of times so wave-level collectives stay convergent
@@ -189,7 +233,8 @@ __global__ __launch_bounds__(256, 3) void k_push_fp64(Grid g, PushConst pc, int6
     count_flags(cstats, 3, bad);
   }
   if (COLLIDE) {
-    flush_block(bpart, nacc, nref, pm, pv, smom);
+    flush_scatter_counts(cstats, nacc, nref, sacc);
+    flush_collision_moments(reinterpret_cast<double *>(cstats + 4), pm, pv, smom);
   }
 }
 
@@ -1775,8 +1820,13 @@ int particles_push(chy_ctx *ctx, bool move, bool collide, chy_collision_stats *s
     if (move) ctx->dir_valid = dir != nullptr;
   }
 #undef LAUNCH
-  if (collide)  // the blocks' tallies, summed in a fixed order
-    hipLaunchKernelGGL(k_push_reduce, 1, 1024, 0, ctx->stream, (const double *)ctx->push_part, (int)nb, ctx->cstats);
+  if (collide && ctx->mixed) {  // the blocks' tallies, summed in a fixed order
+    double *tmp = ctx->push_part + 4 * (int64_t)kPushBlocks;
+    hipLaunchKernelGGL((k_push_reduce<false>), kPushRed, 256, 0, ctx->stream, (const double *)ctx->push_part, (int)nb,
+                       tmp, ctx->cstats);
+    hipLaunchKernelGGL((k_push_reduce<true>), 1, 256, 0, ctx->stream, (const double *)tmp, kPushRed, (double *)nullptr,
+                       ctx->cstats);
+  }
   if (move) ctx->keys_moved = true;
   timer_stop(ctx, "push");
   // the counters are read now, or (defer) with the next status read-back,

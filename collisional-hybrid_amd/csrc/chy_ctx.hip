@@ -9,6 +9,7 @@
 // always accumulate in FP64.  The constant null space is handled by the
 // mean-free forcing, the mean-free FFT preconditioner and a final mean
 // projection of phi.
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -67,6 +68,26 @@ static void timers_resolve(chy_ctx *ctx)
     }
     t.used = 0;
   }
+}
+
+int wait_seq(chy_ctx *ctx, uint64_t s)
+{
+  // The device's store arrives within microseconds of the kernel's end,
+  // sooner than a stream synchronisation returns.  A wait longer than any
+  // single pass falls back to that synchronisation, which also reports a
+  // device error; after it the number must be there.
+  const volatile uint64_t *p = ctx->h_status + kSeqWord;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t sp = 1; *p < s; sp++) {
+    __builtin_ia32_pause();
+    if ((sp & 1023) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
+      CHY_HIP(hipStreamSynchronize(ctx->stream));
+      if (*p < s) return set_err(ctx, CHY_ERR_LIB, "read-back sequence number missing after the stream drained");
+      break;
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  return CHY_OK;
 }
 
 }  // namespace chy
@@ -128,12 +149,12 @@ int pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
   return rc;
 }
 
-// Read nv doubles from the device reduction buffer (one sync).  Values in
-// red_out are already in its host mirror once the stream has drained.
+// Read nv doubles from the device reduction buffer.  Values in red_out are
+// in its host mirror once the last mirrored combine has stored its number.
 int fetch(chy_ctx *ctx, const double *dev, int nv, double *host)
 {
   if (ctx->red_mirror && dev >= ctx->red_out && dev + nv <= ctx->red_out + ctx->red_out_n) {
-    CHY_HIP(hipStreamSynchronize(ctx->stream));
+    TRY(wait_seq(ctx, ctx->seq_issued));
     const double *src = ctx->h_red + (dev - ctx->red_out);
     if (src != host) std::memcpy(host, src, sizeof(double) * nv);
     return CHY_OK;
@@ -280,17 +301,13 @@ int gmres_inner(chy_ctx *ctx, const double v19[19], const T *rhs, T *e, double r
       for (int64_t r = q + 1; r < j; r++) s -= H[q * m + r] * y[r];
       y[q] = s / H[q * m + q];
     }
-    double *negy = ctx->h_scale + (m + 2);  // pinned, so no wait for the copy is needed
-    for (int64_t q = 0; q < j; q++) negy[q] = -y[q] * sc[q];
-    // (to its own device buffer: red_out holds only reduction results, which
-    // its host mirror repeats)
-    CHY_HIP(hipMemcpyAsync(ctx->gs_scale, negy, sizeof(double) * j, hipMemcpyHostToDevice, ctx->stream));
-    launch_mgs_update<T>(ctx, (int)j, Z, nc, ctx->gs_scale, e, nc);
+    // (the coefficients travel by value with the launch)
+    for (int64_t q = 0; q < j; q++) y[q] = -y[q] * sc[q];
+    launch_mgs_update_host<T>(ctx, (int)j, Z, nc, y.data(), e, nc);
     (void)done;
     // the true residual of the new e is checked at the top of the loop (a
     // cycle that ends on GMRES's own estimate without it leaves phi measurably
     // further from the exact solution: 2.1e-10 against 7.9e-11 at 256^3)
-    CHY_HIP(hipStreamSynchronize(ctx->stream));
   }
 }
 
@@ -601,14 +618,14 @@ int chy_create(const chy_params *prm_in, int64_t np_capacity, chy_ctx **out)
   if (hipHostMalloc((void **)&ctx->h_red, sizeof(double) * ctx->red_out_n,
                     hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&ctx->red_mirror, ctx->h_red, 0) != hipSuccess ||
-      hipHostMalloc((void **)&ctx->h_scale, sizeof(double) * 2 * (ctx->restart + 2)) != hipSuccess ||
+      hipHostMalloc((void **)&ctx->h_scale, sizeof(double) * (ctx->restart + 2)) != hipSuccess ||
       hipHostMalloc((void **)&ctx->h_status, sizeof(uint64_t) * kStatusWords,
                     hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&ctx->status_dev, ctx->h_status, 0) != hipSuccess) {
     chy_destroy(ctx);
     return CHY_ERR_MEM;
   }
-  A(dalloc(ctx, &ctx->gs_scale, ctx->restart + 2));
+  std::memset(ctx->h_status, 0, sizeof(uint64_t) * kStatusWords);  // the sequence word starts below every number
   if (p.pc_type == CHY_PC_MG) A(mg_setup(ctx));
   A(dalloc(ctx, &ctx->diag_part, diag_elems()));
   ctx->diag_out = ctx->diag_part + diag_parts();
@@ -655,7 +672,7 @@ int chy_destroy(chy_ctx *ctx)
                 ctx->partials,
                 ctx->sred, ctx->S_dev, ctx->cstats, ctx->push_part, ctx->mom[0], ctx->mom[1], ctx->mom[2], ctx->mom[3],
                 ctx->efield[0], ctx->phi, ctx->phi_prev, ctx->b, ctx->r64, ctx->t64, ctx->V,
-                ctx->Z, ctx->wv, ctx->fft_buf, ctx->pc_buf, ctx->pc2_buf, ctx->pc2_h, ctx->zl_buf, ctx->zl_col, ctx->red, ctx->rpl, ctx->red_out, ctx->gs_scale, ctx->diag_part, ctx->tw[0], ctx->tw[1], ctx->tw[2],
+                ctx->Z, ctx->wv, ctx->fft_buf, ctx->pc_buf, ctx->pc2_buf, ctx->pc2_h, ctx->zl_buf, ctx->zl_col, ctx->red, ctx->rpl, ctx->red_out, ctx->diag_part, ctx->tw[0], ctx->tw[1], ctx->tw[2],
                 ctx->trig[0], ctx->trig[1], ctx->trig[2]};
   for (void *p : ps) if (p) (void)hipFree(p);
   if (ctx->h_red) (void)hipHostFree(ctx->h_red);

@@ -798,9 +798,11 @@ struct DotScale {
 
 // out[k] = sum_blk partial[blk*nv + k], fixed tree order, times sc.s[k] when
 // sc.on; each value also goes to mirror[k] when a mirror is given (host
-// memory: the read-back of a reduced value needs no copy).
+// memory: the read-back of a reduced value needs no copy), and then the
+// launch's number s to the host's sequence word.
 __global__ __launch_bounds__(256) void k_combine(const double *__restrict__ partial, int nblk, int nv,
-                                                 double *__restrict__ out, DotScale sc, double *__restrict__ mirror)
+                                                 double *__restrict__ out, DotScale sc, double *__restrict__ mirror,
+                                                 unsigned long long *__restrict__ seq, unsigned long long s_no)
 {
   __shared__ double sh[4];
   for (int k = 0; k < nv; k++) {
@@ -816,6 +818,11 @@ __global__ __launch_bounds__(256) void k_combine(const double *__restrict__ part
       if (mirror) mirror[k] = v;
     }
     __syncthreads();
+  }
+  // (thread 0 wrote every mirror value: the fence orders them before the number)
+  if (seq && threadIdx.x == 0) {
+    __threadfence_system();
+    *(volatile unsigned long long *)seq = s_no;
   }
 }
 
@@ -834,7 +841,9 @@ static void combine(chy_ctx *ctx, const double *partial, int nblk, int nv, doubl
   DotScale sc;
   sc.on = scale_host != nullptr;
   for (int k = 0; k < kMaxDots; k++) sc.s[k] = (sc.on && k < nv) ? scale_host[k] : 1.0;
-  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, partial, nblk, nv, out, sc, mirror_of(ctx, out, nv));
+  double *mirror = mirror_of(ctx, out, nv);
+  hipLaunchKernelGGL(k_combine, 1, kBlock, 0, ctx->stream, partial, nblk, nv, out, sc, mirror,
+                     mirror ? seq_word(ctx) : nullptr, mirror ? next_seq(ctx) : 0ull);
 }
 
 // Block b of 256 sums partials [b * chunk, (b + 1) * chunk) in a fixed order.
@@ -904,13 +913,14 @@ void launch_sum(chy_ctx *ctx, const T *a, int64_t n, double *out_dev)
   combine(ctx, ctx->red, (int)nb, 1, out_dev);
 }
 
-// w -= sum_k h_k V_k  (h on the device; accumulation in FP64)
+// w -= sum_k h_k V_k  (h on the device, or by value in hv.s when h is null;
+// accumulation in FP64)
 template <typename T, int NV>
 __global__ __launch_bounds__(256) void k_mgs(int nv, const T *__restrict__ V, int64_t stride,
-                                             const double *__restrict__ h, T *__restrict__ w, int64_t n)
+                                             const double *__restrict__ h, DotScale hv, T *__restrict__ w, int64_t n)
 {
   __shared__ double hs[NV];
-  if (threadIdx.x < nv) hs[threadIdx.x] = h[threadIdx.x];
+  if (threadIdx.x < nv) hs[threadIdx.x] = h ? h[threadIdx.x] : hv.s[threadIdx.x];
   __syncthreads();
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double acc = (double)w[i];
@@ -921,17 +931,36 @@ __global__ __launch_bounds__(256) void k_mgs(int nv, const T *__restrict__ V, in
   }
 }
 
+// w -= sum_k h_k V_k with h on the device, or, when h_host is given, with the
+// coefficients from host memory passed by value (no copy to wait for)
 template <typename T>
-void launch_mgs_update(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n)
+static void mgs_update(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, const double *h_host, T *w,
+                       int64_t n)
 {
   for (int k0 = 0; k0 < nv; k0 += kMaxDots) {
     const int m = (nv - k0 < kMaxDots) ? nv - k0 : kMaxDots;
     const unsigned nb = (unsigned)std::min<int64_t>(4096, (n + kBlock - 1) / kBlock);
+    DotScale hv;
+    hv.on = h_host != nullptr;
+    for (int k = 0; k < kMaxDots; k++) hv.s[k] = (h_host && k < m) ? h_host[k0 + k] : 0.0;
+    const double *hd = h_host ? nullptr : h + k0;
     if (m <= 8)
-      hipLaunchKernelGGL((k_mgs<T, 8>), nb, kBlock, 0, ctx->stream, m, V + k0 * stride, stride, h + k0, w, n);
+      hipLaunchKernelGGL((k_mgs<T, 8>), nb, kBlock, 0, ctx->stream, m, V + k0 * stride, stride, hd, hv, w, n);
     else
-      hipLaunchKernelGGL((k_mgs<T, kMaxDots>), nb, kBlock, 0, ctx->stream, m, V + k0 * stride, stride, h + k0, w, n);
+      hipLaunchKernelGGL((k_mgs<T, kMaxDots>), nb, kBlock, 0, ctx->stream, m, V + k0 * stride, stride, hd, hv, w, n);
   }
+}
+
+template <typename T>
+void launch_mgs_update(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n)
+{
+  mgs_update<T>(ctx, nv, V, stride, h, nullptr, w, n);
+}
+
+template <typename T>
+void launch_mgs_update_host(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h_host, T *w, int64_t n)
+{
+  mgs_update<T>(ctx, nv, V, stride, nullptr, h_host, w, n);
 }
 
 // w -= sum_k h_k V_k as k_mgs, also accumulating ||w||^2 of the result in
@@ -1955,6 +1984,9 @@ template void launch_sum<double>(chy_ctx *, const double *, int64_t, double *);
 template void launch_sum<float>(chy_ctx *, const float *, int64_t, double *);
 template void launch_mgs_update<double>(chy_ctx *, int, const double *, int64_t, const double *, double *, int64_t);
 template void launch_mgs_update<float>(chy_ctx *, int, const float *, int64_t, const double *, float *, int64_t);
+template void launch_mgs_update_host<double>(chy_ctx *, int, const double *, int64_t, const double *, double *,
+                                             int64_t);
+template void launch_mgs_update_host<float>(chy_ctx *, int, const float *, int64_t, const double *, float *, int64_t);
 template void launch_mgs_norm<double>(chy_ctx *, int, const double *, int64_t, const double *, double *, int64_t, double *);
 template void launch_mgs_norm<float>(chy_ctx *, int, const float *, int64_t, const double *, float *, int64_t, double *);
 template void launch_gs_dots<double>(chy_ctx *, int, const double *, int64_t, const double *, int64_t, double *,

@@ -138,13 +138,17 @@ struct chy_ctx {
   // read-back costs one stream synchronisation and no copies
   uint64_t *h_status = nullptr;
   uint64_t *status_dev = nullptr;
+  // The last word of h_status is a sequence number: each kernel that writes
+  // coherent host memory for a read-back (the mirrored combines, the status
+  // kernel) stores its launch's number there after its values, so the host
+  // waits for exactly that store (wait_seq) instead of draining the stream.
+  uint64_t seq_issued = 0;
   hipEvent_t ev_step = nullptr, ev_particles = nullptr;  // a step's start and the end of its particle phase
   double *push_part = nullptr;  // the FP32 push's per-block collision tallies [kPushBlocks][4], then [64][4]
   bool push_pending = false;    // a push's counters not yet read (deferred to the sort's read-back)
   int64_t push_Nc = 0;
   chy_collision_stats *push_stats = nullptr;
-  double *h_scale = nullptr;    // pinned: squared scales of the Krylov basis, then the update coefficients (GMRES)
-  double *gs_scale = nullptr;   // device copy of the update coefficients
+  double *h_scale = nullptr;    // pinned: squared scales of the Krylov basis (GMRES)
   // FFT preconditioner
   void *fft_buf = nullptr;      // complex work array
   void *pc_buf = nullptr;       // z-line / Chebyshev / two-level scratch, 3 vectors (T)
@@ -178,6 +182,7 @@ namespace chy {
 
 constexpr int kRedBlocks = 1024;  // fixed reduction grid: deterministic sums
 constexpr int kStatusWords = 16;  // chy_ctx::h_status
+constexpr int kSeqWord = kStatusWords - 1;  // its sequence word
 constexpr int kPushBlocks = 65536;  // the push's largest grid
 constexpr int kBlock = 256;
 
@@ -245,6 +250,12 @@ inline void free_state<float>(PState<float> &s)
 
 // timers (no-ops unless ctx->timing)
 void timer_start(chy_ctx *ctx, const char *name);
+// Waits until the device has stored sequence number s (or a later one) in the
+// context's coherent sequence word.
+int wait_seq(chy_ctx *ctx, uint64_t s);
+// The device address of the sequence word, and the next launch's number.
+inline unsigned long long *seq_word(chy_ctx *ctx) { return (unsigned long long *)(ctx->status_dev + kSeqWord); }
+inline unsigned long long next_seq(chy_ctx *ctx) { return (unsigned long long)++ctx->seq_issued; }
 void timer_stop(chy_ctx *ctx, const char *name);
 
 // ---- grid kernels (chy_grid.hip) ----
@@ -280,6 +291,8 @@ void launch_sum(chy_ctx *ctx, const T *a, int64_t n, double *out_dev);
 // y -= sum_k h_k V_k (h on device, FP64) ; optional normalisation
 template <typename T>
 void launch_mgs_update(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n);
+template <typename T>
+void launch_mgs_update_host(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h_host, T *w, int64_t n);
 template <typename T>
 void launch_mgs_norm(chy_ctx *ctx, int nv, const T *V, int64_t stride, const double *h, T *w, int64_t n,
                      double *norm2_dev);

@@ -1260,14 +1260,19 @@ static unsigned grid_for(int64_t n, unsigned cap = 4096)
 // is lost.  The flag is read with the next sort's other read-backs.
 // Status words to the host: out[q] = *w[q] for the 32-bit words given
 // (q < 4), then out[4 + q] = u[q] for q < nu; out is coherent host memory.
+// Last, the launch's number s_no goes to the sequence word out[kSeqWord].
 __global__ __launch_bounds__(64) void k_status(const uint32_t *w0, const uint32_t *w1, const uint32_t *w2,
                                                const uint32_t *w3, const unsigned long long *u, int nu,
-                                               unsigned long long *out)
+                                               unsigned long long *out, unsigned long long s_no)
 {
   const int t = (int)threadIdx.x;
   const uint32_t *w = t == 0 ? w0 : t == 1 ? w1 : t == 2 ? w2 : w3;
   if (t < 4 && w) out[t] = *w;
   if (t < nu) out[4 + t] = u[t];
+  // every writer's values reach the host before the number does
+  __threadfence_system();
+  __syncthreads();
+  if (t == 0) *(volatile unsigned long long *)(out + kSeqWord) = s_no;
 }
 
 static int push_finish(chy_ctx *ctx, const unsigned long long cs[6]);
@@ -1280,11 +1285,12 @@ static int read_status(chy_ctx *ctx, const uint32_t *const w[4], uint32_t v32[4]
                        unsigned long long *v64)
 {
   const bool push = ctx->push_pending;
-  if (nu > kStatusWords - 4 || (push && nu > 0)) return set_err(ctx, CHY_ERR_LIB, "read_status: bad counter request");
+  if (nu > kSeqWord - 4 || (push && nu > 0)) return set_err(ctx, CHY_ERR_LIB, "read_status: bad counter request");
   if (push) { u = ctx->cstats; nu = 6; }
+  const unsigned long long s_no = next_seq(ctx);
   hipLaunchKernelGGL(k_status, 1, 64, 0, ctx->stream, w[0], w[1], w[2], w[3], u, nu,
-                     (unsigned long long *)ctx->status_dev);
-  CHY_HIP(hipStreamSynchronize(ctx->stream));
+                     (unsigned long long *)ctx->status_dev, s_no);
+  TRY(wait_seq(ctx, s_no));
   for (int q = 0; q < 4; q++) v32[q] = w[q] ? (uint32_t)ctx->h_status[q] : 0u;
   if (push) {
     unsigned long long cs[6];

@@ -127,9 +127,11 @@ int pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
     if (pc == CHY_PC_FFT) {
       rc = fft_pc_apply<T>(ctx, v19, r, z);
     } else {
-      rc = fft2_pc_apply<T>(ctx, v19, r, z, sb);
       // fft2z: z += zline(r - A z) (built for A(n); transposed solves keep fft2)
-      if (rc == CHY_OK && pc == CHY_PC_FFT2Z && !ctx->solve_transpose && ctx->zl_buf) {
+      const bool zl = pc == CHY_PC_FFT2Z && !ctx->solve_transpose && ctx->zl_buf;
+      bool zl_done = false;
+      rc = fft2_pc_apply<T>(ctx, v19, r, z, sb, zl, &zl_done);
+      if (rc == CHY_OK && zl && !zl_done) {
         launch_residual_t<T>(ctx, ctx->prm.lhs_type, mom<T>(ctx, 0), r, z, sb);
         launch_zline_apply<T>(ctx, (const T *)ctx->zl_buf, ctx->zl_col, sb, z);
       }

@@ -1194,11 +1194,14 @@ static int fft2_batch(int n, int64_t lines)
 }
 
 template <typename T>
-int fft2_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z, T *w3)
+int fft2_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z, T *w3, bool zline, bool *zline_done)
 {
-  (void)w3;
+  if (zline_done) *zline_done = false;
   TRY(fft_pc_apply<T>(ctx, v19, r, z));
   if (ctx->solve_transpose) return CHY_OK;  // the correction is built for A(n), not its transpose
+  // the z-line stage fused with the correction: the column-sum pass also keeps
+  // r - A(n) z in w3 for it, and e2 is added by the stage itself
+  const bool fused = zline && w3 && ctx->zl_buf && zline_e2_applies(ctx);
   const int nx = ctx->g.nx, ny = ctx->g.ny, nz = ctx->g.nz;
   const int64_t nxy = (int64_t)nx * ny;
   T *C = (T *)ctx->pc2_buf, *V = C + 9 * nxy, *Z = V + (kFft2Its + 1) * nxy, *w = Z + kFft2Its * nxy;
@@ -1206,7 +1209,7 @@ int fft2_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z, T *w3)
   const H2 h = h2_layout(ctx, sizeof(T) == 4);
   const int nb = h.nb;
   // z-averaged residual of the first level, mean removed, into V0
-  launch_lhs_zsum<T>(ctx, ctx->prm.lhs_type, (const T *)ctx->mom[0], z, r, h.part, h.pl);
+  launch_lhs_zsum<T>(ctx, ctx->prm.lhs_type, (const T *)ctx->mom[0], z, r, h.part, h.pl, fused ? w3 : nullptr);
   hipLaunchKernelGGL((k2_r2<T>), nb, 256, 0, ctx->stream, nxy, nz, lhs_zchunks(ctx->g), (const double *)h.part,
                      (const double *)h.pl, h.nlb, V, h.pn);
   Symbol sym = make_symbol(ctx, v19);
@@ -1231,15 +1234,21 @@ int fft2_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z, T *w3)
   }
   hipLaunchKernelGGL(k2_solve, 1, 256, 0, ctx->stream, (const double *)h.pn, (const double *)h.p1,
                      (const double *)h.p2, nb, h.y);
-  hipLaunchKernelGGL((k2_ext<T>), dim3((unsigned)((nxy + 255) / 256), (unsigned)((nz + kExtZ - 1) / kExtZ)), 256, 0,
-                     ctx->stream, nxy, nz, (const T *)Z, (const double *)h.y, z);
+  if (fused) {
+    launch_zline_apply_e2<T>(ctx, (const T *)ctx->zl_buf, ctx->zl_col, w3, z, (const T *)ctx->mom[0], (const T *)Z,
+                             (const double *)h.y);
+    if (zline_done) *zline_done = true;
+  } else {
+    hipLaunchKernelGGL((k2_ext<T>), dim3((unsigned)((nxy + 255) / 256), (unsigned)((nz + kExtZ - 1) / kExtZ)), 256,
+                       0, ctx->stream, nxy, nz, (const T *)Z, (const double *)h.y, z);
+  }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_err(ctx, CHY_ERR_LIB, std::string("fft2: ") + hipGetErrorString(e));
   return CHY_OK;
 }
 
-template int fft2_pc_apply<double>(chy_ctx *, const double *, const double *, double *, double *);
-template int fft2_pc_apply<float>(chy_ctx *, const double *, const float *, float *, float *);
+template int fft2_pc_apply<double>(chy_ctx *, const double *, const double *, double *, double *, bool, bool *);
+template int fft2_pc_apply<float>(chy_ctx *, const double *, const float *, float *, float *, bool, bool *);
 
 template int fft_pc_apply<double>(chy_ctx *, const double *, const double *, double *);
 template int fft_pc_apply<float>(chy_ctx *, const double *, const float *, float *);

@@ -198,7 +198,8 @@ __global__ __launch_bounds__(256) void k_residual(Grid g, int lhs_type, SCoef s,
 // step with coalesced x-fastest reads.  Output (OUT): 0 y = A x, 1 r = b - A x,
 // 2 the column sums over the block's planes of b - A x, y[zc * nx * ny + col]
 // for z chunk zc, plus the block's total in pl[block] (both in double),
-// 3 r = b - A x and the block's sum of r^2 (in double) in pl[block].
+// 3 r = b - A x and the block's sum of r^2 (in double) in pl[block], 4 as 2
+// and also r = b - A x in d3.
 constexpr int kTX = 64, kTY = 16, kRows = kTY / 4, kZChunk = 16;  // 64 x 4 threads, 4 rows each
 constexpr int kHX = kTX + 2, kHY = kTY + 2;
 
@@ -238,8 +239,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(tiled_waves
 void k_lhs_tiled(Grid g, int lhs_type, SCoef s, LapRow lap,
                                                    const TN *__restrict__ n, const TX *__restrict__ x,
                                                    const TB *__restrict__ b, TY *__restrict__ y,
-                                                   double *__restrict__ pl)
+                                                   double *__restrict__ pl, TX *__restrict__ d3)
 {
+  constexpr bool ZS = OUT == 2 || OUT == 4;  // per-chunk column sums
   // tile TW x TY_ (TW = 64: 64 x 16; TW = 256, a whole x row of a 256-wide
   // grid: 256 x 8 with no x halo beyond the row's own periodic wrap), GW row
   // groups of TW threads, RW rows per thread
@@ -328,6 +330,11 @@ void k_lhs_tiled(Grid g, int lhs_type, SCoef s, LapRow lap,
       const int64_t c = (int64_t)i + (int64_t)g.nx * ((int64_t)j + (int64_t)g.ny * k);
       if (OUT == 1) y[c] = (TY)((double)rb[rr] - (double)acc);
       else if (OUT == 2) zs[rr] += (double)rb[rr] - (double)acc;
+      else if (OUT == 4) {
+        const double rv = (double)rb[rr] - (double)acc;
+        d3[c] = (TX)rv;
+        zs[rr] += rv;
+      }
       else if (OUT == 3) {
         const double rv = (double)rb[rr] - (double)acc;
         y[c] = (TY)rv;
@@ -422,14 +429,14 @@ void k_lhs_tiled(Grid g, int lhs_type, SCoef s, LapRow lap,
     }
     __syncthreads();
   }
-  if (OUT == 2 || OUT == 3) {
+  if (ZS || OUT == 3) {
     __shared__ double wsum[4];
     double t = 0.0;
 #pragma unroll
     for (int rr = 0; rr < RW; rr++) {
       const int j = y0 + row_of(rr);
       if (i < g.nx && j < g.ny) {
-        if (OUT == 2) y[(int64_t)bz * pstride + i + (int64_t)g.nx * j] = (TY)zs[rr];
+        if (ZS) y[(int64_t)bz * pstride + i + (int64_t)g.nx * j] = (TY)zs[rr];
         t += zs[rr];
       }
     }
@@ -473,23 +480,24 @@ static inline bool kz_operator(int lhs_type, const SCoef &s)
 // One tiled stencil pass on grid g, the B-along-z instantiation when it applies.
 template <typename TN, typename TX, typename TY, typename C, bool DIFF, int OUT, typename TB = double, int TW = kTX>
 static void launch_tiled_on(chy_ctx *ctx, const Grid &g, const SCoef &s, const LapRow &L, int lhs_type, const TN *n,
-                            const TX *x, const TB *b, TY *y, double *pl)
+                            const TX *x, const TB *b, TY *y, double *pl, TX *d3 = nullptr)
 {
   if (kz_operator(lhs_type, s))
     hipLaunchKernelGGL((k_lhs_tiled<TN, TX, TY, C, DIFF, OUT, TB, TW, true>), tiled_grid(g, TW), kBlock, 0,
-                       ctx->stream, g, lhs_type, s, L, n, x, b, y, pl);
+                       ctx->stream, g, lhs_type, s, L, n, x, b, y, pl, d3);
   else
     hipLaunchKernelGGL((k_lhs_tiled<TN, TX, TY, C, DIFF, OUT, TB, TW, false>), tiled_grid(g, TW), kBlock, 0,
-                       ctx->stream, g, lhs_type, s, L, n, x, b, y, pl);
+                       ctx->stream, g, lhs_type, s, L, n, x, b, y, pl, d3);
 }
 
 // The same on the context's grid.
 template <typename TN, typename TX, typename TY, typename C, bool DIFF, int OUT, typename TB = double, int TW = kTX>
-static void launch_tiled(chy_ctx *ctx, int lhs_type, const TN *n, const TX *x, const TB *b, TY *y, double *pl)
+static void launch_tiled(chy_ctx *ctx, int lhs_type, const TN *n, const TX *x, const TB *b, TY *y, double *pl,
+                         TX *d3 = nullptr)
 {
   const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
   const LapRow L = lap_row(ctx, lhs_type);
-  launch_tiled_on<TN, TX, TY, C, DIFF, OUT, TB, TW>(ctx, ctx->g, s, L, lhs_type, n, x, b, y, pl);
+  launch_tiled_on<TN, TX, TY, C, DIFF, OUT, TB, TW>(ctx, ctx->g, s, L, lhs_type, n, x, b, y, pl, d3);
 }
 
 template <typename TN, typename TX, typename TY>
@@ -514,7 +522,7 @@ int64_t lhs_zsum_blocks(const Grid &g, bool fp32)
 }
 
 template <typename T>
-void launch_lhs_zsum(chy_ctx *ctx, int lhs_type, const T *n, const T *x, const T *b, double *part, double *pl)
+void launch_lhs_zsum(chy_ctx *ctx, int lhs_type, const T *n, const T *x, const T *b, double *part, double *pl, T *r)
 {
   const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
   const LapRow L = lap_row(ctx, lhs_type);
@@ -522,11 +530,13 @@ void launch_lhs_zsum(chy_ctx *ctx, int lhs_type, const T *n, const T *x, const T
   using C = typename std::conditional<fp32, float, double>::type;
   if constexpr (fp32) {
     if (wide_rows(ctx->g, fp32)) {
-      launch_tiled<T, T, double, C, fp32, 2, T, 256>(ctx, lhs_type, n, x, b, part, pl);
+      if (r) launch_tiled<T, T, double, C, fp32, 4, T, 256>(ctx, lhs_type, n, x, b, part, pl, r);
+      else launch_tiled<T, T, double, C, fp32, 2, T, 256>(ctx, lhs_type, n, x, b, part, pl);
       return;
     }
   }
-  launch_tiled<T, T, double, C, fp32, 2, T>(ctx, lhs_type, n, x, b, part, pl);
+  if (r) launch_tiled<T, T, double, C, fp32, 4, T>(ctx, lhs_type, n, x, b, part, pl, r);
+  else launch_tiled<T, T, double, C, fp32, 2, T>(ctx, lhs_type, n, x, b, part, pl);
 }
 
 // r = b - A x in FP64 arithmetic, stored as TY, and one partial of |r|^2 per
@@ -1569,6 +1579,108 @@ __global__ __launch_bounds__(256) void k_zline_apply(Grid g, const T *__restrict
   }
 }
 
+// The fft2z preconditioner's z-line stage with B along z (K_x = K_y = 0):
+// z += e2 + zline(d0 - A(n) e2), where d0 = r - A(n) z_fft was stored by the
+// column-sum pass and e2 = sum_q y2[q] Z2[q] is the 2-D correction, constant
+// along z.  A(n) applied to a field constant along z keeps only the in-plane
+// entries of each row (0-3 and 6-9; the z entries multiply e2_c - e2_c), and
+// those are linear in the five in-plane densities, so per column
+//     (A e2)(k) = Kc n_c + Kxp n_(x+1) + Kxm n_(x-1) + Kyp n_(y+1) + Kym n_(y-1)
+// with the K formed once from the e2 differences de_q = e2_q - e2_c (the row
+// of lhs.c:480-511 with s_xz = s_yz = s_zx = s_zy = 0, regrouped by density).
+// This replaces a residual pass over r, z and n and the pass adding e2 to z.
+template <typename T, int NQ>
+__global__ __launch_bounds__(256) void k_zline_apply_e2(Grid g, const T *__restrict__ za, const T *__restrict__ zi,
+                                                        const T *__restrict__ zc, const T *__restrict__ zq,
+                                                        const double *__restrict__ cag,
+                                                        const double *__restrict__ cden, T *__restrict__ res,
+                                                        T *__restrict__ z, const T *__restrict__ n, SCoef s,
+                                                        const T *__restrict__ Z2, const double *__restrict__ y2)
+{
+  constexpr int KC = 16;
+  const int64_t plane = (int64_t)g.nx * g.ny;
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= plane) return;
+  const int m = g.nz;
+  const int i = (int)(col % g.nx), j = (int)(col / g.nx);
+  const int ip = i + 1 == g.nx ? 0 : i + 1, im = i == 0 ? g.nx - 1 : i - 1;
+  const int jp = j + 1 == g.ny ? 0 : j + 1, jm = j == 0 ? g.ny - 1 : j - 1;
+  double yq[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) yq[q] = y2[q];
+  auto e2 = [&](int ii, int jj) {
+    const int64_t c = (int64_t)ii + (int64_t)g.nx * jj;
+    double e = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) e += yq[q] * (double)Z2[(int64_t)q * plane + c];
+    return e;
+  };
+  const double ec = e2(i, j);
+  const double de0 = e2(ip, j) - ec, de1 = e2(im, j) - ec, de2 = e2(i, jp) - ec, de3 = e2(i, jm) - ec;
+  const double de6 = e2(ip, jp) - ec, de7 = e2(ip, jm) - ec, de8 = e2(im, jp) - ec, de9 = e2(im, jm) - ec;
+  const double kc = s.sxx * (de0 + de1) + s.syy * (de2 + de3) + (s.sxy + s.syx) * ((de6 - de7) + (de9 - de8));
+  const double kxp = s.sxx * de0 + s.sxy * (de2 - de3) + s.sxy * (de6 - de7);
+  const double kxm = s.sxx * de1 - s.sxy * (de2 - de3) + s.sxy * (de9 - de8);
+  const double kyp = s.syx * (de0 - de1) + s.syy * de2 + s.syx * (de6 - de8);
+  const double kym = -s.syx * (de0 - de1) + s.syy * de3 + s.syx * (de9 - de7);
+  const int64_t oxp = ip - i, oxm = im - i, oyp = (int64_t)g.nx * (jp - j), oym = (int64_t)g.nx * (jm - j);
+  double y = 0.0;
+  // (eight values per plane: half the batch of the other passes keeps the
+  // loads in flight within the register file)
+  constexpr int KF = KC / 2;
+  for (int k0 = 0; k0 < m; k0 += KF) {  // forward substitution on d0 - A e2
+    T rr[KF], aa[KF], ii[KF], nc[KF], nxp[KF], nxm[KF], nyp[KF], nym[KF];
+#pragma unroll
+    for (int u = 0; u < KF; u++) {
+      const int64_t e = col + plane * (k0 + u);
+      if (k0 + u < m) {
+        rr[u] = res[e]; aa[u] = za[e]; ii[u] = zi[e];
+        nc[u] = n[e]; nxp[u] = n[e + oxp]; nxm[u] = n[e + oxm]; nyp[u] = n[e + oyp]; nym[u] = n[e + oym];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KF; u++) {
+      if (k0 + u < m) {
+        const double ae = kc * (double)nc[u] + kxp * (double)nxp[u] + kxm * (double)nxm[u] + kyp * (double)nyp[u] +
+                          kym * (double)nym[u];
+        y = (((double)rr[u] - ae) - (double)aa[u] * y) * (double)ii[u];
+        res[col + plane * (k0 + u)] = (T)y;
+      }
+    }
+  }
+  const double ylast = y;
+  double yn = y;
+  for (int k0 = m - 2; k0 >= 0; k0 -= KC) {  // back substitution, planes k0, k0 - 1, ...
+    T rr[KC], cc[KC];
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+      const int k = k0 - u;
+      if (k >= 0) { rr[u] = res[col + plane * k]; cc[u] = zc[col + plane * k]; }
+    }
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        yn = (double)rr[u] - (double)cc[u] * yn;
+        res[col + plane * k] = (T)yn;
+      }
+    }
+  }
+  const double f = (yn + cag[col] * ylast) / cden[col];
+  for (int k0 = 0; k0 < m; k0 += KC) {  // periodic correction; z += e2 + the solve
+    T rr[KC], qq[KC], zz[KC];
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+      const int64_t e = col + plane * (k0 + u);
+      if (k0 + u < m) { rr[u] = res[e]; qq[u] = zq[e]; zz[u] = z[e]; }
+    }
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+      if (k0 + u < m) z[col + plane * (k0 + u)] = (T)(((double)zz[u] + ec) + ((double)rr[u] - f * (double)qq[u]));
+    }
+  }
+}
+
 // The same z-line solve with one wave per column, for grids of few columns
 // (the multigrid's coarse levels), where one thread per column leaves the
 // chip idle and each call costs the latency of walking the column three
@@ -1794,6 +1906,25 @@ void launch_zline_apply_g(chy_ctx *ctx, const Grid &g, const T *zbuf, const doub
                      zbuf + 2 * nc, zbuf + 3 * nc, colbuf, colbuf + plane, res, z, scale);
 }
 
+// fft2z's z-line stage with the 2-D correction folded in (k_zline_apply_e2):
+// res holds d0 = r - A(n) z_fft on entry, Z2 the 2-D directions, y2 their
+// coefficients.  Only for the full operator with B along z.
+template <typename T>
+void launch_zline_apply_e2(chy_ctx *ctx, const T *zbuf, const double *colbuf, T *res, T *z, const T *n, const T *Z2,
+                           const double *y2)
+{
+  const Grid &g = ctx->g;
+  const int64_t nc = g.nc, plane = (int64_t)g.nx * g.ny;
+  const SCoef s = full_scoef(ctx->prm.d, ctx->prm.kappa_e);
+  hipLaunchKernelGGL((k_zline_apply_e2<T, kFft2Its>), nblocks(plane), kBlock, 0, ctx->stream, g, zbuf, zbuf + nc,
+                     zbuf + 2 * nc, zbuf + 3 * nc, colbuf, colbuf + plane, res, z, n, s, Z2, y2);
+}
+
+bool zline_e2_applies(chy_ctx *ctx)
+{
+  return kz_operator(ctx->prm.lhs_type, full_scoef(ctx->prm.d, ctx->prm.kappa_e));
+}
+
 // The wave-per-column form for grids of at most kZlineWaveCols columns and 3 to
 // 512 planes (the multigrid's coarse levels); otherwise the thread-per-column
 // one.  A wave's loads along its column touch one cache line per value, so
@@ -1889,6 +2020,10 @@ template void launch_residual_g<double>(chy_ctx *, const Grid &, const double *,
 template void launch_residual_g<float>(chy_ctx *, const Grid &, const double *, int, const float *, const float *,
                                        const float *, float *);
 template void launch_zline_apply<double>(chy_ctx *, const double *, const double *, double *, double *);
+template void launch_zline_apply_e2<double>(chy_ctx *, const double *, const double *, double *, double *,
+                                            const double *, const double *, const double *);
+template void launch_zline_apply_e2<float>(chy_ctx *, const float *, const double *, float *, float *, const float *,
+                                           const float *, const double *);
 template void launch_zline_apply<float>(chy_ctx *, const float *, const double *, float *, float *);
 template void launch_residual_t<double>(chy_ctx *, int, const double *, const double *, const double *, double *);
 template void launch_residual_t<float>(chy_ctx *, int, const float *, const float *, const float *, float *);
@@ -1950,8 +2085,9 @@ template void launch_lhs_apply<double, double, double>(chy_ctx *, int, bool, con
 template void launch_lhs_apply<float, float, float>(chy_ctx *, int, bool, const float *, const float *, float *);
 template void launch_lhs_apply<float, double, double>(chy_ctx *, int, bool, const float *, const double *, double *);
 template void launch_lhs_zsum<double>(chy_ctx *, int, const double *, const double *, const double *, double *,
-                                      double *);
-template void launch_lhs_zsum<float>(chy_ctx *, int, const float *, const float *, const float *, double *, double *);
+                                      double *, double *);
+template void launch_lhs_zsum<float>(chy_ctx *, int, const float *, const float *, const float *, double *, double *,
+                                     float *);
 template void launch_residual<double, double>(chy_ctx *, int, const double *, const double *, const double *, double *);
 template void launch_residual<float, double>(chy_ctx *, int, const float *, const double *, const double *, double *);
 template void launch_residual_norm<double, double>(chy_ctx *, int, const double *, const double *, const double *,

@@ -273,9 +273,11 @@ void launch_add_scaled(chy_ctx *ctx, int64_t n, const TS *x, double a, double *y
 template <typename T>
 int64_t launch_residual_sq(chy_ctx *ctx, int lhs_type, const T *n, const T *b, const T *x, T *r, double *pl);
 // part[zc * nx * ny + col] = sum over the planes of z chunk zc of (b - A x) at
-// the column; pl[blk] = the sum over each stencil block (lhs_zsum_blocks of them)
+// the column; pl[blk] = the sum over each stencil block (lhs_zsum_blocks of them);
+// when r is given, b - A x itself is also stored there
 template <typename T>
-void launch_lhs_zsum(chy_ctx *ctx, int lhs_type, const T *n, const T *x, const T *b, double *part, double *pl);
+void launch_lhs_zsum(chy_ctx *ctx, int lhs_type, const T *n, const T *x, const T *b, double *part, double *pl,
+                     T *r = nullptr);
 int lhs_zchunks(const Grid &g);
 int64_t lhs_zsum_blocks(const Grid &g, bool fp32);  // FP32 or FP64 stencil's grid
 template <typename T>
@@ -401,6 +403,13 @@ template <typename T>
 void launch_zcolumn(chy_ctx *ctx, const T *n, T *Cz, T *zbuf, double *colbuf);
 template <typename T>
 void launch_zline_apply(chy_ctx *ctx, const T *zbuf, const double *colbuf, T *res, T *z);
+// fft2z's z-line stage with its 2-D correction folded in, where it applies
+// (the full operator with B along z): z += e2 + zline(res - A(n) e2), e2 =
+// sum_q y2[q] Z2[q] constant along z, res = r - A(n) z on entry
+template <typename T>
+void launch_zline_apply_e2(chy_ctx *ctx, const T *zbuf, const double *colbuf, T *res, T *z, const T *n, const T *Z2,
+                           const double *y2);
+bool zline_e2_applies(chy_ctx *ctx);
 template <typename T>
 void launch_residual_t(chy_ctx *ctx, int lhs_type, const T *n, const T *b, const T *x, T *res);
 // the same pieces on an explicit grid g with spacings d (multigrid levels):
@@ -418,7 +427,12 @@ void launch_zline_apply_coarse(chy_ctx *ctx, const Grid &g, const T *zbuf, const
 template <typename T>
 void launch_residual_g(chy_ctx *ctx, const Grid &g, const double d[3], int lhs_type, const T *n, const T *b,
                        const T *x, T *res);
+// The two-level preconditioner z = fft2(r).  With zline set it goes on to
+// fft2z's z-line stage, z += zline(r - A(n) z), in the fused form where that
+// applies, and says so in *zline_done (else the caller runs the stage); w3 is
+// a scratch vector for the stage's residual.
 template <typename T>
-int fft2_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z, T *w3);
+int fft2_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z, T *w3, bool zline = false,
+                  bool *zline_done = nullptr);
 
 }  // namespace chy

@@ -80,6 +80,15 @@ def _pc_type():
     return "fft2z"
 
 
+def _b_along_z():
+    """True unless --solver-opts gives B an x or y component (the bench's field is B0z)."""
+    o = SOLVER_OPTS
+    for i, a in enumerate(o[:-1]):
+        if a in ("-B0x", "-B0y") and float(o[i + 1]) != 0.0:
+            return False
+    return True
+
+
 # Algorithmic bytes per launch of each kernel: every distinct array argument
 # read once and every output written once at its stored precision.
 # (np = ions, nc = cells = nodes); weights are present in this workload.
@@ -120,11 +129,17 @@ def algorithmic_bytes(kernel, prec, np_, nc, nx=None):
             return None
         # fft2z, per application, the sum of its passes' own reads and writes:
         # 5 FFT passes (each a real-field-sized array in and out: the half
-        # spectrum of nx/2 complex) 10s; the z-summed residual (n, z, r in) 3s;
-        # the 2-D correction's extension (z in and out) 2s; the residual
-        # (n, z, r in, r' out) 4s; the z-line solve (r, 4 factors, z in; the
+        # spectrum of nx/2 complex) 10s.  With B along z: the z-summed
+        # residual, which also stores the residual (n, z, r in, r' out) 4s,
+        # and the z-line solve with the 2-D correction folded in (r', 4
+        # factors, n, z in; the intermediate out and in twice; z out) 12s.
+        # Otherwise: the z-summed residual (n, z, r in) 3s, the 2-D
+        # correction's extension (z in and out) 2s, the residual (n, z, r in,
+        # r' out) 4s and the z-line solve (r', 4 factors, z in; the
         # intermediate out and in twice; z out) 11s.  The 2-D (x-y plane)
         # work is left out.
+        if _b_along_z():
+            return nc * (10 + 4 + 12) * s
         return nc * (10 + 3 + 2 + 4 + 11) * s
     if kernel == "efield":
         return nc * (8 + 3 * s)

@@ -56,6 +56,10 @@ def test_preconditioner_bytes_only_for_fft2z():
     saved = list(bench.SOLVER_OPTS)
     try:
         bench.SOLVER_OPTS[:] = []
+        # B along z: the residual kept by the column-sum pass, the 2-D
+        # correction inside the z-line solve
+        assert bench.algorithmic_bytes("pc", "mixed", 0, nc) == nc * 26 * 4
+        bench.SOLVER_OPTS[:] = ["-B0x", "1e-5"]
         assert bench.algorithmic_bytes("pc", "mixed", 0, nc) == nc * 30 * 4
         bench.SOLVER_OPTS[:] = ["-pc_type", "mg"]
         assert bench.algorithmic_bytes("pc", "mixed", 0, nc) is None

@@ -396,23 +396,20 @@ __global__ __launch_bounds__(256) void k_fft_s(int logn, int64_t A, int64_t estr
 // Block p takes the z-lines of bit-reversed y positions p and p' (ky' = -ky;
 // p' == p for ky = 0, ny/2), transforms both, scales P by the symbol at
 // kx = 0 and Q by the symbol at kx = nx/2, recombines and transforms back.
+// Block p's share (all threads of the block): tws holds the n-point
+// twiddles, s has room for 2 (n + 1) values.
 template <typename T>
-__global__ __launch_bounds__(256) void k_fft_col0(int logn, int64_t estride, int hx, cpx<T> *__restrict__ buf,
-                                                  const double *__restrict__ tw, Symbol sym,
-                                                  const double *__restrict__ trx, const double *__restrict__ try_,
-                                                  const double *__restrict__ trz)
+__device__ __forceinline__ void fft_col0_body(int p, int logn, int64_t estride, int hx, cpx<T> *__restrict__ buf,
+                                              const cpx<T> *tws, cpx<T> *s, const Symbol &sym,
+                                              const double *__restrict__ trx, const double *__restrict__ try_,
+                                              const double *__restrict__ trz)
 {
-  extern __shared__ unsigned char smem_raw[];
   const int n = 1 << logn;
-  cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
-  cpx<T> *s = tws + (n >> 1);     // 2 lines, padded
   const int ny = 1 << sym.ly;
-  const int p = blockIdx.x;
   const int ky = sym.ynat ? p : bitrev(p, sym.ly);
   const int pp = sym.ynat ? (ny - ky) & (ny - 1) : bitrev((ny - ky) & (ny - 1), sym.ly);
   if (pp < p) return;             // the pair is done by block pp
   const int nl = pp == p ? 1 : 2;
-  load_twiddles<T>(tws, tw, n);
   const int64_t base[2] = {(int64_t)hx * p, (int64_t)hx * pp};
   for (int e = threadIdx.x; e < nl * n; e += blockDim.x) {
     const int b = e >> logn, t = e & (n - 1);
@@ -453,6 +450,20 @@ __global__ __launch_bounds__(256) void k_fft_col0(int logn, int64_t estride, int
     const int b = e >> logn, t = e & (n - 1);
     buf[base[b] + (int64_t)t * estride] = s[b * (n + 1) + t];
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fft_col0(int logn, int64_t estride, int hx, cpx<T> *__restrict__ buf,
+                                                  const double *__restrict__ tw, Symbol sym,
+                                                  const double *__restrict__ trx, const double *__restrict__ try_,
+                                                  const double *__restrict__ trz)
+{
+  extern __shared__ unsigned char smem_raw[];
+  const int n = 1 << logn;
+  cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
+  cpx<T> *s = tws + (n >> 1);     // 2 lines, padded
+  load_twiddles<T>(tws, tw, n);
+  fft_col0_body<T>((int)blockIdx.x, logn, estride, hx, buf, tws, s, sym, trx, try_, trz);
 }
 
 // ---- 256- and 512-point lines: register-resident four-step transforms -----------------
@@ -597,6 +608,18 @@ __global__ __launch_bounds__(256) void k_fftq_s(int64_t A, int64_t estride, int6
   __shared__ cpx<T> tws[N / 2];
   load_twiddles<T>(tws, tw, N);
   __syncthreads();
+  if constexpr (MODE == 2) {
+    // blocks past the lines take the packed kx = 0 column (k_fft_col0's work,
+    // the z lines the others leave), so it needs no launch of its own; lre is
+    // their scratch
+    static_assert(2 * (N + 1) * 2 <= 16 * SB, "column-0 scratch");
+    const int64_t nmain = A / 16;
+    if ((int64_t)blockIdx.x >= nmain) {
+      fft_col0_body<T>((int)(blockIdx.x - nmain), R == 16 ? 8 : 9, estride, sym.nx >> 1, buf, tws,
+                       reinterpret_cast<cpx<T> *>(lre), sym, trx, try_, trz);
+      return;
+    }
+  }
   const int b = threadIdx.x & 15, j = threadIdx.x >> 4;
   const int64_t blocks_per_c = A / 16;
   const int64_t c = blockIdx.x / blocks_per_c, a = (blockIdx.x % blocks_per_c) * 16 + b;
@@ -835,16 +858,16 @@ int fft_pc_apply(chy_ctx *ctx, const double v19[19], const T *r, T *z)
   {  // z forward + symbol + inverse: lines a = kx + (nx/2)*py, estride P
     const int n = nz;
     const int64_t A = (int64_t)hx * ny;
-    if (zq) {
-      FFTQ(2, zq, (unsigned)(A / 16), A, P, (int64_t)0, ctx->tw[2]);
+    if (zq) {  // (ny more blocks take the kx = 0 column)
+      FFTQ(2, zq, (unsigned)(A / 16 + ny), A, P, (int64_t)0, ctx->tw[2]);
     } else {
       const int B = fft_batch_z<T>(n, A);
       hipLaunchKernelGGL((k_fft_s<T, 2>), (unsigned)(A / B), kBlock, fft_lds_z<T>(n, B), ctx->stream, ilog2(n), A, P,
                          (int64_t)0, B, buf, ctx->tw[2], sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
+      const size_t lds = ((size_t)(n >> 1) + 2 * (size_t)(n + 1)) * 2 * sizeof(T);
+      hipLaunchKernelGGL((k_fft_col0<T>), (unsigned)ny, kBlock, lds, ctx->stream, ilog2(n), P, hx, buf, ctx->tw[2],
+                         sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
     }
-    const size_t lds = ((size_t)(n >> 1) + 2 * (size_t)(n + 1)) * 2 * sizeof(T);
-    hipLaunchKernelGGL((k_fft_col0<T>), (unsigned)ny, kBlock, lds, ctx->stream, ilog2(n), P, hx, buf, ctx->tw[2],
-                       sym, ctx->trig[0], ctx->trig[1], ctx->trig[2]);
   }
   if (yq) {  // y inverse
     FFTQ(1, yq, nyb, (int64_t)hx, (int64_t)hx, P, ctx->tw[1]);
